@@ -1,0 +1,323 @@
+"""g2s_amd — MI355X-native QuickSampling exact scan behind G2S's device interface.
+
+Python mirror of the host side. The compute path is libqsmi.so (HIP for
+gfx950, include/qsmi.h); this module only marshals arrays.
+
+  * :class:`TrainingImage` / :class:`AcceleratorDevice` mirror G2S's
+    SharedMemoryManager + AcceleratorDevice (include/AcceleratorDevice.hpp:37-90):
+    ``candidate_for_pattern``, ``search_k_biggest``, ``cvt_index_to_position``...
+  * :meth:`AcceleratorDevice.scan_wave` runs a batch of independent path
+    nodes (scan + select + draw) in one launch sequence.
+  * :func:`qs` mirrors ``g2s('-a','qs', ...)`` / ``g2s::qs::runInMemory``
+    (include/qsCore.hpp:17-58) for one TI.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import QsmiError, check, lib
+
+__all__ = [
+    "QsmiError",
+    "lib",
+    "fft_size",
+    "device_count",
+    "categories",
+    "coef_matrix",
+    "ti_subimages",
+    "offsets",
+    "path",
+    "TrainingImage",
+    "AcceleratorDevice",
+    "qs",
+]
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def fft_size(n):
+    """SharedMemoryManager size rule (src/sharedMemoryManager.cpp:26-58)."""
+    return int(lib().qsmi_fft_size(int(n)))
+
+
+def device_count():
+    c = C.c_int(0)
+    check(lib().qsmi_get_device_count(C.byref(c)))
+    return c.value
+
+
+def categories(ti, nvar, var):
+    ti = _f32(ti).reshape(-1)
+    out = np.zeros(4096, np.float32)
+    cnt = C.c_int(0)
+    check(lib().qsmi_categories(ti.size // nvar, nvar, ti, var, out, 4096, C.byref(cnt)))
+    return out[: cnt.value].copy()
+
+
+def coef_matrix(types, ncat=()):
+    """(alpha[A,B], convTypes, subVarOf) of generateCoefMatrix4Xcorr."""
+    types = _i32(types)
+    ncat = _i32(list(ncat) or [0])
+    alpha = np.zeros(256, np.float32)
+    conv = np.zeros(16, np.int32)
+    sub_of = np.zeros(16, np.int32)
+    ns = C.c_int(0)
+    check(lib().qsmi_coef_matrix(len(types), types, ncat, alpha, conv, sub_of, C.byref(ns)))
+    n = ns.value
+    # alpha[A + B*nSub] -> matrix [A, B]
+    return alpha[: n * n].reshape(n, n).T.copy(), conv[:n].copy(), sub_of[:n].copy()
+
+
+def ti_subimages(ti, dims, types, ncat=(), catvals=()):
+    """convertInput4Xcorr images, unreflected, shape (nSub, prod(dims))."""
+    dims = _u32(dims)
+    types = _i32(types)
+    ti = _f32(ti).reshape(-1)
+    nsub = 2 * int(np.sum(types == 0)) + int(np.sum(ncat))
+    out = np.zeros(nsub * int(np.prod(dims)), np.float32)
+    check(lib().qsmi_ti_subimages(len(dims), dims, len(types), types, ti, _i32(list(ncat) or [0]),
+                                  _f32(list(catvals) or [0]), out))
+    return out.reshape(nsub, -1)
+
+
+def offsets(kernel_size, kernel=None, knvar=1):
+    """Sorted neighbour offsets of qs.cpp:1055-1137, shape (count, ndim)."""
+    ks = _u32(kernel_size)
+    cap = int(np.prod([2 * ((int(d) + 1) // 2) - 1 for d in ks]))
+    out = np.zeros(cap * len(ks), np.int32)
+    cnt = C.c_int(0)
+    kp = None
+    if kernel is not None:
+        kernel = _f32(kernel)
+        kp = kernel.ctypes.data
+    check(lib().qsmi_offsets(len(ks), ks, knvar, kp, out, cap, C.byref(cnt)))
+    return out[: cnt.value * len(ks)].reshape(cnt.value, len(ks)).copy()
+
+
+def path(informed, seed):
+    """(path, seeds, beginPath) of qs.cpp:1159-1258."""
+    informed = np.ascontiguousarray(informed, dtype=np.uint8).reshape(-1)
+    p = np.zeros(informed.size, np.uint32)
+    s = np.zeros(informed.size, np.float32)
+    b = C.c_uint(0)
+    check(lib().qsmi_path(informed.size, informed, seed, p, s, C.byref(b)))
+    return p, s, int(b.value)
+
+
+class TrainingImage:
+    """TI spectra resident on one GPU (SharedMemoryManager + allocAndInitSharedMemory)."""
+
+    def __init__(self, sub_images, dims, gpu=0, fft_dims=None):
+        dims = _u32(dims)
+        sub = _f32(sub_images)
+        self.dims = tuple(int(d) for d in dims)
+        self.nsub = sub.shape[0]
+        self.ndim = len(dims)
+        h = C.c_void_p()
+        fd = None if fft_dims is None else _u32(fft_dims)
+        check(lib().qsmi_ti_create(gpu, len(dims), dims, None if fd is None else fd.ctypes.data, self.nsub,
+                                   sub.reshape(-1), C.byref(h)))
+        self._fd = fd
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().qsmi_ti_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class AcceleratorDevice:
+    """Mirror of G2S's AcceleratorDevice contract over the MI355X scan."""
+
+    def __init__(self, ti, alpha, circular=False):
+        alpha = np.asarray(alpha, np.float32)
+        self.ti = ti
+        self.nsub = alpha.shape[0]
+        a = _f32(alpha.T.reshape(-1))  # [A + B*nSub]
+        h = C.c_void_p()
+        check(lib().qsmi_device_create(ti.h, a, int(circular), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().qsmi_device_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_true_mismatch(self, v):
+        check(lib().qsmi_set_true_mismatch(self.h, int(bool(v))))
+
+    def set_tie_quantum(self, q):
+        check(lib().qsmi_set_tie_quantum(self.h, float(q)))
+
+    def set_stage_timing(self, on):
+        check(lib().qsmi_set_stage_timing(self.h, int(bool(on))))
+
+    def last_stage_ms(self):
+        out = np.zeros(4, np.float32)
+        check(lib().qsmi_last_stage_ms(self.h, out))
+        return out
+
+    def candidate_for_pattern(self, offsets, values, delta0):
+        """offsets (nb, ndim) relative to the cell; values (nb, nSub) converted."""
+        offsets = _i32(offsets).reshape(-1)
+        values = _f32(values).reshape(-1)
+        d0 = _f32([delta0])
+        comp = C.c_int(0)
+        nb = values.size // self.nsub if self.nsub else 0
+        check(lib().qsmi_candidate_for_pattern(self.h, nb, offsets, values, d0, C.byref(comp)))
+        return bool(comp.value)
+
+    def search_k_biggest(self, k, seed):
+        err = np.zeros(k, np.float32)
+        pos = np.zeros(k, np.uint32)
+        check(lib().qsmi_search_k_biggest(self.h, err, pos, k, float(seed)))
+        return err, pos
+
+    def cvt_index_to_position(self, index):
+        out = C.c_uint(0)
+        check(lib().qsmi_cvt_index_to_position(self.h, int(index), C.byref(out)))
+        return out.value
+
+    def cvt_position_to_index(self, position):
+        out = C.c_uint(0)
+        check(lib().qsmi_cvt_position_to_index(self.h, int(position), C.byref(out)))
+        return out.value
+
+    def array_size(self):
+        out = C.c_uint(0)
+        check(lib().qsmi_get_array_size(self.h, C.byref(out)))
+        return out.value
+
+    def error_map(self):
+        out = np.zeros(int(np.prod(self.ti.dims)), np.float32)
+        check(lib().qsmi_get_error_map(self.h, out))
+        return out
+
+    def set_value_in_error_array(self, index, value):
+        check(lib().qsmi_set_value_in_error_array(self.h, int(index), float(value)))
+
+    def synchronize(self):
+        check(lib().qsmi_synchronize(self.h))
+
+    @staticmethod
+    def _pack(nodes):
+        """nodes: list of (offsets (nb,ndim), values (nb,nSub), delta0, seed)."""
+        starts = [0]
+        offs, vals, d0, sd = [], [], [], []
+        for o, v, d, s in nodes:
+            o = np.asarray(o, np.int32)
+            v = np.asarray(v, np.float32)
+            starts.append(starts[-1] + o.shape[0])
+            offs.append(o.reshape(-1))
+            vals.append(v.reshape(-1))
+            d0.append(d)
+            sd.append(s)
+        cat = lambda xs, dt: np.ascontiguousarray(np.concatenate(xs) if xs else np.zeros(1, dt), dt)
+        return (_i32(starts), cat(offs, np.int32), cat(vals, np.float32), _f32(d0), _f32(sd))
+
+    def scan_wave(self, nodes, k):
+        """Scan + select + draw for a wave of independent nodes.
+
+        Returns (errors (n,K), positions (n,K), chosen (n,)) in TI positions."""
+        starts, offs, vals, d0, sd = self._pack(nodes)
+        n = len(nodes)
+        K = int(np.ceil(k))
+        err = np.zeros(n * K, np.float32)
+        pos = np.zeros(n * K, np.uint32)
+        ch = np.zeros(n, np.uint32)
+        check(lib().qsmi_scan_wave(self.h, n, starts, offs, vals, d0, sd, float(k), err, pos, ch.ctypes.data))
+        return err.reshape(n, K), pos.reshape(n, K), ch
+
+    def wave(self, nodes, k):
+        return Wave(self, nodes, k)
+
+
+class Wave:
+    """A path wave resident in HBM (for repeated timed launches)."""
+
+    def __init__(self, dev, nodes, k):
+        starts, offs, vals, d0, sd = AcceleratorDevice._pack(nodes)
+        self.dev = dev
+        self.n = len(nodes)
+        self.K = int(np.ceil(k))
+        h = C.c_void_p()
+        check(lib().qsmi_wave_create(dev.h, self.n, starts, offs, vals, d0, sd, float(k), C.byref(h)))
+        self.h = h
+
+    def run(self):
+        check(lib().qsmi_wave_run(self.h))
+
+    def read(self):
+        err = np.zeros(self.n * self.K, np.float32)
+        pos = np.zeros(self.n * self.K, np.uint32)
+        ch = np.zeros(self.n, np.uint32)
+        check(lib().qsmi_wave_read(self.h, err.ctypes.data, pos.ctypes.data, ch.ctypes.data))
+        return err.reshape(self.n, self.K), pos.reshape(self.n, self.K), ch
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().qsmi_wave_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dims=None, gpu=0,
+            wave_size=256, tie_quantum=-1.0):
+    """In-memory QuickSampling (qsmi_qs_run_in_memory).
+
+    Arrays are flat in G2S DataImage order (dimension 0 fastest, variable
+    fastest of all): ti has prod(ti_dims)*nVar values, di prod(sim_dims)*nVar
+    (NaN = simulate). Returns (simulation, index, stats dict)."""
+    ti = _f32(ti).reshape(-1)
+    di = _f32(di).reshape(-1)
+    var_types = _i32(var_types)
+    nvar = len(var_types)
+    D = len(ti_dims)
+    rq = _lib.QsRequest()
+    rq.gpu = gpu
+    rq.ndim = D
+    for i in range(3):
+        rq.tiSize[i] = int(ti_dims[i]) if i < D else 1
+        rq.simSize[i] = int(sim_dims[i]) if i < D else 1
+        rq.kernelSize[i] = (int(kernel_dims[i]) if (kernel_dims is not None and i < D) else 0)
+    rq.nVar = nvar
+    rq.varTypes = var_types.ctypes.data
+    rq.ti = ti.ctypes.data
+    rq.di = di.ctypes.data
+    kern = None
+    if kernel is not None:
+        kern = _f32(kernel).reshape(-1)
+        rq.kernel = kern.ctypes.data
+    rq.n = int(n)
+    rq.k = float(k)
+    rq.seed = int(seed)
+    rq.waveSize = int(wave_size)
+    rq.tieQuantum = float(tie_quantum)
+    sim = np.empty_like(di)
+    idx = np.zeros(int(np.prod(sim_dims)), np.uint32)
+    rs = _lib.QsResult()
+    rs.simulation = sim.ctypes.data
+    rs.index = idx.ctypes.data
+    check(lib().qsmi_qs_run_in_memory(C.byref(rq), C.byref(rs)))
+    stats = dict(setup_ms=rs.setupMs, path_loop_ms=rs.pathLoopMs, scan_ms=rs.scanMs, nodes=rs.nodes,
+                 waves=rs.waves)
+    return sim, idx, stats

@@ -1,0 +1,115 @@
+"""ctypes bindings of libqsmi.so (include/qsmi.h).
+
+The HIP library is the only compute path: loading fails loudly when the
+library was not built (run ``python -m g2s_amd.build``), and there is no CPU
+fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqsmi.so")
+
+_LIB = None
+
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+vp = C.c_void_p
+
+# every symbol include/qsmi.h declares: name -> (restype, argtypes)
+SIGNATURES = {
+    "qsmi_last_error": (C.c_char_p, []),
+    "qsmi_get_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "qsmi_fft_size": (C.c_uint, [C.c_uint]),
+    "qsmi_ti_create": (C.c_int, [C.c_int, C.c_int, u32p, vp, C.c_int, f32p, C.POINTER(vp)]),
+    "qsmi_ti_destroy": (C.c_int, [vp]),
+    "qsmi_device_create": (C.c_int, [vp, f32p, C.c_int, C.POINTER(vp)]),
+    "qsmi_device_destroy": (C.c_int, [vp]),
+    "qsmi_set_true_mismatch": (C.c_int, [vp, C.c_int]),
+    "qsmi_set_tie_quantum": (C.c_int, [vp, C.c_float]),
+    "qsmi_candidate_for_pattern": (C.c_int, [vp, C.c_int, i32p, f32p, f32p, C.POINTER(C.c_int)]),
+    "qsmi_search_k_biggest": (C.c_int, [vp, f32p, u32p, C.c_uint, C.c_float]),
+    "qsmi_cvt_index_to_position": (C.c_int, [vp, C.c_uint, C.POINTER(C.c_uint)]),
+    "qsmi_cvt_position_to_index": (C.c_int, [vp, C.c_uint, C.POINTER(C.c_uint)]),
+    "qsmi_get_array_size": (C.c_int, [vp, C.POINTER(C.c_uint)]),
+    "qsmi_get_error_map": (C.c_int, [vp, f32p]),
+    "qsmi_set_value_in_error_array": (C.c_int, [vp, C.c_uint, C.c_float]),
+    "qsmi_scan_wave": (C.c_int, [vp, C.c_int, i32p, i32p, f32p, f32p, f32p, C.c_float, f32p, u32p, vp]),
+    "qsmi_wave_create": (C.c_int, [vp, C.c_int, i32p, i32p, f32p, f32p, f32p, C.c_float, C.POINTER(vp)]),
+    "qsmi_wave_run": (C.c_int, [vp]),
+    "qsmi_wave_read": (C.c_int, [vp, vp, vp, vp]),
+    "qsmi_wave_destroy": (C.c_int, [vp]),
+    "qsmi_synchronize": (C.c_int, [vp]),
+    "qsmi_last_stage_ms": (C.c_int, [vp, f32p]),
+    "qsmi_set_stage_timing": (C.c_int, [vp, C.c_int]),
+    "qsmi_categories": (C.c_int, [C.c_size_t, C.c_int, f32p, C.c_int, f32p, C.c_int, C.POINTER(C.c_int)]),
+    "qsmi_coef_matrix": (C.c_int, [C.c_int, i32p, i32p, f32p, i32p, i32p, C.POINTER(C.c_int)]),
+    "qsmi_ti_subimages": (C.c_int, [C.c_int, u32p, C.c_int, i32p, f32p, i32p, f32p, f32p]),
+    "qsmi_offsets": (C.c_int, [C.c_int, u32p, C.c_int, vp, i32p, C.c_int, C.POINTER(C.c_int)]),
+    "qsmi_path": (C.c_int, [C.c_size_t, u8p, C.c_uint, u32p, f32p, C.POINTER(C.c_uint)]),
+    "qsmi_qs_run_in_memory": (C.c_int, [vp, vp]),
+}
+
+
+class QsmiError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libqsmi.so; raises when it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise QsmiError(f"{LIB_PATH} is missing: build it with `python -m g2s_amd.build` "
+                        "(the HIP library is the only implementation)")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().qsmi_last_error().decode(errors="replace")
+        raise QsmiError(f"qsmi error {rc}: {msg}")
+    return rc
+
+
+class QsRequest(C.Structure):
+    _fields_ = [
+        ("gpu", C.c_int),
+        ("ndim", C.c_int),
+        ("tiSize", C.c_uint * 3),
+        ("nVar", C.c_int),
+        ("varTypes", vp),
+        ("ti", vp),
+        ("simSize", C.c_uint * 3),
+        ("di", vp),
+        ("kernelSize", C.c_uint * 3),
+        ("kernel", vp),
+        ("n", C.c_int),
+        ("k", C.c_float),
+        ("seed", C.c_uint),
+        ("waveSize", C.c_int),
+        ("tieQuantum", C.c_float),
+    ]
+
+
+class QsResult(C.Structure):
+    _fields_ = [
+        ("simulation", vp),
+        ("index", vp),
+        ("setupMs", C.c_double),
+        ("pathLoopMs", C.c_double),
+        ("scanMs", C.c_double),
+        ("nodes", C.c_longlong),
+        ("waves", C.c_int),
+    ]

@@ -1,0 +1,891 @@
+// C-ABI of the MI355X QuickSampling exact scan (include/qsmi.h).
+// Host side: TI spectra set-up, per-node and per-wave scans, error codes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "../../include/qsmi.h"
+#include "qsmi_kernels.hpp"
+
+namespace qsmi {
+hipError_t set_lds_limits();
+void launch_r2c_rows(const float* in, float2* out, const FftPlan& plan, const float2* tw, int lognv, long long nrows,
+                     int h0, hipStream_t st);
+void launch_colfft(int dir, float2* base, const FftPlan& plan, const float2* tw, int lognv, long long rowStride,
+                   int ncols, long long planes, int pdiv, long long pOuter, long long pInner, const NodeInfo* info,
+                   int planesPerNode, int validDim, hipStream_t st);
+void launch_pencil(const ScanGeom& g, const FftPlan& planP, int lognv, const float2* Yhat, const float2* tw0,
+                   const float2* tw1, const float2* twP, int nNodes, int npb, const NodeInfo* info, const int* nbOff,
+                   const float* nbVal, float2* C, hipStream_t st);
+void launch_direct1d(const ScanGeom& g, const float2* Yhat, const float2* tw0, int nNodes, const NodeInfo* info,
+                     const int* nbOff, const float* nbVal, float2* C, hipStream_t st);
+void launch_rows_select(const ScanGeom& g, const FftPlan& plan0, int lognv, const float2* tw0, const NodeInfo* info,
+                        const float2* C, int nNodes, int rowBlocks, int K, Cand* cand, float* map, hipStream_t st);
+void launch_select_map(const float* map, long long S, int K, float quantum, uint32_t seedbits, Cand* cand, int chunks,
+                       hipStream_t st);
+void launch_merge_draw(const ScanGeom& g, const NodeInfo* info, const Cand* cand, int nNodes, int candBlocks,
+                       int rowsPerBlock, int K, float kf, float* outVal, uint32_t* outPos, uint32_t* chosen,
+                       hipStream_t st);
+void launch_fill(float* p, long long n, float v, hipStream_t st);
+}  // namespace qsmi
+
+using namespace qsmi;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                                  \
+  do {                                                                                                \
+    hipError_t _e = (expr);                                                                           \
+    if (_e != hipSuccess)                                                                             \
+      return fail(QSMI_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));                  \
+  } while (0)
+
+unsigned size_rule(unsigned src) {
+  if (src <= 1) return src;
+  unsigned limit = 1;
+  while (limit < src) limit <<= 1;
+  for (unsigned cand = src; cand <= limit; ++cand) {
+    unsigned v = cand;
+    for (unsigned p = 2; p < 8; ++p)
+      while (v % p == 0) v /= p;
+    if (v % 11 == 0)
+      v /= 11;
+    else if (v % 13 == 0)
+      v /= 13;
+    if (v == 1) return cand;
+  }
+  return src;
+}
+
+// Radix plan of one FFT length; prefers 10, 8, 4 (fewest LDS passes).
+bool make_plan(int L, FftPlan* p) {
+  p->L = L;
+  p->nst = 0;
+  int m = L;
+  static const int pref[] = {10, 8, 4, 5, 3, 2, 7, 11, 13};
+  while (m > 1) {
+    bool ok = false;
+    for (int r : pref) {
+      if (m % r == 0) {
+        if (p->nst >= kMaxStages) return false;
+        p->radix[p->nst++] = r;
+        m /= r;
+        ok = true;
+        break;
+      }
+    }
+    if (!ok) return false;
+  }
+  if (L == 1) {  // a length-1 transform is the identity: one radix-1 stage is not needed
+    p->nst = 0;
+  }
+  return true;
+}
+
+// largest power-of-two vector count with L*nvec <= kMaxVecElems and LDS <= 64 KiB
+int choose_lognv(int L) {
+  int lognv = 0;
+  while (true) {
+    int nv = 1 << (lognv + 1);
+    if ((long long)L * nv > kMaxVecElems) break;
+    if ((size_t)(L + 2 * L * nv) * sizeof(float2) > 64 * 1024) break;
+    if (nv > 64) break;
+    lognv++;
+  }
+  return lognv;
+}
+
+unsigned seed_from_unit_float(float seed) {
+  if (!std::isfinite(seed) || seed <= 0.f) return 0;
+  if (seed >= 1.f) return std::numeric_limits<unsigned>::max();
+  return (unsigned)std::floor((double)seed * (double)std::numeric_limits<unsigned>::max());
+}
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+  return hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T));
+}
+
+}  // namespace
+
+struct qsmi_ti {
+  int gpu = 0;
+  int D = 0;
+  int s[3] = {1, 1, 1};
+  int f[3] = {1, 1, 1};
+  int h0 = 1;
+  long long S = 1, N = 1, H = 1;
+  int nSub = 0;
+  float2* dYhat = nullptr;
+  float2* dTw[3] = {nullptr, nullptr, nullptr};
+  FftPlan plan[3];
+  int lognv[3] = {0, 0, 0};
+  hipStream_t stream = nullptr;
+  int refs = 0;
+};
+
+struct WaveBufs;
+
+struct qsmi_device {
+  qsmi_ti* ti = nullptr;
+  hipStream_t stream = nullptr;
+  std::vector<float> alpha;  // nSub x nSub, [A + B*nSub]
+  std::vector<int> active;   // TI sub-variables with a non-zero alpha row
+  int circular = 0;
+  int trueMismatch = 1;
+  float quantum = 0.f;
+  int mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+  // per-node map state
+  float* dMap = nullptr;
+  bool mapValid = false;
+  // workspace
+  float2* dC = nullptr;
+  size_t capC = 0;
+  Cand* dCand = nullptr;
+  size_t capCand = 0;
+  NodeInfo* dInfo = nullptr;
+  size_t capInfo = 0;
+  int* dOff = nullptr;
+  size_t capOff = 0;
+  float* dVal = nullptr;
+  size_t capVal = 0;
+  float* dOutVal = nullptr;
+  uint32_t* dOutPos = nullptr;
+  uint32_t* dChosen = nullptr;
+  size_t capOut = 0;
+  size_t capOutPos = 0;
+  bool registered = false;
+  // stage timing
+  bool timing = false;
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  float stageMs[4] = {0, 0, 0, 0};
+  WaveBufs* wb = nullptr;  // workspace of qsmi_scan_wave
+};
+
+// Device buffers of one wave (grown on demand) and pinned host staging.
+struct WaveBufs {
+  size_t capNodes = 0, capNb = 0, capC = 0, capCand = 0;
+  NodeInfo* dInfo = nullptr;
+  int* dOff = nullptr;
+  float* dVal = nullptr;
+  float2* dC = nullptr;
+  Cand* dCand = nullptr;
+  float* dOutVal = nullptr;
+  uint32_t* dOutPos = nullptr;
+  uint32_t* dChosen = nullptr;
+  NodeInfo* hInfo = nullptr;  // pinned
+  int* hOff = nullptr;        // pinned
+  float* hVal = nullptr;      // pinned
+  int nNodes = 0, K = 0, rowBlocks = 0;
+  float k = 0.f;
+};
+
+struct qsmi_wave {
+  qsmi_device* dev = nullptr;
+  WaveBufs b;
+};
+
+namespace {
+void free_wave_bufs(WaveBufs* b);
+}
+
+namespace {
+
+ScanGeom geom_of(const qsmi_device* dev) {
+  const qsmi_ti* t = dev->ti;
+  ScanGeom g;
+  memset(&g, 0, sizeof(g));
+  g.D = t->D;
+  for (int i = 0; i < 3; ++i) {
+    g.s[i] = t->s[i];
+    g.f[i] = t->f[i];
+  }
+  g.h0 = t->h0;
+  g.P = t->D - 1;
+  g.V = (int)dev->active.size();
+  for (int a = 0; a < g.V; ++a) g.A[a] = dev->active[a];
+  g.circular = dev->circular;
+  g.invN = (float)(1.0 / (double)t->N);
+  g.H = t->H;
+  return g;
+}
+
+int rows_per_block(const qsmi_ti* t) { return 2 << t->lognv[0]; }
+
+int row_blocks(const qsmi_ti* t) {
+  long long rows = (long long)t->s[1] * t->s[2];
+  return (int)((rows + rows_per_block(t) - 1) / rows_per_block(t));
+}
+
+// Node preparation on the host: bounding box, valid range, alpha-combined
+// values v_ja = sum_B alpha(A_a, B) u_jB.
+void prep_node(const qsmi_device* dev, int nb, const int* off, const float* u, float delta0, float seed,
+               int nbStart, NodeInfo* ni, int* offOut, float* valOut, int* mnOut, int* mxOut) {
+  const qsmi_ti* t = dev->ti;
+  const int D = t->D, nSub = t->nSub, V = (int)dev->active.size();
+  int mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+  for (int j = 0; j < nb; ++j)
+    for (int i = 0; i < D; ++i) {
+      mn[i] = std::min(mn[i], off[j * D + i]);
+      mx[i] = std::max(mx[i], off[j * D + i]);
+    }
+  for (int i = 0; i < 3; ++i) {
+    if (i >= D) {
+      ni->lo[i] = ni->hi[i] = 0;
+    } else if (dev->circular) {
+      ni->lo[i] = 0;
+      ni->hi[i] = t->s[i] - 1;
+    } else {
+      ni->lo[i] = -mn[i];
+      ni->hi[i] = t->s[i] - 1 - mx[i];
+    }
+  }
+  if (mnOut)
+    for (int i = 0; i < 3; ++i) {
+      mnOut[i] = mn[i];
+      mxOut[i] = mx[i];
+    }
+  ni->nbStart = nbStart;
+  ni->nb = nb;
+  ni->scale = dev->trueMismatch ? (float)(1.0 / (double)t->N) : 1.f;
+  ni->delta0 = dev->trueMismatch ? delta0 : 0.f;
+  ni->seed = seed;
+  ni->seedbits = seed_from_unit_float(seed);
+  ni->quantum = dev->quantum;
+  ni->cell = 0;
+  for (int j = 0; j < nb; ++j) {
+    for (int i = 0; i < D; ++i) offOut[j * D + i] = off[j * D + i];
+    for (int a = 0; a < V; ++a) {
+      const int A = dev->active[a];
+      float acc = 0.f;
+      for (int B = 0; B < nSub; ++B) acc += dev->alpha[A + B * nSub] * u[j * nSub + B];
+      valOut[j * V + a] = acc;
+    }
+  }
+}
+
+template <typename T>
+int ensure(T** p, size_t* cap, size_t need) {
+  if (need <= *cap) return QSMI_OK;
+  if (*p) hipFree(*p);
+  *p = nullptr;
+  size_t n = std::max(need, *cap * 3 / 2);
+  HIPCHK(dalloc(p, n));
+  *cap = n;
+  return QSMI_OK;
+}
+
+// Launch the spectrum stage and the dim1 inverse (3-D) for nNodes nodes.
+void launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeInfo* info, const int* off,
+                     const float* val, float2* C) {
+  qsmi_ti* t = dev->ti;
+  hipStream_t st = dev->stream;
+  if (dev->timing) hipEventRecord(dev->ev[0], st);
+  if (t->D == 1) {
+    launch_direct1d(g, t->dYhat, t->dTw[0], nNodes, info, off, val, C, st);
+  } else {
+    const int P = t->D - 1;
+    launch_pencil(g, t->plan[P], t->lognv[P], t->dYhat, t->dTw[0], t->dTw[1], t->dTw[P], nNodes, 1, info, off, val, C,
+                  st);
+  }
+  if (dev->timing) hipEventRecord(dev->ev[1], st);
+  if (t->D == 3) {
+    // inverse along dim1 of the valid c2 planes: plane p = node*f2 + c2
+    launch_colfft(+1, C, t->plan[1], t->dTw[1], t->lognv[1], t->h0, t->h0, (long long)nNodes * t->f[2], t->f[2],
+                  t->H, (long long)t->f[1] * t->h0, info, t->f[2], 2, st);
+  }
+  if (dev->timing) hipEventRecord(dev->ev[2], st);
+}
+
+int check_dev(qsmi_device* dev) {
+  if (!dev || !dev->ti) return fail(QSMI_ERR_ARG, "null device");
+  HIPCHK(hipSetDevice(dev->ti->gpu));
+  return QSMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// error channel shared with the host layer (not part of the public header)
+__attribute__((visibility("hidden"))) int qsmi_internal_fail(int code, const char* msg) { return fail(code, msg); }
+
+const char* qsmi_last_error(void) { return g_err.c_str(); }
+
+int qsmi_get_device_count(int* count) {
+  if (!count) return fail(QSMI_ERR_ARG, "null count");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  *count = c;
+  return QSMI_OK;
+}
+
+unsigned qsmi_fft_size(unsigned src) { return size_rule(src); }
+
+int qsmi_ti_create(int gpu, int ndim, const unsigned* srcSize, const unsigned* fftSize, int nSub,
+                   const float* subImages, qsmi_ti** out) {
+  if (!out || !srcSize || !subImages) return fail(QSMI_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (ndim < 1 || ndim > 3) return fail(QSMI_ERR_UNSUPPORTED, "ndim must be 1, 2 or 3");
+  if (nSub < 1 || nSub > kMaxSub) return fail(QSMI_ERR_UNSUPPORTED, "nSub must be in [1, 16]");
+  auto* t = new qsmi_ti();
+  t->gpu = gpu;
+  t->D = ndim;
+  for (int i = 0; i < ndim; ++i) {
+    if (srcSize[i] < 1) {
+      delete t;
+      return fail(QSMI_ERR_ARG, "zero TI extent");
+    }
+    t->s[i] = (int)srcSize[i];
+    t->f[i] = fftSize ? (int)fftSize[i] : (int)size_rule(srcSize[i]);
+    if (t->f[i] < t->s[i]) {
+      delete t;
+      return fail(QSMI_ERR_ARG, "fftSize smaller than srcSize");
+    }
+    if (t->f[i] > 4096) {
+      delete t;
+      return fail(QSMI_ERR_UNSUPPORTED, "FFT extent above 4096");
+    }
+  }
+  if (ndim >= 2 && t->f[0] < 2) {
+    delete t;
+    return fail(QSMI_ERR_UNSUPPORTED, "dimension 0 of extent 1 in a multi-dimensional TI");
+  }
+  t->h0 = t->f[0] / 2 + 1;
+  t->S = (long long)t->s[0] * t->s[1] * t->s[2];
+  t->N = (long long)t->f[0] * t->f[1] * t->f[2];
+  t->H = (long long)t->h0 * t->f[1] * t->f[2];
+  t->nSub = nSub;
+  for (int i = 0; i < 3; ++i) {
+    if (!make_plan(t->f[i], &t->plan[i])) {
+      delete t;
+      return fail(QSMI_ERR_UNSUPPORTED, "FFT extent with a prime factor above 13");
+    }
+    t->lognv[i] = choose_lognv(t->f[i]);
+  }
+  hipError_t e = hipSetDevice(gpu);
+  if (e != hipSuccess) {
+    delete t;
+    return fail(QSMI_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  }
+  static bool ldsSet = false;
+  if (!ldsSet) {
+    e = set_lds_limits();
+    if (e != hipSuccess) {
+      delete t;
+      return fail(QSMI_ERR_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+    }
+    ldsSet = true;
+  }
+#define TCHK(expr)                                                                  \
+  do {                                                                              \
+    hipError_t _e = (expr);                                                         \
+    if (_e != hipSuccess) {                                                         \
+      qsmi_ti_destroy(t);                                                           \
+      return fail(QSMI_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    }                                                                               \
+  } while (0)
+  TCHK(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
+  // twiddle tables, FP64-generated: tw[k] = exp(-2 pi i k / f)
+  for (int i = 0; i < 3; ++i) {
+    std::vector<float2> tw(t->f[i]);
+    for (int k = 0; k < t->f[i]; ++k) {
+      long double a = -6.283185307179586476925286766559L * (long double)k / (long double)t->f[i];
+      tw[k] = make_float2((float)cosl(a), (float)sinl(a));
+    }
+    TCHK(dalloc(&t->dTw[i], tw.size()));
+    TCHK(hipMemcpy(t->dTw[i], tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+  }
+  // zero-padded real images on the f-grid
+  std::vector<float> pad((size_t)nSub * t->N, 0.f);
+  for (int A = 0; A < nSub; ++A)
+    for (int c2 = 0; c2 < t->s[2]; ++c2)
+      for (int c1 = 0; c1 < t->s[1]; ++c1) {
+        const float* src = subImages + (size_t)A * t->S + ((size_t)c2 * t->s[1] + c1) * t->s[0];
+        float* dst = pad.data() + (size_t)A * t->N + ((size_t)c2 * t->f[1] + c1) * t->f[0];
+        memcpy(dst, src, sizeof(float) * t->s[0]);
+      }
+  float* dReal = nullptr;
+  TCHK(dalloc(&dReal, pad.size()));
+  TCHK(hipMemcpy(dReal, pad.data(), pad.size() * sizeof(float), hipMemcpyHostToDevice));
+  TCHK(dalloc(&t->dYhat, (size_t)nSub * t->H));
+  // rows r2c along dim0: rows (A, k2, k1)
+  const long long nrows = (long long)nSub * t->f[1] * t->f[2];
+  launch_r2c_rows(dReal, t->dYhat, t->plan[0], t->dTw[0], t->lognv[0], nrows, t->h0, t->stream);
+  TCHK(hipGetLastError());
+  if (ndim >= 2) {  // along dim1: planes (A, k2), stride f1*h0
+    launch_colfft(-1, t->dYhat, t->plan[1], t->dTw[1], t->lognv[1], t->h0, t->h0, (long long)nSub * t->f[2], 1,
+                  (long long)t->f[1] * t->h0, 0, nullptr, 1, 0, t->stream);
+    TCHK(hipGetLastError());
+  }
+  if (ndim == 3) {  // along dim2: planes (A, k1)
+    launch_colfft(-1, t->dYhat, t->plan[2], t->dTw[2], t->lognv[2], (long long)t->f[1] * t->h0, t->h0,
+                  (long long)nSub * t->f[1], t->f[1], (long long)t->f[2] * t->f[1] * t->h0, t->h0, nullptr, 1, 0,
+                  t->stream);
+    TCHK(hipGetLastError());
+  }
+  TCHK(hipStreamSynchronize(t->stream));
+  hipFree(dReal);
+#undef TCHK
+  *out = t;
+  return QSMI_OK;
+}
+
+int qsmi_ti_destroy(qsmi_ti* t) {
+  if (!t) return QSMI_OK;
+  if (t->refs > 0) return fail(QSMI_ERR_ARG, "TI still used by a device");
+  hipSetDevice(t->gpu);
+  if (t->stream) hipStreamSynchronize(t->stream);
+  if (t->dYhat) hipFree(t->dYhat);
+  for (int i = 0; i < 3; ++i)
+    if (t->dTw[i]) hipFree(t->dTw[i]);
+  if (t->stream) hipStreamDestroy(t->stream);
+  delete t;
+  return QSMI_OK;
+}
+
+int qsmi_device_create(qsmi_ti* ti, const float* alpha, int circularTI, qsmi_device** out) {
+  if (!ti || !alpha || !out) return fail(QSMI_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (circularTI)
+    for (int i = 0; i < ti->D; ++i)
+      if (ti->f[i] != ti->s[i]) return fail(QSMI_ERR_UNSUPPORTED, "circular TI needs fftSize == srcSize");
+  auto* d = new qsmi_device();
+  d->ti = ti;
+  const int nSub = ti->nSub;
+  d->alpha.assign(alpha, alpha + nSub * nSub);
+  for (int A = 0; A < nSub; ++A) {
+    bool need = false;
+    for (int B = 0; B < nSub; ++B) need |= (alpha[A + B * nSub] != 0.f);
+    if (need) d->active.push_back(A);
+  }
+  d->circular = circularTI;
+  hipError_t e = hipSetDevice(ti->gpu);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = dalloc(&d->dMap, (size_t)ti->S);
+  for (int i = 0; i < 5 && e == hipSuccess; ++i) e = hipEventCreate(&d->ev[i]);
+  if (e != hipSuccess) {
+    qsmi_device_destroy(d);
+    return fail(QSMI_ERR_HIP, std::string("device create: ") + hipGetErrorString(e));
+  }
+  ti->refs++;
+  d->registered = true;
+  *out = d;
+  return QSMI_OK;
+}
+
+int qsmi_device_destroy(qsmi_device* d) {
+  if (!d) return QSMI_OK;
+  if (d->ti) hipSetDevice(d->ti->gpu);
+  if (d->stream) hipStreamSynchronize(d->stream);
+  hipFree(d->dMap);
+  hipFree(d->dC);
+  hipFree(d->dCand);
+  hipFree(d->dInfo);
+  hipFree(d->dOff);
+  hipFree(d->dVal);
+  hipFree(d->dOutVal);
+  hipFree(d->dOutPos);
+  hipFree(d->dChosen);
+  if (d->wb) {
+    free_wave_bufs(d->wb);
+    delete d->wb;
+  }
+  for (int i = 0; i < 5; ++i)
+    if (d->ev[i]) (void)hipEventDestroy(d->ev[i]);
+  if (d->stream) hipStreamDestroy(d->stream);
+  if (d->ti && d->registered) d->ti->refs--;
+  delete d;
+  return QSMI_OK;
+}
+
+int qsmi_set_true_mismatch(qsmi_device* dev, int value) {
+  if (!dev) return fail(QSMI_ERR_ARG, "null device");
+  dev->trueMismatch = value ? 1 : 0;
+  return QSMI_OK;
+}
+
+int qsmi_set_tie_quantum(qsmi_device* dev, float quantum) {
+  if (!dev || !(quantum >= 0.f)) return fail(QSMI_ERR_ARG, "invalid quantum");
+  dev->quantum = quantum;
+  return QSMI_OK;
+}
+
+int qsmi_set_stage_timing(qsmi_device* dev, int enable) {
+  if (!dev) return fail(QSMI_ERR_ARG, "null device");
+  dev->timing = enable != 0;
+  return QSMI_OK;
+}
+
+int qsmi_last_stage_ms(qsmi_device* dev, float* out4) {
+  if (!dev || !out4) return fail(QSMI_ERR_ARG, "null argument");
+  for (int i = 0; i < 4; ++i) out4[i] = dev->stageMs[i];
+  return QSMI_OK;
+}
+
+int qsmi_synchronize(qsmi_device* dev) {
+  int rc = check_dev(dev);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(dev->stream));
+  return QSMI_OK;
+}
+
+int qsmi_candidate_for_pattern(qsmi_device* dev, int nb, const int* offsets, const float* values,
+                               const float* delta0, int* computed) {
+  int rc = check_dev(dev);
+  if (rc) return rc;
+  if (computed) *computed = 0;
+  if (nb < 0) return fail(QSMI_ERR_ARG, "negative neighbour count");
+  if (nb == 0) return QSMI_OK;
+  if (!offsets || !values || !delta0) return fail(QSMI_ERR_ARG, "null argument");
+  qsmi_ti* t = dev->ti;
+  const int V = (int)dev->active.size();
+  NodeInfo ni;
+  std::vector<int> off((size_t)nb * t->D);
+  std::vector<float> val((size_t)nb * std::max(V, 1));
+  prep_node(dev, nb, offsets, values, delta0[0], 0.f, 0, &ni, off.data(), val.data(), dev->mn, dev->mx);
+  if ((rc = ensure(&dev->dInfo, &dev->capInfo, 1))) return rc;
+  if ((rc = ensure(&dev->dOff, &dev->capOff, off.size()))) return rc;
+  if ((rc = ensure(&dev->dVal, &dev->capVal, val.size()))) return rc;
+  if ((rc = ensure(&dev->dC, &dev->capC, (size_t)t->H))) return rc;
+  hipStream_t st = dev->stream;
+  HIPCHK(hipMemcpyAsync(dev->dInfo, &ni, sizeof(ni), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dev->dOff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dev->dVal, val.data(), val.size() * sizeof(float), hipMemcpyHostToDevice, st));
+  launch_fill(dev->dMap, t->S, -INFINITY, st);
+  ScanGeom g = geom_of(dev);
+  launch_spectrum(dev, g, 1, dev->dInfo, dev->dOff, dev->dVal, dev->dC);
+  launch_rows_select(g, t->plan[0], t->lognv[0], t->dTw[0], dev->dInfo, dev->dC, 1, row_blocks(t), 1, nullptr,
+                     dev->dMap, st);
+  HIPCHK(hipGetLastError());
+  // the host arrays above are pageable and go out of scope: finish the copies
+  HIPCHK(hipStreamSynchronize(st));
+  dev->mapValid = true;
+  if (computed) *computed = 1;
+  return QSMI_OK;
+}
+
+int qsmi_search_k_biggest(qsmi_device* dev, float* errors, unsigned* encodedPosition, unsigned extendK, float seed) {
+  int rc = check_dev(dev);
+  if (rc) return rc;
+  if (!errors || !encodedPosition) return fail(QSMI_ERR_ARG, "null argument");
+  if (extendK < 1 || extendK > (unsigned)kMaxK) return fail(QSMI_ERR_UNSUPPORTED, "extendK must be in [1, 16]");
+  if (!dev->mapValid) return fail(QSMI_ERR_ARG, "no map: call qsmi_candidate_for_pattern first");
+  qsmi_ti* t = dev->ti;
+  const int K = (int)extendK;
+  const int chunks = 512;
+  if ((rc = ensure(&dev->dCand, &dev->capCand, (size_t)chunks * K))) return rc;
+  if ((rc = ensure(&dev->dOutVal, &dev->capOut, (size_t)K))) return rc;
+  if ((rc = ensure(&dev->dOutPos, &dev->capOutPos, (size_t)K))) return rc;
+  hipStream_t st = dev->stream;
+  NodeInfo ni;
+  memset(&ni, 0, sizeof(ni));
+  ni.seed = seed;
+  ni.seedbits = seed_from_unit_float(seed);
+  ni.quantum = dev->quantum;
+  if ((rc = ensure(&dev->dInfo, &dev->capInfo, 1))) return rc;
+  HIPCHK(hipMemcpyAsync(dev->dInfo, &ni, sizeof(ni), hipMemcpyHostToDevice, st));
+  launch_select_map(dev->dMap, t->S, K, dev->quantum, ni.seedbits, dev->dCand, chunks, st);
+  ScanGeom g = geom_of(dev);
+  launch_merge_draw(g, dev->dInfo, dev->dCand, 1, chunks, 0, K, (float)K, dev->dOutVal, dev->dOutPos, nullptr, st);
+  HIPCHK(hipGetLastError());
+  std::vector<uint32_t> pos(K);
+  HIPCHK(hipMemcpyAsync(errors, dev->dOutVal, K * sizeof(float), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(pos.data(), dev->dOutPos, K * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int i = 0; i < K; ++i) {
+    if (pos[i] == 0xFFFFFFFFu) {
+      encodedPosition[i] = UINT_MAX;
+      continue;
+    }
+    unsigned idx;
+    qsmi_cvt_position_to_index(dev, pos[i], &idx);
+    encodedPosition[i] = idx;
+  }
+  return QSMI_OK;
+}
+
+// src/CPUThreadDevice.cpp:257-268
+int qsmi_cvt_index_to_position(qsmi_device* dev, unsigned index, unsigned* position) {
+  if (!dev || !position) return fail(QSMI_ERR_ARG, "null argument");
+  const qsmi_ti* t = dev->ti;
+  unsigned pos = 0;
+  unsigned div = (unsigned)t->N;
+  for (int i = t->D - 1; i >= 0; --i) {
+    const unsigned f = t->f[i], s = t->s[i];
+    div /= f;
+    pos = pos * s + ((f - (index / div) % f - dev->mn[i] - 1) + s) % s;
+  }
+  *position = pos;
+  return QSMI_OK;
+}
+
+// src/CPUThreadDevice.cpp:270-284
+int qsmi_cvt_position_to_index(qsmi_device* dev, unsigned position, unsigned* index) {
+  if (!dev || !index) return fail(QSMI_ERR_ARG, "null argument");
+  const qsmi_ti* t = dev->ti;
+  unsigned idx = 0;
+  unsigned div = (unsigned)t->S;
+  for (int i = t->D - 1; i >= 0; --i) {
+    const unsigned f = t->f[i], s = t->s[i];
+    div /= s;
+    idx = idx * f + (f - (position / div + dev->mn[i]) % s - 1);
+  }
+  *index = idx;
+  return QSMI_OK;
+}
+
+int qsmi_get_array_size(qsmi_device* dev, unsigned* size) {
+  if (!dev || !size) return fail(QSMI_ERR_ARG, "null argument");
+  *size = (unsigned)dev->ti->N;
+  return QSMI_OK;
+}
+
+int qsmi_get_error_map(qsmi_device* dev, float* out) {
+  int rc = check_dev(dev);
+  if (rc) return rc;
+  if (!out) return fail(QSMI_ERR_ARG, "null argument");
+  if (!dev->mapValid) return fail(QSMI_ERR_ARG, "no map: call qsmi_candidate_for_pattern first");
+  HIPCHK(hipMemcpyAsync(out, dev->dMap, dev->ti->S * sizeof(float), hipMemcpyDeviceToHost, dev->stream));
+  HIPCHK(hipStreamSynchronize(dev->stream));
+  return QSMI_OK;
+}
+
+int qsmi_set_value_in_error_array(qsmi_device* dev, unsigned index, float value) {
+  int rc = check_dev(dev);
+  if (rc) return rc;
+  if (!dev->mapValid) return fail(QSMI_ERR_ARG, "no map: call qsmi_candidate_for_pattern first");
+  if (index >= (unsigned)dev->ti->N) return fail(QSMI_ERR_ARG, "index out of range");
+  unsigned pos;
+  qsmi_cvt_index_to_position(dev, index, &pos);
+  if (pos >= (unsigned)dev->ti->S) return fail(QSMI_ERR_ARG, "index maps outside the TI");
+  HIPCHK(hipMemcpyAsync(dev->dMap + pos, &value, sizeof(float), hipMemcpyHostToDevice, dev->stream));
+  HIPCHK(hipStreamSynchronize(dev->stream));
+  return QSMI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// waves
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+void free_wave_bufs(WaveBufs* b) {
+  (void)hipFree(b->dInfo);
+  (void)hipFree(b->dOff);
+  (void)hipFree(b->dVal);
+  (void)hipFree(b->dC);
+  (void)hipFree(b->dCand);
+  (void)hipFree(b->dOutVal);
+  (void)hipFree(b->dOutPos);
+  (void)hipFree(b->dChosen);
+  (void)hipHostFree(b->hInfo);
+  (void)hipHostFree(b->hOff);
+  (void)hipHostFree(b->hVal);
+  *b = WaveBufs();
+}
+
+int grow_wave_bufs(qsmi_device* dev, WaveBufs* b, int nNodes, int total, int K) {
+  qsmi_ti* t = dev->ti;
+  const int V = std::max((int)dev->active.size(), 1);
+  const int rb = row_blocks(t);
+  if ((size_t)nNodes > b->capNodes || (size_t)nNodes * rb * K > b->capCand) {
+    size_t n = std::max<size_t>(nNodes, b->capNodes);
+    (void)hipFree(b->dInfo);
+    (void)hipFree(b->dOutVal);
+    (void)hipFree(b->dOutPos);
+    (void)hipFree(b->dChosen);
+    (void)hipFree(b->dCand);
+    (void)hipHostFree(b->hInfo);
+    b->dInfo = nullptr;
+    b->dOutVal = nullptr;
+    b->dOutPos = nullptr;
+    b->dChosen = nullptr;
+    b->dCand = nullptr;
+    b->hInfo = nullptr;
+    b->capNodes = 0;
+    b->capCand = 0;
+    HIPCHK(dalloc(&b->dInfo, n));
+    HIPCHK(dalloc(&b->dOutVal, n * kMaxK));
+    HIPCHK(dalloc(&b->dOutPos, n * kMaxK));
+    HIPCHK(dalloc(&b->dChosen, n));
+    HIPCHK(dalloc(&b->dCand, n * rb * kMaxK));
+    HIPCHK(hipHostMalloc((void**)&b->hInfo, n * sizeof(NodeInfo), hipHostMallocDefault));
+    b->capNodes = n;
+    b->capCand = n * rb * kMaxK;
+  }
+  if ((size_t)nNodes * t->H > b->capC) {
+    (void)hipFree(b->dC);
+    b->dC = nullptr;
+    b->capC = 0;
+    size_t n = (size_t)nNodes * t->H;
+    hipError_t e = dalloc(&b->dC, n);
+    if (e != hipSuccess) return fail(QSMI_ERR_NOMEM, std::string("wave spectrum buffer: ") + hipGetErrorString(e));
+    b->capC = n;
+  }
+  if ((size_t)std::max(total, 1) > b->capNb) {
+    size_t n = std::max<size_t>(std::max(total, 1), b->capNb * 3 / 2);
+    (void)hipFree(b->dOff);
+    (void)hipFree(b->dVal);
+    (void)hipHostFree(b->hOff);
+    (void)hipHostFree(b->hVal);
+    b->dOff = nullptr;
+    b->dVal = nullptr;
+    b->hOff = nullptr;
+    b->hVal = nullptr;
+    b->capNb = 0;
+    HIPCHK(dalloc(&b->dOff, n * 3));
+    HIPCHK(dalloc(&b->dVal, n * V));
+    HIPCHK(hipHostMalloc((void**)&b->hOff, n * 3 * sizeof(int), hipHostMallocDefault));
+    HIPCHK(hipHostMalloc((void**)&b->hVal, n * V * sizeof(float), hipHostMallocDefault));
+    b->capNb = n;
+  }
+  return QSMI_OK;
+}
+
+// Host preparation into pinned staging + asynchronous upload.
+int stage_wave(qsmi_device* dev, WaveBufs* b, int nNodes, const int* nbStart, const int* offsets,
+               const float* values, const float* delta0, const float* seeds, float k) {
+  if (nNodes < 1) return fail(QSMI_ERR_ARG, "empty wave");
+  if (!nbStart || !delta0 || !seeds) return fail(QSMI_ERR_ARG, "null argument");
+  const int K = (int)std::ceil(k);
+  if (!(k > 0.f) || K > kMaxK) return fail(QSMI_ERR_UNSUPPORTED, "ceil(k) must be in [1, 16]");
+  qsmi_ti* t = dev->ti;
+  const int D = t->D, nSub = t->nSub, V = (int)dev->active.size();
+  const int total = nbStart[nNodes];
+  if (nbStart[0] != 0 || total < 0) return fail(QSMI_ERR_ARG, "nbStart must start at 0");
+  for (int i = 0; i < nNodes; ++i)
+    if (nbStart[i + 1] < nbStart[i]) return fail(QSMI_ERR_ARG, "nbStart must be non-decreasing");
+  if (total > 0 && (!offsets || !values)) return fail(QSMI_ERR_ARG, "null neighbour arrays");
+  int rc = grow_wave_bufs(dev, b, nNodes, total, K);
+  if (rc) return rc;
+  for (int i = 0; i < nNodes; ++i) {
+    const int s0 = nbStart[i], nb = nbStart[i + 1] - s0;
+    prep_node(dev, nb, offsets + (size_t)s0 * D, values + (size_t)s0 * nSub, delta0[i], seeds[i], s0, &b->hInfo[i],
+              b->hOff + (size_t)s0 * D, b->hVal + (size_t)s0 * V, nullptr, nullptr);
+  }
+  hipStream_t st = dev->stream;
+  HIPCHK(hipMemcpyAsync(b->dInfo, b->hInfo, nNodes * sizeof(NodeInfo), hipMemcpyHostToDevice, st));
+  if (total > 0) {
+    HIPCHK(hipMemcpyAsync(b->dOff, b->hOff, (size_t)total * D * sizeof(int), hipMemcpyHostToDevice, st));
+    if (V > 0) HIPCHK(hipMemcpyAsync(b->dVal, b->hVal, (size_t)total * V * sizeof(float), hipMemcpyHostToDevice, st));
+  }
+  b->nNodes = nNodes;
+  b->K = K;
+  b->k = k;
+  b->rowBlocks = row_blocks(t);
+  return QSMI_OK;
+}
+
+int run_wave(qsmi_device* dev, WaveBufs* b) {
+  qsmi_ti* t = dev->ti;
+  hipStream_t st = dev->stream;
+  ScanGeom g = geom_of(dev);
+  launch_spectrum(dev, g, b->nNodes, b->dInfo, b->dOff, b->dVal, b->dC);
+  launch_rows_select(g, t->plan[0], t->lognv[0], t->dTw[0], b->dInfo, b->dC, b->nNodes, b->rowBlocks, b->K, b->dCand,
+                     nullptr, st);
+  if (dev->timing) (void)hipEventRecord(dev->ev[3], st);
+  launch_merge_draw(g, b->dInfo, b->dCand, b->nNodes, b->rowBlocks, rows_per_block(t), b->K, b->k, b->dOutVal,
+                    b->dOutPos, b->dChosen, st);
+  if (dev->timing) (void)hipEventRecord(dev->ev[4], st);
+  HIPCHK(hipGetLastError());
+  return QSMI_OK;
+}
+
+int read_wave(qsmi_device* dev, WaveBufs* b, float* outErrors, unsigned* outPositions, unsigned* outChosen) {
+  hipStream_t st = dev->stream;
+  const size_t nk = (size_t)b->nNodes * b->K;
+  if (outErrors) HIPCHK(hipMemcpyAsync(outErrors, b->dOutVal, nk * sizeof(float), hipMemcpyDeviceToHost, st));
+  if (outPositions) HIPCHK(hipMemcpyAsync(outPositions, b->dOutPos, nk * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  if (outChosen) HIPCHK(hipMemcpyAsync(outChosen, b->dChosen, b->nNodes * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (dev->timing) {
+    for (int i = 0; i < 4; ++i) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, dev->ev[i], dev->ev[i + 1]);
+      dev->stageMs[i] = ms;
+    }
+  }
+  return QSMI_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsmi_wave_create(qsmi_device* dev, int nNodes, const int* nbStart, const int* offsets, const float* values,
+                     const float* delta0, const float* seeds, float k, qsmi_wave** out) {
+  int rc = check_dev(dev);
+  if (rc) return rc;
+  if (!out) return fail(QSMI_ERR_ARG, "null argument");
+  *out = nullptr;
+  auto* w = new qsmi_wave();
+  w->dev = dev;
+  rc = stage_wave(dev, &w->b, nNodes, nbStart, offsets, values, delta0, seeds, k);
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(dev->stream);
+    if (e != hipSuccess) rc = fail(QSMI_ERR_HIP, std::string("wave upload: ") + hipGetErrorString(e));
+  }
+  if (rc) {
+    free_wave_bufs(&w->b);
+    delete w;
+    return rc;
+  }
+  *out = w;
+  return QSMI_OK;
+}
+
+int qsmi_wave_run(qsmi_wave* w) {
+  if (!w) return fail(QSMI_ERR_ARG, "null wave");
+  int rc = check_dev(w->dev);
+  if (rc) return rc;
+  return run_wave(w->dev, &w->b);
+}
+
+int qsmi_wave_read(qsmi_wave* w, float* outErrors, unsigned* outPositions, unsigned* outChosen) {
+  if (!w) return fail(QSMI_ERR_ARG, "null wave");
+  int rc = check_dev(w->dev);
+  if (rc) return rc;
+  return read_wave(w->dev, &w->b, outErrors, outPositions, outChosen);
+}
+
+int qsmi_wave_destroy(qsmi_wave* w) {
+  if (!w) return QSMI_OK;
+  if (w->dev && w->dev->ti) {
+    (void)hipSetDevice(w->dev->ti->gpu);
+    (void)hipStreamSynchronize(w->dev->stream);
+  }
+  free_wave_bufs(&w->b);
+  delete w;
+  return QSMI_OK;
+}
+
+int qsmi_scan_wave(qsmi_device* dev, int nNodes, const int* nbStart, const int* offsets, const float* values,
+                   const float* delta0, const float* seeds, float k, float* outErrors, unsigned* outPositions,
+                   unsigned* outChosen) {
+  int rc = check_dev(dev);
+  if (rc) return rc;
+  if (!dev->wb) dev->wb = new WaveBufs();
+  rc = stage_wave(dev, dev->wb, nNodes, nbStart, offsets, values, delta0, seeds, k);
+  if (!rc) rc = run_wave(dev, dev->wb);
+  if (!rc) rc = read_wave(dev, dev->wb, outErrors, outPositions, outChosen);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,558 @@
+// QuickSampling host layer above the qsmi C-ABI: the steps of qs.cpp,
+// QuantileSamplingModule and simulation() around the scan, restated in C++
+// for the MI355X device (include/qsmi.h, "host layer").
+//
+// The path loop differs from the reference's in scheduling only: the
+// neighbour set of every path node depends on the path and the geometry, not
+// on simulated values (include/simulation.hpp:447-511 takes the first n cells
+// along the offset list whose path order is <= the node's), so the host
+// computes all neighbour sets first, gives each node a dependency level
+// (1 + the highest level among the path nodes it reads), and scans each level
+// in waves of independent nodes. Every node therefore reads exactly the
+// values the serial loop (and the reference's spin-waiting OpenMP loop,
+// simulation.hpp:463-473) would give it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/qsmi.h"
+
+namespace {
+
+}  // namespace
+
+extern "C" int qsmi_internal_fail(int code, const char* msg);
+
+namespace {
+
+int hfail(int code, const std::string& m) { return qsmi_internal_fail(code, m.c_str()); }
+
+struct KernelImg {
+  int D = 0;
+  unsigned dims[3] = {1, 1, 1};
+  int nch = 0;
+  std::vector<float> data;  // cells x nch
+};
+
+float dist2_center(int D, const unsigned* dims, const int* d) {
+  float v = 0.f;
+  for (int i = 0; i < D; ++i) v += (float)(d[i] * d[i]);
+  (void)dims;
+  return v;
+}
+
+// kernel cell of an offset relative to the kernel centre `ctr`; false when
+// the offset leaves the kernel (wrapped index still returned), as
+// DataImage::indexWithDelta (include/DataImage.hpp:343-376)
+bool kernel_cell(const KernelImg& k, const int* ctr, const int* d, size_t* cell) {
+  bool ok = true;
+  size_t idx = 0;
+  for (int i = k.D - 1; i >= 0; --i) {
+    int c = ctr[i] + d[i];
+    ok &= (c >= 0 && c < (int)k.dims[i]);
+    c = ((c % (int)k.dims[i]) + (int)k.dims[i]) % (int)k.dims[i];
+    idx = idx * k.dims[i] + (size_t)c;
+  }
+  *cell = idx;
+  return ok;
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsmi_categories(size_t cells, int nVar, const float* data, int var, float* out, int maxOut, int* count) {
+  if (!data || !out || !count || var < 0 || var >= nVar) return hfail(QSMI_ERR_ARG, "invalid argument");
+  int n = 0;
+  for (size_t c = 0; c < cells; ++c) {
+    const float v = data[c * nVar + var];
+    if (std::isnan(v)) continue;
+    bool seen = false;
+    for (int q = 0; q < n && !seen; ++q) seen = (out[q] == v);
+    if (seen) continue;
+    if (n >= maxOut) return hfail(QSMI_ERR_UNSUPPORTED, "too many categories");
+    out[n++] = v;
+  }
+  *count = n;
+  return QSMI_OK;
+}
+
+int qsmi_coef_matrix(int nVar, const int* types, const int* ncat, float* alpha, int* convTypes, int* subVarOf,
+                     int* nSubOut) {
+  if (!types || !alpha || !convTypes || !subVarOf || !nSubOut) return hfail(QSMI_ERR_ARG, "null argument");
+  int nSub = 0, ci = 0;
+  for (int v = 0; v < nVar; ++v) nSub += types[v] == 0 ? 2 : ncat[ci++];
+  if (nSub > 16) return hfail(QSMI_ERR_UNSUPPORTED, "more than 16 sub-variables");
+  std::fill(alpha, alpha + nSub * nSub, 0.f);
+  int b = 0, e = 0;
+  ci = 0;
+  for (int v = 0; v < nVar; ++v) {
+    if (types[v] == 0) {
+      // (y^2, w) with -1 and (y, w*x) with +2; the P2 term goes to delta0
+      alpha[b + b * nSub] = -1.f;
+      convTypes[b] = 0;
+      subVarOf[b++] = e;
+      alpha[b + b * nSub] = 2.f;
+      convTypes[b] = 1;
+      subVarOf[b++] = e++;
+    } else {
+      for (int q = 0; q < ncat[ci]; ++q) {
+        alpha[b + b * nSub] = 1.f;
+        convTypes[b] = 1;
+        subVarOf[b++] = e++;
+      }
+      ci++;
+    }
+  }
+  *nSubOut = nSub;
+  return QSMI_OK;
+}
+
+int qsmi_ti_subimages(int ndim, const unsigned* s, int nVar, const int* types, const float* ti, const int* ncat,
+                      const float* catvals, float* out) {
+  if (!s || !types || !ti || !out || ndim < 1 || ndim > 3) return hfail(QSMI_ERR_ARG, "invalid argument");
+  size_t S = 1;
+  for (int i = 0; i < ndim; ++i) S *= s[i];
+  size_t b = 0;
+  int ci = 0;
+  const float* cv = catvals;
+  for (int v = 0; v < nVar; ++v) {
+    if (types[v] == 0) {
+      float* y2 = out + b * S;
+      float* y1 = out + (b + 1) * S;
+      for (size_t c = 0; c < S; ++c) {
+        const float y = ti[c * nVar + v];
+        y2[c] = std::isnan(y) ? 0.f : y * y;
+        y1[c] = std::isnan(y) ? 0.f : y;
+      }
+      b += 2;
+    } else {
+      for (int q = 0; q < ncat[ci]; ++q) {
+        float* o = out + (b + q) * S;
+        for (size_t c = 0; c < S; ++c) o[c] = (ti[c * nVar + v] == cv[q]) ? 1.f : 0.f;
+      }
+      b += ncat[ci];
+      cv += ncat[ci];
+      ci++;
+    }
+  }
+  return QSMI_OK;
+}
+
+int qsmi_offsets(int ndim, const unsigned* ks, int kNVar, const float* kernel, int* out, int capacity, int* count) {
+  if (!ks || !out || !count || ndim < 1 || ndim > 3) return hfail(QSMI_ERR_ARG, "invalid argument");
+  // generation order of src/qs.cpp:1055-1084: dimension by dimension, each
+  // existing offset followed by its +j then -j copies
+  std::vector<std::vector<int>> list(1);
+  for (int i = 0; i < ndim; ++i) {
+    const size_t base = list.size();
+    const int half = ((int)ks[i] + 1) / 2;
+    for (auto& v : list) v.push_back(0);
+    list.resize(base * (size_t)(2 * half - 1));
+    for (int j = 1; j < half; ++j) {
+      for (size_t q = 0; q < base; ++q) {
+        std::vector<int> a = list[q], b = list[q];
+        a[i] = j;
+        b[i] = -j;
+        list[base * (size_t)(2 * j - 1) + q] = a;
+        list[base * (size_t)(2 * j) + q] = b;
+      }
+    }
+  }
+  if (kernel) {  // NaN kernel weights remove the offset (centre (size-1)/2)
+    std::vector<std::vector<int>> kept;
+    kept.reserve(list.size());
+    for (auto& v : list) {
+      size_t idx = 0;
+      for (int i = ndim - 1; i >= 0; --i) {
+        int c = ((int)ks[i] - 1) / 2 + v[i];
+        c = ((c % (int)ks[i]) + (int)ks[i]) % (int)ks[i];
+        idx = idx * ks[i] + (size_t)c;
+      }
+      if (!std::isnan(kernel[idx * kNVar])) kept.push_back(v);
+    }
+    list.swap(kept);
+  }
+  // the reference compares -|d|^2 with '>'; the same comparisons in the same
+  // libstdc++ introsort give the same order of equal-distance offsets
+  std::sort(list.begin(), list.end(), [&](const std::vector<int>& a, const std::vector<int>& b) {
+    return -dist2_center(ndim, ks, a.data()) > -dist2_center(ndim, ks, b.data());
+  });
+  if ((int)list.size() > capacity) return hfail(QSMI_ERR_ARG, "offset capacity too small");
+  for (size_t q = 0; q < list.size(); ++q)
+    for (int i = 0; i < ndim; ++i) out[q * ndim + i] = list[q][i];
+  *count = (int)list.size();
+  return QSMI_OK;
+}
+
+int qsmi_path(size_t cells, const unsigned char* informed, unsigned seed, unsigned* path, float* seeds,
+              unsigned* beginPath) {
+  if (!informed || !path || !seeds || !beginPath) return hfail(QSMI_ERR_ARG, "null argument");
+  std::mt19937 gen(seed);
+  for (size_t i = 0; i < cells; ++i) path[i] = (unsigned)i;
+  size_t begin = 0;
+  for (size_t i = 0; i < cells; ++i)
+    if (informed[i]) std::swap(path[begin++], path[i]);
+  std::shuffle(path + begin, path + cells, gen);
+  std::uniform_real_distribution<float> u(0.f, 1.f);
+  for (size_t i = 0; i < cells; ++i) {
+    seeds[i] = u(gen);
+    if (seeds[i] == 1.f) seeds[i] = u(gen);
+  }
+  *beginPath = (unsigned)begin;
+  return QSMI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// In-memory QS run
+// ---------------------------------------------------------------------------
+int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
+  using clk = std::chrono::steady_clock;
+  if (!rq || !rs || !rs->simulation || !rs->index) return hfail(QSMI_ERR_ARG, "null argument");
+  const int D = rq->ndim;
+  if (D < 1 || D > 3) return hfail(QSMI_ERR_UNSUPPORTED, "ndim must be 1, 2 or 3");
+  if (rq->nVar < 1 || !rq->varTypes || !rq->ti || !rq->di) return hfail(QSMI_ERR_ARG, "invalid variables");
+  if (rq->n < 1 || !(rq->k > 0.f) || std::ceil(rq->k) > 16.f) return hfail(QSMI_ERR_UNSUPPORTED, "need n >= 1, 0 < k <= 16");
+  const auto t0 = clk::now();
+  const int nVar = rq->nVar;
+  unsigned s[3] = {1, 1, 1}, sd[3] = {1, 1, 1};
+  size_t S = 1, SD = 1;
+  for (int i = 0; i < D; ++i) {
+    s[i] = rq->tiSize[i];
+    sd[i] = rq->simSize[i];
+    S *= s[i];
+    SD *= sd[i];
+  }
+  for (size_t c = 0; c < S * nVar; ++c)
+    if (std::isnan(rq->ti[c])) return hfail(QSMI_ERR_UNSUPPORTED, "TI with NaN (cross-measurement layer) unsupported");
+  // categories and sub-variables (src/qs.cpp:1500-1537)
+  std::vector<int> ncat;
+  std::vector<float> catvals;
+  std::vector<int> factor(nVar, 1);
+  for (int v = 0; v < nVar; ++v)
+    if (rq->varTypes[v] != 0) {
+      std::vector<float> cv(4096);
+      int cnt = 0;
+      int rc = qsmi_categories(S, nVar, rq->ti, v, cv.data(), 4096, &cnt);
+      if (rc) return rc;
+      ncat.push_back(cnt);
+      catvals.insert(catvals.end(), cv.begin(), cv.begin() + cnt);
+      factor[v] = cnt;
+    }
+  int nExp = 0;
+  for (int v = 0; v < nVar; ++v) nExp += factor[v];
+  float alpha[256];
+  int convTypes[16], subVarOf[16], nSub = 0;
+  {
+    int rc = qsmi_coef_matrix(nVar, rq->varTypes, ncat.data(), alpha, convTypes, subVarOf, &nSub);
+    if (rc) return rc;
+  }
+  // kernel (src/qs.cpp:1010-1042) expanded per one-hot variable
+  // (DataImage::offsetKernel4categories, include/DataImage.hpp:482-508)
+  KernelImg ker;
+  ker.D = D;
+  ker.nch = nExp;
+  bool defaultKernel = (rq->kernel == nullptr);
+  size_t KC = 1;
+  for (int i = 0; i < D; ++i) {
+    ker.dims[i] = defaultKernel ? std::min(s[i] / 2 + 1, s[i] / 2 + 1) : rq->kernelSize[i];
+    if (ker.dims[i] < 1) return hfail(QSMI_ERR_ARG, "zero kernel extent");
+    KC *= ker.dims[i];
+  }
+  ker.data.assign(KC * nExp, 1.f);
+  if (!defaultKernel) {
+    int e = 0;
+    for (int v = 0; v < nVar; ++v)
+      for (int q = 0; q < factor[v]; ++q, ++e)
+        for (size_t c = 0; c < KC; ++c) ker.data[c * nExp + e] = rq->kernel[c * nVar + v];
+  }
+  float maxw = 0.f;
+  for (float w : ker.data)
+    if (std::isfinite(w)) maxw = std::max(maxw, std::fabs(w));
+  // offsets (src/qs.cpp:1055-1137)
+  size_t cap = 1;
+  for (int i = 0; i < D; ++i) cap *= (size_t)(2 * (((int)ker.dims[i] + 1) / 2) - 1);
+  std::vector<int> offs(cap * D);
+  int nOff = 0;
+  {
+    int rc = qsmi_offsets(D, ker.dims, nExp, defaultKernel ? nullptr : ker.data.data(), offs.data(), (int)cap, &nOff);
+    if (rc) return rc;
+  }
+  // path and seeds (src/qs.cpp:1159-1258)
+  std::vector<unsigned char> informed(SD, 0);
+  for (size_t c = 0; c < SD; ++c) {
+    int nn = 0;
+    for (int v = 0; v < nVar; ++v) nn += std::isnan(rq->di[c * nVar + v]) ? 1 : 0;
+    if (nn != 0 && nn != nVar) return hfail(QSMI_ERR_UNSUPPORTED, "partially informed simulation cells");
+    informed[c] = (nn == 0);
+  }
+  std::vector<unsigned> path(SD);
+  std::vector<float> seeds(SD);
+  unsigned begin = 0;
+  qsmi_path(SD, informed.data(), rq->seed, path.data(), seeds.data(), &begin);
+  const size_t nPath = SD - begin;
+  // device: TI spectra + scan engine
+  std::vector<float> sub((size_t)nSub * S);
+  qsmi_ti_subimages(D, s, nVar, rq->varTypes, rq->ti, ncat.data(), catvals.data(), sub.data());
+  qsmi_ti* ti = nullptr;
+  qsmi_device* dev = nullptr;
+  int rc = qsmi_ti_create(rq->gpu, D, s, nullptr, nSub, sub.data(), &ti);
+  if (rc) return rc;
+  rc = qsmi_device_create(ti, alpha, 0, &dev);
+  if (rc) {
+    qsmi_ti_destroy(ti);
+    return rc;
+  }
+  bool anyCont = false;
+  for (int v = 0; v < nVar; ++v) anyCont |= rq->varTypes[v] == 0;
+  float quantum = rq->tieQuantum;
+  if (quantum < 0.f) {
+    quantum = 0.f;
+    if (!anyCont && maxw > 0.f) {
+      int e2 = (int)std::ceil(std::log2((double)rq->n * (double)maxw));
+      quantum = std::ldexp(1.f, e2 - 16);
+    }
+  }
+  qsmi_set_tie_quantum(dev, quantum);
+  qsmi_set_stage_timing(dev, 1);
+  // output state
+  float* di = rs->simulation;
+  std::memcpy(di, rq->di, SD * nVar * sizeof(float));
+  std::fill(rs->index, rs->index + SD, 0u);
+  const auto t1 = clk::now();
+
+  // ---- path loop -----------------------------------------------------------
+  // posterior order (src/qs.cpp:1615-1641) and path index of every cell
+  std::vector<unsigned> post(SD, UINT_MAX);
+  std::vector<int> pidx(SD, -1);
+  for (size_t c = 0; c < SD; ++c)
+    if (informed[c]) post[c] = 0;
+  for (size_t i = 0; i < nPath; ++i) {
+    post[path[begin + i]] = (unsigned)i;
+    pidx[path[begin + i]] = (int)i;
+  }
+  // neighbour sets of every node (simulation.hpp:447-511), centre first
+  const int n = rq->n;
+  std::vector<int> nbIdx(nPath * (size_t)n);
+  std::vector<int> nbCnt(nPath, 0);
+  {
+    unsigned hw = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+    std::atomic<size_t> next(0);
+    auto worker = [&]() {
+      const size_t chunk = 256;
+      while (true) {
+        size_t b = next.fetch_add(chunk);
+        if (b >= nPath) break;
+        size_t e = std::min(nPath, b + chunk);
+        for (size_t i = b; i < e; ++i) {
+          const unsigned cell = path[begin + i];
+          int cc[3] = {0, 0, 0};
+          size_t rem = cell;
+          for (int d = 0; d < D; ++d) {
+            cc[d] = (int)(rem % sd[d]);
+            rem /= sd[d];
+          }
+          int cnt = 0;
+          int* out = nbIdx.data() + i * (size_t)n;
+          for (int o = 0; o < nOff && cnt < n; ++o) {
+            const int* dd = offs.data() + (size_t)o * D;
+            bool inside = true;
+            size_t t = 0, mul = 1;
+            for (int d = 0; d < D; ++d) {
+              const int x = cc[d] + dd[d];
+              inside &= (x >= 0 && x < (int)sd[d]);
+              t += (size_t)x * mul;
+              mul *= sd[d];
+            }
+            if (inside && post[t] <= (unsigned)i) out[cnt++] = o;
+          }
+          nbCnt[i] = cnt;
+        }
+      }
+    };
+    std::vector<std::thread> th;
+    for (unsigned q = 0; q < hw; ++q) th.emplace_back(worker);
+    for (auto& t : th) t.join();
+  }
+  // dependency levels
+  std::vector<int> level(nPath, 0);
+  int maxLevel = 0;
+  for (size_t i = 0; i < nPath; ++i) {
+    const unsigned cell = path[begin + i];
+    int cc[3] = {0, 0, 0};
+    size_t rem = cell;
+    for (int d = 0; d < D; ++d) {
+      cc[d] = (int)(rem % sd[d]);
+      rem /= sd[d];
+    }
+    int lv = 0;
+    const int* nb = nbIdx.data() + i * (size_t)n;
+    for (int j = 0; j < nbCnt[i]; ++j) {
+      const int* dd = offs.data() + (size_t)nb[j] * D;
+      size_t t = 0, mul = 1;
+      for (int d = 0; d < D; ++d) {
+        t += (size_t)(cc[d] + dd[d]) * mul;
+        mul *= sd[d];
+      }
+      const int pj = pidx[t];
+      if (pj >= 0 && (size_t)pj != i) lv = std::max(lv, level[pj] + 1);
+    }
+    level[i] = lv;
+    maxLevel = std::max(maxLevel, lv);
+  }
+  // nodes ordered by (level, path order)
+  std::vector<size_t> lvStart(maxLevel + 2, 0);
+  for (size_t i = 0; i < nPath; ++i) lvStart[level[i] + 1]++;
+  for (int l = 0; l <= maxLevel; ++l) lvStart[l + 1] += lvStart[l];
+  std::vector<unsigned> order(nPath);
+  {
+    std::vector<size_t> fillp(lvStart.begin(), lvStart.end() - 1);
+    for (size_t i = 0; i < nPath; ++i) order[fillp[level[i]]++] = (unsigned)i;
+  }
+  const int waveSize = rq->waveSize > 0 ? rq->waveSize : 256;
+  // kernel centre for the weights (QSM uses dims/2, quantileSamplingModule.hpp:362-366)
+  int kctr[3] = {0, 0, 0};
+  for (int i = 0; i < D; ++i) kctr[i] = (int)ker.dims[i] / 2;
+  std::vector<int> wStart, wOff;
+  std::vector<float> wVal, wDelta, wSeed, wErr;
+  std::vector<unsigned> wPos, wChosen, wNode;
+  std::vector<float> xAll;
+  std::vector<size_t> kcAll;
+  std::vector<unsigned char> inAll;
+  double scanMs = 0.0;
+  int waves = 0;
+  const float kf = rq->k;
+  const int K = (int)std::ceil(kf);
+  const unsigned SI = (unsigned)S;
+  for (int l = 0; l <= maxLevel && rc == 0; ++l) {
+    for (size_t w0 = lvStart[l]; w0 < lvStart[l + 1] && rc == 0; w0 += waveSize) {
+      const size_t w1 = std::min(lvStart[l + 1], w0 + waveSize);
+      wStart.assign(1, 0);
+      wOff.clear();
+      wVal.clear();
+      wDelta.clear();
+      wSeed.clear();
+      wNode.clear();
+      for (size_t q = w0; q < w1; ++q) {
+        const unsigned i = order[q];
+        const unsigned cell = path[begin + i];
+        const float seed = seeds[begin + i];
+        const int cnt = nbCnt[i];
+        if (cnt <= 1) {
+          // only the centre: draw from the marginal (simulation.hpp:586-606)
+          const unsigned position = (unsigned)(int)std::floor(seed * (float)SI);
+          rs->index[cell] = position;
+          for (int v = 0; v < nVar; ++v) di[(size_t)cell * nVar + v] = rq->ti[(size_t)position * nVar + v];
+          continue;
+        }
+        int cc[3] = {0, 0, 0};
+        size_t rem = cell;
+        for (int d = 0; d < D; ++d) {
+          cc[d] = (int)(rem % sd[d]);
+          rem /= sd[d];
+        }
+        float delta0 = 0.f;
+        const int* nb = nbIdx.data() + (size_t)i * n;
+        xAll.resize((size_t)cnt * nExp);
+        kcAll.resize(cnt);
+        inAll.resize(cnt);
+        for (int j = 0; j < cnt; ++j) {
+          const int* dd = offs.data() + (size_t)nb[j] * D;
+          size_t t = 0, mul = 1;
+          for (int d = 0; d < D; ++d) {
+            t += (size_t)(cc[d] + dd[d]) * mul;
+            mul *= sd[d];
+            wOff.push_back(dd[d]);
+          }
+          // one-hot expansion (simulation.hpp:512-537)
+          float* xe = xAll.data() + (size_t)j * nExp;
+          int e = 0, ci = 0;
+          const float* cv = catvals.data();
+          for (int v = 0; v < nVar; ++v) {
+            const float x = di[t * nVar + v];
+            if (rq->varTypes[v] == 0) {
+              xe[e++] = x;
+            } else {
+              for (int q2 = 0; q2 < ncat[ci]; ++q2) xe[e++] = (x == cv[q2]) ? 1.f : 0.f;
+              cv += ncat[ci];
+              ci++;
+            }
+          }
+          // QSM conversion (quantileSamplingModule.hpp:368-410)
+          size_t kc;
+          inAll[j] = kernel_cell(ker, kctr, dd, &kc) ? 1 : 0;
+          kcAll[j] = kc;
+          for (int B = 0; B < nSub; ++B) {
+            const int ev = subVarOf[B];
+            const float x = xe[ev];
+            const float w = ker.data[kc * nExp + (ev % nExp)];
+            float r = 0.f;
+            if (inAll[j] && !std::isnan(x)) r = convTypes[B] == 0 ? w * 1.f : w * x;
+            wVal.push_back(r);
+          }
+        }
+        // delta0 = -sum w x^2 per original continuous variable v, read at
+        // expanded index v as the reference does (:412-457)
+        for (int v = 0; v < nVar; ++v) {
+          if (rq->varTypes[v] != 0) continue;
+          for (int j = 0; j < cnt; ++j) {
+            const float xv = xAll[(size_t)j * nExp + v];
+            if (inAll[j] && !std::isnan(xv)) delta0 += -1.f * (ker.data[kcAll[j] * nExp + (v % nExp)] * xv * xv);
+          }
+        }
+        wStart.push_back(wStart.back() + cnt);
+        wDelta.push_back(delta0);
+        wSeed.push_back(seed);
+        wNode.push_back(i);
+      }
+      const int nN = (int)wNode.size();
+      if (nN == 0) continue;
+      wErr.resize((size_t)nN * K);
+      wPos.resize((size_t)nN * K);
+      wChosen.resize(nN);
+      rc = qsmi_scan_wave(dev, nN, wStart.data(), wOff.data(), wVal.data(), wDelta.data(), wSeed.data(), kf,
+                          wErr.data(), wPos.data(), wChosen.data());
+      if (rc) break;
+      float ms[4];
+      qsmi_last_stage_ms(dev, ms);
+      scanMs += ms[0] + ms[1] + ms[2] + ms[3];
+      waves++;
+      for (int q = 0; q < nN; ++q) {
+        const unsigned i = wNode[q];
+        const unsigned cell = path[begin + i];
+        const unsigned position = wChosen[q];
+        if (position == UINT_MAX || position >= SI) {
+          rc = hfail(QSMI_ERR_UNSUPPORTED, "a node has no valid TI position (neighbourhood wider than the TI)");
+          break;
+        }
+        rs->index[cell] = position;
+        for (int v = 0; v < nVar; ++v) di[(size_t)cell * nVar + v] = rq->ti[(size_t)position * nVar + v];
+      }
+    }
+  }
+  const auto t2 = clk::now();
+  qsmi_device_destroy(dev);
+  qsmi_ti_destroy(ti);
+  if (rc) {
+    return rc;
+  }
+  rs->setupMs = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  rs->pathLoopMs = std::chrono::duration<double, std::milli>(t2 - t1).count();
+  rs->scanMs = scanMs;
+  rs->nodes = (long long)nPath;
+  rs->waves = waves;
+  return QSMI_OK;
+}
+
+}  // extern "C"

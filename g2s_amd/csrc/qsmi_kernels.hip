@@ -1,0 +1,608 @@
+// HIP kernels of the MI355X QuickSampling exact scan (gfx950).
+//
+// Per path wave (a batch of independent path nodes) the scan is
+//   [D>=2] k_pencil      : for each block of `nvec` adjacent spectrum columns
+//                          and each node: sparse neighbour DFT along the
+//                          leading dims (direct, <= n terms), forward FFT along
+//                          the last dim, complex MAC with the TI spectra of
+//                          every active sub-variable, inverse FFT along the
+//                          last dim; writes only the rows whose centre
+//                          coordinate is valid.
+//   [D==3] k_colfft      : inverse along dim1 of the valid planes.
+//   [D==1] k_direct1d    : the spectrum of a 1-D node directly.
+//   k_rows_select        : two rows per complex FFT (two-for-one c2r along
+//                          dim0), scale + delta0, wrap-around mask as a
+//                          predicate, block top-K of the device rule
+//                          (optionally the full position-space map).
+//   k_merge_draw         : per node top-K of the block candidates, the rank
+//                          draw floor(seed*k), outputs.
+// The mismatch map of a node is never written to HBM in the wave path.
+//
+// Replaces the reference's per-node cuFFT R2C / complexAddAlphaxCxD /
+// cuFFT C2R / removeBorder / fma / cublasIsamin sequence
+// (src/NvidiaGPUAcceleratorDevice.cu:466-614, 688-717) and the CPU scan
+// (src/CPUThreadDevice.cpp:349-490, src/CPUThreadAcceleratorDevice.cpp:251-357).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "qsmi_kernels.hpp"
+
+namespace qsmi {
+
+// ---------------------------------------------------------------------------
+// top-K of the device rule
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ Cand worst_cand() {
+  Cand c;
+  c.key = -INFINITY;
+  c.h = 0u;
+  c.pos = 0xFFFFFFFFu;
+  c.val = -INFINITY;
+  return c;
+}
+
+template <int KT>
+struct TopK {
+  Cand c[KT];
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int i = 0; i < KT; ++i) c[i] = worst_cand();
+  }
+  __device__ __forceinline__ void push(Cand cur) {
+    if (!cand_better(cur, c[KT - 1])) return;
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+      if (cand_better(cur, c[i])) {
+        Cand t = c[i];
+        c[i] = cur;
+        cur = t;
+      }
+    }
+  }
+  // value fast path: the hash is only computed when the key can enter
+  __device__ __forceinline__ void push_value(float val, float quantum, uint32_t pos, uint32_t seedbits) {
+    float key = (quantum > 0.f) ? rintf(val * (1.f / quantum)) : val;
+    if (!(key >= c[KT - 1].key)) return;  // also rejects NaN
+    Cand cd;
+    cd.key = key;
+    cd.val = val;
+    cd.pos = pos;
+    cd.h = tie_hash(seedbits, pos);
+    push(cd);
+  }
+  __device__ __forceinline__ void pop() {
+#pragma unroll
+    for (int i = 0; i < KT - 1; ++i) c[i] = c[i + 1];
+    c[KT - 1] = worst_cand();
+  }
+};
+
+__device__ __forceinline__ Cand shfl_xor_cand(const Cand& a, int off) {
+  Cand o;
+  o.key = __shfl_xor(a.key, off);
+  o.h = __shfl_xor(a.h, off);
+  o.pos = __shfl_xor(a.pos, off);
+  o.val = __shfl_xor(a.val, off);
+  return o;
+}
+
+// Block-wide K best of the per-thread lists, best first, into out[0..K).
+// blockDim.x must be a multiple of 64. Ends with a barrier.
+template <int KT>
+__device__ void block_topk(TopK<KT>& t, int K, Cand* out, Cand* red) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+  for (int it = 0; it < K; ++it) {
+    Cand best = t.c[0];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      Cand o = shfl_xor_cand(best, off);
+      if (cand_better(o, best)) best = o;
+    }
+    if (lane == 0) red[wave] = best;
+    __syncthreads();
+    Cand w = red[0];
+    for (int i = 1; i < nwaves; ++i)
+      if (cand_better(red[i], w)) w = red[i];
+    if (tid == 0) out[it] = w;
+    if (t.c[0].pos == w.pos && t.c[0].pos != 0xFFFFFFFFu) t.pop();
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// TI spectra, once per run: rows r2c (two real rows per complex FFT)
+// in: real images [nrows][f0]; out: half rows [nrows][h0]
+// ---------------------------------------------------------------------------
+__global__ void k_r2c_rows(const float* __restrict__ in, float2* __restrict__ out, FftPlan plan,
+                           const float2* __restrict__ twg, int lognv, long long nrows, int h0) {
+  extern __shared__ float2 smem[];
+  const int L = plan.L, nvec = 1 << lognv, tid = threadIdx.x, nthr = blockDim.x;
+  float2* tw = smem;
+  float2* a = tw + L;
+  float2* b = a + L * nvec;
+  for (int i = tid; i < L; i += nthr) tw[i] = twg[i];
+  const long long row0 = (long long)blockIdx.x * 2 * nvec;
+  for (int e = tid; e < L * nvec; e += nthr) {
+    int i = e >> lognv, v = e & (nvec - 1);
+    long long ra = row0 + 2 * v, rb = ra + 1;
+    float xa = (ra < nrows) ? in[ra * L + i] : 0.f;
+    float xb = (rb < nrows) ? in[rb * L + i] : 0.f;
+    a[e] = make_float2(xa, xb);
+  }
+  __syncthreads();
+  float2* r = stockham<-1>(a, b, tw, plan, lognv, tid, nthr);
+  for (int e = tid; e < h0 * nvec; e += nthr) {
+    int k = e >> lognv, v = e & (nvec - 1);
+    long long ra = row0 + 2 * v, rb = ra + 1;
+    float2 z = r[e];
+    float2 zc = r[(((L - k) % L) << lognv) + v];
+    zc.y = -zc.y;  // conj(Z[-k])
+    // X_a = (Z + conj Z-)/2 ; X_b = -i (Z - conj Z-)/2
+    float2 xa = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
+    float2 d = make_float2(0.5f * (z.x - zc.x), 0.5f * (z.y - zc.y));
+    float2 xb = make_float2(d.y, -d.x);
+    if (ra < nrows) out[ra * h0 + k] = xa;
+    if (rb < nrows) out[rb * h0 + k] = xb;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Column FFT in place along a strided dimension.
+// Plane p (blockIdx.x) starts at base + (p / pdiv) * pOuter + (p % pdiv) * pInner;
+// element (row r, column c) of the plane at r * rowStride + c, c < ncols.
+// When info != nullptr the plane p belongs to node p / planesPerNode with
+// coordinate c = p % planesPerNode along dimension validDim; planes outside
+// [lo, hi] of that node are skipped.
+// ---------------------------------------------------------------------------
+template <int DIR>
+__global__ void k_colfft(float2* __restrict__ base, FftPlan plan, const float2* __restrict__ twg, int lognv,
+                         long long rowStride, int ncols, int pdiv, long long pOuter, long long pInner,
+                         const NodeInfo* __restrict__ info, int planesPerNode, int validDim) {
+  extern __shared__ float2 smem[];
+  const int L = plan.L, nvec = 1 << lognv, tid = threadIdx.x, nthr = blockDim.x;
+  const int p = blockIdx.x;
+  if (info) {
+    const NodeInfo& ni = info[p / planesPerNode];
+    int c = p % planesPerNode;
+    if (c < ni.lo[validDim] || c > ni.hi[validDim]) return;
+  }
+  float2* tw = smem;
+  float2* a = tw + L;
+  float2* b = a + L * nvec;
+  for (int i = tid; i < L; i += nthr) tw[i] = twg[i];
+  float2* pl = base + (long long)(p / pdiv) * pOuter + (long long)(p % pdiv) * pInner;
+  const int c0 = blockIdx.y * nvec;
+  for (int e = tid; e < L * nvec; e += nthr) {
+    int r = e >> lognv, v = e & (nvec - 1);
+    a[e] = (c0 + v < ncols) ? pl[r * rowStride + c0 + v] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  float2* res = stockham<DIR>(a, b, tw, plan, lognv, tid, nthr);
+  for (int e = tid; e < L * nvec; e += nthr) {
+    int r = e >> lognv, v = e & (nvec - 1);
+    if (c0 + v < ncols) pl[r * rowStride + c0 + v] = res[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused pencil: sparse neighbour DFT + forward FFT + MAC + inverse FFT along
+// the last dimension P. grid.x = column blocks (nblk0 per dim1 row for D=3),
+// grid.y = node groups of `npb` nodes.
+// Yhat: [A][kP][col] with col = k0 + h0*k1 (k1 only for D=3).
+// C (per node, stride H): [cP][col], rows cP in [lo_P, hi_P] written.
+// ---------------------------------------------------------------------------
+constexpr int kPencilThreads = 256;
+constexpr int kMaxEPT = kMaxVecElems / kPencilThreads;  // 16
+
+__global__ void __launch_bounds__(kPencilThreads) k_pencil(ScanGeom g, FftPlan planP, int lognv, int nblk0,
+                                                           const float2* __restrict__ Yhat,
+                                                           const float2* __restrict__ tw0g,
+                                                           const float2* __restrict__ tw1g,
+                                                           const float2* __restrict__ twPg, int nNodes, int npb,
+                                                           const NodeInfo* __restrict__ info,
+                                                           const int* __restrict__ nbOff,
+                                                           const float* __restrict__ nbVal,
+                                                           float2* __restrict__ C) {
+  extern __shared__ float2 smem[];
+  const int LP = planP.L, nvec = 1 << lognv, tid = threadIdx.x, nthr = blockDim.x;
+  const int D = g.D, P = g.P;
+  float2* twP = smem;
+  float2* bufA = twP + LP;
+  float2* bufB = bufA + LP * nvec;
+  for (int i = tid; i < LP; i += nthr) twP[i] = twPg[i];
+
+  const int blk0 = blockIdx.x % nblk0;
+  const int k1 = (D == 3) ? (int)(blockIdx.x / nblk0) : 0;
+  const int k0base = blk0 * nvec;
+  const int ncolsP = g.h0 * ((D == 3) ? g.f[1] : 1);
+  const int colbase = k0base + g.h0 * k1;
+  const int elems = LP * nvec;
+  const int fP = g.f[P];
+  const int f0 = g.f[0], f1 = g.f[1];
+
+  const int nodeBeg = blockIdx.y * npb;
+  const int nodeEnd = min(nNodes, nodeBeg + npb);
+  for (int node = nodeBeg; node < nodeEnd; ++node) {
+    const NodeInfo ni = info[node];
+    if (ni.lo[P] > ni.hi[P]) continue;  // no valid centre
+    float2 M[kMaxEPT];
+#pragma unroll
+    for (int i = 0; i < kMaxEPT; ++i) M[i] = make_float2(0.f, 0.f);
+
+    for (int a = 0; a < g.V; ++a) {
+      for (int e = tid; e < elems; e += nthr) bufA[e] = make_float2(0.f, 0.f);
+      __syncthreads();
+      // sparse scatter of the neighbour kernel column(s)
+      for (int t = tid; t < ni.nb * nvec; t += nthr) {
+        const int j = t >> lognv, v = t & (nvec - 1);
+        const int k0 = k0base + v;
+        if (k0 >= g.h0) continue;
+        const int jj = ni.nbStart + j;
+        const float val = nbVal[(long long)jj * g.V + a];
+        if (val == 0.f) continue;
+        const int d0 = nbOff[jj * D + 0];
+        int t0 = (int)(((long long)k0 * d0) % f0);
+        if (t0 < 0) t0 += f0;
+        float2 ph = tw0g[t0];
+        ph.y = -ph.y;  // exp(+2 pi i k0 d0 / f0)
+        int dP;
+        if (D == 3) {
+          const int d1 = nbOff[jj * D + 1];
+          int t1 = (int)(((long long)k1 * d1) % f1);
+          if (t1 < 0) t1 += f1;
+          float2 p1 = tw1g[t1];
+          p1.y = -p1.y;
+          ph = cmul(ph, p1);
+          dP = nbOff[jj * D + 2];
+        } else {
+          dP = nbOff[jj * D + 1];
+        }
+        int q = (-dP) % fP;
+        if (q < 0) q += fP;
+        float* dst = reinterpret_cast<float*>(&bufA[(q << lognv) + v]);
+        atomicAdd(dst, val * ph.x);
+        atomicAdd(dst + 1, val * ph.y);
+      }
+      __syncthreads();
+      float2* X = stockham<-1>(bufA, bufB, twP, planP, lognv, tid, nthr);
+      const float2* Ya = Yhat + (long long)g.A[a] * g.H;
+#pragma unroll
+      for (int i = 0; i < kMaxEPT; ++i) {
+        const int e = tid + i * nthr;
+        if (e < elems) {
+          const int kP = e >> lognv, v = e & (nvec - 1);
+          if (k0base + v < g.h0) {
+            const float2 y = Ya[(long long)kP * ncolsP + colbase + v];
+            const float2 x = X[e];
+            M[i].x += y.x * x.x - y.y * x.y;
+            M[i].y += y.x * x.y + y.y * x.x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < kMaxEPT; ++i) {
+      const int e = tid + i * nthr;
+      if (e < elems) bufA[e] = M[i];
+    }
+    __syncthreads();
+    float2* R = stockham<+1>(bufA, bufB, twP, planP, lognv, tid, nthr);
+    float2* Cn = C + (long long)node * g.H;
+    const int lo = ni.lo[P], hi = ni.hi[P];
+    for (int e = tid; e < elems; e += nthr) {
+      const int cP = e >> lognv, v = e & (nvec - 1);
+      if (cP >= lo && cP <= hi && k0base + v < g.h0) Cn[(long long)cP * ncolsP + colbase + v] = R[e];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 1-D nodes: the spectrum directly, M(k0) = sum_A Yhat_A(k0) sum_j v_jA e^{+2pi i k0 d_j/f0}
+// ---------------------------------------------------------------------------
+__global__ void k_direct1d(ScanGeom g, const float2* __restrict__ Yhat, const float2* __restrict__ tw0g,
+                           const NodeInfo* __restrict__ info, const int* __restrict__ nbOff,
+                           const float* __restrict__ nbVal, float2* __restrict__ C) {
+  const int node = blockIdx.y;
+  const int k0 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k0 >= g.h0) return;
+  const NodeInfo ni = info[node];
+  const int f0 = g.f[0];
+  float2 acc = make_float2(0.f, 0.f);
+  for (int a = 0; a < g.V; ++a) {
+    float2 x = make_float2(0.f, 0.f);
+    for (int j = 0; j < ni.nb; ++j) {
+      const int jj = ni.nbStart + j;
+      const float val = nbVal[(long long)jj * g.V + a];
+      if (val == 0.f) continue;
+      int t0 = (int)(((long long)k0 * nbOff[jj]) % f0);
+      if (t0 < 0) t0 += f0;
+      float2 ph = tw0g[t0];
+      x.x += val * ph.x;
+      x.y -= val * ph.y;
+    }
+    const float2 y = Yhat[(long long)g.A[a] * g.H + k0];
+    acc.x += y.x * x.x - y.y * x.y;
+    acc.y += y.x * x.y + y.y * x.x;
+  }
+  C[(long long)node * g.H + k0] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Rows: two-for-one c2r along dim0, epilogue, mask, block top-K.
+// grid.x = row-pair blocks (2*nvec valid rows each), grid.y = node.
+// Valid rows q enumerate (c1, c2) in the node's valid box, c1 fastest;
+// lo/hi of unused dimensions are 0.
+// cand (optional): [node][blockIdx.x][K]; map (optional): [node][S] position
+// space (only valid positions written).
+// ---------------------------------------------------------------------------
+template <int KT>
+__global__ void __launch_bounds__(256) k_rows_select(ScanGeom g, FftPlan plan0, int lognv,
+                                                     const float2* __restrict__ tw0g,
+                                                     const NodeInfo* __restrict__ info,
+                                                     const float2* __restrict__ C, int K, Cand* __restrict__ cand,
+                                                     int candBlocks, float* __restrict__ map) {
+  extern __shared__ float2 smem[];
+  __shared__ Cand red[16];
+  const int L = plan0.L, nvec = 1 << lognv, tid = threadIdx.x, nthr = blockDim.x;
+  const int node = blockIdx.y;
+  const NodeInfo ni = info[node];
+  const int n1v = ni.hi[1] - ni.lo[1] + 1;
+  const int n2v = ni.hi[2] - ni.lo[2] + 1;
+  const int n0v = ni.hi[0] - ni.lo[0] + 1;
+  const long long nrv = (n1v > 0 && n2v > 0 && n0v > 0) ? (long long)n1v * n2v : 0;
+  const long long q0 = (long long)blockIdx.x * 2 * nvec;
+  if (q0 >= nrv) return;
+  float2* tw = smem;
+  float2* a = tw + L;
+  float2* b = a + L * nvec;
+  for (int i = tid; i < L; i += nthr) tw[i] = tw0g[i];
+  const int h0 = g.h0, f1 = g.f[1];
+  const float2* Cn = C + (long long)node * g.H;
+  // load Z = Aext + i Bext
+  for (int e = tid; e < L * nvec; e += nthr) {
+    const int k = e >> lognv, v = e & (nvec - 1);
+    const long long qa = q0 + 2 * v, qb = qa + 1;
+    const bool conjk = (k >= h0);
+    const int kk = conjk ? (L - k) : k;
+    float2 A = make_float2(0.f, 0.f), B = make_float2(0.f, 0.f);
+    if (qa < nrv) {
+      const int c1 = ni.lo[1] + (int)(qa % n1v);
+      const int c2 = ni.lo[2] + (int)(qa / n1v);
+      A = Cn[((long long)c2 * f1 + c1) * h0 + kk];
+    }
+    if (qb < nrv) {
+      const int c1 = ni.lo[1] + (int)(qb % n1v);
+      const int c2 = ni.lo[2] + (int)(qb / n1v);
+      B = Cn[((long long)c2 * f1 + c1) * h0 + kk];
+    }
+    // a Hermitian c2r ignores the imaginary part of the DC and Nyquist bins
+    if (kk == 0 || 2 * kk == L) {
+      A.y = 0.f;
+      B.y = 0.f;
+    }
+    if (conjk) {
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    a[e] = make_float2(A.x - B.y, A.y + B.x);
+  }
+  __syncthreads();
+  float2* z = stockham<+1>(a, b, tw, plan0, lognv, tid, nthr);
+  TopK<KT> tk;
+  tk.init();
+  const float scale = ni.scale, d0 = ni.delta0;
+  const int s0 = g.s[0], s1 = g.s[1];
+  for (int e = tid; e < L * nvec; e += nthr) {
+    const int c0 = e >> lognv, v = e & (nvec - 1);
+    if (c0 < ni.lo[0] || c0 > ni.hi[0]) continue;
+    const float2 zz = z[e];
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+      const long long q = q0 + 2 * v + w;
+      if (q >= nrv) continue;
+      const int c1 = ni.lo[1] + (int)(q % n1v);
+      const int c2 = ni.lo[2] + (int)(q / n1v);
+      const float val = (w == 0 ? zz.x : zz.y) * scale + d0;
+      const uint32_t pos = (uint32_t)(c0 + s0 * (c1 + s1 * c2));
+      if (map) map[(long long)node * ((long long)s0 * s1 * g.s[2]) + pos] = val;
+      if (cand) tk.push_value(val, ni.quantum, pos, ni.seedbits);
+    }
+  }
+  if (cand) {
+    block_topk<KT>(tk, K, cand + ((long long)node * candBlocks + blockIdx.x) * K, red);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Top-K over a position-space map (per-node API after map edits).
+// grid.x = chunks, one node; cand [blockIdx.x][K].
+// ---------------------------------------------------------------------------
+template <int KT>
+__global__ void __launch_bounds__(256) k_select_map(const float* __restrict__ map, long long S, int K,
+                                                    float quantum, uint32_t seedbits, Cand* __restrict__ cand) {
+  __shared__ Cand red[16];
+  const long long chunk = (S + gridDim.x - 1) / gridDim.x;
+  const long long beg = (long long)blockIdx.x * chunk;
+  const long long end = min(S, beg + chunk);
+  TopK<KT> tk;
+  tk.init();
+  for (long long i = beg + threadIdx.x; i < end; i += blockDim.x) {
+    const float v = map[i];
+    if (isfinite(v)) tk.push_value(v, quantum, (uint32_t)i, seedbits);
+  }
+  block_topk<KT>(tk, K, cand + (long long)blockIdx.x * K, red);
+}
+
+// ---------------------------------------------------------------------------
+// Per node: K best of the block candidates, rank draw.
+// nblkOf: candidate blocks per node are ceil(validRows / rowsPerBlock)
+// (rowsPerBlock <= 0: use candBlocks for every node).
+// ---------------------------------------------------------------------------
+template <int KT>
+__global__ void __launch_bounds__(256) k_merge_draw(ScanGeom g, const NodeInfo* __restrict__ info,
+                                                    const Cand* __restrict__ cand, int candBlocks,
+                                                    int rowsPerBlock, int K, float kf, float* __restrict__ outVal,
+                                                    uint32_t* __restrict__ outPos, uint32_t* __restrict__ chosen) {
+  __shared__ Cand red[16];
+  __shared__ Cand best[kMaxK];
+  const int node = blockIdx.x;
+  const NodeInfo ni = info[node];
+  int nblk = candBlocks;
+  if (rowsPerBlock > 0) {
+    const int n1v = ni.hi[1] - ni.lo[1] + 1;
+    const int n2v = ni.hi[2] - ni.lo[2] + 1;
+    const int n0v = ni.hi[0] - ni.lo[0] + 1;
+    const long long nrv = (n1v > 0 && n2v > 0 && n0v > 0) ? (long long)n1v * n2v : 0;
+    nblk = (int)((nrv + rowsPerBlock - 1) / rowsPerBlock);
+  }
+  TopK<KT> tk;
+  tk.init();
+  const Cand* cn = cand + (long long)node * candBlocks * K;
+  for (int i = threadIdx.x; i < nblk * K; i += blockDim.x) tk.push(cn[i]);
+  block_topk<KT>(tk, K, best, red);
+  if (threadIdx.x < K) {
+    outVal[(long long)node * K + threadIdx.x] = best[threadIdx.x].val;
+    outPos[(long long)node * K + threadIdx.x] = best[threadIdx.x].pos;
+  }
+  if (threadIdx.x == 0 && chosen) {
+    // QuantileSamplingModule::sample rank, one TI: floor(seed * k * (ceil(1/k)/1))
+    int r = (int)floorf(ni.seed * kf * (ceilf(1.f / kf) / 1.f));
+    if (r > K - 1) r = K - 1;
+    if (r < 0) r = 0;
+    chosen[node] = best[r].pos;
+  }
+}
+
+__global__ void k_fill(float* __restrict__ p, long long n, float v) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// explicit instantiations used by the host
+template __global__ void k_colfft<-1>(float2*, FftPlan, const float2*, int, long long, int, int, long long, long long,
+                                      const NodeInfo*, int, int);
+template __global__ void k_colfft<+1>(float2*, FftPlan, const float2*, int, long long, int, int, long long, long long,
+                                      const NodeInfo*, int, int);
+#define QSMI_INST_K(KT)                                                                                            \
+  template __global__ void k_rows_select<KT>(ScanGeom, FftPlan, int, const float2*, const NodeInfo*,              \
+                                             const float2*, int, Cand*, int, float*);                             \
+  template __global__ void k_select_map<KT>(const float*, long long, int, float, uint32_t, Cand*);                \
+  template __global__ void k_merge_draw<KT>(ScanGeom, const NodeInfo*, const Cand*, int, int, int, float, float*, \
+                                            uint32_t*, uint32_t*);
+QSMI_INST_K(2)
+QSMI_INST_K(4)
+QSMI_INST_K(8)
+QSMI_INST_K(16)
+
+}  // namespace qsmi
+
+// ---------------------------------------------------------------------------
+// Host-side launchers (same translation unit as the kernels).
+// ---------------------------------------------------------------------------
+namespace qsmi {
+
+static size_t fft_lds_bytes(int L, int nvec) { return (size_t)(L + 2 * L * nvec) * sizeof(float2); }
+
+hipError_t set_lds_limits() {
+  const int big = 160 * 1024;
+  hipError_t e = hipSuccess;
+  e = hipFuncSetAttribute((const void*)k_pencil, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)k_r2c_rows, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)k_colfft<-1>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)k_colfft<+1>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)k_rows_select<2>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)k_rows_select<4>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)k_rows_select<8>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute((const void*)k_rows_select<16>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+  return e;
+}
+
+void launch_r2c_rows(const float* in, float2* out, const FftPlan& plan, const float2* tw, int lognv, long long nrows,
+                     int h0, hipStream_t st) {
+  const int nvec = 1 << lognv;
+  long long blocks = (nrows + 2 * nvec - 1) / (2 * nvec);
+  hipLaunchKernelGGL(k_r2c_rows, dim3((unsigned)blocks), dim3(256), fft_lds_bytes(plan.L, nvec), st, in, out, plan, tw,
+                     lognv, nrows, h0);
+}
+
+void launch_colfft(int dir, float2* base, const FftPlan& plan, const float2* tw, int lognv, long long rowStride,
+                   int ncols, long long planes, int pdiv, long long pOuter, long long pInner, const NodeInfo* info,
+                   int planesPerNode, int validDim, hipStream_t st) {
+  const int nvec = 1 << lognv;
+  dim3 grid((unsigned)planes, (unsigned)((ncols + nvec - 1) / nvec));
+  if (dir < 0)
+    hipLaunchKernelGGL(k_colfft<-1>, grid, dim3(256), fft_lds_bytes(plan.L, nvec), st, base, plan, tw, lognv,
+                       rowStride, ncols, pdiv, pOuter, pInner, info, planesPerNode, validDim);
+  else
+    hipLaunchKernelGGL(k_colfft<+1>, grid, dim3(256), fft_lds_bytes(plan.L, nvec), st, base, plan, tw, lognv,
+                       rowStride, ncols, pdiv, pOuter, pInner, info, planesPerNode, validDim);
+}
+
+void launch_pencil(const ScanGeom& g, const FftPlan& planP, int lognv, const float2* Yhat, const float2* tw0,
+                   const float2* tw1, const float2* twP, int nNodes, int npb, const NodeInfo* info, const int* nbOff,
+                   const float* nbVal, float2* C, hipStream_t st) {
+  const int nvec = 1 << lognv;
+  const int nblk0 = (g.h0 + nvec - 1) / nvec;
+  dim3 grid((unsigned)(nblk0 * (g.D == 3 ? g.f[1] : 1)), (unsigned)((nNodes + npb - 1) / npb));
+  hipLaunchKernelGGL(k_pencil, grid, dim3(kPencilThreads), fft_lds_bytes(planP.L, nvec), st, g, planP, lognv, nblk0,
+                     Yhat, tw0, tw1, twP, nNodes, npb, info, nbOff, nbVal, C);
+}
+
+void launch_direct1d(const ScanGeom& g, const float2* Yhat, const float2* tw0, int nNodes, const NodeInfo* info,
+                     const int* nbOff, const float* nbVal, float2* C, hipStream_t st) {
+  dim3 grid((unsigned)((g.h0 + 255) / 256), (unsigned)nNodes);
+  hipLaunchKernelGGL(k_direct1d, grid, dim3(256), 0, st, g, Yhat, tw0, info, nbOff, nbVal, C);
+}
+
+static int kt_for(int K) { return K <= 2 ? 2 : K <= 4 ? 4 : K <= 8 ? 8 : 16; }
+
+void launch_rows_select(const ScanGeom& g, const FftPlan& plan0, int lognv, const float2* tw0, const NodeInfo* info,
+                        const float2* C, int nNodes, int rowBlocks, int K, Cand* cand, float* map, hipStream_t st) {
+  const int nvec = 1 << lognv;
+  dim3 grid((unsigned)rowBlocks, (unsigned)nNodes);
+  size_t lds = fft_lds_bytes(plan0.L, nvec);
+  switch (kt_for(K)) {
+    case 2: hipLaunchKernelGGL(k_rows_select<2>, grid, dim3(256), lds, st, g, plan0, lognv, tw0, info, C, K, cand, rowBlocks, map); break;
+    case 4: hipLaunchKernelGGL(k_rows_select<4>, grid, dim3(256), lds, st, g, plan0, lognv, tw0, info, C, K, cand, rowBlocks, map); break;
+    case 8: hipLaunchKernelGGL(k_rows_select<8>, grid, dim3(256), lds, st, g, plan0, lognv, tw0, info, C, K, cand, rowBlocks, map); break;
+    default: hipLaunchKernelGGL(k_rows_select<16>, grid, dim3(256), lds, st, g, plan0, lognv, tw0, info, C, K, cand, rowBlocks, map); break;
+  }
+}
+
+void launch_select_map(const float* map, long long S, int K, float quantum, uint32_t seedbits, Cand* cand, int chunks,
+                       hipStream_t st) {
+  switch (kt_for(K)) {
+    case 2: hipLaunchKernelGGL(k_select_map<2>, dim3(chunks), dim3(256), 0, st, map, S, K, quantum, seedbits, cand); break;
+    case 4: hipLaunchKernelGGL(k_select_map<4>, dim3(chunks), dim3(256), 0, st, map, S, K, quantum, seedbits, cand); break;
+    case 8: hipLaunchKernelGGL(k_select_map<8>, dim3(chunks), dim3(256), 0, st, map, S, K, quantum, seedbits, cand); break;
+    default: hipLaunchKernelGGL(k_select_map<16>, dim3(chunks), dim3(256), 0, st, map, S, K, quantum, seedbits, cand); break;
+  }
+}
+
+void launch_merge_draw(const ScanGeom& g, const NodeInfo* info, const Cand* cand, int nNodes, int candBlocks,
+                       int rowsPerBlock, int K, float kf, float* outVal, uint32_t* outPos, uint32_t* chosen,
+                       hipStream_t st) {
+  switch (kt_for(K)) {
+    case 2: hipLaunchKernelGGL(k_merge_draw<2>, dim3(nNodes), dim3(256), 0, st, g, info, cand, candBlocks, rowsPerBlock, K, kf, outVal, outPos, chosen); break;
+    case 4: hipLaunchKernelGGL(k_merge_draw<4>, dim3(nNodes), dim3(256), 0, st, g, info, cand, candBlocks, rowsPerBlock, K, kf, outVal, outPos, chosen); break;
+    case 8: hipLaunchKernelGGL(k_merge_draw<8>, dim3(nNodes), dim3(256), 0, st, g, info, cand, candBlocks, rowsPerBlock, K, kf, outVal, outPos, chosen); break;
+    default: hipLaunchKernelGGL(k_merge_draw<16>, dim3(nNodes), dim3(256), 0, st, g, info, cand, candBlocks, rowsPerBlock, K, kf, outVal, outPos, chosen); break;
+  }
+}
+
+void launch_fill(float* p, long long n, float v, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+
+}  // namespace qsmi

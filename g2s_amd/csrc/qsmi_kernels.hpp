@@ -1,0 +1,69 @@
+// Shared device-side types of the MI355X QuickSampling scan.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "qsmi_fft.hpp"
+
+namespace qsmi {
+
+constexpr int kMaxSub = 16;     // TI sub-variable spectra per device
+constexpr int kMaxK = 16;       // ceil(k) supported by the select
+constexpr int kMaxVecElems = 4096;  // L * nvec of one LDS column block
+
+// Geometry of one scan (TI grid s, FFT grid f, half spectrum along dim0).
+struct ScanGeom {
+  int D;
+  int s[3];
+  int f[3];
+  int h0;       // f0/2 + 1
+  int P;        // pencil dimension = D-1
+  int V;        // active TI sub-variable spectra
+  int A[kMaxSub];  // their indices in the TI spectra array
+  int circular;
+  float invN;   // 1 / prod(f)
+  long long H;  // half-spectrum size h0*f1*f2
+};
+
+// Per path node of a wave. Valid TI centres: lo[i] <= c_i <= hi[i].
+struct NodeInfo {
+  int lo[3];
+  int hi[3];
+  int nbStart;
+  int nb;
+  float delta0;   // added to the map (true mismatch) — see scale
+  float scale;    // map = raw * scale + delta0
+  float seed;
+  uint32_t seedbits;  // seedFromUnitFloat(seed)
+  float quantum;      // tie quantum (0: none)
+  uint32_t cell;      // simulation cell (simulation driver only)
+};
+
+// Candidate of the device selection rule.
+struct Cand {
+  float key;    // value (or rint(value/quantum))
+  uint32_t h;   // tie hash
+  uint32_t pos; // TI position (flattened, dim0 fastest)
+  float val;    // map value
+};
+
+__device__ __forceinline__ bool cand_better(const Cand& a, const Cand& b) {
+  if (a.key != b.key) return a.key > b.key;
+  if (a.h != b.h) return a.h > b.h;
+  return a.pos < b.pos;
+}
+
+__host__ __device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+
+__host__ __device__ __forceinline__ uint32_t tie_hash(uint32_t seedbits, uint32_t pos) {
+  return fmix32(pos * 0x9E3779B1u + seedbits);
+}
+
+}  // namespace qsmi

@@ -1,0 +1,231 @@
+/*
+ * qsmi — C-ABI of the MI355X-native QuickSampling exact scan.
+ *
+ * Plain pointers, sizes and integer status codes; no C++ or torch types.
+ * Every entry point names the reference interface it replaces
+ * (paths relative to the G2S source tree).
+ *
+ * Conventions
+ *  - Grids are flattened with dimension 0 fastest (the reference's
+ *    DataImage order). ndim is 1, 2 or 3.
+ *  - "position" = flattened TI cell index on the TI grid (srcSize);
+ *    "index"    = the reference's encoded map index on the FFT grid
+ *                 (src/CPUThreadDevice.cpp:257-284).
+ *  - Neighbour offsets are relative to the simulated cell, NOT yet shifted
+ *    by the bounding-box minimum (as candidateForPattern receives them).
+ *  - Neighbour values are the converted sub-variable values the reference's
+ *    QuantileSamplingModule passes on (P0 = w, P1 = w*x, ...,
+ *    include/quantileSamplingModule.hpp:368-410), nNeighbors x nSub,
+ *    sub-variable fastest.
+ *  - Calls on one device are stream-ordered on that device's own HIP stream;
+ *    a device must not be used from two host threads at once.
+ *  - Host buffers may be pageable; the library stages them.
+ */
+#ifndef QSMI_H
+#define QSMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qsmi_ti qsmi_ti;          /* TI spectra resident on one GPU */
+typedef struct qsmi_device qsmi_device;  /* one scan engine over a qsmi_ti */
+typedef struct qsmi_wave qsmi_wave;      /* a device-resident path wave */
+
+enum {
+  QSMI_OK = 0,
+  QSMI_ERR_ARG = 1,          /* invalid argument (message in qsmi_last_error) */
+  QSMI_ERR_HIP = 2,          /* HIP runtime error */
+  QSMI_ERR_UNSUPPORTED = 3,  /* valid request outside what this build does */
+  QSMI_ERR_NOMEM = 4
+};
+
+/* Last error message of the calling thread ("" when none). */
+const char* qsmi_last_error(void);
+
+/* Replaces g2s_cudaGetDeviceCount (src/NvidiaGPUAcceleratorDevice.cu:26-29). */
+int qsmi_get_device_count(int* count);
+
+/* FFT size rule of SharedMemoryManager (src/sharedMemoryManager.cpp:26-58):
+ * smallest size >= src of the form 2^a 3^b 5^c 7^d {1, 11, 13}. */
+unsigned qsmi_fft_size(unsigned src);
+
+/* Replaces AcceleratorDevice::allocAndInitSharedMemory
+ * (include/AcceleratorDevice.hpp:46; src/NvidiaGPUAcceleratorDevice.cu:350-389;
+ * src/CPUThreadDevice.cpp:171-211) together with the SharedMemoryManager that
+ * calls it (src/sharedMemoryManager.cpp:68-129).
+ * subImages: nSub images of prod(srcSize) floats on the TI grid, NOT
+ * reflected (y^2, y for a continuous variable, 1(y==c) per class; NaN -> 0,
+ * include/DataImage.hpp:538-599). fftSize may be NULL (size rule applied).
+ * Every FFT extent must be <= 4096. The spectra stay in HBM until destroy. */
+int qsmi_ti_create(int gpu, int ndim, const unsigned* srcSize, const unsigned* fftSize, int nSub,
+                   const float* subImages, qsmi_ti** out);
+/* Replaces AcceleratorDevice::freeSharedMemory (include/AcceleratorDevice.hpp:47). */
+int qsmi_ti_destroy(qsmi_ti* ti);
+
+/* Replaces the c_NvidiaGPUAcceleratorDevice factory
+ * (src/NvidiaGPUAcceleratorDevice.cu:229-232) for one coefficient layer
+ * (no cross-measurement layer): alpha is the OperationMatrix of
+ * include/OperationMatrix.hpp:9-73, nSub x nSub, element (A,B) at
+ * alpha[A + B*nSub]. circularTI as in the reference (requires
+ * fftSize == srcSize). The device starts with trueMismatch = 1 (what qs sets
+ * for accelerator devices, src/qs.cpp:1583). */
+int qsmi_device_create(qsmi_ti* ti, const float* alpha, int circularTI, qsmi_device** out);
+int qsmi_device_destroy(qsmi_device* dev);
+
+/* AcceleratorDevice::setTrueMismatch (include/AcceleratorDevice.hpp:59):
+ * 1 -> map = E (true mismatch incl. delta0); 0 -> map = N * (E - delta0),
+ * N = prod(fftSize), the unnormalised CPU-device map. */
+int qsmi_set_true_mismatch(qsmi_device* dev, int value);
+
+/* Tie quantum of the device selection rule (no reference counterpart).
+ * 0 (default): candidates ordered by value; > 0: by rint(value/quantum),
+ * which makes the exact ties of integer-valued categorical maps exact.
+ * Ties are broken by a hash of (seed, position): uniform among ties and
+ * deterministic given the node seed. */
+int qsmi_set_tie_quantum(qsmi_device* dev, float quantum);
+
+/* Replaces AcceleratorDevice::candidateForPattern
+ * (src/AcceleratorDevice.cpp:92-136 -> computeFreqMismatchMap +
+ * computeRealMissmatchAndRemoveWrongPattern, include/AcceleratorDevice.hpp:71-72):
+ * computes the node's mismatch map on the device (position space, -inf at
+ * positions whose neighbourhood leaves the TI). delta0 has one entry.
+ * Returns QSMI_OK; nNeighbors == 0 leaves the map untouched and returns
+ * QSMI_OK with *computed = 0 (the reference returns false). */
+int qsmi_candidate_for_pattern(qsmi_device* dev, int nNeighbors, const int* offsets, const float* values,
+                               const float* delta0, int* computed);
+
+/* Replaces AcceleratorDevice::searchKBiggest (include/AcceleratorDevice.hpp:68):
+ * the extendK best entries of the current map, best first, as values and
+ * encoded indices (reference index space). extendK <= 16. */
+int qsmi_search_k_biggest(qsmi_device* dev, float* errors, unsigned* encodedPosition, unsigned extendK, float seed);
+
+/* cvtIndexToPosition / cvtPositionToIndex (include/AcceleratorDevice.hpp:57-58),
+ * using the bounding box of the last candidate_for_pattern call. */
+int qsmi_cvt_index_to_position(qsmi_device* dev, unsigned index, unsigned* position);
+int qsmi_cvt_position_to_index(qsmi_device* dev, unsigned position, unsigned* index);
+
+/* getArraySize / getErrorsArraySize (include/AcceleratorDevice.hpp:51-52): prod(fftSize). */
+int qsmi_get_array_size(qsmi_device* dev, unsigned* size);
+
+/* getErrorAtPosition for every position (include/AcceleratorDevice.hpp:54):
+ * copies the current map, prod(srcSize) floats in position space. */
+int qsmi_get_error_map(qsmi_device* dev, float* out);
+
+/* setValueInErrorArray (include/AcceleratorDevice.hpp:65), index space. */
+int qsmi_set_value_in_error_array(qsmi_device* dev, unsigned index, float value);
+
+/* Path-wave batch: nNodes independent nodes in one launch sequence
+ * (replaces nNodes x [candidateForPattern + searchKBiggest + the
+ * QuantileSamplingModule::sample draw, include/quantileSamplingModule.hpp:90-126]).
+ * nbStart: nNodes+1 prefix offsets into offsets (x ndim) / values (x nSub).
+ * delta0, seeds: per node. k: the QS k (K = ceil(k) <= 16).
+ * Outputs (host): outErrors/outPositions nNodes x K (TI positions, best
+ * first; UINT32_MAX when fewer valid positions exist), outChosen (nullable)
+ * the TI position drawn with rank floor(seed*k). Blocks until done. */
+int qsmi_scan_wave(qsmi_device* dev, int nNodes, const int* nbStart, const int* offsets, const float* values,
+                   const float* delta0, const float* seeds, float k, float* outErrors, unsigned* outPositions,
+                   unsigned* outChosen);
+
+/* The same wave kept resident in HBM, for repeated timed launches:
+ * upload once, run (asynchronous, stream-ordered), read results. */
+int qsmi_wave_create(qsmi_device* dev, int nNodes, const int* nbStart, const int* offsets, const float* values,
+                     const float* delta0, const float* seeds, float k, qsmi_wave** out);
+int qsmi_wave_run(qsmi_wave* wave);
+int qsmi_wave_read(qsmi_wave* wave, float* outErrors, unsigned* outPositions, unsigned* outChosen);
+int qsmi_wave_destroy(qsmi_wave* wave);
+
+/* Blocks until all work queued on the device's stream has finished. */
+int qsmi_synchronize(qsmi_device* dev);
+
+/* Device time of the last qsmi_wave_run / qsmi_scan_wave per kernel stage,
+ * measured with HIP events on the device stream (milliseconds):
+ * out[0] spectrum stage (pencil / direct), out[1] dim1 inverse (3-D only),
+ * out[2] rows + select, out[3] merge + draw. */
+int qsmi_last_stage_ms(qsmi_device* dev, float* out4);
+/* Enables (1) or disables (0) the per-stage events (off by default). */
+int qsmi_set_stage_timing(qsmi_device* dev, int enable);
+
+/* ======================================================================
+ * QuickSampling host layer: the qs.cpp / QuantileSamplingModule /
+ * simulation() steps around the scan, restated in C++ above this C-ABI.
+ * ====================================================================== */
+
+/* Category values of variable `var` in order of first appearance, NaN
+ * skipped (src/qs.cpp:1500-1522). *count receives the number found. */
+int qsmi_categories(size_t cells, int nVar, const float* data, int var, float* out, int maxOut, int* count);
+
+/* Coefficient matrix and conversions of DataImage::generateCoefMatrix4Xcorr
+ * (include/DataImage.hpp:626-710), no cross-measurement layer.
+ * types: nVar (0 continuous, 1 categorical); ncat: classes per categorical
+ * variable. Outputs (capacity 16 sub-variables): alpha [A + B*nSub],
+ * convTypes[B] (0 = P0 = w, 1 = P1 = w*x), subVarOf[B] (one-hot expanded
+ * variable of B), *nSub. */
+int qsmi_coef_matrix(int nVar, const int* types, const int* ncat, float* alpha, int* convTypes, int* subVarOf,
+                     int* nSub);
+
+/* TI sub-variable images of DataImage::convertInput4Xcorr
+ * (include/DataImage.hpp:538-599), unreflected, on the TI grid:
+ * out = nSub x prod(srcSize). catvals: classes of each categorical variable
+ * back to back. */
+int qsmi_ti_subimages(int ndim, const unsigned* srcSize, int nVar, const int* types, const float* ti,
+                      const int* ncat, const float* catvals, float* out);
+
+/* Sorted neighbour offset list of src/qs.cpp:1055-1137: every offset of the
+ * kernel box (half-width (size+1)/2-1), NaN-kernel offsets removed
+ * (kernel may be NULL), std::sort by increasing |d|^2 (libstdc++, so ties
+ * keep the reference's order). out: capacity x ndim ints. */
+int qsmi_offsets(int ndim, const unsigned* kernelSize, int kernelNVar, const float* kernel, int* out,
+                 int capacity, int* count);
+
+/* Random path and per-node seeds of src/qs.cpp:975, 1159-1201, 1250-1258:
+ * fully informed cells first, the rest std::shuffle'd with mt19937(seed),
+ * then one U[0,1) float per path entry from the same generator. */
+int qsmi_path(size_t cells, const unsigned char* informed, unsigned seed, unsigned* path, float* seeds,
+              unsigned* beginPath);
+
+/* In-memory QuickSampling run, the counterpart of g2s::qs::runInMemory
+ * (include/qsCore.hpp:17-58, src/qsCore.cpp:86-170) for one TI without NaN:
+ * qs.cpp's set-up, the simulation() path loop (include/simulation.hpp:43-710)
+ * with the scan on the GPU, path nodes packed into waves of independent
+ * nodes (serial-equivalent result). */
+typedef struct {
+  int gpu;
+  int ndim;
+  unsigned tiSize[3];
+  int nVar;
+  const int* varTypes;     /* nVar: 0 continuous, 1 categorical */
+  const float* ti;         /* prod(tiSize) x nVar, no NaN */
+  unsigned simSize[3];
+  const float* di;         /* prod(simSize) x nVar; all-NaN cells are simulated,
+                              fully informed cells are hard data */
+  unsigned kernelSize[3];  /* all 0: default uniform kernel (src/qs.cpp:1010-1042) */
+  const float* kernel;     /* prod(kernelSize) x nVar, or NULL */
+  int n;                   /* -n, centre included */
+  float k;                 /* -k */
+  unsigned seed;           /* -s */
+  int waveSize;            /* max nodes per wave, 0 -> 256 */
+  float tieQuantum;        /* < 0 automatic: 0 with a continuous variable,
+                              2^(ceil(log2(n*max|w|))-16) otherwise */
+} qsmi_qs_request;
+
+typedef struct {
+  float* simulation;       /* out: prod(simSize) x nVar */
+  unsigned* index;         /* out: prod(simSize), importDataIndex (0 where not simulated) */
+  double setupMs;          /* out: TI spectra, offsets, path */
+  double pathLoopMs;       /* out: neighbour search + waves + writes (the reference's timed loop) */
+  double scanMs;           /* out: device time of the scan stages inside the loop */
+  long long nodes;         /* out: simulated path nodes */
+  int waves;               /* out: waves launched */
+} qsmi_qs_result;
+
+int qsmi_qs_run_in_memory(const qsmi_qs_request* req, qsmi_qs_result* res);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QSMI_H */

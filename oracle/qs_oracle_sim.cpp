@@ -1,0 +1,313 @@
+// QuickSampling CPU oracle, simulation driver — TEST INFRASTRUCTURE ONLY.
+//
+// Serial restatement of the reference QS node loop for one training image:
+//   include/simulation.hpp:304-666  (node body: gather, one-hot, sample, write)
+//   include/quantileSamplingModule.hpp:90-126, 342-577 (sample, conversion,
+//                                                      scan, select, draw)
+//   src/qs.cpp:1604-1641            (posterior path)
+// The reference runs the loop over OpenMP threads and spin-waits on
+// neighbours that are earlier in the path but not yet written
+// (simulation.hpp:463-473), which makes its result identical to the serial
+// order restated here.
+//
+// Supported here (and by the device): fully-NaN cells to simulate, fully
+// informed hard-data cells, one value of n for all variables, one TI without
+// NaN, no -nV / -fs / kernel images / transforms.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "fft_cpu.hpp"
+
+using namespace orc;
+
+extern "C" {
+// from qs_oracle.cpp
+int orc_ti_subimages(int D, const unsigned* s, const unsigned* f, int nVar, const int* types, const float* data,
+                     const int* ncat, const float* catvals, int reflected, float* out);
+int orc_coef_matrix(int nVar, const int* types, const int* ncat, float* alpha, int* convTypes, int* subVarOf);
+void orc_fft_size(int D, const unsigned* src, unsigned* out);
+float orc_convert_neighbors(int D, int nb, const int* off, int nExp, const float* vals, int nSub,
+                            const int* convTypes, const int* subVarOf, int nVarOrig, const int* typesOrig,
+                            const unsigned* kdims, const float* kernel, float* u);
+void orc_map_direct_f64(int D, const unsigned* s, int nSub, const float* Y, const float* alpha, int nb,
+                        const int* off, const float* u, double delta0, int circular, double* out);
+void orc_map_refflow_f32(int D, const unsigned* s, const unsigned* f, int nSub, const float* Yhat,
+                         const float* alpha, int nb, const int* off, const float* u, float delta0, int circular,
+                         int trueMismatch, float* out);
+void orc_map_refflow_f64(int D, const unsigned* s, const unsigned* f, int nSub, const double* Yhat,
+                         const float* alpha, int nb, const int* off, const float* u, float delta0, int circular,
+                         int trueMismatch, float* out);
+void orc_ti_spectra_f32(int D, const unsigned* f, int nSub, const float* imgs, float* out);
+void orc_ti_spectra_f64(int D, const unsigned* f, int nSub, const float* imgs, double* out);
+void orc_fkst(const float* data, unsigned N, unsigned short k, float seed, float* out, unsigned* pos);
+unsigned orc_index_to_position(int D, const unsigned* f, const unsigned* s, const int* mn, unsigned index);
+unsigned orc_seed_from_unit_float(float seed);
+uint32_t orc_tie_hash(uint32_t seedbits, uint32_t position);
+void orc_bbox(int D, int nb, const int* off, int* mn, int* mx);
+unsigned orc_draw_rank(float seed, float k, unsigned nTI);
+
+struct OrcSim {
+  int D;
+  unsigned tiDims[3];
+  int nVar;
+  const int* types;  // 0 continuous, 1 categorical
+  const float* ti;   // cells x nVar
+  const int* ncat;   // per categorical variable
+  const float* catvals;
+  unsigned simDims[3];
+  unsigned kdims[3];
+  const float* kernel;  // expanded kernel, cells(kdims) x nExp
+  int nOffsets;
+  const int* offsets;  // sorted offset list, nOffsets x D
+  int n;
+  float k;
+  const unsigned* path;  // full path (hard data first)
+  const float* seeds;    // per path entry
+  unsigned beginPath;
+  unsigned pathLen;
+  int mode;       // 0: direct FP64 + device rule; 1: refflow FP32 + fKst; 2: refflow FP64 + fKst
+  float quantum;  // tie quantum of the device rule (0: none)
+};
+
+struct OrcSimCtx {
+  OrcSim p;
+  int D;
+  unsigned s[3], f[3], sd[3];
+  size_t S, F, SD;
+  int nSub = 0, nExp = 0;
+  std::vector<float> alpha;
+  std::vector<int> convTypes, subVarOf;
+  std::vector<float> Y;       // unreflected sub-images on the TI grid (nSub x S)
+  std::vector<float> Yhat32;  // reflected spectra (FP32)
+  std::vector<double> Yhat64; // reflected spectra (FP64)
+  std::vector<float> catflat;
+  std::vector<int> typesOrig;
+  std::vector<unsigned> posterior;
+};
+
+void* orc_sim_create(const OrcSim* p) {
+  auto* c = new OrcSimCtx();
+  c->p = *p;
+  c->D = p->D;
+  c->S = c->F = c->SD = 1;
+  for (int i = 0; i < 3; ++i) {
+    c->s[i] = (i < p->D) ? p->tiDims[i] : 1;
+    c->sd[i] = (i < p->D) ? p->simDims[i] : 1;
+  }
+  orc_fft_size(p->D, c->s, c->f);
+  for (int i = p->D; i < 3; ++i) c->f[i] = 1;
+  for (int i = 0; i < 3; ++i) {
+    c->S *= c->s[i];
+    c->F *= c->f[i];
+    c->SD *= c->sd[i];
+  }
+  int nCatVars = 0;
+  c->nExp = 0;
+  for (int v = 0; v < p->nVar; ++v) {
+    c->typesOrig.push_back(p->types[v]);
+    if (p->types[v] == 0)
+      c->nExp += 1;
+    else
+      c->nExp += p->ncat[nCatVars++];
+  }
+  c->alpha.resize(64 * 64);
+  c->convTypes.resize(64);
+  c->subVarOf.resize(64);
+  c->nSub = orc_coef_matrix(p->nVar, p->types, p->ncat, c->alpha.data(), c->convTypes.data(), c->subVarOf.data());
+  c->alpha.resize((size_t)c->nSub * c->nSub);
+  // unreflected sub-images on the TI grid
+  c->Y.resize((size_t)c->nSub * c->S);
+  orc_ti_subimages(p->D, c->s, c->s, p->nVar, p->types, p->ti, p->ncat, p->catvals, 0, c->Y.data());
+  if (p->mode == 1 || p->mode == 2) {
+    std::vector<float> refl((size_t)c->nSub * c->F);
+    orc_ti_subimages(p->D, c->s, c->f, p->nVar, p->types, p->ti, p->ncat, p->catvals, 1, refl.data());
+    size_t H = (size_t)(c->f[0] / 2 + 1) * c->f[1] * c->f[2];
+    if (p->mode == 1) {
+      c->Yhat32.resize(2 * H * c->nSub);
+      orc_ti_spectra_f32(p->D, c->f, c->nSub, refl.data(), c->Yhat32.data());
+    } else {
+      c->Yhat64.resize(2 * H * c->nSub);
+      orc_ti_spectra_f64(p->D, c->f, c->nSub, refl.data(), c->Yhat64.data());
+    }
+  }
+  // posterior path — src/qs.cpp:1615-1641: hard data 0, path entry i -> i
+  c->posterior.assign(c->SD, UINT_MAX);
+  for (unsigned i = 0; i < p->beginPath; ++i) c->posterior[p->path[i]] = 0;
+  for (unsigned i = p->beginPath; i < p->pathLen; ++i) c->posterior[p->path[i]] = i - p->beginPath;
+  return c;
+}
+
+void orc_sim_free(void* h) { delete static_cast<OrcSimCtx*>(h); }
+
+void orc_sim_posterior(void* h, unsigned* out) {
+  auto* c = static_cast<OrcSimCtx*>(h);
+  std::copy(c->posterior.begin(), c->posterior.end(), out);
+}
+
+int orc_sim_nsub(void* h) { return static_cast<OrcSimCtx*>(h)->nSub; }
+
+// Neighbour gather of one path node — include/simulation.hpp:447-511 (the
+// centre counts as the first neighbour; a cell is informed when its
+// posterior order is <= the node's). Writes offset-list indices; returns the
+// count.
+int orc_sim_gather(void* h, unsigned pathIdx, int* outIdx) {
+  auto* c = static_cast<OrcSimCtx*>(h);
+  const OrcSim& p = c->p;
+  unsigned cell = p.path[p.beginPath + pathIdx];
+  unsigned ord = pathIdx;
+  int cnt = 0;
+  int cc[3];
+  size_t rem = cell;
+  for (int i = 0; i < c->D; ++i) {
+    cc[i] = int(rem % c->sd[i]);
+    rem /= c->sd[i];
+  }
+  for (int o = 0; o < p.nOffsets && cnt < p.n; ++o) {
+    bool inside = true;
+    size_t t = 0, mul = 1;
+    for (int i = 0; i < c->D; ++i) {
+      int x = cc[i] + p.offsets[o * c->D + i];
+      inside &= (x >= 0 && x < int(c->sd[i]));
+      t += (size_t)x * mul;
+      mul *= c->sd[i];
+    }
+    if (!inside) continue;
+    if (c->posterior[t] <= ord) outIdx[cnt++] = o;
+  }
+  return cnt;
+}
+
+// One node's scan and draw given the DI state. Writes the K best
+// (value, TI position) pairs sorted best first, and returns the TI position
+// chosen by the draw (or the marginal draw when only the centre is a
+// neighbour, simulation.hpp:586-606). *nbOut = neighbour count.
+unsigned orc_sim_node(void* h, const float* di, unsigned pathIdx, int K, float* outVal, unsigned* outPos,
+                      int* nbOut) {
+  auto* c = static_cast<OrcSimCtx*>(h);
+  const OrcSim& p = c->p;
+  const int D = c->D;
+  std::vector<int> idx(p.n > 0 ? p.n : 1);
+  int nb = orc_sim_gather(h, pathIdx, idx.data());
+  *nbOut = nb;
+  unsigned cell = p.path[p.beginPath + pathIdx];
+  float seed = p.seeds[p.beginPath + pathIdx];
+  for (int i = 0; i < K; ++i) {
+    outVal[i] = -INFINITY;
+    outPos[i] = UINT_MAX;
+  }
+  if (nb <= 1) {
+    // marginal draw: position = floor(seed * cells) in float
+    unsigned position = (unsigned)int(std::floor(seed * (float)(unsigned)c->S));
+    return position;
+  }
+  // gather offsets and one-hot expanded values
+  std::vector<int> off((size_t)nb * D);
+  std::vector<float> vals((size_t)nb * c->nExp);
+  int cc[3];
+  size_t rem = cell;
+  for (int i = 0; i < D; ++i) {
+    cc[i] = int(rem % c->sd[i]);
+    rem /= c->sd[i];
+  }
+  for (int j = 0; j < nb; ++j) {
+    size_t t = 0, mul = 1;
+    for (int i = 0; i < D; ++i) {
+      int d = p.offsets[idx[j] * D + i];
+      off[(size_t)j * D + i] = d;
+      t += (size_t)(cc[i] + d) * mul;
+      mul *= c->sd[i];
+    }
+    // one-hot expansion — simulation.hpp:512-537
+    int e = 0, catIdx = 0;
+    const float* cv = p.catvals;
+    for (int v = 0; v < p.nVar; ++v) {
+      float x = di[t * p.nVar + v];
+      if (p.types[v] == 0) {
+        vals[(size_t)j * c->nExp + e++] = x;
+      } else {
+        for (int q = 0; q < p.ncat[catIdx]; ++q) vals[(size_t)j * c->nExp + e++] = (x == cv[q]) ? 1.f : 0.f;
+        cv += p.ncat[catIdx];
+        catIdx++;
+      }
+    }
+  }
+  std::vector<float> u((size_t)nb * c->nSub);
+  float delta0 = orc_convert_neighbors(D, nb, off.data(), c->nExp, vals.data(), c->nSub, c->convTypes.data(),
+                                       c->subVarOf.data(), p.nVar, c->typesOrig.data(), p.kdims, p.kernel, u.data());
+  if (p.mode == 0) {
+    std::vector<double> map(c->S);
+    orc_map_direct_f64(D, c->s, c->nSub, c->Y.data(), c->alpha.data(), nb, off.data(), u.data(), (double)delta0, 0,
+                       map.data());
+    // device rule on the exact map
+    uint32_t sb = orc_seed_from_unit_float(seed);
+    struct Cd {
+      double key;
+      uint32_t h, pos;
+      double val;
+    };
+    auto better = [](const Cd& a, const Cd& b) {
+      if (a.key != b.key) return a.key > b.key;
+      if (a.h != b.h) return a.h > b.h;
+      return a.pos < b.pos;
+    };
+    std::vector<Cd> best;
+    for (size_t q = 0; q < c->S; ++q) {
+      double v = map[q];
+      if (!std::isfinite(v)) continue;
+      Cd cd{p.quantum > 0 ? std::rint(v / (double)p.quantum) : v, orc_tie_hash(sb, (uint32_t)q), (uint32_t)q, v};
+      if ((int)best.size() < K) {
+        best.push_back(cd);
+        std::sort(best.begin(), best.end(), better);
+      } else if (better(cd, best.back())) {
+        best.back() = cd;
+        std::sort(best.begin(), best.end(), better);
+      }
+    }
+    for (int i = 0; i < (int)best.size(); ++i) {
+      outVal[i] = (float)best[i].val;
+      outPos[i] = best[i].pos;
+    }
+  } else {
+    std::vector<float> map(c->F);
+    if (p.mode == 1)
+      orc_map_refflow_f32(D, c->s, c->f, c->nSub, c->Yhat32.data(), c->alpha.data(), nb, off.data(), u.data(),
+                          delta0, 0, 1, map.data());
+    else
+      orc_map_refflow_f64(D, c->s, c->f, c->nSub, c->Yhat64.data(), c->alpha.data(), nb, off.data(), u.data(),
+                          delta0, 0, 1, map.data());
+    std::vector<unsigned> pos(K);
+    orc_fkst(map.data(), (unsigned)c->F, (unsigned short)K, seed, outVal, pos.data());
+    int mn[3], mx[3];
+    orc_bbox(D, nb, off.data(), mn, mx);
+    for (int i = 0; i < K; ++i)
+      outPos[i] = (pos[i] == UINT_MAX) ? UINT_MAX : orc_index_to_position(D, c->f, c->s, mn, pos[i]);
+  }
+  // QSM::sample — sort the candidates (already best first), rank by the seed
+  unsigned r = orc_draw_rank(seed, p.k, 1);
+  if ((int)r >= K) r = K - 1;
+  return outPos[r];
+}
+
+// Serial simulation: fills di (cells x nVar) and idx (importDataIndex).
+void orc_sim_run(void* h, float* di, unsigned* idxOut) {
+  auto* c = static_cast<OrcSimCtx*>(h);
+  const OrcSim& p = c->p;
+  int K = (int)std::ceil(p.k);
+  std::vector<float> v(K);
+  std::vector<unsigned> pos(K);
+  for (unsigned i = 0; i + p.beginPath < p.pathLen; ++i) {
+    unsigned cell = p.path[p.beginPath + i];
+    int nb;
+    unsigned tpos = orc_sim_node(h, di, i, K, v.data(), pos.data(), &nb);
+    if (tpos == UINT_MAX) continue;
+    idxOut[cell] = tpos;  // importDataIndex = index*nTI + TI with nTI = 1
+    for (int q = 0; q < p.nVar; ++q) di[(size_t)cell * p.nVar + q] = p.ti[(size_t)tpos * p.nVar + q];
+  }
+}
+
+}  // extern "C"
