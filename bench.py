@@ -1,0 +1,148 @@
+"""Benchmark: QuickSampling nodes/s at 1000^2 TI, n=80, k=1.5 (continuous).
+
+One step = one full QS realisation of the 1000x1000 simulation grid from the
+1000x1000 log-normal GRF training image (1,000,000 path nodes), i.e. the
+reference's timed path loop (src/qs.cpp:2153,2229-2235): neighbour search,
+path waves on the MI355X scan, draws and writes. The TI spectra are built
+before the timed region (inputs resident in HBM).
+
+Prints one JSON line (rank 0). Single GPU only.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=1000, help="TI and simulation extent (default 1000)")
+    ap.add_argument("--wave", type=int, default=256, help="max path nodes per wave")
+    ap.add_argument("--cpu-sample", type=int, default=48, help="path nodes of the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_baseline(cfg, sim, threads, sample):
+    """The oracle in reference-dataflow mode (FP32 FFT scan + fKst select,
+    src/CPUThreadDevice.cpp:349-490 restated) timed on this host for a
+    contiguous slice of the same path, one node per thread at a time."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import g2s_amd
+    import oracle
+
+    oracle.build()
+    ti, dims = cfg["ti"], cfg["ti_dims"]
+    kd = [d // 2 + 1 for d in dims]
+    offs = oracle.offsets(kd)
+    kern = np.ones(int(np.prod(kd)), np.float32)
+    informed = np.zeros(int(np.prod(cfg["sim_dims"])), np.uint8)
+    p, s, b = oracle.path(informed, cfg["seed"])
+    t0 = time.time()
+    orc = oracle.Sim(ti, dims, cfg["types"], [], [], cfg["sim_dims"], kd, kern, offs, cfg["n"], cfg["k"], p, s, b,
+                     mode=1)
+    setup = time.time() - t0
+    start = len(p) // 10  # a slice 10% into the path
+    nodes = list(range(start, start + sample))
+    di = np.ascontiguousarray(sim, np.float32)
+
+    def one(i):
+        return orc.node(di, i)
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(one, nodes[:threads]))  # warm (FFT plans, caches)
+        t0 = time.time()
+        res = list(ex.map(one, nodes))
+        dt = time.time() - t0
+    return dict(value=len(nodes) / dt, unit="nodes/s", cores=threads, kind="port",
+                sample=(f"{len(nodes)} path nodes {start}..{start + len(nodes) - 1} of the same 1000^2 "
+                        f"realisation, oracle reference-dataflow FP32 FFT + fKst, {threads} threads, "
+                        f"own mixed-radix CPU FFT (no FFTW on the host); spectra setup {setup:.1f}s untimed"))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
+        sys.exit(2)
+    import g2s_amd
+    from g2s_amd import synth
+
+    cfg = synth.config2_continuous_2d(args.size)
+    ti, dims, sdims = cfg["ti"], cfg["ti_dims"], cfg["sim_dims"]
+    di = np.full(int(np.prod(sdims)), np.nan, np.float32)
+    n, k = cfg["n"], cfg["k"]
+    print(f"[bench] workload TI {dims} sim {sdims} n={n} k={k}", file=sys.stderr, flush=True)
+
+    # warmup: full realisations with other seeds (kernel load, allocations)
+    for w in range(args.warmup):
+        _, _, st = g2s_amd.qs(ti, dims, di, sdims, cfg["types"], n, k, seed=1000 + w, wave_size=args.wave)
+        print(f"[bench] warmup {w}: {st}", file=sys.stderr, flush=True)
+
+    times, stats = [], []
+    sim = None
+    for s in range(args.steps):
+        t0 = time.perf_counter()
+        sim, idx, st = g2s_amd.qs(ti, dims, di, sdims, cfg["types"], n, k, seed=cfg["seed"] + s, wave_size=args.wave)
+        wall = time.perf_counter() - t0
+        times.append(st["path_loop_ms"])
+        stats.append(st)
+        print(f"[bench] step {s}: wall {wall:.2f}s {st}", file=sys.stderr, flush=True)
+
+    nodes = stats[-1]["nodes"]
+    ms = float(np.mean(times))
+    value = nodes / (ms / 1e3)
+    # roofline of the dominant kernel (the fused spectrum stage): algorithmic
+    # bytes per node A = V*S*8 + 2*N*4 (V=2 spectra of S=501*1000 complex64,
+    # N=10^6 map floats written and read) over its event-timed duration
+    V, S, N = 2, (dims[0] // 2 + 1) * dims[1], dims[0] * dims[1]
+    a_node = V * S * 8 + 2 * N * 4
+    st = stats[-1]
+    pencil_ms = st.get("pencil_ms", 0.0)
+    roof = None
+    if pencil_ms > 0:
+        achieved = a_node * st["scanned_nodes"] / (pencil_ms / 1e3) / 1e9
+        roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                    traffic=None)
+    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    cpu = None
+    if args.cpu_sample > 0:
+        cpu = cpu_baseline(cfg, sim, threads, args.cpu_sample)
+    out = {
+        "metric": "QS nodes/sec at 1000^2 TI n=80 k=1.5",
+        "value": value,
+        "unit": "nodes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"QS unconditional 2D continuous, TI {dims[0]}x{dims[1]} exp(GRF range 20), "
+                               f"sim {sdims[0]}x{sdims[1]}, n={n}, k={k}, waves<={args.wave}",
+                   "nodes_per_step": nodes},
+        "hbm_roofline_frac_whole_loop": a_node * value / (HBM_PEAK_GBS * 1e9),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "stats": st,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

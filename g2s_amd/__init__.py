@@ -318,6 +318,7 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
     rs.simulation = sim.ctypes.data
     rs.index = idx.ctypes.data
     check(lib().qsmi_qs_run_in_memory(C.byref(rq), C.byref(rs)))
-    stats = dict(setup_ms=rs.setupMs, path_loop_ms=rs.pathLoopMs, scan_ms=rs.scanMs, nodes=rs.nodes,
-                 waves=rs.waves)
+    stats = dict(setup_ms=rs.setupMs, path_loop_ms=rs.pathLoopMs, scan_ms=rs.scanMs, pencil_ms=rs.stageMs[0],
+                 dim1_ms=rs.stageMs[1], rows_ms=rs.stageMs[2], merge_ms=rs.stageMs[3], nodes=rs.nodes,
+                 scanned_nodes=rs.scannedNodes, waves=rs.waves)
     return sim, idx, stats

@@ -110,6 +110,8 @@ class QsResult(C.Structure):
         ("setupMs", C.c_double),
         ("pathLoopMs", C.c_double),
         ("scanMs", C.c_double),
+        ("stageMs", C.c_double * 4),
         ("nodes", C.c_longlong),
+        ("scannedNodes", C.c_longlong),
         ("waves", C.c_int),
     ]

@@ -15,7 +15,8 @@
 #include "qsmi_kernels.hpp"
 
 namespace qsmi {
-hipError_t set_lds_limits();
+hipError_t set_lds_limits(int device, size_t* limitOut);
+size_t fft_lds_bytes_of(int L, int nvec);
 void launch_r2c_rows(const float* in, float2* out, const FftPlan& plan, const float2* tw, int lognv, long long nrows,
                      int h0, hipStream_t st);
 void launch_colfft(int dir, float2* base, const FftPlan& plan, const float2* tw, int lognv, long long rowStride,
@@ -383,15 +384,17 @@ int qsmi_ti_create(int gpu, int ndim, const unsigned* srcSize, const unsigned* f
     delete t;
     return fail(QSMI_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
   }
-  static bool ldsSet = false;
-  if (!ldsSet) {
-    e = set_lds_limits();
-    if (e != hipSuccess) {
-      delete t;
-      return fail(QSMI_ERR_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
-    }
-    ldsSet = true;
+  size_t ldsLimit = 0;
+  e = set_lds_limits(gpu, &ldsLimit);
+  if (e != hipSuccess) {
+    delete t;
+    return fail(QSMI_ERR_HIP, std::string("LDS limit: ") + hipGetErrorString(e));
   }
+  for (int i = 0; i < 3; ++i)
+    if (fft_lds_bytes_of(t->f[i], 1 << t->lognv[i]) > ldsLimit) {
+      delete t;
+      return fail(QSMI_ERR_UNSUPPORTED, "FFT extent too large for one LDS block");
+    }
 #define TCHK(expr)                                                                  \
   do {                                                                              \
     hipError_t _e = (expr);                                                         \

@@ -431,6 +431,8 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   std::vector<size_t> kcAll;
   std::vector<unsigned char> inAll;
   double scanMs = 0.0;
+  double stageMs[4] = {0, 0, 0, 0};
+  long long scanned = 0;
   int waves = 0;
   const float kf = rq->k;
   const int K = (int)std::ceil(kf);
@@ -527,6 +529,8 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
       float ms[4];
       qsmi_last_stage_ms(dev, ms);
       scanMs += ms[0] + ms[1] + ms[2] + ms[3];
+      for (int q = 0; q < 4; ++q) stageMs[q] += ms[q];
+      scanned += nN;
       waves++;
       for (int q = 0; q < nN; ++q) {
         const unsigned i = wNode[q];
@@ -550,6 +554,8 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   rs->setupMs = std::chrono::duration<double, std::milli>(t1 - t0).count();
   rs->pathLoopMs = std::chrono::duration<double, std::milli>(t2 - t1).count();
   rs->scanMs = scanMs;
+  for (int q = 0; q < 4; ++q) rs->stageMs[q] = stageMs[q];
+  rs->scannedNodes = scanned;
   rs->nodes = (long long)nPath;
   rs->waves = waves;
   return QSMI_OK;

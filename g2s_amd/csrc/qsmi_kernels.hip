@@ -506,26 +506,25 @@ namespace qsmi {
 
 static size_t fft_lds_bytes(int L, int nvec) { return (size_t)(L + 2 * L * nvec) * sizeof(float2); }
 
-hipError_t set_lds_limits() {
-  const int big = 160 * 1024;
-  hipError_t e = hipSuccess;
-  e = hipFuncSetAttribute((const void*)k_pencil, hipFuncAttributeMaxDynamicSharedMemorySize, big);
+// Raises the dynamic LDS limit of the FFT kernels to what the device allows
+// (160 KiB per workgroup on gfx950) minus a margin for static LDS.
+hipError_t set_lds_limits(int device, size_t* limitOut) {
+  int maxShared = 0;
+  hipError_t e = hipDeviceGetAttribute(&maxShared, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
   if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)k_r2c_rows, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)k_colfft<-1>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)k_colfft<+1>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)k_rows_select<2>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)k_rows_select<4>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)k_rows_select<8>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute((const void*)k_rows_select<16>, hipFuncAttributeMaxDynamicSharedMemorySize, big);
-  return e;
+  const int lim = maxShared - 1024;
+  *limitOut = (size_t)lim;
+  const void* fns[] = {(const void*)k_pencil,         (const void*)k_r2c_rows,        (const void*)k_colfft<-1>,
+                       (const void*)k_colfft<+1>,     (const void*)k_rows_select<2>,  (const void*)k_rows_select<4>,
+                       (const void*)k_rows_select<8>, (const void*)k_rows_select<16>};
+  for (const void* f : fns) {
+    e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
+
+size_t fft_lds_bytes_of(int L, int nvec) { return fft_lds_bytes(L, nvec); }
 
 void launch_r2c_rows(const float* in, float2* out, const FftPlan& plan, const float2* tw, int lognv, long long nrows,
                      int h0, hipStream_t st) {

@@ -218,7 +218,10 @@ typedef struct {
   double setupMs;          /* out: TI spectra, offsets, path */
   double pathLoopMs;       /* out: neighbour search + waves + writes (the reference's timed loop) */
   double scanMs;           /* out: device time of the scan stages inside the loop */
+  double stageMs[4];       /* out: the same per stage (see qsmi_last_stage_ms) */
   long long nodes;         /* out: simulated path nodes */
+  long long scannedNodes;  /* out: nodes that went through the scan (the rest
+                              drew from the marginal) */
   int waves;               /* out: waves launched */
 } qsmi_qs_result;
 

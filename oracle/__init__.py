@@ -105,6 +105,7 @@ def lib():
     _sig(L, "orc_sim_gather", C.c_int, [C.c_void_p, C.c_uint, i32p])
     _sig(L, "orc_sim_node", C.c_uint, [C.c_void_p, f32p, C.c_uint, C.c_int, f32p, u32p, C.POINTER(C.c_int)])
     _sig(L, "orc_sim_run", None, [C.c_void_p, f32p, u32p])
+    _sig(L, "orc_sim_node_map", C.c_int, [C.c_void_p, f32p, C.c_uint, f64p, C.c_void_p])
     _LIB = L
     return L
 
@@ -174,7 +175,8 @@ def coef_matrix(types, ncat):
     conv = np.zeros(64, np.int32)
     sub_of = np.zeros(64, np.int32)
     nsub = lib().orc_coef_matrix(len(types), types, ncat, alpha, conv, sub_of)
-    return alpha[: nsub * nsub].reshape(nsub, nsub).copy(), conv[:nsub].copy(), sub_of[:nsub].copy()
+    # alpha[A + B*nSub] -> matrix [A, B]
+    return alpha[: nsub * nsub].reshape(nsub, nsub).T.copy(), conv[:nsub].copy(), sub_of[:nsub].copy()
 
 
 def ti_subimages(ti, dims, types, ncat, catvals, fdims=None, reflected=False):
@@ -298,6 +300,13 @@ class Sim:
         nb = C.c_int(0)
         chosen = lib().orc_sim_node(self.h, f32(di).reshape(-1), i, self.K, v, p, C.byref(nb))
         return chosen, v, p, nb.value
+
+    def node_map(self, di, i, with_scale=True):
+        """The node's exact FP64 map and s(p) (position space)."""
+        m = np.zeros(int(np.prod(self.spec.tiDims[: self.spec.D])), np.float64)
+        sc = np.zeros_like(m) if with_scale else None
+        nb = lib().orc_sim_node_map(self.h, f32(di).reshape(-1), i, m, None if sc is None else sc.ctypes.data)
+        return m, sc, nb
 
     def run(self, di):
         di = f32(di).reshape(-1).copy()

@@ -182,32 +182,18 @@ int orc_sim_gather(void* h, unsigned pathIdx, int* outIdx) {
   return cnt;
 }
 
-// One node's scan and draw given the DI state. Writes the K best
-// (value, TI position) pairs sorted best first, and returns the TI position
-// chosen by the draw (or the marginal draw when only the centre is a
-// neighbour, simulation.hpp:586-606). *nbOut = neighbour count.
-unsigned orc_sim_node(void* h, const float* di, unsigned pathIdx, int K, float* outVal, unsigned* outPos,
-                      int* nbOut) {
-  auto* c = static_cast<OrcSimCtx*>(h);
+// Neighbours of one node as the serial loop sees them: offsets, converted
+// values (QSM conversion) and delta0. Returns the neighbour count.
+static int node_inputs(OrcSimCtx* c, const float* di, unsigned pathIdx, std::vector<int>& off, std::vector<float>& u,
+                       float* delta0, std::vector<float>* rawOut) {
   const OrcSim& p = c->p;
   const int D = c->D;
   std::vector<int> idx(p.n > 0 ? p.n : 1);
-  int nb = orc_sim_gather(h, pathIdx, idx.data());
-  *nbOut = nb;
+  int nb = orc_sim_gather(c, pathIdx, idx.data());
   unsigned cell = p.path[p.beginPath + pathIdx];
-  float seed = p.seeds[p.beginPath + pathIdx];
-  for (int i = 0; i < K; ++i) {
-    outVal[i] = -INFINITY;
-    outPos[i] = UINT_MAX;
-  }
-  if (nb <= 1) {
-    // marginal draw: position = floor(seed * cells) in float
-    unsigned position = (unsigned)int(std::floor(seed * (float)(unsigned)c->S));
-    return position;
-  }
-  // gather offsets and one-hot expanded values
-  std::vector<int> off((size_t)nb * D);
+  off.assign((size_t)nb * D, 0);
   std::vector<float> vals((size_t)nb * c->nExp);
+  if (rawOut) rawOut->assign((size_t)nb * p.nVar, NAN);
   int cc[3];
   size_t rem = cell;
   for (int i = 0; i < D; ++i) {
@@ -222,11 +208,14 @@ unsigned orc_sim_node(void* h, const float* di, unsigned pathIdx, int K, float* 
       t += (size_t)(cc[i] + d) * mul;
       mul *= c->sd[i];
     }
-    // one-hot expansion — simulation.hpp:512-537
+    // one-hot expansion — simulation.hpp:512-537. The node's own cell is
+    // still NaN when the serial loop reaches it, whatever `di` holds.
+    const bool isCentre = (t == cell);
     int e = 0, catIdx = 0;
     const float* cv = p.catvals;
     for (int v = 0; v < p.nVar; ++v) {
-      float x = di[t * p.nVar + v];
+      float x = isCentre ? NAN : di[t * p.nVar + v];
+      if (rawOut) (*rawOut)[(size_t)j * p.nVar + v] = x;
       if (p.types[v] == 0) {
         vals[(size_t)j * c->nExp + e++] = x;
       } else {
@@ -236,9 +225,36 @@ unsigned orc_sim_node(void* h, const float* di, unsigned pathIdx, int K, float* 
       }
     }
   }
-  std::vector<float> u((size_t)nb * c->nSub);
-  float delta0 = orc_convert_neighbors(D, nb, off.data(), c->nExp, vals.data(), c->nSub, c->convTypes.data(),
-                                       c->subVarOf.data(), p.nVar, c->typesOrig.data(), p.kdims, p.kernel, u.data());
+  u.assign((size_t)nb * c->nSub, 0.f);
+  *delta0 = orc_convert_neighbors(D, nb, off.data(), c->nExp, vals.data(), c->nSub, c->convTypes.data(),
+                                  c->subVarOf.data(), p.nVar, c->typesOrig.data(), p.kdims, p.kernel, u.data());
+  return nb;
+}
+
+// One node's scan and draw given the DI state. Writes the K best
+// (value, TI position) pairs sorted best first, and returns the TI position
+// chosen by the draw (or the marginal draw when only the centre is a
+// neighbour, simulation.hpp:586-606). *nbOut = neighbour count.
+unsigned orc_sim_node(void* h, const float* di, unsigned pathIdx, int K, float* outVal, unsigned* outPos,
+                      int* nbOut) {
+  auto* c = static_cast<OrcSimCtx*>(h);
+  const OrcSim& p = c->p;
+  const int D = c->D;
+  float seed = p.seeds[p.beginPath + pathIdx];
+  for (int i = 0; i < K; ++i) {
+    outVal[i] = -INFINITY;
+    outPos[i] = UINT_MAX;
+  }
+  std::vector<int> off;
+  std::vector<float> u;
+  float delta0 = 0.f;
+  int nb = node_inputs(c, di, pathIdx, off, u, &delta0, nullptr);
+  *nbOut = nb;
+  if (nb <= 1) {
+    // marginal draw: position = floor(seed * cells) in float
+    unsigned position = (unsigned)int(std::floor(seed * (float)(unsigned)c->S));
+    return position;
+  }
   if (p.mode == 0) {
     std::vector<double> map(c->S);
     orc_map_direct_f64(D, c->s, c->nSub, c->Y.data(), c->alpha.data(), nb, off.data(), u.data(), (double)delta0, 0,
@@ -291,6 +307,59 @@ unsigned orc_sim_node(void* h, const float* di, unsigned pathIdx, int K, float* 
   unsigned r = orc_draw_rank(seed, p.k, 1);
   if ((int)r >= K) r = K - 1;
   return outPos[r];
+}
+
+// The node's exact FP64 map (position space, -inf outside the valid box) and
+// the magnitude s(p) of its cancelling terms: per continuous variable
+// sum_j |w_j| (x_j^2 + y(p+d_j)^2), per categorical variable 2 sum_j |w_j|.
+// Returns the neighbour count.
+int orc_sim_node_map(void* h, const float* di, unsigned pathIdx, double* map, double* scale) {
+  auto* c = static_cast<OrcSimCtx*>(h);
+  const int D = c->D;
+  std::vector<int> off;
+  std::vector<float> u;
+  std::vector<float> raw;
+  float delta0 = 0.f;
+  int nb = node_inputs(c, di, pathIdx, off, u, &delta0, &raw);
+  orc_map_direct_f64(D, c->s, c->nSub, c->Y.data(), c->alpha.data(), nb, off.data(), u.data(), (double)delta0, 0,
+                     map);
+  if (!scale) return nb;
+  std::fill(scale, scale + c->S, 0.0);
+  // |w| per neighbour and sub-variable from the converted values: w = u of the
+  // P0 sub-variable (continuous) or |u| summed over the classes (categorical)
+  int b = 0;
+  for (int v = 0; v < c->p.nVar; ++v) {
+    if (c->p.types[v] == 0) {
+      // s += sum_j |w_j| y^2(p+d_j) + |w_j| x_j^2 : correlate |w| with the y^2 image
+      std::vector<float> us((size_t)nb * c->nSub, 0.f);
+      std::vector<float> al((size_t)c->nSub * c->nSub, 0.f);
+      al[b + b * c->nSub] = 1.f;
+      double cst = 0.0;
+      for (int j = 0; j < nb; ++j) {
+        float w = std::fabs(u[(size_t)j * c->nSub + b]);  // P0 sub-variable = w (0 for NaN values)
+        us[(size_t)j * c->nSub + b] = w;
+        float x = raw[(size_t)j * c->p.nVar + v];
+        if (!std::isnan(x)) cst += (double)w * x * x;
+      }
+      std::vector<double> part(c->S);
+      orc_map_direct_f64(D, c->s, c->nSub, c->Y.data(), al.data(), nb, off.data(), us.data(), cst, 0, part.data());
+      for (size_t q = 0; q < c->S; ++q) scale[q] += part[q];
+      b += 2;
+    } else {
+      int ci = 0;
+      for (int v2 = 0; v2 < v; ++v2) ci += (c->p.types[v2] != 0);
+      int ncl = c->p.ncat[ci];
+      double sw = 0.0;
+      for (int j = 0; j < nb; ++j) {
+        double wj = 0.0;
+        for (int q = 0; q < ncl; ++q) wj = std::max(wj, (double)std::fabs(u[(size_t)j * c->nSub + b + q]));
+        sw += wj;
+      }
+      for (size_t q = 0; q < c->S; ++q) scale[q] += 2.0 * sw;
+      b += ncl;
+    }
+  }
+  return nb;
 }
 
 // Serial simulation: fills di (cells x nVar) and idx (importDataIndex).

@@ -26,7 +26,7 @@ def grf2d(shape, rng, smooth=3.0):
         k2 = (np.fft.fftfreq(shape[0])[:, None, None] ** 2 + np.fft.fftfreq(shape[1])[None, :, None] ** 2
               + np.fft.rfftfreq(shape[2])[None, None, :] ** 2)
     f *= np.exp(-k2 * (smooth * 2 * np.pi) ** 2 / 2)
-    out = np.fft.irfftn(f, s=shape)
+    out = np.fft.irfftn(f, s=shape, axes=tuple(range(len(shape))))
     out = (out - out.mean()) / (out.std() + 1e-12)
     return out.astype(np.float32)
 

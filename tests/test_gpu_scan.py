@@ -177,7 +177,7 @@ def test_scan_wave_matches_per_node(kind, dims):
     nodes, refs = [], []
     for i in range(37):
         nb = int(rng.integers(2, 40))
-        off, x = neighbourhood(prob, rng, nb, int(rng.integers(2, 8)) if len(dims) < 3 else 3)
+        off, x = neighbourhood(prob, rng, nb, int(rng.integers(4, 8)) if len(dims) < 3 else 3)
         u, d0 = prob.convert(x)
         seed = float(rng.random())
         nodes.append((off, u, d0, seed))

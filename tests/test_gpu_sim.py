@@ -1,0 +1,140 @@
+"""Whole-path QuickSampling on the GPU (qsmi_qs_run_in_memory) against the
+oracle's serial node loop, node by node ("lockstep").
+
+The GPU run packs path nodes into waves of independent nodes; its result must
+be the serial one. For every path node i the oracle recomputes, from the GPU
+realisation restricted to the cells the serial loop would see, the node's
+neighbours, its FP64 map, the device-rule K best and the draw; the GPU's
+choice must equal it, except where the oracle's candidates are closer than
+the map tolerance (FP32 vs FP64), which is then reported and bounded.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import g2s_amd
+import oracle
+from g2s_amd import synth
+from qs_helpers import MAP_RTOL
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(m):
+    print(m, flush=True)
+    sys.stdout.flush()
+
+
+def default_kernel(ti_dims, nexp):
+    kd = [d // 2 + 1 for d in ti_dims]
+    return kd, np.ones(int(np.prod(kd)) * nexp, np.float32)
+
+
+def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatch_frac=0.01):
+    types = list(types)
+    nvar = len(types)
+    sim, idx, st = g2s_amd.qs(ti, ti_dims, di0, sim_dims, types, n, k, seed, wave_size=64, tie_quantum=quantum)
+    _log(f"  gpu run: {st}")
+    # oracle set-up identical to the reference's qs.cpp
+    ncat, catvals = [], []
+    for v in range(nvar):
+        if types[v] != 0:
+            c = oracle.categories(ti.reshape(-1, nvar), nvar, v)
+            ncat.append(len(c))
+            catvals.extend(c.tolist())
+    nexp = sum(1 if t == 0 else ncat[list(np.nonzero(types)[0]).index(i)] for i, t in enumerate(types))
+    kd, kern = default_kernel(ti_dims, nexp)
+    offs = oracle.offsets(kd)
+    di0f = di0.reshape(-1, nvar)
+    informed = (~np.isnan(di0f).any(axis=1)).astype(np.uint8)
+    p, s, b = oracle.path(informed, seed)
+    gp, gs, gb = g2s_amd.path(informed, seed)
+    assert np.array_equal(p, gp) and np.array_equal(s, gs) and b == gb
+    orc = oracle.Sim(ti, ti_dims, types, ncat, catvals, sim_dims, kd, kern, offs, n, k, p, s, b, mode=0,
+                     quantum=quantum)
+    K = int(np.ceil(k))
+    simf = sim.reshape(-1, nvar)
+    # hard data kept
+    hard = informed.astype(bool)
+    assert np.array_equal(simf[hard], di0f[hard])
+    assert not np.isnan(simf).any()
+    tif = ti.reshape(-1, nvar)
+    mismatches = 0
+    worst_gap = 0.0
+    for i in range(len(p) - b):
+        cell = p[b + i]
+        chosen, v, pos, nb = orc.node(simf, i)
+        if chosen == idx[cell]:
+            continue
+        mismatches += 1
+        # explained only by a near tie: the exact FP64 values at the two
+        # positions must be within the map tolerance of each other
+        assert nb > 1, f"marginal draw differs at node {i}"
+        m, sc, _ = orc.node_map(simf, i)
+        a, g = int(chosen), int(idx[cell])
+        assert np.isfinite(m[g]), f"node {i}: GPU chose an invalid position {g}"
+        gap = abs(m[a] - m[g])
+        bound = MAP_RTOL * (sc[a] + sc[g])
+        assert gap <= bound, f"node {i}: gpu {g} ({m[g]}) oracle {a} ({m[a]}) gap {gap} > {bound}"
+        worst_gap = max(worst_gap, gap / bound)
+        # the value written is the TI value at the GPU's choice
+        assert np.array_equal(simf[cell], tif[idx[cell]])
+    frac = mismatches / max(1, len(p) - b)
+    _log(f"  lockstep mismatches {mismatches} of {len(p) - b} nodes (largest gap / tolerance {worst_gap:.3f})")
+    assert frac <= max_mismatch_frac
+    return sim, idx, st
+
+
+def test_sim_continuous_2d():
+    g = synth.grf((48, 56), 4.0, seed=21)
+    ti, dims = synth.to_g2s(np.exp(g))
+    sdims = (40, 36)
+    di = np.full(int(np.prod(sdims)), np.nan, np.float32)
+    _log("continuous 2-D")
+    lockstep(ti, dims, di, sdims, [0], 20, 1.5, 7, 0.0)
+
+
+def test_sim_categorical_2d():
+    g = synth.grf((50, 50), 5.0, seed=22, anis=(3.0,), angle_deg=30.0)
+    ti, dims = synth.to_g2s((g > np.quantile(g, 0.72)).astype(np.float32))
+    sdims = (40, 40)
+    di = np.full(int(np.prod(sdims)), np.nan, np.float32)
+    q = float(2.0 ** (np.ceil(np.log2(15)) - 16))
+    _log("categorical 2-D")
+    lockstep(ti, dims, di, sdims, [1], 15, 1.2, 3, q)
+
+
+def test_sim_multivariate_hard_data():
+    a = synth.grf((40, 44), 4.0, seed=23)
+    b_ = synth.grf((40, 44), 4.0, seed=24)
+    c = 0.7 * a + 0.71 * b_
+    cat = np.digitize(c, np.quantile(c, [0.25, 0.5, 0.75])).astype(np.float32)
+    ti, dims = synth.to_g2s(a, cat)
+    sdims = (30, 32)
+    rng = np.random.default_rng(5)
+    di = np.full((int(np.prod(sdims)), 2), np.nan, np.float32)
+    pick = rng.choice(di.shape[0], size=di.shape[0] // 10, replace=False)
+    di[pick] = ti[rng.integers(0, ti.shape[0], size=pick.size)]
+    _log("multivariate 2-D with hard data")
+    lockstep(ti, dims, di.reshape(-1), sdims, [0, 1], 12, 1.5, 4, 0.0)
+
+
+def test_sim_categorical_3d():
+    g1 = synth.grf((12, 16, 18), (3.0, 5.0, 5.0), seed=25)
+    fac = (g1 > np.quantile(g1, 0.5)).astype(np.float32) + (g1 > np.quantile(g1, 0.8)).astype(np.float32)
+    ti, dims = synth.to_g2s(fac)
+    sdims = (14, 12, 8)
+    di = np.full(int(np.prod(sdims)), np.nan, np.float32)
+    q = float(2.0 ** (np.ceil(np.log2(20)) - 16))
+    _log("categorical 3-D")
+    lockstep(ti, dims, di, sdims, [1], 20, 1.5, 9, q)
+
+
+def test_sim_1d():
+    g = synth.grf((300,), 6.0, seed=26)
+    ti, dims = synth.to_g2s(np.exp(g))
+    sdims = (200,)
+    di = np.full(200, np.nan, np.float32)
+    _log("continuous 1-D")
+    lockstep(ti, dims, di, sdims, [0], 10, 1.5, 11, 0.0)
