@@ -28,6 +28,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=1000, help="TI and simulation extent (default 1000)")
+    ap.add_argument("--sim-size", type=int, default=0, help="simulation extent when smaller than the TI (profiling)")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="also run a plain 256 MiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     ap.add_argument("--wave", type=int, default=256, help="max path nodes per wave")
     ap.add_argument("--cpu-sample", type=int, default=48, help="path nodes of the CPU baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -81,7 +84,19 @@ def main():
     from g2s_amd import synth
 
     cfg = synth.config2_continuous_2d(args.size)
+    if args.sim_size:
+        cfg["sim_dims"] = (args.sim_size, args.sim_size)
     ti, dims, sdims = cfg["ti"], cfg["ti_dims"], cfg["sim_dims"]
+    if args.copy_calib:
+        import torch
+
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        b.copy_(a)  # reads 256 MiB, writes 256 MiB
+        torch.cuda.synchronize()
+        print("[bench] copy calibration: 268435456 B read + 268435456 B written", file=sys.stderr, flush=True)
     di = np.full(int(np.prod(sdims)), np.nan, np.float32)
     n, k = cfg["n"], cfg["k"]
     print(f"[bench] workload TI {dims} sim {sdims} n={n} k={k}", file=sys.stderr, flush=True)

@@ -35,6 +35,13 @@ void launch_merge_draw(const ScanGeom& g, const NodeInfo* info, const Cand* cand
                        int rowsPerBlock, int K, float kf, float* outVal, uint32_t* outPos, uint32_t* chosen,
                        hipStream_t st);
 void launch_fill(float* p, long long n, float v, hipStream_t st);
+long long ct1k_stride(int h0);
+void launch_transpose_spectra(const float2* Y, float2* YT, int h0, int f1, int nA, hipStream_t st);
+void launch_pencil1k(const ScanGeom& g, const float2* YT, const float2* tw1k, int nNodes, int npb,
+                     const NodeInfo* info, const int* nbOff, const float* nbVal, float2* CT, hipStream_t st);
+int rows1k_pairs(int s1);
+void launch_rows1k(const ScanGeom& g, const float2* tw1k, const NodeInfo* info, const float2* CT, int nNodes, int K,
+                   Cand* cand, float* map, hipStream_t st);
 }  // namespace qsmi
 
 using namespace qsmi;
@@ -132,6 +139,8 @@ struct qsmi_ti {
   long long S = 1, N = 1, H = 1;
   int nSub = 0;
   float2* dYhat = nullptr;
+  float2* dYT = nullptr;  // column-contiguous copy (1000 x 1000 fast path)
+  bool fast1k = false;
   float2* dTw[3] = {nullptr, nullptr, nullptr};
   FftPlan plan[3];
   int lognv[3] = {0, 0, 0};
@@ -225,12 +234,16 @@ ScanGeom geom_of(const qsmi_device* dev) {
   return g;
 }
 
-int rows_per_block(const qsmi_ti* t) { return 2 << t->lognv[0]; }
+int rows_per_block(const qsmi_ti* t) { return t->fast1k ? 2 : (2 << t->lognv[0]); }
 
 int row_blocks(const qsmi_ti* t) {
+  if (t->fast1k) return rows1k_pairs(t->s[1]);
   long long rows = (long long)t->s[1] * t->s[2];
   return (int)((rows + rows_per_block(t) - 1) / rows_per_block(t));
 }
+
+// spectrum-stage intermediate per node (float2 elements)
+long long c_stride(const qsmi_ti* t) { return t->fast1k ? std::max<long long>(t->H, ct1k_stride(t->h0)) : t->H; }
 
 // Node preparation on the host: bounding box, valid range, alpha-combined
 // values v_ja = sum_B alpha(A_a, B) u_jB.
@@ -296,7 +309,10 @@ void launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const Node
   qsmi_ti* t = dev->ti;
   hipStream_t st = dev->stream;
   if (dev->timing) hipEventRecord(dev->ev[0], st);
-  if (t->D == 1) {
+  if (t->fast1k) {
+    const int npb = std::max(1, std::min(8, nNodes / 8));
+    launch_pencil1k(g, t->dYT, t->dTw[0], nNodes, npb, info, off, val, C, st);
+  } else if (t->D == 1) {
     launch_direct1d(g, t->dYhat, t->dTw[0], nNodes, info, off, val, C, st);
   } else {
     const int P = t->D - 1;
@@ -310,6 +326,15 @@ void launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const Node
                   t->H, (long long)t->f[1] * t->h0, info, t->f[2], 2, st);
   }
   if (dev->timing) hipEventRecord(dev->ev[2], st);
+}
+
+void launch_rows(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const float2* C, int nNodes, int rowBlocks,
+                 int K, Cand* cand, float* map) {
+  qsmi_ti* t = dev->ti;
+  if (t->fast1k)
+    launch_rows1k(g, t->dTw[0], info, C, nNodes, K, cand, map, dev->stream);
+  else
+    launch_rows_select(g, t->plan[0], t->lognv[0], t->dTw[0], info, C, nNodes, rowBlocks, K, cand, map, dev->stream);
 }
 
 int check_dev(qsmi_device* dev) {
@@ -442,8 +467,14 @@ int qsmi_ti_create(int gpu, int ndim, const unsigned* srcSize, const unsigned* f
                   t->stream);
     TCHK(hipGetLastError());
   }
+  t->fast1k = (ndim == 2 && t->f[0] == 1000 && t->f[1] == 1000);
+  if (t->fast1k) {
+    TCHK(dalloc(&t->dYT, (size_t)nSub * t->H));
+    launch_transpose_spectra(t->dYhat, t->dYT, t->h0, t->f[1], nSub, t->stream);
+    TCHK(hipGetLastError());
+  }
   TCHK(hipStreamSynchronize(t->stream));
-  hipFree(dReal);
+  (void)hipFree(dReal);
 #undef TCHK
   *out = t;
   return QSMI_OK;
@@ -454,7 +485,8 @@ int qsmi_ti_destroy(qsmi_ti* t) {
   if (t->refs > 0) return fail(QSMI_ERR_ARG, "TI still used by a device");
   hipSetDevice(t->gpu);
   if (t->stream) hipStreamSynchronize(t->stream);
-  if (t->dYhat) hipFree(t->dYhat);
+  if (t->dYhat) (void)hipFree(t->dYhat);
+  if (t->dYT) (void)hipFree(t->dYT);
   for (int i = 0; i < 3; ++i)
     if (t->dTw[i]) hipFree(t->dTw[i]);
   if (t->stream) hipStreamDestroy(t->stream);
@@ -565,7 +597,7 @@ int qsmi_candidate_for_pattern(qsmi_device* dev, int nb, const int* offsets, con
   if ((rc = ensure(&dev->dInfo, &dev->capInfo, 1))) return rc;
   if ((rc = ensure(&dev->dOff, &dev->capOff, off.size()))) return rc;
   if ((rc = ensure(&dev->dVal, &dev->capVal, val.size()))) return rc;
-  if ((rc = ensure(&dev->dC, &dev->capC, (size_t)t->H))) return rc;
+  if ((rc = ensure(&dev->dC, &dev->capC, (size_t)c_stride(t)))) return rc;
   hipStream_t st = dev->stream;
   HIPCHK(hipMemcpyAsync(dev->dInfo, &ni, sizeof(ni), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(dev->dOff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, st));
@@ -573,8 +605,7 @@ int qsmi_candidate_for_pattern(qsmi_device* dev, int nb, const int* offsets, con
   launch_fill(dev->dMap, t->S, -INFINITY, st);
   ScanGeom g = geom_of(dev);
   launch_spectrum(dev, g, 1, dev->dInfo, dev->dOff, dev->dVal, dev->dC);
-  launch_rows_select(g, t->plan[0], t->lognv[0], t->dTw[0], dev->dInfo, dev->dC, 1, row_blocks(t), 1, nullptr,
-                     dev->dMap, st);
+  launch_rows(dev, g, dev->dInfo, dev->dC, 1, row_blocks(t), 1, nullptr, dev->dMap);
   HIPCHK(hipGetLastError());
   // the host arrays above are pageable and go out of scope: finish the copies
   HIPCHK(hipStreamSynchronize(st));
@@ -733,11 +764,11 @@ int grow_wave_bufs(qsmi_device* dev, WaveBufs* b, int nNodes, int total, int K) 
     b->capNodes = n;
     b->capCand = n * rb * kMaxK;
   }
-  if ((size_t)nNodes * t->H > b->capC) {
+  if ((size_t)nNodes * c_stride(t) > b->capC) {
     (void)hipFree(b->dC);
     b->dC = nullptr;
     b->capC = 0;
-    size_t n = (size_t)nNodes * t->H;
+    size_t n = (size_t)nNodes * c_stride(t);
     hipError_t e = dalloc(&b->dC, n);
     if (e != hipSuccess) return fail(QSMI_ERR_NOMEM, std::string("wave spectrum buffer: ") + hipGetErrorString(e));
     b->capC = n;
@@ -801,8 +832,7 @@ int run_wave(qsmi_device* dev, WaveBufs* b) {
   hipStream_t st = dev->stream;
   ScanGeom g = geom_of(dev);
   launch_spectrum(dev, g, b->nNodes, b->dInfo, b->dOff, b->dVal, b->dC);
-  launch_rows_select(g, t->plan[0], t->lognv[0], t->dTw[0], b->dInfo, b->dC, b->nNodes, b->rowBlocks, b->K, b->dCand,
-                     nullptr, st);
+  launch_rows(dev, g, b->dInfo, b->dC, b->nNodes, b->rowBlocks, b->K, b->dCand, nullptr);
   if (dev->timing) (void)hipEventRecord(dev->ev[3], st);
   launch_merge_draw(g, b->dInfo, b->dCand, b->nNodes, b->rowBlocks, rows_per_block(t), b->K, b->k, b->dOutVal,
                     b->dOutPos, b->dChosen, st);

@@ -98,6 +98,7 @@ def lib():
     _sig(L, "orc_rfftn_f64", None, [C.c_int, u32p, f64p, f64p])
     _sig(L, "orc_irfftn_f64", None, [C.c_int, u32p, f64p, f64p])
     _sig(L, "orc_convert_neighbors", C.c_float, [C.c_int, C.c_int, i32p, C.c_int, f32p, C.c_int, i32p, i32p, C.c_int, i32p, u32p, f32p, f32p])
+    _sig(L, "orc_index_with_delta", C.c_int, [C.c_int, u32p, C.c_uint, i32p, u32p])
     _sig(L, "orc_sim_create", C.c_void_p, [C.POINTER(OrcSim)])
     _sig(L, "orc_sim_free", None, [C.c_void_p])
     _sig(L, "orc_sim_posterior", None, [C.c_void_p, u32p])

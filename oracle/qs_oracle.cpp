@@ -208,13 +208,14 @@ void orc_kernel_generate(int D, const unsigned* dims, int nVar, const int* ktype
     for (size_t i = 0; i < cells; ++i) {
       float v = 0.f;
       if (ktype[j] == 0) v = weight[j];
+      // the reference calls the global (double) exp/sqrt on float arguments
       if (ktype[j] == 2) {
-        float dist = std::sqrt(dist2_to_center(D, dims, i));
-        v = std::exp(-alphas[j] * dist);
+        float dist = (float)::sqrt((double)dist2_to_center(D, dims, i));
+        v = (float)::exp((double)(-alphas[j] * dist));
       }
       if (ktype[j] == 7) {
         float d2 = dist2_to_center(D, dims, i) * alphas[j] * alphas[j];
-        v = float(1 / std::sqrt(2 * M_PI) * std::exp(-d2 / 2.f));
+        v = (float)(1 / ::sqrt(2 * M_PI) * ::exp((double)(-d2 / 2.f)));
       }
       out[i * nVar + j] = v;
     }
@@ -677,17 +678,33 @@ void orc_irfftn_f64(int D, const unsigned* f, double* specInterleaved, double* o
 // expanded index whenever continuous variables come first), accumulated in
 // float in the reference's loop order.
 // ---------------------------------------------------------------------------
-static bool kernel_index(int D, const unsigned* kdims, const int* d, size_t* kidx) {
-  bool inside = true;
-  size_t idx = 0;
-  for (int i = D - 1; i >= 0; --i) {
-    int c = int(kdims[i] / 2) + d[i];
-    inside &= (c >= 0 && c < int(kdims[i]));
-    c = ((c % int(kdims[i])) + int(kdims[i])) % int(kdims[i]);
-    idx = idx * kdims[i] + (size_t)c;
+// DataImage::indexWithDelta — include/DataImage.hpp:343-376: the cell at
+// `position + delta` with each coordinate wrapped into the grid; returns
+// whether no wrap was needed.
+int orc_index_with_delta(int D, const unsigned* dims, unsigned position, const int* delta, unsigned* location) {
+  int val[3];
+  bool ok = true;
+  for (int i = 0; i < D; ++i) {
+    val[i] = int(position % dims[i]) + delta[i];
+    ok &= (val[i] >= 0) && (val[i] < int(dims[i]));
+    const int dim = int(dims[i]);
+    val[i] = ((val[i] % dim) + dim) % dim;
+    position /= dims[i];
   }
-  *kidx = idx;
-  return inside;
+  unsigned fin = 0;
+  for (int i = D - 1; i >= 0; --i) fin = fin * dims[i] + (unsigned)val[i];
+  *location = fin;
+  return ok ? 1 : 0;
+}
+
+static bool kernel_index(int D, const unsigned* kdims, const int* d, size_t* kidx) {
+  // QSM's kernel centre: dims/2 per dimension (quantileSamplingModule.hpp:362-366)
+  unsigned centre = 0;
+  for (int i = D - 1; i >= 0; --i) centre = centre * kdims[i] + kdims[i] / 2;
+  unsigned loc = 0;
+  int ok = orc_index_with_delta(D, kdims, centre, d, &loc);
+  *kidx = loc;
+  return ok != 0;
 }
 
 float orc_convert_neighbors(int D, int nb, const int* off, int nExp, const float* vals, int nSub,

@@ -1,0 +1,162 @@
+"""Writes tests/golden/ref_vectors.npz from the reference's own header-only
+QS pieces compiled in place (oracle/_ref/libqs_ref.so, built by
+oracle/Makefile from /root/reference). Inputs and outputs only (data):
+
+  size_rule        SharedMemoryManager size rule for 1..4096
+  fkst_*           fKst::findKBiggest with the QSM RNG seeding on tie-heavy
+                   maps with leading -inf runs (values, positions)
+  conv_*           DataImage::convertInput4Xcorr sub-images (reflected f-grid)
+  coef_*           DataImage::generateCoefMatrix4Xcorr matrices / conversions
+  kern_*           DataImage::genearteKernel (uniform / exponential / gaussian)
+  kexp_*           DataImage::offsetKernel4categories
+  iwd_*            DataImage::indexWithDelta
+  cadd_*           g2s::complexAddAlphaxCxD
+
+Run: python tests/golden/make_golden.py  (needs oracle/_ref built).
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+
+OUT = os.path.join(HERE, "ref_vectors.npz")
+
+
+def ti_cases():
+    rng = np.random.default_rng(1234)
+    cases = []
+    # (dims, types, data)
+    d = rng.standard_normal(7 * 5).astype(np.float32)
+    d[3] = np.nan
+    cases.append(((7, 5), [0], d))
+    c = rng.integers(0, 3, 11).astype(np.float32) * 2.5
+    cases.append(((11,), [1], c))
+    m = np.stack([rng.standard_normal(4 * 3 * 5), rng.integers(0, 4, 4 * 3 * 5)], 1).astype(np.float32).reshape(-1)
+    cases.append(((4, 3, 5), [0, 1], m))
+    # categorical before continuous. (Two categorical variables are left out:
+    # convertInput4Xcorr never advances its category index,
+    # include/DataImage.hpp:540-575, so the reference builds the second
+    # variable's images from the first variable's classes.)
+    m2 = np.stack([rng.integers(0, 2, 6 * 4), rng.standard_normal(6 * 4)], 1).astype(np.float32).reshape(-1)
+    cases.append(((6, 4), [1, 0], m2))
+    return cases
+
+
+def main():
+    R = oracle.ref()
+    if R is None:
+        raise SystemExit("oracle/_ref/libqs_ref.so not built (run make -C oracle with the reference present)")
+    out = {}
+    # size rule
+    sizes = np.arange(1, 4097, dtype=np.uint32)
+    fs = np.zeros_like(sizes)
+    for i, n in enumerate(sizes):
+        o = np.zeros(1, np.uint32)
+        R.ref_fft_size(1, oracle.u32([n]), o)
+        fs[i] = o[0]
+    out["size_rule_in"] = sizes
+    out["size_rule_out"] = fs
+    # fKst on tie-heavy maps
+    rng = np.random.default_rng(99)
+    for t in range(40):
+        N = int(rng.integers(1, 400))
+        k = int(rng.integers(1, 9))
+        m = rng.integers(-3, 4, N).astype(np.float32)
+        if t % 3 == 0:
+            m[: rng.integers(0, N + 1)] = -np.inf
+        if t % 5 == 0:
+            m = rng.standard_normal(N).astype(np.float32)
+        seed = np.float32(rng.random()) if t % 7 else np.float32(0.0)
+        v = np.zeros(k, np.float32)
+        p = np.zeros(k, np.uint32)
+        R.ref_fkst(m, N, k, float(seed), v, p)
+        out[f"fkst_{t}_map"] = m
+        out[f"fkst_{t}_k"] = np.int32(k)
+        out[f"fkst_{t}_seed"] = seed
+        out[f"fkst_{t}_val"] = v
+        out[f"fkst_{t}_pos"] = p
+    # convertInput4Xcorr / generateCoefMatrix4Xcorr
+    for ci, (dims, types, data) in enumerate(ti_cases()):
+        dims = oracle.u32(dims)
+        types = oracle.i32(types)
+        nvar = len(types)
+        ncat, catvals = [], []
+        for v in range(nvar):
+            if types[v]:
+                cv = oracle.categories(data, nvar, v)
+                ncat.append(len(cv))
+                catvals.extend(cv.tolist())
+        fd = oracle.u32([oracle.lib().orc_fft_size_1d(int(x) + 3) for x in dims])  # padded f-grid
+        nsub = 2 * int(np.sum(types == 0)) + int(np.sum(ncat))
+        img = np.zeros(nsub * int(np.prod(fd)), np.float32)
+        cnt = R.ref_convert_input4xcorr(len(dims), dims, fd, nvar, types, oracle.f32(data),
+                                        oracle.i32(ncat or [0]), oracle.f32(catvals or [0]), img)
+        assert cnt == nsub
+        alpha = np.zeros(nsub * nsub, np.float32)
+        conv = np.zeros(nsub, np.int32)
+        ctypes_ = np.zeros(nvar, np.int32)
+        ccoef = np.zeros(nvar, np.float32)
+        ns = R.ref_coef_matrix(nvar, types, oracle.i32(ncat or [0]), oracle.f32(catvals or [0]), alpha, conv, ctypes_,
+                               ccoef)
+        assert ns == nsub
+        out[f"conv_{ci}_dims"] = dims
+        out[f"conv_{ci}_fdims"] = fd
+        out[f"conv_{ci}_types"] = types
+        out[f"conv_{ci}_data"] = oracle.f32(data)
+        out[f"conv_{ci}_ncat"] = oracle.i32(ncat or [0])
+        out[f"conv_{ci}_catvals"] = oracle.f32(catvals or [0])
+        out[f"conv_{ci}_images"] = img
+        out[f"coef_{ci}_alpha"] = alpha
+        out[f"coef_{ci}_conv"] = conv
+        out[f"coef_{ci}_consttype"] = ctypes_
+        out[f"coef_{ci}_constcoef"] = ccoef
+    # kernels
+    for ki, (dims, kt, w, a) in enumerate([((5, 7), [0, 2], [1.0, 0.5], [0.0, 0.3]),
+                                          ((9,), [7], [1.0], [0.4]),
+                                          ((3, 4, 5), [0, 7, 2], [2.0, 1.0, 1.0], [0.0, 0.2, 1.1])]):
+        dims = oracle.u32(dims)
+        kern = np.zeros(int(np.prod(dims)) * len(kt), np.float32)
+        R.ref_generate_kernel(len(dims), dims, len(kt), oracle.i32(kt), oracle.f32(w), oracle.f32(a), kern)
+        out[f"kern_{ki}_dims"] = dims
+        out[f"kern_{ki}_types"] = oracle.i32(kt)
+        out[f"kern_{ki}_w"] = oracle.f32(w)
+        out[f"kern_{ki}_a"] = oracle.f32(a)
+        out[f"kern_{ki}_out"] = kern
+        fac = oracle.i32([1 + (i % 3) for i in range(len(kt))])
+        ex = np.zeros(int(np.prod(dims)) * int(fac.sum()), np.float32)
+        R.ref_kernel_expand(len(dims), dims, len(kt), kern, fac, ex)
+        out[f"kexp_{ki}_factor"] = fac
+        out[f"kexp_{ki}_out"] = ex
+    # indexWithDelta
+    rng = np.random.default_rng(5)
+    iw = []
+    for t in range(200):
+        D = int(rng.integers(1, 4))
+        dims = oracle.u32(rng.integers(1, 9, D))
+        pos = int(rng.integers(0, int(np.prod(dims))))
+        delta = oracle.i32(rng.integers(-10, 11, D))
+        loc = np.zeros(1, np.uint32)
+        ok = R.ref_index_with_delta(D, dims, pos, delta, loc)
+        iw.append([D] + list(dims) + [0] * (3 - D) + [pos] + list(delta) + [0] * (3 - D) + [int(loc[0]), ok])
+    out["iwd"] = np.array(iw, np.int64)
+    # complexAddAlphaxCxD
+    rng = np.random.default_rng(6)
+    n = 37
+    dst = rng.standard_normal(2 * n).astype(np.float32)
+    Cm = rng.standard_normal(2 * n).astype(np.float32)
+    Dm = rng.standard_normal(2 * n).astype(np.float32)
+    res = dst.copy()
+    R.ref_complex_add(res, Cm, Dm, np.float32(-1.5), n)
+    out.update(cadd_dst=dst, cadd_C=Cm, cadd_D=Dm, cadd_alpha=np.float32(-1.5), cadd_out=res)
+    np.savez_compressed(OUT, **out)
+    print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes)")
+
+
+if __name__ == "__main__":
+    main()

@@ -200,3 +200,44 @@ def test_scan_wave_matches_per_node(kind, dims):
         assert ch[i] == pos[i][min(r, 1)]
     dev.close()
     ti.close()
+
+
+@pytest.mark.parametrize("kind", ["cont", "cat"])
+def test_fast_path_1000(kind):
+    """The specialised 1000 x 1000 kernels (one FFT-1000 per block) against the
+    FP64 oracle: maps, select, and a wave with neighbourhood spans up to 250."""
+    _log(f"fast path 1000x1000 {kind}")
+    rng = np.random.default_rng(1000)
+    prob = make_problem(kind, (1000, 1000), rng)
+    ti = g2s_amd.TrainingImage(prob.sub, prob.dims)
+    dev = g2s_amd.AcceleratorDevice(ti, prob.alpha)
+    quantum = 0.0
+    if kind == "cat":
+        quantum = float(2.0 ** (np.ceil(np.log2(80)) - 16))
+        dev.set_tie_quantum(quantum)
+    nodes, refs = [], []
+    for i, radius in enumerate((5, 12, 40, 120, 250)):
+        off, x = neighbourhood(prob, rng, 80, radius)
+        gm, om, u, d0, w = check_map(prob, dev, off, x)
+        _log(f"  radius {radius}: map error / s(p) = {w:.3e}")
+        lo, hi = valid_box(prob.dims, off)
+        err, idx = dev.search_k_biggest(2, 0.37)
+        pos = np.array([dev.cvt_index_to_position(int(q)) for q in idx])
+        ov, op, _ = oracle.select_device(prob.dims, gm.astype(np.float32), lo, hi, 2, 0.37, quantum)
+        assert np.array_equal(pos, op.astype(np.int64))
+        seed = float(rng.random())
+        nodes.append((off, u, d0, seed))
+        refs.append((off, x, om, seed))
+    err, pos, ch = dev.scan_wave(nodes, 1.5)
+    for i, (off, x, om, seed) in enumerate(refs):
+        lo, hi = valid_box(prob.dims, off)
+        ov, op, _ = oracle.select_device(prob.dims, om.astype(np.float32), lo, hi, 3, seed, quantum)
+        if list(pos[i]) != list(op[:2]):
+            sc = prob.scale_map(off, x)
+            for a, b in ((0, 1), (1, 2)):
+                if float(om[op[a]] - om[op[b]]) <= MAP_RTOL * float(sc[op[a]] + sc[op[b]]):
+                    break
+            else:
+                raise AssertionError(f"node {i}: {pos[i]} vs oracle {op[:2]}")
+    dev.close()
+    ti.close()
