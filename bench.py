@@ -110,7 +110,8 @@ def main():
     sim = None
     for s in range(args.steps):
         t0 = time.perf_counter()
-        sim, idx, st = g2s_amd.qs(ti, dims, di, sdims, cfg["types"], n, k, seed=cfg["seed"] + s, wave_size=args.wave)
+        sim, idx, st = g2s_amd.qs(ti, dims, di, sdims, cfg["types"], n, k, seed=cfg["seed"] + s, wave_size=args.wave,
+                                  stage_timing=True)
         wall = time.perf_counter() - t0
         times.append(st["path_loop_ms"])
         stats.append(st)
@@ -125,7 +126,7 @@ def main():
     V, S, N = 2, (dims[0] // 2 + 1) * dims[1], dims[0] * dims[1]
     a_node = V * S * 8 + 2 * N * 4
     st = stats[-1]
-    pencil_ms = st.get("pencil_ms", 0.0)
+    pencil_ms = st.get("spectrum_ms", 0.0)
     roof = None
     if pencil_ms > 0:
         achieved = a_node * st["scanned_nodes"] / (pencil_ms / 1e3) / 1e9

@@ -281,7 +281,7 @@ class Wave:
 
 
 def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dims=None, gpu=0,
-            wave_size=256, tie_quantum=-1.0):
+            wave_size=256, tie_quantum=-1.0, stage_timing=False):
     """In-memory QuickSampling (qsmi_qs_run_in_memory).
 
     Arrays are flat in G2S DataImage order (dimension 0 fastest, variable
@@ -312,13 +312,15 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
     rq.seed = int(seed)
     rq.waveSize = int(wave_size)
     rq.tieQuantum = float(tie_quantum)
+    rq.stageTiming = int(bool(stage_timing))
     sim = np.empty_like(di)
     idx = np.zeros(int(np.prod(sim_dims)), np.uint32)
     rs = _lib.QsResult()
     rs.simulation = sim.ctypes.data
     rs.index = idx.ctypes.data
     check(lib().qsmi_qs_run_in_memory(C.byref(rq), C.byref(rs)))
-    stats = dict(setup_ms=rs.setupMs, path_loop_ms=rs.pathLoopMs, scan_ms=rs.scanMs, pencil_ms=rs.stageMs[0],
-                 dim1_ms=rs.stageMs[1], rows_ms=rs.stageMs[2], merge_ms=rs.stageMs[3], nodes=rs.nodes,
-                 scanned_nodes=rs.scannedNodes, waves=rs.waves)
+    stats = dict(setup_ms=rs.setupMs, path_loop_ms=rs.pathLoopMs, scan_ms=rs.scanMs, spectrum_ms=rs.stageMs[0],
+                 gather_ms=rs.stageMs[1], rows_ms=rs.stageMs[2], merge_ms=rs.stageMs[3], nodes=rs.nodes,
+                 scanned_nodes=rs.scannedNodes, waves=rs.waves, search_ms=rs.phaseMs[0], levels_ms=rs.phaseMs[1],
+                 upload_ms=rs.phaseMs[2], device_loop_ms=rs.phaseMs[3])
     return sim, idx, stats

@@ -100,6 +100,7 @@ class QsRequest(C.Structure):
         ("seed", C.c_uint),
         ("waveSize", C.c_int),
         ("tieQuantum", C.c_float),
+        ("stageTiming", C.c_int),
     ]
 
 
@@ -114,4 +115,5 @@ class QsResult(C.Structure):
         ("nodes", C.c_longlong),
         ("scannedNodes", C.c_longlong),
         ("waves", C.c_int),
+        ("phaseMs", C.c_double * 4),
     ]

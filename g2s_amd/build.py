@@ -12,8 +12,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqsmi.so")
 
-SOURCES = ["qsmi_kernels.hip", "qsmi_fast1000.hip", "qsmi_api.hip", "qsmi_host.cpp"]
-HEADERS = ["qsmi_fft.hpp", "qsmi_kernels.hpp"]
+SOURCES = ["qsmi_kernels.hip", "qsmi_fast1000.hip", "qsmi_sim.hip", "qsmi_api.hip", "qsmi_host.cpp"]
+HEADERS = ["qsmi_fft.hpp", "qsmi_kernels.hpp", "qsmi_internal.hpp"]
 
 FLAGS = [
     "-O3",

@@ -11,8 +11,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/qsmi.h"
-#include "qsmi_kernels.hpp"
+#include "qsmi_internal.hpp"
 
 namespace qsmi {
 hipError_t set_lds_limits(int device, size_t* limitOut);
@@ -31,22 +30,20 @@ void launch_rows_select(const ScanGeom& g, const FftPlan& plan0, int lognv, cons
                         const float2* C, int nNodes, int rowBlocks, int K, Cand* cand, float* map, hipStream_t st);
 void launch_select_map(const float* map, long long S, int K, float quantum, uint32_t seedbits, Cand* cand, int chunks,
                        hipStream_t st);
-void launch_merge_draw(const ScanGeom& g, const NodeInfo* info, const Cand* cand, int nNodes, int candBlocks,
-                       int rowsPerBlock, int K, float kf, float* outVal, uint32_t* outPos, uint32_t* chosen,
-                       hipStream_t st);
 void launch_fill(float* p, long long n, float v, hipStream_t st);
 long long ct1k_stride(int h0);
 void launch_transpose_spectra(const float2* Y, float2* YT, int h0, int f1, int nA, hipStream_t st);
 void launch_pencil1k(const ScanGeom& g, const float2* YT, const float2* tw1k, int nNodes, int npb,
                      const NodeInfo* info, const int* nbOff, const float* nbVal, float2* CT, hipStream_t st);
-int rows1k_pairs(int s1);
+int rows1k_tiles();
+hipError_t set_rows1k_lds();
 void launch_rows1k(const ScanGeom& g, const float2* tw1k, const NodeInfo* info, const float2* CT, int nNodes, int K,
                    Cand* cand, float* map, hipStream_t st);
 }  // namespace qsmi
 
 using namespace qsmi;
 
-namespace {
+namespace qsmi {
 
 thread_local std::string g_err;
 
@@ -128,92 +125,10 @@ hipError_t dalloc(T** p, size_t count) {
   return hipMalloc((void**)p, std::max<size_t>(count, 1) * sizeof(T));
 }
 
-}  // namespace
+}  // namespace qsmi
 
-struct qsmi_ti {
-  int gpu = 0;
-  int D = 0;
-  int s[3] = {1, 1, 1};
-  int f[3] = {1, 1, 1};
-  int h0 = 1;
-  long long S = 1, N = 1, H = 1;
-  int nSub = 0;
-  float2* dYhat = nullptr;
-  float2* dYT = nullptr;  // column-contiguous copy (1000 x 1000 fast path)
-  bool fast1k = false;
-  float2* dTw[3] = {nullptr, nullptr, nullptr};
-  FftPlan plan[3];
-  int lognv[3] = {0, 0, 0};
-  hipStream_t stream = nullptr;
-  int refs = 0;
-};
 
-struct WaveBufs;
-
-struct qsmi_device {
-  qsmi_ti* ti = nullptr;
-  hipStream_t stream = nullptr;
-  std::vector<float> alpha;  // nSub x nSub, [A + B*nSub]
-  std::vector<int> active;   // TI sub-variables with a non-zero alpha row
-  int circular = 0;
-  int trueMismatch = 1;
-  float quantum = 0.f;
-  int mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
-  // per-node map state
-  float* dMap = nullptr;
-  bool mapValid = false;
-  // workspace
-  float2* dC = nullptr;
-  size_t capC = 0;
-  Cand* dCand = nullptr;
-  size_t capCand = 0;
-  NodeInfo* dInfo = nullptr;
-  size_t capInfo = 0;
-  int* dOff = nullptr;
-  size_t capOff = 0;
-  float* dVal = nullptr;
-  size_t capVal = 0;
-  float* dOutVal = nullptr;
-  uint32_t* dOutPos = nullptr;
-  uint32_t* dChosen = nullptr;
-  size_t capOut = 0;
-  size_t capOutPos = 0;
-  bool registered = false;
-  // stage timing
-  bool timing = false;
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  float stageMs[4] = {0, 0, 0, 0};
-  WaveBufs* wb = nullptr;  // workspace of qsmi_scan_wave
-};
-
-// Device buffers of one wave (grown on demand) and pinned host staging.
-struct WaveBufs {
-  size_t capNodes = 0, capNb = 0, capC = 0, capCand = 0;
-  NodeInfo* dInfo = nullptr;
-  int* dOff = nullptr;
-  float* dVal = nullptr;
-  float2* dC = nullptr;
-  Cand* dCand = nullptr;
-  float* dOutVal = nullptr;
-  uint32_t* dOutPos = nullptr;
-  uint32_t* dChosen = nullptr;
-  NodeInfo* hInfo = nullptr;  // pinned
-  int* hOff = nullptr;        // pinned
-  float* hVal = nullptr;      // pinned
-  int nNodes = 0, K = 0, rowBlocks = 0;
-  float k = 0.f;
-};
-
-struct qsmi_wave {
-  qsmi_device* dev = nullptr;
-  WaveBufs b;
-};
-
-namespace {
-void free_wave_bufs(WaveBufs* b);
-}
-
-namespace {
+namespace qsmi {
 
 ScanGeom geom_of(const qsmi_device* dev) {
   const qsmi_ti* t = dev->ti;
@@ -234,10 +149,12 @@ ScanGeom geom_of(const qsmi_device* dev) {
   return g;
 }
 
-int rows_per_block(const qsmi_ti* t) { return t->fast1k ? 2 : (2 << t->lognv[0]); }
+// rows per candidate block; 0: the merge reads every block (tiles write -inf
+// candidates when they hold no valid row)
+int rows_per_block(const qsmi_ti* t) { return t->fast1k ? 0 : (2 << t->lognv[0]); }
 
 int row_blocks(const qsmi_ti* t) {
-  if (t->fast1k) return rows1k_pairs(t->s[1]);
+  if (t->fast1k) return rows1k_tiles();
   long long rows = (long long)t->s[1] * t->s[2];
   return (int)((rows + rows_per_block(t) - 1) / rows_per_block(t));
 }
@@ -343,7 +260,7 @@ int check_dev(qsmi_device* dev) {
   return QSMI_OK;
 }
 
-}  // namespace
+}  // namespace qsmi
 
 extern "C" {
 
@@ -469,6 +386,7 @@ int qsmi_ti_create(int gpu, int ndim, const unsigned* srcSize, const unsigned* f
   }
   t->fast1k = (ndim == 2 && t->f[0] == 1000 && t->f[1] == 1000);
   if (t->fast1k) {
+    TCHK(set_rows1k_lds());
     TCHK(dalloc(&t->dYT, (size_t)nSub * t->H));
     launch_transpose_spectra(t->dYhat, t->dYT, t->h0, t->f[1], nSub, t->stream);
     TCHK(hipGetLastError());
@@ -543,6 +461,7 @@ int qsmi_device_destroy(qsmi_device* d) {
   }
   for (int i = 0; i < 5; ++i)
     if (d->ev[i]) (void)hipEventDestroy(d->ev[i]);
+  for (hipEvent_t ev : d->evPool) (void)hipEventDestroy(ev);
   if (d->stream) hipStreamDestroy(d->stream);
   if (d->ti && d->registered) d->ti->refs--;
   delete d;
@@ -636,7 +555,8 @@ int qsmi_search_k_biggest(qsmi_device* dev, float* errors, unsigned* encodedPosi
   HIPCHK(hipMemcpyAsync(dev->dInfo, &ni, sizeof(ni), hipMemcpyHostToDevice, st));
   launch_select_map(dev->dMap, t->S, K, dev->quantum, ni.seedbits, dev->dCand, chunks, st);
   ScanGeom g = geom_of(dev);
-  launch_merge_draw(g, dev->dInfo, dev->dCand, 1, chunks, 0, K, (float)K, dev->dOutVal, dev->dOutPos, nullptr, st);
+  launch_merge_draw(g, dev->dInfo, dev->dCand, 1, chunks, 0, K, (float)K, dev->dOutVal, dev->dOutPos, nullptr,
+                    nullptr, nullptr, 0, nullptr, st);
   HIPCHK(hipGetLastError());
   std::vector<uint32_t> pos(K);
   HIPCHK(hipMemcpyAsync(errors, dev->dOutVal, K * sizeof(float), hipMemcpyDeviceToHost, st));
@@ -718,7 +638,7 @@ int qsmi_set_value_in_error_array(qsmi_device* dev, unsigned index, float value)
 // ---------------------------------------------------------------------------
 }  // extern "C"
 
-namespace {
+namespace qsmi {
 
 void free_wave_bufs(WaveBufs* b) {
   (void)hipFree(b->dInfo);
@@ -835,7 +755,7 @@ int run_wave(qsmi_device* dev, WaveBufs* b) {
   launch_rows(dev, g, b->dInfo, b->dC, b->nNodes, b->rowBlocks, b->K, b->dCand, nullptr);
   if (dev->timing) (void)hipEventRecord(dev->ev[3], st);
   launch_merge_draw(g, b->dInfo, b->dCand, b->nNodes, b->rowBlocks, rows_per_block(t), b->K, b->k, b->dOutVal,
-                    b->dOutPos, b->dChosen, st);
+                    b->dOutPos, b->dChosen, nullptr, nullptr, 0, nullptr, st);
   if (dev->timing) (void)hipEventRecord(dev->ev[4], st);
   HIPCHK(hipGetLastError());
   return QSMI_OK;
@@ -858,7 +778,7 @@ int read_wave(qsmi_device* dev, WaveBufs* b, float* outErrors, unsigned* outPosi
   return QSMI_OK;
 }
 
-}  // namespace
+}  // namespace qsmi
 
 extern "C" {
 

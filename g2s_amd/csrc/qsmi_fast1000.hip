@@ -11,9 +11,11 @@
 //
 // Layouts:
 //   YT   [A][k0][k1]            TI spectra, column-contiguous (k1 fastest)
-//   CT   [node][c1/16][k0][c1%16] the column-inverse intermediate in tiles of
-//                               16 rows, so the pencil kernel writes and the
-//                               rows kernel reads whole 128-byte lines
+//   CT   [node][c1/2][k0][c1%2]  the column-inverse intermediate in row
+//                               pairs: the rows kernel reads one pair as
+//                               contiguous 16-byte loads; the pencil kernel's
+//                               16-byte writes of neighbouring columns land in
+//                               the same lines on one XCD (L2 merges them)
 // Block-to-work mapping is XCD-aware: consecutive block ids go to different
 // XCDs, so block id b works on column slice (b % 8); each XCD's L2 then holds
 // only 1/8 of the TI spectra.
@@ -27,7 +29,7 @@ namespace qsmi {
 namespace f1k {
 
 constexpr int L = 1000;
-constexpr int TR = 16;  // rows per CT tile
+constexpr int TR = 2;  // rows per CT tile (a row pair: one two-for-one c2r)
 constexpr int NXCD = 8;
 
 // butterfly-local twiddle + radix-10 for one lane's butterfly j at stage Ns
@@ -164,46 +166,86 @@ __global__ void __launch_bounds__(128) k_pencil1k(ScanGeom g, const float2* __re
   }
 }
 
-// ---------------------------------------------------------------------------
-// Rows for 1000 x 1000: one 128-thread block per pair of valid rows (two-for-one
-// c2r along dim0), epilogue, mask, wave top-K -> cand[node][pair][K].
-// Block ids are mapped so the 8 pairs of one 16-row tile run on one XCD.
-// ---------------------------------------------------------------------------
+// Block-wide K best of per-thread sorted lists (blockDim multiple of 64),
+// best first into out[0..K). Ends with a barrier.
 template <int KT>
-__device__ __forceinline__ void wave_topk(Cand* c, int K, Cand* out, int lane);
+__device__ void block_topk(Cand* c, int K, Cand* out, Cand* red) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+  for (int it = 0; it < K; ++it) {
+    Cand best = c[0];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      Cand o;
+      o.key = __shfl_xor(best.key, off);
+      o.h = __shfl_xor(best.h, off);
+      o.pos = __shfl_xor(best.pos, off);
+      o.val = __shfl_xor(best.val, off);
+      if (cand_better(o, best)) best = o;
+    }
+    if (lane == 0) red[wave] = best;
+    __syncthreads();
+    Cand w = red[0];
+    for (int i = 1; i < nwaves; ++i)
+      if (cand_better(red[i], w)) w = red[i];
+    if (tid == 0) out[it] = w;
+    if (c[0].pos == w.pos && c[0].pos != 0xFFFFFFFFu) {
+#pragma unroll
+      for (int i = 0; i < KT - 1; ++i) c[i] = c[i + 1];
+      c[KT - 1].key = -INFINITY;
+      c[KT - 1].h = 0u;
+      c[KT - 1].pos = 0xFFFFFFFFu;
+      c[KT - 1].val = -INFINITY;
+    }
+    __syncthreads();
+  }
+}
 
+// ---------------------------------------------------------------------------
+// Rows for 1000 x 1000: one 128-thread block per row pair (2p, 2p+1) of one
+// node (two-for-one c2r along dim0); thread j loads Z[j + 100 q] of both rows
+// as one 16-byte load; epilogue, mask, block top-K -> cand[node][p][K].
+// Pairs without a valid row write -inf candidates.
+// ---------------------------------------------------------------------------
 template <int KT>
-__global__ void __launch_bounds__(128) k_rows1k(ScanGeom g, const float2* __restrict__ tw1k,
-                                               const NodeInfo* __restrict__ info, const float2* __restrict__ CT,
-                                               long long ctStride, int K, Cand* __restrict__ cand, int candPairs,
-                                               float* __restrict__ map) {
+__global__ void __launch_bounds__(128) k_rows1k_pair(ScanGeom g, const float2* __restrict__ tw1k,
+                                                     const NodeInfo* __restrict__ info,
+                                                     const float2* __restrict__ CT, long long ctStride, int K,
+                                                     Cand* __restrict__ cand, int candPairs,
+                                                     float* __restrict__ map) {
   using namespace f1k;
   __shared__ float2 buf[L];
-  const int lane = threadIdx.x;
+  __shared__ Cand red[2];
+  const int j = threadIdx.x;
   const int node = blockIdx.y;
+  const int p = blockIdx.x;
   const NodeInfo ni = info[node];
-  const int nrv = ni.hi[1] - ni.lo[1] + 1;
-  if (nrv <= 0 || ni.hi[0] < ni.lo[0]) return;
-  const int npairs = (nrv + 1) / 2;
-  // bid = 64 m + 8 w + x  ->  pair p = 64 m + 8 x + w
-  const int bid = blockIdx.x;
-  const int p = (bid / 64) * 64 + (bid % NXCD) * 8 + (bid / NXCD) % 8;
-  if (p >= npairs) return;
   const int h0 = g.h0;
-  const int c1a = ni.lo[1] + 2 * p;
-  const bool hasB = (c1a + 1 <= ni.hi[1]);
-  const float2* Cn = CT + (long long)node * ctStride;
-  const bool act = lane < 100;
+  Cand* out = cand ? cand + ((long long)node * candPairs + p) * K : nullptr;
+  const int ca = 2 * p;
+  const bool va = (ca >= ni.lo[1] && ca <= ni.hi[1]), vb = (ca + 1 >= ni.lo[1] && ca + 1 <= ni.hi[1]);
+  if ((!va && !vb) || ni.lo[0] > ni.hi[0]) {
+    if (out && j < K) {
+      Cand w;
+      w.key = -INFINITY;
+      w.h = 0u;
+      w.pos = 0xFFFFFFFFu;
+      w.val = -INFINITY;
+      out[j] = w;
+    }
+    return;
+  }
+  const float4* rowp = reinterpret_cast<const float4*>(CT + (long long)node * ctStride + (long long)p * h0 * TR);
+  const bool act = j < 100;
   float2 u[10];
   if (act) {
 #pragma unroll
     for (int q = 0; q < 10; ++q) {
-      const int k = lane + 100 * q;
+      const int k = j + 100 * q;
       const bool cj = (k >= h0);
       const int kk = cj ? (L - k) : k;
-      float2 A = Cn[((long long)(c1a / TR) * h0 + kk) * TR + (c1a % TR)];
-      float2 B = make_float2(0.f, 0.f);
-      if (hasB) B = Cn[((long long)((c1a + 1) / TR) * h0 + kk) * TR + ((c1a + 1) % TR)];
+      const float4 ab = rowp[kk];  // (row 2p, row 2p+1) at column kk
+      float2 A = va ? make_float2(ab.x, ab.y) : make_float2(0.f, 0.f);
+      float2 B = vb ? make_float2(ab.z, ab.w) : make_float2(0.f, 0.f);
       if (kk == 0 || 2 * kk == L) {  // a Hermitian c2r ignores these imaginary parts
         A.y = 0.f;
         B.y = 0.f;
@@ -215,7 +257,7 @@ __global__ void __launch_bounds__(128) k_rows1k(ScanGeom g, const float2* __rest
       u[q] = make_float2(A.x - B.y, A.y + B.x);
     }
   }
-  fft1000<+1>(buf, u, tw1k, lane, false);
+  fft1000<+1>(buf, u, tw1k, j, false);
   Cand c[KT];
 #pragma unroll
   for (int i = 0; i < KT; ++i) {
@@ -229,15 +271,15 @@ __global__ void __launch_bounds__(128) k_rows1k(ScanGeom g, const float2* __rest
   if (act) {
 #pragma unroll
     for (int q = 0; q < 10; ++q) {
-      const int c0 = lane + 100 * q;
+      const int c0 = j + 100 * q;
       if (c0 < ni.lo[0] || c0 > ni.hi[0]) continue;
 #pragma unroll
       for (int w = 0; w < 2; ++w) {
-        if (w == 1 && !hasB) continue;
+        if (w == 0 ? !va : !vb) continue;
         const float val = (w == 0 ? u[q].x : u[q].y) * scale + d0;
-        const uint32_t pos = (uint32_t)(c0 + s0 * (c1a + w));
+        const uint32_t pos = (uint32_t)(c0 + s0 * (ca + w));
         if (map) map[(long long)node * g.s[0] * g.s[1] + pos] = val;
-        if (cand) {
+        if (out) {
           const float key = (quantum > 0.f) ? rintf(val * (1.f / quantum)) : val;
           if (!(key >= c[KT - 1].key)) continue;
           Cand cd;
@@ -249,48 +291,16 @@ __global__ void __launch_bounds__(128) k_rows1k(ScanGeom g, const float2* __rest
 #pragma unroll
           for (int i = 0; i < KT; ++i) {
             if (cand_better(cd, c[i])) {
-              Cand t = c[i];
+              Cand tt = c[i];
               c[i] = cd;
-              cd = t;
+              cd = tt;
             }
           }
         }
       }
     }
   }
-  if (cand) wave_topk<KT>(c, K, cand + ((long long)node * candPairs + p) * K, lane);
-}
-
-// Block-wide K best of per-thread sorted lists over the two wavefronts of a
-// 128-thread block (idle threads hold -inf).
-template <int KT>
-__device__ __forceinline__ void wave_topk(Cand* c, int K, Cand* out, int lane) {
-  __shared__ Cand red[2];
-  for (int it = 0; it < K; ++it) {
-    Cand best = c[0];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      Cand o;
-      o.key = __shfl_xor(best.key, off);
-      o.h = __shfl_xor(best.h, off);
-      o.pos = __shfl_xor(best.pos, off);
-      o.val = __shfl_xor(best.val, off);
-      if (cand_better(o, best)) best = o;
-    }
-    if ((lane & 63) == 0) red[lane >> 6] = best;
-    __syncthreads();
-    Cand w = cand_better(red[1], red[0]) ? red[1] : red[0];
-    if (lane == 0) out[it] = w;
-    if (c[0].pos == w.pos && c[0].pos != 0xFFFFFFFFu) {
-#pragma unroll
-      for (int i = 0; i < KT - 1; ++i) c[i] = c[i + 1];
-      c[KT - 1].key = -INFINITY;
-      c[KT - 1].h = 0u;
-      c[KT - 1].pos = 0xFFFFFFFFu;
-      c[KT - 1].val = -INFINITY;
-    }
-    __syncthreads();
-  }
+  if (out) block_topk<KT>(c, K, out, red);
 }
 
 // YT[A][k0][k1] <- Yhat[A][k1][k0] (once per TI)
@@ -330,22 +340,23 @@ void launch_pencil1k(const ScanGeom& g, const float2* YT, const float2* tw1k, in
                      CT, ct1k_stride(g.h0));
 }
 
-int rows1k_pairs(int s1) { return (s1 + 1) / 2; }
+int rows1k_tiles() { return (1000 + f1k::TR - 1) / f1k::TR; }
+
+hipError_t set_rows1k_lds() { return hipSuccess; }
 
 void launch_rows1k(const ScanGeom& g, const float2* tw1k, const NodeInfo* info, const float2* CT, int nNodes, int K,
                    Cand* cand, float* map, hipStream_t st) {
-  const int pairs = rows1k_pairs(g.s[1]);
-  const int blocks = ((pairs + 63) / 64) * 64;  // whole groups of 64 for the XCD mapping
-  dim3 grid((unsigned)blocks, (unsigned)nNodes);
+  const int pairs = rows1k_tiles();
+  dim3 grid((unsigned)pairs, (unsigned)nNodes);
   const long long cs = ct1k_stride(g.h0);
   if (K <= 2)
-    hipLaunchKernelGGL(k_rows1k<2>, grid, dim3(128), 0, st, g, tw1k, info, CT, cs, K, cand, pairs, map);
+    hipLaunchKernelGGL(k_rows1k_pair<2>, grid, dim3(128), 0, st, g, tw1k, info, CT, cs, K, cand, pairs, map);
   else if (K <= 4)
-    hipLaunchKernelGGL(k_rows1k<4>, grid, dim3(128), 0, st, g, tw1k, info, CT, cs, K, cand, pairs, map);
+    hipLaunchKernelGGL(k_rows1k_pair<4>, grid, dim3(128), 0, st, g, tw1k, info, CT, cs, K, cand, pairs, map);
   else if (K <= 8)
-    hipLaunchKernelGGL(k_rows1k<8>, grid, dim3(128), 0, st, g, tw1k, info, CT, cs, K, cand, pairs, map);
+    hipLaunchKernelGGL(k_rows1k_pair<8>, grid, dim3(128), 0, st, g, tw1k, info, CT, cs, K, cand, pairs, map);
   else
-    hipLaunchKernelGGL(k_rows1k<16>, grid, dim3(128), 0, st, g, tw1k, info, CT, cs, K, cand, pairs, map);
+    hipLaunchKernelGGL(k_rows1k_pair<16>, grid, dim3(128), 0, st, g, tw1k, info, CT, cs, K, cand, pairs, map);
 }
 
 }  // namespace qsmi

@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "../../include/qsmi.h"
+#include "qsmi_internal.hpp"
 
 namespace {
 
@@ -325,7 +326,6 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     }
   }
   qsmi_set_tie_quantum(dev, quantum);
-  qsmi_set_stage_timing(dev, 1);
   // output state
   float* di = rs->simulation;
   std::memcpy(di, rq->di, SD * nVar * sizeof(float));
@@ -385,6 +385,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     for (unsigned q = 0; q < hw; ++q) th.emplace_back(worker);
     for (auto& t : th) t.join();
   }
+  const auto tNb = clk::now();
   // dependency levels
   std::vector<int> level(nPath, 0);
   int maxLevel = 0;
@@ -421,130 +422,102 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     for (size_t i = 0; i < nPath; ++i) order[fillp[level[i]]++] = (unsigned)i;
   }
   const int waveSize = rq->waveSize > 0 ? rq->waveSize : 256;
-  // kernel centre for the weights (QSM uses dims/2, quantileSamplingModule.hpp:362-366)
-  int kctr[3] = {0, 0, 0};
-  for (int i = 0; i < D; ++i) kctr[i] = (int)ker.dims[i] / 2;
-  std::vector<int> wStart, wOff;
-  std::vector<float> wVal, wDelta, wSeed, wErr;
-  std::vector<unsigned> wPos, wChosen, wNode;
-  std::vector<float> xAll;
-  std::vector<size_t> kcAll;
-  std::vector<unsigned char> inAll;
-  double scanMs = 0.0;
-  double stageMs[4] = {0, 0, 0, 0};
-  long long scanned = 0;
-  int waves = 0;
-  const float kf = rq->k;
-  const int K = (int)std::ceil(kf);
-  const unsigned SI = (unsigned)S;
-  for (int l = 0; l <= maxLevel && rc == 0; ++l) {
-    for (size_t w0 = lvStart[l]; w0 < lvStart[l + 1] && rc == 0; w0 += waveSize) {
-      const size_t w1 = std::min(lvStart[l + 1], w0 + waveSize);
-      wStart.assign(1, 0);
-      wOff.clear();
-      wVal.clear();
-      wDelta.clear();
-      wSeed.clear();
-      wNode.clear();
-      for (size_t q = w0; q < w1; ++q) {
-        const unsigned i = order[q];
-        const unsigned cell = path[begin + i];
-        const float seed = seeds[begin + i];
-        const int cnt = nbCnt[i];
-        if (cnt <= 1) {
-          // only the centre: draw from the marginal (simulation.hpp:586-606)
-          const unsigned position = (unsigned)(int)std::floor(seed * (float)SI);
-          rs->index[cell] = position;
-          for (int v = 0; v < nVar; ++v) di[(size_t)cell * nVar + v] = rq->ti[(size_t)position * nVar + v];
-          continue;
+  const auto tLv = clk::now();
+  // waves: each level cut into chunks of at most waveSize nodes
+  qsmi::ResidentLoop R;
+  R.waveStart.push_back(0);
+  for (int l = 0; l <= maxLevel; ++l)
+    for (size_t w0 = lvStart[l]; w0 < lvStart[l + 1]; w0 += waveSize)
+      R.waveStart.push_back((int)std::min(lvStart[l + 1], w0 + waveSize));
+  for (int d = 0; d < 3; ++d) {
+    R.sd[d] = (int)sd[d];
+    R.kd[d] = (int)ker.dims[d];
+  }
+  R.nVar = nVar;
+  R.nExp = nExp;
+  R.n = n;
+  R.k = rq->k;
+  for (int v = 0; v < nVar && v < 8; ++v) R.types[v] = rq->varTypes[v];
+  {
+    int e = 0, ci = 0, off = 0;
+    for (int v = 0; v < nVar; ++v) {
+      if (rq->varTypes[v] == 0) {
+        R.expVar[e] = v;
+        R.expCat[e] = -1;
+        e++;
+      } else {
+        for (int q = 0; q < ncat[ci]; ++q, ++e) {
+          R.expVar[e] = v;
+          R.expCat[e] = off + q;
         }
-        int cc[3] = {0, 0, 0};
-        size_t rem = cell;
-        for (int d = 0; d < D; ++d) {
-          cc[d] = (int)(rem % sd[d]);
-          rem /= sd[d];
-        }
-        float delta0 = 0.f;
-        const int* nb = nbIdx.data() + (size_t)i * n;
-        xAll.resize((size_t)cnt * nExp);
-        kcAll.resize(cnt);
-        inAll.resize(cnt);
-        for (int j = 0; j < cnt; ++j) {
-          const int* dd = offs.data() + (size_t)nb[j] * D;
-          size_t t = 0, mul = 1;
-          for (int d = 0; d < D; ++d) {
-            t += (size_t)(cc[d] + dd[d]) * mul;
-            mul *= sd[d];
-            wOff.push_back(dd[d]);
-          }
-          // one-hot expansion (simulation.hpp:512-537)
-          float* xe = xAll.data() + (size_t)j * nExp;
-          int e = 0, ci = 0;
-          const float* cv = catvals.data();
-          for (int v = 0; v < nVar; ++v) {
-            const float x = di[t * nVar + v];
-            if (rq->varTypes[v] == 0) {
-              xe[e++] = x;
-            } else {
-              for (int q2 = 0; q2 < ncat[ci]; ++q2) xe[e++] = (x == cv[q2]) ? 1.f : 0.f;
-              cv += ncat[ci];
-              ci++;
-            }
-          }
-          // QSM conversion (quantileSamplingModule.hpp:368-410)
-          size_t kc;
-          inAll[j] = kernel_cell(ker, kctr, dd, &kc) ? 1 : 0;
-          kcAll[j] = kc;
-          for (int B = 0; B < nSub; ++B) {
-            const int ev = subVarOf[B];
-            const float x = xe[ev];
-            const float w = ker.data[kc * nExp + (ev % nExp)];
-            float r = 0.f;
-            if (inAll[j] && !std::isnan(x)) r = convTypes[B] == 0 ? w * 1.f : w * x;
-            wVal.push_back(r);
-          }
-        }
-        // delta0 = -sum w x^2 per original continuous variable v, read at
-        // expanded index v as the reference does (:412-457)
-        for (int v = 0; v < nVar; ++v) {
-          if (rq->varTypes[v] != 0) continue;
-          for (int j = 0; j < cnt; ++j) {
-            const float xv = xAll[(size_t)j * nExp + v];
-            if (inAll[j] && !std::isnan(xv)) delta0 += -1.f * (ker.data[kcAll[j] * nExp + (v % nExp)] * xv * xv);
-          }
-        }
-        wStart.push_back(wStart.back() + cnt);
-        wDelta.push_back(delta0);
-        wSeed.push_back(seed);
-        wNode.push_back(i);
-      }
-      const int nN = (int)wNode.size();
-      if (nN == 0) continue;
-      wErr.resize((size_t)nN * K);
-      wPos.resize((size_t)nN * K);
-      wChosen.resize(nN);
-      rc = qsmi_scan_wave(dev, nN, wStart.data(), wOff.data(), wVal.data(), wDelta.data(), wSeed.data(), kf,
-                          wErr.data(), wPos.data(), wChosen.data());
-      if (rc) break;
-      float ms[4];
-      qsmi_last_stage_ms(dev, ms);
-      scanMs += ms[0] + ms[1] + ms[2] + ms[3];
-      for (int q = 0; q < 4; ++q) stageMs[q] += ms[q];
-      scanned += nN;
-      waves++;
-      for (int q = 0; q < nN; ++q) {
-        const unsigned i = wNode[q];
-        const unsigned cell = path[begin + i];
-        const unsigned position = wChosen[q];
-        if (position == UINT_MAX || position >= SI) {
-          rc = hfail(QSMI_ERR_UNSUPPORTED, "a node has no valid TI position (neighbourhood wider than the TI)");
-          break;
-        }
-        rs->index[cell] = position;
-        for (int v = 0; v < nVar; ++v) di[(size_t)cell * nVar + v] = rq->ti[(size_t)position * nVar + v];
+        off += ncat[ci];
+        ci++;
       }
     }
   }
+  for (int B = 0; B < nSub; ++B) {
+    R.conv[B] = convTypes[B];
+    R.subOf[B] = subVarOf[B];
+  }
+  R.stageTiming = rq->stageTiming != 0;
+  // device-resident state
+  std::vector<void*> allocs;
+  auto dev_upload = [&](const void* src, size_t bytes, void** dst) -> int {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 4));
+    if (e != hipSuccess) return hfail(QSMI_ERR_NOMEM, std::string("path loop state: ") + hipGetErrorString(e));
+    allocs.push_back(p);
+    if (src && bytes) {
+      e = hipMemcpy(p, src, bytes, hipMemcpyHostToDevice);
+      if (e != hipSuccess) return hfail(QSMI_ERR_HIP, std::string("path loop upload: ") + hipGetErrorString(e));
+    }
+    *dst = p;
+    return QSMI_OK;
+  };
+  std::vector<int> orderI(order.begin(), order.end());
+  void *dOrder, *dCells, *dSeeds, *dNb, *dCnt, *dOffs, *dKer, *dCat, *dDi, *dIdx, *dTi;
+  std::vector<float> catv = catvals.empty() ? std::vector<float>(1, 0.f) : catvals;
+  if (!rc) rc = dev_upload(orderI.data(), nPath * sizeof(int), &dOrder);
+  if (!rc) rc = dev_upload(path.data() + begin, nPath * sizeof(unsigned), &dCells);
+  if (!rc) rc = dev_upload(seeds.data() + begin, nPath * sizeof(float), &dSeeds);
+  if (!rc) rc = dev_upload(nbIdx.data(), nbIdx.size() * sizeof(int), &dNb);
+  if (!rc) rc = dev_upload(nbCnt.data(), nbCnt.size() * sizeof(int), &dCnt);
+  if (!rc) rc = dev_upload(offs.data(), (size_t)nOff * D * sizeof(int), &dOffs);
+  if (!rc) rc = dev_upload(ker.data.data(), ker.data.size() * sizeof(float), &dKer);
+  if (!rc) rc = dev_upload(catv.data(), catv.size() * sizeof(float), &dCat);
+  if (!rc) rc = dev_upload(di, SD * nVar * sizeof(float), &dDi);
+  if (!rc) rc = dev_upload(rs->index, SD * sizeof(unsigned), &dIdx);
+  if (!rc) rc = dev_upload(rq->ti, S * nVar * sizeof(float), &dTi);
+  const auto tUp = clk::now();
+  double stageMs[4] = {0, 0, 0, 0};
+  long long scanned = 0;
+  int waves = 0;
+  if (!rc) {
+    R.dOrder = (const int*)dOrder;
+    R.dPathCells = (const unsigned*)dCells;
+    R.dPathSeeds = (const float*)dSeeds;
+    R.dNbIdx = (const int*)dNb;
+    R.dNbCnt = (const int*)dCnt;
+    R.dOffs = (const int*)dOffs;
+    R.dKernel = (const float*)dKer;
+    R.dCatvals = (const float*)dCat;
+    R.dDi = (float*)dDi;
+    R.dIndex = (uint32_t*)dIdx;
+    R.dTi = (const float*)dTi;
+    rc = qsmi::run_resident_waves(dev, R, stageMs, &scanned, &waves);
+  }
+  if (!rc) {
+    hipError_t e = hipMemcpy(di, dDi, SD * nVar * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(rs->index, dIdx, SD * sizeof(unsigned), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hfail(QSMI_ERR_HIP, std::string("path loop download: ") + hipGetErrorString(e));
+  }
+  for (void* p : allocs) (void)hipFree(p);
+  const auto tLoop = clk::now();
+  double scanMs = stageMs[0] + stageMs[1] + stageMs[2] + stageMs[3];
+  rs->phaseMs[0] = std::chrono::duration<double, std::milli>(tNb - t1).count();
+  rs->phaseMs[1] = std::chrono::duration<double, std::milli>(tLv - tNb).count();
+  rs->phaseMs[2] = std::chrono::duration<double, std::milli>(tUp - tLv).count();
+  rs->phaseMs[3] = std::chrono::duration<double, std::milli>(tLoop - tUp).count();
   const auto t2 = clk::now();
   qsmi_device_destroy(dev);
   qsmi_ti_destroy(ti);

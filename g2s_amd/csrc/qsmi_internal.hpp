@@ -1,0 +1,151 @@
+// Private header of the qsmi library: device-side state behind the C-ABI
+// handles and the internal entry points shared by the library's translation
+// units. Not installed; include/qsmi.h is the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/qsmi.h"
+#include "qsmi_kernels.hpp"
+
+using qsmi::Cand;
+using qsmi::FftPlan;
+using qsmi::NodeInfo;
+using qsmi::ScanGeom;
+
+struct qsmi_ti {
+  int gpu = 0;
+  int D = 0;
+  int s[3] = {1, 1, 1};
+  int f[3] = {1, 1, 1};
+  int h0 = 1;
+  long long S = 1, N = 1, H = 1;
+  int nSub = 0;
+  float2* dYhat = nullptr;
+  float2* dYT = nullptr;  // column-contiguous copy (1000 x 1000 fast path)
+  bool fast1k = false;
+  float2* dTw[3] = {nullptr, nullptr, nullptr};
+  FftPlan plan[3];
+  int lognv[3] = {0, 0, 0};
+  hipStream_t stream = nullptr;
+  int refs = 0;
+};
+
+struct WaveBufs;
+
+struct qsmi_device {
+  qsmi_ti* ti = nullptr;
+  hipStream_t stream = nullptr;
+  std::vector<float> alpha;  // nSub x nSub, [A + B*nSub]
+  std::vector<int> active;   // TI sub-variables with a non-zero alpha row
+  int circular = 0;
+  int trueMismatch = 1;
+  float quantum = 0.f;
+  int mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+  // per-node map state
+  float* dMap = nullptr;
+  bool mapValid = false;
+  // workspace
+  float2* dC = nullptr;
+  size_t capC = 0;
+  Cand* dCand = nullptr;
+  size_t capCand = 0;
+  NodeInfo* dInfo = nullptr;
+  size_t capInfo = 0;
+  int* dOff = nullptr;
+  size_t capOff = 0;
+  float* dVal = nullptr;
+  size_t capVal = 0;
+  float* dOutVal = nullptr;
+  uint32_t* dOutPos = nullptr;
+  uint32_t* dChosen = nullptr;
+  size_t capOut = 0;
+  size_t capOutPos = 0;
+  bool registered = false;
+  // stage timing
+  bool timing = false;
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  float stageMs[4] = {0, 0, 0, 0};
+  WaveBufs* wb = nullptr;  // workspace of qsmi_scan_wave
+  std::vector<hipEvent_t> evPool;  // per-wave stage events of the resident loop
+};
+
+// Device buffers of one wave (grown on demand) and pinned host staging.
+struct WaveBufs {
+  size_t capNodes = 0, capNb = 0, capC = 0, capCand = 0;
+  NodeInfo* dInfo = nullptr;
+  int* dOff = nullptr;
+  float* dVal = nullptr;
+  float2* dC = nullptr;
+  Cand* dCand = nullptr;
+  float* dOutVal = nullptr;
+  uint32_t* dOutPos = nullptr;
+  uint32_t* dChosen = nullptr;
+  NodeInfo* hInfo = nullptr;  // pinned
+  int* hOff = nullptr;        // pinned
+  float* hVal = nullptr;      // pinned
+  int nNodes = 0, K = 0, rowBlocks = 0;
+  float k = 0.f;
+};
+
+struct qsmi_wave {
+  qsmi_device* dev = nullptr;
+  WaveBufs b;
+};
+
+
+namespace qsmi {
+
+// error channel (thread-local message + code)
+int fail(int code, const std::string& msg);
+unsigned seed_from_unit_float(float seed);
+
+ScanGeom geom_of(const qsmi_device* dev);
+int rows_per_block(const qsmi_ti* t);
+int row_blocks(const qsmi_ti* t);
+long long c_stride(const qsmi_ti* t);
+int grow_wave_bufs(qsmi_device* dev, WaveBufs* b, int nNodes, int total, int K);
+void free_wave_bufs(WaveBufs* b);
+// spectrum stage (+ dim1 inverse for 3-D) and rows stage of nNodes nodes
+void launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeInfo* info, const int* off,
+                     const float* val, float2* C);
+void launch_rows(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const float2* C, int nNodes,
+                 int rowBlocks, int K, Cand* cand, float* map);
+// merge + draw; when simVals != nullptr also writes the drawn TI values and
+// the index image of each node's cell (NodeInfo::cell)
+void launch_merge_draw(const ScanGeom& g, const NodeInfo* info, const Cand* cand, int nNodes, int candBlocks,
+                       int rowsPerBlock, int K, float kf, float* outVal, uint32_t* outPos, uint32_t* chosen,
+                       float* simVals, const float* tiVals, int nVar, uint32_t* indexOut, hipStream_t st);
+
+// Inputs of the device-resident path loop (qsmi_sim.hip). Device pointers
+// are indexed by path position after the hard data (i in [0, nPath)).
+struct ResidentLoop {
+  int sd[3];      // simulation grid
+  int kd[3];      // expanded kernel grid
+  int nVar, nExp, n;
+  float k;
+  int types[8];   // per original variable
+  int expVar[16], expCat[16];
+  int conv[16], subOf[16];
+  std::vector<int> waveStart;  // wave w = order[waveStart[w] .. waveStart[w+1])
+  bool stageTiming = false;
+  const int* dOrder;
+  const unsigned* dPathCells;
+  const float* dPathSeeds;
+  const int* dNbIdx;  // nPath x n offset-list indices
+  const int* dNbCnt;
+  const int* dOffs;
+  const float* dKernel;
+  const float* dCatvals;
+  float* dDi;
+  uint32_t* dIndex;
+  const float* dTi;
+};
+
+// stageMs: [0] spectrum (+ dim1 inverse), [1] gather, [2] rows + select,
+// [3] merge + draw + write; filled when R.stageTiming.
+int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMs, long long* scanned, int* waves);
+
+}  // namespace qsmi

@@ -445,11 +445,14 @@ template <int KT>
 __global__ void __launch_bounds__(256) k_merge_draw(ScanGeom g, const NodeInfo* __restrict__ info,
                                                     const Cand* __restrict__ cand, int candBlocks,
                                                     int rowsPerBlock, int K, float kf, float* __restrict__ outVal,
-                                                    uint32_t* __restrict__ outPos, uint32_t* __restrict__ chosen) {
+                                                    uint32_t* __restrict__ outPos, uint32_t* __restrict__ chosen,
+                                                    float* __restrict__ simVals, const float* __restrict__ tiVals,
+                                                    int nVar, uint32_t* __restrict__ indexOut) {
   __shared__ Cand red[16];
   __shared__ Cand best[kMaxK];
   const int node = blockIdx.x;
   const NodeInfo ni = info[node];
+  if (simVals && ni.nb == 0) return;  // drawn from the marginal by the gather stage
   int nblk = candBlocks;
   if (rowsPerBlock > 0) {
     const int n1v = ni.hi[1] - ni.lo[1] + 1;
@@ -467,12 +470,19 @@ __global__ void __launch_bounds__(256) k_merge_draw(ScanGeom g, const NodeInfo* 
     outVal[(long long)node * K + threadIdx.x] = best[threadIdx.x].val;
     outPos[(long long)node * K + threadIdx.x] = best[threadIdx.x].pos;
   }
-  if (threadIdx.x == 0 && chosen) {
+  if (threadIdx.x == 0 && (chosen || simVals)) {
     // QuantileSamplingModule::sample rank, one TI: floor(seed * k * (ceil(1/k)/1))
     int r = (int)floorf(ni.seed * kf * (ceilf(1.f / kf) / 1.f));
     if (r > K - 1) r = K - 1;
     if (r < 0) r = 0;
-    chosen[node] = best[r].pos;
+    const uint32_t p = best[r].pos;
+    if (chosen) chosen[node] = p;
+    // simulation write (include/simulation.hpp:658-666): the TI value of the
+    // chosen position and importDataIndex = index * nTI + TI (one TI)
+    if (simVals && p != 0xFFFFFFFFu) {
+      for (int v = 0; v < nVar; ++v) simVals[(long long)ni.cell * nVar + v] = tiVals[(long long)p * nVar + v];
+      indexOut[ni.cell] = p;
+    }
   }
 }
 
@@ -491,7 +501,7 @@ template __global__ void k_colfft<+1>(float2*, FftPlan, const float2*, int, long
                                              const float2*, int, Cand*, int, float*);                             \
   template __global__ void k_select_map<KT>(const float*, long long, int, float, uint32_t, Cand*);                \
   template __global__ void k_merge_draw<KT>(ScanGeom, const NodeInfo*, const Cand*, int, int, int, float, float*, \
-                                            uint32_t*, uint32_t*);
+                                            uint32_t*, uint32_t*, float*, const float*, int, uint32_t*);
 QSMI_INST_K(2)
 QSMI_INST_K(4)
 QSMI_INST_K(8)
@@ -590,13 +600,17 @@ void launch_select_map(const float* map, long long S, int K, float quantum, uint
 
 void launch_merge_draw(const ScanGeom& g, const NodeInfo* info, const Cand* cand, int nNodes, int candBlocks,
                        int rowsPerBlock, int K, float kf, float* outVal, uint32_t* outPos, uint32_t* chosen,
-                       hipStream_t st) {
+                       float* simVals, const float* tiVals, int nVar, uint32_t* indexOut, hipStream_t st) {
+#define QSMI_MERGE(KT)                                                                                              \
+  hipLaunchKernelGGL(k_merge_draw<KT>, dim3(nNodes), dim3(256), 0, st, g, info, cand, candBlocks, rowsPerBlock, K, \
+                     kf, outVal, outPos, chosen, simVals, tiVals, nVar, indexOut)
   switch (kt_for(K)) {
-    case 2: hipLaunchKernelGGL(k_merge_draw<2>, dim3(nNodes), dim3(256), 0, st, g, info, cand, candBlocks, rowsPerBlock, K, kf, outVal, outPos, chosen); break;
-    case 4: hipLaunchKernelGGL(k_merge_draw<4>, dim3(nNodes), dim3(256), 0, st, g, info, cand, candBlocks, rowsPerBlock, K, kf, outVal, outPos, chosen); break;
-    case 8: hipLaunchKernelGGL(k_merge_draw<8>, dim3(nNodes), dim3(256), 0, st, g, info, cand, candBlocks, rowsPerBlock, K, kf, outVal, outPos, chosen); break;
-    default: hipLaunchKernelGGL(k_merge_draw<16>, dim3(nNodes), dim3(256), 0, st, g, info, cand, candBlocks, rowsPerBlock, K, kf, outVal, outPos, chosen); break;
+    case 2: QSMI_MERGE(2); break;
+    case 4: QSMI_MERGE(4); break;
+    case 8: QSMI_MERGE(8); break;
+    default: QSMI_MERGE(16); break;
   }
+#undef QSMI_MERGE
 }
 
 void launch_fill(float* p, long long n, float v, hipStream_t st) {

@@ -1,0 +1,292 @@
+// Device-resident QuickSampling path loop: the simulation grid, the index
+// image, the TI values and every node's neighbour list live in HBM; per path
+// wave a gather kernel builds the scan inputs of each node from the current
+// grid (include/simulation.hpp:447-537 and the QuantileSamplingModule
+// conversion, include/quantileSamplingModule.hpp:368-457), the scan kernels
+// run, and the merge kernel writes the drawn TI values back into the grid
+// (simulation.hpp:658-666). The host only enqueues launches; waves follow each
+// other in stream order, so a wave reads what earlier waves wrote.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "qsmi_internal.hpp"
+
+namespace qsmi {
+
+struct GatherParams {
+  int D;
+  int sd[3];       // simulation grid
+  int nVar, nExp, nSub, V, n;
+  int kd[3];       // expanded kernel grid
+  int kctr[3];     // QSM kernel centre dims/2
+  int types[8];    // per original variable: 0 continuous, 1 categorical
+  int expVar[16];  // expanded index -> original variable
+  int expCat[16];  // expanded index -> offset of its class value in catvals, -1 continuous
+  int conv[16];    // per neighbour sub-variable: 0 = P0 (w), 1 = P1 (w*x)
+  int subOf[16];   // per neighbour sub-variable: expanded index
+  int act[16];     // active TI sub-variables (alpha row non-zero)
+  float alpha[256];  // [A + B*nSub]
+  int s[3];        // TI grid (valid box)
+  int circular;
+  float scale;     // map scale (1/N true mismatch, 1 otherwise)
+  int trueMismatch;
+  float quantum;
+  unsigned tiCells;
+};
+
+__device__ __forceinline__ uint32_t seed_bits_dev(float seed) {
+  if (!isfinite(seed) || seed <= 0.f) return 0u;
+  if (seed >= 1.f) return 0xFFFFFFFFu;
+  return (uint32_t)floor((double)seed * 4294967295.0);
+}
+
+// One wavefront per node of the wave (order[w0 + q]).
+__global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, const int* __restrict__ order,
+                                               const unsigned* __restrict__ pathCells,
+                                               const float* __restrict__ pathSeeds, const int* __restrict__ nbIdx,
+                                               const int* __restrict__ nbCnt, const int* __restrict__ offs,
+                                               const float* __restrict__ kern, const float* __restrict__ catvals,
+                                               float* __restrict__ di, uint32_t* __restrict__ index,
+                                               const float* __restrict__ tiVals, NodeInfo* __restrict__ info,
+                                               int* __restrict__ nbOff, float* __restrict__ nbVal) {
+  const int q = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int i = order[w0 + q];
+  const unsigned cell = pathCells[i];
+  const float seed = pathSeeds[i];
+  const int cnt = nbCnt[i];
+  const int D = P.D;
+  NodeInfo ni;
+  ni.cell = cell;
+  ni.seed = seed;
+  ni.seedbits = seed_bits_dev(seed);
+  ni.quantum = P.quantum;
+  ni.scale = P.scale;
+  ni.nbStart = q * P.n;
+  if (cnt <= 1) {
+    // only the cell itself: draw from the marginal (simulation.hpp:586-606)
+    if (lane == 0) {
+      const unsigned position = (unsigned)(int)floorf(seed * (float)P.tiCells);
+      for (int v = 0; v < P.nVar; ++v) di[(long long)cell * P.nVar + v] = tiVals[(long long)position * P.nVar + v];
+      index[cell] = position;
+      ni.nb = 0;
+      ni.delta0 = 0.f;
+      for (int d = 0; d < 3; ++d) {
+        ni.lo[d] = 1;
+        ni.hi[d] = 0;
+      }
+      info[q] = ni;
+    }
+    return;
+  }
+  int cc[3] = {0, 0, 0};
+  {
+    unsigned rem = cell;
+    for (int d = 0; d < D; ++d) {
+      cc[d] = (int)(rem % (unsigned)P.sd[d]);
+      rem /= (unsigned)P.sd[d];
+    }
+  }
+  const int* idx = nbIdx + (long long)i * P.n;
+  int mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+  for (int j = lane; j < cnt; j += 64) {
+    const int* dd = offs + (long long)idx[j] * D;
+    long long t = 0, mul = 1;
+    bool inK = true;
+    long long kc = 0;
+    for (int d = 0; d < D; ++d) {
+      const int x = cc[d] + dd[d];
+      t += (long long)x * mul;
+      mul *= P.sd[d];
+      mn[d] = min(mn[d], dd[d]);
+      mx[d] = max(mx[d], dd[d]);
+    }
+    for (int d = D - 1; d >= 0; --d) {  // DataImage::indexWithDelta from the kernel centre
+      int c = P.kctr[d] + dd[d];
+      inK &= (c >= 0 && c < P.kd[d]);
+      c = ((c % P.kd[d]) + P.kd[d]) % P.kd[d];
+      kc = kc * P.kd[d] + c;
+    }
+    const float* x = di + t * P.nVar;
+    for (int d = 0; d < D; ++d) nbOff[((long long)q * P.n + j) * D + d] = dd[d];
+    // v_a = sum_B alpha(A_a, B) u_B with u_B the QSM conversion of sub-variable B
+    for (int a = 0; a < P.V; ++a) {
+      const int A = P.act[a];
+      float acc = 0.f;
+      for (int B = 0; B < P.nSub; ++B) {
+        const float al = P.alpha[A + B * P.nSub];
+        if (al == 0.f) continue;
+        const int e = P.subOf[B];
+        const float xv = x[P.expVar[e]];
+        const float xe = (P.expCat[e] < 0) ? xv : ((xv == catvals[P.expCat[e]]) ? 1.f : 0.f);
+        const float w = kern[kc * P.nExp + e];
+        const float u = (inK && !isnan(xe)) ? (P.conv[B] == 0 ? w * 1.f : w * xe) : 0.f;
+        acc += al * u;
+      }
+      nbVal[((long long)q * P.n + j) * P.V + a] = acc;
+    }
+  }
+  // bounding box (min/max start at 0, src/AcceleratorDevice.cpp:95-108)
+  for (int d = 0; d < D; ++d) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      mn[d] = min(mn[d], __shfl_xor(mn[d], off));
+      mx[d] = max(mx[d], __shfl_xor(mx[d], off));
+    }
+  }
+  if (lane == 0) {
+    // delta0 in the reference's order: variable by variable, neighbour by
+    // neighbour, float accumulation of -w x^2 read at expanded index v
+    float delta0 = 0.f;
+    for (int v = 0; v < P.nVar; ++v) {
+      if (P.types[v] != 0) continue;
+      for (int j = 0; j < cnt; ++j) {
+        const int* dd = offs + (long long)idx[j] * D;
+        long long t = 0, mul = 1, kc = 0;
+        bool inK = true;
+        for (int d = 0; d < D; ++d) {
+          t += (long long)(cc[d] + dd[d]) * mul;
+          mul *= P.sd[d];
+        }
+        for (int d = D - 1; d >= 0; --d) {
+          int c = P.kctr[d] + dd[d];
+          inK &= (c >= 0 && c < P.kd[d]);
+          c = ((c % P.kd[d]) + P.kd[d]) % P.kd[d];
+          kc = kc * P.kd[d] + c;
+        }
+        const int e = v;  // the reference reads the expanded value at the original index
+        const float xv = di[t * P.nVar + P.expVar[e]];
+        const float xe = (P.expCat[e] < 0) ? xv : ((xv == catvals[P.expCat[e]]) ? 1.f : 0.f);
+        if (inK && !isnan(xe)) delta0 += -1.f * (kern[kc * P.nExp + (v % P.nExp)] * xe * xe);
+      }
+    }
+    for (int d = 0; d < 3; ++d) {
+      if (d >= D) {
+        ni.lo[d] = ni.hi[d] = 0;
+      } else if (P.circular) {
+        ni.lo[d] = 0;
+        ni.hi[d] = P.s[d] - 1;
+      } else {
+        ni.lo[d] = -mn[d];
+        ni.hi[d] = P.s[d] - 1 - mx[d];
+      }
+    }
+    ni.nb = cnt;
+    ni.delta0 = P.trueMismatch ? delta0 : 0.f;
+    info[q] = ni;
+  }
+}
+
+int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsOut, long long* scannedOut,
+                       int* wavesOut) {
+  qsmi_ti* t = dev->ti;
+  hipStream_t st = dev->stream;
+  const int D = t->D;
+  if (R.nVar > 8 || R.nExp > 16) return fail(QSMI_ERR_UNSUPPORTED, "too many variables for the device gather");
+  GatherParams P;
+  memset(&P, 0, sizeof(P));
+  P.D = D;
+  for (int d = 0; d < 3; ++d) {
+    P.sd[d] = d < D ? R.sd[d] : 1;
+    P.kd[d] = d < D ? R.kd[d] : 1;
+    P.kctr[d] = P.kd[d] / 2;
+    P.s[d] = t->s[d];
+  }
+  P.nVar = R.nVar;
+  P.nExp = R.nExp;
+  P.nSub = t->nSub;
+  P.V = (int)dev->active.size();
+  P.n = R.n;
+  for (int v = 0; v < R.nVar; ++v) P.types[v] = R.types[v];
+  for (int e = 0; e < R.nExp; ++e) {
+    P.expVar[e] = R.expVar[e];
+    P.expCat[e] = R.expCat[e];
+  }
+  for (int B = 0; B < t->nSub; ++B) {
+    P.conv[B] = R.conv[B];
+    P.subOf[B] = R.subOf[B];
+  }
+  for (int a = 0; a < P.V; ++a) P.act[a] = dev->active[a];
+  for (int q = 0; q < t->nSub * t->nSub; ++q) P.alpha[q] = dev->alpha[q];
+  P.circular = dev->circular;
+  P.trueMismatch = dev->trueMismatch;
+  P.scale = dev->trueMismatch ? (float)(1.0 / (double)t->N) : 1.f;
+  P.quantum = dev->quantum;
+  P.tiCells = (unsigned)t->S;
+
+  const int K = (int)std::ceil(R.k);
+  int maxWave = 0;
+  for (size_t w = 0; w + 1 < R.waveStart.size(); ++w) maxWave = std::max(maxWave, R.waveStart[w + 1] - R.waveStart[w]);
+  if (!dev->wb) dev->wb = new WaveBufs();
+  WaveBufs* b = dev->wb;
+  int rc = grow_wave_bufs(dev, b, std::max(maxWave, 1), std::max(maxWave, 1) * R.n, K);
+  if (rc) return rc;
+  ScanGeom g = geom_of(dev);
+  const int rb = row_blocks(t);
+  const bool savedTiming = dev->timing;
+  dev->timing = false;  // the per-call events of launch_spectrum are not used here
+  const size_t nWaves = R.waveStart.size() > 0 ? R.waveStart.size() - 1 : 0;
+  const bool timing = R.stageTiming;
+  if (timing) {
+    const size_t need = nWaves * 5;
+    while (dev->evPool.size() < need) {
+      hipEvent_t ev;
+      hipError_t e = hipEventCreate(&ev);
+      if (e != hipSuccess) {
+        dev->timing = savedTiming;
+        return fail(QSMI_ERR_HIP, std::string("event pool: ") + hipGetErrorString(e));
+      }
+      dev->evPool.push_back(ev);
+    }
+  }
+  long long scanned = 0;
+  int waves = 0;
+  for (size_t w = 0; w < nWaves; ++w) {
+    const int w0 = R.waveStart[w], nN = R.waveStart[w + 1] - w0;
+    if (nN <= 0) continue;
+    hipEvent_t* ev = timing ? &dev->evPool[w * 5] : nullptr;
+    if (ev) (void)hipEventRecord(ev[0], st);
+    hipLaunchKernelGGL(k_gather, dim3(nN), dim3(64), 0, st, P, w0, R.dOrder, R.dPathCells, R.dPathSeeds, R.dNbIdx,
+                       R.dNbCnt, R.dOffs, R.dKernel, R.dCatvals, R.dDi, R.dIndex, R.dTi, b->dInfo, b->dOff, b->dVal);
+    if (ev) (void)hipEventRecord(ev[1], st);
+    launch_spectrum(dev, g, nN, b->dInfo, b->dOff, b->dVal, b->dC);
+    if (ev) (void)hipEventRecord(ev[2], st);
+    launch_rows(dev, g, b->dInfo, b->dC, nN, rb, K, b->dCand, nullptr);
+    if (ev) (void)hipEventRecord(ev[3], st);
+    launch_merge_draw(g, b->dInfo, b->dCand, nN, rb, rows_per_block(t), K, R.k, b->dOutVal, b->dOutPos, nullptr,
+                      R.dDi, R.dTi, R.nVar, R.dIndex, st);
+    if (ev) (void)hipEventRecord(ev[4], st);
+    scanned += nN;
+    waves++;
+  }
+  dev->timing = savedTiming;
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(QSMI_ERR_HIP, std::string("resident waves: ") + hipGetErrorString(e));
+  e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return fail(QSMI_ERR_HIP, std::string("resident waves: ") + hipGetErrorString(e));
+  double stage[4] = {0, 0, 0, 0};
+  if (timing) {
+    for (size_t w = 0; w < nWaves; ++w) {
+      if (R.waveStart[w + 1] - R.waveStart[w] <= 0) continue;
+      hipEvent_t* ev = &dev->evPool[w * 5];
+      float ms[4] = {0, 0, 0, 0};
+      for (int q = 0; q < 4; ++q) (void)hipEventElapsedTime(&ms[q], ev[q], ev[q + 1]);
+      stage[1] += ms[0];
+      stage[0] += ms[1];
+      stage[2] += ms[2];
+      stage[3] += ms[3];
+    }
+  }
+  for (int q = 0; q < 4; ++q) stageMsOut[q] = stage[q];
+  *scannedOut = scanned;
+  *wavesOut = waves;
+  return QSMI_OK;
+}
+
+}  // namespace qsmi

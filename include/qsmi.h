@@ -210,6 +210,7 @@ typedef struct {
   int waveSize;            /* max nodes per wave, 0 -> 256 */
   float tieQuantum;        /* < 0 automatic: 0 with a continuous variable,
                               2^(ceil(log2(n*max|w|))-16) otherwise */
+  int stageTiming;         /* 1: time every wave's kernels with HIP events (stageMs) */
 } qsmi_qs_request;
 
 typedef struct {
@@ -218,11 +219,14 @@ typedef struct {
   double setupMs;          /* out: TI spectra, offsets, path */
   double pathLoopMs;       /* out: neighbour search + waves + writes (the reference's timed loop) */
   double scanMs;           /* out: device time of the scan stages inside the loop */
-  double stageMs[4];       /* out: the same per stage (see qsmi_last_stage_ms) */
+  double stageMs[4];       /* out: per stage: [0] spectrum, [1] gather, [2] rows +
+                              select, [3] merge + draw + write (stageTiming) */
   long long nodes;         /* out: simulated path nodes */
   long long scannedNodes;  /* out: nodes that went through the scan (the rest
                               drew from the marginal) */
   int waves;               /* out: waves launched */
+  double phaseMs[4];       /* out: path loop phases: neighbour search, dependency
+                              levels and waves, uploads, device loop */
 } qsmi_qs_result;
 
 int qsmi_qs_run_in_memory(const qsmi_qs_request* req, qsmi_qs_result* res);
