@@ -20,6 +20,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+PROFILE = "profiles/k_pencil1k__qs_2d_continuous_1000.md"
+
+
+def measured_bytes_per_node(kernel):
+    """HBM bytes per node of `kernel` (calibrated FETCH_SIZE + WRITE_SIZE)
+    from the committed rocprofv3 counter summary, or None."""
+    try:
+        with open(os.path.join(ROOT, PROFILE)) as fh:
+            for line in fh:
+                if line.startswith(f"hbm_bytes_per_node {kernel}:"):
+                    f, w = line.split(":", 1)[1].split()
+                    return float(f) + float(w)
+    except OSError:
+        pass
+    return None
 
 
 def parse():
@@ -130,8 +145,13 @@ def main():
     roof = None
     if pencil_ms > 0:
         achieved = a_node * st["scanned_nodes"] / (pencil_ms / 1e3) / 1e9
+        nodes_per_launch = st["scanned_nodes"] / max(1, st["waves"])
+        per_node = measured_bytes_per_node("k_pencil1k") if dims == (1000, 1000) else None
         roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                    traffic=None)
+                    traffic=(per_node * nodes_per_launch if per_node else None),
+                    kernel="k_pencil1k", avg_launch_us=pencil_ms * 1e3 / max(1, st["waves"]),
+                    nodes_per_launch=nodes_per_launch, algorithmic_bytes_per_node=a_node,
+                    traffic_source=(PROFILE if per_node else None))
     threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
     cpu = None
     if args.cpu_sample > 0:
