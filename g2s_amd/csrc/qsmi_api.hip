@@ -34,7 +34,8 @@ void launch_fill(float* p, long long n, float v, hipStream_t st);
 long long ct1k_stride(int h0);
 void launch_transpose_spectra(const float2* Y, float2* YT, int h0, int f1, int nA, hipStream_t st);
 void launch_pencil1k(const ScanGeom& g, const float2* YT, const float2* tw1k, int nNodes, int npb,
-                     const NodeInfo* info, const int* nbOff, const float* nbVal, float2* CT, hipStream_t st);
+                     const NodeInfo* info, const int* nbOff, const float* nbVal, TapInfo* taps, uint8_t* tapOf,
+                     float2* CT, hipStream_t st);
 int rows1k_tiles();
 hipError_t set_rows1k_lds();
 void launch_rows1k(const ScanGeom& g, const float2* tw1k, const NodeInfo* info, const float2* CT, int nNodes, int K,
@@ -221,14 +222,31 @@ int ensure(T** p, size_t* cap, size_t need) {
 }
 
 // Launch the spectrum stage and the dim1 inverse (3-D) for nNodes nodes.
-void launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeInfo* info, const int* off,
-                     const float* val, float2* C) {
+int ensure_tap_tables(qsmi_device* dev, int nNodes) {
+  if (dev->capTap >= (size_t)nNodes) return QSMI_OK;
+  // grown rarely: hipFree waits for queued work that may still read them
+  if (dev->dTap) (void)hipFree(dev->dTap);
+  if (dev->dTapOf) (void)hipFree(dev->dTapOf);
+  dev->dTap = nullptr;
+  dev->dTapOf = nullptr;
+  dev->capTap = 0;
+  const size_t cap = std::max<size_t>((size_t)nNodes, 256);
+  if (hipMalloc(&dev->dTap, cap * sizeof(TapInfo)) != hipSuccess || hipMalloc(&dev->dTapOf, cap * 1000) != hipSuccess)
+    return fail(QSMI_ERR_NOMEM, "tap tables: out of device memory");
+  dev->capTap = cap;
+  return QSMI_OK;
+}
+
+int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeInfo* info, const int* off,
+                    const float* val, float2* C) {
   qsmi_ti* t = dev->ti;
   hipStream_t st = dev->stream;
   if (dev->timing) hipEventRecord(dev->ev[0], st);
   if (t->fast1k) {
+    int rc = ensure_tap_tables(dev, nNodes);
+    if (rc) return rc;
     const int npb = std::max(1, std::min(8, nNodes / 8));
-    launch_pencil1k(g, t->dYT, t->dTw[0], nNodes, npb, info, off, val, C, st);
+    launch_pencil1k(g, t->dYT, t->dTw[0], nNodes, npb, info, off, val, dev->dTap, dev->dTapOf, C, st);
   } else if (t->D == 1) {
     launch_direct1d(g, t->dYhat, t->dTw[0], nNodes, info, off, val, C, st);
   } else {
@@ -243,6 +261,7 @@ void launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const Node
                   t->H, (long long)t->f[1] * t->h0, info, t->f[2], 2, st);
   }
   if (dev->timing) hipEventRecord(dev->ev[2], st);
+  return QSMI_OK;
 }
 
 void launch_rows(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const float2* C, int nNodes, int rowBlocks,
@@ -448,6 +467,8 @@ int qsmi_device_destroy(qsmi_device* d) {
   if (d->stream) hipStreamSynchronize(d->stream);
   hipFree(d->dMap);
   hipFree(d->dC);
+  hipFree(d->dTap);
+  hipFree(d->dTapOf);
   hipFree(d->dCand);
   hipFree(d->dInfo);
   hipFree(d->dOff);
@@ -523,7 +544,7 @@ int qsmi_candidate_for_pattern(qsmi_device* dev, int nb, const int* offsets, con
   HIPCHK(hipMemcpyAsync(dev->dVal, val.data(), val.size() * sizeof(float), hipMemcpyHostToDevice, st));
   launch_fill(dev->dMap, t->S, -INFINITY, st);
   ScanGeom g = geom_of(dev);
-  launch_spectrum(dev, g, 1, dev->dInfo, dev->dOff, dev->dVal, dev->dC);
+  if ((rc = launch_spectrum(dev, g, 1, dev->dInfo, dev->dOff, dev->dVal, dev->dC))) return rc;
   launch_rows(dev, g, dev->dInfo, dev->dC, 1, row_blocks(t), 1, nullptr, dev->dMap);
   HIPCHK(hipGetLastError());
   // the host arrays above are pageable and go out of scope: finish the copies
@@ -751,7 +772,8 @@ int run_wave(qsmi_device* dev, WaveBufs* b) {
   qsmi_ti* t = dev->ti;
   hipStream_t st = dev->stream;
   ScanGeom g = geom_of(dev);
-  launch_spectrum(dev, g, b->nNodes, b->dInfo, b->dOff, b->dVal, b->dC);
+  int rc = launch_spectrum(dev, g, b->nNodes, b->dInfo, b->dOff, b->dVal, b->dC);
+  if (rc) return rc;
   launch_rows(dev, g, b->dInfo, b->dC, b->nNodes, b->rowBlocks, b->K, b->dCand, nullptr);
   if (dev->timing) (void)hipEventRecord(dev->ev[3], st);
   launch_merge_draw(g, b->dInfo, b->dCand, b->nNodes, b->rowBlocks, rows_per_block(t), b->K, b->k, b->dOutVal,

@@ -87,17 +87,73 @@ __device__ __forceinline__ void fft1000(float2* buf, float2 (&u)[10], const floa
 }  // namespace f1k
 
 // ---------------------------------------------------------------------------
+// Distinct neighbour rows of each node (one 64-thread block per node):
+// TapInfo with the rows ordered by (row mod 10, row), and tapOf[node][row] =
+// tap index of every row that holds a neighbour.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) k_taps1k(const NodeInfo* __restrict__ info, const int* __restrict__ nbOff,
+                                              TapInfo* __restrict__ taps, uint8_t* __restrict__ tapOf) {
+  using namespace f1k;
+  __shared__ int flag[L];
+  const int node = blockIdx.x, lane = threadIdx.x;
+  const NodeInfo ni = info[node];
+  for (int i = lane; i < L; i += 64) flag[i] = 0;
+  __syncthreads();
+  for (int t = lane; t < ni.nb; t += 64) {
+    int r = (-nbOff[(ni.nbStart + t) * 2 + 1]) % L;
+    if (r < 0) r += L;
+    flag[r] = 1;
+  }
+  __syncthreads();
+  TapInfo* tp = taps + node;
+  uint8_t* map = tapOf + (long long)node * L;
+  int count = 0;
+  if (lane == 0) tp->cls[0] = 0;
+  for (int c = 0; c < 10; ++c) {
+    for (int h = 0; h < 2; ++h) {
+      const int m = lane + 64 * h;
+      const int row = c + 10 * m;
+      const bool present = m < 100 && flag[row] != 0;
+      const unsigned long long mask = __ballot(present);
+      const int before = __popcll(mask & ((1ull << lane) - 1ull));
+      if (present && count + before < kTapMax) {
+        tp->row[count + before] = row;
+        map[row] = (uint8_t)(count + before);
+      }
+      count += __popcll(mask);
+    }
+    if (lane == 0) tp->cls[c + 1] = count;
+  }
+  if (lane == 0) tp->T = count <= kTapMax ? count : 0;
+}
+
+// ---------------------------------------------------------------------------
 // Pencil along dim1 for 1000 x 1000 grids. 128 threads per block; block b
-// works on column k0 = slice(b % 8) and a chunk of `npb` nodes, one active
-// sub-variable at a time through one 8 KiB scatter/exchange buffer.
+// works on column k0 = slice(b % 8) and a chunk of `npb` nodes.
+//
+// Forward transform of a node's sparse kernel column, two ways:
+//  * T <= kTapMax distinct neighbour rows r_t: with k1 = j + 100 q,
+//      X(k1) = sum_c exp(-2 pi i q c / 10) S_j(c),
+//      S_j(c) = sum_{t: r_t mod 10 = c} G(t) exp(-2 pi i j r_t / 1000),
+//    where G(t) sums value * exp(+2 pi i k0 d0 / 1000) over the row's
+//    neighbours: T twiddled MACs and one radix-10 butterfly per lane, no
+//    data exchange, and X arrives in exactly the registers (elements
+//    j + 100 q) the MAC and the inverse transform use. Two sub-variables
+//    share each twiddle.
+//  * otherwise: scatter into an 8 KiB column and a full Stockham transform.
+// Twiddles are read from an LDS copy of the table.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(128) k_pencil1k(ScanGeom g, const float2* __restrict__ YT,
                                                  const float2* __restrict__ tw1k, int nNodes, int npb,
                                                  const NodeInfo* __restrict__ info, const int* __restrict__ nbOff,
-                                                 const float* __restrict__ nbVal, float2* __restrict__ CT,
+                                                 const float* __restrict__ nbVal, const TapInfo* __restrict__ taps,
+                                                 const uint8_t* __restrict__ tapOf, float2* __restrict__ CT,
                                                  long long ctStride) {
   using namespace f1k;
-  __shared__ float2 sb[L];
+  __shared__ float2 sb[L];            // scatter / exchange buffer
+  __shared__ float2 twl[L];           // exp(-2 pi i t / 1000)
+  __shared__ float2 G[2 * kTapMax];   // [tap][sub-variable of the pair]
+  __shared__ int rowL[kTapMax];
   const int lane = threadIdx.x;  // butterfly index j for lane < 100
   const int h0 = g.h0;  // 501
   const int slice = (h0 + NXCD - 1) / NXCD;
@@ -111,48 +167,122 @@ __global__ void __launch_bounds__(128) k_pencil1k(ScanGeom g, const float2* __re
   const int nodeEnd = min(nNodes, nodeBeg + npb);
   const int V = g.V;
   const bool act = lane < 100;
+  for (int i = lane; i < L; i += 128) twl[i] = tw1k[i];  // first read after a barrier below
   for (int node = nodeBeg; node < nodeEnd; ++node) {
     const NodeInfo ni = info[node];
     if (ni.lo[1] > ni.hi[1] || ni.lo[0] > ni.hi[0]) continue;
     float2 M[10];
 #pragma unroll
     for (int q = 0; q < 10; ++q) M[q] = make_float2(0.f, 0.f);
-    for (int a = 0; a < V; ++a) {
-      // zero the scatter buffer, then the sparse neighbour column:
-      // value * exp(+2 pi i k0 d0 / 1000) at row (-d1) mod 1000
-      for (int i = lane; i < L; i += 128) sb[i] = make_float2(0.f, 0.f);
-      __syncthreads();
-      for (int t = lane; t < ni.nb; t += 128) {
-        const int jj = ni.nbStart + t;
-        const float v = nbVal[(long long)jj * V + a];
-        if (v == 0.f) continue;
-        const int d0 = nbOff[jj * 2 + 0];
-        const int d1 = nbOff[jj * 2 + 1];
-        int e = (k0 * d0) % L;
-        if (e < 0) e += L;
-        const float2 w = tw1k[e];
-        int q = (-d1) % L;
-        if (q < 0) q += L;
-        atomicAdd(&sb[q].x, v * w.x);
-        atomicAdd(&sb[q].y, -v * w.y);
-      }
-      __syncthreads();
-      float2 u[10];
-      fft1000<-1>(sb, u, tw1k, lane, true);
-      if (act) {
-        const float2* Y = YT + ((long long)g.A[a] * h0 + k0) * L;
-#pragma unroll
-        for (int q = 0; q < 10; ++q) {
-          const float2 y = Y[lane + 100 * q];
-          const float2 x = u[q];
-          M[q].x += y.x * x.x - y.y * x.y;
-          M[q].y += y.x * x.y + y.y * x.x;
+    const TapInfo* tp = taps + node;
+    const int T = tp->T;
+    if (T > 0) {
+      const uint8_t* tmap = tapOf + (long long)node * L;
+      for (int a0 = 0; a0 < V; a0 += 2) {
+        const bool two = a0 + 1 < V;
+        for (int i = lane; i < 2 * T; i += 128) G[i] = make_float2(0.f, 0.f);
+        if (lane < T) rowL[lane] = tp->row[lane];
+        __syncthreads();
+        for (int t = lane; t < ni.nb; t += 128) {
+          const int jj = ni.nbStart + t;
+          const float v0 = nbVal[(long long)jj * V + a0];
+          const float v1 = two ? nbVal[(long long)jj * V + a0 + 1] : 0.f;
+          if (v0 == 0.f && v1 == 0.f) continue;
+          const int d0 = nbOff[jj * 2 + 0];
+          const int d1 = nbOff[jj * 2 + 1];
+          int e = (k0 * d0) % L;
+          if (e < 0) e += L;
+          const float2 w = twl[e];
+          int q = (-d1) % L;
+          if (q < 0) q += L;
+          const int tap = tmap[q];
+          atomicAdd(&G[2 * tap].x, v0 * w.x);
+          atomicAdd(&G[2 * tap].y, -v0 * w.y);
+          if (two) {
+            atomicAdd(&G[2 * tap + 1].x, v1 * w.x);
+            atomicAdd(&G[2 * tap + 1].y, -v1 * w.y);
+          }
         }
+        __syncthreads();
+        if (act) {
+          float2 S0[10], S1[10];
+#pragma unroll
+          for (int c = 0; c < 10; ++c) {
+            float2 s0 = make_float2(0.f, 0.f), s1 = make_float2(0.f, 0.f);
+            const int te = tp->cls[c + 1];
+            for (int t = tp->cls[c]; t < te; ++t) {
+              // (j * r) mod 1000 in float: exact for every j < 100, r < 1000
+              const uint32_t p = (uint32_t)lane * (uint32_t)rowL[t];
+              const uint32_t qd = (uint32_t)((float)p * 0.001f);
+              const float2 w = twl[p - qd * 1000u];
+              const float2 g0 = G[2 * t], g1 = G[2 * t + 1];
+              s0.x += g0.x * w.x - g0.y * w.y;
+              s0.y += g0.x * w.y + g0.y * w.x;
+              s1.x += g1.x * w.x - g1.y * w.y;
+              s1.y += g1.x * w.y + g1.y * w.x;
+            }
+            S0[c] = s0;
+            S1[c] = s1;
+          }
+          dft10<-1>(S0);
+          const float2* Y0 = YT + ((long long)g.A[a0] * h0 + k0) * L;
+#pragma unroll
+          for (int q = 0; q < 10; ++q) {
+            const float2 y = Y0[lane + 100 * q];
+            M[q].x += y.x * S0[q].x - y.y * S0[q].y;
+            M[q].y += y.x * S0[q].y + y.y * S0[q].x;
+          }
+          if (two) {
+            dft10<-1>(S1);
+            const float2* Y1 = YT + ((long long)g.A[a0 + 1] * h0 + k0) * L;
+#pragma unroll
+            for (int q = 0; q < 10; ++q) {
+              const float2 y = Y1[lane + 100 * q];
+              M[q].x += y.x * S1[q].x - y.y * S1[q].y;
+              M[q].y += y.x * S1[q].y + y.y * S1[q].x;
+            }
+          }
+        }
+        __syncthreads();
       }
-      __syncthreads();
+    } else {
+      for (int a = 0; a < V; ++a) {
+        // zero the scatter buffer, then the sparse neighbour column:
+        // value * exp(+2 pi i k0 d0 / 1000) at row (-d1) mod 1000
+        for (int i = lane; i < L; i += 128) sb[i] = make_float2(0.f, 0.f);
+        __syncthreads();
+        for (int t = lane; t < ni.nb; t += 128) {
+          const int jj = ni.nbStart + t;
+          const float v = nbVal[(long long)jj * V + a];
+          if (v == 0.f) continue;
+          const int d0 = nbOff[jj * 2 + 0];
+          const int d1 = nbOff[jj * 2 + 1];
+          int e = (k0 * d0) % L;
+          if (e < 0) e += L;
+          const float2 w = twl[e];
+          int q = (-d1) % L;
+          if (q < 0) q += L;
+          atomicAdd(&sb[q].x, v * w.x);
+          atomicAdd(&sb[q].y, -v * w.y);
+        }
+        __syncthreads();
+        float2 u[10];
+        fft1000<-1>(sb, u, twl, lane, true);
+        if (act) {
+          const float2* Y = YT + ((long long)g.A[a] * h0 + k0) * L;
+#pragma unroll
+          for (int q = 0; q < 10; ++q) {
+            const float2 y = Y[lane + 100 * q];
+            const float2 x = u[q];
+            M[q].x += y.x * x.x - y.y * x.y;
+            M[q].y += y.x * x.y + y.y * x.x;
+          }
+        }
+        __syncthreads();
+      }
     }
     // inverse along dim1, input already in registers
-    fft1000<+1>(sb, M, tw1k, lane, false);
+    fft1000<+1>(sb, M, twl, lane, false);
     if (act) {
       float2* Cn = CT + (long long)node * ctStride;
       const int lo = ni.lo[1], hi = ni.hi[1];
@@ -166,39 +296,157 @@ __global__ void __launch_bounds__(128) k_pencil1k(ScanGeom g, const float2* __re
   }
 }
 
-// Block-wide K best of per-thread sorted lists (blockDim multiple of 64),
-// best first into out[0..K). Ends with a barrier.
-template <int KT>
-__device__ void block_topk(Cand* c, int K, Cand* out, Cand* red) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
-  for (int it = 0; it < K; ++it) {
-    Cand best = c[0];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      Cand o;
-      o.key = __shfl_xor(best.key, off);
-      o.h = __shfl_xor(best.h, off);
-      o.pos = __shfl_xor(best.pos, off);
-      o.val = __shfl_xor(best.val, off);
-      if (cand_better(o, best)) best = o;
-    }
-    if (lane == 0) red[wave] = best;
-    __syncthreads();
-    Cand w = red[0];
-    for (int i = 1; i < nwaves; ++i)
-      if (cand_better(red[i], w)) w = red[i];
-    if (tid == 0) out[it] = w;
-    if (c[0].pos == w.pos && c[0].pos != 0xFFFFFFFFu) {
-#pragma unroll
-      for (int i = 0; i < KT - 1; ++i) c[i] = c[i + 1];
-      c[KT - 1].key = -INFINITY;
-      c[KT - 1].h = 0u;
-      c[KT - 1].pos = 0xFFFFFFFFu;
-      c[KT - 1].val = -INFINITY;
-    }
-    __syncthreads();
-  }
+// ---------------------------------------------------------------------------
+// Block top-K of the device rule in two passes (blockDim a multiple of 64,
+// at most 8 wavefronts):
+//  1. every thread keeps its KT largest keys (floats only); the block finds
+//     tau, a lower bound of its K-th largest key counted with multiplicity;
+//  2. only values with key >= tau (normally exactly K per block) get the tie
+//     hash and enter a per-thread list ordered by the 64-bit rank
+//     (orderable key bits << 32 | hash), from which K rounds of block maxima
+//     extract the result. The hash is a bijection of the position for a
+//     fixed seed, so ranks are distinct and (key desc, hash desc) is the
+//     whole order of the rule (the position tie-break never triggers).
+// ---------------------------------------------------------------------------
+struct TopkScratch {
+  float f[2][8];
+  int n[2][8];
+  unsigned long long r[2][8];
+  uint32_t pos;
+  float val;
+};
+
+// orderable image of a key: larger key -> larger integer; -0 folds into +0
+__device__ __forceinline__ uint32_t key_bits(float k) {
+  uint32_t u = __float_as_uint(k);
+  if (u == 0x80000000u) u = 0u;
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
+
+__device__ __forceinline__ float key_of_bits(uint32_t b) {
+  return __uint_as_float((b & 0x80000000u) ? (b & 0x7FFFFFFFu) : ~b);
+}
+
+// tau <= the block's K-th largest key (multiplicity counted); -inf when the
+// block holds fewer than K keys. m: this thread's KT largest keys, descending
+// (consumed). Block-uniform result; ends after a barrier.
+template <int KT>
+__device__ float block_kth_key(float (&m)[KT], int K, TopkScratch* s) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+  int need = K;
+  float tau = -INFINITY;
+  for (int it = 0; it < K; ++it) {
+    const int b = it & 1;
+    float M = m[0];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) M = fmaxf(M, __shfl_xor(M, off));
+    if (lane == 0) s->f[b][wave] = M;
+    __syncthreads();
+    M = s->f[b][0];
+    for (int i = 1; i < nw; ++i) M = fmaxf(M, s->f[b][i]);
+    tau = M;
+    if (M == -INFINITY) break;
+    int c = 0;
+#pragma unroll
+    for (int i = 0; i < KT; ++i) c += (m[i] == M) ? 1 : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if (lane == 0) s->n[b][wave] = c;
+    __syncthreads();
+    int tot = 0;
+    for (int i = 0; i < nw; ++i) tot += s->n[b][i];
+    if (tot >= need) break;
+    need -= tot;
+    // the entries equal to M sit at the head of the list: pop them
+#pragma unroll
+    for (int r = 0; r < KT; ++r) {
+      if (m[0] == M) {
+#pragma unroll
+        for (int i = 0; i < KT - 1; ++i) m[i] = m[i + 1];
+        m[KT - 1] = -INFINITY;
+      }
+    }
+  }
+  return tau;
+}
+
+template <int KT>
+struct RankList {
+  unsigned long long r[KT];  // 0 = empty
+  uint32_t pos[KT];
+  float val[KT];
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+      r[i] = 0ull;
+      pos[i] = 0xFFFFFFFFu;
+      val[i] = -INFINITY;
+    }
+  }
+  __device__ __forceinline__ void insert(float key, uint32_t h, uint32_t p, float v) {
+    unsigned long long c = ((unsigned long long)key_bits(key) << 32) | h;
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+      if (c > r[i]) {
+        const unsigned long long tc = r[i];
+        const uint32_t tp = pos[i];
+        const float tv = val[i];
+        r[i] = c;
+        pos[i] = p;
+        val[i] = v;
+        c = tc;
+        p = tp;
+        v = tv;
+      }
+    }
+  }
+  // K rounds of block maxima -> out[0..K), best first; ends after a barrier
+  __device__ void block_extract(int K, Cand* out, TopkScratch* s) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+    for (int it = 0; it < K; ++it) {
+      const int b = it & 1;
+      unsigned long long M = r[0];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(M, off);
+        M = o > M ? o : M;
+      }
+      if (lane == 0) s->r[b][wave] = M;
+      __syncthreads();
+      M = s->r[b][0];
+      for (int i = 1; i < nw; ++i) M = s->r[b][i] > M ? s->r[b][i] : M;
+      if (M != 0ull && r[0] == M) {  // exactly one thread holds it
+        s->pos = pos[0];
+        s->val = val[0];
+#pragma unroll
+        for (int i = 0; i < KT - 1; ++i) {
+          r[i] = r[i + 1];
+          pos[i] = pos[i + 1];
+          val[i] = val[i + 1];
+        }
+        r[KT - 1] = 0ull;
+        pos[KT - 1] = 0xFFFFFFFFu;
+        val[KT - 1] = -INFINITY;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        Cand w;
+        if (M != 0ull) {
+          w.key = key_of_bits((uint32_t)(M >> 32));
+          w.h = (uint32_t)M;
+          w.pos = s->pos;
+          w.val = s->val;
+        } else {
+          w.key = -INFINITY;
+          w.h = 0u;
+          w.pos = 0xFFFFFFFFu;
+          w.val = -INFINITY;
+        }
+        out[it] = w;
+      }
+    }
+  }
+};
 
 // ---------------------------------------------------------------------------
 // Rows for 1000 x 1000: one 128-thread block per row pair (2p, 2p+1) of one
@@ -214,7 +462,7 @@ __global__ void __launch_bounds__(128) k_rows1k_pair(ScanGeom g, const float2* _
                                                      float* __restrict__ map) {
   using namespace f1k;
   __shared__ float2 buf[L];
-  __shared__ Cand red[2];
+  __shared__ TopkScratch red;
   const int j = threadIdx.x;
   const int node = blockIdx.y;
   const int p = blockIdx.x;
@@ -258,49 +506,67 @@ __global__ void __launch_bounds__(128) k_rows1k_pair(ScanGeom g, const float2* _
     }
   }
   fft1000<+1>(buf, u, tw1k, j, false);
-  Cand c[KT];
-#pragma unroll
-  for (int i = 0; i < KT; ++i) {
-    c[i].key = -INFINITY;
-    c[i].h = 0u;
-    c[i].pos = 0xFFFFFFFFu;
-    c[i].val = -INFINITY;
-  }
   const float scale = ni.scale, d0 = ni.delta0, quantum = ni.quantum;
+  const float invq = quantum > 0.f ? 1.f / quantum : 0.f;
   const int s0 = g.s[0];
-  if (act) {
+  const bool va2[2] = {va, vb};
+  if (map && act) {
 #pragma unroll
     for (int q = 0; q < 10; ++q) {
       const int c0 = j + 100 * q;
       if (c0 < ni.lo[0] || c0 > ni.hi[0]) continue;
 #pragma unroll
-      for (int w = 0; w < 2; ++w) {
-        if (w == 0 ? !va : !vb) continue;
-        const float val = (w == 0 ? u[q].x : u[q].y) * scale + d0;
-        const uint32_t pos = (uint32_t)(c0 + s0 * (ca + w));
-        if (map) map[(long long)node * g.s[0] * g.s[1] + pos] = val;
-        if (out) {
-          const float key = (quantum > 0.f) ? rintf(val * (1.f / quantum)) : val;
-          if (!(key >= c[KT - 1].key)) continue;
-          Cand cd;
-          cd.key = key;
-          cd.val = val;
-          cd.pos = pos;
-          cd.h = tie_hash(ni.seedbits, pos);
-          if (!cand_better(cd, c[KT - 1])) continue;
+      for (int w = 0; w < 2; ++w)
+        if (va2[w])
+          map[(long long)node * g.s[0] * g.s[1] + (uint32_t)(c0 + s0 * (ca + w))] =
+              (w == 0 ? u[q].x : u[q].y) * scale + d0;
+    }
+  }
+  if (!out) return;
+  // pass 1: per-thread KT largest keys (keys only, min/max network)
+  float m[KT];
 #pragma unroll
-          for (int i = 0; i < KT; ++i) {
-            if (cand_better(cd, c[i])) {
-              Cand tt = c[i];
-              c[i] = cd;
-              cd = tt;
-            }
-          }
+  for (int i = 0; i < KT; ++i) m[i] = -INFINITY;
+  if (act) {
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+      const int c0 = j + 100 * q;
+      const bool okc = (c0 >= ni.lo[0] && c0 <= ni.hi[0]);
+#pragma unroll
+      for (int w = 0; w < 2; ++w) {
+        const float val = (w == 0 ? u[q].x : u[q].y) * scale + d0;
+        float key = quantum > 0.f ? rintf(val * invq) : val;
+        key = (okc && va2[w]) ? key : -INFINITY;
+#pragma unroll
+        for (int i = 0; i < KT; ++i) {
+          const float hi = fmaxf(m[i], key);
+          key = fminf(m[i], key);
+          m[i] = hi;
         }
       }
     }
   }
-  if (out) block_topk<KT>(c, K, out, red);
+  const float tau = block_kth_key<KT>(m, K, &red);
+  // pass 2: exact device rule among the keys >= tau (normally K of them)
+  RankList<KT> rl;
+  rl.init();
+  if (act) {
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+      const int c0 = j + 100 * q;
+      const bool okc = (c0 >= ni.lo[0] && c0 <= ni.hi[0]);
+#pragma unroll
+      for (int w = 0; w < 2; ++w) {
+        const float val = (w == 0 ? u[q].x : u[q].y) * scale + d0;
+        const float key = quantum > 0.f ? rintf(val * invq) : val;
+        if (okc && va2[w] && key >= tau && key > -INFINITY) {
+          const uint32_t pos = (uint32_t)(c0 + s0 * (ca + w));
+          rl.insert(key, tie_hash(ni.seedbits, pos), pos, val);
+        }
+      }
+    }
+  }
+  rl.block_extract(K, out, &red);
 }
 
 // YT[A][k0][k1] <- Yhat[A][k1][k0] (once per TI)
@@ -332,12 +598,14 @@ void launch_transpose_spectra(const float2* Y, float2* YT, int h0, int f1, int n
 }
 
 void launch_pencil1k(const ScanGeom& g, const float2* YT, const float2* tw1k, int nNodes, int npb,
-                     const NodeInfo* info, const int* nbOff, const float* nbVal, float2* CT, hipStream_t st) {
+                     const NodeInfo* info, const int* nbOff, const float* nbVal, TapInfo* taps, uint8_t* tapOf,
+                     float2* CT, hipStream_t st) {
+  hipLaunchKernelGGL(k_taps1k, dim3((unsigned)nNodes), dim3(64), 0, st, info, nbOff, taps, tapOf);
   const int slice = (g.h0 + f1k::NXCD - 1) / f1k::NXCD;
   const int chunks = (nNodes + npb - 1) / npb;
   const long long blocks = (long long)f1k::NXCD * slice * chunks;
   hipLaunchKernelGGL(k_pencil1k, dim3((unsigned)blocks), dim3(128), 0, st, g, YT, tw1k, nNodes, npb, info, nbOff, nbVal,
-                     CT, ct1k_stride(g.h0));
+                     taps, tapOf, CT, ct1k_stride(g.h0));
 }
 
 int rows1k_tiles() { return (1000 + f1k::TR - 1) / f1k::TR; }

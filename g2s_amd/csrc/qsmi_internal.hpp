@@ -50,6 +50,9 @@ struct qsmi_device {
   // workspace
   float2* dC = nullptr;
   size_t capC = 0;
+  qsmi::TapInfo* dTap = nullptr;  // 1000^2 path: distinct neighbour rows per node
+  uint8_t* dTapOf = nullptr;   // [node][1000] tap index of a row
+  size_t capTap = 0;
   Cand* dCand = nullptr;
   size_t capCand = 0;
   NodeInfo* dInfo = nullptr;
@@ -109,8 +112,11 @@ long long c_stride(const qsmi_ti* t);
 int grow_wave_bufs(qsmi_device* dev, WaveBufs* b, int nNodes, int total, int K);
 void free_wave_bufs(WaveBufs* b);
 // spectrum stage (+ dim1 inverse for 3-D) and rows stage of nNodes nodes
-void launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeInfo* info, const int* off,
-                     const float* val, float2* C);
+// (returns a qsmi status: tap tables may be allocated)
+int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeInfo* info, const int* off,
+                    const float* val, float2* C);
+// tap tables of the 1000^2 spectrum stage for nNodes nodes per launch
+int ensure_tap_tables(qsmi_device* dev, int nNodes);
 void launch_rows(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const float2* C, int nNodes,
                  int rowBlocks, int K, Cand* cand, float* map);
 // merge + draw; when simVals != nullptr also writes the drawn TI values and

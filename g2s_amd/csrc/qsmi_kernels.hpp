@@ -25,6 +25,16 @@ struct ScanGeom {
   long long H;  // half-spectrum size h0*f1*f2
 };
 
+// Distinct neighbour rows of one node on a 1000-row grid (row = (-d1) mod
+// 1000), ordered by (row mod 10, row), for the sparse column transform.
+// T = 0 when the node has more than kTapMax distinct rows (full FFT used).
+constexpr int kTapMax = 48;
+struct TapInfo {
+  int T;
+  int cls[11];        // rows of class c: [cls[c], cls[c+1])
+  int row[kTapMax];
+};
+
 // Per path node of a wave. Valid TI centres: lo[i] <= c_i <= hi[i].
 struct NodeInfo {
   int lo[3];

@@ -227,6 +227,7 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   WaveBufs* b = dev->wb;
   int rc = grow_wave_bufs(dev, b, std::max(maxWave, 1), std::max(maxWave, 1) * R.n, K);
   if (rc) return rc;
+  if (t->fast1k && (rc = ensure_tap_tables(dev, std::max(maxWave, 1)))) return rc;
   ScanGeom g = geom_of(dev);
   const int rb = row_blocks(t);
   const bool savedTiming = dev->timing;
@@ -255,7 +256,7 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
     hipLaunchKernelGGL(k_gather, dim3(nN), dim3(64), 0, st, P, w0, R.dOrder, R.dPathCells, R.dPathSeeds, R.dNbIdx,
                        R.dNbCnt, R.dOffs, R.dKernel, R.dCatvals, R.dDi, R.dIndex, R.dTi, b->dInfo, b->dOff, b->dVal);
     if (ev) (void)hipEventRecord(ev[1], st);
-    launch_spectrum(dev, g, nN, b->dInfo, b->dOff, b->dVal, b->dC);
+    if ((rc = launch_spectrum(dev, g, nN, b->dInfo, b->dOff, b->dVal, b->dC))) return rc;
     if (ev) (void)hipEventRecord(ev[2], st);
     launch_rows(dev, g, b->dInfo, b->dC, nN, rb, K, b->dCand, nullptr);
     if (ev) (void)hipEventRecord(ev[3], st);
