@@ -58,11 +58,11 @@ __device__ __forceinline__ void store_y(float2* buf, const float2 (&u)[10], int 
 
 // Full transform. On entry: `fromLds` -> input in buf (natural order), else
 // input already in u (elements j + 100 q). On exit u holds elements j + 100 q.
-// Every thread of the block must call it (barriers inside); threads >= 100
-// idle.
+// Every thread of the block must call it (barriers inside); only threads
+// with `act` (butterfly index j < 100 of a busy group) compute.
 template <int DIR>
-__device__ __forceinline__ void fft1000(float2* buf, float2 (&u)[10], const float2* tw, int j, bool fromLds) {
-  const bool act = j < 100;
+__device__ __forceinline__ void fft1000(float2* buf, float2 (&u)[10], const float2* tw, int j, bool act,
+                                        bool fromLds) {
   if (fromLds) {
     if (act) load_x(buf, u, j);
     __syncthreads();
@@ -82,6 +82,11 @@ __device__ __forceinline__ void fft1000(float2* buf, float2 (&u)[10], const floa
   if (act) load_x(buf, u, j);
   __syncthreads();
   if (act) stage_compute<DIR>(u, j, 100, tw);  // stage 3: outputs j + 100 q stay in registers
+}
+
+template <int DIR>
+__device__ __forceinline__ void fft1000(float2* buf, float2 (&u)[10], const float2* tw, int j, bool fromLds) {
+  fft1000<DIR>(buf, u, tw, j, j < 100, fromLds);
 }
 
 }  // namespace f1k
@@ -143,7 +148,7 @@ __global__ void __launch_bounds__(64) k_taps1k(const NodeInfo* __restrict__ info
 //  * otherwise: scatter into an 8 KiB column and a full Stockham transform.
 // Twiddles are read from an LDS copy of the table.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(128) k_pencil1k(ScanGeom g, const float2* __restrict__ YT,
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) k_pencil1k(ScanGeom g, const float2* __restrict__ YT,
                                                  const float2* __restrict__ tw1k, int nNodes, int npb,
                                                  const NodeInfo* __restrict__ info, const int* __restrict__ nbOff,
                                                  const float* __restrict__ nbVal, const TapInfo* __restrict__ taps,
@@ -449,52 +454,68 @@ struct RankList {
 };
 
 // ---------------------------------------------------------------------------
-// Rows for 1000 x 1000: one 128-thread block per row pair (2p, 2p+1) of one
-// node (two-for-one c2r along dim0); thread j loads Z[j + 100 q] of both rows
-// as one 16-byte load; epilogue, mask, block top-K -> cand[node][p][K].
-// Pairs without a valid row write -inf candidates.
+// Rows for 1000 x 1000: G row pairs (2p, 2p+1) of one node per block, one
+// group of 100 threads per pair (block of 64*ceil(100G/64) threads); a
+// pair is one two-for-one c2r along dim0: thread j of a group loads
+// Z[j + 100 q] of both rows as one 16-byte load. Epilogue: scale + delta0,
+// the valid-position mask, and the block's K best of all G pairs by the
+// device rule -> cand[node][block][K]. Blocks without a valid row write -inf
+// candidates.
 // ---------------------------------------------------------------------------
-template <int KT>
-__global__ void __launch_bounds__(128) k_rows1k_pair(ScanGeom g, const float2* __restrict__ tw1k,
-                                                     const NodeInfo* __restrict__ info,
-                                                     const float2* __restrict__ CT, long long ctStride, int K,
-                                                     Cand* __restrict__ cand, int candPairs,
-                                                     float* __restrict__ map) {
+template <int KT, int G>
+__global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k(ScanGeom g, const float2* __restrict__ tw1k,
+                                                                     const NodeInfo* __restrict__ info,
+                                                                     const float2* __restrict__ CT,
+                                                                     long long ctStride, int K,
+                                                                     Cand* __restrict__ cand, int candBlocks,
+                                                                     float* __restrict__ map) {
   using namespace f1k;
-  __shared__ float2 buf[L];
+  __shared__ float2 buf[G][L];
+  __shared__ float2 twl[L];
   __shared__ TopkScratch red;
-  const int j = threadIdx.x;
+  const int tid = threadIdx.x;
+  const int grp = tid / 100;
+  const int j = tid - 100 * grp;
   const int node = blockIdx.y;
-  const int p = blockIdx.x;
   const NodeInfo ni = info[node];
   const int h0 = g.h0;
-  Cand* out = cand ? cand + ((long long)node * candPairs + p) * K : nullptr;
+  Cand* out = cand ? cand + ((long long)node * candBlocks + blockIdx.x) * K : nullptr;
+  const int p = blockIdx.x * G + grp;  // this group's pair
   const int ca = 2 * p;
-  const bool va = (ca >= ni.lo[1] && ca <= ni.hi[1]), vb = (ca + 1 >= ni.lo[1] && ca + 1 <= ni.hi[1]);
-  if ((!va && !vb) || ni.lo[0] > ni.hi[0]) {
-    if (out && j < K) {
+  const bool gin = grp < G && p < L / 2;
+  const bool va = gin && (ca >= ni.lo[1] && ca <= ni.hi[1]);
+  const bool vb = gin && (ca + 1 >= ni.lo[1] && ca + 1 <= ni.hi[1]);
+  // block-uniform exit when none of the block's rows is valid
+  const int rlo = max(ni.lo[1], 2 * blockIdx.x * G), rhi = min(ni.hi[1], 2 * (blockIdx.x * G + G) - 1);
+  if (rlo > rhi || ni.lo[0] > ni.hi[0]) {
+    if (out && tid < K) {
       Cand w;
       w.key = -INFINITY;
       w.h = 0u;
       w.pos = 0xFFFFFFFFu;
       w.val = -INFINITY;
-      out[j] = w;
+      out[tid] = w;
     }
     return;
   }
-  const float4* rowp = reinterpret_cast<const float4*>(CT + (long long)node * ctStride + (long long)p * h0 * TR);
-  const bool act = j < 100;
+  for (int i = tid; i < L; i += blockDim.x) twl[i] = tw1k[i];  // first read after the FFT's first barrier
+  const bool act = va || vb;  // this thread's group has a valid row (j < 100 by construction)
   float2 u[10];
   if (act) {
+    const float4* rowp =
+        reinterpret_cast<const float4*>(CT + (long long)node * ctStride + (long long)p * h0 * TR);
+    // Hermitian completion of the half spectrum (h0 = 501): columns
+    // k = j + 100 q >= 501 are conj(Z[1000 - k]); the imaginary parts of
+    // columns 0 and 500 do not enter a c2r
 #pragma unroll
     for (int q = 0; q < 10; ++q) {
       const int k = j + 100 * q;
-      const bool cj = (k >= h0);
+      const bool cj = (q >= 6) || (q == 5 && j > 0);
       const int kk = cj ? (L - k) : k;
       const float4 ab = rowp[kk];  // (row 2p, row 2p+1) at column kk
       float2 A = va ? make_float2(ab.x, ab.y) : make_float2(0.f, 0.f);
       float2 B = vb ? make_float2(ab.z, ab.w) : make_float2(0.f, 0.f);
-      if (kk == 0 || 2 * kk == L) {  // a Hermitian c2r ignores these imaginary parts
+      if ((q == 0 || q == 5) && j == 0) {
         A.y = 0.f;
         B.y = 0.f;
       }
@@ -505,65 +526,68 @@ __global__ void __launch_bounds__(128) k_rows1k_pair(ScanGeom g, const float2* _
       u[q] = make_float2(A.x - B.y, A.y + B.x);
     }
   }
-  fft1000<+1>(buf, u, tw1k, j, false);
+  fft1000<+1>(buf[gin ? grp : 0], u, twl, j, act, false);
   const float scale = ni.scale, d0 = ni.delta0, quantum = ni.quantum;
-  const float invq = quantum > 0.f ? 1.f / quantum : 0.f;
   const int s0 = g.s[0];
-  const bool va2[2] = {va, vb};
+  const int lo0 = ni.lo[0], hi0 = ni.hi[0];
   if (map && act) {
 #pragma unroll
     for (int q = 0; q < 10; ++q) {
       const int c0 = j + 100 * q;
-      if (c0 < ni.lo[0] || c0 > ni.hi[0]) continue;
-#pragma unroll
-      for (int w = 0; w < 2; ++w)
-        if (va2[w])
-          map[(long long)node * g.s[0] * g.s[1] + (uint32_t)(c0 + s0 * (ca + w))] =
-              (w == 0 ? u[q].x : u[q].y) * scale + d0;
+      if (c0 < lo0 || c0 > hi0) continue;
+      if (va) map[(long long)node * g.s[0] * g.s[1] + (uint32_t)(c0 + s0 * ca)] = u[q].x * scale + d0;
+      if (vb) map[(long long)node * g.s[0] * g.s[1] + (uint32_t)(c0 + s0 * (ca + 1))] = u[q].y * scale + d0;
     }
   }
   if (!out) return;
-  // pass 1: per-thread KT largest keys (keys only, min/max network)
+  // keys of this thread's 20 positions, -inf where invalid
+  float key[20];
+  if (quantum > 0.f) {
+    const float invq = 1.f / quantum;
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+      key[2 * q] = rintf((u[q].x * scale + d0) * invq);
+      key[2 * q + 1] = rintf((u[q].y * scale + d0) * invq);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+      key[2 * q] = u[q].x * scale + d0;
+      key[2 * q + 1] = u[q].y * scale + d0;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 10; ++q) {
+    const int c0 = j + 100 * q;
+    const bool okc = act && c0 >= lo0 && c0 <= hi0;
+    key[2 * q] = (okc && va) ? key[2 * q] : -INFINITY;
+    key[2 * q + 1] = (okc && vb) ? key[2 * q + 1] : -INFINITY;
+  }
+  // pass 1: per-thread KT largest keys (min/max network)
   float m[KT];
 #pragma unroll
   for (int i = 0; i < KT; ++i) m[i] = -INFINITY;
-  if (act) {
 #pragma unroll
-    for (int q = 0; q < 10; ++q) {
-      const int c0 = j + 100 * q;
-      const bool okc = (c0 >= ni.lo[0] && c0 <= ni.hi[0]);
+  for (int e = 0; e < 20; ++e) {
+    float x = key[e];
 #pragma unroll
-      for (int w = 0; w < 2; ++w) {
-        const float val = (w == 0 ? u[q].x : u[q].y) * scale + d0;
-        float key = quantum > 0.f ? rintf(val * invq) : val;
-        key = (okc && va2[w]) ? key : -INFINITY;
-#pragma unroll
-        for (int i = 0; i < KT; ++i) {
-          const float hi = fmaxf(m[i], key);
-          key = fminf(m[i], key);
-          m[i] = hi;
-        }
-      }
+    for (int i = 0; i < KT; ++i) {
+      const float hi = fmaxf(m[i], x);
+      x = fminf(m[i], x);
+      m[i] = hi;
     }
   }
-  const float tau = block_kth_key<KT>(m, K, &red);
-  // pass 2: exact device rule among the keys >= tau (normally K of them)
+  // tau2 also excludes the -inf of invalid positions
+  const float tau2 = fmaxf(block_kth_key<KT>(m, K, &red), -3.402823466e38f);
+  // pass 2: exact device rule among the keys >= tau (normally K per block)
   RankList<KT> rl;
   rl.init();
-  if (act) {
 #pragma unroll
-    for (int q = 0; q < 10; ++q) {
-      const int c0 = j + 100 * q;
-      const bool okc = (c0 >= ni.lo[0] && c0 <= ni.hi[0]);
-#pragma unroll
-      for (int w = 0; w < 2; ++w) {
-        const float val = (w == 0 ? u[q].x : u[q].y) * scale + d0;
-        const float key = quantum > 0.f ? rintf(val * invq) : val;
-        if (okc && va2[w] && key >= tau && key > -INFINITY) {
-          const uint32_t pos = (uint32_t)(c0 + s0 * (ca + w));
-          rl.insert(key, tie_hash(ni.seedbits, pos), pos, val);
-        }
-      }
+  for (int e = 0; e < 20; ++e) {
+    if (key[e] >= tau2) {
+      const int q = e >> 1, w = e & 1;
+      const uint32_t pos = (uint32_t)(j + 100 * q + s0 * (ca + w));
+      rl.insert(key[e], tie_hash(ni.seedbits, pos), pos, (w ? u[q].y : u[q].x) * scale + d0);
     }
   }
   rl.block_extract(K, out, &red);
@@ -608,23 +632,27 @@ void launch_pencil1k(const ScanGeom& g, const float2* YT, const float2* tw1k, in
                      taps, tapOf, CT, ct1k_stride(g.h0));
 }
 
-int rows1k_tiles() { return (1000 + f1k::TR - 1) / f1k::TR; }
+// row pairs per rows block: 5 pairs -> 512 threads, 98 % of the lanes busy
+constexpr int kRowsG = 5;
+
+int rows1k_tiles() { return (1000 / 2 + kRowsG - 1) / kRowsG; }
 
 hipError_t set_rows1k_lds() { return hipSuccess; }
 
 void launch_rows1k(const ScanGeom& g, const float2* tw1k, const NodeInfo* info, const float2* CT, int nNodes, int K,
                    Cand* cand, float* map, hipStream_t st) {
-  const int pairs = rows1k_tiles();
-  dim3 grid((unsigned)pairs, (unsigned)nNodes);
+  const int blocks = rows1k_tiles();
+  dim3 grid((unsigned)blocks, (unsigned)nNodes);
+  dim3 block(64 * ((100 * kRowsG + 63) / 64));
   const long long cs = ct1k_stride(g.h0);
   if (K <= 2)
-    hipLaunchKernelGGL(k_rows1k_pair<2>, grid, dim3(128), 0, st, g, tw1k, info, CT, cs, K, cand, pairs, map);
+    hipLaunchKernelGGL((k_rows1k<2, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, blocks, map);
   else if (K <= 4)
-    hipLaunchKernelGGL(k_rows1k_pair<4>, grid, dim3(128), 0, st, g, tw1k, info, CT, cs, K, cand, pairs, map);
+    hipLaunchKernelGGL((k_rows1k<4, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, blocks, map);
   else if (K <= 8)
-    hipLaunchKernelGGL(k_rows1k_pair<8>, grid, dim3(128), 0, st, g, tw1k, info, CT, cs, K, cand, pairs, map);
+    hipLaunchKernelGGL((k_rows1k<8, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, blocks, map);
   else
-    hipLaunchKernelGGL(k_rows1k_pair<16>, grid, dim3(128), 0, st, g, tw1k, info, CT, cs, K, cand, pairs, map);
+    hipLaunchKernelGGL((k_rows1k<16, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, blocks, map);
 }
 
 }  // namespace qsmi
