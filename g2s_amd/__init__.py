@@ -281,12 +281,18 @@ class Wave:
 
 
 def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dims=None, gpu=0,
-            wave_size=256, tie_quantum=-1.0, stage_timing=False):
+            wave_size=256, tie_quantum=-1.0, stage_timing=False, record=0, lp=None, lp_shortlist=0,
+            lp_fallback_every=0, lp_fallback_mse=0.0, lp_audit=False):
     """In-memory QuickSampling (qsmi_qs_run_in_memory).
 
     Arrays are flat in G2S DataImage order (dimension 0 fastest, variable
     fastest of all): ti has prod(ti_dims)*nVar values, di prod(sim_dims)*nVar
-    (NaN = simulate). Returns (simulation, index, stats dict)."""
+    (NaN = simulate). Returns (simulation, index, stats dict).
+
+    record > 0: stats["pairs"] = (offsets [m,80,2], values [m,80], best [m])
+    of the first `record` path nodes (2-D, one continuous variable).
+    lp: a LearnedProposal with keys built for this TI (learned-proposal
+    form; the stats then carry its counters)."""
     ti = _f32(ti).reshape(-1)
     di = _f32(di).reshape(-1)
     var_types = _i32(var_types)
@@ -313,6 +319,20 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
     rq.waveSize = int(wave_size)
     rq.tieQuantum = float(tie_quantum)
     rq.stageTiming = int(bool(stage_timing))
+    rec = None
+    if record:
+        rec = (np.zeros((int(record), 80, 2), np.int32), np.zeros((int(record), 80), np.float32),
+               np.zeros(int(record), np.uint32))
+        rq.recordCap = int(record)
+        rq.recOffsets = rec[0].ctypes.data
+        rq.recValues = rec[1].ctypes.data
+        rq.recBest = rec[2].ctypes.data
+    if lp is not None:
+        rq.lp = lp.h
+        rq.lpShortlist = int(lp_shortlist)
+        rq.lpFallbackEvery = int(lp_fallback_every)
+        rq.lpFallbackMse = float(lp_fallback_mse)
+        rq.lpAudit = int(bool(lp_audit))
     sim = np.empty_like(di)
     idx = np.zeros(int(np.prod(sim_dims)), np.uint32)
     rs = _lib.QsResult()
@@ -323,4 +343,11 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
                  gather_ms=rs.stageMs[1], rows_ms=rs.stageMs[2], merge_ms=rs.stageMs[3], nodes=rs.nodes,
                  scanned_nodes=rs.scannedNodes, waves=rs.waves, search_ms=rs.phaseMs[0], levels_ms=rs.phaseMs[1],
                  upload_ms=rs.phaseMs[2], device_loop_ms=rs.phaseMs[3])
+    if rec is not None:
+        m = min(int(record), int(rs.nodes))
+        stats["pairs"] = (rec[0][:m], rec[1][:m], rec[2][:m])
+    if lp is not None:
+        stats.update(lp_proposed=rs.lpProposed, lp_fallbacks=rs.lpFallbacks, lp_overflows=rs.lpOverflows,
+                     lp_shortlist_sum=rs.lpShortlistSum, lp_audit_nodes=rs.lpAuditNodes,
+                     lp_audit_hits=rs.lpAuditHits, lp_audit_best_hits=rs.lpAuditBestHits)
     return sim, idx, stats

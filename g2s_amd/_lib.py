@@ -52,6 +52,16 @@ SIGNATURES = {
     "qsmi_offsets": (C.c_int, [C.c_int, u32p, C.c_int, vp, i32p, C.c_int, C.POINTER(C.c_int)]),
     "qsmi_path": (C.c_int, [C.c_size_t, u8p, C.c_uint, u32p, f32p, C.POINTER(C.c_uint)]),
     "qsmi_qs_run_in_memory": (C.c_int, [vp, vp]),
+    "qsmi_lp_create": (C.c_int, [C.c_int, C.c_uint, C.POINTER(vp)]),
+    "qsmi_lp_destroy": (C.c_int, [vp]),
+    "qsmi_lp_param_count": (C.c_int, [C.POINTER(C.c_longlong)]),
+    "qsmi_lp_get_params": (C.c_int, [vp, f32p]),
+    "qsmi_lp_set_params": (C.c_int, [vp, f32p]),
+    "qsmi_lp_gemm": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, f32p, f32p, f32p]),
+    "qsmi_lp_encode": (C.c_int, [vp, C.c_int, C.c_int, f32p, f32p]),
+    "qsmi_lp_loss_grad": (C.c_int, [vp, C.c_int, C.c_int, f32p, f32p, C.c_float, C.POINTER(C.c_float), vp]),
+    "qsmi_lp_train": (C.c_int, [vp, vp, vp, vp]),
+    "qsmi_lp_build_keys": (C.c_int, [vp, u32p, f32p, i32p]),
 }
 
 
@@ -101,6 +111,15 @@ class QsRequest(C.Structure):
         ("waveSize", C.c_int),
         ("tieQuantum", C.c_float),
         ("stageTiming", C.c_int),
+        ("recordCap", C.c_int),
+        ("recOffsets", vp),
+        ("recValues", vp),
+        ("recBest", vp),
+        ("lp", vp),
+        ("lpShortlist", C.c_int),
+        ("lpFallbackEvery", C.c_int),
+        ("lpFallbackMse", C.c_float),
+        ("lpAudit", C.c_int),
     ]
 
 
@@ -116,4 +135,34 @@ class QsResult(C.Structure):
         ("scannedNodes", C.c_longlong),
         ("waves", C.c_int),
         ("phaseMs", C.c_double * 4),
+        ("lpProposed", C.c_longlong),
+        ("lpFallbacks", C.c_longlong),
+        ("lpOverflows", C.c_longlong),
+        ("lpShortlistSum", C.c_longlong),
+        ("lpAuditNodes", C.c_longlong),
+        ("lpAuditHits", C.c_longlong),
+        ("lpAuditBestHits", C.c_longlong),
+    ]
+
+
+class LpTrainSet(C.Structure):
+    _fields_ = [
+        ("tiSize", C.c_uint * 2),
+        ("ti", vp),
+        ("canon", vp),
+        ("nPairs", C.c_int),
+        ("offsets", vp),
+        ("values", vp),
+        ("best", vp),
+    ]
+
+
+class LpTrainOpts(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int),
+        ("batch", C.c_int),
+        ("negatives", C.c_int),
+        ("lr", C.c_float),
+        ("invTau", C.c_float),
+        ("seed", C.c_uint),
     ]

@@ -384,6 +384,9 @@ int qsmi_ti_create(int gpu, int ndim, const unsigned* srcSize, const unsigned* f
         float* dst = pad.data() + (size_t)A * t->N + ((size_t)c2 * t->f[1] + c1) * t->f[0];
         memcpy(dst, src, sizeof(float) * t->s[0]);
       }
+  // spatial copy on the TI grid (exact rescoring of proposed positions)
+  TCHK(dalloc(&t->dSub, (size_t)nSub * t->S));
+  TCHK(hipMemcpy(t->dSub, subImages, (size_t)nSub * t->S * sizeof(float), hipMemcpyHostToDevice));
   float* dReal = nullptr;
   TCHK(dalloc(&dReal, pad.size()));
   TCHK(hipMemcpy(dReal, pad.data(), pad.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -424,6 +427,7 @@ int qsmi_ti_destroy(qsmi_ti* t) {
   if (t->stream) hipStreamSynchronize(t->stream);
   if (t->dYhat) (void)hipFree(t->dYhat);
   if (t->dYT) (void)hipFree(t->dYT);
+  if (t->dSub) (void)hipFree(t->dSub);
   for (int i = 0; i < 3; ++i)
     if (t->dTw[i]) hipFree(t->dTw[i]);
   if (t->stream) hipStreamDestroy(t->stream);

@@ -237,6 +237,10 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   }
   for (size_t c = 0; c < S * nVar; ++c)
     if (std::isnan(rq->ti[c])) return hfail(QSMI_ERR_UNSUPPORTED, "TI with NaN (cross-measurement layer) unsupported");
+  if ((rq->lp || rq->recordCap > 0) && !(D == 2 && nVar == 1 && rq->varTypes[0] == 0))
+    return hfail(QSMI_ERR_UNSUPPORTED, "learned proposal and pair record: 2-D, one continuous variable");
+  if (rq->recordCap > 0 && (!rq->recOffsets || !rq->recValues || !rq->recBest))
+    return hfail(QSMI_ERR_ARG, "record arrays missing");
   // categories and sub-variables (src/qs.cpp:1500-1537)
   std::vector<int> ncat;
   std::vector<float> catvals;
@@ -488,6 +492,22 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   if (!rc) rc = dev_upload(di, SD * nVar * sizeof(float), &dDi);
   if (!rc) rc = dev_upload(rs->index, SD * sizeof(unsigned), &dIdx);
   if (!rc) rc = dev_upload(rq->ti, S * nVar * sizeof(float), &dTi);
+  const size_t recCap = rq->recordCap > 0 ? std::min<size_t>((size_t)rq->recordCap, nPath) : 0;
+  void *dRecOff = nullptr, *dRecVal = nullptr, *dRecBest = nullptr;
+  if (!rc && recCap) {
+    rc = dev_upload(nullptr, recCap * 80 * 2 * sizeof(int), &dRecOff);
+    if (!rc) rc = dev_upload(nullptr, recCap * 80 * sizeof(float), &dRecVal);
+    if (!rc) rc = dev_upload(nullptr, recCap * sizeof(unsigned), &dRecBest);
+  }
+  R.recCap = (int)recCap;
+  R.dRecOff = (int*)dRecOff;
+  R.dRecVal = (float*)dRecVal;
+  R.dRecBest = (uint32_t*)dRecBest;
+  R.lp = rq->lp;
+  R.lpShortlist = rq->lpShortlist;
+  R.lpEvery = rq->lpFallbackEvery;
+  R.lpMse = rq->lpFallbackMse;
+  R.lpAudit = rq->lpAudit;
   const auto tUp = clk::now();
   double stageMs[4] = {0, 0, 0, 0};
   long long scanned = 0;
@@ -509,8 +529,15 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   if (!rc) {
     hipError_t e = hipMemcpy(di, dDi, SD * nVar * sizeof(float), hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(rs->index, dIdx, SD * sizeof(unsigned), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && recCap) {
+      e = hipMemcpy(rq->recOffsets, dRecOff, recCap * 80 * 2 * sizeof(int), hipMemcpyDeviceToHost);
+      if (e == hipSuccess) e = hipMemcpy(rq->recValues, dRecVal, recCap * 80 * sizeof(float), hipMemcpyDeviceToHost);
+      if (e == hipSuccess) e = hipMemcpy(rq->recBest, dRecBest, recCap * sizeof(unsigned), hipMemcpyDeviceToHost);
+    }
     if (e != hipSuccess) rc = hfail(QSMI_ERR_HIP, std::string("path loop download: ") + hipGetErrorString(e));
   }
+  long long lpc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (!rc && rq->lp) rc = qsmi::lp_read_counters(rq->lp, lpc);
   for (void* p : allocs) (void)hipFree(p);
   const auto tLoop = clk::now();
   double scanMs = stageMs[0] + stageMs[1] + stageMs[2] + stageMs[3];
@@ -531,6 +558,13 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   rs->scannedNodes = scanned;
   rs->nodes = (long long)nPath;
   rs->waves = waves;
+  rs->lpProposed = lpc[0];
+  rs->lpFallbacks = lpc[1];
+  rs->lpOverflows = lpc[2];
+  rs->lpShortlistSum = lpc[3];
+  rs->lpAuditNodes = lpc[4];
+  rs->lpAuditHits = lpc[5];
+  rs->lpAuditBestHits = lpc[6];
   return QSMI_OK;
 }
 

@@ -25,6 +25,7 @@ struct qsmi_ti {
   int nSub = 0;
   float2* dYhat = nullptr;
   float2* dYT = nullptr;  // column-contiguous copy (1000 x 1000 fast path)
+  float* dSub = nullptr;  // the sub-images on the TI grid, nSub x S
   bool fast1k = false;
   float2* dTw[3] = {nullptr, nullptr, nullptr};
   FftPlan plan[3];
@@ -148,7 +149,33 @@ struct ResidentLoop {
   float* dDi;
   uint32_t* dIndex;
   const float* dTi;
+  // learned proposal (nullptr: exact scan for every node)
+  qsmi_lp* lp = nullptr;
+  int lpShortlist = 0, lpEvery = 0, lpAudit = 0;
+  float lpMse = 0.f;
+  // training-pair record (device arrays; recCap 0: off)
+  int recCap = 0;
+  int* dRecOff = nullptr;
+  float* dRecVal = nullptr;
+  uint32_t* dRecBest = nullptr;
 };
+
+// learned proposal inside the path loop (qsmi_lp.hip)
+int lp_prepare(qsmi_lp* lp, const qsmi_ti* t, int maxNodes, int K, int shortlist, const int* active, int V,
+               int circular);
+void lp_propose(qsmi_lp* lp, const qsmi_ti* t, int V, int nN, const NodeInfo* info, const int* nbOff,
+                const float* nbVal, const float* di, int nVar, int sd0, int K, hipStream_t st);
+void lp_flags(qsmi_lp* lp, int nN, int w0, const int* order, const NodeInfo* info, int K, int every, float mseSigma2,
+              int audit, hipStream_t st);
+void lp_combine(qsmi_lp* lp, int nN, const NodeInfo* info, const Cand* candEx, int blocks, int rowsPerBlock, int K,
+                int audit, hipStream_t st);
+NodeInfo* lp_info_ex(qsmi_lp* lp);
+Cand* lp_cand(qsmi_lp* lp);
+// proposed, fallbacks, overflows, shortlist sum, audit nodes, audit hits, audit best hits, audit positions
+int lp_read_counters(qsmi_lp* lp, long long out[8]);
+void lp_record(int nN, int w0, const int* order, const NodeInfo* info, const int* nbOff, const float* di, int nVar,
+               int sd0, const uint32_t* outPos, int K, int cap, int* recOff, float* recVal, uint32_t* recBest,
+               hipStream_t st);
 
 // stageMs: [0] spectrum (+ dim1 inverse), [1] gather, [2] rows + select,
 // [3] merge + draw + write; filled when R.stageTiming.

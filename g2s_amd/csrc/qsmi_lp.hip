@@ -1,0 +1,1226 @@
+// Learned proposal for QuickSampling on gfx950 (the "learned-proposal QS"
+// form): a pattern encoder proposes a shortlist of TI positions for each path
+// node, the shortlist is rescored exactly, and the full FFT scan stays the
+// fallback and the judge.
+//
+//  * Two MLP towers, 240 -> 256 -> 256 -> 256 -> 128 (ReLU), fp32 master
+//    weights; every product runs on MFMA (v_mfma_f32_32x32x16_bf16: operands
+//    rounded to bf16, fp32 accumulation):
+//      key tower:   a TI position p: 80 slots, slot i = canonical offset d_i
+//                   (the 80 nearest non-centre offsets of the sorted offset
+//                   list): (d_i/16, (y(p + d_i) - mu)/sigma);
+//      query tower: a node's neighbours in the same slots: the neighbour at
+//                   offset d_i fills slot i with (d_i/16, (x - mu)/sigma);
+//                   neighbours off the stencil are left to the exact rescoring,
+//                   empty slots are zero.
+//    In this layout the exact mismatch over the stencil is a bilinear form of
+//    the two inputs, which the towers approximate.
+//  * score(node, p) = <q(node), k(p)>; training minimises the softmax
+//    cross-entropy of the exact best position against in-batch and random
+//    negatives (InfoNCE), Adam on the fp32 weights.
+//  * Proposal per node: a per-node score threshold from a fixed sample of
+//    key positions, one scoring GEMM whose epilogue appends the positions
+//    above the threshold, then the exact mismatch of every proposed position
+//    (direct sum over the neighbours) and the device selection rule.
+//
+// Lane layout of v_mfma_f32_32x32x16_bf16 (64 lanes, l = lane):
+//   A operand: A[l % 32][8 (l / 32) + e], e = 0..7
+//   B operand: B[8 (l / 32) + e][l % 32]
+//   result r:  C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32], r = 0..15
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "qsmi_internal.hpp"
+#include "qsmi_kernels.hpp"
+#include "qsmi_select.hpp"
+
+namespace qsmi {
+namespace lp {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int NB = 80;    // neighbour slots of the encoders
+constexpr int IN = 3 * NB;  // 240 input features
+constexpr int HID = 256;
+constexpr int EMB = 128;
+constexpr float OFFSET_SCALE = 1.f / 16.f;
+constexpr int SLOT_R = 8;                  // slot map covers |d0|, |d1| <= 8
+constexpr int SLOT_W = 2 * SLOT_R + 1;     // 17 x 17 map: canonical slot or -1
+
+constexpr int GT = 64;   // block tile (M and N)
+constexpr int GK = 32;   // K step
+constexpr int LDT = GK + 8;  // LDS row stride (bf16), 16-byte aligned fragments
+
+// ---------------------------------------------------------------------------
+// GEMM on MFMA: C[M,N] = op(A)[M,K] op(B)[K,N] (fp32 storage).
+//   TA: A stored [K][M] (row stride lda), else [M][K]
+//   TB: B stored [N][K] (row stride ldb), else [K][N]
+// Epilogue EPI_STORE: C = (acc ? C : 0) + AB (+ bias[n]) (ReLU when relu);
+//          EPI_SELECT: positions n with AB[m][n] >= thr[m] are appended to
+//          list[m][...] (counter cnt[m], capacity cap; overflow only counts).
+// 256 threads: four wavefronts, each a 32 x 32 quarter of the 64 x 64 tile.
+// ---------------------------------------------------------------------------
+enum { EPI_STORE = 0, EPI_SELECT = 1 };
+
+struct GemmArgs {
+  int M, N, K;
+  const float* A;
+  int lda;
+  const float* B;
+  int ldb;
+  float* C;
+  int ldc;
+  int accumulate;
+  const float* bias;
+  int relu;
+  // select epilogue
+  const float* thr;
+  uint32_t* list;
+  int* cnt;
+  int cap;
+};
+
+template <bool TA, bool TB, int EPI>
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+  __shared__ __attribute__((aligned(16))) __bf16 As[GT * LDT];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[GT * LDT];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int bm = blockIdx.y * GT, bn = blockIdx.x * GT;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+  for (int k0 = 0; k0 < g.K; k0 += GK) {
+    // A tile -> As[m][k]
+    if (!TA) {
+      const int m = tid >> 2, kb = (tid & 3) * 8;
+      const int gm = bm + m;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int gk = k0 + kb + e;
+        const float v = (gm < g.M && gk < g.K) ? g.A[(long long)gm * g.lda + gk] : 0.f;
+        As[m * LDT + kb + e] = (__bf16)v;
+      }
+    } else {
+      const int k = tid >> 3, mb = (tid & 7) * 8;
+      const int gk = k0 + k;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int gm = bm + mb + e;
+        const float v = (gm < g.M && gk < g.K) ? g.A[(long long)gk * g.lda + gm] : 0.f;
+        As[(mb + e) * LDT + k] = (__bf16)v;
+      }
+    }
+    // B tile -> Bs[n][k]
+    if (TB) {
+      const int n = tid >> 2, kb = (tid & 3) * 8;
+      const int gn = bn + n;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int gk = k0 + kb + e;
+        const float v = (gn < g.N && gk < g.K) ? g.B[(long long)gn * g.ldb + gk] : 0.f;
+        Bs[n * LDT + kb + e] = (__bf16)v;
+      }
+    } else {
+      const int k = tid >> 3, nb = (tid & 7) * 8;
+      const int gk = k0 + k;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int gn = bn + nb + e;
+        const float v = (gn < g.N && gk < g.K) ? g.B[(long long)gk * g.ldb + gn] : 0.f;
+        Bs[(nb + e) * LDT + k] = (__bf16)v;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < GK / 16; ++s) {
+      const bf16x8 a = *reinterpret_cast<const bf16x8*>(&As[(wm + (lane & 31)) * LDT + 16 * s + 8 * (lane >> 5)]);
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(&Bs[(wn + (lane & 31)) * LDT + 16 * s + 8 * (lane >> 5)]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const int gn = bn + wn + (lane & 31);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int gm = bm + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (gm >= g.M || gn >= g.N) continue;
+    float v = acc[r];
+    if (EPI == EPI_STORE) {
+      float* c = g.C + (long long)gm * g.ldc + gn;
+      if (g.accumulate) v += *c;
+      if (g.bias) v += g.bias[gn];
+      if (g.relu) v = fmaxf(v, 0.f);
+      *c = v;
+    } else {
+      if (v >= g.thr[gm]) {
+        const int slot = atomicAdd(&g.cnt[gm], 1);
+        if (slot < g.cap) g.list[(long long)gm * g.cap + slot] = (uint32_t)gn;
+      }
+    }
+  }
+}
+
+void gemm(bool ta, bool tb, const GemmArgs& a, hipStream_t st) {
+  dim3 grid((a.N + GT - 1) / GT, (a.M + GT - 1) / GT);
+  if (!ta && !tb) hipLaunchKernelGGL((k_gemm<false, false, EPI_STORE>), grid, dim3(256), 0, st, a);
+  else if (!ta && tb) hipLaunchKernelGGL((k_gemm<false, true, EPI_STORE>), grid, dim3(256), 0, st, a);
+  else if (ta && !tb) hipLaunchKernelGGL((k_gemm<true, false, EPI_STORE>), grid, dim3(256), 0, st, a);
+  else hipLaunchKernelGGL((k_gemm<true, true, EPI_STORE>), grid, dim3(256), 0, st, a);
+}
+
+void gemm_select(const GemmArgs& a, hipStream_t st) {
+  dim3 grid((a.N + GT - 1) / GT, (a.M + GT - 1) / GT);
+  hipLaunchKernelGGL((k_gemm<false, true, EPI_SELECT>), grid, dim3(256), 0, st, a);
+}
+
+GemmArgs gargs(int M, int N, int K, const float* A, int lda, const float* B, int ldb, float* C, int ldc) {
+  GemmArgs a;
+  memset(&a, 0, sizeof(a));
+  a.M = M;
+  a.N = N;
+  a.K = K;
+  a.A = A;
+  a.lda = lda;
+  a.B = B;
+  a.ldb = ldb;
+  a.C = C;
+  a.ldc = ldc;
+  return a;
+}
+
+// ---------------------------------------------------------------------------
+// element-wise kernels
+// ---------------------------------------------------------------------------
+// dY *= (Y > 0) (ReLU backward, Y = the layer's output after ReLU)
+__global__ void k_relu_bwd(float* __restrict__ dy, const float* __restrict__ y, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && !(y[i] > 0.f)) dy[i] = 0.f;
+}
+
+// db[n] (+)= sum_r dY[r][n]
+__global__ void k_colsum(const float* __restrict__ dy, int rows, int cols, float* __restrict__ db) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  float s = 0.f;
+  for (int r = 0; r < rows; ++r) s += dy[(long long)r * cols + c];
+  db[c] = s;
+}
+
+// Softmax cross-entropy of row b of G * invTau against column b (one block
+// per row); dG = invTau * (softmax - onehot) / B; loss[b] = lse - logit_b.
+__global__ void __launch_bounds__(256) k_softmax_ce(const float* __restrict__ Gm, int B, int C, float invTau,
+                                                    float* __restrict__ dG, float* __restrict__ loss) {
+  __shared__ float red[256];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const float* row = Gm + (long long)b * C;
+  float mx = -INFINITY;
+  for (int c = t; c < C; c += 256) mx = fmaxf(mx, row[c] * invTau);
+  red[t] = mx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) red[t] = fmaxf(red[t], red[t + s]);
+    __syncthreads();
+  }
+  mx = red[0];
+  __syncthreads();
+  float sum = 0.f;
+  for (int c = t; c < C; c += 256) sum += expf(row[c] * invTau - mx);
+  red[t] = sum;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) red[t] += red[t + s];
+    __syncthreads();
+  }
+  sum = red[0];
+  const float lse = mx + logf(sum);
+  for (int c = t; c < C; c += 256) {
+    const float p = expf(row[c] * invTau - lse);
+    dG[(long long)b * C + c] = invTau * (p - (c == b ? 1.f : 0.f)) / (float)B;
+  }
+  if (t == 0) loss[b] = lse - row[b] * invTau;
+}
+
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ gr, float* __restrict__ m,
+                       float* __restrict__ v, long long n, float lr, float b1, float b2, float eps, float c1,
+                       float c2) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float g = gr[i];
+  const float mi = b1 * m[i] + (1.f - b1) * g;
+  const float vi = b2 * v[i] + (1.f - b2) * g * g;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// features
+// ---------------------------------------------------------------------------
+// Key features of TI positions (pos == nullptr: positions first..first+n-1):
+// slot i = canonical offset d_i: (d0/16, d1/16, (y(p + d) - mu)/sigma),
+// all zero when p + d leaves the TI.
+__global__ void k_key_features(int n, const uint32_t* __restrict__ pos, long long first, const float* __restrict__ ti,
+                               int nVar, int s0, int s1, const int* __restrict__ canon, float mu, float invSigma,
+                               float* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)n * NB) return;
+  const long long r = i / NB;
+  const int slot = (int)(i - r * NB);
+  const long long p = pos ? (long long)pos[r] : first + r;
+  const int x0 = (int)(p % s0), x1 = (int)(p / s0);
+  const int d0 = canon[2 * slot], d1 = canon[2 * slot + 1];
+  const int t0 = x0 + d0, t1 = x1 + d1;
+  float* o = out + r * IN + 3 * slot;
+  if (t0 >= 0 && t0 < s0 && t1 >= 0 && t1 < s1) {
+    o[0] = d0 * OFFSET_SCALE;
+    o[1] = d1 * OFFSET_SCALE;
+    o[2] = (ti[((long long)t1 * s0 + t0) * nVar] - mu) * invSigma;
+  } else {
+    o[0] = o[1] = o[2] = 0.f;
+  }
+}
+
+__device__ __forceinline__ int slot_of(const int* __restrict__ slotMap, int d0, int d1) {
+  if (d0 < -SLOT_R || d0 > SLOT_R || d1 < -SLOT_R || d1 > SLOT_R) return -1;
+  return slotMap[(d1 + SLOT_R) * SLOT_W + (d0 + SLOT_R)];
+}
+
+// Query features of recorded pairs (one block per row): offsets
+// [pair][NB][2], values [pair][NB] (NaN = empty), placed in canonical slots.
+__global__ void __launch_bounds__(128) k_pair_features(const int* __restrict__ idx, const int* __restrict__ offs,
+                                                      const float* __restrict__ vals,
+                                                      const int* __restrict__ slotMap, float mu, float invSigma,
+                                                      float* __restrict__ out) {
+  const long long r = blockIdx.x;
+  const long long q = idx[r];
+  float* o = out + r * IN;
+  for (int i = threadIdx.x; i < IN; i += blockDim.x) o[i] = 0.f;
+  __syncthreads();
+  for (int j = threadIdx.x; j < NB; j += blockDim.x) {
+    const float v = vals[q * NB + j];
+    if (isnan(v)) continue;
+    const int d0 = offs[(q * NB + j) * 2], d1 = offs[(q * NB + j) * 2 + 1];
+    const int s = slot_of(slotMap, d0, d1);
+    if (s < 0) continue;
+    o[3 * s] = d0 * OFFSET_SCALE;
+    o[3 * s + 1] = d1 * OFFSET_SCALE;
+    o[3 * s + 2] = (v - mu) * invSigma;
+  }
+}
+
+// Query features of a path wave (one block per node): neighbours 1..nb-1
+// (entry 0 is the simulated cell itself), values from the simulation grid.
+__global__ void __launch_bounds__(128) k_wave_features(const NodeInfo* __restrict__ info, const int* __restrict__ nbOff,
+                                                      const float* __restrict__ di, int nVar, int sd0,
+                                                      const int* __restrict__ slotMap, float mu, float invSigma,
+                                                      float* __restrict__ out) {
+  const int node = blockIdx.x;
+  const NodeInfo ni = info[node];
+  float* o = out + (long long)node * IN;
+  for (int i = threadIdx.x; i < IN; i += blockDim.x) o[i] = 0.f;
+  __syncthreads();
+  const int c0 = (int)(ni.cell % (unsigned)sd0), c1 = (int)(ni.cell / (unsigned)sd0);
+  for (int j = 1 + threadIdx.x; j < ni.nb; j += blockDim.x) {
+    const int d0 = nbOff[(ni.nbStart + j) * 2], d1 = nbOff[(ni.nbStart + j) * 2 + 1];
+    const int s = slot_of(slotMap, d0, d1);
+    if (s < 0) continue;
+    const float v = di[((long long)(c1 + d1) * sd0 + (c0 + d0)) * nVar];
+    o[3 * s] = d0 * OFFSET_SCALE;
+    o[3 * s + 1] = d1 * OFFSET_SCALE;
+    o[3 * s + 2] = (v - mu) * invSigma;
+  }
+}
+
+// value of the r-th largest element (1-based) of each row: bitonic sort of
+// one row of n = 2^m <= 8192 floats per block, descending
+__global__ void __launch_bounds__(256) k_row_kth(const float* __restrict__ Gm, int n, int r, float* __restrict__ thr) {
+  __shared__ float v[8192];
+  const float* row = Gm + (long long)blockIdx.x * n;
+  for (int i = threadIdx.x; i < n; i += 256) v[i] = row[i];
+  __syncthreads();
+  for (int k = 2; k <= n; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < n; i += 256) {
+        const int l = i ^ j;
+        if (l > i) {
+          const bool desc = (i & k) == 0;
+          const float a = v[i], b = v[l];
+          if (desc ? (a < b) : (a > b)) {
+            v[i] = b;
+            v[l] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (threadIdx.x == 0) thr[blockIdx.x] = v[min(max(r, 1), n) - 1];
+}
+
+// ---------------------------------------------------------------------------
+// Exact rescoring of each node's proposed positions (one 256-thread block per
+// node; lanes run over the neighbours of one position at a time), then the
+// block's K best by the device rule -> cand[node][K]; best[node] = the
+// largest E found (-inf when none is valid).
+// ---------------------------------------------------------------------------
+template <int KT>
+__global__ void __launch_bounds__(256) k_rescore(const NodeInfo* __restrict__ info, const int* __restrict__ nbOff,
+                                                const float* __restrict__ nbVal, int V, const int* __restrict__ act,
+                                                const float* __restrict__ sub, long long S, int s0,
+                                                const uint32_t* __restrict__ list, const int* __restrict__ cnt, int cap,
+                                                int K, float* __restrict__ ebuf, Cand* __restrict__ cand,
+                                                float* __restrict__ best) {
+  __shared__ int doff[128];
+  __shared__ float dval[128 * 4];
+  __shared__ TopkScratch red;
+  __shared__ float bestW[4];
+  const int node = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const NodeInfo ni = info[node];
+  const int m = min(cnt[node], cap);
+  const int nb = min(ni.nb, 128);
+  float* eb = ebuf + (long long)node * cap;
+  for (int j = tid; j < nb; j += 256) {
+    doff[j] = nbOff[(ni.nbStart + j) * 2] + s0 * nbOff[(ni.nbStart + j) * 2 + 1];
+    for (int a = 0; a < V && a < 4; ++a) dval[j * 4 + a] = nbVal[(long long)(ni.nbStart + j) * V + a];
+  }
+  __syncthreads();
+  const bool usable = ni.nb > 0 && ni.nb <= 128 && V <= 4 && ni.lo[0] <= ni.hi[0] && ni.lo[1] <= ni.hi[1];
+  for (int c = wave; c < m; c += 4) {
+    const uint32_t p = list[(long long)node * cap + c];
+    const int p0 = (int)(p % (uint32_t)s0), p1 = (int)(p / (uint32_t)s0);
+    const bool valid = usable && p0 >= ni.lo[0] && p0 <= ni.hi[0] && p1 >= ni.lo[1] && p1 <= ni.hi[1];
+    float e = 0.f;
+    if (valid) {
+      for (int j = lane; j < nb; j += 64) {
+        const long long t = (long long)p + doff[j];
+        for (int a = 0; a < V; ++a) e += dval[j * 4 + a] * sub[(long long)act[a] * S + t];
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off);
+    if (lane == 0) eb[c] = valid ? e + ni.delta0 : -INFINITY;
+  }
+  __syncthreads();
+  float mk[KT];
+#pragma unroll
+  for (int i = 0; i < KT; ++i) mk[i] = -INFINITY;
+  for (int c = tid; c < m; c += 256) {
+    float x = eb[c];
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+      const float hi = fmaxf(mk[i], x);
+      x = fminf(mk[i], x);
+      mk[i] = hi;
+    }
+  }
+  const float tau2 = fmaxf(block_kth_key<KT>(mk, K, &red), -3.402823466e38f);
+  RankList<KT> rl;
+  rl.init();
+  float bestE = -INFINITY;
+  for (int c = tid; c < m; c += 256) {
+    const float e = eb[c];
+    bestE = fmaxf(bestE, e);
+    if (e >= tau2) {
+      const uint32_t p = list[(long long)node * cap + c];
+      rl.insert(e, tie_hash(ni.seedbits, p), p, e);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) bestE = fmaxf(bestE, __shfl_xor(bestE, off));
+  if (lane == 0) bestW[wave] = bestE;
+  rl.block_extract(K, cand + (long long)node * K, &red);  // its barriers order bestW
+  if (tid == 0) {
+    float b = bestW[0];
+    for (int w = 1; w < 4; ++w) b = fmaxf(b, bestW[w]);
+    best[node] = b;
+  }
+}
+
+}  // namespace lp
+}  // namespace qsmi
+
+// ===========================================================================
+// host side
+// ===========================================================================
+namespace qsmi {
+namespace lp {
+
+constexpr int kDin[4] = {IN, HID, HID, HID};
+constexpr int kDout[4] = {HID, HID, HID, EMB};
+constexpr long long kWoff[4] = {0, 240LL * 256 + 256, 240LL * 256 + 256 + 256LL * 256 + 256,
+                                240LL * 256 + 256 + 2 * (256LL * 256 + 256)};
+constexpr long long kTower = 240LL * 256 + 256 + 2 * (256LL * 256 + 256) + 256LL * 128 + 128;  // 226,176
+constexpr int kSample = 8192;   // key positions of the per-node threshold sample
+constexpr int kKeyChunk = 32768;
+
+__host__ __device__ constexpr long long boff(int l) { return kWoff[l] + (long long)kDin[l] * kDout[l]; }
+
+// gather rows of a [n][128] array
+__global__ void k_gather_rows(const float* __restrict__ src, const long long* __restrict__ rows, int n,
+                              float* __restrict__ dst) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)n * EMB) return;
+  const long long r = i / EMB;
+  dst[i] = src[rows[r] * EMB + (i - r * EMB)];
+}
+
+// block-mean of loss[0..B) added to acc[0]
+__global__ void k_loss_accum(const float* __restrict__ loss, int B, float w, float* __restrict__ acc) {
+  __shared__ float red[256];
+  float s = 0.f;
+  for (int i = threadIdx.x; i < B; i += 256) s += loss[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) acc[0] += red[0] * w;
+}
+
+}  // namespace lp
+}  // namespace qsmi
+
+using namespace qsmi;
+using namespace qsmi::lp;
+
+struct qsmi_lp {
+  int gpu = 0;
+  hipStream_t stream = nullptr;
+  float* dW = nullptr;   // parameters (query tower, key tower)
+  float* dG = nullptr;   // gradients
+  float* dM = nullptr;   // Adam moments
+  float* dV = nullptr;
+  int adamT = 0;
+  // scratch, grown on demand
+  char* ws = nullptr;
+  size_t wsCap = 0;
+  // keys of one TI
+  float* dKeys = nullptr;
+  float* dKsample = nullptr;
+  int* dSlot = nullptr;  // canonical slot map of the keys' stencil
+  long long keyS = 0;
+  int keyDims[2] = {0, 0};
+  float mu = 0.f, sigma = 1.f;
+  // proposal state of one wave
+  int pCap = 0, pMaxNodes = 0, pK = 0, pShort = 0;
+  char* pws = nullptr;
+  float *pXq = nullptr, *pH = nullptr, *pQ = nullptr, *pGs = nullptr, *pThr = nullptr, *pEbuf = nullptr,
+        *pBest = nullptr;
+  uint32_t* pList = nullptr;
+  int* pCnt = nullptr;
+  int* pFlag = nullptr;
+  Cand* pCand = nullptr;
+  NodeInfo* pInfoEx = nullptr;
+  int* pAct = nullptr;
+  // proposed, fallbacks, overflows, shortlist sum, audit nodes, audit hits, audit best hits, audit positions
+  unsigned long long* pCounters = nullptr;
+};
+
+namespace {
+
+struct Acts {
+  float* X;
+  float* H[3];
+  float* E;
+};
+
+// bump allocation in the handle's scratch buffer (device, 256-byte aligned)
+struct Bump {
+  qsmi_lp* lp;
+  size_t off = 0;
+  std::vector<std::pair<void**, size_t>> reqs;
+  template <class T>
+  void add(T** p, size_t n) {
+    reqs.push_back({(void**)p, ((n * sizeof(T) + 255) / 256) * 256});
+  }
+  int commit() {
+    size_t total = 0;
+    for (auto& r : reqs) total += r.second;
+    if (total > lp->wsCap) {
+      if (lp->ws) (void)hipFree(lp->ws);
+      lp->ws = nullptr;
+      lp->wsCap = 0;
+      if (hipMalloc(&lp->ws, total) != hipSuccess) return fail(QSMI_ERR_NOMEM, "learned proposal scratch");
+      lp->wsCap = total;
+    }
+    size_t o = 0;
+    for (auto& r : reqs) {
+      *r.first = lp->ws + o;
+      o += r.second;
+    }
+    return QSMI_OK;
+  }
+};
+
+void add_acts(Bump& b, Acts& a, int n) {
+  b.add(&a.X, (size_t)n * IN);
+  for (int l = 0; l < 3; ++l) b.add(&a.H[l], (size_t)n * HID);
+  b.add(&a.E, (size_t)n * EMB);
+}
+
+void forward(qsmi_lp* lp, int t, const Acts& a, int n, hipStream_t st) {
+  const float* W = lp->dW + t * kTower;
+  const float* in = a.X;
+  for (int l = 0; l < 4; ++l) {
+    float* out = l < 3 ? a.H[l] : a.E;
+    GemmArgs g = gargs(n, kDout[l], kDin[l], in, kDin[l], W + kWoff[l], kDout[l], out, kDout[l]);
+    g.bias = W + boff(l);
+    g.relu = l < 3;
+    gemm(false, false, g, st);
+    in = out;
+  }
+}
+
+// gradients of tower t's parameters from dE (n x 128); dHa, dHb: n x 256 scratch
+void backward(qsmi_lp* lp, int t, const Acts& a, float* dE, int n, float* dHa, float* dHb) {
+  const float* W = lp->dW + t * kTower;
+  float* G = lp->dG + t * kTower;
+  float* dOut = dE;
+  for (int l = 3; l >= 0; --l) {
+    const float* in = l > 0 ? a.H[l - 1] : a.X;
+    gemm(true, false, gargs(kDin[l], kDout[l], n, in, kDin[l], dOut, kDout[l], G + kWoff[l], kDout[l]), lp->stream);
+    hipLaunchKernelGGL(k_colsum, dim3((kDout[l] + 255) / 256), dim3(256), 0, lp->stream, dOut, n, kDout[l],
+                       G + boff(l));
+    if (l > 0) {
+      float* dIn = (l & 1) ? dHa : dHb;
+      gemm(false, true, gargs(n, kDin[l], kDout[l], dOut, kDout[l], W + kWoff[l], kDout[l], dIn, kDin[l]),
+           lp->stream);
+      const long long cnt = (long long)n * kDin[l];
+      hipLaunchKernelGGL(k_relu_bwd, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, lp->stream, dIn, a.H[l - 1],
+                         cnt);
+      dOut = dIn;
+    }
+  }
+}
+
+struct LossBufs {
+  float *Gm, *dG, *loss, *dQ, *dK, *dHa, *dHb;
+};
+
+void add_loss(Bump& b, LossBufs& L, int B, int C) {
+  b.add(&L.Gm, (size_t)B * C);
+  b.add(&L.dG, (size_t)B * C);
+  b.add(&L.loss, (size_t)B);
+  b.add(&L.dQ, (size_t)B * EMB);
+  b.add(&L.dK, (size_t)C * EMB);
+  b.add(&L.dHa, (size_t)std::max(B, C) * HID);
+  b.add(&L.dHb, (size_t)std::max(B, C) * HID);
+}
+
+// loss over (q, k) activations already forwarded; all gradients into lp->dG
+void loss_backward(qsmi_lp* lp, const Acts& q, const Acts& k, const LossBufs& L, int B, int C, float invTau) {
+  hipStream_t st = lp->stream;
+  gemm(false, true, gargs(B, C, EMB, q.E, EMB, k.E, EMB, L.Gm, C), st);
+  hipLaunchKernelGGL(k_softmax_ce, dim3(B), dim3(256), 0, st, L.Gm, B, C, invTau, L.dG, L.loss);
+  gemm(false, false, gargs(B, EMB, C, L.dG, C, k.E, EMB, L.dQ, EMB), st);
+  gemm(true, false, gargs(C, EMB, B, L.dG, C, q.E, EMB, L.dK, EMB), st);
+  backward(lp, 0, q, L.dQ, B, L.dHa, L.dHb);
+  backward(lp, 1, k, L.dK, C, L.dHa, L.dHb);
+}
+
+void adam_step(qsmi_lp* lp, float lr) {
+  const float b1 = 0.9f, b2 = 0.999f;
+  lp->adamT++;
+  const float c1 = 1.f - powf(b1, (float)lp->adamT), c2 = 1.f - powf(b2, (float)lp->adamT);
+  const long long n = 2 * kTower;
+  hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, lp->stream, lp->dW, lp->dG, lp->dM,
+                     lp->dV, n, lr, b1, b2, 1e-8f, c1, c2);
+}
+
+int hchk(hipError_t e, const char* what) {
+  if (e == hipSuccess) return QSMI_OK;
+  return fail(e == hipErrorOutOfMemory ? QSMI_ERR_NOMEM : QSMI_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+int upload(const T* src, size_t n, T** dst, std::vector<void*>& owned, hipStream_t st) {
+  int rc = hchk(hipMalloc((void**)dst, std::max<size_t>(n, 1) * sizeof(T)), "learned proposal upload");
+  if (rc) return rc;
+  owned.push_back(*dst);
+  if (n && src) return hchk(hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, st), "learned proposal upload");
+  return QSMI_OK;
+}
+
+void free_all(std::vector<void*>& v) {
+  for (void* p : v) (void)hipFree(p);
+  v.clear();
+}
+
+// canonical slot of every offset of the 17 x 17 window (-1: off the stencil)
+std::vector<int> slot_map(const int* canon) {
+  std::vector<int> m(SLOT_W * SLOT_W, -1);
+  for (int i = 0; i < NB; ++i) {
+    const int d0 = canon[2 * i], d1 = canon[2 * i + 1];
+    if (d0 >= -SLOT_R && d0 <= SLOT_R && d1 >= -SLOT_R && d1 <= SLOT_R) m[(d1 + SLOT_R) * SLOT_W + d0 + SLOT_R] = i;
+  }
+  return m;
+}
+
+void mean_sigma(const float* ti, size_t S, float* mu, float* sigma) {
+  double s = 0, s2 = 0;
+  for (size_t i = 0; i < S; ++i) {
+    s += ti[i];
+    s2 += (double)ti[i] * ti[i];
+  }
+  const double m = s / (double)S;
+  *mu = (float)m;
+  *sigma = (float)std::sqrt(std::max(1e-30, s2 / (double)S - m * m));
+}
+
+}  // namespace
+
+extern "C" {
+
+int qsmi_lp_param_count(long long* count) {
+  if (!count) return fail(QSMI_ERR_ARG, "null argument");
+  *count = 2 * kTower;
+  return QSMI_OK;
+}
+
+int qsmi_lp_create(int gpu, unsigned seed, qsmi_lp** out) {
+  if (!out) return fail(QSMI_ERR_ARG, "null argument");
+  int rc = hchk(hipSetDevice(gpu), "hipSetDevice");
+  if (rc) return rc;
+  qsmi_lp* lp = new qsmi_lp();
+  lp->gpu = gpu;
+  const long long n = 2 * kTower;
+  if ((rc = hchk(hipStreamCreateWithFlags(&lp->stream, hipStreamNonBlocking), "stream")) ||
+      (rc = hchk(hipMalloc(&lp->dW, n * sizeof(float)), "weights")) ||
+      (rc = hchk(hipMalloc(&lp->dG, n * sizeof(float)), "gradients")) ||
+      (rc = hchk(hipMalloc(&lp->dM, n * sizeof(float)), "moments")) ||
+      (rc = hchk(hipMalloc(&lp->dV, n * sizeof(float)), "moments"))) {
+    qsmi_lp_destroy(lp);
+    return rc;
+  }
+  // He-uniform weights, zero biases (mt19937 + uniform_real_distribution)
+  std::vector<float> w(n, 0.f);
+  std::mt19937 gen(seed);
+  for (int t = 0; t < 2; ++t)
+    for (int l = 0; l < 4; ++l) {
+      const float lim = std::sqrt(6.f / (float)kDin[l]) * (l == 3 ? 0.5f : 1.f);
+      std::uniform_real_distribution<float> U(-lim, lim);
+      float* W = w.data() + t * kTower + kWoff[l];
+      for (long long i = 0; i < (long long)kDin[l] * kDout[l]; ++i) W[i] = U(gen);
+    }
+  if ((rc = hchk(hipMemcpy(lp->dW, w.data(), n * sizeof(float), hipMemcpyHostToDevice), "weights")) ||
+      (rc = hchk(hipMemset(lp->dM, 0, n * sizeof(float)), "moments")) ||
+      (rc = hchk(hipMemset(lp->dV, 0, n * sizeof(float)), "moments"))) {
+    qsmi_lp_destroy(lp);
+    return rc;
+  }
+  *out = lp;
+  return QSMI_OK;
+}
+
+int qsmi_lp_destroy(qsmi_lp* lp) {
+  if (!lp) return QSMI_OK;
+  (void)hipSetDevice(lp->gpu);
+  if (lp->stream) (void)hipStreamSynchronize(lp->stream);
+  for (void* p : {(void*)lp->dW, (void*)lp->dG, (void*)lp->dM, (void*)lp->dV, (void*)lp->ws, (void*)lp->dKeys,
+                  (void*)lp->dKsample, (void*)lp->dSlot, (void*)lp->pws})
+    if (p) (void)hipFree(p);
+  if (lp->stream) (void)hipStreamDestroy(lp->stream);
+  delete lp;
+  return QSMI_OK;
+}
+
+int qsmi_lp_get_params(qsmi_lp* lp, float* out) {
+  if (!lp || !out) return fail(QSMI_ERR_ARG, "null argument");
+  (void)hipSetDevice(lp->gpu);
+  int rc = hchk(hipMemcpyAsync(out, lp->dW, 2 * kTower * sizeof(float), hipMemcpyDeviceToHost, lp->stream), "copy");
+  if (!rc) rc = hchk(hipStreamSynchronize(lp->stream), "sync");
+  return rc;
+}
+
+int qsmi_lp_set_params(qsmi_lp* lp, const float* in) {
+  if (!lp || !in) return fail(QSMI_ERR_ARG, "null argument");
+  (void)hipSetDevice(lp->gpu);
+  int rc = hchk(hipMemcpyAsync(lp->dW, in, 2 * kTower * sizeof(float), hipMemcpyHostToDevice, lp->stream), "copy");
+  if (!rc) rc = hchk(hipStreamSynchronize(lp->stream), "sync");
+  lp->adamT = 0;
+  if (!rc) rc = hchk(hipMemsetAsync(lp->dM, 0, 2 * kTower * sizeof(float), lp->stream), "moments");
+  if (!rc) rc = hchk(hipMemsetAsync(lp->dV, 0, 2 * kTower * sizeof(float), lp->stream), "moments");
+  if (!rc) rc = hchk(hipStreamSynchronize(lp->stream), "sync");
+  return rc;
+}
+
+int qsmi_lp_gemm(qsmi_lp* lp, int transA, int transB, int M, int N, int K, const float* A, const float* B,
+                 float* C) {
+  if (!lp || !A || !B || !C) return fail(QSMI_ERR_ARG, "null argument");
+  if (M < 1 || N < 1 || K < 1) return fail(QSMI_ERR_ARG, "empty product");
+  (void)hipSetDevice(lp->gpu);
+  std::vector<void*> own;
+  float *dA, *dB, *dC;
+  int rc = upload(A, (size_t)M * K, &dA, own, lp->stream);
+  if (!rc) rc = upload(B, (size_t)K * N, &dB, own, lp->stream);
+  if (!rc) rc = upload<float>(nullptr, (size_t)M * N, &dC, own, lp->stream);
+  if (!rc) {
+    GemmArgs g = gargs(M, N, K, dA, transA ? M : K, dB, transB ? K : N, dC, N);
+    gemm(transA != 0, transB != 0, g, lp->stream);
+    rc = hchk(hipGetLastError(), "gemm");
+    if (!rc) rc = hchk(hipMemcpyAsync(C, dC, (size_t)M * N * sizeof(float), hipMemcpyDeviceToHost, lp->stream), "copy");
+    if (!rc) rc = hchk(hipStreamSynchronize(lp->stream), "sync");
+  }
+  (void)hipStreamSynchronize(lp->stream);
+  free_all(own);
+  return rc;
+}
+
+int qsmi_lp_encode(qsmi_lp* lp, int tower, int n, const float* features, float* embeddings) {
+  if (!lp || !features || !embeddings) return fail(QSMI_ERR_ARG, "null argument");
+  if (tower < 0 || tower > 1 || n < 1) return fail(QSMI_ERR_ARG, "tower must be 0 or 1, n >= 1");
+  (void)hipSetDevice(lp->gpu);
+  Bump b{lp};
+  Acts a;
+  add_acts(b, a, n);
+  int rc = b.commit();
+  if (rc) return rc;
+  rc = hchk(hipMemcpyAsync(a.X, features, (size_t)n * IN * sizeof(float), hipMemcpyHostToDevice, lp->stream), "copy");
+  if (rc) return rc;
+  forward(lp, tower, a, n, lp->stream);
+  rc = hchk(hipGetLastError(), "encode");
+  if (!rc) rc = hchk(hipMemcpyAsync(embeddings, a.E, (size_t)n * EMB * sizeof(float), hipMemcpyDeviceToHost, lp->stream), "copy");
+  if (!rc) rc = hchk(hipStreamSynchronize(lp->stream), "sync");
+  return rc;
+}
+
+int qsmi_lp_loss_grad(qsmi_lp* lp, int B, int C, const float* qfeat, const float* kfeat, float invTau, float* loss,
+                      float* grad) {
+  if (!lp || !qfeat || !kfeat || !loss) return fail(QSMI_ERR_ARG, "null argument");
+  if (B < 1 || C < B || C > 65536) return fail(QSMI_ERR_ARG, "need 1 <= B <= C <= 65536");
+  (void)hipSetDevice(lp->gpu);
+  Bump b{lp};
+  Acts q, k;
+  LossBufs L;
+  add_acts(b, q, B);
+  add_acts(b, k, C);
+  add_loss(b, L, B, C);
+  int rc = b.commit();
+  if (rc) return rc;
+  hipStream_t st = lp->stream;
+  rc = hchk(hipMemcpyAsync(q.X, qfeat, (size_t)B * IN * sizeof(float), hipMemcpyHostToDevice, st), "copy");
+  if (!rc) rc = hchk(hipMemcpyAsync(k.X, kfeat, (size_t)C * IN * sizeof(float), hipMemcpyHostToDevice, st), "copy");
+  if (rc) return rc;
+  forward(lp, 0, q, B, st);
+  forward(lp, 1, k, C, st);
+  loss_backward(lp, q, k, L, B, C, invTau);
+  rc = hchk(hipGetLastError(), "loss");
+  std::vector<float> lv(B);
+  if (!rc) rc = hchk(hipMemcpyAsync(lv.data(), L.loss, B * sizeof(float), hipMemcpyDeviceToHost, st), "copy");
+  if (!rc && grad) rc = hchk(hipMemcpyAsync(grad, lp->dG, 2 * kTower * sizeof(float), hipMemcpyDeviceToHost, st), "copy");
+  if (!rc) rc = hchk(hipStreamSynchronize(st), "sync");
+  if (!rc) {
+    double s = 0;
+    for (float x : lv) s += x;
+    *loss = (float)(s / B);
+  }
+  return rc;
+}
+
+int qsmi_lp_train(qsmi_lp* lp, const qsmi_lp_train_set* set, const qsmi_lp_train_opts* opts, float* lossOut) {
+  if (!lp || !set || !opts) return fail(QSMI_ERR_ARG, "null argument");
+  if (!set->ti || !set->canon || !set->offsets || !set->values || !set->best)
+    return fail(QSMI_ERR_ARG, "null training set array");
+  const int B = opts->batch > 0 ? opts->batch : 256;
+  const int R = opts->negatives > 0 ? opts->negatives : 768;
+  const int C = B + R;
+  const float lr = opts->lr > 0.f ? opts->lr : 1e-3f;
+  const float invTau = opts->invTau > 0.f ? opts->invTau : 10.f;
+  const int steps = opts->steps;
+  if (steps < 0 || B > 4096 || C > 16384) return fail(QSMI_ERR_ARG, "batch/negatives out of range");
+  const size_t S = (size_t)set->tiSize[0] * set->tiSize[1];
+  if (S < 2) return fail(QSMI_ERR_ARG, "empty TI");
+  std::vector<int> usable;
+  for (int i = 0; i < set->nPairs; ++i)
+    if (set->best[i] != 0xFFFFFFFFu && set->best[i] < S) usable.push_back(i);
+  if (usable.empty()) return fail(QSMI_ERR_ARG, "no usable training pair");
+  (void)hipSetDevice(lp->gpu);
+  hipStream_t st = lp->stream;
+  float mu, sigma;
+  mean_sigma(set->ti, S, &mu, &sigma);
+  // every step's query indices and candidate positions, drawn up front
+  std::mt19937 gen(opts->seed);
+  std::uniform_int_distribution<int> pickPair(0, (int)usable.size() - 1);
+  std::uniform_int_distribution<unsigned> pickPos(0, (unsigned)(S - 1));
+  std::vector<int> qidx((size_t)steps * B);
+  std::vector<uint32_t> cpos((size_t)steps * C);
+  for (int s = 0; s < steps; ++s) {
+    for (int b = 0; b < B; ++b) {
+      const int pi = usable[pickPair(gen)];
+      qidx[(size_t)s * B + b] = pi;
+      cpos[(size_t)s * C + b] = set->best[pi];
+    }
+    for (int r = 0; r < R; ++r) cpos[(size_t)s * C + B + r] = pickPos(gen);
+  }
+  std::vector<void*> own;
+  float *dTi, *dVal, *dAcc;
+  int *dCanon, *dOff, *dQidx;
+  uint32_t* dCpos;
+  const int nBlk = (steps + 49) / 50;
+  int rc = upload(set->ti, S, &dTi, own, st);
+  if (!rc) rc = upload(set->canon, (size_t)2 * NB, &dCanon, own, st);
+  const std::vector<int> smap = slot_map(set->canon);
+  int* dSlot;
+  if (!rc) rc = upload(smap.data(), smap.size(), &dSlot, own, st);
+  if (!rc) rc = upload(set->offsets, (size_t)set->nPairs * NB * 2, &dOff, own, st);
+  if (!rc) rc = upload(set->values, (size_t)set->nPairs * NB, &dVal, own, st);
+  if (!rc) rc = upload(qidx.data(), qidx.size(), &dQidx, own, st);
+  if (!rc) rc = upload(cpos.data(), cpos.size(), &dCpos, own, st);
+  if (!rc) rc = upload<float>(nullptr, (size_t)std::max(nBlk, 1), &dAcc, own, st);
+  if (!rc) rc = hchk(hipMemsetAsync(dAcc, 0, std::max(nBlk, 1) * sizeof(float), st), "memset");
+  Bump b{lp};
+  Acts q, k;
+  LossBufs L;
+  add_acts(b, q, B);
+  add_acts(b, k, C);
+  add_loss(b, L, B, C);
+  if (!rc) rc = b.commit();
+  if (rc) {
+    (void)hipStreamSynchronize(st);
+    free_all(own);
+    return rc;
+  }
+  const float invS = 1.f / sigma;
+  for (int s = 0; s < steps; ++s) {
+    hipLaunchKernelGGL(k_pair_features, dim3(B), dim3(128), 0, st, dQidx + (size_t)s * B, dOff, dVal, dSlot, mu, invS,
+                       q.X);
+    hipLaunchKernelGGL(k_key_features, dim3((C * NB + 255) / 256), dim3(256), 0, st, C, dCpos + (size_t)s * C, 0LL,
+                       dTi, 1, (int)set->tiSize[0], (int)set->tiSize[1], dCanon, mu, invS, k.X);
+    forward(lp, 0, q, B, st);
+    forward(lp, 1, k, C, st);
+    loss_backward(lp, q, k, L, B, C, invTau);
+    adam_step(lp, lr);
+    hipLaunchKernelGGL(k_loss_accum, dim3(1), dim3(256), 0, st, L.loss, B,
+                       1.f / (float)B / (float)std::min(50, steps - (s / 50) * 50), dAcc + s / 50);
+  }
+  rc = hchk(hipGetLastError(), "training step");
+  std::vector<float> acc(std::max(nBlk, 1));
+  if (!rc) rc = hchk(hipMemcpyAsync(acc.data(), dAcc, acc.size() * sizeof(float), hipMemcpyDeviceToHost, st), "copy");
+  if (!rc) rc = hchk(hipStreamSynchronize(st), "sync");
+  (void)hipStreamSynchronize(st);
+  free_all(own);
+  if (!rc && lossOut)
+    for (int i = 0; i < nBlk; ++i) lossOut[i] = acc[i];
+  return rc;
+}
+
+int qsmi_lp_build_keys(qsmi_lp* lp, const unsigned* tiSize, const float* ti, const int* canon) {
+  if (!lp || !tiSize || !ti || !canon) return fail(QSMI_ERR_ARG, "null argument");
+  const long long S = (long long)tiSize[0] * tiSize[1];
+  if (S < kSample || S > 0x7FFFFFFFLL) return fail(QSMI_ERR_ARG, "TI must hold between 8192 and 2^31 cells");
+  (void)hipSetDevice(lp->gpu);
+  hipStream_t st = lp->stream;
+  (void)hipStreamSynchronize(st);
+  if (lp->dKeys) (void)hipFree(lp->dKeys);
+  if (lp->dKsample) (void)hipFree(lp->dKsample);
+  lp->dKeys = lp->dKsample = nullptr;
+  lp->keyS = 0;
+  int rc = hchk(hipMalloc(&lp->dKeys, (size_t)S * EMB * sizeof(float)), "keys");
+  if (!rc) rc = hchk(hipMalloc(&lp->dKsample, (size_t)kSample * EMB * sizeof(float)), "keys");
+  if (!rc && !lp->dSlot) rc = hchk(hipMalloc(&lp->dSlot, SLOT_W * SLOT_W * sizeof(int)), "slots");
+  if (rc) return rc;
+  mean_sigma(ti, (size_t)S, &lp->mu, &lp->sigma);
+  std::vector<void*> own;
+  float* dTi;
+  int* dCanon;
+  long long* dRows;
+  std::vector<long long> rows(kSample);
+  for (int i = 0; i < kSample; ++i) rows[i] = ((long long)i * S + S / 2) / kSample;
+  rc = upload(ti, (size_t)S, &dTi, own, st);
+  if (!rc) rc = upload(canon, (size_t)2 * NB, &dCanon, own, st);
+  if (!rc) rc = upload(rows.data(), rows.size(), &dRows, own, st);
+  if (!rc) {
+    const std::vector<int> smap = slot_map(canon);
+    rc = hchk(hipMemcpyAsync(lp->dSlot, smap.data(), smap.size() * sizeof(int), hipMemcpyHostToDevice, st), "slots");
+  }
+  Bump b{lp};
+  Acts a;
+  add_acts(b, a, kKeyChunk);
+  if (!rc) rc = b.commit();
+  if (!rc) {
+    for (long long first = 0; first < S; first += kKeyChunk) {
+      const int n = (int)std::min<long long>(kKeyChunk, S - first);
+      hipLaunchKernelGGL(k_key_features, dim3((unsigned)(((long long)n * NB + 255) / 256)), dim3(256), 0, st, n,
+                         (const uint32_t*)nullptr, first, dTi, 1, (int)tiSize[0], (int)tiSize[1], dCanon, lp->mu,
+                         1.f / lp->sigma, a.X);
+      Acts o = a;
+      o.E = lp->dKeys + first * EMB;
+      forward(lp, 1, o, n, st);
+    }
+    hipLaunchKernelGGL(k_gather_rows, dim3((kSample * EMB + 255) / 256), dim3(256), 0, st, lp->dKeys, dRows, kSample,
+                       lp->dKsample);
+    rc = hchk(hipGetLastError(), "keys");
+  }
+  if (!rc) rc = hchk(hipStreamSynchronize(st), "sync");
+  (void)hipStreamSynchronize(st);
+  free_all(own);
+  if (!rc) {
+    lp->keyS = S;
+    lp->keyDims[0] = (int)tiSize[0];
+    lp->keyDims[1] = (int)tiSize[1];
+  }
+  return rc;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// learned proposal inside the path loop (one wave at a time, device stream)
+// ===========================================================================
+namespace qsmi {
+namespace lp {
+
+// Route each node: 0 = drew from the marginal, 1 = exact scan (fallback),
+// 2 = proposal. infoEx: the wave's NodeInfo with the valid box emptied for
+// the nodes the exact scan must skip.
+__global__ void k_lp_flags(int nN, int w0, const int* __restrict__ order, const NodeInfo* __restrict__ info,
+                           const int* __restrict__ cnt, int cap, const float* __restrict__ best,
+                           const Cand* __restrict__ cand, int K, int every, float mseLimit, int audit,
+                           int* __restrict__ flag, NodeInfo* __restrict__ infoEx,
+                           unsigned long long* __restrict__ counters) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nN) return;
+  NodeInfo ni = info[q];
+  int f = 0;
+  if (ni.nb > 0) {
+    const int i = order[w0 + q];
+    const bool overflow = cnt[q] > cap;
+    const bool few = !(cand[(long long)q * K + K - 1].key > -INFINITY);
+    const bool periodic = every > 0 && (i % every) == 0;
+    const float mse = -best[q] / (float)max(1, ni.nb - 1);
+    const bool poor = mseLimit > 0.f && !(mse <= mseLimit);
+    f = (overflow || few || periodic || poor) ? 1 : 2;
+    if (f == 1) {
+      atomicAdd(&counters[1], 1ull);
+      if (overflow) atomicAdd(&counters[2], 1ull);
+    } else {
+      atomicAdd(&counters[0], 1ull);
+      atomicAdd(&counters[3], (unsigned long long)min(cnt[q], cap));
+    }
+  }
+  flag[q] = f;
+  if (!(f == 1 || (f == 2 && audit)))
+    for (int d = 0; d < 3; ++d) {  // empty valid box: every scan kernel skips the node
+      ni.lo[d] = 1;
+      ni.hi[d] = 0;
+    }
+  infoEx[q] = ni;
+}
+
+// Fallback nodes: cand[node] = the K best of the exact scan's block
+// candidates. Audit of proposal nodes: how many of the exact K best the
+// shortlist held.
+template <int KT>
+__global__ void __launch_bounds__(256) k_lp_combine(const int* __restrict__ flag, const NodeInfo* __restrict__ info,
+                                                   const Cand* __restrict__ candEx, int blocks, int rowsPerBlock,
+                                                   int K, int audit,
+                                                   const uint32_t* __restrict__ list, const int* __restrict__ cnt,
+                                                   int cap, Cand* __restrict__ cand,
+                                                   unsigned long long* __restrict__ counters) {
+  __shared__ TopkScratch red;
+  __shared__ Cand ex[kMaxK];
+  __shared__ int found[kMaxK];
+  const int node = blockIdx.x, tid = threadIdx.x;
+  const int f = flag[node];
+  if (f == 0 || (f == 2 && !audit)) return;
+  // blocks holding candidates: all of them, or (generic rows kernel) only
+  // those covering the node's valid rows, as the merge stage counts them
+  int nblk = blocks;
+  if (rowsPerBlock > 0) {
+    const NodeInfo ni = info[node];
+    const int n1v = ni.hi[1] - ni.lo[1] + 1, n2v = ni.hi[2] - ni.lo[2] + 1, n0v = ni.hi[0] - ni.lo[0] + 1;
+    const long long nrv = (n1v > 0 && n2v > 0 && n0v > 0) ? (long long)n1v * n2v : 0;
+    nblk = (int)((nrv + rowsPerBlock - 1) / rowsPerBlock);
+  }
+  RankList<KT> rl;
+  rl.init();
+  const Cand* ce = candEx + (long long)node * blocks * K;
+  for (int i = tid; i < nblk * K; i += 256) {
+    const Cand c = ce[i];
+    if (c.key > -INFINITY) rl.insert(c.key, c.h, c.pos, c.val);
+  }
+  rl.block_extract(K, f == 1 ? cand + (long long)node * K : ex, &red);
+  if (f == 1) return;
+  // audit
+  if (tid < K) found[tid] = 0;
+  __syncthreads();
+  const int m = min(cnt[node], cap);
+  for (int i = tid; i < m; i += 256) {
+    const uint32_t p = list[(long long)node * cap + i];
+    for (int r = 0; r < K; ++r)
+      if (ex[r].pos == p) found[r] = 1;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int hits = 0, valid = 0;
+    for (int r = 0; r < K; ++r)
+      if (ex[r].pos != 0xFFFFFFFFu) {
+        valid++;
+        hits += found[r];
+      }
+    atomicAdd(&counters[4], 1ull);
+    atomicAdd(&counters[5], (unsigned long long)hits);
+    atomicAdd(&counters[6], (unsigned long long)((valid > 0 && found[0]) ? 1 : 0));
+    atomicAdd(&counters[7], (unsigned long long)valid);
+  }
+}
+
+// training-pair record of one wave (block per node)
+__global__ void __launch_bounds__(128) k_lp_record(int w0, const int* __restrict__ order,
+                                                  const NodeInfo* __restrict__ info, const int* __restrict__ nbOff,
+                                                  const float* __restrict__ di, int nVar, int sd0,
+                                                  const uint32_t* __restrict__ outPos, int K, int cap,
+                                                  int* __restrict__ recOff, float* __restrict__ recVal,
+                                                  uint32_t* __restrict__ recBest) {
+  const int q = blockIdx.x;
+  const int i = order[w0 + q];
+  if (i >= cap) return;
+  const NodeInfo ni = info[q];
+  const int c0 = (int)(ni.cell % (unsigned)sd0), c1 = (int)(ni.cell / (unsigned)sd0);
+  for (int slot = threadIdx.x; slot < NB; slot += blockDim.x) {
+    const int j = slot + 1;
+    int d0 = 0, d1 = 0;
+    float v = NAN;
+    if (ni.nb > 0 && j < ni.nb) {
+      d0 = nbOff[(ni.nbStart + j) * 2];
+      d1 = nbOff[(ni.nbStart + j) * 2 + 1];
+      v = di[((long long)(c1 + d1) * sd0 + (c0 + d0)) * nVar];
+    }
+    recOff[((long long)i * NB + slot) * 2] = d0;
+    recOff[((long long)i * NB + slot) * 2 + 1] = d1;
+    recVal[(long long)i * NB + slot] = v;
+  }
+  if (threadIdx.x == 0) recBest[i] = ni.nb > 0 ? outPos[(long long)q * K] : 0xFFFFFFFFu;
+}
+
+}  // namespace lp
+
+int lp_prepare(qsmi_lp* lp, const qsmi_ti* t, int maxNodes, int K, int shortlist, const int* active, int V,
+               int circular) {
+  using namespace lp;
+  if (circular) return fail(QSMI_ERR_UNSUPPORTED, "learned proposal: circular TI unsupported");
+  if (!lp->dKeys || lp->keyS != t->S || lp->keyDims[0] != t->s[0] || lp->keyDims[1] != t->s[1])
+    return fail(QSMI_ERR_ARG, "learned proposal: keys were not built for this TI");
+  if (t->D != 2) return fail(QSMI_ERR_UNSUPPORTED, "learned proposal: 2-D only");
+  if (V > 4) return fail(QSMI_ERR_UNSUPPORTED, "learned proposal: at most 4 active sub-variables");
+  const int L = shortlist > 0 ? shortlist : 4096;
+  const int cap = 2 * L;
+  if (!(lp->pws && lp->pMaxNodes >= maxNodes && lp->pK == K && lp->pShort == L)) {
+  if (lp->pws) (void)hipFree(lp->pws);
+  lp->pws = nullptr;
+  const size_t n = (size_t)std::max(maxNodes, 1);
+  std::vector<std::pair<void**, size_t>> req = {
+      {(void**)&lp->pXq, n * IN * 4},          {(void**)&lp->pH, 3 * n * HID * 4},
+      {(void**)&lp->pQ, n * EMB * 4},          {(void**)&lp->pGs, n * kSample * 4},
+      {(void**)&lp->pThr, n * 4},              {(void**)&lp->pList, n * cap * 4},
+      {(void**)&lp->pCnt, n * 4},              {(void**)&lp->pEbuf, n * cap * 4},
+      {(void**)&lp->pBest, n * 4},             {(void**)&lp->pFlag, n * 4},
+      {(void**)&lp->pCand, n * K * sizeof(Cand)}, {(void**)&lp->pInfoEx, n * sizeof(NodeInfo)},
+      {(void**)&lp->pCounters, 8 * 8},         {(void**)&lp->pAct, 4 * 4}};
+  size_t total = 0;
+  for (auto& r : req) total += (r.second + 255) / 256 * 256;
+  if (hipMalloc(&lp->pws, total) != hipSuccess) return fail(QSMI_ERR_NOMEM, "learned proposal: wave buffers");
+  size_t o = 0;
+  for (auto& r : req) {
+    *r.first = lp->pws + o;
+    o += (r.second + 255) / 256 * 256;
+  }
+  lp->pMaxNodes = (int)n;
+  lp->pK = K;
+  lp->pShort = L;
+  lp->pCap = cap;
+  }
+  int act[4] = {0, 0, 0, 0};
+  for (int a = 0; a < V; ++a) act[a] = active[a];
+  if (hipMemcpy(lp->pAct, act, sizeof(act), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(lp->pCounters, 0, 8 * 8) != hipSuccess)
+    return fail(QSMI_ERR_HIP, "learned proposal: set-up copies");
+  return QSMI_OK;
+}
+
+void lp_propose(qsmi_lp* lp, const qsmi_ti* t, int V, int nN, const NodeInfo* info, const int* nbOff,
+                const float* nbVal, const float* di, int nVar, int sd0, int K, hipStream_t st) {
+  using namespace lp;
+  Acts a;
+  a.X = lp->pXq;
+  for (int l = 0; l < 3; ++l) a.H[l] = lp->pH + (size_t)l * lp->pMaxNodes * HID;
+  a.E = lp->pQ;
+  hipLaunchKernelGGL(k_wave_features, dim3(nN), dim3(128), 0, st, info, nbOff, di, nVar, sd0, lp->dSlot, lp->mu,
+                     1.f / lp->sigma, a.X);
+  forward(lp, 0, a, nN, st);
+  // per-node threshold: the r-th best sampled score, r = L * sample / S
+  gemm(false, true, gargs(nN, kSample, EMB, a.E, EMB, lp->dKsample, EMB, lp->pGs, kSample), st);
+  const int r = (int)std::max<long long>(1, ((long long)lp->pShort * kSample + lp->keyS - 1) / lp->keyS);
+  hipLaunchKernelGGL(k_row_kth, dim3(nN), dim3(256), 0, st, lp->pGs, kSample, r, lp->pThr);
+  (void)hipMemsetAsync(lp->pCnt, 0, (size_t)nN * sizeof(int), st);
+  GemmArgs g = gargs(nN, (int)lp->keyS, EMB, a.E, EMB, lp->dKeys, EMB, nullptr, 0);
+  g.thr = lp->pThr;
+  g.list = lp->pList;
+  g.cnt = lp->pCnt;
+  g.cap = lp->pCap;
+  gemm_select(g, st);
+  const long long S = t->S;
+  if (K <= 2)
+    hipLaunchKernelGGL(k_rescore<2>, dim3(nN), dim3(256), 0, st, info, nbOff, nbVal, V, lp->pAct, t->dSub, S, t->s[0],
+                       lp->pList, lp->pCnt, lp->pCap, K, lp->pEbuf, lp->pCand, lp->pBest);
+  else if (K <= 4)
+    hipLaunchKernelGGL(k_rescore<4>, dim3(nN), dim3(256), 0, st, info, nbOff, nbVal, V, lp->pAct, t->dSub, S, t->s[0],
+                       lp->pList, lp->pCnt, lp->pCap, K, lp->pEbuf, lp->pCand, lp->pBest);
+  else if (K <= 8)
+    hipLaunchKernelGGL(k_rescore<8>, dim3(nN), dim3(256), 0, st, info, nbOff, nbVal, V, lp->pAct, t->dSub, S, t->s[0],
+                       lp->pList, lp->pCnt, lp->pCap, K, lp->pEbuf, lp->pCand, lp->pBest);
+  else
+    hipLaunchKernelGGL(k_rescore<16>, dim3(nN), dim3(256), 0, st, info, nbOff, nbVal, V, lp->pAct, t->dSub, S, t->s[0],
+                       lp->pList, lp->pCnt, lp->pCap, K, lp->pEbuf, lp->pCand, lp->pBest);
+}
+
+void lp_flags(qsmi_lp* lp, int nN, int w0, const int* order, const NodeInfo* info, int K, int every, float mseSigma2,
+              int audit, hipStream_t st) {
+  using namespace lp;
+  hipLaunchKernelGGL(k_lp_flags, dim3((nN + 127) / 128), dim3(128), 0, st, nN, w0, order, info, lp->pCnt, lp->pCap,
+                     lp->pBest, lp->pCand, K, every, mseSigma2 > 0.f ? mseSigma2 * lp->sigma * lp->sigma : 0.f, audit,
+                     lp->pFlag, lp->pInfoEx, lp->pCounters);
+}
+
+void lp_combine(qsmi_lp* lp, int nN, const NodeInfo* info, const Cand* candEx, int blocks, int rowsPerBlock, int K,
+                int audit, hipStream_t st) {
+  using namespace lp;
+#define QSMI_LP_COMBINE(KT)                                                                                          \
+  hipLaunchKernelGGL(k_lp_combine<KT>, dim3(nN), dim3(256), 0, st, lp->pFlag, info, candEx, blocks, rowsPerBlock, K, \
+                     audit,                                                                                          \
+                     lp->pList, lp->pCnt, lp->pCap, lp->pCand, lp->pCounters)
+  if (K <= 2) QSMI_LP_COMBINE(2);
+  else if (K <= 4) QSMI_LP_COMBINE(4);
+  else if (K <= 8) QSMI_LP_COMBINE(8);
+  else QSMI_LP_COMBINE(16);
+#undef QSMI_LP_COMBINE
+}
+
+NodeInfo* lp_info_ex(qsmi_lp* lp) { return lp->pInfoEx; }
+Cand* lp_cand(qsmi_lp* lp) { return lp->pCand; }
+
+int lp_read_counters(qsmi_lp* lp, long long out[8]) {
+  unsigned long long c[8];
+  if (hipMemcpy(c, lp->pCounters, sizeof(c), hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(QSMI_ERR_HIP, "learned proposal: counters");
+  for (int i = 0; i < 8; ++i) out[i] = (long long)c[i];
+  return QSMI_OK;
+}
+
+void lp_record(int nN, int w0, const int* order, const NodeInfo* info, const int* nbOff, const float* di, int nVar,
+               int sd0, const uint32_t* outPos, int K, int cap, int* recOff, float* recVal, uint32_t* recBest,
+               hipStream_t st) {
+  hipLaunchKernelGGL(lp::k_lp_record, dim3(nN), dim3(128), 0, st, w0, order, info, nbOff, di, nVar, sd0, outPos, K,
+                     cap, recOff, recVal, recBest);
+}
+
+}  // namespace qsmi

@@ -1,0 +1,165 @@
+// Block top-K of the device selection rule (shared by the scan's rows
+// kernels and the learned proposal's rescoring kernel).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "qsmi_kernels.hpp"
+
+namespace qsmi {
+
+// ---------------------------------------------------------------------------
+// Block top-K of the device rule in two passes (blockDim a multiple of 64,
+// at most 8 wavefronts):
+//  1. every thread keeps its KT largest keys (floats only); the block finds
+//     tau, a lower bound of its K-th largest key counted with multiplicity;
+//  2. only values with key >= tau (normally exactly K per block) get the tie
+//     hash and enter a per-thread list ordered by the 64-bit rank
+//     (orderable key bits << 32 | hash), from which K rounds of block maxima
+//     extract the result. The hash is a bijection of the position for a
+//     fixed seed, so ranks are distinct and (key desc, hash desc) is the
+//     whole order of the rule (the position tie-break never triggers).
+// ---------------------------------------------------------------------------
+struct TopkScratch {
+  float f[2][8];
+  int n[2][8];
+  unsigned long long r[2][8];
+  uint32_t pos;
+  float val;
+};
+
+// orderable image of a key: larger key -> larger integer; -0 folds into +0
+__device__ __forceinline__ uint32_t key_bits(float k) {
+  uint32_t u = __float_as_uint(k);
+  if (u == 0x80000000u) u = 0u;
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ float key_of_bits(uint32_t b) {
+  return __uint_as_float((b & 0x80000000u) ? (b & 0x7FFFFFFFu) : ~b);
+}
+
+// tau <= the block's K-th largest key (multiplicity counted); -inf when the
+// block holds fewer than K keys. m: this thread's KT largest keys, descending
+// (consumed). Block-uniform result; ends after a barrier.
+template <int KT>
+__device__ float block_kth_key(float (&m)[KT], int K, TopkScratch* s) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+  int need = K;
+  float tau = -INFINITY;
+  for (int it = 0; it < K; ++it) {
+    const int b = it & 1;
+    float M = m[0];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) M = fmaxf(M, __shfl_xor(M, off));
+    if (lane == 0) s->f[b][wave] = M;
+    __syncthreads();
+    M = s->f[b][0];
+    for (int i = 1; i < nw; ++i) M = fmaxf(M, s->f[b][i]);
+    tau = M;
+    if (M == -INFINITY) break;
+    int c = 0;
+#pragma unroll
+    for (int i = 0; i < KT; ++i) c += (m[i] == M) ? 1 : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if (lane == 0) s->n[b][wave] = c;
+    __syncthreads();
+    int tot = 0;
+    for (int i = 0; i < nw; ++i) tot += s->n[b][i];
+    if (tot >= need) break;
+    need -= tot;
+    // the entries equal to M sit at the head of the list: pop them
+#pragma unroll
+    for (int r = 0; r < KT; ++r) {
+      if (m[0] == M) {
+#pragma unroll
+        for (int i = 0; i < KT - 1; ++i) m[i] = m[i + 1];
+        m[KT - 1] = -INFINITY;
+      }
+    }
+  }
+  return tau;
+}
+
+template <int KT>
+struct RankList {
+  unsigned long long r[KT];  // 0 = empty
+  uint32_t pos[KT];
+  float val[KT];
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+      r[i] = 0ull;
+      pos[i] = 0xFFFFFFFFu;
+      val[i] = -INFINITY;
+    }
+  }
+  __device__ __forceinline__ void insert(float key, uint32_t h, uint32_t p, float v) {
+    unsigned long long c = ((unsigned long long)key_bits(key) << 32) | h;
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+      if (c > r[i]) {
+        const unsigned long long tc = r[i];
+        const uint32_t tp = pos[i];
+        const float tv = val[i];
+        r[i] = c;
+        pos[i] = p;
+        val[i] = v;
+        c = tc;
+        p = tp;
+        v = tv;
+      }
+    }
+  }
+  // K rounds of block maxima -> out[0..K), best first; ends after a barrier
+  __device__ void block_extract(int K, Cand* out, TopkScratch* s) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = blockDim.x >> 6;
+    for (int it = 0; it < K; ++it) {
+      const int b = it & 1;
+      unsigned long long M = r[0];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(M, off);
+        M = o > M ? o : M;
+      }
+      if (lane == 0) s->r[b][wave] = M;
+      __syncthreads();
+      M = s->r[b][0];
+      for (int i = 1; i < nw; ++i) M = s->r[b][i] > M ? s->r[b][i] : M;
+      if (M != 0ull && r[0] == M) {  // exactly one thread holds it
+        s->pos = pos[0];
+        s->val = val[0];
+#pragma unroll
+        for (int i = 0; i < KT - 1; ++i) {
+          r[i] = r[i + 1];
+          pos[i] = pos[i + 1];
+          val[i] = val[i + 1];
+        }
+        r[KT - 1] = 0ull;
+        pos[KT - 1] = 0xFFFFFFFFu;
+        val[KT - 1] = -INFINITY;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        Cand w;
+        if (M != 0ull) {
+          w.key = key_of_bits((uint32_t)(M >> 32));
+          w.h = (uint32_t)M;
+          w.pos = s->pos;
+          w.val = s->val;
+        } else {
+          w.key = -INFINITY;
+          w.h = 0u;
+          w.pos = 0xFFFFFFFFu;
+          w.val = -INFINITY;
+        }
+        out[it] = w;
+      }
+    }
+  }
+};
+
+
+}  // namespace qsmi

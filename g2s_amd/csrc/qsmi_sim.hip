@@ -228,6 +228,9 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   int rc = grow_wave_bufs(dev, b, std::max(maxWave, 1), std::max(maxWave, 1) * R.n, K);
   if (rc) return rc;
   if (t->fast1k && (rc = ensure_tap_tables(dev, std::max(maxWave, 1)))) return rc;
+  if (R.lp && (rc = lp_prepare(R.lp, t, std::max(maxWave, 1), K, R.lpShortlist, dev->active.data(),
+                               (int)dev->active.size(), dev->circular)))
+    return rc;
   ScanGeom g = geom_of(dev);
   const int rb = row_blocks(t);
   const bool savedTiming = dev->timing;
@@ -256,12 +259,29 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
     hipLaunchKernelGGL(k_gather, dim3(nN), dim3(64), 0, st, P, w0, R.dOrder, R.dPathCells, R.dPathSeeds, R.dNbIdx,
                        R.dNbCnt, R.dOffs, R.dKernel, R.dCatvals, R.dDi, R.dIndex, R.dTi, b->dInfo, b->dOff, b->dVal);
     if (ev) (void)hipEventRecord(ev[1], st);
-    if ((rc = launch_spectrum(dev, g, nN, b->dInfo, b->dOff, b->dVal, b->dC))) return rc;
-    if (ev) (void)hipEventRecord(ev[2], st);
-    launch_rows(dev, g, b->dInfo, b->dC, nN, rb, K, b->dCand, nullptr);
-    if (ev) (void)hipEventRecord(ev[3], st);
-    launch_merge_draw(g, b->dInfo, b->dCand, nN, rb, rows_per_block(t), K, R.k, b->dOutVal, b->dOutPos, nullptr,
-                      R.dDi, R.dTi, R.nVar, R.dIndex, st);
+    if (R.lp) {
+      // learned proposal for every node, the exact scan for the routed ones
+      lp_propose(R.lp, t, P.V, nN, b->dInfo, b->dOff, b->dVal, R.dDi, R.nVar, R.sd[0], K, st);
+      lp_flags(R.lp, nN, w0, R.dOrder, b->dInfo, K, R.lpEvery, R.lpMse, R.lpAudit, st);
+      const NodeInfo* infoEx = lp_info_ex(R.lp);
+      if ((rc = launch_spectrum(dev, g, nN, infoEx, b->dOff, b->dVal, b->dC))) return rc;
+      if (ev) (void)hipEventRecord(ev[2], st);
+      launch_rows(dev, g, infoEx, b->dC, nN, rb, K, b->dCand, nullptr);
+      lp_combine(R.lp, nN, infoEx, b->dCand, rb, rows_per_block(t), K, R.lpAudit, st);
+      if (ev) (void)hipEventRecord(ev[3], st);
+      launch_merge_draw(g, b->dInfo, lp_cand(R.lp), nN, 1, 0, K, R.k, b->dOutVal, b->dOutPos, nullptr, R.dDi, R.dTi,
+                        R.nVar, R.dIndex, st);
+    } else {
+      if ((rc = launch_spectrum(dev, g, nN, b->dInfo, b->dOff, b->dVal, b->dC))) return rc;
+      if (ev) (void)hipEventRecord(ev[2], st);
+      launch_rows(dev, g, b->dInfo, b->dC, nN, rb, K, b->dCand, nullptr);
+      if (ev) (void)hipEventRecord(ev[3], st);
+      launch_merge_draw(g, b->dInfo, b->dCand, nN, rb, rows_per_block(t), K, R.k, b->dOutVal, b->dOutPos, nullptr,
+                        R.dDi, R.dTi, R.nVar, R.dIndex, st);
+    }
+    if (R.recCap > 0)
+      lp_record(nN, w0, R.dOrder, b->dInfo, b->dOff, R.dDi, R.nVar, R.sd[0], b->dOutPos, K, R.recCap, R.dRecOff,
+                R.dRecVal, R.dRecBest, st);
     if (ev) (void)hipEventRecord(ev[4], st);
     scanned += nN;
     waves++;

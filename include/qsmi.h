@@ -211,6 +211,23 @@ typedef struct {
   float tieQuantum;        /* < 0 automatic: 0 with a continuous variable,
                               2^(ceil(log2(n*max|w|))-16) otherwise */
   int stageTiming;         /* 1: time every wave's kernels with HIP events (stageMs) */
+  /* optional training-pair record (2-D, one continuous variable, exact scan
+     only): for path node i (path order, hard data excluded) with i <
+     recordCap, its neighbours 1..n-1 (the cell itself excluded, nearest
+     first) and the exact best TI position */
+  int recordCap;
+  int* recOffsets;         /* recordCap x 80 x 2 (dx, dy) */
+  float* recValues;        /* recordCap x 80, NaN in empty slots */
+  unsigned* recBest;       /* recordCap; UINT32_MAX when the node drew from the marginal */
+  /* optional learned proposal (qsmi_lp with keys built for this TI): */
+  struct qsmi_lp* lp;      /* NULL: exact scan for every node */
+  int lpShortlist;         /* target positions per node (0 -> 4096) */
+  int lpFallbackEvery;     /* exact scan for every m-th path node (0: never) */
+  float lpFallbackMse;     /* exact scan when the best proposed mismatch per
+                              neighbour exceeds this x sigma^2 (<= 0: off) */
+  int lpAudit;             /* 1: also run the exact scan for every node and
+                              count how many exact k-best positions the
+                              proposals held (slow; for measurements) */
 } qsmi_qs_request;
 
 typedef struct {
@@ -227,9 +244,76 @@ typedef struct {
   int waves;               /* out: waves launched */
   double phaseMs[4];       /* out: path loop phases: neighbour search, dependency
                               levels and waves, uploads, device loop */
+  long long lpProposed;    /* out: nodes served by the learned proposal */
+  long long lpFallbacks;   /* out: nodes sent to the exact scan by the rules */
+  long long lpOverflows;   /* out: of which shortlist overflows */
+  long long lpShortlistSum;  /* out: proposed positions over all proposed nodes */
+  long long lpAuditNodes;  /* out: audited proposal nodes */
+  long long lpAuditHits;   /* out: exact k-best positions found in the proposals */
+  long long lpAuditBestHits; /* out: audited nodes whose exact best position was proposed */
 } qsmi_qs_result;
 
 int qsmi_qs_run_in_memory(const qsmi_qs_request* req, qsmi_qs_result* res);
+
+/* ======================================================================
+ * Learned proposal (no counterpart in the reference): pattern encoders
+ * that propose a shortlist of TI positions per path node; the shortlist is
+ * rescored exactly and the FFT scan stays the fallback. Two MLP towers
+ * 240 -> 256 -> 256 -> 256 -> 128 (ReLU), products on bf16 MFMA with fp32
+ * accumulation, fp32 weights trained with Adam on an InfoNCE loss.
+ * Features: 80 slots x (dx/16, dy/16, (x - mu)/sigma); query = a node's
+ * neighbours (nearest first), key = a TI position on the 80 canonical
+ * offsets. Parameters: qsmi_lp_param_count floats, query tower then key
+ * tower, each W1 [240][256], b1, W2 [256][256], b2, W3 [256][256], b3,
+ * W4 [256][128], b4 (W[in][out], row-major).
+ * ====================================================================== */
+typedef struct qsmi_lp qsmi_lp;
+
+int qsmi_lp_create(int gpu, unsigned seed, qsmi_lp** out);
+int qsmi_lp_destroy(qsmi_lp* lp);
+int qsmi_lp_param_count(long long* count);
+int qsmi_lp_get_params(qsmi_lp* lp, float* out);
+int qsmi_lp_set_params(qsmi_lp* lp, const float* in);
+
+/* C = op(A) op(B) on the lp GEMM (host arrays, row-major; transA: A given
+ * as [K][M]; transB: B given as [N][K]). For tests of the MFMA kernel. */
+int qsmi_lp_gemm(qsmi_lp* lp, int transA, int transB, int M, int N, int K, const float* A, const float* B,
+                 float* C);
+/* embeddings (n x 128) of n feature rows (n x 240) by tower 0 (query) or 1 (key) */
+int qsmi_lp_encode(qsmi_lp* lp, int tower, int n, const float* features, float* embeddings);
+/* InfoNCE loss of B queries against C >= B key rows (row b's positive is key
+ * b) with logits scaled by invTau; mean loss and the gradient of every
+ * parameter (qsmi_lp_param_count floats). */
+int qsmi_lp_loss_grad(qsmi_lp* lp, int B, int C, const float* qfeat, const float* kfeat, float invTau,
+                      float* loss, float* grad);
+
+/* One TI and the training pairs recorded on it (qsmi_qs_request record). */
+typedef struct {
+  unsigned tiSize[2];
+  const float* ti;          /* prod(tiSize), the continuous variable */
+  const int* canon;         /* 80 x 2 canonical offsets of the key features */
+  int nPairs;
+  const int* offsets;       /* nPairs x 80 x 2 */
+  const float* values;      /* nPairs x 80 (NaN: empty) */
+  const unsigned* best;     /* nPairs (UINT32_MAX pairs are skipped) */
+} qsmi_lp_train_set;
+
+typedef struct {
+  int steps;
+  int batch;                /* queries per step (0 -> 256) */
+  int negatives;            /* random TI positions per step (0 -> 768) */
+  float lr;                 /* Adam step (0 -> 1e-3) */
+  float invTau;             /* logit scale (0 -> 10) */
+  unsigned seed;
+} qsmi_lp_train_opts;
+
+/* steps of Adam on batches drawn from one set; lossOut (nullable): mean loss
+ * of each block of 50 steps (ceil(steps / 50) floats). */
+int qsmi_lp_train(qsmi_lp* lp, const qsmi_lp_train_set* set, const qsmi_lp_train_opts* opts, float* lossOut);
+
+/* Keys of every position of a TI (kept in HBM for proposals) and a fixed
+ * sample of 8192 of them for the per-node thresholds. */
+int qsmi_lp_build_keys(qsmi_lp* lp, const unsigned* tiSize, const float* ti, const int* canon);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,130 @@
+"""Learned proposal on the GPU against its CPU oracle (oracle/lp_oracle.py).
+
+- the MFMA GEMM (all four transpose forms, ragged sizes) equals the float64
+  product of the bf16-rounded operands within the fp32 accumulation bound
+  K * 2^-22 * (|A| |B|);
+- both encoder towers and the InfoNCE loss + full parameter gradient equal
+  the oracle, which follows the same bf16 rounding points (the oracle's
+  float64 gradient is pinned to finite differences in test_lp_oracle.py).
+  Tolerances: a bf16 rounding of an fp32 intermediate can land one ulp
+  (2^-8 relative) away from the float64 one, so embeddings are compared at
+  2e-2 of their scale and gradients per tensor at 3e-2 relative L2;
+- training on recorded pairs lowers the loss, and the trained network's
+  proposals hold the exact k-best positions far more often than the same
+  network untrained (the audit of a learned-proposal realisation).
+"""
+import numpy as np
+import pytest
+
+import g2s_amd
+from g2s_amd import lp as L
+from g2s_amd import synth
+from oracle import lp_oracle as lo
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def net():
+    m = L.LearnedProposal(seed=7)
+    yield m
+    m.close()
+
+
+@pytest.mark.parametrize("ta", [False, True])
+@pytest.mark.parametrize("tb", [False, True])
+@pytest.mark.parametrize("M,N,K", [(70, 90, 50), (64, 64, 32), (1, 130, 257), (200, 3, 17)])
+def test_gemm_matches_bf16_product(net, ta, tb, M, N, K):
+    rng = np.random.default_rng(M * 1000 + N + K)
+    A = rng.normal(size=(K, M) if ta else (M, K)).astype(np.float32)
+    B = rng.normal(size=(N, K) if tb else (K, N)).astype(np.float32)
+    C = net.gemm(A, B, ta, tb)
+    a = lo.bf16(A.T if ta else A)
+    b = lo.bf16(B.T if tb else B)
+    ref = a @ b
+    bound = K * 2.0 ** -22 * (np.abs(a) @ np.abs(b)) + 1e-30
+    assert np.all(np.abs(C - ref) <= 4 * bound), np.max(np.abs(C - ref) / bound)
+
+
+def test_encoders_match_oracle(net):
+    p = net.get_params().astype(np.float64)
+    rng = np.random.default_rng(3)
+    X = rng.normal(size=(70, 240)).astype(np.float32)
+    X[:, 150:] = 0  # empty slots as in sparse neighbourhoods
+    for tower in (0, 1):
+        E = net.encode(tower, X)
+        ref = lo.forward(p, tower, X)[-1]
+        scale = np.sqrt(np.mean(ref ** 2))
+        err = np.max(np.abs(E - ref)) / scale
+        print(f"  tower {tower}: max |E - oracle| / rms = {err:.2e}")
+        assert err < 2e-2
+
+
+def test_loss_and_gradient_match_oracle(net):
+    p = net.get_params().astype(np.float64)
+    rng = np.random.default_rng(4)
+    B, C = 32, 96
+    q = rng.normal(size=(B, 240)).astype(np.float32)
+    k = rng.normal(size=(C, 240)).astype(np.float32)
+    k[:B] = q + 0.1 * rng.normal(size=(B, 240)).astype(np.float32)  # positives resemble their queries
+    loss, g = net.loss_grad(q, k, 5.0)
+    rl, rg = lo.loss_grad(p, q, k, 5.0)
+    print(f"  loss gpu {loss:.6f} oracle {rl:.6f}")
+    assert abs(loss - rl) <= 2e-3 * abs(rl) + 1e-5
+    for t in range(2):
+        for l in range(4):
+            w, b = lo.LAYERS[l]
+            base = t * lo.TOWER
+            rw = rg[base + w:base + w + lo.DIN[l] * lo.DOUT[l]]
+            for (o, n) in ((w, lo.DIN[l] * lo.DOUT[l]), (b, lo.DOUT[l])):
+                a_, r_ = g[base + o:base + o + n], rg[base + o:base + o + n]
+                # the key tower's last bias gradient cancels exactly (each
+                # softmax row sums to one): scale it by its layer's weights
+                ref = max(np.linalg.norm(r_), 1e-2 * np.linalg.norm(rw))
+                rel = np.linalg.norm(a_ - r_) / (ref + 1e-12)
+                assert rel < 3e-2, (t, l, o == b, rel)
+
+
+def _ti(seed, size):
+    g = synth.grf((size, size), 20.0, seed=seed)
+    ti, dims = synth.to_g2s(np.exp(g))
+    return np.ascontiguousarray(ti[:, 0]), dims
+
+
+def _record(seed, size, sim):
+    ti, dims = _ti(seed, size)
+    di = np.full(sim * sim, np.nan, np.float32)
+    _, _, st = g2s_amd.qs(ti, dims, di, (sim, sim), [0], 80, 1.5, seed=seed, record=sim * sim)
+    return ti, dims, st["pairs"]
+
+
+def _audit(net, ti, dims, sim, seed):
+    net.build_keys(ti, dims)
+    di = np.full(sim * sim, np.nan, np.float32)
+    _, _, st = g2s_amd.qs(ti, dims, di, (sim, sim), [0], 80, 1.5, seed=seed, lp=net, lp_shortlist=512,
+                          lp_audit=True)
+    n = st["lp_audit_nodes"]
+    return st["lp_audit_best_hits"] / max(1, n), st["lp_audit_hits"] / max(1, 2 * n), st
+
+
+def test_training_gains_over_untrained():
+    size, sim = 200, 100
+    sets = [_record(s, size, sim) for s in (211, 212, 213)]
+    trained = L.LearnedProposal(seed=11)
+    untrained = L.LearnedProposal(seed=11)
+    first = last = None
+    for epoch in range(3):
+        for ti, dims, pairs in sets:
+            loss = trained.train(ti, dims, pairs, steps=200, batch=256, negatives=768, lr=1e-3, seed=epoch)
+            first = loss[0] if first is None else first
+            last = loss[-1]
+    print(f"  training loss {first:.3f} -> {last:.3f}")
+    assert last < 0.7 * first
+    ti, dims = _ti(219, size)
+    b_t, r_t, st = _audit(trained, ti, dims, sim, 5)
+    b_u, r_u, _ = _audit(untrained, ti, dims, sim, 5)
+    print(f"  best-position recall: trained {b_t:.3f} untrained {b_u:.3f}; k-best recall {r_t:.3f} vs {r_u:.3f}")
+    print(f"  proposals {st['lp_proposed']} fallbacks {st['lp_fallbacks']} overflows {st['lp_overflows']}")
+    assert b_t > b_u + 0.2 and r_t > r_u + 0.2
+    trained.close()
+    untrained.close()
