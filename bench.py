@@ -49,7 +49,70 @@ def parse():
     ap.add_argument("--wave", type=int, default=256, help="max path nodes per wave")
     ap.add_argument("--cpu-sample", type=int, default=48, help="path nodes of the CPU baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--learned", type=int, default=1,
+                    help="also measure the learned-proposal form (1) or not (0)")
+    ap.add_argument("--lp-train-sets", type=int, default=4, help="GRF TIs whose plain QS runs give training pairs")
+    ap.add_argument("--lp-record", type=int, default=500, help="simulation extent of each recording run")
+    ap.add_argument("--lp-steps", type=int, default=1000, help="Adam steps per TI per epoch")
+    ap.add_argument("--lp-epochs", type=int, default=2)
     return ap.parse_args()
+
+
+def learned_form(cfg, args, exact_ms):
+    """Learned-proposal QS on the same workload: pairs recorded by plain QS on
+    other seeded GRF TIs (range 20, exp), the encoders trained on the GPU,
+    keys built for this TI, then one timed realisation (every 64th node and
+    every node the encoders cannot see on the exact scan) and an audited
+    300x300 realisation for the k-best recall of the proposals."""
+    import g2s_amd
+    from g2s_amd import lp as L
+    from g2s_amd import synth
+
+    t0 = time.time()
+    sets = []
+    for s in range(args.lp_train_sets):
+        g = synth.grf(tuple(cfg["ti_dims"][::-1]), 20.0, seed=200 + s)
+        ti_s, dims_s = synth.to_g2s(np.exp(g))
+        ti_s = np.ascontiguousarray(ti_s[:, 0])
+        r = args.lp_record
+        _, _, st = g2s_amd.qs(ti_s, dims_s, np.full(r * r, np.nan, np.float32), (r, r), [0], cfg["n"], cfg["k"],
+                              seed=300 + s, record=r * r)
+        sets.append((ti_s, dims_s, st["pairs"]))
+    t_rec = time.time() - t0
+    net = L.LearnedProposal(seed=1)
+    t0 = time.time()
+    losses = []
+    for e in range(args.lp_epochs):
+        for i, (ti_s, dims_s, pairs) in enumerate(sets):
+            losses.append(net.train(ti_s, dims_s, pairs, steps=args.lp_steps, lr=1e-3 if e == 0 else 3e-4,
+                                    seed=100 * e + i))
+    t_train = time.time() - t0
+    print(f"[bench] learned: {len(sets)} x {args.lp_record}^2 pair sets in {t_rec:.1f}s, training {t_train:.1f}s, "
+          f"loss {losses[0][0]:.3f} -> {losses[-1][-1]:.3f}", file=sys.stderr, flush=True)
+    ti = np.ascontiguousarray(cfg["ti"][:, 0])
+    dims, sdims = cfg["ti_dims"], cfg["sim_dims"]
+    net.build_keys(ti, dims)
+    kw = dict(lp=net, lp_shortlist=4096, lp_fallback_every=64, lp_fallback_mse=0.5)
+    di = np.full(int(np.prod(sdims)), np.nan, np.float32)
+    _, _, st = g2s_amd.qs(ti, dims, di, sdims, cfg["types"], cfg["n"], cfg["k"], seed=cfg["seed"], **kw)
+    a = 300
+    _, _, au = g2s_amd.qs(ti, dims, np.full(a * a, np.nan, np.float32), (a, a), cfg["types"], cfg["n"], cfg["k"],
+                          seed=cfg["seed"], lp_audit=True, **kw)
+    n_a = max(1, au["lp_audit_nodes"])
+    nodes = st["nodes"]
+    out = dict(value=nodes / (st["path_loop_ms"] / 1e3), unit="nodes/s", ms_per_step=st["path_loop_ms"],
+               speedup_vs_exact=exact_ms / st["path_loop_ms"],
+               proposed_frac=st["lp_proposed"] / max(1, nodes), fallback_frac=st["lp_fallbacks"] / max(1, nodes),
+               mean_shortlist=st["lp_shortlist_sum"] / max(1, st["lp_proposed"]),
+               recall_best_proposed=au["lp_audit_best_hits"] / n_a,
+               recall_kbest_proposed=au["lp_audit_hits"] / (n_a * int(np.ceil(cfg["k"]))),
+               recall_audit_sample=f"{a}x{a} realisation on the same TI, {au['lp_audit_nodes']} proposed nodes",
+               training=dict(pairs=int(sum(len(p[2]) for _, _, p in sets)), steps=args.lp_steps * args.lp_epochs *
+                             len(sets), seconds=t_rec + t_train, final_loss=float(losses[-1][-1])),
+               rules="exact scan for every 64th node, nodes with < 50% of neighbours on the key stencil, "
+                     "best proposed mismatch > 0.5 sigma^2 per neighbour, shortlist overflow")
+    net.close()
+    return out
 
 
 def cpu_baseline(cfg, sim, threads, sample):
@@ -152,6 +215,7 @@ def main():
                     kernel="k_pencil1k", avg_launch_us=pencil_ms * 1e3 / max(1, st["waves"]),
                     nodes_per_launch=nodes_per_launch, algorithmic_bytes_per_node=a_node,
                     traffic_source=(PROFILE if per_node else None))
+    learned = learned_form(cfg, args, ms) if args.learned else None
     threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
     cpu = None
     if args.cpu_sample > 0:
@@ -175,6 +239,7 @@ def main():
         "hbm_roofline_frac_whole_loop": a_node * value / (HBM_PEAK_GBS * 1e9),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "learned_proposal": learned,
         "stats": st,
     }
     print(json.dumps(out))

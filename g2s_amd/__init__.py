@@ -282,7 +282,7 @@ class Wave:
 
 def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dims=None, gpu=0,
             wave_size=256, tie_quantum=-1.0, stage_timing=False, record=0, lp=None, lp_shortlist=0,
-            lp_fallback_every=0, lp_fallback_mse=0.0, lp_audit=False):
+            lp_fallback_every=0, lp_fallback_mse=0.0, lp_audit=False, lp_min_cover=0.0):
     """In-memory QuickSampling (qsmi_qs_run_in_memory).
 
     Arrays are flat in G2S DataImage order (dimension 0 fastest, variable
@@ -333,6 +333,7 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
         rq.lpFallbackEvery = int(lp_fallback_every)
         rq.lpFallbackMse = float(lp_fallback_mse)
         rq.lpAudit = int(bool(lp_audit))
+        rq.lpMinCover = float(lp_min_cover)
     sim = np.empty_like(di)
     idx = np.zeros(int(np.prod(sim_dims)), np.uint32)
     rs = _lib.QsResult()

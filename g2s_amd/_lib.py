@@ -120,6 +120,7 @@ class QsRequest(C.Structure):
         ("lpFallbackEvery", C.c_int),
         ("lpFallbackMse", C.c_float),
         ("lpAudit", C.c_int),
+        ("lpMinCover", C.c_float),
     ]
 
 

@@ -508,6 +508,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   R.lpEvery = rq->lpFallbackEvery;
   R.lpMse = rq->lpFallbackMse;
   R.lpAudit = rq->lpAudit;
+  R.lpMinCover = rq->lpMinCover > 0.f ? rq->lpMinCover : 0.5f;
   const auto tUp = clk::now();
   double stageMs[4] = {0, 0, 0, 0};
   long long scanned = 0;

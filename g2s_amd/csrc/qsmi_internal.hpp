@@ -152,7 +152,7 @@ struct ResidentLoop {
   // learned proposal (nullptr: exact scan for every node)
   qsmi_lp* lp = nullptr;
   int lpShortlist = 0, lpEvery = 0, lpAudit = 0;
-  float lpMse = 0.f;
+  float lpMse = 0.f, lpMinCover = 0.75f;
   // training-pair record (device arrays; recCap 0: off)
   int recCap = 0;
   int* dRecOff = nullptr;
@@ -164,9 +164,9 @@ struct ResidentLoop {
 int lp_prepare(qsmi_lp* lp, const qsmi_ti* t, int maxNodes, int K, int shortlist, const int* active, int V,
                int circular);
 void lp_propose(qsmi_lp* lp, const qsmi_ti* t, int V, int nN, const NodeInfo* info, const int* nbOff,
-                const float* nbVal, const float* di, int nVar, int sd0, int K, hipStream_t st);
+                const float* nbVal, const float* di, int nVar, int sd0, int K, float minCover, hipStream_t st);
 void lp_flags(qsmi_lp* lp, int nN, int w0, const int* order, const NodeInfo* info, int K, int every, float mseSigma2,
-              int audit, hipStream_t st);
+              float minCover, int audit, hipStream_t st);
 void lp_combine(qsmi_lp* lp, int nN, const NodeInfo* info, const Cand* candEx, int blocks, int rowsPerBlock, int K,
                 int audit, hipStream_t st);
 NodeInfo* lp_info_ex(qsmi_lp* lp);

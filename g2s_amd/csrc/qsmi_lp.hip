@@ -51,8 +51,16 @@ constexpr int IN = 3 * NB;  // 240 input features
 constexpr int HID = 256;
 constexpr int EMB = 128;
 constexpr float OFFSET_SCALE = 1.f / 16.f;
+// bias offsets of the four layers inside one tower's parameters
+__device__ __forceinline__ long long boff_dev(int l) {
+  return l == 0 ? 240LL * 256 : l == 1 ? 240LL * 256 + 256 + 256LL * 256
+                              : l == 2 ? 240LL * 256 + 256 + 2 * (256LL * 256 + 256) - 256
+                                       : 240LL * 256 + 256 + 2 * (256LL * 256 + 256) + 256LL * 128;
+}
 constexpr int SLOT_R = 8;                  // slot map covers |d0|, |d1| <= 8
 constexpr int SLOT_W = 2 * SLOT_R + 1;     // 17 x 17 map: canonical slot or -1
+
+constexpr int NSEG = 16;  // shortlist segments per node (one counter each)
 
 constexpr int GT = 64;   // block tile (M and N)
 constexpr int GK = 32;   // K step
@@ -321,11 +329,13 @@ __global__ void __launch_bounds__(128) k_pair_features(const int* __restrict__ i
 __global__ void __launch_bounds__(128) k_wave_features(const NodeInfo* __restrict__ info, const int* __restrict__ nbOff,
                                                       const float* __restrict__ di, int nVar, int sd0,
                                                       const int* __restrict__ slotMap, float mu, float invSigma,
-                                                      float* __restrict__ out) {
+                                                      float* __restrict__ out, float* __restrict__ cover) {
+  __shared__ int covered;
   const int node = blockIdx.x;
   const NodeInfo ni = info[node];
   float* o = out + (long long)node * IN;
   for (int i = threadIdx.x; i < IN; i += blockDim.x) o[i] = 0.f;
+  if (threadIdx.x == 0) covered = 0;
   __syncthreads();
   const int c0 = (int)(ni.cell % (unsigned)sd0), c1 = (int)(ni.cell / (unsigned)sd0);
   for (int j = 1 + threadIdx.x; j < ni.nb; j += blockDim.x) {
@@ -336,19 +346,43 @@ __global__ void __launch_bounds__(128) k_wave_features(const NodeInfo* __restric
     o[3 * s] = d0 * OFFSET_SCALE;
     o[3 * s + 1] = d1 * OFFSET_SCALE;
     o[3 * s + 2] = (v - mu) * invSigma;
+    atomicAdd(&covered, 1);
   }
+  __syncthreads();
+  if (threadIdx.x == 0) cover[node] = ni.nb > 1 ? (float)covered / (float)(ni.nb - 1) : 0.f;
 }
 
-// value of the r-th largest element (1-based) of each row: bitonic sort of
-// one row of n = 2^m <= 8192 floats per block, descending
-__global__ void __launch_bounds__(256) k_row_kth(const float* __restrict__ Gm, int n, int r, float* __restrict__ thr) {
-  __shared__ float v[8192];
+// Per-node score threshold from the sampled scores (one block per row of
+// n = 8192 floats): every thread keeps the 4 largest of its 32 values, the
+// 1024 survivors are sorted (bitonic, descending) and thr = the r-th
+// largest survivor (r <= 1024). When a thread held more than 4 values above
+// it, thr is a lower bound of the r-th largest, which only lengthens the
+// shortlist.
+__global__ void __launch_bounds__(256) k_row_kth(const float* __restrict__ Gm, int n, int r,
+                                                const float* __restrict__ cover, float minCover,
+                                                float* __restrict__ thr) {
+  __shared__ float v[1024];
+  if (!(cover[blockIdx.x] >= minCover)) {  // routed to the exact scan: propose nothing
+    if (threadIdx.x == 0) thr[blockIdx.x] = INFINITY;
+    return;
+  }
   const float* row = Gm + (long long)blockIdx.x * n;
-  for (int i = threadIdx.x; i < n; i += 256) v[i] = row[i];
+  float m0 = -INFINITY, m1 = -INFINITY, m2 = -INFINITY, m3 = -INFINITY;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    float x = row[i];
+    float h = fmaxf(m0, x); x = fminf(m0, x); m0 = h;
+    h = fmaxf(m1, x); x = fminf(m1, x); m1 = h;
+    h = fmaxf(m2, x); x = fminf(m2, x); m2 = h;
+    m3 = fmaxf(m3, x);
+  }
+  v[threadIdx.x] = m0;
+  v[256 + threadIdx.x] = m1;
+  v[512 + threadIdx.x] = m2;
+  v[768 + threadIdx.x] = m3;
   __syncthreads();
-  for (int k = 2; k <= n; k <<= 1) {
+  for (int k = 2; k <= 1024; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < n; i += 256) {
+      for (int i = threadIdx.x; i < 1024; i += 256) {
         const int l = i ^ j;
         if (l > i) {
           const bool desc = (i & k) == 0;
@@ -362,53 +396,221 @@ __global__ void __launch_bounds__(256) k_row_kth(const float* __restrict__ Gm, i
       __syncthreads();
     }
   }
-  if (threadIdx.x == 0) thr[blockIdx.x] = v[min(max(r, 1), n) - 1];
+  if (threadIdx.x == 0) thr[blockIdx.x] = v[min(max(r, 1), 1024) - 1];
+}
+
+// fp32 -> bf16 copy (the keys of the scoring product)
+__global__ void k_to_bf16(const float* __restrict__ src, __bf16* __restrict__ dst, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (__bf16)src[i];
 }
 
 // ---------------------------------------------------------------------------
-// Exact rescoring of each node's proposed positions (one 256-thread block per
-// node; lanes run over the neighbours of one position at a time), then the
-// block's K best by the device rule -> cand[node][K]; best[node] = the
-// largest E found (-inf when none is valid).
+// Scoring of every TI position for the nodes of a wave with the threshold
+// epilogue: S[m][n] = <Q[m], keys[n]> (bf16 operands, fp32 accumulation);
+// positions with S >= thr[m] are appended to segment (block % NSEG) of
+// list[m] (counter cnt[m][seg], segment capacity cap / NSEG): sixteen
+// counters per node keep the appends from serialising on one address.
+// Q (M <= 256 rows) sits in LDS as bf16; each wavefront takes 64 keys at a
+// time, holds their B fragments for all of K = 128 in registers and sweeps
+// the M tiles.
 // ---------------------------------------------------------------------------
-template <int KT>
+__global__ void __launch_bounds__(256) k_score(int M, long long N, const float* __restrict__ Q,
+                                              const __bf16* __restrict__ keys, const float* __restrict__ thr,
+                                              uint32_t* __restrict__ list, int* __restrict__ cnt, int cap) {
+  constexpr int QLD = EMB + 8;  // bf16 row stride of Q in LDS (16-byte aligned fragments)
+  __shared__ __attribute__((aligned(16))) __bf16 qs[256 * QLD];
+  __shared__ float th[256];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int mt = (M + 31) / 32;
+  for (int i = tid; i < mt * 32 * EMB; i += 256) {
+    const int m = i / EMB, k = i - m * EMB;
+    qs[m * QLD + k] = (__bf16)(m < M ? Q[(long long)m * EMB + k] : 0.f);
+  }
+  for (int i = tid; i < mt * 32; i += 256) th[i] = i < M ? thr[i] : INFINITY;
+  __syncthreads();
+  const long long tiles = (N + 63) / 64;
+  const int seg = blockIdx.x % NSEG, segCap = cap / NSEG;
+  for (long long tile = (long long)blockIdx.x * 4 + wave; tile < tiles; tile += (long long)gridDim.x * 4) {
+    const long long n0 = tile * 64;
+    bf16x8 b0[8], b1[8];
+    {
+      const long long ka = n0 + (lane & 31), kb = ka + 32;
+      const bf16x8 z = {};
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        b0[s] = ka < N ? *reinterpret_cast<const bf16x8*>(keys + ka * EMB + 16 * s + 8 * (lane >> 5)) : z;
+        b1[s] = kb < N ? *reinterpret_cast<const bf16x8*>(keys + kb * EMB + 16 * s + 8 * (lane >> 5)) : z;
+      }
+    }
+    for (int m = 0; m < mt; ++m) {
+      f32x16 c0, c1;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) c0[r] = c1[r] = 0.f;
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&qs[(m * 32 + (lane & 31)) * QLD + 16 * s + 8 * (lane >> 5)]);
+        c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0[s], c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1[s], c1, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = m * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const float t = th[row];
+        const long long na = n0 + (lane & 31), nb2 = na + 32;
+        if (c0[r] >= t && na < N) {
+          const int slot = atomicAdd(&cnt[row * NSEG + seg], 1);
+          if (slot < segCap) list[(long long)row * cap + seg * segCap + slot] = (uint32_t)na;
+        }
+        if (c1[r] >= t && nb2 < N) {
+          const int slot = atomicAdd(&cnt[row * NSEG + seg], 1);
+          if (slot < segCap) list[(long long)row * cap + seg * segCap + slot] = (uint32_t)nb2;
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused query-tower forward for inference: 32 rows per block (four
+// wavefronts), activations in LDS as bf16 (exactly the operands the MFMA
+// GEMM path rounds to), weights read as transposed bf16 copies
+// Wt[n][k] (k padded to 256) so every B fragment is one 16-byte load.
+// Same products, same K order and same rounding points as the layer-by-layer
+// GEMMs.
+// ---------------------------------------------------------------------------
+constexpr int KP = 256;        // padded K of every layer
+constexpr int ALD = KP + 8;    // LDS row stride (bf16)
+
+__global__ void k_wt_bf16(const float* __restrict__ W, int din, int dout, __bf16* __restrict__ Wt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dout * KP) return;
+  const int n = i / KP, k = i - n * KP;
+  Wt[i] = (__bf16)(k < din ? W[(long long)k * dout + n] : 0.f);
+}
+
+__global__ void __launch_bounds__(256) k_mlp_fwd(int M, const float* __restrict__ X, const __bf16* __restrict__ Wt,
+                                                const float* __restrict__ W, float* __restrict__ E) {
+  __shared__ __attribute__((aligned(16))) __bf16 act[2][32 * ALD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r0 = blockIdx.x * 32;
+  for (int i = tid; i < 32 * KP; i += 256) {
+    const int r = i / KP, k = i - r * KP;
+    act[0][r * ALD + k] = (__bf16)((r0 + r < M && k < IN) ? X[(long long)(r0 + r) * IN + k] : 0.f);
+  }
+  __syncthreads();
+  int cur = 0;
+  long long wtOff = 0;
+  for (int l = 0; l < 4; ++l) {
+    const int dout = l < 3 ? HID : EMB;
+    const float* bias = W + boff_dev(l);
+    const __bf16* wt = Wt + wtOff;
+    const int subPerWave = dout / 32 / 4;  // 2 for 256 outputs, 1 for 128
+    for (int sub = 0; sub < subPerWave; ++sub) {
+      const int n0 = (wave * subPerWave + sub) * 32;
+      f32x16 c;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) c[r] = 0.f;
+      for (int s = 0; s < KP / 16; ++s) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&act[cur][(lane & 31) * ALD + 16 * s + 8 * (lane >> 5)]);
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(wt + (long long)(n0 + (lane & 31)) * KP + 16 * s + 8 * (lane >> 5));
+        c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+      }
+      const int n = n0 + (lane & 31);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        float v = c[r] + bias[n];
+        if (l < 3) {
+          act[cur ^ 1][row * ALD + n] = (__bf16)fmaxf(v, 0.f);
+        } else if (r0 + row < M) {
+          E[(long long)(r0 + row) * EMB + n] = v;
+        }
+      }
+    }
+    wtOff += (long long)dout * KP;
+    cur ^= 1;
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Exact rescoring of the proposed positions. Stage 1 (block = 256 proposals
+// of one node, grid (chunks, nodes)): each wavefront evaluates one position
+// at a time with its lanes over the node's neighbours (a position's
+// neighbours share a few cache lines) -> ebuf. Stage 2 (block per node): the
+// K best by the device rule -> cand[node][K], best[node] = the largest E
+// (-inf when no proposed position is valid).
+// ---------------------------------------------------------------------------
+constexpr int kRescoreChunk = 256;
+
 __global__ void __launch_bounds__(256) k_rescore(const NodeInfo* __restrict__ info, const int* __restrict__ nbOff,
-                                                const float* __restrict__ nbVal, int V, const int* __restrict__ act,
+                                                const float* __restrict__ nbVal, int V, int yy, const int* __restrict__ act,
                                                 const float* __restrict__ sub, long long S, int s0,
                                                 const uint32_t* __restrict__ list, const int* __restrict__ cnt, int cap,
-                                                int K, float* __restrict__ ebuf, Cand* __restrict__ cand,
-                                                float* __restrict__ best) {
+                                                float* __restrict__ ebuf) {
   __shared__ int doff[128];
   __shared__ float dval[128 * 4];
-  __shared__ TopkScratch red;
-  __shared__ float bestW[4];
-  const int node = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int node = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int c0 = blockIdx.x * kRescoreChunk;
+  const int m = cap;  // every entry of the segmented list; empty ones score -inf
+  const int segCap = cap / NSEG;
   const NodeInfo ni = info[node];
-  const int m = min(cnt[node], cap);
+  int any = 0;
+  for (int g = 0; g < NSEG; ++g) any |= cnt[node * NSEG + g];
+  if (!any) {  // nothing proposed (e.g. routed to the exact scan)
+    float* eb0 = ebuf + (long long)node * cap;
+    for (int c = c0 + tid; c < min(m, c0 + kRescoreChunk); c += 256) eb0[c] = -INFINITY;
+    return;
+  }
   const int nb = min(ni.nb, 128);
-  float* eb = ebuf + (long long)node * cap;
   for (int j = tid; j < nb; j += 256) {
     doff[j] = nbOff[(ni.nbStart + j) * 2] + s0 * nbOff[(ni.nbStart + j) * 2 + 1];
     for (int a = 0; a < V && a < 4; ++a) dval[j * 4 + a] = nbVal[(long long)(ni.nbStart + j) * V + a];
   }
   __syncthreads();
   const bool usable = ni.nb > 0 && ni.nb <= 128 && V <= 4 && ni.lo[0] <= ni.hi[0] && ni.lo[1] <= ni.hi[1];
-  for (int c = wave; c < m; c += 4) {
-    const uint32_t p = list[(long long)node * cap + c];
+  float* eb = ebuf + (long long)node * cap;
+  const int cEnd = min(m, c0 + kRescoreChunk);
+  // 16 lanes per position: four positions per wavefront at a time
+  const int g = lane >> 4, gl = lane & 15;
+  for (int c = c0 + wave * 4 + g; c - g < cEnd; c += 16) {
+    const bool inRange = c < cEnd;
+    const bool have = inRange && (c % segCap) < cnt[node * NSEG + c / segCap];
+    const uint32_t p = have ? list[(long long)node * cap + c] : 0u;
     const int p0 = (int)(p % (uint32_t)s0), p1 = (int)(p / (uint32_t)s0);
-    const bool valid = usable && p0 >= ni.lo[0] && p0 <= ni.hi[0] && p1 >= ni.lo[1] && p1 <= ni.hi[1];
+    const bool valid = have && usable && p0 >= ni.lo[0] && p0 <= ni.hi[0] && p1 >= ni.lo[1] && p1 <= ni.hi[1];
     float e = 0.f;
     if (valid) {
-      for (int j = lane; j < nb; j += 64) {
-        const long long t = (long long)p + doff[j];
-        for (int a = 0; a < V; ++a) e += dval[j * 4 + a] * sub[(long long)act[a] * S + t];
+      if (yy) {  // one continuous variable: images (y^2, y), one load per neighbour
+        for (int j = gl; j < nb; j += 16) {
+          const float y = sub[S + (long long)p + doff[j]];
+          e += dval[j * 4] * (y * y) + dval[j * 4 + 1] * y;
+        }
+      } else {
+        for (int j = gl; j < nb; j += 16) {
+          const long long t = (long long)p + doff[j];
+          for (int a = 0; a < V; ++a) e += dval[j * 4 + a] * sub[(long long)act[a] * S + t];
+        }
       }
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off);
-    if (lane == 0) eb[c] = valid ? e + ni.delta0 : -INFINITY;
+    for (int off = 8; off > 0; off >>= 1) e += __shfl_xor(e, off);
+    if (gl == 0 && inRange) eb[c] = valid ? e + ni.delta0 : -INFINITY;
   }
-  __syncthreads();
+}
+
+template <int KT>
+__global__ void __launch_bounds__(256) k_rescore_select(const NodeInfo* __restrict__ info,
+                                                       const uint32_t* __restrict__ list, const int* __restrict__ cnt,
+                                                       int cap, int K, const float* __restrict__ ebuf,
+                                                       Cand* __restrict__ cand, float* __restrict__ best) {
+  __shared__ TopkScratch red;
+  __shared__ float bestW[4];
+  const int node = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const NodeInfo ni = info[node];
+  const int m = cap;  // segmented list: empty entries hold -inf
+  const float* eb = ebuf + (long long)node * cap;
   float mk[KT];
 #pragma unroll
   for (int i = 0; i < KT; ++i) mk[i] = -INFINITY;
@@ -506,15 +708,18 @@ struct qsmi_lp {
   // keys of one TI
   float* dKeys = nullptr;
   float* dKsample = nullptr;
+  __bf16* dKeysB = nullptr;  // the keys as bf16 (scoring operand)
+  __bf16* dWqT = nullptr;    // query tower, transposed bf16 (fused forward)
   int* dSlot = nullptr;  // canonical slot map of the keys' stencil
   long long keyS = 0;
   int keyDims[2] = {0, 0};
   float mu = 0.f, sigma = 1.f;
   // proposal state of one wave
   int pCap = 0, pMaxNodes = 0, pK = 0, pShort = 0;
+  int pYY = 0;  // the TI's images are (y^2, y) of one continuous variable
   char* pws = nullptr;
   float *pXq = nullptr, *pH = nullptr, *pQ = nullptr, *pGs = nullptr, *pThr = nullptr, *pEbuf = nullptr,
-        *pBest = nullptr;
+        *pBest = nullptr, *pCover = nullptr;
   uint32_t* pList = nullptr;
   int* pCnt = nullptr;
   int* pFlag = nullptr;
@@ -726,7 +931,7 @@ int qsmi_lp_destroy(qsmi_lp* lp) {
   (void)hipSetDevice(lp->gpu);
   if (lp->stream) (void)hipStreamSynchronize(lp->stream);
   for (void* p : {(void*)lp->dW, (void*)lp->dG, (void*)lp->dM, (void*)lp->dV, (void*)lp->ws, (void*)lp->dKeys,
-                  (void*)lp->dKsample, (void*)lp->dSlot, (void*)lp->pws})
+                  (void*)lp->dKsample, (void*)lp->dKeysB, (void*)lp->dSlot, (void*)lp->dWqT, (void*)lp->pws})
     if (p) (void)hipFree(p);
   if (lp->stream) (void)hipStreamDestroy(lp->stream);
   delete lp;
@@ -922,10 +1127,13 @@ int qsmi_lp_build_keys(qsmi_lp* lp, const unsigned* tiSize, const float* ti, con
   (void)hipStreamSynchronize(st);
   if (lp->dKeys) (void)hipFree(lp->dKeys);
   if (lp->dKsample) (void)hipFree(lp->dKsample);
+  if (lp->dKeysB) (void)hipFree(lp->dKeysB);
   lp->dKeys = lp->dKsample = nullptr;
+  lp->dKeysB = nullptr;
   lp->keyS = 0;
   int rc = hchk(hipMalloc(&lp->dKeys, (size_t)S * EMB * sizeof(float)), "keys");
   if (!rc) rc = hchk(hipMalloc(&lp->dKsample, (size_t)kSample * EMB * sizeof(float)), "keys");
+  if (!rc) rc = hchk(hipMalloc(&lp->dKeysB, (size_t)S * EMB * sizeof(__bf16)), "keys");
   if (!rc && !lp->dSlot) rc = hchk(hipMalloc(&lp->dSlot, SLOT_W * SLOT_W * sizeof(int)), "slots");
   if (rc) return rc;
   mean_sigma(ti, (size_t)S, &lp->mu, &lp->sigma);
@@ -958,6 +1166,8 @@ int qsmi_lp_build_keys(qsmi_lp* lp, const unsigned* tiSize, const float* ti, con
     }
     hipLaunchKernelGGL(k_gather_rows, dim3((kSample * EMB + 255) / 256), dim3(256), 0, st, lp->dKeys, dRows, kSample,
                        lp->dKsample);
+    hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((S * EMB + 255) / 256)), dim3(256), 0, st, lp->dKeys, lp->dKeysB,
+                       S * EMB);
     rc = hchk(hipGetLastError(), "keys");
   }
   if (!rc) rc = hchk(hipStreamSynchronize(st), "sync");
@@ -984,7 +1194,8 @@ namespace lp {
 // the nodes the exact scan must skip.
 __global__ void k_lp_flags(int nN, int w0, const int* __restrict__ order, const NodeInfo* __restrict__ info,
                            const int* __restrict__ cnt, int cap, const float* __restrict__ best,
-                           const Cand* __restrict__ cand, int K, int every, float mseLimit, int audit,
+                           const Cand* __restrict__ cand, int K, int every, float mseLimit,
+                           const float* __restrict__ cover, float minCover, int audit,
                            int* __restrict__ flag, NodeInfo* __restrict__ infoEx,
                            unsigned long long* __restrict__ counters) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -993,18 +1204,25 @@ __global__ void k_lp_flags(int nN, int w0, const int* __restrict__ order, const 
   int f = 0;
   if (ni.nb > 0) {
     const int i = order[w0 + q];
-    const bool overflow = cnt[q] > cap;
+    bool overflow = false;
+    int total = 0;
+    for (int g = 0; g < NSEG; ++g) {
+      const int c = cnt[q * NSEG + g];
+      overflow |= c > cap / NSEG;
+      total += min(c, cap / NSEG);
+    }
     const bool few = !(cand[(long long)q * K + K - 1].key > -INFINITY);
     const bool periodic = every > 0 && (i % every) == 0;
     const float mse = -best[q] / (float)max(1, ni.nb - 1);
     const bool poor = mseLimit > 0.f && !(mse <= mseLimit);
-    f = (overflow || few || periodic || poor) ? 1 : 2;
+    const bool unseen = !(cover[q] >= minCover);
+    f = (overflow || few || periodic || poor || unseen) ? 1 : 2;
     if (f == 1) {
       atomicAdd(&counters[1], 1ull);
       if (overflow) atomicAdd(&counters[2], 1ull);
     } else {
       atomicAdd(&counters[0], 1ull);
-      atomicAdd(&counters[3], (unsigned long long)min(cnt[q], cap));
+      atomicAdd(&counters[3], (unsigned long long)total);
     }
   }
   flag[q] = f;
@@ -1053,8 +1271,9 @@ __global__ void __launch_bounds__(256) k_lp_combine(const int* __restrict__ flag
   // audit
   if (tid < K) found[tid] = 0;
   __syncthreads();
-  const int m = min(cnt[node], cap);
-  for (int i = tid; i < m; i += 256) {
+  const int segCap = cap / NSEG;
+  for (int i = tid; i < cap; i += 256) {
+    if ((i % segCap) >= cnt[node * NSEG + i / segCap]) continue;
     const uint32_t p = list[(long long)node * cap + i];
     for (int r = 0; r < K; ++r)
       if (ex[r].pos == p) found[r] = 1;
@@ -1112,8 +1331,9 @@ int lp_prepare(qsmi_lp* lp, const qsmi_ti* t, int maxNodes, int K, int shortlist
     return fail(QSMI_ERR_ARG, "learned proposal: keys were not built for this TI");
   if (t->D != 2) return fail(QSMI_ERR_UNSUPPORTED, "learned proposal: 2-D only");
   if (V > 4) return fail(QSMI_ERR_UNSUPPORTED, "learned proposal: at most 4 active sub-variables");
+  if (maxNodes > 256) return fail(QSMI_ERR_UNSUPPORTED, "learned proposal: waves of at most 256 nodes");
   const int L = shortlist > 0 ? shortlist : 4096;
-  const int cap = 2 * L;
+  const int cap = NSEG * ((2 * L + NSEG - 1) / NSEG);  // room for twice the target, in NSEG equal segments
   if (!(lp->pws && lp->pMaxNodes >= maxNodes && lp->pK == K && lp->pShort == L)) {
   if (lp->pws) (void)hipFree(lp->pws);
   lp->pws = nullptr;
@@ -1122,8 +1342,9 @@ int lp_prepare(qsmi_lp* lp, const qsmi_ti* t, int maxNodes, int K, int shortlist
       {(void**)&lp->pXq, n * IN * 4},          {(void**)&lp->pH, 3 * n * HID * 4},
       {(void**)&lp->pQ, n * EMB * 4},          {(void**)&lp->pGs, n * kSample * 4},
       {(void**)&lp->pThr, n * 4},              {(void**)&lp->pList, n * cap * 4},
-      {(void**)&lp->pCnt, n * 4},              {(void**)&lp->pEbuf, n * cap * 4},
+      {(void**)&lp->pCnt, n * NSEG * 4},       {(void**)&lp->pEbuf, n * cap * 4},
       {(void**)&lp->pBest, n * 4},             {(void**)&lp->pFlag, n * 4},
+      {(void**)&lp->pCover, n * 4},
       {(void**)&lp->pCand, n * K * sizeof(Cand)}, {(void**)&lp->pInfoEx, n * sizeof(NodeInfo)},
       {(void**)&lp->pCounters, 8 * 8},         {(void**)&lp->pAct, 4 * 4}};
   size_t total = 0;
@@ -1139,8 +1360,20 @@ int lp_prepare(qsmi_lp* lp, const qsmi_ti* t, int maxNodes, int K, int shortlist
   lp->pShort = L;
   lp->pCap = cap;
   }
+  if (!lp->dWqT && hipMalloc(&lp->dWqT, (size_t)(3 * HID + EMB) * KP * sizeof(__bf16)) != hipSuccess)
+    return fail(QSMI_ERR_NOMEM, "learned proposal: transposed weights");
+  {
+    long long o = 0;
+    for (int l = 0; l < 4; ++l) {
+      hipLaunchKernelGGL(k_wt_bf16, dim3((kDout[l] * KP + 255) / 256), dim3(256), 0, lp->stream, lp->dW + kWoff[l],
+                         kDin[l], kDout[l], lp->dWqT + o);
+      o += (long long)kDout[l] * KP;
+    }
+    if (hipStreamSynchronize(lp->stream) != hipSuccess) return fail(QSMI_ERR_HIP, "learned proposal: weights");
+  }
   int act[4] = {0, 0, 0, 0};
   for (int a = 0; a < V; ++a) act[a] = active[a];
+  lp->pYY = (t->nSub == 2 && V == 2 && active[0] == 0 && active[1] == 1) ? 1 : 0;
   if (hipMemcpy(lp->pAct, act, sizeof(act), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(lp->pCounters, 0, 8 * 8) != hipSuccess)
     return fail(QSMI_ERR_HIP, "learned proposal: set-up copies");
@@ -1148,47 +1381,41 @@ int lp_prepare(qsmi_lp* lp, const qsmi_ti* t, int maxNodes, int K, int shortlist
 }
 
 void lp_propose(qsmi_lp* lp, const qsmi_ti* t, int V, int nN, const NodeInfo* info, const int* nbOff,
-                const float* nbVal, const float* di, int nVar, int sd0, int K, hipStream_t st) {
+                const float* nbVal, const float* di, int nVar, int sd0, int K, float minCover, hipStream_t st) {
   using namespace lp;
   Acts a;
   a.X = lp->pXq;
   for (int l = 0; l < 3; ++l) a.H[l] = lp->pH + (size_t)l * lp->pMaxNodes * HID;
   a.E = lp->pQ;
   hipLaunchKernelGGL(k_wave_features, dim3(nN), dim3(128), 0, st, info, nbOff, di, nVar, sd0, lp->dSlot, lp->mu,
-                     1.f / lp->sigma, a.X);
-  forward(lp, 0, a, nN, st);
+                     1.f / lp->sigma, a.X, lp->pCover);
+  hipLaunchKernelGGL(k_mlp_fwd, dim3((nN + 31) / 32), dim3(256), 0, st, nN, a.X, lp->dWqT, lp->dW, a.E);
   // per-node threshold: the r-th best sampled score, r = L * sample / S
   gemm(false, true, gargs(nN, kSample, EMB, a.E, EMB, lp->dKsample, EMB, lp->pGs, kSample), st);
   const int r = (int)std::max<long long>(1, ((long long)lp->pShort * kSample + lp->keyS - 1) / lp->keyS);
-  hipLaunchKernelGGL(k_row_kth, dim3(nN), dim3(256), 0, st, lp->pGs, kSample, r, lp->pThr);
-  (void)hipMemsetAsync(lp->pCnt, 0, (size_t)nN * sizeof(int), st);
-  GemmArgs g = gargs(nN, (int)lp->keyS, EMB, a.E, EMB, lp->dKeys, EMB, nullptr, 0);
-  g.thr = lp->pThr;
-  g.list = lp->pList;
-  g.cnt = lp->pCnt;
-  g.cap = lp->pCap;
-  gemm_select(g, st);
-  const long long S = t->S;
-  if (K <= 2)
-    hipLaunchKernelGGL(k_rescore<2>, dim3(nN), dim3(256), 0, st, info, nbOff, nbVal, V, lp->pAct, t->dSub, S, t->s[0],
-                       lp->pList, lp->pCnt, lp->pCap, K, lp->pEbuf, lp->pCand, lp->pBest);
-  else if (K <= 4)
-    hipLaunchKernelGGL(k_rescore<4>, dim3(nN), dim3(256), 0, st, info, nbOff, nbVal, V, lp->pAct, t->dSub, S, t->s[0],
-                       lp->pList, lp->pCnt, lp->pCap, K, lp->pEbuf, lp->pCand, lp->pBest);
-  else if (K <= 8)
-    hipLaunchKernelGGL(k_rescore<8>, dim3(nN), dim3(256), 0, st, info, nbOff, nbVal, V, lp->pAct, t->dSub, S, t->s[0],
-                       lp->pList, lp->pCnt, lp->pCap, K, lp->pEbuf, lp->pCand, lp->pBest);
-  else
-    hipLaunchKernelGGL(k_rescore<16>, dim3(nN), dim3(256), 0, st, info, nbOff, nbVal, V, lp->pAct, t->dSub, S, t->s[0],
-                       lp->pList, lp->pCnt, lp->pCap, K, lp->pEbuf, lp->pCand, lp->pBest);
+  hipLaunchKernelGGL(k_row_kth, dim3(nN), dim3(256), 0, st, lp->pGs, kSample, r, lp->pCover, minCover, lp->pThr);
+  (void)hipMemsetAsync(lp->pCnt, 0, (size_t)nN * NSEG * sizeof(int), st);
+  hipLaunchKernelGGL(k_score, dim3(512), dim3(256), 0, st, nN, lp->keyS, a.E, lp->dKeysB, lp->pThr, lp->pList,
+                     lp->pCnt, lp->pCap);
+  hipLaunchKernelGGL(k_rescore, dim3((lp->pCap + kRescoreChunk - 1) / kRescoreChunk, nN), dim3(256), 0, st, info,
+                     nbOff, nbVal, V, lp->pYY, lp->pAct, t->dSub, t->S, t->s[0], lp->pList, lp->pCnt, lp->pCap,
+                     lp->pEbuf);
+#define QSMI_LP_SELECT(KT)                                                                                     \
+  hipLaunchKernelGGL(k_rescore_select<KT>, dim3(nN), dim3(256), 0, st, info, lp->pList, lp->pCnt, lp->pCap, K, \
+                     lp->pEbuf, lp->pCand, lp->pBest)
+  if (K <= 2) QSMI_LP_SELECT(2);
+  else if (K <= 4) QSMI_LP_SELECT(4);
+  else if (K <= 8) QSMI_LP_SELECT(8);
+  else QSMI_LP_SELECT(16);
+#undef QSMI_LP_SELECT
 }
 
 void lp_flags(qsmi_lp* lp, int nN, int w0, const int* order, const NodeInfo* info, int K, int every, float mseSigma2,
-              int audit, hipStream_t st) {
+              float minCover, int audit, hipStream_t st) {
   using namespace lp;
   hipLaunchKernelGGL(k_lp_flags, dim3((nN + 127) / 128), dim3(128), 0, st, nN, w0, order, info, lp->pCnt, lp->pCap,
-                     lp->pBest, lp->pCand, K, every, mseSigma2 > 0.f ? mseSigma2 * lp->sigma * lp->sigma : 0.f, audit,
-                     lp->pFlag, lp->pInfoEx, lp->pCounters);
+                     lp->pBest, lp->pCand, K, every, mseSigma2 > 0.f ? mseSigma2 * lp->sigma * lp->sigma : 0.f,
+                     lp->pCover, minCover, audit, lp->pFlag, lp->pInfoEx, lp->pCounters);
 }
 
 void lp_combine(qsmi_lp* lp, int nN, const NodeInfo* info, const Cand* candEx, int blocks, int rowsPerBlock, int K,

@@ -261,8 +261,8 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
     if (ev) (void)hipEventRecord(ev[1], st);
     if (R.lp) {
       // learned proposal for every node, the exact scan for the routed ones
-      lp_propose(R.lp, t, P.V, nN, b->dInfo, b->dOff, b->dVal, R.dDi, R.nVar, R.sd[0], K, st);
-      lp_flags(R.lp, nN, w0, R.dOrder, b->dInfo, K, R.lpEvery, R.lpMse, R.lpAudit, st);
+      lp_propose(R.lp, t, P.V, nN, b->dInfo, b->dOff, b->dVal, R.dDi, R.nVar, R.sd[0], K, R.lpMinCover, st);
+      lp_flags(R.lp, nN, w0, R.dOrder, b->dInfo, K, R.lpEvery, R.lpMse, R.lpMinCover, R.lpAudit, st);
       const NodeInfo* infoEx = lp_info_ex(R.lp);
       if ((rc = launch_spectrum(dev, g, nN, infoEx, b->dOff, b->dVal, b->dC))) return rc;
       if (ev) (void)hipEventRecord(ev[2], st);

@@ -228,6 +228,9 @@ typedef struct {
   int lpAudit;             /* 1: also run the exact scan for every node and
                               count how many exact k-best positions the
                               proposals held (slow; for measurements) */
+  float lpMinCover;        /* exact scan when fewer than this fraction of a
+                              node's neighbours lie on the key stencil (the
+                              encoders cannot see the others; <= 0 -> 0.5) */
 } qsmi_qs_request;
 
 typedef struct {

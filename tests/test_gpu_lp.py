@@ -11,7 +11,9 @@
   2e-2 of their scale and gradients per tensor at 3e-2 relative L2;
 - training on recorded pairs lowers the loss, and the trained network's
   proposals hold the exact k-best positions far more often than the same
-  network untrained (the audit of a learned-proposal realisation).
+  network untrained (the audit of a learned-proposal realisation);
+- a learned-proposal realisation matches a plain one's histogram and
+  variogram within stated tolerances.
 """
 import numpy as np
 import pytest
@@ -107,24 +109,67 @@ def _audit(net, ti, dims, sim, seed):
     return st["lp_audit_best_hits"] / max(1, n), st["lp_audit_hits"] / max(1, 2 * n), st
 
 
-def test_training_gains_over_untrained():
+@pytest.fixture(scope="module")
+def trained():
     size, sim = 200, 100
     sets = [_record(s, size, sim) for s in (211, 212, 213)]
-    trained = L.LearnedProposal(seed=11)
-    untrained = L.LearnedProposal(seed=11)
-    first = last = None
+    m = L.LearnedProposal(seed=11)
+    curve = []
     for epoch in range(3):
         for ti, dims, pairs in sets:
-            loss = trained.train(ti, dims, pairs, steps=200, batch=256, negatives=768, lr=1e-3, seed=epoch)
-            first = loss[0] if first is None else first
-            last = loss[-1]
+            curve.append(m.train(ti, dims, pairs, steps=200, batch=256, negatives=768, lr=1e-3, seed=epoch))
+    yield m, curve
+    m.close()
+
+
+def test_training_gains_over_untrained(trained):
+    net_t, curve = trained
+    first, last = curve[0][0], curve[-1][-1]
     print(f"  training loss {first:.3f} -> {last:.3f}")
     assert last < 0.7 * first
+    size, sim = 200, 100
+    untrained = L.LearnedProposal(seed=11)
     ti, dims = _ti(219, size)
-    b_t, r_t, st = _audit(trained, ti, dims, sim, 5)
+    b_t, r_t, st = _audit(net_t, ti, dims, sim, 5)
     b_u, r_u, _ = _audit(untrained, ti, dims, sim, 5)
     print(f"  best-position recall: trained {b_t:.3f} untrained {b_u:.3f}; k-best recall {r_t:.3f} vs {r_u:.3f}")
     print(f"  proposals {st['lp_proposed']} fallbacks {st['lp_fallbacks']} overflows {st['lp_overflows']}")
     assert b_t > b_u + 0.2 and r_t > r_u + 0.2
-    trained.close()
     untrained.close()
+
+
+def _variogram(img, hmax=10):
+    g = []
+    for h in range(1, hmax + 1):
+        gx = np.mean((img[:, h:] - img[:, :-h]) ** 2) / 2
+        gy = np.mean((img[h:, :] - img[:-h, :]) ** 2) / 2
+        g.append((gx + gy) / 2)
+    return np.array(g)
+
+
+def test_learned_realisation_statistics(trained):
+    """The learned form (shortlist + exact rescoring, every 64th node and
+    poor proposals on the exact scan) against plain QS on the same TI and
+    path: histogram within a KS distance of 0.08 and variogram within 20 %
+    at lags 1..10 (both realisations carry their own sampling noise; the
+    tolerances hold for exact-vs-exact realisations with different seeds)."""
+    net_t, _ = trained
+    size, sim = 200, 100
+    ti, dims = _ti(219, size)
+    net_t.build_keys(ti, dims)
+    di = np.full(sim * sim, np.nan, np.float32)
+    out = {}
+    for name, kw in (("exact", {}), ("learned", dict(lp=net_t, lp_shortlist=512, lp_fallback_every=64,
+                                                      lp_fallback_mse=0.5))):
+        sim_vals, idx, st = g2s_amd.qs(ti, dims, di, (sim, sim), [0], 80, 1.5, seed=31, **kw)
+        assert not np.isnan(sim_vals).any()
+        # every value is the TI value at its recorded position
+        assert np.array_equal(sim_vals, ti[idx])
+        out[name] = sim_vals.reshape(sim, sim)
+    a, b = np.sort(out["exact"].ravel()), np.sort(out["learned"].ravel())
+    grid = np.concatenate([a, b])
+    ks = np.max(np.abs(np.searchsorted(a, grid, side="right") - np.searchsorted(b, grid, side="right"))) / a.size
+    va, vb = _variogram(np.log(out["exact"])), _variogram(np.log(out["learned"]))
+    rel = np.max(np.abs(va - vb) / va)
+    print(f"  KS {ks:.3f}, variogram max rel diff {rel:.3f}")
+    assert ks < 0.08 and rel < 0.2
