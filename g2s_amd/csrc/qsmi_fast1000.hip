@@ -222,10 +222,14 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) k
               const uint32_t qd = (uint32_t)((float)p * 0.001f);
               const float2 w = twl[p - qd * 1000u];
               const float2 g0 = G[2 * t], g1 = G[2 * t + 1];
-              s0.x += g0.x * w.x - g0.y * w.y;
-              s0.y += g0.x * w.y + g0.y * w.x;
-              s1.x += g1.x * w.x - g1.y * w.y;
-              s1.y += g1.x * w.y + g1.y * w.x;
+              s0.x = fmaf(g0.x, w.x, s0.x);
+              s0.x = fmaf(-g0.y, w.y, s0.x);
+              s0.y = fmaf(g0.x, w.y, s0.y);
+              s0.y = fmaf(g0.y, w.x, s0.y);
+              s1.x = fmaf(g1.x, w.x, s1.x);
+              s1.x = fmaf(-g1.y, w.y, s1.x);
+              s1.y = fmaf(g1.x, w.y, s1.y);
+              s1.y = fmaf(g1.y, w.x, s1.y);
             }
             S0[c] = s0;
             S1[c] = s1;
@@ -235,8 +239,10 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) k
 #pragma unroll
           for (int q = 0; q < 10; ++q) {
             const float2 y = Y0[lane + 100 * q];
-            M[q].x += y.x * S0[q].x - y.y * S0[q].y;
-            M[q].y += y.x * S0[q].y + y.y * S0[q].x;
+            M[q].x = fmaf(y.x, S0[q].x, M[q].x);
+            M[q].x = fmaf(-y.y, S0[q].y, M[q].x);
+            M[q].y = fmaf(y.x, S0[q].y, M[q].y);
+            M[q].y = fmaf(y.y, S0[q].x, M[q].y);
           }
           if (two) {
             dft10<-1>(S1);
@@ -244,8 +250,10 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) k
 #pragma unroll
             for (int q = 0; q < 10; ++q) {
               const float2 y = Y1[lane + 100 * q];
-              M[q].x += y.x * S1[q].x - y.y * S1[q].y;
-              M[q].y += y.x * S1[q].y + y.y * S1[q].x;
+              M[q].x = fmaf(y.x, S1[q].x, M[q].x);
+              M[q].x = fmaf(-y.y, S1[q].y, M[q].x);
+              M[q].y = fmaf(y.x, S1[q].y, M[q].y);
+              M[q].y = fmaf(y.y, S1[q].x, M[q].y);
             }
           }
         }
@@ -280,8 +288,10 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) k
           for (int q = 0; q < 10; ++q) {
             const float2 y = Y[lane + 100 * q];
             const float2 x = u[q];
-            M[q].x += y.x * x.x - y.y * x.y;
-            M[q].y += y.x * x.y + y.y * x.x;
+            M[q].x = fmaf(y.x, x.x, M[q].x);
+            M[q].x = fmaf(-y.y, x.y, M[q].x);
+            M[q].y = fmaf(y.x, x.y, M[q].y);
+            M[q].y = fmaf(y.y, x.x, M[q].y);
           }
         }
         __syncthreads();

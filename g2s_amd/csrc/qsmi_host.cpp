@@ -337,82 +337,54 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   const auto t1 = clk::now();
 
   // ---- path loop -----------------------------------------------------------
-  // posterior order (src/qs.cpp:1615-1641) and path index of every cell
-  std::vector<unsigned> post(SD, UINT_MAX);
-  std::vector<int> pidx(SD, -1);
-  for (size_t c = 0; c < SD; ++c)
-    if (informed[c]) post[c] = 0;
-  for (size_t i = 0; i < nPath; ++i) {
-    post[path[begin + i]] = (unsigned)i;
-    pidx[path[begin + i]] = (int)i;
-  }
-  // neighbour sets of every node (simulation.hpp:447-511), centre first
+  // device-resident state
+  std::vector<void*> allocs;
+  auto dev_upload = [&](const void* src, size_t bytes, void** dst) -> int {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 4));
+    if (e != hipSuccess) return hfail(QSMI_ERR_NOMEM, std::string("path loop state: ") + hipGetErrorString(e));
+    allocs.push_back(p);
+    if (src && bytes) {
+      e = hipMemcpy(p, src, bytes, hipMemcpyHostToDevice);
+      if (e != hipSuccess) return hfail(QSMI_ERR_HIP, std::string("path loop upload: ") + hipGetErrorString(e));
+    }
+    *dst = p;
+    return QSMI_OK;
+  };
+  // path index of every cell, -1 for hard data (the posterior order of
+  // src/qs.cpp:1615-1641: hard data first, then the path)
   const int n = rq->n;
-  std::vector<int> nbIdx(nPath * (size_t)n);
+  std::vector<int> postS(SD, -1);
+  for (size_t i = 0; i < nPath; ++i) postS[path[begin + i]] = (int)i;
+  // neighbour sets of every node (simulation.hpp:447-511), centre first, on
+  // the device; their path indices come back for the dependency levels
+  void *dCells = nullptr, *dPostS = nullptr, *dOffs = nullptr, *dNb = nullptr, *dPj = nullptr, *dCnt = nullptr;
+  if (!rc) rc = dev_upload(path.data() + begin, nPath * sizeof(unsigned), &dCells);
+  if (!rc) rc = dev_upload(postS.data(), SD * sizeof(int), &dPostS);
+  if (!rc) rc = dev_upload(offs.data(), (size_t)nOff * D * sizeof(int), &dOffs);
+  if (!rc) rc = dev_upload(nullptr, nPath * (size_t)n * sizeof(int), &dNb);
+  if (!rc) rc = dev_upload(nullptr, nPath * (size_t)n * sizeof(int), &dPj);
+  if (!rc) rc = dev_upload(nullptr, nPath * sizeof(int), &dCnt);
+  if (!rc)
+    rc = qsmi::nb_search_device(D, sd, nPath, (const unsigned*)dCells, (const int*)dPostS, (const int*)dOffs, nOff, n,
+                                (int*)dNb, (int*)dPj, (int*)dCnt, dev->stream);
+  std::vector<int> nbPj(nPath * (size_t)n);
   std::vector<int> nbCnt(nPath, 0);
-  {
-    unsigned hw = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
-    std::atomic<size_t> next(0);
-    auto worker = [&]() {
-      const size_t chunk = 256;
-      while (true) {
-        size_t b = next.fetch_add(chunk);
-        if (b >= nPath) break;
-        size_t e = std::min(nPath, b + chunk);
-        for (size_t i = b; i < e; ++i) {
-          const unsigned cell = path[begin + i];
-          int cc[3] = {0, 0, 0};
-          size_t rem = cell;
-          for (int d = 0; d < D; ++d) {
-            cc[d] = (int)(rem % sd[d]);
-            rem /= sd[d];
-          }
-          int cnt = 0;
-          int* out = nbIdx.data() + i * (size_t)n;
-          for (int o = 0; o < nOff && cnt < n; ++o) {
-            const int* dd = offs.data() + (size_t)o * D;
-            bool inside = true;
-            size_t t = 0, mul = 1;
-            for (int d = 0; d < D; ++d) {
-              const int x = cc[d] + dd[d];
-              inside &= (x >= 0 && x < (int)sd[d]);
-              t += (size_t)x * mul;
-              mul *= sd[d];
-            }
-            if (inside && post[t] <= (unsigned)i) out[cnt++] = o;
-          }
-          nbCnt[i] = cnt;
-        }
-      }
-    };
-    std::vector<std::thread> th;
-    for (unsigned q = 0; q < hw; ++q) th.emplace_back(worker);
-    for (auto& t : th) t.join();
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(dev->stream);
+    if (e == hipSuccess) e = hipMemcpy(nbCnt.data(), dCnt, nPath * sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(nbPj.data(), dPj, nbPj.size() * sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hfail(QSMI_ERR_HIP, std::string("neighbour search: ") + hipGetErrorString(e));
   }
   const auto tNb = clk::now();
-  // dependency levels
+  // dependency levels: 1 + the deepest level among a node's path neighbours
   std::vector<int> level(nPath, 0);
   int maxLevel = 0;
   for (size_t i = 0; i < nPath; ++i) {
-    const unsigned cell = path[begin + i];
-    int cc[3] = {0, 0, 0};
-    size_t rem = cell;
-    for (int d = 0; d < D; ++d) {
-      cc[d] = (int)(rem % sd[d]);
-      rem /= sd[d];
-    }
     int lv = 0;
-    const int* nb = nbIdx.data() + i * (size_t)n;
-    for (int j = 0; j < nbCnt[i]; ++j) {
-      const int* dd = offs.data() + (size_t)nb[j] * D;
-      size_t t = 0, mul = 1;
-      for (int d = 0; d < D; ++d) {
-        t += (size_t)(cc[d] + dd[d]) * mul;
-        mul *= sd[d];
-      }
-      const int pj = pidx[t];
-      if (pj >= 0 && (size_t)pj != i) lv = std::max(lv, level[pj] + 1);
-    }
+    const int* pj = nbPj.data() + i * (size_t)n;
+    for (int j = 0; j < nbCnt[i]; ++j)
+      if (pj[j] >= 0 && (size_t)pj[j] != i) lv = std::max(lv, level[pj[j]] + 1);
     level[i] = lv;
     maxLevel = std::max(maxLevel, lv);
   }
@@ -464,29 +436,12 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     R.subOf[B] = subVarOf[B];
   }
   R.stageTiming = rq->stageTiming != 0;
-  // device-resident state
-  std::vector<void*> allocs;
-  auto dev_upload = [&](const void* src, size_t bytes, void** dst) -> int {
-    void* p = nullptr;
-    hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 4));
-    if (e != hipSuccess) return hfail(QSMI_ERR_NOMEM, std::string("path loop state: ") + hipGetErrorString(e));
-    allocs.push_back(p);
-    if (src && bytes) {
-      e = hipMemcpy(p, src, bytes, hipMemcpyHostToDevice);
-      if (e != hipSuccess) return hfail(QSMI_ERR_HIP, std::string("path loop upload: ") + hipGetErrorString(e));
-    }
-    *dst = p;
-    return QSMI_OK;
-  };
   std::vector<int> orderI(order.begin(), order.end());
-  void *dOrder, *dCells, *dSeeds, *dNb, *dCnt, *dOffs, *dKer, *dCat, *dDi, *dIdx, *dTi;
+  void *dOrder = nullptr, *dSeeds = nullptr, *dKer = nullptr, *dCat = nullptr, *dDi = nullptr, *dIdx = nullptr,
+       *dTi = nullptr;
   std::vector<float> catv = catvals.empty() ? std::vector<float>(1, 0.f) : catvals;
   if (!rc) rc = dev_upload(orderI.data(), nPath * sizeof(int), &dOrder);
-  if (!rc) rc = dev_upload(path.data() + begin, nPath * sizeof(unsigned), &dCells);
   if (!rc) rc = dev_upload(seeds.data() + begin, nPath * sizeof(float), &dSeeds);
-  if (!rc) rc = dev_upload(nbIdx.data(), nbIdx.size() * sizeof(int), &dNb);
-  if (!rc) rc = dev_upload(nbCnt.data(), nbCnt.size() * sizeof(int), &dCnt);
-  if (!rc) rc = dev_upload(offs.data(), (size_t)nOff * D * sizeof(int), &dOffs);
   if (!rc) rc = dev_upload(ker.data.data(), ker.data.size() * sizeof(float), &dKer);
   if (!rc) rc = dev_upload(catv.data(), catv.size() * sizeof(float), &dCat);
   if (!rc) rc = dev_upload(di, SD * nVar * sizeof(float), &dDi);

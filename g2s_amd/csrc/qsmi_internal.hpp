@@ -177,6 +177,10 @@ void lp_record(int nN, int w0, const int* order, const NodeInfo* info, const int
                int sd0, const uint32_t* outPos, int K, int cap, int* recOff, float* recVal, uint32_t* recBest,
                hipStream_t st);
 
+// neighbour sets of every path node on the device (stream-ordered)
+int nb_search_device(int D, const unsigned* sd, size_t nPath, const unsigned* dCells, const int* dPostS,
+                     const int* dOffs, int nOff, int n, int* dNbIdx, int* dNbPj, int* dNbCnt, hipStream_t st);
+
 // stageMs: [0] spectrum (+ dim1 inverse), [1] gather, [2] rows + select,
 // [3] merge + draw + write; filled when R.stageTiming.
 int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMs, long long* scanned, int* waves);

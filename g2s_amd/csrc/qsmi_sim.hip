@@ -140,31 +140,50 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, const int
       mx[d] = max(mx[d], __shfl_xor(mx[d], off));
     }
   }
-  if (lane == 0) {
-    // delta0 in the reference's order: variable by variable, neighbour by
-    // neighbour, float accumulation of -w x^2 read at expanded index v
-    float delta0 = 0.f;
+  // delta0 in the reference's order: variable by variable, neighbour by
+  // neighbour, float accumulation of -w x^2 read at expanded index v. The
+  // terms are formed in parallel (one lane per neighbour) and summed by lane
+  // 0 in that order, so the float result is the sequential one.
+  extern __shared__ float term[];  // [nVar][n] terms, then [nVar][n] presence flags
+  unsigned char* has = reinterpret_cast<unsigned char*>(term + P.nVar * P.n);
+  const int nT = cnt;
+  for (int j = lane; j < nT; j += 64) {
+    const int* dd = offs + (long long)idx[j] * D;
+    long long t = 0, mul = 1, kc = 0;
+    bool inK = true;
+    for (int d = 0; d < D; ++d) {
+      t += (long long)(cc[d] + dd[d]) * mul;
+      mul *= P.sd[d];
+    }
+    for (int d = D - 1; d >= 0; --d) {
+      int c = P.kctr[d] + dd[d];
+      inK &= (c >= 0 && c < P.kd[d]);
+      c = ((c % P.kd[d]) + P.kd[d]) % P.kd[d];
+      kc = kc * P.kd[d] + c;
+    }
     for (int v = 0; v < P.nVar; ++v) {
-      if (P.types[v] != 0) continue;
-      for (int j = 0; j < cnt; ++j) {
-        const int* dd = offs + (long long)idx[j] * D;
-        long long t = 0, mul = 1, kc = 0;
-        bool inK = true;
-        for (int d = 0; d < D; ++d) {
-          t += (long long)(cc[d] + dd[d]) * mul;
-          mul *= P.sd[d];
-        }
-        for (int d = D - 1; d >= 0; --d) {
-          int c = P.kctr[d] + dd[d];
-          inK &= (c >= 0 && c < P.kd[d]);
-          c = ((c % P.kd[d]) + P.kd[d]) % P.kd[d];
-          kc = kc * P.kd[d] + c;
-        }
+      float tv = 0.f;
+      bool h = false;
+      if (P.types[v] == 0) {
         const int e = v;  // the reference reads the expanded value at the original index
         const float xv = di[t * P.nVar + P.expVar[e]];
         const float xe = (P.expCat[e] < 0) ? xv : ((xv == catvals[P.expCat[e]]) ? 1.f : 0.f);
-        if (inK && !isnan(xe)) delta0 += -1.f * (kern[kc * P.nExp + (v % P.nExp)] * xe * xe);
+        if (inK && !isnan(xe)) {
+          tv = -1.f * (kern[kc * P.nExp + (v % P.nExp)] * xe * xe);
+          h = true;
+        }
       }
+      term[v * P.n + j] = tv;
+      has[v * P.n + j] = h ? 1 : 0;
+    }
+  }
+  __syncthreads();
+  if (lane == 0) {
+    float delta0 = 0.f;
+    for (int v = 0; v < P.nVar; ++v) {
+      if (P.types[v] != 0) continue;
+      for (int j = 0; j < nT; ++j)
+        if (has[v * P.n + j]) delta0 += term[v * P.n + j];
     }
     for (int d = 0; d < 3; ++d) {
       if (d >= D) {
@@ -183,12 +202,80 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, const int
   }
 }
 
+// ---------------------------------------------------------------------------
+// Neighbour sets of every path node on the device (include/simulation.hpp:
+// 447-511: walk the distance-sorted offset list and take the first n cells
+// informed before the node, the cell itself first). One wavefront per node
+// tests 64 offsets at a time; a ballot keeps the hits in offset order, so the
+// result is the serial walk's. postS[cell] = -1 for hard data, the path index
+// otherwise. Outputs: nbIdx [node][n] offset indices, nbPj [node][n] the
+// neighbours' path indices (-1: hard data), nbCnt [node].
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) k_nbsearch(int D, int sd0, int sd1, int sd2, const unsigned* __restrict__ cells,
+                                                const int* __restrict__ postS, const int* __restrict__ offs, int nOff,
+                                                int n, int* __restrict__ nbIdx, int* __restrict__ nbPj,
+                                                int* __restrict__ nbCnt) {
+  const long long i = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int sd[3] = {sd0, sd1, sd2};
+  int cc[3] = {0, 0, 0};
+  {
+    unsigned rem = cells[i];
+    for (int d = 0; d < D; ++d) {
+      cc[d] = (int)(rem % (unsigned)sd[d]);
+      rem /= (unsigned)sd[d];
+    }
+  }
+  int* out = nbIdx + i * n;
+  int* outP = nbPj + i * n;
+  int cnt = 0;
+  for (int base = 0; base < nOff && cnt < n; base += 64) {
+    const int o = base + lane;
+    bool hit = false;
+    int pj = -1;
+    if (o < nOff) {
+      bool inside = true;
+      long long t = 0, mul = 1;
+      for (int d = 0; d < D; ++d) {
+        const int x = cc[d] + offs[(long long)o * D + d];
+        inside &= (x >= 0 && x < sd[d]);
+        t += (long long)x * mul;
+        mul *= sd[d];
+      }
+      if (inside) {
+        pj = postS[t];
+        hit = pj <= (int)i;
+      }
+    }
+    const unsigned long long m = __ballot(hit);
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (hit && cnt + before < n) {
+      out[cnt + before] = o;
+      outP[cnt + before] = pj;
+    }
+    cnt += __popcll(m);
+  }
+  if (lane == 0) nbCnt[i] = min(cnt, n);
+}
+
+int nb_search_device(int D, const unsigned* sd, size_t nPath, const unsigned* dCells, const int* dPostS,
+                     const int* dOffs, int nOff, int n, int* dNbIdx, int* dNbPj, int* dNbCnt, hipStream_t st) {
+  if (nPath == 0) return QSMI_OK;
+  hipLaunchKernelGGL(k_nbsearch, dim3((unsigned)nPath), dim3(64), 0, st, D, (int)sd[0], (int)sd[1], (int)sd[2], dCells,
+                     dPostS, dOffs, nOff, n, dNbIdx, dNbPj, dNbCnt);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(QSMI_ERR_HIP, std::string("neighbour search: ") + hipGetErrorString(e));
+  return QSMI_OK;
+}
+
 int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsOut, long long* scannedOut,
                        int* wavesOut) {
   qsmi_ti* t = dev->ti;
   hipStream_t st = dev->stream;
   const int D = t->D;
   if (R.nVar > 8 || R.nExp > 16) return fail(QSMI_ERR_UNSUPPORTED, "too many variables for the device gather");
+  const size_t gatherLds = (size_t)R.nVar * R.n * (sizeof(float) + 1);
+  if (gatherLds > 60 * 1024) return fail(QSMI_ERR_UNSUPPORTED, "n x nVar too large for the device gather (<= 12288)");
   GatherParams P;
   memset(&P, 0, sizeof(P));
   P.D = D;
@@ -256,7 +343,7 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
     if (nN <= 0) continue;
     hipEvent_t* ev = timing ? &dev->evPool[w * 5] : nullptr;
     if (ev) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL(k_gather, dim3(nN), dim3(64), 0, st, P, w0, R.dOrder, R.dPathCells, R.dPathSeeds, R.dNbIdx,
+    hipLaunchKernelGGL(k_gather, dim3(nN), dim3(64), gatherLds, st, P, w0, R.dOrder, R.dPathCells, R.dPathSeeds, R.dNbIdx,
                        R.dNbCnt, R.dOffs, R.dKernel, R.dCatvals, R.dDi, R.dIndex, R.dTi, b->dInfo, b->dOff, b->dVal);
     if (ev) (void)hipEventRecord(ev[1], st);
     if (R.lp) {
