@@ -331,14 +331,15 @@ __global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k(ScanGeom 
   using namespace f1k;
   __shared__ float2 buf[G][L];
   __shared__ float2 twl[L];
-  __shared__ TopkScratch red;
   const int tid = threadIdx.x;
   const int grp = tid / 100;
   const int j = tid - 100 * grp;
   const int node = blockIdx.y;
   const NodeInfo ni = info[node];
   const int h0 = g.h0;
-  Cand* out = cand ? cand + ((long long)node * candBlocks + blockIdx.x) * K : nullptr;
+  // one candidate group of K per wavefront of the block
+  const int wavesPerBlock = blockDim.x >> 6;
+  Cand* out = cand ? cand + ((long long)node * candBlocks + (long long)blockIdx.x * wavesPerBlock) * K : nullptr;
   const int p = blockIdx.x * G + grp;  // this group's pair
   const int ca = 2 * p;
   const bool gin = grp < G && p < L / 2;
@@ -347,7 +348,7 @@ __global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k(ScanGeom 
   // block-uniform exit when none of the block's rows is valid
   const int rlo = max(ni.lo[1], 2 * blockIdx.x * G), rhi = min(ni.hi[1], 2 * (blockIdx.x * G + G) - 1);
   if (rlo > rhi || ni.lo[0] > ni.hi[0]) {
-    if (out && tid < K) {
+    if (out && tid < K * wavesPerBlock) {
       Cand w;
       w.key = -INFINITY;
       w.h = 0u;
@@ -437,7 +438,7 @@ __global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k(ScanGeom 
     }
   }
   // tau2 also excludes the -inf of invalid positions
-  const float tau2 = fmaxf(block_kth_key<KT>(m, K, &red), -3.402823466e38f);
+  const float tau2 = fmaxf(wave_kth_key<KT>(m, K), -3.402823466e38f);
   // pass 2: exact device rule among the keys >= tau (normally K per block)
   RankList<KT> rl;
   rl.init();
@@ -449,7 +450,7 @@ __global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k(ScanGeom 
       rl.insert(key[e], tie_hash(ni.seedbits, pos), pos, (w ? u[q].y : u[q].x) * scale + d0);
     }
   }
-  rl.block_extract(K, out, &red);
+  wave_extract<KT>(rl, K, out + (threadIdx.x >> 6) * K);
 }
 
 // YT[A][k0][k1] <- Yhat[A][k1][k0] (once per TI)
@@ -494,24 +495,28 @@ void launch_pencil1k(const ScanGeom& g, const float2* YT, const float2* tw1k, in
 // row pairs per rows block: 5 pairs -> 512 threads, 98 % of the lanes busy
 constexpr int kRowsG = 5;
 
-int rows1k_tiles() { return (1000 / 2 + kRowsG - 1) / kRowsG; }
+constexpr int kRowsBlocks = (1000 / 2 + kRowsG - 1) / kRowsG;
+constexpr int kRowsWaves = (100 * kRowsG + 63) / 64;
+
+// candidate groups per node: one per wavefront of every rows block
+int rows1k_tiles() { return kRowsBlocks * kRowsWaves; }
 
 hipError_t set_rows1k_lds() { return hipSuccess; }
 
 void launch_rows1k(const ScanGeom& g, const float2* tw1k, const NodeInfo* info, const float2* CT, int nNodes, int K,
                    Cand* cand, float* map, hipStream_t st) {
-  const int blocks = rows1k_tiles();
+  const int blocks = kRowsBlocks, groups = rows1k_tiles();
   dim3 grid((unsigned)blocks, (unsigned)nNodes);
   dim3 block(64 * ((100 * kRowsG + 63) / 64));
   const long long cs = ct1k_stride(g.h0);
   if (K <= 2)
-    hipLaunchKernelGGL((k_rows1k<2, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, blocks, map);
+    hipLaunchKernelGGL((k_rows1k<2, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map);
   else if (K <= 4)
-    hipLaunchKernelGGL((k_rows1k<4, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, blocks, map);
+    hipLaunchKernelGGL((k_rows1k<4, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map);
   else if (K <= 8)
-    hipLaunchKernelGGL((k_rows1k<8, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, blocks, map);
+    hipLaunchKernelGGL((k_rows1k<8, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map);
   else
-    hipLaunchKernelGGL((k_rows1k<16, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, blocks, map);
+    hipLaunchKernelGGL((k_rows1k<16, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map);
 }
 
 }  // namespace qsmi

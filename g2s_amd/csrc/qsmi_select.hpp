@@ -162,4 +162,82 @@ struct RankList {
 };
 
 
+
+// ---------------------------------------------------------------------------
+// The same two passes inside one wavefront (shuffles only, no barriers).
+// ---------------------------------------------------------------------------
+template <int KT>
+__device__ float wave_kth_key(float (&m)[KT], int K) {
+  int need = K;
+  float tau = -INFINITY;
+  for (int it = 0; it < K; ++it) {
+    float M = m[0];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) M = fmaxf(M, __shfl_xor(M, off));
+    tau = M;
+    if (M == -INFINITY) break;
+    int c = 0;
+#pragma unroll
+    for (int i = 0; i < KT; ++i) c += (m[i] == M) ? 1 : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if (c >= need) break;
+    need -= c;
+#pragma unroll
+    for (int r = 0; r < KT; ++r) {
+      if (m[0] == M) {
+#pragma unroll
+        for (int i = 0; i < KT - 1; ++i) m[i] = m[i + 1];
+        m[KT - 1] = -INFINITY;
+      }
+    }
+  }
+  return tau;
+}
+
+// K rounds of wavefront maxima of a RankList -> out[0..K) (lane 0 writes)
+template <int KT>
+__device__ void wave_extract(RankList<KT>& rl, int K, Cand* out) {
+  const int lane = threadIdx.x & 63;
+  for (int it = 0; it < K; ++it) {
+    unsigned long long M = rl.r[0];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(M, off);
+      M = o > M ? o : M;
+    }
+    const bool mine = M != 0ull && rl.r[0] == M;  // at most one lane
+    const unsigned long long ball = __ballot(mine);
+    const int src = ball ? __ffsll((long long)ball) - 1 : 0;
+    const uint32_t pos = __shfl(rl.pos[0], src);
+    const float val = __shfl(rl.val[0], src);
+    if (mine) {
+#pragma unroll
+      for (int i = 0; i < KT - 1; ++i) {
+        rl.r[i] = rl.r[i + 1];
+        rl.pos[i] = rl.pos[i + 1];
+        rl.val[i] = rl.val[i + 1];
+      }
+      rl.r[KT - 1] = 0ull;
+      rl.pos[KT - 1] = 0xFFFFFFFFu;
+      rl.val[KT - 1] = -INFINITY;
+    }
+    if (lane == 0) {
+      Cand w;
+      if (M != 0ull) {
+        w.key = key_of_bits((uint32_t)(M >> 32));
+        w.h = (uint32_t)M;
+        w.pos = pos;
+        w.val = val;
+      } else {
+        w.key = -INFINITY;
+        w.h = 0u;
+        w.pos = 0xFFFFFFFFu;
+        w.val = -INFINITY;
+      }
+      out[it] = w;
+    }
+  }
+}
+
 }  // namespace qsmi
