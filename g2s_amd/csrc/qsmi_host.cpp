@@ -368,12 +368,26 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   if (!rc)
     rc = qsmi::nb_search_device(D, sd, nPath, (const unsigned*)dCells, (const int*)dPostS, (const int*)dOffs, nOff, n,
                                 (int*)dNb, (int*)dPj, (int*)dCnt, dev->stream);
+  // learned proposal: which nodes reach the stencil coverage it needs
+  const float lpMinCover = rq->lpMinCover > 0.f ? rq->lpMinCover : 0.5f;
+  std::vector<unsigned char> lpOk;
+  void* dOk = nullptr;
+  if (!rc && rq->lp) {
+    rc = dev_upload(nullptr, nPath, &dOk);
+    if (!rc)
+      rc = qsmi::lp_cover_all(rq->lp, (int)nPath, n, (const int*)dNb, (const int*)dCnt, (const int*)dOffs, lpMinCover,
+                              (unsigned char*)dOk, dev->stream);
+  }
   std::vector<int> nbPj(nPath * (size_t)n);
   std::vector<int> nbCnt(nPath, 0);
   if (!rc) {
     hipError_t e = hipStreamSynchronize(dev->stream);
     if (e == hipSuccess) e = hipMemcpy(nbCnt.data(), dCnt, nPath * sizeof(int), hipMemcpyDeviceToHost);
     if (e == hipSuccess) e = hipMemcpy(nbPj.data(), dPj, nbPj.size() * sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && dOk) {
+      lpOk.resize(nPath);
+      e = hipMemcpy(lpOk.data(), dOk, nPath, hipMemcpyDeviceToHost);
+    }
     if (e != hipSuccess) rc = hfail(QSMI_ERR_HIP, std::string("neighbour search: ") + hipGetErrorString(e));
   }
   const auto tNb = clk::now();
@@ -463,7 +477,19 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   R.lpEvery = rq->lpFallbackEvery;
   R.lpMse = rq->lpFallbackMse;
   R.lpAudit = rq->lpAudit;
-  R.lpMinCover = rq->lpMinCover > 0.f ? rq->lpMinCover : 0.5f;
+  R.lpMinCover = lpMinCover;
+  if (rq->lp && !lpOk.empty()) {
+    // a wave runs the proposal only if one of its nodes can be proposed
+    // (enough stencil coverage and not an every-m-th exact node)
+    const size_t nW = R.waveStart.size() - 1;
+    R.waveLp.assign(nW, 0);
+    for (size_t w = 0; w < nW; ++w)
+      for (int q = R.waveStart[w]; q < R.waveStart[w + 1] && !R.waveLp[w]; ++q) {
+        const unsigned i = order[q];
+        const bool periodic = R.lpEvery > 0 && (i % (unsigned)R.lpEvery) == 0;
+        if (lpOk[i] && !periodic) R.waveLp[w] = 1;
+      }
+  }
   const auto tUp = clk::now();
   double stageMs[4] = {0, 0, 0, 0};
   long long scanned = 0;
@@ -515,7 +541,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   rs->nodes = (long long)nPath;
   rs->waves = waves;
   rs->lpProposed = lpc[0];
-  rs->lpFallbacks = lpc[1];
+  rs->lpFallbacks = lpc[1] + R.lpSkipped;
   rs->lpOverflows = lpc[2];
   rs->lpShortlistSum = lpc[3];
   rs->lpAuditNodes = lpc[4];

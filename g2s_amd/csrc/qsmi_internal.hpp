@@ -152,7 +152,9 @@ struct ResidentLoop {
   // learned proposal (nullptr: exact scan for every node)
   qsmi_lp* lp = nullptr;
   int lpShortlist = 0, lpEvery = 0, lpAudit = 0;
-  float lpMse = 0.f, lpMinCover = 0.75f;
+  float lpMse = 0.f, lpMinCover = 0.5f;
+  std::vector<unsigned char> waveLp;  // per wave: 1 = run the proposal (empty: every wave)
+  mutable long long lpSkipped = 0;    // out: nodes of waves that skipped the proposal
   // training-pair record (device arrays; recCap 0: off)
   int recCap = 0;
   int* dRecOff = nullptr;
@@ -167,6 +169,10 @@ void lp_propose(qsmi_lp* lp, const qsmi_ti* t, int V, int nN, const NodeInfo* in
                 const float* nbVal, const float* di, int nVar, int sd0, int K, float minCover, hipStream_t st);
 void lp_flags(qsmi_lp* lp, int nN, int w0, const int* order, const NodeInfo* info, int K, int every, float mseSigma2,
               float minCover, int audit, hipStream_t st);
+// ok[i] = 1 for path nodes whose stencil coverage reaches minCover (the only
+// nodes the learned proposal serves); 2-D offsets [nOff][2].
+int lp_cover_all(qsmi_lp* lp, int nPath, int n, const int* dNbIdx, const int* dNbCnt, const int* dOffs, float minCover,
+                 unsigned char* dOk, hipStream_t st);
 void lp_combine(qsmi_lp* lp, int nN, const NodeInfo* info, const Cand* candEx, int blocks, int rowsPerBlock, int K,
                 int audit, hipStream_t st);
 NodeInfo* lp_info_ex(qsmi_lp* lp);

@@ -352,6 +352,27 @@ __global__ void __launch_bounds__(128) k_wave_features(const NodeInfo* __restric
   if (threadIdx.x == 0) cover[node] = ni.nb > 1 ? (float)covered / (float)(ni.nb - 1) : 0.f;
 }
 
+// Stencil coverage of every path node before the loop (one wavefront per
+// node): ok[i] = 1 when at least minCover of the node's neighbours 1..nb-1
+// fall on the key stencil (the rule k_wave_features + k_lp_flags apply), so
+// waves without such a node skip the proposal entirely.
+__global__ void __launch_bounds__(64) k_cover_all(int nPath, int n, const int* __restrict__ nbIdx,
+                                                 const int* __restrict__ nbCnt, const int* __restrict__ offs,
+                                                 const int* __restrict__ slotMap, float minCover,
+                                                 unsigned char* __restrict__ ok) {
+  const int i = blockIdx.x;
+  if (i >= nPath) return;
+  const int nb = nbCnt[i];
+  int c = 0;
+  for (int j = 1 + (int)threadIdx.x; j < nb; j += 64) {
+    const int o = nbIdx[(long long)i * n + j];
+    c += slot_of(slotMap, offs[2 * o], offs[2 * o + 1]) >= 0 ? 1 : 0;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if (threadIdx.x == 0) ok[i] = (nb > 1 && (float)c / (float)(nb - 1) >= minCover) ? 1 : 0;
+}
+
 // Per-node score threshold from the sampled scores (one block per row of
 // n = 8192 floats): every thread keeps the 4 largest of its 32 values, the
 // 1024 survivors are sorted (bitonic, descending) and thr = the r-th
@@ -409,65 +430,123 @@ __global__ void k_to_bf16(const float* __restrict__ src, __bf16* __restrict__ ds
 // Scoring of every TI position for the nodes of a wave with the threshold
 // epilogue: S[m][n] = <Q[m], keys[n]> (bf16 operands, fp32 accumulation);
 // positions with S >= thr[m] are appended to segment (block % NSEG) of
-// list[m] (counter cnt[m][seg], segment capacity cap / NSEG): sixteen
-// counters per node keep the appends from serialising on one address.
+// list[m] (counter cnt[m][seg], segment capacity cap / NSEG).
 // Q (M <= 256 rows) sits in LDS as bf16; each wavefront takes 64 keys at a
 // time, holds their B fragments for all of K = 128 in registers and sweeps
 // the M tiles.
+// Appends are staged in LDS as (row << 24 | position) (positions < 2^24):
+// one LDS atomic per lane and M tile reserves the lane's hits; after every
+// block iteration (four tiles) the block counts its staged hits per row,
+// reserves each row's room in the global list with one atomic per row, and
+// scatters. A full stage spills single hits straight to the global list.
 // ---------------------------------------------------------------------------
+constexpr int kStage = 1024;  // staged hits per block iteration
+
+__device__ __forceinline__ void score_emit(uint32_t e, uint32_t* __restrict__ list, int* __restrict__ cnt, int cap,
+                                           int seg, int segCap) {
+  const int row = (int)(e >> 24);
+  const int slot = atomicAdd(&cnt[row * NSEG + seg], 1);
+  if (slot < segCap) list[(long long)row * cap + seg * segCap + slot] = e & 0xFFFFFFu;
+}
+
 __global__ void __launch_bounds__(256) k_score(int M, long long N, const float* __restrict__ Q,
                                               const __bf16* __restrict__ keys, const float* __restrict__ thr,
                                               uint32_t* __restrict__ list, int* __restrict__ cnt, int cap) {
   constexpr int QLD = EMB + 8;  // bf16 row stride of Q in LDS (16-byte aligned fragments)
   __shared__ __attribute__((aligned(16))) __bf16 qs[256 * QLD];
   __shared__ float th[256];
+  __shared__ int rowCnt[256];
+  __shared__ int rowBase[256];
+  __shared__ uint32_t stg[kStage];
+  __shared__ int stgN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int mt = (M + 31) / 32;
   for (int i = tid; i < mt * 32 * EMB; i += 256) {
     const int m = i / EMB, k = i - m * EMB;
     qs[m * QLD + k] = (__bf16)(m < M ? Q[(long long)m * EMB + k] : 0.f);
   }
-  for (int i = tid; i < mt * 32; i += 256) th[i] = i < M ? thr[i] : INFINITY;
+  for (int i = tid; i < 256; i += 256) {
+    th[i] = i < M ? thr[i] : INFINITY;
+    rowCnt[i] = 0;
+  }
+  if (tid == 0) stgN = 0;
   __syncthreads();
   const long long tiles = (N + 63) / 64;
   const int seg = blockIdx.x % NSEG, segCap = cap / NSEG;
-  for (long long tile = (long long)blockIdx.x * 4 + wave; tile < tiles; tile += (long long)gridDim.x * 4) {
-    const long long n0 = tile * 64;
-    bf16x8 b0[8], b1[8];
-    {
-      const long long ka = n0 + (lane & 31), kb = ka + 32;
-      const bf16x8 z = {};
+  // block-uniform loop: every wavefront runs the same iterations (barriers)
+  for (long long t0 = (long long)blockIdx.x * 4; t0 < tiles; t0 += (long long)gridDim.x * 4) {
+    const long long tile = t0 + wave;
+    if (tile < tiles) {
+      const long long n0 = tile * 64;
+      bf16x8 b0[8], b1[8];
+      {
+        const long long ka = n0 + (lane & 31), kb = ka + 32;
+        const bf16x8 z = {};
 #pragma unroll
-      for (int s = 0; s < 8; ++s) {
-        b0[s] = ka < N ? *reinterpret_cast<const bf16x8*>(keys + ka * EMB + 16 * s + 8 * (lane >> 5)) : z;
-        b1[s] = kb < N ? *reinterpret_cast<const bf16x8*>(keys + kb * EMB + 16 * s + 8 * (lane >> 5)) : z;
-      }
-    }
-    for (int m = 0; m < mt; ++m) {
-      f32x16 c0, c1;
-#pragma unroll
-      for (int r = 0; r < 16; ++r) c0[r] = c1[r] = 0.f;
-#pragma unroll
-      for (int s = 0; s < 8; ++s) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&qs[(m * 32 + (lane & 31)) * QLD + 16 * s + 8 * (lane >> 5)]);
-        c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0[s], c0, 0, 0, 0);
-        c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1[s], c1, 0, 0, 0);
-      }
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int row = m * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const float t = th[row];
-        const long long na = n0 + (lane & 31), nb2 = na + 32;
-        if (c0[r] >= t && na < N) {
-          const int slot = atomicAdd(&cnt[row * NSEG + seg], 1);
-          if (slot < segCap) list[(long long)row * cap + seg * segCap + slot] = (uint32_t)na;
-        }
-        if (c1[r] >= t && nb2 < N) {
-          const int slot = atomicAdd(&cnt[row * NSEG + seg], 1);
-          if (slot < segCap) list[(long long)row * cap + seg * segCap + slot] = (uint32_t)nb2;
+        for (int s = 0; s < 8; ++s) {
+          b0[s] = ka < N ? *reinterpret_cast<const bf16x8*>(keys + ka * EMB + 16 * s + 8 * (lane >> 5)) : z;
+          b1[s] = kb < N ? *reinterpret_cast<const bf16x8*>(keys + kb * EMB + 16 * s + 8 * (lane >> 5)) : z;
         }
       }
+      const long long na = n0 + (lane & 31), nb2 = na + 32;
+      for (int m = 0; m < mt; ++m) {
+        f32x16 c0, c1;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) c0[r] = c1[r] = 0.f;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+          const bf16x8 a = *reinterpret_cast<const bf16x8*>(&qs[(m * 32 + (lane & 31)) * QLD + 16 * s + 8 * (lane >> 5)]);
+          c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0[s], c0, 0, 0, 0);
+          c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1[s], c1, 0, 0, 0);
+        }
+        // hit mask of this lane: bit 2r <- (row(r), na), bit 2r+1 <- (row(r), nb2)
+        uint32_t hits = 0;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const float t = th[m * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)];
+          hits |= (c0[r] >= t && na < N ? 1u : 0u) << (2 * r);
+          hits |= (c1[r] >= t && nb2 < N ? 1u : 0u) << (2 * r + 1);
+        }
+        if (hits) {
+          int slot = atomicAdd(&stgN, __popc(hits));
+          while (hits) {
+            const int bit = __ffs(hits) - 1;
+            hits &= hits - 1;
+            const int r = bit >> 1;
+            const int row = m * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+            const uint32_t e = ((uint32_t)row << 24) | (uint32_t)((bit & 1) ? nb2 : na);
+            if (slot < kStage)
+              stg[slot] = e;
+            else
+              score_emit(e, list, cnt, cap, seg, segCap);
+            ++slot;
+          }
+        }
+      }
     }
+    // flush the stage: per-row counts, one global reservation per row, scatter
+    __syncthreads();
+    const int n = min(stgN, kStage);
+    if (n) {
+      for (int i = tid; i < n; i += 256) atomicAdd(&rowCnt[stg[i] >> 24], 1);
+      __syncthreads();
+      {
+        const int c = rowCnt[tid];
+        rowBase[tid] = c ? atomicAdd(&cnt[tid * NSEG + seg], c) : 0;
+        rowCnt[tid] = 0;
+      }
+      __syncthreads();
+      for (int i = tid; i < n; i += 256) {
+        const uint32_t e = stg[i];
+        const int row = (int)(e >> 24);
+        const int slot = rowBase[row] + atomicAdd(&rowCnt[row], 1);
+        if (slot < segCap) list[(long long)row * cap + seg * segCap + slot] = e & 0xFFFFFFu;
+      }
+      __syncthreads();
+      rowCnt[tid] = 0;
+    }
+    if (tid == 0) stgN = 0;
+    __syncthreads();
   }
 }
 
@@ -572,30 +651,40 @@ __global__ void __launch_bounds__(256) k_rescore(const NodeInfo* __restrict__ in
   const bool usable = ni.nb > 0 && ni.nb <= 128 && V <= 4 && ni.lo[0] <= ni.hi[0] && ni.lo[1] <= ni.hi[1];
   float* eb = ebuf + (long long)node * cap;
   const int cEnd = min(m, c0 + kRescoreChunk);
-  // 16 lanes per position: four positions per wavefront at a time
-  const int g = lane >> 4, gl = lane & 15;
-  for (int c = c0 + wave * 4 + g; c - g < cEnd; c += 16) {
+  // 8 lanes per position, eight positions per wavefront at a time; each lane
+  // issues all of its (<= 16) neighbour loads before using any of them
+  const int g = lane >> 3, gl = lane & 7;
+  const int* segCnt = cnt + node * NSEG;
+  for (int c = c0 + wave * 8 + g; c - g < cEnd; c += 32) {
     const bool inRange = c < cEnd;
-    const bool have = inRange && (c % segCap) < cnt[node * NSEG + c / segCap];
+    const bool have = inRange && (c % segCap) < segCnt[c / segCap];
     const uint32_t p = have ? list[(long long)node * cap + c] : 0u;
     const int p0 = (int)(p % (uint32_t)s0), p1 = (int)(p / (uint32_t)s0);
     const bool valid = have && usable && p0 >= ni.lo[0] && p0 <= ni.hi[0] && p1 >= ni.lo[1] && p1 <= ni.hi[1];
     float e = 0.f;
     if (valid) {
       if (yy) {  // one continuous variable: images (y^2, y), one load per neighbour
-        for (int j = gl; j < nb; j += 16) {
-          const float y = sub[S + (long long)p + doff[j]];
-          e += dval[j * 4] * (y * y) + dval[j * 4 + 1] * y;
+        const float* ys = sub + S + p;
+        float y[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int j = gl + 8 * u;
+          y[u] = j < nb ? ys[doff[j]] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+          const int j = gl + 8 * u;
+          if (j < nb) e += dval[j * 4] * (y[u] * y[u]) + dval[j * 4 + 1] * y[u];
         }
       } else {
-        for (int j = gl; j < nb; j += 16) {
+        for (int j = gl; j < nb; j += 8) {
           const long long t = (long long)p + doff[j];
           for (int a = 0; a < V; ++a) e += dval[j * 4 + a] * sub[(long long)act[a] * S + t];
         }
       }
     }
 #pragma unroll
-    for (int off = 8; off > 0; off >>= 1) e += __shfl_xor(e, off);
+    for (int off = 4; off > 0; off >>= 1) e += __shfl_xor(e, off);
     if (gl == 0 && inRange) eb[c] = valid ? e + ni.delta0 : -INFINITY;
   }
 }
@@ -1332,6 +1421,7 @@ int lp_prepare(qsmi_lp* lp, const qsmi_ti* t, int maxNodes, int K, int shortlist
   if (t->D != 2) return fail(QSMI_ERR_UNSUPPORTED, "learned proposal: 2-D only");
   if (V > 4) return fail(QSMI_ERR_UNSUPPORTED, "learned proposal: at most 4 active sub-variables");
   if (maxNodes > 256) return fail(QSMI_ERR_UNSUPPORTED, "learned proposal: waves of at most 256 nodes");
+  if (t->S > (1LL << 24)) return fail(QSMI_ERR_UNSUPPORTED, "learned proposal: TI of at most 2^24 cells");
   const int L = shortlist > 0 ? shortlist : 4096;
   const int cap = NSEG * ((2 * L + NSEG - 1) / NSEG);  // room for twice the target, in NSEG equal segments
   if (!(lp->pws && lp->pMaxNodes >= maxNodes && lp->pK == K && lp->pShort == L)) {
@@ -1416,6 +1506,16 @@ void lp_flags(qsmi_lp* lp, int nN, int w0, const int* order, const NodeInfo* inf
   hipLaunchKernelGGL(k_lp_flags, dim3((nN + 127) / 128), dim3(128), 0, st, nN, w0, order, info, lp->pCnt, lp->pCap,
                      lp->pBest, lp->pCand, K, every, mseSigma2 > 0.f ? mseSigma2 * lp->sigma * lp->sigma : 0.f,
                      lp->pCover, minCover, audit, lp->pFlag, lp->pInfoEx, lp->pCounters);
+}
+
+int lp_cover_all(qsmi_lp* lp, int nPath, int n, const int* dNbIdx, const int* dNbCnt, const int* dOffs, float minCover,
+                 unsigned char* dOk, hipStream_t st) {
+  using namespace lp;
+  if (!lp->dSlot) return fail(QSMI_ERR_ARG, "learned proposal: keys were not built");
+  if (nPath <= 0) return QSMI_OK;
+  hipLaunchKernelGGL(k_cover_all, dim3(nPath), dim3(64), 0, st, nPath, n, dNbIdx, dNbCnt, dOffs, lp->dSlot,
+                     minCover > 0.f ? minCover : 0.5f, dOk);
+  return QSMI_OK;
 }
 
 void lp_combine(qsmi_lp* lp, int nN, const NodeInfo* info, const Cand* candEx, int blocks, int rowsPerBlock, int K,

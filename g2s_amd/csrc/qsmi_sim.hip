@@ -346,7 +346,9 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
     hipLaunchKernelGGL(k_gather, dim3(nN), dim3(64), gatherLds, st, P, w0, R.dOrder, R.dPathCells, R.dPathSeeds, R.dNbIdx,
                        R.dNbCnt, R.dOffs, R.dKernel, R.dCatvals, R.dDi, R.dIndex, R.dTi, b->dInfo, b->dOff, b->dVal);
     if (ev) (void)hipEventRecord(ev[1], st);
-    if (R.lp) {
+    const bool useLp = R.lp && (R.waveLp.empty() || R.waveLp[w]);
+    if (R.lp && !useLp) R.lpSkipped += nN;
+    if (useLp) {
       // learned proposal for every node, the exact scan for the routed ones
       lp_propose(R.lp, t, P.V, nN, b->dInfo, b->dOff, b->dVal, R.dDi, R.nVar, R.sd[0], K, R.lpMinCover, st);
       lp_flags(R.lp, nN, w0, R.dOrder, b->dInfo, K, R.lpEvery, R.lpMse, R.lpMinCover, R.lpAudit, st);

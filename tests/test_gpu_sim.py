@@ -31,10 +31,18 @@ def default_kernel(ti_dims, nexp):
     return kd, np.ones(int(np.prod(kd)) * nexp, np.float32)
 
 
-def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatch_frac=0.01):
+def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatch_frac=0.01, nodes=None,
+             weights=None, wave_size=64, threads=8):
+    """nodes: path-relative node indices to check (None: every node);
+    weights: one kernel weight per variable (None: the default uniform 1)."""
     types = list(types)
     nvar = len(types)
-    sim, idx, st = g2s_amd.qs(ti, ti_dims, di0, sim_dims, types, n, k, seed, wave_size=64, tie_quantum=quantum)
+    gkw = {}
+    if weights is not None:
+        kd0 = [d // 2 + 1 for d in ti_dims]
+        gkw = dict(kernel=np.tile(np.asarray(weights, np.float32), int(np.prod(kd0))), kernel_dims=kd0)
+    sim, idx, st = g2s_amd.qs(ti, ti_dims, di0, sim_dims, types, n, k, seed, wave_size=wave_size,
+                              tie_quantum=quantum, **gkw)
     _log(f"  gpu run: {st}")
     # oracle set-up identical to the reference's qs.cpp
     ncat, catvals = [], []
@@ -45,6 +53,13 @@ def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatc
             catvals.extend(c.tolist())
     nexp = sum(1 if t == 0 else ncat[list(np.nonzero(types)[0]).index(i)] for i, t in enumerate(types))
     kd, kern = default_kernel(ti_dims, nexp)
+    if weights is not None:
+        # expanded channels: a categorical variable's classes share its weight
+        ch = []
+        for v, t in enumerate(types):
+            nc = 1 if t == 0 else ncat[list(np.nonzero(types)[0]).index(v)]
+            ch.extend([float(weights[v])] * nc)
+        kern = np.tile(np.asarray(ch, np.float32), int(np.prod(kd)))
     offs = oracle.offsets(kd)
     di0f = di0.reshape(-1, nvar)
     informed = (~np.isnan(di0f).any(axis=1)).astype(np.uint8)
@@ -62,9 +77,15 @@ def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatc
     tif = ti.reshape(-1, nvar)
     mismatches = 0
     worst_gap = 0.0
-    for i in range(len(p) - b):
+    check = list(range(len(p) - b)) if nodes is None else [int(i) for i in nodes]
+    if threads > 1:
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            results = list(ex.map(lambda i: orc.node(simf, i), check))
+    else:
+        results = [orc.node(simf, i) for i in check]
+    for i, (chosen, v, pos, nb) in zip(check, results):
         cell = p[b + i]
-        chosen, v, pos, nb = orc.node(simf, i)
         if chosen == idx[cell]:
             continue
         mismatches += 1
@@ -80,8 +101,9 @@ def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatc
         worst_gap = max(worst_gap, gap / bound)
         # the value written is the TI value at the GPU's choice
         assert np.array_equal(simf[cell], tif[idx[cell]])
-    frac = mismatches / max(1, len(p) - b)
-    _log(f"  lockstep mismatches {mismatches} of {len(p) - b} nodes (largest gap / tolerance {worst_gap:.3f})")
+    frac = mismatches / max(1, len(check))
+    _log(f"  lockstep mismatches {mismatches} of {len(check)} checked nodes (of {len(p) - b}; largest gap / "
+         f"tolerance {worst_gap:.3f})")
     assert frac <= max_mismatch_frac
     return sim, idx, st
 
