@@ -12,7 +12,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqsmi.so")
 
-SOURCES = ["qsmi_kernels.hip", "qsmi_fast1000.hip", "qsmi_sim.hip", "qsmi_api.hip", "qsmi_lp.hip", "qsmi_host.cpp"]
+SOURCES = ["qsmi_kernels.hip", "qsmi_fast1000.hip", "qsmi_sim.hip", "qsmi_api.hip", "qsmi_lp.hip", "qsmi_host.cpp",
+           "qsmi_bgrid.cpp"]
 HEADERS = ["qsmi_fft.hpp", "qsmi_kernels.hpp", "qsmi_internal.hpp"]
 
 FLAGS = [
@@ -27,6 +28,7 @@ FLAGS = [
     "-Wno-unused-value",
     "-I" + os.path.join(ROOT, "include"),
     "-lpthread",
+    "-lz",
 ]
 
 

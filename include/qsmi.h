@@ -318,6 +318,44 @@ int qsmi_lp_train(qsmi_lp* lp, const qsmi_lp_train_set* set, const qsmi_lp_train
  * sample of 8192 of them for the per-node thresholds. */
 int qsmi_lp_build_keys(qsmi_lp* lp, const unsigned* tiSize, const float* ti, const int* canon);
 
+/* ======================================================================
+ * .bgrid files: G2S's serialized DataImage, the format its qs reads
+ * training images and destination grids from and writes results to.
+ *   size_t fullSize | unsigned nbDim | unsigned dims[nbDim] | unsigned
+ *   nbVariable | unsigned types[nbVariable] (0 continuous, 1 categorical) |
+ *   unsigned encoding (0 float, 1 integer, 2 unsigned) | float data[...]
+ * (data in DataImage order: variable fastest, then dimension 0).
+ * ====================================================================== */
+
+/* Serialized size of a grid. Replaces the size computed in
+ * DataImage::serialize (include/DataImage.hpp:145). */
+int qsmi_bgrid_bytes(int ndim, const unsigned* dims, int nVar, size_t* bytes);
+
+/* DataImage::serialize (include/DataImage.hpp:143-170) into a caller buffer
+ * of at least qsmi_bgrid_bytes bytes. */
+int qsmi_bgrid_serialize(int ndim, const unsigned* dims, int nVar, const int* varTypes, int encoding,
+                         const float* data, void* out, size_t outBytes);
+
+/* 1 when raw[0, bytes) is a well-formed serialized grid, else 0:
+ * DataImage::validateSerializedData (include/DataImage.hpp:172-224). */
+int qsmi_bgrid_valid(const void* raw, size_t bytes);
+
+/* Header fields of a valid buffer (dims: up to 16 entries; varTypes:
+ * typesCap entries) and the byte offset of its float data; the
+ * DataImage(char*, size_t) constructor (include/DataImage.hpp:226-254). */
+int qsmi_bgrid_header(const void* raw, size_t bytes, int* ndim, unsigned* dims, int* nVar, int* varTypes,
+                      int typesCap, int* encoding, size_t* dataOffset);
+
+/* Write a valid buffer to path, gzip-compressed when gzip != 0 (the file
+ * forms of writeRawData, src/DataImage.cpp:126-163; the content-addressed
+ * naming under /tmp/G2S/data is the server's and not reproduced). */
+int qsmi_bgrid_write_file(const char* path, const void* raw, size_t bytes, int gzip);
+
+/* Read a plain or gzip .bgrid file (loadRawData's readers,
+ * src/DataImage.cpp:35-124) and validate it. out == NULL: only *bytes is
+ * set (the file is read twice by a size query followed by a read). */
+int qsmi_bgrid_read_file(const char* path, void* out, size_t outBytes, size_t* bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -128,6 +128,8 @@ def ref():
     _sig(R, "ref_kernel_expand", C.c_int, [C.c_int, u32p, C.c_int, f32p, i32p, f32p])
     _sig(R, "ref_index_with_delta", C.c_int, [C.c_int, u32p, C.c_uint, i32p, u32p])
     _sig(R, "ref_complex_add", None, [f32p, f32p, f32p, C.c_float, C.c_uint])
+    _sig(R, "ref_bgrid_serialize", C.c_size_t, [C.c_int, u32p, C.c_int, i32p, C.c_int, f32p, C.c_void_p, C.c_size_t])
+    _sig(R, "ref_bgrid_valid", C.c_int, [C.c_void_p, C.c_size_t])
     _REF = R
     return R
 

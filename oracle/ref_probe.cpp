@@ -11,8 +11,11 @@
 //   DataImage::offsetKernel4categories include/DataImage.hpp:482-508
 //   DataImage::indexWithDelta         include/DataImage.hpp:343-376
 //   g2s::complexAddAlphaxCxD          include/complexMulti.hpp:58-67,303-331
+//   DataImage::serialize              include/DataImage.hpp:143-170
+//   DataImage::validateSerializedData include/DataImage.hpp:172-224
 // The glue below only marshals plain arrays in and out.
 #include <climits>
+#include <cstring>
 #include <cmath>
 #include <functional>
 #include <random>
@@ -146,6 +149,28 @@ int ref_index_with_delta(int D, const unsigned* dims, unsigned position, const i
 
 void ref_complex_add(float* dst, const float* C, const float* Dm, float alpha, unsigned size) {
   g2s::complexAddAlphaxCxD(dst, C, Dm, alpha, size);
+}
+
+// serialized bytes of a grid (returns the full size; copies at most cap bytes)
+size_t ref_bgrid_serialize(int nd, const unsigned* dims, int nVar, const int* types, int enc, const float* data,
+                           char* out, size_t cap) {
+  std::vector<unsigned> d(dims, dims + nd);
+  g2s::DataImage im((unsigned)nd, d.data(), (unsigned)nVar);
+  for (int v = 0; v < nVar; ++v) im._types[v] = (g2s::DataImage::VariableType)types[v];
+  im.setEncoding((g2s::DataImage::EncodingType)enc);
+  size_t n = nVar;
+  for (int i = 0; i < nd; ++i) n *= dims[i];
+  memcpy(im._data, data, n * sizeof(float));
+  char* raw = im.serialize();
+  size_t full = 0;
+  memcpy(&full, raw, sizeof(size_t));
+  memcpy(out, raw, full < cap ? full : cap);
+  free(raw);
+  return full;
+}
+
+int ref_bgrid_valid(const char* raw, size_t size) {
+  return g2s::DataImage::validateSerializedData(raw, size) ? 1 : 0;
 }
 
 }  // extern "C"

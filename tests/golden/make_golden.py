@@ -12,6 +12,14 @@ oracle/Makefile from /root/reference). Inputs and outputs only (data):
   iwd_*            DataImage::indexWithDelta
   cadd_*           g2s::complexAddAlphaxCxD
 
+and tests/golden/bgrid_vectors.npz:
+
+  grid_*           DataImage::serialize bytes of small grids (1-3 D, one or
+                   several variables, every variable type and encoding)
+  mut_*            buffers derived from them with one header field, the
+                   size field or the length changed, and
+                   DataImage::validateSerializedData's verdict on each
+
 Run: python tests/golden/make_golden.py  (needs oracle/_ref built).
 """
 import os
@@ -26,6 +34,7 @@ sys.path.insert(0, ROOT)
 import oracle  # noqa: E402
 
 OUT = os.path.join(HERE, "ref_vectors.npz")
+OUT_BGRID = os.path.join(HERE, "bgrid_vectors.npz")
 
 
 def ti_cases():
@@ -158,5 +167,81 @@ def main():
     print(f"wrote {OUT} ({os.path.getsize(OUT)} bytes)")
 
 
+def bgrid_cases():
+    """(dims, types, encoding, data) of the serialized-grid vectors."""
+    rng = np.random.default_rng(77)
+    cases = [((5,), [0], 0), ((4, 3), [1], 0), ((3, 2, 2), [0, 1], 1), ((2, 3), [1, 0, 1], 2), ((1, 1, 1, 2), [0], 0)]
+    out = []
+    for dims, types, enc in cases:
+        n = int(np.prod(dims)) * len(types)
+        data = rng.standard_normal(n).astype(np.float32)
+        data[::3] = np.round(data[::3] * 3)
+        if n > 4:
+            data[4] = np.nan
+        out.append((dims, types, enc, data))
+    return out
+
+
+def bgrid_mutations(raw, nd, nv):
+    """(name, buffer) variants of one serialized grid: every header field,
+    the size field and the length changed one at a time."""
+    muts = []
+    b = bytearray(raw)
+
+    def put(buf, off, v):
+        buf = bytearray(buf)
+        buf[off:off + 4] = np.uint32(v).tobytes()
+        return bytes(buf)
+
+    sz = 8
+    muts.append(("ok", bytes(b)))
+    muts.append(("short", bytes(b[:-4])))
+    muts.append(("long", bytes(b) + b"\0\0\0\0"))
+    muts.append(("size_field", bytes(np.uint64(len(b) + 4).tobytes()) + bytes(b[8:])))
+    muts.append(("ndim0", put(b, sz, 0)))
+    muts.append(("ndim17", put(b, sz, 17)))
+    muts.append(("ndim_plus1", put(b, sz, nd + 1)))
+    muts.append(("dim0", put(b, sz + 4, 0)))
+    muts.append(("dim_plus1", put(b, sz + 4, int(np.frombuffer(b[sz + 4:sz + 8], np.uint32)[0]) + 1)))
+    vo = sz + 4 + 4 * nd
+    muts.append(("nvar0", put(b, vo, 0)))
+    muts.append(("nvar4097", put(b, vo, 4097)))
+    muts.append(("type2", put(b, vo + 4, 2)))
+    muts.append(("type1", put(b, vo + 4, 1)))
+    eo = vo + 4 + 4 * nv
+    muts.append(("enc3", put(b, eo, 3)))
+    muts.append(("enc2", put(b, eo, 2)))
+    muts.append(("tiny", bytes(b[:12])))
+    return muts
+
+
+def write_bgrid():
+    R = oracle.ref()
+    if R is None:
+        raise SystemExit("oracle/_ref is not built: run make -C oracle with the reference tree present")
+    out = {}
+    for i, (dims, types, enc, data) in enumerate(bgrid_cases()):
+        cap = 1 << 16
+        buf = np.zeros(cap, np.uint8)
+        n = R.ref_bgrid_serialize(len(dims), oracle.u32(dims), len(types), oracle.i32(types), enc, data,
+                                  buf.ctypes.data, cap)
+        raw = buf[:n].tobytes()
+        out[f"grid_{i}_dims"] = np.array(dims, np.uint32)
+        out[f"grid_{i}_types"] = np.array(types, np.int32)
+        out[f"grid_{i}_enc"] = np.int32(enc)
+        out[f"grid_{i}_data"] = data
+        out[f"grid_{i}_raw"] = np.frombuffer(raw, np.uint8)
+        for name, m in bgrid_mutations(raw, len(dims), len(types)):
+            mb = np.frombuffer(m, np.uint8).copy()
+            out[f"mut_{i}_{name}_raw"] = mb
+            out[f"mut_{i}_{name}_valid"] = np.int32(R.ref_bgrid_valid(mb.ctypes.data if mb.size else None, mb.size))
+    np.savez_compressed(OUT_BGRID, **out)
+    print(f"wrote {OUT_BGRID} ({os.path.getsize(OUT_BGRID)} bytes)")
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "bgrid":
+        write_bgrid()
+    else:
+        main()
+        write_bgrid()
