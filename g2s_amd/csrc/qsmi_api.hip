@@ -35,11 +35,11 @@ long long ct1k_stride(int h0);
 void launch_transpose_spectra(const float2* Y, float2* YT, int h0, int f1, int nA, hipStream_t st);
 void launch_pencil1k(const ScanGeom& g, const float2* YT, const float2* tw1k, int nNodes, int npb,
                      const NodeInfo* info, const int* nbOff, const float* nbVal, TapInfo* taps, uint8_t* tapOf,
-                     float2* CT, hipStream_t st);
+                     float2* CT, const int* count, hipStream_t st);
 int rows1k_tiles();
 hipError_t set_rows1k_lds();
 void launch_rows1k(const ScanGeom& g, const float2* tw1k, const NodeInfo* info, const float2* CT, int nNodes, int K,
-                   Cand* cand, float* map, hipStream_t st);
+                   Cand* cand, float* map, const int* count, hipStream_t st);
 }  // namespace qsmi
 
 using namespace qsmi;
@@ -238,7 +238,7 @@ int ensure_tap_tables(qsmi_device* dev, int nNodes) {
 }
 
 int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeInfo* info, const int* off,
-                    const float* val, float2* C) {
+                    const float* val, float2* C, const int* count) {
   qsmi_ti* t = dev->ti;
   hipStream_t st = dev->stream;
   if (dev->timing) hipEventRecord(dev->ev[0], st);
@@ -246,7 +246,9 @@ int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeI
     int rc = ensure_tap_tables(dev, nNodes);
     if (rc) return rc;
     const int npb = std::max(1, std::min(8, nNodes / 8));
-    launch_pencil1k(g, t->dYT, t->dTw[0], nNodes, npb, info, off, val, dev->dTap, dev->dTapOf, C, st);
+    launch_pencil1k(g, t->dYT, t->dTw[0], nNodes, npb, info, off, val, dev->dTap, dev->dTapOf, C, count, st);
+  } else if (count) {
+    return fail(QSMI_ERR_UNSUPPORTED, "a device node count is supported on the 1000x1000 path only");
   } else if (t->D == 1) {
     launch_direct1d(g, t->dYhat, t->dTw[0], nNodes, info, off, val, C, st);
   } else {
@@ -265,10 +267,10 @@ int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeI
 }
 
 void launch_rows(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const float2* C, int nNodes, int rowBlocks,
-                 int K, Cand* cand, float* map) {
+                 int K, Cand* cand, float* map, const int* count) {
   qsmi_ti* t = dev->ti;
   if (t->fast1k)
-    launch_rows1k(g, t->dTw[0], info, C, nNodes, K, cand, map, dev->stream);
+    launch_rows1k(g, t->dTw[0], info, C, nNodes, K, cand, map, count, dev->stream);
   else
     launch_rows_select(g, t->plan[0], t->lognv[0], t->dTw[0], info, C, nNodes, rowBlocks, K, cand, map, dev->stream);
 }

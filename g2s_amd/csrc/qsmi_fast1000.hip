@@ -98,10 +98,12 @@ __device__ __forceinline__ void fft1000(float2* buf, float2 (&u)[10], const floa
 // tap index of every row that holds a neighbour.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(64) k_taps1k(const NodeInfo* __restrict__ info, const int* __restrict__ nbOff,
-                                              TapInfo* __restrict__ taps, uint8_t* __restrict__ tapOf) {
+                                              TapInfo* __restrict__ taps, uint8_t* __restrict__ tapOf,
+                                              const int* __restrict__ nodeCount) {
   using namespace f1k;
   __shared__ int flag[L];
   const int node = blockIdx.x, lane = threadIdx.x;
+  if (nodeCount && node >= *nodeCount) return;
   const NodeInfo ni = info[node];
   for (int i = lane; i < L; i += 64) flag[i] = 0;
   __syncthreads();
@@ -154,7 +156,7 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) k
                                                  const NodeInfo* __restrict__ info, const int* __restrict__ nbOff,
                                                  const float* __restrict__ nbVal, const TapInfo* __restrict__ taps,
                                                  const uint8_t* __restrict__ tapOf, float2* __restrict__ CT,
-                                                 long long ctStride) {
+                                                 long long ctStride, const int* __restrict__ count, int chunks) {
   using namespace f1k;
   __shared__ float2 sb[L];            // scatter / exchange buffer
   __shared__ float2 twl[L];           // exp(-2 pi i t / 1000)
@@ -169,8 +171,16 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) k
   const int k0 = xcd * slice + (r % slice);
   const int chunk = r / slice;
   if (k0 >= h0) return;
-  const int nodeBeg = chunk * npb;
-  const int nodeEnd = min(nNodes, nodeBeg + npb);
+  // with a device count only nodes [0, count) are scanned, spread over all
+  // node chunks of the grid
+  int nAct = nNodes, per = npb;
+  if (count) {
+    nAct = min(*count, nNodes);
+    per = max(1, (nAct + chunks - 1) / chunks);
+  }
+  const int nodeBeg = chunk * per;
+  const int nodeEnd = min(nAct, nodeBeg + per);
+  if (nodeBeg >= nodeEnd) return;
   const int V = g.V;
   const bool act = lane < 100;
   for (int i = lane; i < L; i += 128) twl[i] = tw1k[i];  // first read after a barrier below
@@ -327,7 +337,8 @@ __global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k(ScanGeom 
                                                                      const float2* __restrict__ CT,
                                                                      long long ctStride, int K,
                                                                      Cand* __restrict__ cand, int candBlocks,
-                                                                     float* __restrict__ map) {
+                                                                     float* __restrict__ map,
+                                                                     const int* __restrict__ count) {
   using namespace f1k;
   __shared__ float2 buf[G][L];
   __shared__ float2 twl[L];
@@ -335,6 +346,7 @@ __global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k(ScanGeom 
   const int grp = tid / 100;
   const int j = tid - 100 * grp;
   const int node = blockIdx.y;
+  if (count && node >= *count) return;  // beyond the compacted nodes
   const NodeInfo ni = info[node];
   const int h0 = g.h0;
   // one candidate group of K per wavefront of the block
@@ -483,13 +495,13 @@ void launch_transpose_spectra(const float2* Y, float2* YT, int h0, int f1, int n
 
 void launch_pencil1k(const ScanGeom& g, const float2* YT, const float2* tw1k, int nNodes, int npb,
                      const NodeInfo* info, const int* nbOff, const float* nbVal, TapInfo* taps, uint8_t* tapOf,
-                     float2* CT, hipStream_t st) {
-  hipLaunchKernelGGL(k_taps1k, dim3((unsigned)nNodes), dim3(64), 0, st, info, nbOff, taps, tapOf);
+                     float2* CT, const int* count, hipStream_t st) {
+  hipLaunchKernelGGL(k_taps1k, dim3((unsigned)nNodes), dim3(64), 0, st, info, nbOff, taps, tapOf, count);
   const int slice = (g.h0 + f1k::NXCD - 1) / f1k::NXCD;
   const int chunks = (nNodes + npb - 1) / npb;
   const long long blocks = (long long)f1k::NXCD * slice * chunks;
   hipLaunchKernelGGL(k_pencil1k, dim3((unsigned)blocks), dim3(128), 0, st, g, YT, tw1k, nNodes, npb, info, nbOff, nbVal,
-                     taps, tapOf, CT, ct1k_stride(g.h0));
+                     taps, tapOf, CT, ct1k_stride(g.h0), count, chunks);
 }
 
 // row pairs per rows block: 5 pairs -> 512 threads, 98 % of the lanes busy
@@ -504,19 +516,23 @@ int rows1k_tiles() { return kRowsBlocks * kRowsWaves; }
 hipError_t set_rows1k_lds() { return hipSuccess; }
 
 void launch_rows1k(const ScanGeom& g, const float2* tw1k, const NodeInfo* info, const float2* CT, int nNodes, int K,
-                   Cand* cand, float* map, hipStream_t st) {
+                   Cand* cand, float* map, const int* count, hipStream_t st) {
   const int blocks = kRowsBlocks, groups = rows1k_tiles();
   dim3 grid((unsigned)blocks, (unsigned)nNodes);
   dim3 block(64 * ((100 * kRowsG + 63) / 64));
   const long long cs = ct1k_stride(g.h0);
   if (K <= 2)
-    hipLaunchKernelGGL((k_rows1k<2, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map);
+    hipLaunchKernelGGL((k_rows1k<2, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map,
+                       count);
   else if (K <= 4)
-    hipLaunchKernelGGL((k_rows1k<4, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map);
+    hipLaunchKernelGGL((k_rows1k<4, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map,
+                       count);
   else if (K <= 8)
-    hipLaunchKernelGGL((k_rows1k<8, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map);
+    hipLaunchKernelGGL((k_rows1k<8, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map,
+                       count);
   else
-    hipLaunchKernelGGL((k_rows1k<16, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map);
+    hipLaunchKernelGGL((k_rows1k<16, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map,
+                       count);
 }
 
 }  // namespace qsmi

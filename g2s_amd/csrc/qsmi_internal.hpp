@@ -114,12 +114,14 @@ int grow_wave_bufs(qsmi_device* dev, WaveBufs* b, int nNodes, int total, int K);
 void free_wave_bufs(WaveBufs* b);
 // spectrum stage (+ dim1 inverse for 3-D) and rows stage of nNodes nodes
 // (returns a qsmi status: tap tables may be allocated)
+// count (device, optional; 1000x1000 path only): nodes [0, *count) of the
+// nNodes are scanned, the rest skipped at no cost
 int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeInfo* info, const int* off,
-                    const float* val, float2* C);
+                    const float* val, float2* C, const int* count = nullptr);
 // tap tables of the 1000^2 spectrum stage for nNodes nodes per launch
 int ensure_tap_tables(qsmi_device* dev, int nNodes);
 void launch_rows(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const float2* C, int nNodes,
-                 int rowBlocks, int K, Cand* cand, float* map);
+                 int rowBlocks, int K, Cand* cand, float* map, const int* count = nullptr);
 // merge + draw; when simVals != nullptr also writes the drawn TI values and
 // the index image of each node's cell (NodeInfo::cell)
 void launch_merge_draw(const ScanGeom& g, const NodeInfo* info, const Cand* cand, int nNodes, int candBlocks,
@@ -173,9 +175,12 @@ void lp_flags(qsmi_lp* lp, int nN, int w0, const int* order, const NodeInfo* inf
 // nodes the learned proposal serves); 2-D offsets [nOff][2].
 int lp_cover_all(qsmi_lp* lp, int nPath, int n, const int* dNbIdx, const int* dNbCnt, const int* dOffs, float minCover,
                  unsigned char* dOk, hipStream_t st);
-void lp_combine(qsmi_lp* lp, int nN, const NodeInfo* info, const Cand* candEx, int blocks, int rowsPerBlock, int K,
-                int audit, hipStream_t st);
+// compacted: candEx holds the scanned nodes in lp_info_compact order
+void lp_combine(qsmi_lp* lp, int nN, const NodeInfo* info, const Cand* candEx, bool compacted, int blocks,
+                int rowsPerBlock, int K, int audit, hipStream_t st);
 NodeInfo* lp_info_ex(qsmi_lp* lp);
+NodeInfo* lp_info_compact(qsmi_lp* lp);
+const int* lp_compact_count(qsmi_lp* lp);
 Cand* lp_cand(qsmi_lp* lp);
 // proposed, fallbacks, overflows, shortlist sum, audit nodes, audit hits, audit best hits, audit positions
 int lp_read_counters(qsmi_lp* lp, long long out[8]);

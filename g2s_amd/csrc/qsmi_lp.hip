@@ -329,9 +329,11 @@ __global__ void __launch_bounds__(128) k_pair_features(const int* __restrict__ i
 __global__ void __launch_bounds__(128) k_wave_features(const NodeInfo* __restrict__ info, const int* __restrict__ nbOff,
                                                       const float* __restrict__ di, int nVar, int sd0,
                                                       const int* __restrict__ slotMap, float mu, float invSigma,
-                                                      float* __restrict__ out, float* __restrict__ cover) {
+                                                      float* __restrict__ out, float* __restrict__ cover,
+                                                      int* __restrict__ exCnt) {
   __shared__ int covered;
   const int node = blockIdx.x;
+  if (node == 0 && threadIdx.x == 0) *exCnt = 0;  // k_lp_flags of this wave compacts into it
   const NodeInfo ni = info[node];
   float* o = out + (long long)node * IN;
   for (int i = threadIdx.x; i < IN; i += blockDim.x) o[i] = 0.f;
@@ -374,50 +376,77 @@ __global__ void __launch_bounds__(64) k_cover_all(int nPath, int n, const int* _
 }
 
 // Per-node score threshold from the sampled scores (one block per row of
-// n = 8192 floats): every thread keeps the 4 largest of its 32 values, the
-// 1024 survivors are sorted (bitonic, descending) and thr = the r-th
-// largest survivor (r <= 1024). When a thread held more than 4 values above
-// it, thr is a lower bound of the r-th largest, which only lengthens the
+// n = 8192 floats): every thread keeps the 4 largest of its 32 values and
+// thr = the r-th largest of the 1024 survivors (r <= 1024), found by a radix
+// select over their order-preserving bit patterns (four 8-bit digits, one
+// LDS histogram per digit). When a thread held more than 4 values above it,
+// thr is a lower bound of the r-th largest, which only lengthens the
 // shortlist.
 __global__ void __launch_bounds__(256) k_row_kth(const float* __restrict__ Gm, int n, int r,
                                                 const float* __restrict__ cover, float minCover,
                                                 float* __restrict__ thr) {
-  __shared__ float v[1024];
+  __shared__ int hist[256];
+  __shared__ uint32_t sel[2];  // [0] prefix of the selected digits, [1] rank left
   if (!(cover[blockIdx.x] >= minCover)) {  // routed to the exact scan: propose nothing
     if (threadIdx.x == 0) thr[blockIdx.x] = INFINITY;
     return;
   }
+  const int tid = threadIdx.x, lane = tid & 63;
   const float* row = Gm + (long long)blockIdx.x * n;
   float m0 = -INFINITY, m1 = -INFINITY, m2 = -INFINITY, m3 = -INFINITY;
-  for (int i = threadIdx.x; i < n; i += 256) {
+  for (int i = tid; i < n; i += 256) {
     float x = row[i];
     float h = fmaxf(m0, x); x = fminf(m0, x); m0 = h;
     h = fmaxf(m1, x); x = fminf(m1, x); m1 = h;
     h = fmaxf(m2, x); x = fminf(m2, x); m2 = h;
     m3 = fmaxf(m3, x);
   }
-  v[threadIdx.x] = m0;
-  v[256 + threadIdx.x] = m1;
-  v[512 + threadIdx.x] = m2;
-  v[768 + threadIdx.x] = m3;
-  __syncthreads();
-  for (int k = 2; k <= 1024; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < 1024; i += 256) {
-        const int l = i ^ j;
-        if (l > i) {
-          const bool desc = (i & k) == 0;
-          const float a = v[i], b = v[l];
-          if (desc ? (a < b) : (a > b)) {
-            v[i] = b;
-            v[l] = a;
-          }
-        }
-      }
-      __syncthreads();
-    }
+  const uint32_t k4[4] = {key_bits(m0), key_bits(m1), key_bits(m2), key_bits(m3)};
+  if (tid == 0) {
+    sel[0] = 0u;
+    sel[1] = (uint32_t)min(max(r, 1), 1024);
   }
-  if (threadIdx.x == 0) thr[blockIdx.x] = v[min(max(r, 1), 1024) - 1];
+  for (int d = 3; d >= 0; --d) {
+    const int sh = 8 * d;
+    const uint32_t hiMask = d == 3 ? 0u : (0xFFFFFFFFu << (sh + 8));
+    hist[tid] = 0;
+    __syncthreads();
+    const uint32_t pre = sel[0];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      if ((k4[q] & hiMask) == pre) atomicAdd(&hist[(k4[q] >> sh) & 255], 1);
+    __syncthreads();
+    if (tid < 64) {
+      // digits from the top: lane l holds digits 255-4l .. 252-4l
+      const int c0 = hist[255 - 4 * lane], c1 = hist[254 - 4 * lane], c2 = hist[253 - 4 * lane],
+                c3 = hist[252 - 4 * lane];
+      const int sum = c0 + c1 + c2 + c3;
+      int incl = sum;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off);
+        if (lane >= off) incl += v;
+      }
+      const int rem = (int)sel[1];
+      const int excl = incl - sum;
+      const bool mine = excl < rem && incl >= rem;
+      if (mine) {  // exactly one lane
+        int acc = excl, digit = 255 - 4 * lane;
+        const int cs[4] = {c0, c1, c2, c3};
+        for (int q = 0; q < 4; ++q) {
+          if (acc + cs[q] >= rem) {
+            digit = 255 - 4 * lane - q;
+            break;
+          }
+          acc += cs[q];
+        }
+        sel[0] = pre | ((uint32_t)digit << sh);
+        sel[1] = (uint32_t)(rem - acc);
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) thr[blockIdx.x] = key_of_bits(sel[0]);
 }
 
 // fp32 -> bf16 copy (the keys of the scoring product)
@@ -434,13 +463,16 @@ __global__ void k_to_bf16(const float* __restrict__ src, __bf16* __restrict__ ds
 // Q (M <= 256 rows) sits in LDS as bf16; each wavefront takes 64 keys at a
 // time, holds their B fragments for all of K = 128 in registers and sweeps
 // the M tiles.
-// Appends are staged in LDS as (row << 24 | position) (positions < 2^24):
-// one LDS atomic per lane and M tile reserves the lane's hits; after every
-// block iteration (four tiles) the block counts its staged hits per row,
-// reserves each row's room in the global list with one atomic per row, and
-// scatters. A full stage spills single hits straight to the global list.
+// Appends are staged in LDS as (row << 24 | position) (positions < 2^24),
+// each wavefront in its own region of the stage with a wave-uniform fill
+// count: a ballot per accumulator register finds the hits, the rare
+// registers with a hit write their positions at the count plus the lane's
+// rank in the ballot. After a block iteration (four tiles) in which a region
+// passed half full, and after the last one, the block counts its staged hits
+// per row, reserves each row's room in the global list with one atomic per
+// row, and scatters. A full region spills single hits straight to the list.
 // ---------------------------------------------------------------------------
-constexpr int kStage = 1024;  // staged hits per block iteration
+constexpr int kStageW = 256;  // staged hits per wavefront region
 
 __device__ __forceinline__ void score_emit(uint32_t e, uint32_t* __restrict__ list, int* __restrict__ cnt, int cap,
                                            int seg, int segCap) {
@@ -454,11 +486,11 @@ __global__ void __launch_bounds__(256) k_score(int M, long long N, const float* 
                                               uint32_t* __restrict__ list, int* __restrict__ cnt, int cap) {
   constexpr int QLD = EMB + 8;  // bf16 row stride of Q in LDS (16-byte aligned fragments)
   __shared__ __attribute__((aligned(16))) __bf16 qs[256 * QLD];
-  __shared__ float th[256];
+  __shared__ __attribute__((aligned(16))) float th[256];
   __shared__ int rowCnt[256];
   __shared__ int rowBase[256];
-  __shared__ uint32_t stg[kStage];
-  __shared__ int stgN;
+  __shared__ uint32_t stg[4 * kStageW];
+  __shared__ int stgW[4];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int mt = (M + 31) / 32;
   for (int i = tid; i < mt * 32 * EMB; i += 256) {
@@ -469,28 +501,31 @@ __global__ void __launch_bounds__(256) k_score(int M, long long N, const float* 
     th[i] = i < M ? thr[i] : INFINITY;
     rowCnt[i] = 0;
   }
-  if (tid == 0) stgN = 0;
   __syncthreads();
   const long long tiles = (N + 63) / 64;
   const int seg = blockIdx.x % NSEG, segCap = cap / NSEG;
+  const uint64_t below = (1ull << lane) - 1ull;  // lanes ranked before this one
+  uint32_t* myStg = stg + wave * kStageW;
+  int nW = 0;  // wave-uniform fill of this wavefront's region
   // block-uniform loop: every wavefront runs the same iterations (barriers)
-  for (long long t0 = (long long)blockIdx.x * 4; t0 < tiles; t0 += (long long)gridDim.x * 4) {
+  const long long step = (long long)gridDim.x * 4;
+  const bf16x8 z = {};
+  for (long long t0 = (long long)blockIdx.x * 4; t0 < tiles; t0 += step) {
     const long long tile = t0 + wave;
     if (tile < tiles) {
       const long long n0 = tile * 64;
-      bf16x8 b0[8], b1[8];
-      {
-        const long long ka = n0 + (lane & 31), kb = ka + 32;
-        const bf16x8 z = {};
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-          b0[s] = ka < N ? *reinterpret_cast<const bf16x8*>(keys + ka * EMB + 16 * s + 8 * (lane >> 5)) : z;
-          b1[s] = kb < N ? *reinterpret_cast<const bf16x8*>(keys + kb * EMB + 16 * s + 8 * (lane >> 5)) : z;
-        }
-      }
       const long long na = n0 + (lane & 31), nb2 = na + 32;
-      for (int m = 0; m < mt; ++m) {
-        f32x16 c0, c1;
+      bf16x8 b0[8], b1[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        b0[s] = na < N ? *reinterpret_cast<const bf16x8*>(keys + na * EMB + 16 * s + 8 * (lane >> 5)) : z;
+        b1[s] = nb2 < N ? *reinterpret_cast<const bf16x8*>(keys + nb2 * EMB + 16 * s + 8 * (lane >> 5)) : z;
+      }
+      // lanes whose key exists (wave masks: the compares below stay scalar)
+      const uint64_t inA = __ballot(na < N), inB = __ballot(nb2 < N);
+      // M tiles software-pipelined: the MFMAs of tile m + 1 are issued before
+      // the threshold epilogue of tile m, which then overlaps them
+      auto mfma_tile = [&](int m, f32x16& c0, f32x16& c1) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) c0[r] = c1[r] = 0.f;
 #pragma unroll
@@ -499,36 +534,54 @@ __global__ void __launch_bounds__(256) k_score(int M, long long N, const float* 
           c0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0[s], c0, 0, 0, 0);
           c1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1[s], c1, 0, 0, 0);
         }
-        // hit mask of this lane: bit 2r <- (row(r), na), bit 2r+1 <- (row(r), nb2)
-        uint32_t hits = 0;
+      };
+      f32x16 c0, c1, d0, d1;
+      mfma_tile(0, c0, c1);
+      for (int m = 0; m < mt; ++m) {
+        if (m + 1 < mt) mfma_tile(m + 1, d0, d1);
+        // thresholds of this lane's 16 rows: four runs of four consecutive rows
+        float4 tq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          tq[q] = *reinterpret_cast<const float4*>(&th[m * 32 + 8 * q + 4 * (lane >> 5)]);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-          const float t = th[m * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)];
-          hits |= (c0[r] >= t && na < N ? 1u : 0u) << (2 * r);
-          hits |= (c1[r] >= t && nb2 < N ? 1u : 0u) << (2 * r + 1);
-        }
-        if (hits) {
-          int slot = atomicAdd(&stgN, __popc(hits));
-          while (hits) {
-            const int bit = __ffs(hits) - 1;
-            hits &= hits - 1;
-            const int r = bit >> 1;
+          const float4 t4 = tq[r >> 2];
+          const float t = (r & 3) == 0 ? t4.x : (r & 3) == 1 ? t4.y : (r & 3) == 2 ? t4.z : t4.w;
+          const uint64_t ma = __ballot(c0[r] >= t) & inA, mb = __ballot(c1[r] >= t) & inB;
+          if (ma | mb) {  // wave-uniform: rare
             const int row = m * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-            const uint32_t e = ((uint32_t)row << 24) | (uint32_t)((bit & 1) ? nb2 : na);
-            if (slot < kStage)
-              stg[slot] = e;
-            else
-              score_emit(e, list, cnt, cap, seg, segCap);
-            ++slot;
+            const int ca = __popcll(ma);
+            const int ia = nW + __popcll(ma & below), ib = nW + ca + __popcll(mb & below);
+            const bool ha = (ma >> lane) & 1ull, hb = (mb >> lane) & 1ull;
+            if (ha) {
+              const uint32_t e = ((uint32_t)row << 24) | (uint32_t)na;
+              if (ia < kStageW) myStg[ia] = e;
+              else score_emit(e, list, cnt, cap, seg, segCap);
+            }
+            if (hb) {
+              const uint32_t e = ((uint32_t)row << 24) | (uint32_t)nb2;
+              if (ib < kStageW) myStg[ib] = e;
+              else score_emit(e, list, cnt, cap, seg, segCap);
+            }
+            nW += ca + __popcll(mb);
           }
         }
+        c0 = d0;
+        c1 = d1;
       }
     }
-    // flush the stage: per-row counts, one global reservation per row, scatter
+    // flush when a region passed half full, and after the last iteration:
+    // per-row counts, one global reservation per row, scatter
+    if (lane == 0) stgW[wave] = min(nW, kStageW);
     __syncthreads();
-    const int n = min(stgN, kStage);
-    if (n) {
-      for (int i = tid; i < n; i += 256) atomicAdd(&rowCnt[stg[i] >> 24], 1);
+    const int n0w = stgW[0], n1w = stgW[1], n2w = stgW[2], n3w = stgW[3];
+    const bool last = t0 + step >= tiles;
+    const int nMax = max(max(n0w, n1w), max(n2w, n3w));
+    if (nMax >= kStageW / 2 || (last && nMax > 0)) {
+      const int nAll[4] = {n0w, n1w, n2w, n3w};
+      for (int w = 0; w < 4; ++w)
+        for (int i = tid; i < nAll[w]; i += 256) atomicAdd(&rowCnt[stg[w * kStageW + i] >> 24], 1);
       __syncthreads();
       {
         const int c = rowCnt[tid];
@@ -536,16 +589,17 @@ __global__ void __launch_bounds__(256) k_score(int M, long long N, const float* 
         rowCnt[tid] = 0;
       }
       __syncthreads();
-      for (int i = tid; i < n; i += 256) {
-        const uint32_t e = stg[i];
-        const int row = (int)(e >> 24);
-        const int slot = rowBase[row] + atomicAdd(&rowCnt[row], 1);
-        if (slot < segCap) list[(long long)row * cap + seg * segCap + slot] = e & 0xFFFFFFu;
-      }
+      for (int w = 0; w < 4; ++w)
+        for (int i = tid; i < nAll[w]; i += 256) {
+          const uint32_t e = stg[w * kStageW + i];
+          const int row = (int)(e >> 24);
+          const int slot = rowBase[row] + atomicAdd(&rowCnt[row], 1);
+          if (slot < segCap) list[(long long)row * cap + seg * segCap + slot] = e & 0xFFFFFFu;
+        }
       __syncthreads();
       rowCnt[tid] = 0;
+      nW = 0;
     }
-    if (tid == 0) stgN = 0;
     __syncthreads();
   }
 }
@@ -651,8 +705,7 @@ __global__ void __launch_bounds__(256) k_rescore(const NodeInfo* __restrict__ in
   const bool usable = ni.nb > 0 && ni.nb <= 128 && V <= 4 && ni.lo[0] <= ni.hi[0] && ni.lo[1] <= ni.hi[1];
   float* eb = ebuf + (long long)node * cap;
   const int cEnd = min(m, c0 + kRescoreChunk);
-  // 8 lanes per position, eight positions per wavefront at a time; each lane
-  // issues all of its (<= 16) neighbour loads before using any of them
+  // 8 lanes per position, eight positions per wavefront at a time
   const int g = lane >> 3, gl = lane & 7;
   const int* segCnt = cnt + node * NSEG;
   for (int c = c0 + wave * 8 + g; c - g < cEnd; c += 32) {
@@ -814,6 +867,9 @@ struct qsmi_lp {
   int* pFlag = nullptr;
   Cand* pCand = nullptr;
   NodeInfo* pInfoEx = nullptr;
+  NodeInfo* pInfoC = nullptr;  // the exact-scan nodes of the wave, compacted
+  int* pPos = nullptr;         // node -> index in pInfoC (-1: not scanned)
+  int* pExCnt = nullptr;       // number of compacted nodes
   int* pAct = nullptr;
   // proposed, fallbacks, overflows, shortlist sum, audit nodes, audit hits, audit best hits, audit positions
   unsigned long long* pCounters = nullptr;
@@ -1285,7 +1341,8 @@ __global__ void k_lp_flags(int nN, int w0, const int* __restrict__ order, const 
                            const int* __restrict__ cnt, int cap, const float* __restrict__ best,
                            const Cand* __restrict__ cand, int K, int every, float mseLimit,
                            const float* __restrict__ cover, float minCover, int audit,
-                           int* __restrict__ flag, NodeInfo* __restrict__ infoEx,
+                           int* __restrict__ flag, NodeInfo* __restrict__ infoEx, NodeInfo* __restrict__ infoC,
+                           int* __restrict__ pos, int* __restrict__ exCnt,
                            unsigned long long* __restrict__ counters) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nN) return;
@@ -1315,7 +1372,16 @@ __global__ void k_lp_flags(int nN, int w0, const int* __restrict__ order, const 
     }
   }
   flag[q] = f;
-  if (!(f == 1 || (f == 2 && audit)))
+  const bool scan = f == 1 || (f == 2 && audit);
+  // compacted list of the nodes to scan (slot order is irrelevant: every
+  // node's scan is independent), and the full list with empty boxes
+  int k = -1;
+  if (scan) {
+    k = atomicAdd(exCnt, 1);
+    infoC[k] = ni;
+  }
+  pos[q] = k;
+  if (!scan)
     for (int d = 0; d < 3; ++d) {  // empty valid box: every scan kernel skips the node
       ni.lo[d] = 1;
       ni.hi[d] = 0;
@@ -1328,8 +1394,8 @@ __global__ void k_lp_flags(int nN, int w0, const int* __restrict__ order, const 
 // shortlist held.
 template <int KT>
 __global__ void __launch_bounds__(256) k_lp_combine(const int* __restrict__ flag, const NodeInfo* __restrict__ info,
-                                                   const Cand* __restrict__ candEx, int blocks, int rowsPerBlock,
-                                                   int K, int audit,
+                                                   const Cand* __restrict__ candEx, const int* __restrict__ pos,
+                                                   int blocks, int rowsPerBlock, int K, int audit,
                                                    const uint32_t* __restrict__ list, const int* __restrict__ cnt,
                                                    int cap, Cand* __restrict__ cand,
                                                    unsigned long long* __restrict__ counters) {
@@ -1350,7 +1416,7 @@ __global__ void __launch_bounds__(256) k_lp_combine(const int* __restrict__ flag
   }
   RankList<KT> rl;
   rl.init();
-  const Cand* ce = candEx + (long long)node * blocks * K;
+  const Cand* ce = candEx + (long long)(pos ? pos[node] : node) * blocks * K;
   for (int i = tid; i < nblk * K; i += 256) {
     const Cand c = ce[i];
     if (c.key > -INFINITY) rl.insert(c.key, c.h, c.pos, c.val);
@@ -1436,6 +1502,7 @@ int lp_prepare(qsmi_lp* lp, const qsmi_ti* t, int maxNodes, int K, int shortlist
       {(void**)&lp->pBest, n * 4},             {(void**)&lp->pFlag, n * 4},
       {(void**)&lp->pCover, n * 4},
       {(void**)&lp->pCand, n * K * sizeof(Cand)}, {(void**)&lp->pInfoEx, n * sizeof(NodeInfo)},
+      {(void**)&lp->pInfoC, n * sizeof(NodeInfo)}, {(void**)&lp->pPos, n * 4}, {(void**)&lp->pExCnt, 4},
       {(void**)&lp->pCounters, 8 * 8},         {(void**)&lp->pAct, 4 * 4}};
   size_t total = 0;
   for (auto& r : req) total += (r.second + 255) / 256 * 256;
@@ -1478,7 +1545,7 @@ void lp_propose(qsmi_lp* lp, const qsmi_ti* t, int V, int nN, const NodeInfo* in
   for (int l = 0; l < 3; ++l) a.H[l] = lp->pH + (size_t)l * lp->pMaxNodes * HID;
   a.E = lp->pQ;
   hipLaunchKernelGGL(k_wave_features, dim3(nN), dim3(128), 0, st, info, nbOff, di, nVar, sd0, lp->dSlot, lp->mu,
-                     1.f / lp->sigma, a.X, lp->pCover);
+                     1.f / lp->sigma, a.X, lp->pCover, lp->pExCnt);
   hipLaunchKernelGGL(k_mlp_fwd, dim3((nN + 31) / 32), dim3(256), 0, st, nN, a.X, lp->dWqT, lp->dW, a.E);
   // per-node threshold: the r-th best sampled score, r = L * sample / S
   gemm(false, true, gargs(nN, kSample, EMB, a.E, EMB, lp->dKsample, EMB, lp->pGs, kSample), st);
@@ -1505,7 +1572,8 @@ void lp_flags(qsmi_lp* lp, int nN, int w0, const int* order, const NodeInfo* inf
   using namespace lp;
   hipLaunchKernelGGL(k_lp_flags, dim3((nN + 127) / 128), dim3(128), 0, st, nN, w0, order, info, lp->pCnt, lp->pCap,
                      lp->pBest, lp->pCand, K, every, mseSigma2 > 0.f ? mseSigma2 * lp->sigma * lp->sigma : 0.f,
-                     lp->pCover, minCover, audit, lp->pFlag, lp->pInfoEx, lp->pCounters);
+                     lp->pCover, minCover, audit, lp->pFlag, lp->pInfoEx, lp->pInfoC, lp->pPos, lp->pExCnt,
+                     lp->pCounters);
 }
 
 int lp_cover_all(qsmi_lp* lp, int nPath, int n, const int* dNbIdx, const int* dNbCnt, const int* dOffs, float minCover,
@@ -1518,12 +1586,13 @@ int lp_cover_all(qsmi_lp* lp, int nPath, int n, const int* dNbIdx, const int* dN
   return QSMI_OK;
 }
 
-void lp_combine(qsmi_lp* lp, int nN, const NodeInfo* info, const Cand* candEx, int blocks, int rowsPerBlock, int K,
-                int audit, hipStream_t st) {
+void lp_combine(qsmi_lp* lp, int nN, const NodeInfo* info, const Cand* candEx, bool compacted, int blocks,
+                int rowsPerBlock, int K, int audit, hipStream_t st) {
   using namespace lp;
-#define QSMI_LP_COMBINE(KT)                                                                                          \
-  hipLaunchKernelGGL(k_lp_combine<KT>, dim3(nN), dim3(256), 0, st, lp->pFlag, info, candEx, blocks, rowsPerBlock, K, \
-                     audit,                                                                                          \
+  const int* pos = compacted ? lp->pPos : nullptr;
+#define QSMI_LP_COMBINE(KT)                                                                                         \
+  hipLaunchKernelGGL(k_lp_combine<KT>, dim3(nN), dim3(256), 0, st, lp->pFlag, info, candEx, pos, blocks,            \
+                     rowsPerBlock, K, audit,                                                                        \
                      lp->pList, lp->pCnt, lp->pCap, lp->pCand, lp->pCounters)
   if (K <= 2) QSMI_LP_COMBINE(2);
   else if (K <= 4) QSMI_LP_COMBINE(4);
@@ -1533,6 +1602,8 @@ void lp_combine(qsmi_lp* lp, int nN, const NodeInfo* info, const Cand* candEx, i
 }
 
 NodeInfo* lp_info_ex(qsmi_lp* lp) { return lp->pInfoEx; }
+NodeInfo* lp_info_compact(qsmi_lp* lp) { return lp->pInfoC; }
+const int* lp_compact_count(qsmi_lp* lp) { return lp->pExCnt; }
 Cand* lp_cand(qsmi_lp* lp) { return lp->pCand; }
 
 int lp_read_counters(qsmi_lp* lp, long long out[8]) {

@@ -352,11 +352,16 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
       // learned proposal for every node, the exact scan for the routed ones
       lp_propose(R.lp, t, P.V, nN, b->dInfo, b->dOff, b->dVal, R.dDi, R.nVar, R.sd[0], K, R.lpMinCover, st);
       lp_flags(R.lp, nN, w0, R.dOrder, b->dInfo, K, R.lpEvery, R.lpMse, R.lpMinCover, R.lpAudit, st);
-      const NodeInfo* infoEx = lp_info_ex(R.lp);
-      if ((rc = launch_spectrum(dev, g, nN, infoEx, b->dOff, b->dVal, b->dC))) return rc;
+      // the nodes to scan: compacted to the front on the 1000x1000 path (the
+      // kernels read the count on the device), else the whole wave with the
+      // other nodes' boxes emptied
+      const bool compact = t->fast1k;
+      const NodeInfo* infoEx = compact ? lp_info_compact(R.lp) : lp_info_ex(R.lp);
+      const int* exCount = compact ? lp_compact_count(R.lp) : nullptr;
+      if ((rc = launch_spectrum(dev, g, nN, infoEx, b->dOff, b->dVal, b->dC, exCount))) return rc;
       if (ev) (void)hipEventRecord(ev[2], st);
-      launch_rows(dev, g, infoEx, b->dC, nN, rb, K, b->dCand, nullptr);
-      lp_combine(R.lp, nN, infoEx, b->dCand, rb, rows_per_block(t), K, R.lpAudit, st);
+      launch_rows(dev, g, infoEx, b->dC, nN, rb, K, b->dCand, nullptr, exCount);
+      lp_combine(R.lp, nN, compact ? b->dInfo : infoEx, b->dCand, compact, rb, rows_per_block(t), K, R.lpAudit, st);
       if (ev) (void)hipEventRecord(ev[3], st);
       launch_merge_draw(g, b->dInfo, lp_cand(R.lp), nN, 1, 0, K, R.k, b->dOutVal, b->dOutPos, nullptr, R.dDi, R.dTi,
                         R.nVar, R.dIndex, st);
