@@ -61,6 +61,7 @@ class OrcSim(C.Structure):
         ("pathLen", C.c_uint),
         ("mode", C.c_int),
         ("quantum", C.c_float),
+        ("cross", C.c_int),
     ]
 
 
@@ -77,6 +78,8 @@ def lib():
     _sig(L, "orc_categories", C.c_int, [C.c_size_t, C.c_int, f32p, C.c_int, f32p, C.c_int])
     _sig(L, "orc_ti_subimages", C.c_int, [C.c_int, u32p, u32p, C.c_int, i32p, f32p, i32p, f32p, C.c_int, f32p])
     _sig(L, "orc_coef_matrix", C.c_int, [C.c_int, i32p, i32p, f32p, i32p, i32p])
+    _sig(L, "orc_coef_matrix_x", C.c_int, [C.c_int, i32p, f32p, f32p, i32p, i32p])
+    _sig(L, "orc_ti_subimages_x", C.c_int, [C.c_int, u32p, u32p, C.c_int, i32p, f32p, C.c_int, f32p])
     _sig(L, "orc_kernel_generate", None, [C.c_int, u32p, C.c_int, i32p, f32p, f32p, f32p])
     _sig(L, "orc_kernel_expand", C.c_int, [C.c_size_t, C.c_int, f32p, i32p, f32p])
     _sig(L, "orc_offsets", C.c_int, [C.c_int, u32p, C.c_int, C.c_void_p, i32p])
@@ -130,6 +133,8 @@ def ref():
     _sig(R, "ref_complex_add", None, [f32p, f32p, f32p, C.c_float, C.c_uint])
     _sig(R, "ref_bgrid_serialize", C.c_size_t, [C.c_int, u32p, C.c_int, i32p, C.c_int, f32p, C.c_void_p, C.c_size_t])
     _sig(R, "ref_bgrid_valid", C.c_int, [C.c_void_p, C.c_size_t])
+    _sig(R, "ref_convert_input4xcorr_x", C.c_int, [C.c_int, u32p, u32p, C.c_int, i32p, f32p, i32p, f32p, f32p])
+    _sig(R, "ref_coef_matrix_x", C.c_int, [C.c_int, i32p, i32p, f32p, f32p, f32p, i32p])
     _REF = R
     return R
 
@@ -243,7 +248,7 @@ class Sim:
     """Serial QS simulation / per-node lockstep checker over the oracle."""
 
     def __init__(self, ti, tidims, types, ncat, catvals, simdims, kdims, kernel_exp, offs, n, k, path, seeds,
-                 begin, mode=0, quantum=0.0):
+                 begin, mode=0, quantum=0.0, cross=False):
         self._keep = []
 
         def keep(a):
@@ -276,6 +281,7 @@ class Sim:
         s.pathLen = path.size
         s.mode = int(mode)
         s.quantum = float(quantum)
+        s.cross = int(bool(cross))
         self.spec = s
         self.nvar = len(types)
         self.K = int(np.ceil(k))

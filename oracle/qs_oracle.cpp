@@ -143,6 +143,70 @@ int orc_ti_subimages(int D, const unsigned* s, const unsigned* f, int nVar, cons
 }
 
 // ---------------------------------------------------------------------------
+// Cross-measurement sub-images (TIs with NaN; continuous variables only):
+// include/DataImage.hpp:538-599 with needCrossMeasurement — per variable
+// [y^2, y, m] with NaN -> 0 in the first two and m = 1 where y is a number,
+// 0 where it is NaN; same f-grid / reflection layout as orc_ti_subimages.
+// Returns nSub (3 per variable), or -1 for a categorical variable.
+// ---------------------------------------------------------------------------
+int orc_ti_subimages_x(int D, const unsigned* s, const unsigned* f, int nVar, const int* types,
+                       const float* data, int reflected, float* out) {
+  size_t F = 1, S = 1;
+  for (int i = 0; i < D; ++i) {
+    F *= f[i];
+    S *= s[i];
+  }
+  for (int v = 0; v < nVar; ++v)
+    if (types[v] != 0) return -1;
+  std::fill(out, out + (size_t)3 * nVar * F, 0.f);
+  for (int v = 0; v < nVar; ++v) {
+    for (size_t c = 0; c < S; ++c) {
+      size_t rem = c, fidx = 0, mul = 1;
+      for (int i = 0; i < D; ++i) {
+        size_t ci = rem % s[i];
+        rem /= s[i];
+        fidx += ci * mul;
+        mul *= f[i];
+      }
+      size_t dst = reflected ? (F - 1 - fidx) : fidx;
+      float y = data[c * nVar + v];
+      out[(size_t)(3 * v) * F + dst] = std::isnan(y) ? 0.f : y * y;
+      out[(size_t)(3 * v + 1) * F + dst] = std::isnan(y) ? 0.f : y;
+      out[(size_t)(3 * v + 2) * F + dst] = std::isnan(y) ? 0.f : 1.f;
+    }
+  }
+  return 3 * nVar;
+}
+
+// ---------------------------------------------------------------------------
+// Cross-measurement coefficient layers — include/DataImage.hpp:626-710 with
+// forXMeasurement (continuous variables only): regular layer diag(-1, +2, -1)
+// over [y^2 x P0, y x P1, m x P2] (no constant term: delta0 = 0), and the
+// cross layer alphaX(m, P0) = 1, i.e. the weighted count of TI cells with a
+// value under the neighbours. convTypes: 0 = P0 (w), 1 = P1 (w x),
+// 2 = P2 (w x^2). Returns nSub, or -1 for a categorical variable.
+// ---------------------------------------------------------------------------
+int orc_coef_matrix_x(int nVar, const int* types, float* alpha, float* alphaX, int* convTypes, int* subVarOf) {
+  for (int v = 0; v < nVar; ++v)
+    if (types[v] != 0) return -1;
+  const int nSub = 3 * nVar;
+  std::fill(alpha, alpha + nSub * nSub, 0.f);
+  std::fill(alphaX, alphaX + nSub * nSub, 0.f);
+  for (int v = 0; v < nVar; ++v) {
+    const int b = 3 * v;
+    alpha[b + b * nSub] = -1.f;
+    alpha[(b + 1) + (b + 1) * nSub] = 2.f;
+    alpha[(b + 2) + (b + 2) * nSub] = -1.f;
+    alphaX[(b + 2) + b * nSub] = 1.f;
+    for (int q = 0; q < 3; ++q) {
+      convTypes[b + q] = q;
+      subVarOf[b + q] = v;
+    }
+  }
+  return nSub;
+}
+
+// ---------------------------------------------------------------------------
 // a3. Coefficient matrix — include/DataImage.hpp:626-710 with
 // include/OperationMatrix.hpp:9-73 (layout alpha[A + B*nSub]).
 // No cross-measurement layer (TIs without NaN). Continuous: (P0, -1) for the
@@ -718,7 +782,7 @@ float orc_convert_neighbors(int D, int nb, const int* off, int nExp, const float
       float x = vals[j * nExp + e];
       float w = kernel[kidx * nExp + (e % nExp)];
       float r = 0.f;
-      if (inside && !std::isnan(x)) r = (convTypes[B] == 0) ? w * 1.f : w * x;
+      if (inside && !std::isnan(x)) r = convTypes[B] == 0 ? w * 1.f : convTypes[B] == 2 ? w * x * x : w * x;
       u[j * nSub + B] = r;
     }
   }

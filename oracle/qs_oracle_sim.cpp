@@ -11,8 +11,15 @@
 // order restated here.
 //
 // Supported here (and by the device): fully-NaN cells to simulate, fully
-// informed hard-data cells, one value of n for all variables, one TI without
-// NaN, no -nV / -fs / kernel images / transforms.
+// informed hard-data cells, one value of n for all variables, one TI, no
+// -nV / -fs / kernel images / transforms. A TI with NaN (continuous
+// variables) runs in the cross-measurement form (`cross`): a cell with any
+// NaN variable loses all of them (src/qs.cpp:1436-1466), the map is
+// -|E / C^4| with E the masked mismatch layer and C the weighted count of
+// TI values under the neighbours times the centre's validity
+// (CPUThreadDevice::maskLayerWithVariable, src/CPUThreadDevice.cpp:500-524;
+// compensateMissingData, src/CPUThreadAcceleratorDevice.cpp:414-425), and
+// the marginal draw skips incomplete cells (simulation.hpp:586-654).
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -30,6 +37,9 @@ extern "C" {
 int orc_ti_subimages(int D, const unsigned* s, const unsigned* f, int nVar, const int* types, const float* data,
                      const int* ncat, const float* catvals, int reflected, float* out);
 int orc_coef_matrix(int nVar, const int* types, const int* ncat, float* alpha, int* convTypes, int* subVarOf);
+int orc_coef_matrix_x(int nVar, const int* types, float* alpha, float* alphaX, int* convTypes, int* subVarOf);
+int orc_ti_subimages_x(int D, const unsigned* s, const unsigned* f, int nVar, const int* types, const float* data,
+                       int reflected, float* out);
 void orc_fft_size(int D, const unsigned* src, unsigned* out);
 float orc_convert_neighbors(int D, int nb, const int* off, int nExp, const float* vals, int nSub,
                             const int* convTypes, const int* subVarOf, int nVarOrig, const int* typesOrig,
@@ -72,6 +82,7 @@ struct OrcSim {
   unsigned pathLen;
   int mode;       // 0: direct FP64 + device rule; 1: refflow FP32 + fKst; 2: refflow FP64 + fKst
   float quantum;  // tie quantum of the device rule (0: none)
+  int cross;      // 1: TI with NaN, cross-measurement form (mode 0 only)
 };
 
 struct OrcSimCtx {
@@ -80,8 +91,10 @@ struct OrcSimCtx {
   unsigned s[3], f[3], sd[3];
   size_t S, F, SD;
   int nSub = 0, nExp = 0;
-  std::vector<float> alpha;
+  std::vector<float> alpha, alphaX;
   std::vector<int> convTypes, subVarOf;
+  std::vector<float> tiX;        // cross form: the TI with incomplete cells all-NaN
+  std::vector<uint8_t> centreOk; // cross form: TI cell complete
   std::vector<float> Y;       // unreflected sub-images on the TI grid (nSub x S)
   std::vector<float> Yhat32;  // reflected spectra (FP32)
   std::vector<double> Yhat64; // reflected spectra (FP64)
@@ -93,6 +106,7 @@ struct OrcSimCtx {
 void* orc_sim_create(const OrcSim* p) {
   auto* c = new OrcSimCtx();
   c->p = *p;
+  if (p->cross) c->p.mode = 0;  // the cross form has the direct FP64 map only
   c->D = p->D;
   c->S = c->F = c->SD = 1;
   for (int i = 0; i < 3; ++i) {
@@ -118,11 +132,33 @@ void* orc_sim_create(const OrcSim* p) {
   c->alpha.resize(64 * 64);
   c->convTypes.resize(64);
   c->subVarOf.resize(64);
-  c->nSub = orc_coef_matrix(p->nVar, p->types, p->ncat, c->alpha.data(), c->convTypes.data(), c->subVarOf.data());
-  c->alpha.resize((size_t)c->nSub * c->nSub);
-  // unreflected sub-images on the TI grid
-  c->Y.resize((size_t)c->nSub * c->S);
-  orc_ti_subimages(p->D, c->s, c->s, p->nVar, p->types, p->ti, p->ncat, p->catvals, 0, c->Y.data());
+  if (p->cross) {
+    // src/qs.cpp:1447-1466: a cell with any NaN variable becomes all-NaN
+    c->tiX.assign(p->ti, p->ti + c->S * p->nVar);
+    c->centreOk.assign(c->S, 1);
+    for (size_t q = 0; q < c->S; ++q) {
+      bool nan = false;
+      for (int v = 0; v < p->nVar; ++v) nan |= std::isnan(c->tiX[q * p->nVar + v]);
+      if (nan) {
+        for (int v = 0; v < p->nVar; ++v) c->tiX[q * p->nVar + v] = NAN;
+        c->centreOk[q] = 0;
+      }
+    }
+    c->p.ti = c->tiX.data();
+    c->alphaX.resize(64 * 64);
+    c->nSub = orc_coef_matrix_x(p->nVar, p->types, c->alpha.data(), c->alphaX.data(), c->convTypes.data(),
+                                c->subVarOf.data());
+    c->alpha.resize((size_t)c->nSub * c->nSub);
+    c->alphaX.resize((size_t)c->nSub * c->nSub);
+    c->Y.resize((size_t)c->nSub * c->S);
+    orc_ti_subimages_x(p->D, c->s, c->s, p->nVar, p->types, c->tiX.data(), 0, c->Y.data());
+  } else {
+    c->nSub = orc_coef_matrix(p->nVar, p->types, p->ncat, c->alpha.data(), c->convTypes.data(), c->subVarOf.data());
+    c->alpha.resize((size_t)c->nSub * c->nSub);
+    // unreflected sub-images on the TI grid
+    c->Y.resize((size_t)c->nSub * c->S);
+    orc_ti_subimages(p->D, c->s, c->s, p->nVar, p->types, p->ti, p->ncat, p->catvals, 0, c->Y.data());
+  }
   if (p->mode == 1 || p->mode == 2) {
     std::vector<float> refl((size_t)c->nSub * c->F);
     orc_ti_subimages(p->D, c->s, c->f, p->nVar, p->types, p->ti, p->ncat, p->catvals, 1, refl.data());
@@ -231,6 +267,53 @@ static int node_inputs(OrcSimCtx* c, const float* di, unsigned pathIdx, std::vec
   return nb;
 }
 
+// Cross-measurement map of one node (FP64): E from the regular layer, C from
+// the cross layer times the centre cell's completeness, out = -|E / C^4|
+// (-inf where C = 0 or outside the valid box). scale (nullable) receives
+// the magnitude the FP32 error of out is measured against:
+// sE / C^4 + 4 |E| sC / C^5 with sE = sum_j w (x^2 m + y^2) and sC = sum_j w.
+static void cross_map(OrcSimCtx* c, int nb, const std::vector<int>& off, const std::vector<float>& u, double* out,
+                      double* scale) {
+  const int D = c->D;
+  std::vector<double> E(c->S), C(c->S);
+  orc_map_direct_f64(D, c->s, c->nSub, c->Y.data(), c->alpha.data(), nb, off.data(), u.data(), 0.0, 0, E.data());
+  orc_map_direct_f64(D, c->s, c->nSub, c->Y.data(), c->alphaX.data(), nb, off.data(), u.data(), 0.0, 0, C.data());
+  std::vector<double> sE;
+  double sC = 0.0;
+  if (scale) {
+    // sum_j w (x^2 m(p+d_j) + y^2(p+d_j)) via the same direct form: |P2| on
+    // the mask image and |P0| on the y^2 image, per variable
+    std::vector<float> al((size_t)c->nSub * c->nSub, 0.f);
+    std::vector<float> ua(u.size());
+    for (size_t q = 0; q < u.size(); ++q) ua[q] = std::fabs(u[q]);
+    for (int v = 0; v < c->p.nVar; ++v) {
+      al[(3 * v) + (3 * v) * c->nSub] = 1.f;
+      al[(3 * v + 2) + (3 * v + 2) * c->nSub] = 1.f;
+    }
+    sE.resize(c->S);
+    orc_map_direct_f64(D, c->s, c->nSub, c->Y.data(), al.data(), nb, off.data(), ua.data(), 0.0, 0, sE.data());
+    for (int j = 0; j < nb; ++j)
+      for (int v = 0; v < c->p.nVar; ++v) sC += std::fabs(u[(size_t)j * c->nSub + 3 * v]);
+  }
+  for (size_t q = 0; q < c->S; ++q) {
+    const double cm = std::isfinite(C[q]) ? C[q] * (double)c->centreOk[q] : C[q];
+    double v;
+    if (!std::isfinite(E[q]) || !std::isfinite(cm) || cm == 0.0)
+      v = -INFINITY;
+    else
+      v = -std::fabs(E[q] / (cm * cm * cm * cm));
+    out[q] = v;
+    if (scale) {
+      if (std::isfinite(v)) {
+        const double c4 = cm * cm * cm * cm;
+        scale[q] = std::fabs(sE[q]) / c4 + 4.0 * std::fabs(E[q]) * sC / (c4 * std::fabs(cm));
+      } else {
+        scale[q] = 0.0;
+      }
+    }
+  }
+}
+
 // One node's scan and draw given the DI state. Writes the K best
 // (value, TI position) pairs sorted best first, and returns the TI position
 // chosen by the draw (or the marginal draw when only the centre is a
@@ -253,12 +336,27 @@ unsigned orc_sim_node(void* h, const float* di, unsigned pathIdx, int K, float* 
   if (nb <= 1) {
     // marginal draw: position = floor(seed * cells) in float
     unsigned position = (unsigned)int(std::floor(seed * (float)(unsigned)c->S));
+    if (p.cross && !c->centreOk[position]) {
+      // simulation.hpp:617-654: rank among the complete cells, the first
+      // cell whose running count reaches it
+      unsigned total = 0;
+      for (size_t q = 0; q < c->S; ++q) total += c->centreOk[q];
+      const unsigned r = (unsigned)int(std::floor(seed * (float)(total / (unsigned)p.nVar))) * (unsigned)p.nVar;
+      unsigned run = 0;
+      for (size_t q = 0; q < c->S; ++q) {
+        run += c->centreOk[q];
+        if (r <= run) return (unsigned)q;
+      }
+    }
     return position;
   }
   if (p.mode == 0) {
     std::vector<double> map(c->S);
-    orc_map_direct_f64(D, c->s, c->nSub, c->Y.data(), c->alpha.data(), nb, off.data(), u.data(), (double)delta0, 0,
-                       map.data());
+    if (p.cross)
+      cross_map(c, nb, off, u, map.data(), nullptr);
+    else
+      orc_map_direct_f64(D, c->s, c->nSub, c->Y.data(), c->alpha.data(), nb, off.data(), u.data(), (double)delta0, 0,
+                         map.data());
     // device rule on the exact map
     uint32_t sb = orc_seed_from_unit_float(seed);
     struct Cd {
@@ -321,6 +419,10 @@ int orc_sim_node_map(void* h, const float* di, unsigned pathIdx, double* map, do
   std::vector<float> raw;
   float delta0 = 0.f;
   int nb = node_inputs(c, di, pathIdx, off, u, &delta0, &raw);
+  if (c->p.cross) {
+    cross_map(c, nb, off, u, map, scale);
+    return nb;
+  }
   orc_map_direct_f64(D, c->s, c->nSub, c->Y.data(), c->alpha.data(), nb, off.data(), u.data(), (double)delta0, 0,
                      map);
   if (!scale) return nb;
@@ -375,7 +477,8 @@ void orc_sim_run(void* h, float* di, unsigned* idxOut) {
     unsigned tpos = orc_sim_node(h, di, i, K, v.data(), pos.data(), &nb);
     if (tpos == UINT_MAX) continue;
     idxOut[cell] = tpos;  // importDataIndex = index*nTI + TI with nTI = 1
-    for (int q = 0; q < p.nVar; ++q) di[(size_t)cell * p.nVar + q] = p.ti[(size_t)tpos * p.nVar + q];
+    const float* tiv = p.cross ? c->tiX.data() : p.ti;
+    for (int q = 0; q < p.nVar; ++q) di[(size_t)cell * p.nVar + q] = tiv[(size_t)tpos * p.nVar + q];
   }
 }
 

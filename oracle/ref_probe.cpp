@@ -11,6 +11,7 @@
 //   DataImage::offsetKernel4categories include/DataImage.hpp:482-508
 //   DataImage::indexWithDelta         include/DataImage.hpp:343-376
 //   g2s::complexAddAlphaxCxD          include/complexMulti.hpp:58-67,303-331
+//   the same two with the cross-measurement layer (TIs with NaN)
 //   DataImage::serialize              include/DataImage.hpp:143-170
 //   DataImage::validateSerializedData include/DataImage.hpp:172-224
 // The glue below only marshals plain arrays in and out.
@@ -149,6 +150,48 @@ int ref_index_with_delta(int D, const unsigned* dims, unsigned position, const i
 
 void ref_complex_add(float* dst, const float* C, const float* Dm, float alpha, unsigned size) {
   g2s::complexAddAlphaxCxD(dst, C, Dm, alpha, size);
+}
+
+// convertInput4Xcorr with needCrossMeasurement (mask image per variable)
+int ref_convert_input4xcorr_x(int D, const unsigned* s, const unsigned* f, int nVar, const int* types,
+                              const float* data, const int* ncat, const float* catvals, float* out) {
+  g2s::DataImage ti = make_image(D, s, nVar, types, data);
+  std::vector<unsigned> fs(f, f + D);
+  auto cats = make_cats(nVar, types, ncat, catvals);
+  auto imgs = ti.convertInput4Xcorr(fs, true, cats);
+  int cnt = 0;
+  for (auto& grp : imgs)
+    for (auto& im : grp) {
+      size_t n = im.dataSize();
+      for (size_t i = 0; i < n; ++i) out[cnt * n + i] = im._data[i];
+      cnt++;
+    }
+  return cnt;
+}
+
+// generateCoefMatrix4Xcorr with forXMeasurement: both layers [A + B*nSub]
+// and the main conversion types flattened; returns nSub.
+int ref_coef_matrix_x(int nVar, const int* types, const int* ncat, const float* catvals, float* alpha, float* alphaX,
+                      int* convTypes) {
+  std::vector<unsigned> dims = {1};
+  std::vector<float> dummy(nVar, 0.f);
+  g2s::DataImage ti = make_image(1, dims.data(), nVar, types, dummy.data());
+  auto cats = make_cats(nVar, types, ncat, catvals);
+  std::vector<g2s::OperationMatrix> coef;
+  std::vector<std::vector<conversionType>> convMain;
+  std::vector<std::vector<std::vector<conversionType>>> convConst;
+  std::vector<std::vector<std::vector<float>>> coefConst;
+  ti.generateCoefMatrix4Xcorr(coef, convMain, convConst, coefConst, true, cats);
+  int nSub = coef[0].getNumberOfVariable();
+  for (int A = 0; A < nSub; ++A)
+    for (int B = 0; B < nSub; ++B) {
+      alpha[A + B * nSub] = coef[0].getVariableAt(A, B);
+      alphaX[A + B * nSub] = coef.size() > 1 ? coef[1].getVariableAt(A, B) : 0.f;
+    }
+  int q = 0;
+  for (auto& v : convMain)
+    for (auto t : v) convTypes[q++] = int(t);
+  return nSub;
 }
 
 // serialized bytes of a grid (returns the full size; copies at most cap bytes)

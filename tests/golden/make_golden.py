@@ -20,6 +20,13 @@ and tests/golden/bgrid_vectors.npz:
                    size field or the length changed, and
                    DataImage::validateSerializedData's verdict on each
 
+and tests/golden/cross_vectors.npz (TIs with NaN, cross-measurement form):
+
+  cx_coef_*        DataImage::generateCoefMatrix4Xcorr with forXMeasurement
+                   (both layers, conversion types) for 1-3 continuous variables
+  cx_sub_*         DataImage::convertInput4Xcorr with needCrossMeasurement
+                   (reflected f-grid images incl. the 0/1 mask images)
+
 Run: python tests/golden/make_golden.py  (needs oracle/_ref built).
 """
 import os
@@ -35,6 +42,7 @@ import oracle  # noqa: E402
 
 OUT = os.path.join(HERE, "ref_vectors.npz")
 OUT_BGRID = os.path.join(HERE, "bgrid_vectors.npz")
+OUT_CROSS = os.path.join(HERE, "cross_vectors.npz")
 
 
 def ti_cases():
@@ -239,9 +247,45 @@ def write_bgrid():
     print(f"wrote {OUT_BGRID} ({os.path.getsize(OUT_BGRID)} bytes)")
 
 
+def write_cross():
+    R = oracle.ref()
+    if R is None:
+        raise SystemExit("oracle/_ref is not built: run make -C oracle with the reference tree present")
+    out = {}
+    for nv in (1, 2, 3):
+        t = np.zeros(nv, np.int32)
+        a = np.zeros(64 * 64, np.float32)
+        ax = np.zeros(64 * 64, np.float32)
+        ct = np.zeros(64, np.int32)
+        n = R.ref_coef_matrix_x(nv, t, np.zeros(1, np.int32), np.zeros(1, np.float32), a, ax, ct)
+        out[f"cx_coef_{nv}_alpha"] = a[:n * n].copy()
+        out[f"cx_coef_{nv}_alphaX"] = ax[:n * n].copy()
+        out[f"cx_coef_{nv}_conv"] = ct[:n].copy()
+    rng = np.random.default_rng(4321)
+    cases = [((7, 5), 1), ((9,), 2), ((4, 3, 5), 1), ((6, 4), 3)]
+    for i, (dims, nv) in enumerate(cases):
+        cells = int(np.prod(dims))
+        data = rng.standard_normal(cells * nv).astype(np.float32)
+        data[rng.random(cells * nv) < 0.2] = np.nan
+        f = oracle.fft_size(dims)
+        F = int(np.prod(f))
+        buf = np.zeros(3 * nv * F, np.float32)
+        n = R.ref_convert_input4xcorr_x(len(dims), oracle.u32(dims), f, nv, np.zeros(nv, np.int32), data,
+                                        np.zeros(1, np.int32), np.zeros(1, np.float32), buf)
+        out[f"cx_sub_{i}_dims"] = np.array(dims, np.uint32)
+        out[f"cx_sub_{i}_nvar"] = np.int32(nv)
+        out[f"cx_sub_{i}_data"] = data
+        out[f"cx_sub_{i}_out"] = buf[:n * F].copy()
+    np.savez_compressed(OUT_CROSS, **out)
+    print(f"wrote {OUT_CROSS} ({os.path.getsize(OUT_CROSS)} bytes)")
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "bgrid":
         write_bgrid()
+    elif len(sys.argv) > 1 and sys.argv[1] == "cross":
+        write_cross()
     else:
         main()
         write_bgrid()
+        write_cross()
