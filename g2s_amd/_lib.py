@@ -62,6 +62,8 @@ SIGNATURES = {
     "qsmi_lp_loss_grad": (C.c_int, [vp, C.c_int, C.c_int, f32p, f32p, C.c_float, C.POINTER(C.c_float), vp]),
     "qsmi_lp_train": (C.c_int, [vp, vp, vp, vp]),
     "qsmi_lp_build_keys": (C.c_int, [vp, u32p, f32p, i32p]),
+    "qsmi_coef_matrix_x": (C.c_int, [C.c_int, i32p, f32p, f32p, i32p, i32p, C.POINTER(C.c_int)]),
+    "qsmi_ti_subimages_x": (C.c_int, [C.c_int, u32p, C.c_int, i32p, f32p, f32p]),
     "qsmi_bgrid_bytes": (C.c_int, [C.c_int, u32p, C.c_int, C.POINTER(C.c_size_t)]),
     "qsmi_bgrid_serialize": (C.c_int, [C.c_int, u32p, C.c_int, i32p, C.c_int, f32p, vp, C.c_size_t]),
     "qsmi_bgrid_valid": (C.c_int, [vp, C.c_size_t]),

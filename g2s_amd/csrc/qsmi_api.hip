@@ -27,7 +27,8 @@ void launch_pencil(const ScanGeom& g, const FftPlan& planP, int lognv, const flo
 void launch_direct1d(const ScanGeom& g, const float2* Yhat, const float2* tw0, int nNodes, const NodeInfo* info,
                      const int* nbOff, const float* nbVal, float2* C, hipStream_t st);
 void launch_rows_select(const ScanGeom& g, const FftPlan& plan0, int lognv, const float2* tw0, const NodeInfo* info,
-                        const float2* C, int nNodes, int rowBlocks, int K, Cand* cand, float* map, hipStream_t st);
+                        const float2* C, int nNodes, int rowBlocks, int K, Cand* cand, float* map, hipStream_t st,
+                        int crossOff = 0, const uint8_t* centreOk = nullptr);
 void launch_select_map(const float* map, long long S, int K, float quantum, uint32_t seedbits, Cand* cand, int chunks,
                        hipStream_t st);
 void launch_fill(float* p, long long n, float v, hipStream_t st);
@@ -267,12 +268,38 @@ int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeI
 }
 
 void launch_rows(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const float2* C, int nNodes, int rowBlocks,
-                 int K, Cand* cand, float* map, const int* count) {
+                 int K, Cand* cand, float* map, const int* count, int crossOff) {
   qsmi_ti* t = dev->ti;
   if (t->fast1k)
     launch_rows1k(g, t->dTw[0], info, C, nNodes, K, cand, map, count, dev->stream);
   else
-    launch_rows_select(g, t->plan[0], t->lognv[0], t->dTw[0], info, C, nNodes, rowBlocks, K, cand, map, dev->stream);
+    launch_rows_select(g, t->plan[0], t->lognv[0], t->dTw[0], info, C, nNodes, rowBlocks, K, cand, map, dev->stream,
+                       crossOff, dev->dCentreOk);
+}
+
+int set_cross(qsmi_device* dev, const float* alphaX, const uint8_t* centreOk) {
+  qsmi_ti* t = dev->ti;
+  const int nSub = t->nSub;
+  dev->alphaX.assign(alphaX, alphaX + nSub * nSub);
+  for (int A = 0; A < nSub; ++A) {  // the count layer's images are scanned too
+    bool need = false;
+    for (int B = 0; B < nSub; ++B) need |= alphaX[A + B * nSub] != 0.f;
+    if (need && std::find(dev->active.begin(), dev->active.end(), A) == dev->active.end()) dev->active.push_back(A);
+  }
+  std::sort(dev->active.begin(), dev->active.end());
+  std::vector<uint32_t> complete;
+  for (long long q = 0; q < t->S; ++q)
+    if (centreOk[q]) complete.push_back((uint32_t)q);
+  dev->nComplete = (uint32_t)complete.size();
+  HIPCHK(hipSetDevice(t->gpu));
+  HIPCHK(dalloc(&dev->dCentreOk, (size_t)t->S));
+  HIPCHK(dalloc(&dev->dComplete, std::max<size_t>(complete.size(), 1)));
+  HIPCHK(hipMemcpy(dev->dCentreOk, centreOk, (size_t)t->S, hipMemcpyHostToDevice));
+  if (!complete.empty())
+    HIPCHK(hipMemcpy(dev->dComplete, complete.data(), complete.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  t->fast1k = false;  // the two-layer rows stage exists on the generic path
+  dev->cross = true;
+  return QSMI_OK;
 }
 
 int check_dev(qsmi_device* dev) {
@@ -482,6 +509,8 @@ int qsmi_device_destroy(qsmi_device* d) {
   hipFree(d->dOutVal);
   hipFree(d->dOutPos);
   hipFree(d->dChosen);
+  hipFree(d->dCentreOk);
+  hipFree(d->dComplete);
   if (d->wb) {
     free_wave_bufs(d->wb);
     delete d->wb;

@@ -151,6 +151,52 @@ int qsmi_ti_subimages(int ndim, const unsigned* s, int nVar, const int* types, c
   return QSMI_OK;
 }
 
+int qsmi_coef_matrix_x(int nVar, const int* types, float* alpha, float* alphaX, int* convTypes, int* subVarOf,
+                       int* nSubOut) {
+  if (!types || !alpha || !alphaX || !convTypes || !subVarOf || !nSubOut) return hfail(QSMI_ERR_ARG, "null argument");
+  for (int v = 0; v < nVar; ++v)
+    if (types[v] != 0) return hfail(QSMI_ERR_UNSUPPORTED, "cross-measurement layer: continuous variables only");
+  const int nSub = 3 * nVar;
+  if (nSub > 16) return hfail(QSMI_ERR_UNSUPPORTED, "more than 16 sub-variables");
+  std::fill(alpha, alpha + nSub * nSub, 0.f);
+  std::fill(alphaX, alphaX + nSub * nSub, 0.f);
+  for (int v = 0; v < nVar; ++v) {
+    // [y^2 x P0] -1, [y x P1] +2, [m x P2] -1; count layer [m x P0] +1
+    const int b = 3 * v;
+    alpha[b + b * nSub] = -1.f;
+    alpha[(b + 1) + (b + 1) * nSub] = 2.f;
+    alpha[(b + 2) + (b + 2) * nSub] = -1.f;
+    alphaX[(b + 2) + b * nSub] = 1.f;
+    for (int q = 0; q < 3; ++q) {
+      convTypes[b + q] = q;
+      subVarOf[b + q] = v;
+    }
+  }
+  *nSubOut = nSub;
+  return QSMI_OK;
+}
+
+int qsmi_ti_subimages_x(int ndim, const unsigned* s, int nVar, const int* types, const float* ti, float* out) {
+  if (!s || !types || !ti || !out || ndim < 1 || ndim > 3) return hfail(QSMI_ERR_ARG, "invalid argument");
+  for (int v = 0; v < nVar; ++v)
+    if (types[v] != 0) return hfail(QSMI_ERR_UNSUPPORTED, "cross-measurement layer: continuous variables only");
+  size_t S = 1;
+  for (int i = 0; i < ndim; ++i) S *= s[i];
+  for (int v = 0; v < nVar; ++v) {
+    float* y2 = out + (size_t)(3 * v) * S;
+    float* y1 = out + (size_t)(3 * v + 1) * S;
+    float* m = out + (size_t)(3 * v + 2) * S;
+    for (size_t c = 0; c < S; ++c) {
+      const float y = ti[c * nVar + v];
+      const bool nan = std::isnan(y);
+      y2[c] = nan ? 0.f : y * y;
+      y1[c] = nan ? 0.f : y;
+      m[c] = nan ? 0.f : 1.f;
+    }
+  }
+  return QSMI_OK;
+}
+
 int qsmi_offsets(int ndim, const unsigned* ks, int kNVar, const float* kernel, int* out, int capacity, int* count) {
   if (!ks || !out || !count || ndim < 1 || ndim > 3) return hfail(QSMI_ERR_ARG, "invalid argument");
   // generation order of src/qs.cpp:1055-1084: dimension by dimension, each
@@ -235,8 +281,32 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     S *= s[i];
     SD *= sd[i];
   }
-  for (size_t c = 0; c < S * nVar; ++c)
-    if (std::isnan(rq->ti[c])) return hfail(QSMI_ERR_UNSUPPORTED, "TI with NaN (cross-measurement layer) unsupported");
+  // a TI with NaN runs in the cross-measurement form (src/qs.cpp:1436-1466):
+  // cells with any NaN variable lose all of them
+  bool cross = false;
+  for (size_t c = 0; c < S * nVar && !cross; ++c) cross = std::isnan(rq->ti[c]);
+  std::vector<float> tiX;
+  std::vector<uint8_t> centreOk;
+  if (cross) {
+    for (int v = 0; v < nVar; ++v)
+      if (rq->varTypes[v] != 0)
+        return hfail(QSMI_ERR_UNSUPPORTED,
+                     "TI with NaN and a categorical variable: the reference's cross-measurement conversion reads one "
+                     "value past each neighbour's expanded values (include/simulation.hpp:512-537 against "
+                     "include/DataImage.hpp:681-692); continuous variables only");
+    if (rq->lp || rq->recordCap > 0) return hfail(QSMI_ERR_UNSUPPORTED, "learned proposal: TI with NaN unsupported");
+    tiX.assign(rq->ti, rq->ti + S * nVar);
+    centreOk.assign(S, 1);
+    for (size_t c = 0; c < S; ++c) {
+      bool nan = false;
+      for (int v = 0; v < nVar; ++v) nan |= std::isnan(tiX[c * nVar + v]);
+      if (nan) {
+        for (int v = 0; v < nVar; ++v) tiX[c * nVar + v] = NAN;
+        centreOk[c] = 0;
+      }
+    }
+  }
+  const float* tiData = cross ? tiX.data() : rq->ti;
   if ((rq->lp || rq->recordCap > 0) && !(D == 2 && nVar == 1 && rq->varTypes[0] == 0))
     return hfail(QSMI_ERR_UNSUPPORTED, "learned proposal and pair record: 2-D, one continuous variable");
   if (rq->recordCap > 0 && (!rq->recOffsets || !rq->recValues || !rq->recBest))
@@ -257,10 +327,11 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     }
   int nExp = 0;
   for (int v = 0; v < nVar; ++v) nExp += factor[v];
-  float alpha[256];
+  float alpha[256], alphaX[256];
   int convTypes[16], subVarOf[16], nSub = 0;
   {
-    int rc = qsmi_coef_matrix(nVar, rq->varTypes, ncat.data(), alpha, convTypes, subVarOf, &nSub);
+    int rc = cross ? qsmi_coef_matrix_x(nVar, rq->varTypes, alpha, alphaX, convTypes, subVarOf, &nSub)
+                   : qsmi_coef_matrix(nVar, rq->varTypes, ncat.data(), alpha, convTypes, subVarOf, &nSub);
     if (rc) return rc;
   }
   // kernel (src/qs.cpp:1010-1042) expanded per one-hot variable
@@ -309,13 +380,18 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   const size_t nPath = SD - begin;
   // device: TI spectra + scan engine
   std::vector<float> sub((size_t)nSub * S);
-  qsmi_ti_subimages(D, s, nVar, rq->varTypes, rq->ti, ncat.data(), catvals.data(), sub.data());
+  if (cross)
+    qsmi_ti_subimages_x(D, s, nVar, rq->varTypes, tiData, sub.data());
+  else
+    qsmi_ti_subimages(D, s, nVar, rq->varTypes, rq->ti, ncat.data(), catvals.data(), sub.data());
   qsmi_ti* ti = nullptr;
   qsmi_device* dev = nullptr;
   int rc = qsmi_ti_create(rq->gpu, D, s, nullptr, nSub, sub.data(), &ti);
   if (rc) return rc;
   rc = qsmi_device_create(ti, alpha, 0, &dev);
+  if (!rc && cross) rc = qsmi::set_cross(dev, alphaX, centreOk.data());
   if (rc) {
+    if (dev) qsmi_device_destroy(dev);
     qsmi_ti_destroy(ti);
     return rc;
   }
@@ -460,7 +536,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   if (!rc) rc = dev_upload(catv.data(), catv.size() * sizeof(float), &dCat);
   if (!rc) rc = dev_upload(di, SD * nVar * sizeof(float), &dDi);
   if (!rc) rc = dev_upload(rs->index, SD * sizeof(unsigned), &dIdx);
-  if (!rc) rc = dev_upload(rq->ti, S * nVar * sizeof(float), &dTi);
+  if (!rc) rc = dev_upload(tiData, S * nVar * sizeof(float), &dTi);
   const size_t recCap = rq->recordCap > 0 ? std::min<size_t>((size_t)rq->recordCap, nPath) : 0;
   void *dRecOff = nullptr, *dRecVal = nullptr, *dRecBest = nullptr;
   if (!rc && recCap) {

@@ -40,6 +40,13 @@ struct qsmi_device {
   qsmi_ti* ti = nullptr;
   hipStream_t stream = nullptr;
   std::vector<float> alpha;  // nSub x nSub, [A + B*nSub]
+  // cross-measurement form (TI with NaN): the count layer's coefficients,
+  // per TI cell 1 when complete, and the complete cells in order
+  bool cross = false;
+  std::vector<float> alphaX;
+  uint8_t* dCentreOk = nullptr;
+  uint32_t* dComplete = nullptr;
+  uint32_t nComplete = 0;
   std::vector<int> active;   // TI sub-variables with a non-zero alpha row
   int circular = 0;
   int trueMismatch = 1;
@@ -120,8 +127,10 @@ int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeI
                     const float* val, float2* C, const int* count = nullptr);
 // tap tables of the 1000^2 spectrum stage for nNodes nodes per launch
 int ensure_tap_tables(qsmi_device* dev, int nNodes);
+// crossOff > 0 (cross-measurement form, generic path): node q's count layer
+// is node q + crossOff of C; rowBlocks then covers one row per FFT
 void launch_rows(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const float2* C, int nNodes,
-                 int rowBlocks, int K, Cand* cand, float* map, const int* count = nullptr);
+                 int rowBlocks, int K, Cand* cand, float* map, const int* count = nullptr, int crossOff = 0);
 // merge + draw; when simVals != nullptr also writes the drawn TI values and
 // the index image of each node's cell (NodeInfo::cell)
 void launch_merge_draw(const ScanGeom& g, const NodeInfo* info, const Cand* cand, int nNodes, int candBlocks,
@@ -195,5 +204,8 @@ int nb_search_device(int D, const unsigned* sd, size_t nPath, const unsigned* dC
 // stageMs: [0] spectrum (+ dim1 inverse), [1] gather, [2] rows + select,
 // [3] merge + draw + write; filled when R.stageTiming.
 int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMs, long long* scanned, int* waves);
+// cross-measurement form: the count layer's coefficients and the TI cells'
+// completeness (forces the generic kernels)
+int set_cross(qsmi_device* dev, const float* alphaX, const uint8_t* centreOk);
 
 }  // namespace qsmi

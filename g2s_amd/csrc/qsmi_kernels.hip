@@ -343,7 +343,8 @@ __global__ void __launch_bounds__(256) k_rows_select(ScanGeom g, FftPlan plan0, 
                                                      const float2* __restrict__ tw0g,
                                                      const NodeInfo* __restrict__ info,
                                                      const float2* __restrict__ C, int K, Cand* __restrict__ cand,
-                                                     int candBlocks, float* __restrict__ map) {
+                                                     int candBlocks, float* __restrict__ map, int crossOff,
+                                                     const uint8_t* __restrict__ centreOk) {
   extern __shared__ float2 smem[];
   __shared__ Cand red[16];
   const int L = plan0.L, nvec = 1 << lognv, tid = threadIdx.x, nthr = blockDim.x;
@@ -353,7 +354,11 @@ __global__ void __launch_bounds__(256) k_rows_select(ScanGeom g, FftPlan plan0, 
   const int n2v = ni.hi[2] - ni.lo[2] + 1;
   const int n0v = ni.hi[0] - ni.lo[0] + 1;
   const long long nrv = (n1v > 0 && n2v > 0 && n0v > 0) ? (long long)n1v * n2v : 0;
-  const long long q0 = (long long)blockIdx.x * 2 * nvec;
+  // cross-measurement form (crossOff > 0): one row per complex FFT, the
+  // mismatch layer (node) in the real part, the count layer (node +
+  // crossOff) in the imaginary part; else two rows of the one layer
+  const bool cross = crossOff > 0;
+  const long long q0 = (long long)blockIdx.x * (cross ? 1 : 2) * nvec;
   if (q0 >= nrv) return;
   float2* tw = smem;
   float2* a = tw + L;
@@ -361,10 +366,11 @@ __global__ void __launch_bounds__(256) k_rows_select(ScanGeom g, FftPlan plan0, 
   for (int i = tid; i < L; i += nthr) tw[i] = tw0g[i];
   const int h0 = g.h0, f1 = g.f[1];
   const float2* Cn = C + (long long)node * g.H;
+  const float2* Cx = cross ? C + (long long)(node + crossOff) * g.H : Cn;
   // load Z = Aext + i Bext
   for (int e = tid; e < L * nvec; e += nthr) {
     const int k = e >> lognv, v = e & (nvec - 1);
-    const long long qa = q0 + 2 * v, qb = qa + 1;
+    const long long qa = q0 + (cross ? v : 2 * v), qb = cross ? qa : qa + 1;
     const bool conjk = (k >= h0);
     const int kk = conjk ? (L - k) : k;
     float2 A = make_float2(0.f, 0.f), B = make_float2(0.f, 0.f);
@@ -376,7 +382,7 @@ __global__ void __launch_bounds__(256) k_rows_select(ScanGeom g, FftPlan plan0, 
     if (qb < nrv) {
       const int c1 = ni.lo[1] + (int)(qb % n1v);
       const int c2 = ni.lo[2] + (int)(qb / n1v);
-      B = Cn[((long long)c2 * f1 + c1) * h0 + kk];
+      B = Cx[((long long)c2 * f1 + c1) * h0 + kk];
     }
     // a Hermitian c2r ignores the imaginary part of the DC and Nyquist bins
     if (kk == 0 || 2 * kk == L) {
@@ -399,6 +405,24 @@ __global__ void __launch_bounds__(256) k_rows_select(ScanGeom g, FftPlan plan0, 
     const int c0 = e >> lognv, v = e & (nvec - 1);
     if (c0 < ni.lo[0] || c0 > ni.hi[0]) continue;
     const float2 zz = z[e];
+    if (cross) {
+      // CPUThreadDevice::maskLayerWithVariable + compensateMissingData:
+      // count times the centre's completeness, map = -|E / C^4|, no
+      // candidate where the count is zero
+      const long long q = q0 + v;
+      if (q >= nrv) continue;
+      const int c1 = ni.lo[1] + (int)(q % n1v);
+      const int c2 = ni.lo[2] + (int)(q / n1v);
+      const uint32_t pos = (uint32_t)(c0 + s0 * (c1 + s1 * c2));
+      // the count is a sum of neighbour weights: values below cmin (half
+      // the smallest weight) are FFT noise around an exact zero
+      const float E = zz.x * scale;
+      const float Cc = (centreOk[pos] && zz.y * g.invN >= ni.cmin) ? zz.y * scale : 0.f;
+      const float val = (Cc == 0.f) ? -INFINITY : -fabsf(E / (Cc * Cc * Cc * Cc));
+      if (map) map[(long long)node * ((long long)s0 * s1 * g.s[2]) + pos] = val;
+      if (cand && val > -INFINITY) tk.push_value(val, 0.f, pos, ni.seedbits);
+      continue;
+    }
 #pragma unroll
     for (int w = 0; w < 2; ++w) {
       const long long q = q0 + 2 * v + w;
@@ -498,7 +522,7 @@ template __global__ void k_colfft<+1>(float2*, FftPlan, const float2*, int, long
                                       const NodeInfo*, int, int);
 #define QSMI_INST_K(KT)                                                                                            \
   template __global__ void k_rows_select<KT>(ScanGeom, FftPlan, int, const float2*, const NodeInfo*,              \
-                                             const float2*, int, Cand*, int, float*);                             \
+                                             const float2*, int, Cand*, int, float*, int, const uint8_t*);        \
   template __global__ void k_select_map<KT>(const float*, long long, int, float, uint32_t, Cand*);                \
   template __global__ void k_merge_draw<KT>(ScanGeom, const NodeInfo*, const Cand*, int, int, int, float, float*, \
                                             uint32_t*, uint32_t*, float*, const float*, int, uint32_t*);
@@ -576,16 +600,21 @@ void launch_direct1d(const ScanGeom& g, const float2* Yhat, const float2* tw0, i
 static int kt_for(int K) { return K <= 2 ? 2 : K <= 4 ? 4 : K <= 8 ? 8 : 16; }
 
 void launch_rows_select(const ScanGeom& g, const FftPlan& plan0, int lognv, const float2* tw0, const NodeInfo* info,
-                        const float2* C, int nNodes, int rowBlocks, int K, Cand* cand, float* map, hipStream_t st) {
+                        const float2* C, int nNodes, int rowBlocks, int K, Cand* cand, float* map, hipStream_t st,
+                        int crossOff, const uint8_t* centreOk) {
   const int nvec = 1 << lognv;
   dim3 grid((unsigned)rowBlocks, (unsigned)nNodes);
   size_t lds = fft_lds_bytes(plan0.L, nvec);
+#define QSMI_ROWS(KT)                                                                                        \
+  hipLaunchKernelGGL(k_rows_select<KT>, grid, dim3(256), lds, st, g, plan0, lognv, tw0, info, C, K, cand, rowBlocks, \
+                     map, crossOff, centreOk)
   switch (kt_for(K)) {
-    case 2: hipLaunchKernelGGL(k_rows_select<2>, grid, dim3(256), lds, st, g, plan0, lognv, tw0, info, C, K, cand, rowBlocks, map); break;
-    case 4: hipLaunchKernelGGL(k_rows_select<4>, grid, dim3(256), lds, st, g, plan0, lognv, tw0, info, C, K, cand, rowBlocks, map); break;
-    case 8: hipLaunchKernelGGL(k_rows_select<8>, grid, dim3(256), lds, st, g, plan0, lognv, tw0, info, C, K, cand, rowBlocks, map); break;
-    default: hipLaunchKernelGGL(k_rows_select<16>, grid, dim3(256), lds, st, g, plan0, lognv, tw0, info, C, K, cand, rowBlocks, map); break;
+    case 2: QSMI_ROWS(2); break;
+    case 4: QSMI_ROWS(4); break;
+    case 8: QSMI_ROWS(8); break;
+    default: QSMI_ROWS(16); break;
   }
+#undef QSMI_ROWS
 }
 
 void launch_select_map(const float* map, long long S, int K, float quantum, uint32_t seedbits, Cand* cand, int chunks,

@@ -47,6 +47,9 @@ struct NodeInfo {
   uint32_t seedbits;  // seedFromUnitFloat(seed)
   float quantum;      // tie quantum (0: none)
   uint32_t cell;      // simulation cell (simulation driver only)
+  float cmin;         // cross-measurement form: a count layer value (a sum of
+                      // neighbour weights) below it is zero: half the
+                      // smallest neighbour weight (0 otherwise)
 };
 
 // Candidate of the device selection rule.

@@ -28,10 +28,12 @@ struct GatherParams {
   int types[8];    // per original variable: 0 continuous, 1 categorical
   int expVar[16];  // expanded index -> original variable
   int expCat[16];  // expanded index -> offset of its class value in catvals, -1 continuous
-  int conv[16];    // per neighbour sub-variable: 0 = P0 (w), 1 = P1 (w*x)
+  int conv[16];    // per neighbour sub-variable: 0 = P0 (w), 1 = P1 (w*x), 2 = P2 (w*x^2)
   int subOf[16];   // per neighbour sub-variable: expanded index
   int act[16];     // active TI sub-variables (alpha row non-zero)
   float alpha[256];  // [A + B*nSub]
+  float alphaX[256]; // cross-measurement form: the count layer's coefficients
+  int cross;         // 1: TI with NaN (two layers per node, no delta0)
   int s[3];        // TI grid (valid box)
   int circular;
   float scale;     // map scale (1/N true mismatch, 1 otherwise)
@@ -46,15 +48,19 @@ __device__ __forceinline__ uint32_t seed_bits_dev(float seed) {
   return (uint32_t)floor((double)seed * 4294967295.0);
 }
 
-// One wavefront per node of the wave (order[w0 + q]).
-__global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, const int* __restrict__ order,
+// One wavefront per node of the wave (order[w0 + q]). In the cross-
+// measurement form node q's count layer is written as node nN + q (same
+// neighbours, the count-layer values, an offset copy of its own).
+__global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, const int* __restrict__ order,
                                                const unsigned* __restrict__ pathCells,
                                                const float* __restrict__ pathSeeds, const int* __restrict__ nbIdx,
                                                const int* __restrict__ nbCnt, const int* __restrict__ offs,
                                                const float* __restrict__ kern, const float* __restrict__ catvals,
                                                float* __restrict__ di, uint32_t* __restrict__ index,
                                                const float* __restrict__ tiVals, NodeInfo* __restrict__ info,
-                                               int* __restrict__ nbOff, float* __restrict__ nbVal) {
+                                               int* __restrict__ nbOff, float* __restrict__ nbVal,
+                                               const uint8_t* __restrict__ centreOk,
+                                               const uint32_t* __restrict__ complete, uint32_t nComplete) {
   const int q = blockIdx.x;
   const int lane = threadIdx.x;
   const int i = order[w0 + q];
@@ -69,10 +75,17 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, const int
   ni.quantum = P.quantum;
   ni.scale = P.scale;
   ni.nbStart = q * P.n;
+  ni.cmin = 0.f;
   if (cnt <= 1) {
     // only the cell itself: draw from the marginal (simulation.hpp:586-606)
     if (lane == 0) {
-      const unsigned position = (unsigned)(int)floorf(seed * (float)P.tiCells);
+      unsigned position = (unsigned)(int)floorf(seed * (float)P.tiCells);
+      if (P.cross && !centreOk[position]) {
+        // incomplete cell: rank among the complete cells (simulation.hpp:
+        // 617-654; rank 0 keeps cell 0 as the reference's walk does)
+        const unsigned r = (unsigned)(int)floorf(seed * (float)(nComplete / (unsigned)P.nVar)) * (unsigned)P.nVar;
+        position = r == 0 ? 0u : complete[r - 1];
+      }
       for (int v = 0; v < P.nVar; ++v) di[(long long)cell * P.nVar + v] = tiVals[(long long)position * P.nVar + v];
       index[cell] = position;
       ni.nb = 0;
@@ -82,6 +95,7 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, const int
         ni.hi[d] = 0;
       }
       info[q] = ni;
+      if (P.cross) info[nN + q] = ni;
     }
     return;
   }
@@ -95,6 +109,7 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, const int
   }
   const int* idx = nbIdx + (long long)i * P.n;
   int mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+  float wmin = INFINITY;  // smallest weight of a neighbour value (cross form)
   for (int j = lane; j < cnt; j += 64) {
     const int* dd = offs + (long long)idx[j] * D;
     long long t = 0, mul = 1;
@@ -115,21 +130,32 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, const int
     }
     const float* x = di + t * P.nVar;
     for (int d = 0; d < D; ++d) nbOff[((long long)q * P.n + j) * D + d] = dd[d];
-    // v_a = sum_B alpha(A_a, B) u_B with u_B the QSM conversion of sub-variable B
-    for (int a = 0; a < P.V; ++a) {
-      const int A = P.act[a];
-      float acc = 0.f;
-      for (int B = 0; B < P.nSub; ++B) {
-        const float al = P.alpha[A + B * P.nSub];
-        if (al == 0.f) continue;
-        const int e = P.subOf[B];
-        const float xv = x[P.expVar[e]];
-        const float xe = (P.expCat[e] < 0) ? xv : ((xv == catvals[P.expCat[e]]) ? 1.f : 0.f);
-        const float w = kern[kc * P.nExp + e];
-        const float u = (inK && !isnan(xe)) ? (P.conv[B] == 0 ? w * 1.f : w * xe) : 0.f;
-        acc += al * u;
+    if (P.cross) {
+      for (int d = 0; d < D; ++d) nbOff[((long long)(nN + q) * P.n + j) * D + d] = dd[d];
+      for (int v = 0; v < P.nVar; ++v) {
+        const float w = kern[kc * P.nExp + v];
+        if (inK && !isnan(x[v]) && w > 0.f) wmin = fminf(wmin, w);
       }
-      nbVal[((long long)q * P.n + j) * P.V + a] = acc;
+    }
+    // v_a = sum_B alpha(A_a, B) u_B with u_B the QSM conversion of sub-variable B
+    for (int layer = 0; layer < (P.cross ? 2 : 1); ++layer) {
+      const float* al_ = layer == 0 ? P.alpha : P.alphaX;
+      const long long row = (long long)(layer == 0 ? q : nN + q) * P.n + j;
+      for (int a = 0; a < P.V; ++a) {
+        const int A = P.act[a];
+        float acc = 0.f;
+        for (int B = 0; B < P.nSub; ++B) {
+          const float al = al_[A + B * P.nSub];
+          if (al == 0.f) continue;
+          const int e = P.subOf[B];
+          const float xv = x[P.expVar[e]];
+          const float xe = (P.expCat[e] < 0) ? xv : ((xv == catvals[P.expCat[e]]) ? 1.f : 0.f);
+          const float w = kern[kc * P.nExp + e];
+          const float u = (inK && !isnan(xe)) ? (P.conv[B] == 0 ? w * 1.f : P.conv[B] == 2 ? w * xe * xe : w * xe) : 0.f;
+          acc += al * u;
+        }
+        nbVal[row * P.V + a] = acc;
+      }
     }
   }
   // bounding box (min/max start at 0, src/AcceleratorDevice.cpp:95-108)
@@ -140,6 +166,8 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, const int
       mx[d] = max(mx[d], __shfl_xor(mx[d], off));
     }
   }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) wmin = fminf(wmin, __shfl_xor(wmin, off));
   // delta0 in the reference's order: variable by variable, neighbour by
   // neighbour, float accumulation of -w x^2 read at expanded index v. The
   // terms are formed in parallel (one lane per neighbour) and summed by lane
@@ -197,8 +225,16 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, const int
       }
     }
     ni.nb = cnt;
-    ni.delta0 = P.trueMismatch ? delta0 : 0.f;
+    ni.delta0 = (P.trueMismatch && !P.cross) ? delta0 : 0.f;
+    // the count layer is a sum of neighbour weights (0 or >= the smallest
+    // one): below half of it the FFT value is noise around zero
+    ni.cmin = (P.cross && wmin < INFINITY) ? 0.5f * wmin : 0.f;
     info[q] = ni;
+    if (P.cross) {
+      ni.nbStart = (nN + q) * P.n;
+      ni.delta0 = 0.f;
+      info[nN + q] = ni;
+    }
   }
 }
 
@@ -301,6 +337,11 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   }
   for (int a = 0; a < P.V; ++a) P.act[a] = dev->active[a];
   for (int q = 0; q < t->nSub * t->nSub; ++q) P.alpha[q] = dev->alpha[q];
+  P.cross = dev->cross ? 1 : 0;
+  if (dev->cross) {
+    if (R.lp) return fail(QSMI_ERR_UNSUPPORTED, "learned proposal: TI with NaN unsupported");
+    for (int q = 0; q < t->nSub * t->nSub; ++q) P.alphaX[q] = dev->alphaX[q];
+  }
   P.circular = dev->circular;
   P.trueMismatch = dev->trueMismatch;
   P.scale = dev->trueMismatch ? (float)(1.0 / (double)t->N) : 1.f;
@@ -312,14 +353,18 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   for (size_t w = 0; w + 1 < R.waveStart.size(); ++w) maxWave = std::max(maxWave, R.waveStart[w + 1] - R.waveStart[w]);
   if (!dev->wb) dev->wb = new WaveBufs();
   WaveBufs* b = dev->wb;
-  int rc = grow_wave_bufs(dev, b, std::max(maxWave, 1), std::max(maxWave, 1) * R.n, K);
+  // the cross-measurement form scans two layers per node (nodes q, nN + q)
+  const int layers = dev->cross ? 2 : 1;
+  int rc = grow_wave_bufs(dev, b, layers * std::max(maxWave, 1), layers * std::max(maxWave, 1) * R.n, K);
   if (rc) return rc;
   if (t->fast1k && (rc = ensure_tap_tables(dev, std::max(maxWave, 1)))) return rc;
   if (R.lp && (rc = lp_prepare(R.lp, t, std::max(maxWave, 1), K, R.lpShortlist, dev->active.data(),
                                (int)dev->active.size(), dev->circular)))
     return rc;
   ScanGeom g = geom_of(dev);
-  const int rb = row_blocks(t);
+  // rows stage: the cross form puts one row (both layers) in each FFT
+  const int rpb = dev->cross ? (1 << t->lognv[0]) : rows_per_block(t);
+  const int rb = dev->cross ? (int)(((long long)t->s[1] * t->s[2] + rpb - 1) / rpb) : row_blocks(t);
   const bool savedTiming = dev->timing;
   dev->timing = false;  // the per-call events of launch_spectrum are not used here
   const size_t nWaves = R.waveStart.size() > 0 ? R.waveStart.size() - 1 : 0;
@@ -343,8 +388,9 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
     if (nN <= 0) continue;
     hipEvent_t* ev = timing ? &dev->evPool[w * 5] : nullptr;
     if (ev) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL(k_gather, dim3(nN), dim3(64), gatherLds, st, P, w0, R.dOrder, R.dPathCells, R.dPathSeeds, R.dNbIdx,
-                       R.dNbCnt, R.dOffs, R.dKernel, R.dCatvals, R.dDi, R.dIndex, R.dTi, b->dInfo, b->dOff, b->dVal);
+    hipLaunchKernelGGL(k_gather, dim3(nN), dim3(64), gatherLds, st, P, w0, nN, R.dOrder, R.dPathCells, R.dPathSeeds,
+                       R.dNbIdx, R.dNbCnt, R.dOffs, R.dKernel, R.dCatvals, R.dDi, R.dIndex, R.dTi, b->dInfo, b->dOff,
+                       b->dVal, dev->dCentreOk, dev->dComplete, dev->nComplete);
     if (ev) (void)hipEventRecord(ev[1], st);
     const bool useLp = R.lp && (R.waveLp.empty() || R.waveLp[w]);
     if (R.lp && !useLp) R.lpSkipped += nN;
@@ -361,17 +407,17 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
       if ((rc = launch_spectrum(dev, g, nN, infoEx, b->dOff, b->dVal, b->dC, exCount))) return rc;
       if (ev) (void)hipEventRecord(ev[2], st);
       launch_rows(dev, g, infoEx, b->dC, nN, rb, K, b->dCand, nullptr, exCount);
-      lp_combine(R.lp, nN, compact ? b->dInfo : infoEx, b->dCand, compact, rb, rows_per_block(t), K, R.lpAudit, st);
+      lp_combine(R.lp, nN, compact ? b->dInfo : infoEx, b->dCand, compact, rb, rpb, K, R.lpAudit, st);
       if (ev) (void)hipEventRecord(ev[3], st);
       launch_merge_draw(g, b->dInfo, lp_cand(R.lp), nN, 1, 0, K, R.k, b->dOutVal, b->dOutPos, nullptr, R.dDi, R.dTi,
                         R.nVar, R.dIndex, st);
     } else {
-      if ((rc = launch_spectrum(dev, g, nN, b->dInfo, b->dOff, b->dVal, b->dC))) return rc;
+      if ((rc = launch_spectrum(dev, g, layers * nN, b->dInfo, b->dOff, b->dVal, b->dC))) return rc;
       if (ev) (void)hipEventRecord(ev[2], st);
-      launch_rows(dev, g, b->dInfo, b->dC, nN, rb, K, b->dCand, nullptr);
+      launch_rows(dev, g, b->dInfo, b->dC, nN, rb, K, b->dCand, nullptr, nullptr, dev->cross ? nN : 0);
       if (ev) (void)hipEventRecord(ev[3], st);
-      launch_merge_draw(g, b->dInfo, b->dCand, nN, rb, rows_per_block(t), K, R.k, b->dOutVal, b->dOutPos, nullptr,
-                        R.dDi, R.dTi, R.nVar, R.dIndex, st);
+      launch_merge_draw(g, b->dInfo, b->dCand, nN, rb, rpb, K, R.k, b->dOutVal, b->dOutPos, nullptr, R.dDi, R.dTi,
+                        R.nVar, R.dIndex, st);
     }
     if (R.recCap > 0)
       lp_record(nN, w0, R.dOrder, b->dInfo, b->dOff, R.dDi, R.nVar, R.sd[0], b->dOutPos, K, R.recCap, R.dRecOff,

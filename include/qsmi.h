@@ -159,7 +159,8 @@ int qsmi_set_stage_timing(qsmi_device* dev, int enable);
 int qsmi_categories(size_t cells, int nVar, const float* data, int var, float* out, int maxOut, int* count);
 
 /* Coefficient matrix and conversions of DataImage::generateCoefMatrix4Xcorr
- * (include/DataImage.hpp:626-710), no cross-measurement layer.
+ * (include/DataImage.hpp:626-710), no cross-measurement layer (see
+ * qsmi_coef_matrix_x).
  * types: nVar (0 continuous, 1 categorical); ncat: classes per categorical
  * variable. Outputs (capacity 16 sub-variables): alpha [A + B*nSub],
  * convTypes[B] (0 = P0 = w, 1 = P1 = w*x), subVarOf[B] (one-hot expanded
@@ -173,6 +174,17 @@ int qsmi_coef_matrix(int nVar, const int* types, const int* ncat, float* alpha, 
  * back to back. */
 int qsmi_ti_subimages(int ndim, const unsigned* srcSize, int nVar, const int* types, const float* ti,
                       const int* ncat, const float* catvals, float* out);
+
+/* The cross-measurement form of the two set-up steps above, used when the
+ * TI holds NaN (DataImage::generateCoefMatrix4Xcorr with forXMeasurement and
+ * convertInput4Xcorr with needCrossMeasurement, include/DataImage.hpp:
+ * 538-599, 626-710; continuous variables only): per variable the images
+ * [y^2, y, m] (m = 1 where y is a number, NaN -> 0) and the conversions
+ * [P0, P1, P2] (convTypes 0, 1, 2 = w, w x, w x^2); alpha is the mismatch
+ * layer diag(-1, +2, -1), alphaX the count layer (m against P0). */
+int qsmi_coef_matrix_x(int nVar, const int* types, float* alpha, float* alphaX, int* convTypes, int* subVarOf,
+                       int* nSub);
+int qsmi_ti_subimages_x(int ndim, const unsigned* srcSize, int nVar, const int* types, const float* ti, float* out);
 
 /* Sorted neighbour offset list of src/qs.cpp:1055-1137: every offset of the
  * kernel box (half-width (size+1)/2-1), NaN-kernel offsets removed
@@ -188,17 +200,22 @@ int qsmi_path(size_t cells, const unsigned char* informed, unsigned seed, unsign
               unsigned* beginPath);
 
 /* In-memory QuickSampling run, the counterpart of g2s::qs::runInMemory
- * (include/qsCore.hpp:17-58, src/qsCore.cpp:86-170) for one TI without NaN:
+ * (include/qsCore.hpp:17-58, src/qsCore.cpp:86-170) for one TI:
  * qs.cpp's set-up, the simulation() path loop (include/simulation.hpp:43-710)
  * with the scan on the GPU, path nodes packed into waves of independent
- * nodes (serial-equivalent result). */
+ * nodes (serial-equivalent result). A TI with NaN (continuous variables)
+ * runs in the cross-measurement form: the map is -|E / C^4| with E the
+ * mismatch over the TI cells that hold values and C their weighted count
+ * times the centre cell's completeness (CPUThreadDevice::maskLayerWithVariable
+ * and compensateMissingData, src/CPUThreadDevice.cpp:500-524,
+ * src/CPUThreadAcceleratorDevice.cpp:414-425). */
 typedef struct {
   int gpu;
   int ndim;
   unsigned tiSize[3];
   int nVar;
   const int* varTypes;     /* nVar: 0 continuous, 1 categorical */
-  const float* ti;         /* prod(tiSize) x nVar, no NaN */
+  const float* ti;         /* prod(tiSize) x nVar (NaN: no value, continuous variables only) */
   unsigned simSize[3];
   const float* di;         /* prod(simSize) x nVar; all-NaN cells are simulated,
                               fully informed cells are hard data */

@@ -32,7 +32,7 @@ def default_kernel(ti_dims, nexp):
 
 
 def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatch_frac=0.01, nodes=None,
-             weights=None, wave_size=64, threads=8):
+             weights=None, wave_size=64, threads=8, cross=False):
     """nodes: path-relative node indices to check (None: every node);
     weights: one kernel weight per variable (None: the default uniform 1)."""
     types = list(types)
@@ -67,14 +67,16 @@ def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatc
     gp, gs, gb = g2s_amd.path(informed, seed)
     assert np.array_equal(p, gp) and np.array_equal(s, gs) and b == gb
     orc = oracle.Sim(ti, ti_dims, types, ncat, catvals, sim_dims, kd, kern, offs, n, k, p, s, b, mode=0,
-                     quantum=quantum)
+                     quantum=quantum, cross=cross)
     K = int(np.ceil(k))
     simf = sim.reshape(-1, nvar)
     # hard data kept
     hard = informed.astype(bool)
     assert np.array_equal(simf[hard], di0f[hard])
     assert not np.isnan(simf).any()
-    tif = ti.reshape(-1, nvar)
+    tif = ti.reshape(-1, nvar).copy()
+    if cross:  # cells with any NaN variable lose all of them (src/qs.cpp:1447-1466)
+        tif[np.isnan(tif).any(axis=1)] = np.nan
     mismatches = 0
     worst_gap = 0.0
     check = list(range(len(p) - b)) if nodes is None else [int(i) for i in nodes]
@@ -160,3 +162,33 @@ def test_sim_1d():
     di = np.full(200, np.nan, np.float32)
     _log("continuous 1-D")
     lockstep(ti, dims, di, sdims, [0], 10, 1.5, 11, 0.0)
+
+
+def test_sim_nan_ti_continuous_2d():
+    """A TI with NaN holes: the cross-measurement form (-|E/C^4|)."""
+    g = synth.grf((48, 56), 4.0, seed=27)
+    ti = np.exp(g).astype(np.float32).reshape(-1)
+    rng = np.random.default_rng(27)
+    ti[rng.random(ti.size) < 0.1] = np.nan
+    ti[(np.arange(ti.size) % 56 > 40) & (np.arange(ti.size) // 56 < 10)] = np.nan  # a NaN block
+    sdims = (40, 36)
+    di = np.full(int(np.prod(sdims)), np.nan, np.float32)
+    _log("continuous 2-D, TI with NaN")
+    sim, idx, st = lockstep(ti, (56, 48), di, sdims, [0], 20, 1.5, 7, 0.0, cross=True)
+    assert not np.isnan(ti[idx]).any()
+
+
+def test_sim_nan_ti_two_continuous_hard_data():
+    a = synth.grf((40, 44), 4.0, seed=28)
+    b_ = synth.grf((40, 44), 6.0, seed=29)
+    ti, dims = synth.to_g2s(a, np.exp(b_))
+    rng = np.random.default_rng(28)
+    ti[rng.random(ti.shape[0]) < 0.08, 0] = np.nan
+    ti[rng.random(ti.shape[0]) < 0.05, 1] = np.nan
+    sdims = (30, 32)
+    di = np.full((int(np.prod(sdims)), 2), np.nan, np.float32)
+    tiok = ti[~np.isnan(ti).any(axis=1)]
+    pick = rng.choice(di.shape[0], size=di.shape[0] // 10, replace=False)
+    di[pick] = tiok[rng.integers(0, tiok.shape[0], size=pick.size)]
+    _log("two continuous variables, TI with NaN, hard data")
+    lockstep(ti.reshape(-1), dims, di.reshape(-1), sdims, [0, 0], 12, 1.5, 4, 0.0, cross=True)

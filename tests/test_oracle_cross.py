@@ -141,3 +141,32 @@ def test_serial_realisation_no_nan():
     assert not np.isnan(di).any()
     assert not np.isnan(ti[idx]).any()
     assert np.array_equal(di, ti[idx])
+
+
+def test_product_setup_matches_oracle():
+    """qsmi_coef_matrix_x / qsmi_ti_subimages_x (the library's set-up, CPU)
+    against the oracle's pinned restatement."""
+    import ctypes as C
+
+    from g2s_amd._lib import lib
+
+    for nv in (1, 2, 3):
+        a, ax, ct = _coef_x(nv)
+        pa = np.zeros(256, np.float32)
+        pax = np.zeros(256, np.float32)
+        pct = np.zeros(16, np.int32)
+        pso = np.zeros(16, np.int32)
+        n = C.c_int(0)
+        assert lib().qsmi_coef_matrix_x(nv, np.zeros(nv, np.int32), pa, pax, pct, pso, C.byref(n)) == 0
+        assert n.value == 3 * nv
+        assert np.array_equal(pa[:n.value ** 2], a) and np.array_equal(pax[:n.value ** 2], ax)
+        assert np.array_equal(pct[:n.value], ct)
+    rng = np.random.default_rng(12)
+    for dims, nv in (((7, 5), 1), ((4, 3, 5), 2)):
+        data = rng.standard_normal(int(np.prod(dims)) * nv).astype(np.float32)
+        data[rng.random(data.size) < 0.25] = np.nan
+        out = np.zeros(3 * nv * int(np.prod(dims)), np.float32)
+        assert lib().qsmi_ti_subimages_x(len(dims), oracle.u32(dims), nv, np.zeros(nv, np.int32), data, out) == 0
+        ref = np.zeros_like(out)
+        L.orc_ti_subimages_x(len(dims), oracle.u32(dims), oracle.u32(dims), nv, np.zeros(nv, np.int32), data, 0, ref)
+        assert np.array_equal(out, ref)
