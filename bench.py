@@ -51,10 +51,16 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--learned", type=int, default=1,
                     help="also measure the learned-proposal form (1) or not (0)")
-    ap.add_argument("--lp-train-sets", type=int, default=4, help="GRF TIs whose plain QS runs give training pairs")
-    ap.add_argument("--lp-record", type=int, default=500, help="simulation extent of each recording run")
+    ap.add_argument("--lp-train-sets", type=int, default=8,
+                    help="GRF TIs (seeds 200..) whose plain QS runs give training pairs")
+    ap.add_argument("--lp-record", type=int, default=500,
+                    help="simulation extent of each recording run (8 x 500^2 = 2M pairs)")
     ap.add_argument("--lp-steps", type=int, default=1000, help="Adam steps per TI per epoch")
-    ap.add_argument("--lp-epochs", type=int, default=2)
+    ap.add_argument("--lp-epochs", type=int, default=4)
+    ap.add_argument("--lp-min-cover", type=float, default=0.35,
+                    help="exact scan for nodes with a smaller fraction of neighbours on the key stencil")
+    ap.add_argument("--lp-audit", type=int, default=1,
+                    help="1: audited 300x300 learned realisation for the recall figures (0: skip, for profiling)")
     return ap.parse_args()
 
 
@@ -84,20 +90,22 @@ def learned_form(cfg, args, exact_ms):
     losses = []
     for e in range(args.lp_epochs):
         for i, (ti_s, dims_s, pairs) in enumerate(sets):
-            losses.append(net.train(ti_s, dims_s, pairs, steps=args.lp_steps, lr=1e-3 if e == 0 else 3e-4,
-                                    seed=100 * e + i))
+            losses.append(net.train(ti_s, dims_s, pairs, steps=args.lp_steps,
+                                    lr=(1e-3, 3e-4, 1e-4, 5e-5)[min(e, 3)], seed=100 * e + i))
     t_train = time.time() - t0
     print(f"[bench] learned: {len(sets)} x {args.lp_record}^2 pair sets in {t_rec:.1f}s, training {t_train:.1f}s, "
           f"loss {losses[0][0]:.3f} -> {losses[-1][-1]:.3f}", file=sys.stderr, flush=True)
     ti = np.ascontiguousarray(cfg["ti"][:, 0])
     dims, sdims = cfg["ti_dims"], cfg["sim_dims"]
     net.build_keys(ti, dims)
-    kw = dict(lp=net, lp_shortlist=4096, lp_fallback_every=64, lp_fallback_mse=0.5)
+    kw = dict(lp=net, lp_shortlist=4096, lp_fallback_every=64, lp_fallback_mse=0.5, lp_min_cover=args.lp_min_cover)
     di = np.full(int(np.prod(sdims)), np.nan, np.float32)
     _, _, st = g2s_amd.qs(ti, dims, di, sdims, cfg["types"], cfg["n"], cfg["k"], seed=cfg["seed"], **kw)
     a = 300
-    _, _, au = g2s_amd.qs(ti, dims, np.full(a * a, np.nan, np.float32), (a, a), cfg["types"], cfg["n"], cfg["k"],
-                          seed=cfg["seed"], lp_audit=True, **kw)
+    au = dict(lp_audit_nodes=0, lp_audit_hits=0, lp_audit_best_hits=0)
+    if args.lp_audit:
+        _, _, au = g2s_amd.qs(ti, dims, np.full(a * a, np.nan, np.float32), (a, a), cfg["types"], cfg["n"], cfg["k"],
+                              seed=cfg["seed"], lp_audit=True, **kw)
     n_a = max(1, au["lp_audit_nodes"])
     nodes = st["nodes"]
     out = dict(value=nodes / (st["path_loop_ms"] / 1e3), unit="nodes/s", ms_per_step=st["path_loop_ms"],
@@ -109,8 +117,8 @@ def learned_form(cfg, args, exact_ms):
                recall_audit_sample=f"{a}x{a} realisation on the same TI, {au['lp_audit_nodes']} proposed nodes",
                training=dict(pairs=int(sum(len(p[2]) for _, _, p in sets)), steps=args.lp_steps * args.lp_epochs *
                              len(sets), seconds=t_rec + t_train, final_loss=float(losses[-1][-1])),
-               rules="exact scan for every 64th node, nodes with < 50% of neighbours on the key stencil, "
-                     "best proposed mismatch > 0.5 sigma^2 per neighbour, shortlist overflow")
+               rules=f"exact scan for every 64th node, nodes with < {args.lp_min_cover:.0%} of neighbours on the key "
+                     "stencil, best proposed mismatch > 0.5 sigma^2 per neighbour, shortlist overflow")
     net.close()
     return out
 
