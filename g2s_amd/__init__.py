@@ -31,6 +31,7 @@ __all__ = [
     "TrainingImage",
     "AcceleratorDevice",
     "qs",
+    "calibration",
 ]
 
 
@@ -213,6 +214,11 @@ class AcceleratorDevice:
     def set_value_in_error_array(self, index, value):
         check(lib().qsmi_set_value_in_error_array(self.h, int(index), float(value)))
 
+    def set_value_in_error_array_with_radius(self, index, value, radius):
+        """AcceleratorDevice::setValueInErrorArrayWithRadius: the Euclidean
+        disk of ``radius`` around the position of ``index``."""
+        check(lib().qsmi_set_value_in_error_array_with_radius(self.h, int(index), float(value), float(radius)))
+
     def synchronize(self):
         check(lib().qsmi_synchronize(self.h))
 
@@ -244,6 +250,21 @@ class AcceleratorDevice:
         ch = np.zeros(n, np.uint32)
         check(lib().qsmi_scan_wave(self.h, n, starts, offs, vals, d0, sd, float(k), err, pos, ch.ctypes.data))
         return err.reshape(n, K), pos.reshape(n, K), ch
+
+    def distribution_wave(self, nodes, k, verbatim_pos, verbatim_radius):
+        """QuantileSamplingModule::distribution for a wave of nodes on this TI:
+        the ceil(k) best positions, best first, with no candidate within
+        ``verbatim_radius[i]`` of ``verbatim_pos[i]`` (UINT32_MAX: none).
+
+        Returns (errors (n,K), positions (n,K))."""
+        starts, offs, vals, d0, sd = self._pack(nodes)
+        n = len(nodes)
+        K = int(np.ceil(k))
+        err = np.zeros(n * K, np.float32)
+        pos = np.zeros(n * K, np.uint32)
+        check(lib().qsmi_distribution_wave(self.h, n, starts, offs, vals, d0, sd, float(k), _u32(verbatim_pos),
+                                           _f32(verbatim_radius), err, pos))
+        return err.reshape(n, K), pos.reshape(n, K)
 
     def wave(self, nodes, k):
         return Wave(self, nodes, k)
@@ -352,3 +373,69 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
                      lp_shortlist_sum=rs.lpShortlistSum, lp_audit_nodes=rs.lpAuditNodes,
                      lp_audit_hits=rs.lpAuditHits, lp_audit_best_hits=rs.lpAuditBestHits)
     return sim, idx, stats
+
+
+# auto_qs's defaults (src/auto_qs.cpp:115-121, 642-666)
+DEFAULT_DENSITIES = (0.0078, 0.0156, 0.0312, 0.0625, 0.1250, 0.2500, 0.5000)
+
+
+def default_neighbour_configs(max_nb=(80,)):
+    """auto_qs's list of neighbour counts for ``-mnn`` values ``max_nb``:
+    configurations i = 1 .. max_nb[0]-1, entry j = round(i*max_nb[j]/max_nb[0])
+    with the integer division first, as src/auto_qs.cpp:654-664 computes it."""
+    max_nb = [int(m) for m in max_nb]
+    return np.array([[(i * m) // max_nb[0] for m in max_nb] for i in range(1, max_nb[0])], np.uint32)
+
+
+def calibration(tis, ti_dims, var_types, kernels, kernel_dims, nb_configs=None, densities=DEFAULT_DENSITIES,
+                max_k=1, power=2.0, max_iter=25000, min_iter=1000, max_seconds=0.0, noise=0.0, seed=0, gpu=0,
+                wave_size=0, log_cap=0):
+    """auto_qs calibration (src/auto_qs.cpp, include/calibration.hpp:30-253)
+    with the scans on the GPU.
+
+    tis: list of arrays prod(ti_dims) x nVar (one shape for all); kernels:
+    list of arrays prod(kernel_dims) x nVar (one shape for all, kernel 0's
+    extent sets the offsets). nb_configs: (nConfigs, width) neighbour counts
+    (default: :func:`default_neighbour_configs`). Returns a dict with mean,
+    dev, samples shaped (nDensity, nKernels, nConfigs, max_k) — auto_qs's
+    im_1 / im_2 / im_3 images — and the run's statistics; with log_cap > 0
+    also the first log_cap samples (setting, (TI, cell), max_k x (TI, pos))."""
+    D = len(ti_dims)
+    tis = [_f32(t) for t in tis]
+    nvar = len(var_types)
+    kernels = [_f32(k) for k in kernels]
+    if nb_configs is None:
+        nb_configs = default_neighbour_configs()
+    nb = np.ascontiguousarray(np.atleast_2d(np.asarray(nb_configs, np.uint32)))
+    dens = _f32(densities)
+    tsz = np.ones((len(tis), 3), np.uint32)
+    tsz[:, :D] = ti_dims
+    ksz = np.ones((len(kernels), 3), np.uint32)
+    ksz[:, :D] = kernel_dims
+    tptr = (C.c_void_p * len(tis))(*[t.ctypes.data for t in tis])
+    kptr = (C.c_void_p * len(kernels))(*[k.ctypes.data for k in kernels])
+    types = _i32(var_types)
+    n_set = nb.shape[0] * len(kernels) * dens.size
+    mean = np.zeros(n_set * max_k, np.float32)
+    dev = np.zeros(n_set * max_k, np.float32)
+    cnt = np.zeros(n_set * max_k, np.uint32)
+    lset = np.zeros(max(log_cap, 1), np.int32)
+    lcell = np.zeros((max(log_cap, 1), 2), np.uint32)
+    lpos = np.zeros((max(log_cap, 1), max_k, 2), np.uint32)
+    rq = _lib.CalibRequest(gpu=gpu, ndim=D, nTI=len(tis), tiSize=tsz.ctypes.data, ti=C.cast(tptr, C.c_void_p),
+                           nVar=nvar, varTypes=types.ctypes.data, nKernels=len(kernels), kernelSize=ksz.ctypes.data,
+                           kernels=C.cast(kptr, C.c_void_p), nConfigs=nb.shape[0], nbWidth=nb.shape[1],
+                           nbList=nb.ctypes.data, nDensity=dens.size, density=dens.ctypes.data, maxK=max_k,
+                           power=power, maxIter=max_iter, minIter=min_iter, maxSeconds=max_seconds, noise=noise,
+                           seed=seed, waveSize=wave_size, logCap=log_cap, logSetting=lset.ctypes.data,
+                           logCell=lcell.ctypes.data, logPos=lpos.ctypes.data)
+    rs = _lib.CalibResult(meanError=mean.ctypes.data, devError=dev.ctypes.data, samples=cnt.ctypes.data)
+    check(lib().qsmi_calibration(C.byref(rq), C.byref(rs)))
+    shape = (dens.size, len(kernels), nb.shape[0], max_k)
+    out = dict(mean=mean.reshape(shape), dev=dev.reshape(shape), samples=cnt.reshape(shape),
+               distributions=rs.distributions, iterations=rs.iterations, waves=rs.waves, ms=rs.ms,
+               scan_ms=rs.scanMs)
+    if log_cap > 0:
+        n = min(log_cap, rs.distributions)
+        out.update(log_setting=lset[:n], log_cell=lcell[:n], log_pos=lpos[:n])
+    return out

@@ -39,6 +39,10 @@ SIGNATURES = {
     "qsmi_get_error_map": (C.c_int, [vp, f32p]),
     "qsmi_set_value_in_error_array": (C.c_int, [vp, C.c_uint, C.c_float]),
     "qsmi_scan_wave": (C.c_int, [vp, C.c_int, i32p, i32p, f32p, f32p, f32p, C.c_float, f32p, u32p, vp]),
+    "qsmi_distribution_wave": (C.c_int, [vp, C.c_int, i32p, i32p, f32p, f32p, f32p, C.c_float, u32p, f32p, f32p,
+                                         u32p]),
+    "qsmi_set_value_in_error_array_with_radius": (C.c_int, [vp, C.c_uint, C.c_float, C.c_float]),
+    "qsmi_calibration": (C.c_int, [vp, vp]),
     "qsmi_wave_create": (C.c_int, [vp, C.c_int, i32p, i32p, f32p, f32p, f32p, C.c_float, C.POINTER(vp)]),
     "qsmi_wave_run": (C.c_int, [vp]),
     "qsmi_wave_read": (C.c_int, [vp, vp, vp, vp]),
@@ -72,6 +76,51 @@ SIGNATURES = {
     "qsmi_bgrid_write_file": (C.c_int, [C.c_char_p, vp, C.c_size_t, C.c_int]),
     "qsmi_bgrid_read_file": (C.c_int, [C.c_char_p, vp, C.c_size_t, C.POINTER(C.c_size_t)]),
 }
+
+
+class CalibRequest(C.Structure):
+    _fields_ = [
+        ("gpu", C.c_int),
+        ("ndim", C.c_int),
+        ("nTI", C.c_int),
+        ("tiSize", C.c_void_p),
+        ("ti", C.c_void_p),
+        ("nVar", C.c_int),
+        ("varTypes", C.c_void_p),
+        ("nKernels", C.c_int),
+        ("kernelSize", C.c_void_p),
+        ("kernels", C.c_void_p),
+        ("nConfigs", C.c_int),
+        ("nbWidth", C.c_int),
+        ("nbList", C.c_void_p),
+        ("nDensity", C.c_int),
+        ("density", C.c_void_p),
+        ("maxK", C.c_int),
+        ("power", C.c_float),
+        ("maxIter", C.c_uint),
+        ("minIter", C.c_uint),
+        ("maxSeconds", C.c_double),
+        ("noise", C.c_float),
+        ("seed", C.c_uint),
+        ("waveSize", C.c_int),
+        ("logCap", C.c_int),
+        ("logSetting", C.c_void_p),
+        ("logCell", C.c_void_p),
+        ("logPos", C.c_void_p),
+    ]
+
+
+class CalibResult(C.Structure):
+    _fields_ = [
+        ("meanError", C.c_void_p),
+        ("devError", C.c_void_p),
+        ("samples", C.c_void_p),
+        ("distributions", C.c_longlong),
+        ("iterations", C.c_int),
+        ("waves", C.c_int),
+        ("ms", C.c_double),
+        ("scanMs", C.c_double),
+    ]
 
 
 class QsmiError(RuntimeError):

@@ -32,6 +32,7 @@ void launch_rows_select(const ScanGeom& g, const FftPlan& plan0, int lognv, cons
 void launch_select_map(const float* map, long long S, int K, float quantum, uint32_t seedbits, Cand* cand, int chunks,
                        hipStream_t st);
 void launch_fill(float* p, long long n, float v, hipStream_t st);
+void launch_fill_disk(float* map, const int* s, uint32_t o, float r2, float v, hipStream_t st);
 long long ct1k_stride(int h0);
 void launch_transpose_spectra(const float2* Y, float2* YT, int h0, int f1, int nA, hipStream_t st);
 void launch_pencil1k(const ScanGeom& g, const float2* YT, const float2* tw1k, int nNodes, int npb,
@@ -200,6 +201,10 @@ void prep_node(const qsmi_device* dev, int nb, const int* off, const float* u, f
   ni->seedbits = seed_from_unit_float(seed);
   ni->quantum = dev->quantum;
   ni->cell = 0;
+  ni->cmin = 0.f;
+  ni->exOn = 0;
+  ni->exPos = 0;
+  ni->exR2 = 0.f;
   for (int j = 0; j < nb; ++j) {
     for (int i = 0; i < D; ++i) offOut[j * D + i] = off[j * D + i];
     for (int a = 0; a < V; ++a) {
@@ -771,7 +776,8 @@ int grow_wave_bufs(qsmi_device* dev, WaveBufs* b, int nNodes, int total, int K) 
 
 // Host preparation into pinned staging + asynchronous upload.
 int stage_wave(qsmi_device* dev, WaveBufs* b, int nNodes, const int* nbStart, const int* offsets,
-               const float* values, const float* delta0, const float* seeds, float k) {
+               const float* values, const float* delta0, const float* seeds, float k,
+               const unsigned* exPos = nullptr, const float* exRadius = nullptr) {
   if (nNodes < 1) return fail(QSMI_ERR_ARG, "empty wave");
   if (!nbStart || !delta0 || !seeds) return fail(QSMI_ERR_ARG, "null argument");
   const int K = (int)std::ceil(k);
@@ -789,6 +795,15 @@ int stage_wave(qsmi_device* dev, WaveBufs* b, int nNodes, const int* nbStart, co
     const int s0 = nbStart[i], nb = nbStart[i + 1] - s0;
     prep_node(dev, nb, offsets + (size_t)s0 * D, values + (size_t)s0 * nSub, delta0[i], seeds[i], s0, &b->hInfo[i],
               b->hOff + (size_t)s0 * D, b->hVal + (size_t)s0 * V, nullptr, nullptr);
+    if (exPos && exPos[i] != 0xFFFFFFFFu) {
+      if (exPos[i] >= (unsigned)t->S) return fail(QSMI_ERR_ARG, "verbatim position outside the TI");
+      // QuantileSamplingModule::determineDistribution: a radius below 1
+      // removes the position alone (include/quantileSamplingModule.hpp:566-571)
+      const float r = exRadius ? exRadius[i] : 0.f;
+      b->hInfo[i].exOn = 1;
+      b->hInfo[i].exPos = exPos[i];
+      b->hInfo[i].exR2 = (r >= 1.f) ? r * r : 0.f;
+    }
   }
   hipStream_t st = dev->stream;
   HIPCHK(hipMemcpyAsync(b->dInfo, b->hInfo, nNodes * sizeof(NodeInfo), hipMemcpyHostToDevice, st));
@@ -883,6 +898,34 @@ int qsmi_wave_destroy(qsmi_wave* w) {
   }
   free_wave_bufs(&w->b);
   delete w;
+  return QSMI_OK;
+}
+
+int qsmi_distribution_wave(qsmi_device* dev, int nNodes, const int* nbStart, const int* offsets, const float* values,
+                           const float* delta0, const float* seeds, float k, const unsigned* verbatimPos,
+                           const float* verbatimRadius, float* outErrors, unsigned* outPositions) {
+  int rc = check_dev(dev);
+  if (rc) return rc;
+  if (!verbatimPos || !verbatimRadius) return fail(QSMI_ERR_ARG, "null verbatim arrays");
+  if (!dev->wb) dev->wb = new WaveBufs();
+  rc = stage_wave(dev, dev->wb, nNodes, nbStart, offsets, values, delta0, seeds, k, verbatimPos, verbatimRadius);
+  if (!rc) rc = run_wave(dev, dev->wb);
+  if (!rc) rc = read_wave(dev, dev->wb, outErrors, outPositions, nullptr);
+  return rc;
+}
+
+int qsmi_set_value_in_error_array_with_radius(qsmi_device* dev, unsigned index, float value, float radius) {
+  int rc = check_dev(dev);
+  if (rc) return rc;
+  if (!dev->mapValid) return fail(QSMI_ERR_ARG, "no map: call qsmi_candidate_for_pattern first");
+  if (index >= (unsigned)dev->ti->N) return fail(QSMI_ERR_ARG, "index out of range");
+  if (!(radius >= 0.f) || !std::isfinite(radius)) return fail(QSMI_ERR_ARG, "radius must be finite and >= 0");
+  unsigned pos;
+  qsmi_cvt_index_to_position(dev, index, &pos);
+  if (pos >= (unsigned)dev->ti->S) return fail(QSMI_ERR_ARG, "index maps outside the TI");
+  launch_fill_disk(dev->dMap, dev->ti->s, pos, radius * radius, value, dev->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(dev->stream));
   return QSMI_OK;
 }
 

@@ -435,6 +435,14 @@ __global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k(ScanGeom 
     key[2 * q] = (okc && va) ? key[2 * q] : -INFINITY;
     key[2 * q + 1] = (okc && vb) ? key[2 * q + 1] : -INFINITY;
   }
+  if (ni.exOn) {  // node-uniform: verbatim exclusion disk
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+      const int c0 = j + 100 * q;
+      if (excluded(ni, s0, g.s[1], c0, ca, 0)) key[2 * q] = -INFINITY;
+      if (excluded(ni, s0, g.s[1], c0, ca + 1, 0)) key[2 * q + 1] = -INFINITY;
+    }
+  }
   // pass 1: per-thread KT largest keys (min/max network)
   float m[KT];
 #pragma unroll

@@ -24,6 +24,8 @@
 // (src/CPUThreadDevice.cpp:349-490, src/CPUThreadAcceleratorDevice.cpp:251-357).
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <algorithm>
+#include <cmath>
 #include <stdint.h>
 
 #include "qsmi_kernels.hpp"
@@ -418,7 +420,8 @@ __global__ void __launch_bounds__(256) k_rows_select(ScanGeom g, FftPlan plan0, 
       // the smallest weight) are FFT noise around an exact zero
       const float E = zz.x * scale;
       const float Cc = (centreOk[pos] && zz.y * g.invN >= ni.cmin) ? zz.y * scale : 0.f;
-      const float val = (Cc == 0.f) ? -INFINITY : -fabsf(E / (Cc * Cc * Cc * Cc));
+      float val = (Cc == 0.f) ? -INFINITY : -fabsf(E / (Cc * Cc * Cc * Cc));
+      if (ni.exOn && excluded(ni, s0, s1, c0, c1, c2)) val = -INFINITY;
       if (map) map[(long long)node * ((long long)s0 * s1 * g.s[2]) + pos] = val;
       if (cand && val > -INFINITY) tk.push_value(val, 0.f, pos, ni.seedbits);
       continue;
@@ -429,10 +432,12 @@ __global__ void __launch_bounds__(256) k_rows_select(ScanGeom g, FftPlan plan0, 
       if (q >= nrv) continue;
       const int c1 = ni.lo[1] + (int)(q % n1v);
       const int c2 = ni.lo[2] + (int)(q / n1v);
-      const float val = (w == 0 ? zz.x : zz.y) * scale + d0;
+      float val = (w == 0 ? zz.x : zz.y) * scale + d0;
       const uint32_t pos = (uint32_t)(c0 + s0 * (c1 + s1 * c2));
+      const bool ex = ni.exOn && excluded(ni, s0, s1, c0, c1, c2);
+      if (ex) val = -INFINITY;
       if (map) map[(long long)node * ((long long)s0 * s1 * g.s[2]) + pos] = val;
-      if (cand) tk.push_value(val, ni.quantum, pos, ni.seedbits);
+      if (cand && !ex) tk.push_value(val, ni.quantum, pos, ni.seedbits);
     }
   }
   if (cand) {
@@ -513,6 +518,22 @@ __global__ void __launch_bounds__(256) k_merge_draw(ScanGeom g, const NodeInfo* 
 __global__ void k_fill(float* __restrict__ p, long long n, float v) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// Every position of the map within the disk sum_d (p_d - o_d)^2 <= r2 set to
+// v; the grid covers the disk's bounding box [bl, bl + bn) (clipped to the
+// TI), one thread per position.
+__global__ void k_fill_disk(float* __restrict__ map, int s0, int s1, uint32_t o, float r2, int bl0, int bl1, int bl2,
+                            int bn0, int bn1, long long total, float v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int c0 = bl0 + (int)(i % bn0);
+  const int c1 = bl1 + (int)((i / bn0) % bn1);
+  const int c2 = bl2 + (int)(i / ((long long)bn0 * bn1));
+  NodeInfo ni;
+  ni.exPos = o;
+  ni.exR2 = r2;
+  if (excluded(ni, s0, s1, c0, c1, c2)) map[(long long)c0 + (long long)s0 * (c1 + (long long)s1 * c2)] = v;
 }
 
 // explicit instantiations used by the host
@@ -640,6 +661,23 @@ void launch_merge_draw(const ScanGeom& g, const NodeInfo* info, const Cand* cand
     default: QSMI_MERGE(16); break;
   }
 #undef QSMI_MERGE
+}
+
+void launch_fill_disk(float* map, const int* s, uint32_t o, float r2, float v, hipStream_t st) {
+  const int oc[3] = {(int)(o % (uint32_t)s[0]), (int)((o / (uint32_t)s[0]) % (uint32_t)s[1]),
+                     (int)(o / ((uint32_t)s[0] * (uint32_t)s[1]))};
+  const int r = (int)std::ceil(std::sqrt((double)r2));
+  int bl[3], bn[3];
+  long long total = 1;
+  for (int d = 0; d < 3; ++d) {
+    bl[d] = std::max(0, oc[d] - r);
+    const int bh = std::min(s[d] - 1, oc[d] + r);
+    bn[d] = bh - bl[d] + 1;
+    if (bn[d] <= 0) return;
+    total *= bn[d];
+  }
+  hipLaunchKernelGGL(k_fill_disk, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, map, s[0], s[1], o, r2, bl[0],
+                     bl[1], bl[2], bn[0], bn[1], total, v);
 }
 
 void launch_fill(float* p, long long n, float v, hipStream_t st) {

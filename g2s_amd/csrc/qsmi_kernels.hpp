@@ -50,7 +50,19 @@ struct NodeInfo {
   float cmin;         // cross-measurement form: a count layer value (a sum of
                       // neighbour weights) below it is zero: half the
                       // smallest neighbour weight (0 otherwise)
+  int exOn;           // 1: no candidate within the disk below (verbatim exclusion)
+  uint32_t exPos;     // disk centre, TI position
+  float exR2;         // squared radius: positions p with sum_d (p_d - o_d)^2 <= exR2
 };
+
+// Verbatim exclusion test of TI position (c0, c1, c2) for a node.
+__device__ __forceinline__ bool excluded(const NodeInfo& ni, int s0, int s1, int c0, int c1, int c2) {
+  const int o0 = (int)(ni.exPos % (uint32_t)s0);
+  const int o1 = (int)((ni.exPos / (uint32_t)s0) % (uint32_t)s1);
+  const int o2 = (int)(ni.exPos / ((uint32_t)s0 * (uint32_t)s1));
+  const float d0 = (float)(c0 - o0), d1 = (float)(c1 - o1), d2 = (float)(c2 - o2);
+  return d0 * d0 + d1 * d1 + d2 * d2 <= ni.exR2;
+}
 
 // Candidate of the device selection rule.
 struct Cand {

@@ -76,6 +76,9 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
   ni.scale = P.scale;
   ni.nbStart = q * P.n;
   ni.cmin = 0.f;
+  ni.exOn = 0;
+  ni.exPos = 0;
+  ni.exR2 = 0.f;
   if (cnt <= 1) {
     // only the cell itself: draw from the marginal (simulation.hpp:586-606)
     if (lane == 0) {

@@ -118,6 +118,18 @@ int qsmi_get_error_map(qsmi_device* dev, float* out);
 /* setValueInErrorArray (include/AcceleratorDevice.hpp:65), index space. */
 int qsmi_set_value_in_error_array(qsmi_device* dev, unsigned index, float value);
 
+/* setValueInErrorArrayWithRadius (include/AcceleratorDevice.hpp:66), the
+ * verbatim exclusion of QuantileSamplingModule::distribution
+ * (include/quantileSamplingModule.hpp:566-571): every map position p with
+ * sum_d (p_d - o_d)^2 <= radius^2, o = cvtIndexToPosition(index), is set to
+ * value. The reference's devices disagree here (src/CPUThreadDevice.cpp:
+ * 310-341 steps each level by an FFT extent, not by the dimension's stride,
+ * src/CPUThreadAcceleratorDevice.cpp:385-407 compares an L1 distance with
+ * radius^2, src/NvidiaGPUAcceleratorDevice.cu:646-669 does nothing); this is
+ * the Euclidean disk their commented form states (src/CPUThreadDevice.cpp:
+ * 286-308), without wrap-around. */
+int qsmi_set_value_in_error_array_with_radius(qsmi_device* dev, unsigned index, float value, float radius);
+
 /* Path-wave batch: nNodes independent nodes in one launch sequence
  * (replaces nNodes x [candidateForPattern + searchKBiggest + the
  * QuantileSamplingModule::sample draw, include/quantileSamplingModule.hpp:90-126]).
@@ -129,6 +141,18 @@ int qsmi_set_value_in_error_array(qsmi_device* dev, unsigned index, float value)
 int qsmi_scan_wave(qsmi_device* dev, int nNodes, const int* nbStart, const int* offsets, const float* values,
                    const float* delta0, const float* seeds, float k, float* outErrors, unsigned* outPositions,
                    unsigned* outChosen);
+
+/* nNodes x QuantileSamplingModule::distribution for one TI
+ * (include/quantileSamplingModule.hpp:128-169 with determineDistribution
+ * :342-577, the k best of the map without the draw): the scan_wave inputs,
+ * plus per node a verbatim TI position (UINT32_MAX: none) whose disk of
+ * radius verbatimRadius (below 1: the position alone) holds no candidate,
+ * as qsmi_set_value_in_error_array_with_radius states it. Outputs
+ * outErrors/outPositions nNodes x ceil(k), best first (UINT32_MAX past the
+ * valid positions). Blocks until done. */
+int qsmi_distribution_wave(qsmi_device* dev, int nNodes, const int* nbStart, const int* offsets, const float* values,
+                           const float* delta0, const float* seeds, float k, const unsigned* verbatimPos,
+                           const float* verbatimRadius, float* outErrors, unsigned* outPositions);
 
 /* The same wave kept resident in HBM, for repeated timed launches:
  * upload once, run (asynchronous, stream-ordered), read results. */
@@ -334,6 +358,68 @@ int qsmi_lp_train(qsmi_lp* lp, const qsmi_lp_train_set* set, const qsmi_lp_train
 /* Keys of every position of a TI (kept in HBM for proposals) and a fixed
  * sample of 8192 of them for the per-node thresholds. */
 int qsmi_lp_build_keys(qsmi_lp* lp, const unsigned* tiSize, const float* ti, const int* canon);
+
+/* ======================================================================
+ * auto_qs calibration: the Monte Carlo search over (number of neighbours,
+ * kernel, density) of calibration() (include/calibration.hpp:30-253) with
+ * auto_qs's set-up (src/auto_qs.cpp:449-492 offsets sorted by distance,
+ * :561-582 categories, :640 the QuantileSamplingModule with k = maxK, the
+ * verbatim exclusion on and no kernel of its own), its distribution() calls
+ * batched into waves of the GPU scan (qsmi_distribution_wave).
+ *
+ * One iteration visits every setting s = config + nConfigs * (kernel +
+ * nKernels * density); a setting is skipped once its sample count exceeds
+ * minIter and its error minus 1/power deviations exceeds the best of its
+ * density plus 1/power of that best's deviations. Draws per setting, from one
+ * std::mt19937 seeded with `seed` and std::uniform_real_distribution<float>
+ * over [0, 1), in the reference's order: TI, cell, then per offset tried a
+ * density draw, the noise swaps, the distribution seed. Every setting's
+ * sample is drawn before the iteration's scans run, so the skip test of an
+ * iteration sees the state at its start; the reference, a parallel loop with
+ * a dynamic schedule and a clock seed, leaves this order open.
+ * ====================================================================== */
+typedef struct {
+  int gpu;
+  int ndim;                   /* 1..3 */
+  int nTI;                    /* >= 1, one device each */
+  const unsigned* tiSize;     /* nTI x 3 (unused dimensions 1) */
+  const float* const* ti;     /* nTI pointers: prod(tiSize) x nVar, no NaN */
+  int nVar;
+  const int* varTypes;        /* nVar: 0 continuous, 1 categorical */
+  int nKernels;               /* >= 1 */
+  const unsigned* kernelSize; /* nKernels x 3; kernel 0's extent sets the offsets */
+  const float* const* kernels;/* nKernels pointers: prod(kernelSize) x nVar */
+  int nConfigs;               /* numbers of neighbours to try */
+  int nbWidth;                /* entries per configuration (per variable, cycled) */
+  const unsigned* nbList;     /* nConfigs x nbWidth */
+  int nDensity;
+  const float* density;       /* nDensity probabilities of keeping an offset */
+  int maxK;                   /* -k of auto_qs: candidates scored per sample, <= 16 */
+  float power;                /* error metric |x - y|^power */
+  unsigned maxIter, minIter;
+  double maxSeconds;          /* <= 0: no time limit */
+  float noise;                /* offset swap level (0: none) */
+  unsigned seed;
+  int waveSize;               /* max samples per GPU wave, 0 -> 256, at most 1024 */
+  /* optional per-sample record (samples taken in order, up to logCap) */
+  int logCap;
+  int* logSetting;            /* logCap */
+  unsigned* logCell;          /* logCap x 2: TI index, cell */
+  unsigned* logPos;           /* logCap x maxK x 2: TI index, position (UINT32_MAX past the valid ones) */
+} qsmi_calib_request;
+
+typedef struct {
+  float* meanError;           /* out: settings x maxK, (mean |x - y|^power)^(1/power) */
+  float* devError;            /* out: settings x maxK */
+  unsigned* samples;          /* out: settings x maxK */
+  long long distributions;    /* out: samples scored */
+  int iterations;             /* out */
+  int waves;                  /* out */
+  double ms;                  /* out: wall time of the loop */
+  double scanMs;              /* out: of which in the GPU waves */
+} qsmi_calib_result;
+
+int qsmi_calibration(const qsmi_calib_request* req, qsmi_calib_result* res);
 
 /* ======================================================================
  * .bgrid files: G2S's serialized DataImage, the format its qs reads

@@ -110,6 +110,12 @@ def lib():
     _sig(L, "orc_sim_node", C.c_uint, [C.c_void_p, f32p, C.c_uint, C.c_int, f32p, u32p, C.POINTER(C.c_int)])
     _sig(L, "orc_sim_run", None, [C.c_void_p, f32p, u32p])
     _sig(L, "orc_sim_node_map", C.c_int, [C.c_void_p, f32p, C.c_uint, f64p, C.c_void_p])
+    _sig(L, "orc_calib_offsets", C.c_int, [C.c_int, u32p, i32p, C.c_int])
+    _sig(L, "orc_exclude_disk", None, [C.c_int, u32p, f32p, C.c_uint, C.c_float])
+    _sig(L, "orc_calibration", C.c_int, [C.c_int, C.c_int, u32p, C.c_void_p, C.c_int, i32p, C.c_int, u32p, C.c_void_p,
+                                         C.c_int, C.c_int, u32p, C.c_int, f32p, C.c_int, C.c_float, C.c_uint,
+                                         C.c_uint, C.c_float, C.c_uint, C.c_float, f32p, f32p, u32p, C.c_int, i32p,
+                                         u32p, u32p, C.POINTER(C.c_longlong)])
     _LIB = L
     return L
 
@@ -322,3 +328,50 @@ class Sim:
         idx = np.full(self.ncells, 0xFFFFFFFF, np.uint32)
         lib().orc_sim_run(self.h, di, idx)
         return di, idx
+
+
+def calib_offsets(kdims):
+    """Offsets in auto_qs's calibration order (src/auto_qs.cpp:449-492)."""
+    kd = u32(kdims)
+    D = len(kdims)
+    cap = int(np.prod([2 * ((k + 1) // 2) - 1 for k in kdims]))
+    out = np.zeros(cap * D, np.int32)
+    n = lib().orc_calib_offsets(D, kd, out, cap)
+    return out[:n * D].reshape(n, D)
+
+
+def calibration(tis, ti_dims, types, kernels, kernel_dims, nb_configs, densities, max_k, power, max_iter, min_iter,
+                noise, seed, quantum=0.0, log_cap=0):
+    """The calibration loop restated on the CPU (brute-force maps); arrays
+    shaped (nDensity, nKernels, nConfigs, max_k) as the product returns."""
+    D = len(ti_dims)
+    tis = [f32(t) for t in tis]
+    kernels = [f32(k) for k in kernels]
+    nb = np.ascontiguousarray(np.atleast_2d(np.asarray(nb_configs, np.uint32)))
+    dens = f32(densities)
+    tsz = np.ones((len(tis), 3), np.uint32)
+    tsz[:, :D] = ti_dims
+    ksz = np.ones((len(kernels), 3), np.uint32)
+    ksz[:, :D] = kernel_dims
+    tptr = (C.c_void_p * len(tis))(*[t.ctypes.data for t in tis])
+    kptr = (C.c_void_p * len(kernels))(*[k.ctypes.data for k in kernels])
+    n_set = nb.shape[0] * len(kernels) * dens.size
+    mean = np.zeros(n_set * max_k, np.float32)
+    dev = np.zeros(n_set * max_k, np.float32)
+    cnt = np.zeros(n_set * max_k, np.uint32)
+    lc = max(log_cap, 1)
+    lset = np.zeros(lc, np.int32)
+    lcell = np.zeros(lc * 2, np.uint32)
+    lpos = np.zeros(lc * max_k * 2, np.uint32)
+    nd = C.c_longlong(0)
+    lib().orc_calibration(D, len(tis), tsz.reshape(-1), C.cast(tptr, C.c_void_p), len(types), i32(types),
+                          len(kernels), ksz.reshape(-1), C.cast(kptr, C.c_void_p), nb.shape[0], nb.shape[1],
+                          nb.reshape(-1), dens.size, dens, max_k, power, max_iter, min_iter, noise, seed, quantum,
+                          mean, dev, cnt, log_cap, lset, lcell, lpos, C.byref(nd))
+    shape = (dens.size, len(kernels), nb.shape[0], max_k)
+    out = dict(mean=mean.reshape(shape), dev=dev.reshape(shape), samples=cnt.reshape(shape), distributions=nd.value)
+    if log_cap > 0:
+        n = min(log_cap, nd.value)
+        out.update(log_setting=lset[:n], log_cell=lcell.reshape(-1, 2)[:n],
+                   log_pos=lpos.reshape(-1, max_k, 2)[:n])
+    return out
