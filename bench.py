@@ -59,6 +59,8 @@ def parse():
     ap.add_argument("--lp-epochs", type=int, default=4)
     ap.add_argument("--lp-min-cover", type=float, default=0.35,
                     help="exact scan for nodes with a smaller fraction of neighbours on the key stencil")
+    ap.add_argument("--calib-iters", type=int, default=4,
+                    help="auto_qs calibration iterations on the same TI (0: skip the calibration leg)")
     ap.add_argument("--lp-audit", type=int, default=1,
                     help="1: audited 300x300 learned realisation for the recall figures (0: skip, for profiling)")
     return ap.parse_args()
@@ -161,6 +163,52 @@ def cpu_baseline(cfg, sim, threads, sample):
                         f"own mixed-radix CPU FFT (no FFTW on the host); spectra setup {setup:.1f}s untimed"))
 
 
+def calibration_leg(cfg, iters, cpu_sample):
+    """auto_qs calibration on the benchmark TI with auto_qs's defaults
+    (neighbour counts 1..79, the seven densities, maxK 1, power 2, minIter
+    1000) and a uniform 201x201 kernel: samples scored per second. The CPU
+    baseline is the oracle (FP64 brute-force maps, OpenMP) on a bounded
+    subset of the settings, one iteration."""
+    import g2s_amd
+
+    ti, dims = cfg["ti"], cfg["ti_dims"]
+    kd = (201, 201)
+    ker = np.ones(kd[0] * kd[1], np.float32)
+    nb = g2s_amd.default_neighbour_configs((80,))
+    g2s_amd.calibration([ti], dims, [0], [ker], [kd], nb_configs=nb[:4], max_iter=1, seed=4)  # warm
+    r = g2s_amd.calibration([ti], dims, [0], [ker], [kd], nb_configs=nb, max_iter=iters, seed=5, stage_timing=True)
+    # the dominant kernel is the exact scan's spectrum stage (k_pencil1k):
+    # the same algorithmic bytes per scanned sample as per path node
+    V, S, N = 2, (dims[0] // 2 + 1) * dims[1], dims[0] * dims[1]
+    a_node = V * S * 8 + 2 * N * 4
+    roof = None
+    if r["spectrum_ms"] > 0:
+        achieved = a_node * r["distributions"] / (r["spectrum_ms"] / 1e3) / 1e9
+        roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
+                    traffic=None, kernel="k_pencil1k", avg_launch_us=r["spectrum_ms"] * 1e3 / max(1, r["waves"]),
+                    samples_per_launch=r["distributions"] / max(1, r["waves"]), algorithmic_bytes_per_sample=a_node)
+    out = dict(value=r["distributions"] / (r["ms"] / 1e3), unit="samples/s", samples=r["distributions"],
+               iterations=r["iterations"], waves=r["waves"], ms=r["ms"], scan_ms=r["scan_ms"],
+               draw_ms=r["draw_ms"], setup_ms=r["setup_ms"], spectrum_ms=r["spectrum_ms"], rows_ms=r["rows_ms"],
+               roofline=roof,
+               config=(f"auto_qs calibration, TI {dims[0]}x{dims[1]} exp(GRF), uniform kernel {kd[0]}x{kd[1]}, "
+                       f"{nb.shape[0]} neighbour counts x 7 densities, maxK 1, power 2, {iters} iterations, "
+                       f"exclusion radius {(dims[0] * dims[1] * 0.005) ** 0.5:.1f}"))
+    if cpu_sample > 0:
+        import oracle
+
+        oracle.build()
+        sub = nb[[9, 39, 78]]
+        t0 = time.perf_counter()
+        c = oracle.calibration([ti], dims, [0], [ker], [kd], sub, g2s_amd.DEFAULT_DENSITIES, 1, 2.0, 1, 1000, 0.0, 5)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = dict(value=c["distributions"] / dt, unit="samples/s",
+                                   cores=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), kind="port",
+                                   sample=(f"{c['distributions']} samples: one iteration over neighbour counts "
+                                           f"10, 40, 79 x 7 densities, oracle FP64 brute-force maps (OpenMP)"))
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -224,6 +272,9 @@ def main():
                     nodes_per_launch=nodes_per_launch, algorithmic_bytes_per_node=a_node,
                     traffic_source=(PROFILE if per_node else None))
     learned = learned_form(cfg, args, ms) if args.learned else None
+    calib = calibration_leg(cfg, args.calib_iters, args.cpu_sample) if args.calib_iters > 0 else None
+    if calib:
+        print(f"[bench] calibration: {calib}", file=sys.stderr, flush=True)
     threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
     cpu = None
     if args.cpu_sample > 0:
@@ -248,6 +299,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "learned_proposal": learned,
+        "calibration": calib,
         "stats": st,
     }
     print(json.dumps(out))

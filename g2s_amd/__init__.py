@@ -389,7 +389,7 @@ def default_neighbour_configs(max_nb=(80,)):
 
 def calibration(tis, ti_dims, var_types, kernels, kernel_dims, nb_configs=None, densities=DEFAULT_DENSITIES,
                 max_k=1, power=2.0, max_iter=25000, min_iter=1000, max_seconds=0.0, noise=0.0, seed=0, gpu=0,
-                wave_size=0, log_cap=0):
+                wave_size=0, log_cap=0, stage_timing=False):
     """auto_qs calibration (src/auto_qs.cpp, include/calibration.hpp:30-253)
     with the scans on the GPU.
 
@@ -428,13 +428,14 @@ def calibration(tis, ti_dims, var_types, kernels, kernel_dims, nb_configs=None, 
                            nbList=nb.ctypes.data, nDensity=dens.size, density=dens.ctypes.data, maxK=max_k,
                            power=power, maxIter=max_iter, minIter=min_iter, maxSeconds=max_seconds, noise=noise,
                            seed=seed, waveSize=wave_size, logCap=log_cap, logSetting=lset.ctypes.data,
-                           logCell=lcell.ctypes.data, logPos=lpos.ctypes.data)
+                           logCell=lcell.ctypes.data, logPos=lpos.ctypes.data, stageTiming=int(stage_timing))
     rs = _lib.CalibResult(meanError=mean.ctypes.data, devError=dev.ctypes.data, samples=cnt.ctypes.data)
     check(lib().qsmi_calibration(C.byref(rq), C.byref(rs)))
     shape = (dens.size, len(kernels), nb.shape[0], max_k)
     out = dict(mean=mean.reshape(shape), dev=dev.reshape(shape), samples=cnt.reshape(shape),
                distributions=rs.distributions, iterations=rs.iterations, waves=rs.waves, ms=rs.ms,
-               scan_ms=rs.scanMs)
+               scan_ms=rs.scanMs, draw_ms=rs.drawMs, setup_ms=rs.setupMs, spectrum_ms=rs.stageMs[0],
+               rows_ms=rs.stageMs[2], merge_ms=rs.stageMs[3])
     if log_cap > 0:
         n = min(log_cap, rs.distributions)
         out.update(log_setting=lset[:n], log_cell=lcell[:n], log_pos=lpos[:n])

@@ -107,6 +107,7 @@ class CalibRequest(C.Structure):
         ("logSetting", C.c_void_p),
         ("logCell", C.c_void_p),
         ("logPos", C.c_void_p),
+        ("stageTiming", C.c_int),
     ]
 
 
@@ -120,6 +121,9 @@ class CalibResult(C.Structure):
         ("waves", C.c_int),
         ("ms", C.c_double),
         ("scanMs", C.c_double),
+        ("drawMs", C.c_double),
+        ("setupMs", C.c_double),
+        ("stageMs", C.c_double * 4),
     ]
 
 

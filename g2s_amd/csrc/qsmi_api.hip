@@ -776,8 +776,8 @@ int grow_wave_bufs(qsmi_device* dev, WaveBufs* b, int nNodes, int total, int K) 
 
 // Host preparation into pinned staging + asynchronous upload.
 int stage_wave(qsmi_device* dev, WaveBufs* b, int nNodes, const int* nbStart, const int* offsets,
-               const float* values, const float* delta0, const float* seeds, float k,
-               const unsigned* exPos = nullptr, const float* exRadius = nullptr) {
+               const float* values, const float* delta0, const float* seeds, float k, const unsigned* exPos,
+               const float* exRadius) {
   if (nNodes < 1) return fail(QSMI_ERR_ARG, "empty wave");
   if (!nbStart || !delta0 || !seeds) return fail(QSMI_ERR_ARG, "null argument");
   const int K = (int)std::ceil(k);

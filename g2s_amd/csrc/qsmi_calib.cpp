@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/qsmi.h"
+#include "qsmi_internal.hpp"
 
 extern "C" int qsmi_internal_fail(int code, const char* msg);
 
@@ -76,6 +77,15 @@ bool index_with_delta(int D, const unsigned* dims, unsigned position, const int*
   for (int i = D - 1; i >= 0; --i) fin = fin * dims[i] + v[i];
   *loc = fin;
   return ok;
+}
+
+// std::uniform_real_distribution<float>(0, 1) over std::mt19937 as
+// libstdc++ computes it (generate_canonical<float, 24> with one 32-bit
+// word: float(word) / 2^32, values rounding to 1 replaced by the largest
+// float below 1), without its per-call logarithms
+inline float u01(std::mt19937& g) {
+  const float r = (float)g() * (1.0f / 4294967296.0f);
+  return r >= 1.f ? 0.99999994f : r;
 }
 
 struct Sample {
@@ -214,177 +224,220 @@ extern "C" int qsmi_calibration(const qsmi_calib_request* rq, qsmi_calib_result*
   // to the power 1/D
   const float radius = (float)std::pow((double)(tis[0].cells) * 0.005, 1.f / (float)D);
   std::mt19937 gen(rq->seed);
-  std::uniform_real_distribution<float> U(0.f, 1.f);
   const int waveMax = rq->waveSize > 0 ? std::min(rq->waveSize, 1024) : 256;
   const double maxSec = rq->maxSeconds > 0 ? rq->maxSeconds : INFINITY;
   const float nbSigmas = 1.f / rq->power;
   long long logged = 0, dists = 0;
   int waves = 0, iter = 0;
-  double scanMs = 0.0;
+  double scanMs = 0.0, drawMs = 0.0;
   int rc = QSMI_OK;
   std::vector<Sample> smp;
   // per-wave host arrays
   std::vector<int> nbStart, offs;
-  std::vector<float> uvals, d0, seeds, exR, oErr;
-  std::vector<unsigned> exP, oPos;
+  std::vector<float> uvals, d0, seeds, exR, oErr, allErr;
+  std::vector<unsigned> exP, oPos, allPos;
   std::vector<float> ubuf;
+  std::vector<unsigned> kidx;
+  std::vector<char> kin;
+  std::vector<unsigned> perVar(nVar);
+  std::vector<float> raw;
+  double stage[4] = {0, 0, 0, 0};
+  for (auto& T : tis) {
+    if (!T.dev->wb) T.dev->wb = new WaveBufs();
+    if (rq->stageTiming) qsmi_set_stage_timing(T.dev, 1);
+  }
+  const auto t1 = clk::now();
+
+  // Draws of one setting's sample (include/calibration.hpp:101-203);
+  // appends to smp unless the setting is skipped or the draw fails.
+  size_t si = 0;
+  auto draw_one = [&]() {
+    const size_t s0 = si++;
+    const int cfg = (int)(s0 % nCfg);
+    const int den = (int)(s0 / ((size_t)nCfg * nKer));
+    // :101-107: skip a setting clearly worse than its density's best
+    const unsigned n0 = nSamp[s0 * K];
+    const float eb = (float)(cumErr[s0 * K] / n0);
+    const float db = (float)std::sqrt(cumSq[s0 * K] / n0 - (double)(eb * eb));
+    if (n0 > rq->minIter && (eb - nbSigmas * db) > (best[den] + nbSigmas * devBest[den])) return;
+    const float density = rq->density[den];
+    const unsigned* nbc = rq->nbList + (size_t)cfg * rq->nbWidth;
+    const int tiIndex = (int)std::floor(u01(gen) * (float)nTI);
+    const Tiv& T = tis[tiIndex];
+    // float arithmetic as the reference's U * dataSize() / nbVariable
+    const unsigned cell = (unsigned)std::floor(u01(gen) * (float)(T.cells * nVar) / (float)nVar);
+    // :115-154: offsets in order, each kept with probability `density`
+    // while inside the TI, until every variable has its count
+    Sample sm;
+    sm.setting = (int)s0;
+    sm.tiIndex = tiIndex;
+    sm.cell = cell;
+    std::fill(perVar.begin(), perVar.end(), 0u);
+    raw.clear();
+    size_t ps = 0;
+    while ((rq->nbWidth > 1 || sm.off.size() < nbc[0]) && ps < pp.size()) {
+      unsigned loc;
+      if (u01(gen) < density && index_with_delta(D, T.s, cell, pp[ps].data(), &loc)) {
+        unsigned cpt = 0;
+        for (int v = 0; v < nVar; ++v) {
+          if (perVar[v] < nbc[v % rq->nbWidth]) {
+            raw.push_back(T.data[(size_t)loc * nVar + v]);
+            cpt++;
+            perVar[v]++;
+          } else {
+            raw.push_back(std::nanf("0"));
+          }
+        }
+        sm.off.push_back(pp[ps]);
+        if (cpt == 0) break;
+      }
+      ps++;
+    }
+    if (ps >= pp.size()) return;
+    // :158-180: one value per expanded variable (a category's indicator)
+    const size_t nb = sm.off.size();
+    sm.val.resize(nb * nExp);
+    for (size_t j = 0; j < nb; ++j) {
+      int e = 0;
+      for (int v = 0; v < nVar; ++v) {
+        const float x = raw[j * nVar + v];
+        if (rq->varTypes[v] == 0) {
+          sm.val[j * nExp + e++] = x;
+        } else {
+          for (float c : catv[v]) sm.val[j * nExp + e++] = (x == c) ? 1.f : 0.f;
+        }
+      }
+    }
+    // :184-203: noise, offsets swapped between neighbours
+    if (nb > 1 && rq->noise > 0.f) {
+      float fraction = rq->noise / (float)nb / 2.f;
+      if (rq->noise < 1.0f) fraction = rq->noise / 2.f;
+      for (size_t j = 0; j < nb; ++j) {
+        if (u01(gen) < fraction) {
+          const size_t a = (size_t)std::floor(u01(gen) * (float)nb);
+          const size_t b = (size_t)std::floor(u01(gen) * (float)nb);
+          std::iter_swap(sm.off.begin() + a, sm.off.begin() + b);
+        }
+      }
+    }
+    if (nb > 1) {
+      sm.seed = u01(gen);
+      smp.push_back(std::move(sm));
+    }
+  };
+  // samples until `target` are drawn or the iteration's settings run out
+  auto draw_until = [&](size_t target) {
+    const auto td = clk::now();
+    while (smp.size() < target && si < settings) draw_one();
+    drawMs += std::chrono::duration<double, std::milli>(clk::now() - td).count();
+  };
+  // host conversion of samples [w0, w0 + nn) for TI t, staged and launched
+  auto launch = [&](size_t w0, int nn, int t) -> int {
+    nbStart.assign(nn + 1, 0);
+    offs.clear();
+    uvals.clear();
+    d0.assign(nn, 0.f);
+    seeds.assign(nn, 0.f);
+    exP.assign(nn, 0xFFFFFFFFu);
+    exR.assign(nn, 0.f);
+    for (int q = 0; q < nn; ++q) {
+      const Sample& sm = smp[w0 + q];
+      const Ker& kq = kers[(sm.setting / nCfg) % nKer];
+      const int nb = (int)sm.off.size();
+      nbStart[q + 1] = nbStart[q] + nb;
+      // include/quantileSamplingModule.hpp:368-457 with this sample's
+      // kernel: u = w, w x, w x^2 per sub-variable (0 outside the kernel or
+      // for NaN), delta0 = -sum w x^2 over the continuous variables at
+      // their original index
+      unsigned centre = 0;
+      for (int i = D - 1; i >= 0; --i) centre = centre * kq.dims[i] + kq.dims[i] / 2;
+      ubuf.assign((size_t)nb * nSub, 0.f);
+      kidx.resize(nb);
+      kin.resize(nb);
+      for (int j = 0; j < nb; ++j) {
+        kin[j] = index_with_delta(D, kq.dims, centre, sm.off[j].data(), &kidx[j]);
+        for (int i = 0; i < D; ++i) offs.push_back(sm.off[j][i]);
+        for (int B = 0; B < nSub; ++B) {
+          const int e = subVarOf[B];
+          const float x = sm.val[(size_t)j * nExp + e];
+          const float w = kq.w[(size_t)kidx[j] * nExp + (e % nExp)];
+          float r = 0.f;
+          if (kin[j] && !std::isnan(x)) r = convTypes[B] == 0 ? w * 1.f : convTypes[B] == 2 ? w * x * x : w * x;
+          ubuf[(size_t)j * nSub + B] = r;
+        }
+      }
+      uvals.insert(uvals.end(), ubuf.begin(), ubuf.end());
+      float delta0 = 0.f;
+      for (int v = 0; v < nVar; ++v) {
+        if (rq->varTypes[v] != 0) continue;
+        for (int j = 0; j < nb; ++j) {
+          const float x = sm.val[(size_t)j * nExp + v];
+          if (kin[j] && !std::isnan(x)) delta0 += -1.f * (kq.w[(size_t)kidx[j] * nExp + (v % nExp)] * x * x);
+        }
+      }
+      d0[q] = delta0;
+      seeds[q] = sm.seed;
+      if (sm.tiIndex == t) {
+        exP[q] = sm.cell;
+        exR[q] = radius;
+      }
+    }
+    qsmi_device* dv = tis[t].dev;
+    int r = qsmi::stage_wave(dv, dv->wb, nn, nbStart.data(), offs.data(), uvals.data(), d0.data(), seeds.data(),
+                             (float)K, exP.data(), exR.data());
+    if (!r) r = qsmi::run_wave(dv, dv->wb);
+    return r;
+  };
+  auto collect = [&](size_t w0, int nn, int t) -> int {
+    oErr.resize((size_t)nn * K);
+    oPos.resize((size_t)nn * K);
+    qsmi_device* dv = tis[t].dev;
+    int r = qsmi::read_wave(dv, dv->wb, oErr.data(), oPos.data(), nullptr);
+    if (r) return r;
+    if (rq->stageTiming)
+      for (int i = 0; i < 4; ++i) stage[i] += dv->stageMs[i];
+    allErr.resize(smp.size() * (size_t)nTI * K);
+    allPos.resize(smp.size() * (size_t)nTI * K);
+    for (int q = 0; q < nn; ++q)
+      for (int i = 0; i < K; ++i) {
+        allErr[((w0 + q) * nTI + t) * K + i] = oErr[(size_t)q * K + i];
+        allPos[((w0 + q) * nTI + t) * K + i] = oPos[(size_t)q * K + i];
+      }
+    waves++;
+    return QSMI_OK;
+  };
 
   for (unsigned it = 0; it < rq->maxIter && !rc; ++it) {
     smp.clear();
-    for (size_t si = 0; si < settings; ++si) {
-      const int cfg = (int)(si % nCfg);
-      const int ker = (int)((si / nCfg) % nKer);
-      const int den = (int)(si / ((size_t)nCfg * nKer));
-      // include/calibration.hpp:101-107: skip a setting clearly worse than
-      // its density's best
-      const unsigned n0 = nSamp[si * K];
-      const float eb = (float)(cumErr[si * K] / n0);
-      const float db = (float)std::sqrt(cumSq[si * K] / n0 - (double)(eb * eb));
-      if (n0 > rq->minIter && (eb - nbSigmas * db) > (best[den] + nbSigmas * devBest[den])) continue;
-      const float density = rq->density[den];
-      const unsigned* nbc = rq->nbList + (size_t)cfg * rq->nbWidth;
-      const int tiIndex = (int)std::floor(U(gen) * (float)nTI);
-      const Tiv& T = tis[tiIndex];
-      // float arithmetic as the reference's U * dataSize() / nbVariable
-      const unsigned cell = (unsigned)std::floor(U(gen) * (float)(T.cells * nVar) / (float)nVar);
-      // :115-154: offsets in order, each kept with probability `density`
-      // while inside the TI, until every variable has its count
-      Sample sm;
-      sm.setting = (int)si;
-      sm.tiIndex = tiIndex;
-      sm.cell = cell;
-      std::vector<unsigned> perVar(nVar, 0);
-      std::vector<std::vector<float>> raw;
-      size_t ps = 0;
-      while ((rq->nbWidth > 1 || sm.off.size() < nbc[0]) && ps < pp.size()) {
-        unsigned loc;
-        if (U(gen) < density && index_with_delta(D, T.s, cell, pp[ps].data(), &loc)) {
-          std::vector<float> x(nVar);
-          unsigned cpt = 0;
-          for (int v = 0; v < nVar; ++v) {
-            if (perVar[v] < nbc[v % rq->nbWidth]) {
-              x[v] = T.data[(size_t)loc * nVar + v];
-              cpt++;
-              perVar[v]++;
-            } else {
-              x[v] = std::nanf("0");
-            }
-          }
-          raw.push_back(x);
-          sm.off.push_back(pp[ps]);
-          if (cpt == 0) break;
-        }
-        ps++;
-      }
-      if (ps >= pp.size()) continue;
-      // :158-180: one value per expanded variable (a category's indicator)
-      sm.val.resize(raw.size() * nExp);
-      for (size_t j = 0; j < raw.size(); ++j) {
-        int e = 0;
-        for (int v = 0; v < nVar; ++v) {
-          if (rq->varTypes[v] == 0) {
-            sm.val[j * nExp + e++] = raw[j][v];
-          } else {
-            for (float c : catv[v]) sm.val[j * nExp + e++] = (raw[j][v] == c) ? 1.f : 0.f;
-          }
-        }
-      }
-      // :184-203: noise, offsets swapped between neighbours
-      if (sm.off.size() > 1 && rq->noise > 0.f) {
-        float fraction = rq->noise / (float)sm.off.size() / 2.f;
-        if (rq->noise < 1.0f) fraction = rq->noise / 2.f;
-        for (size_t j = 0; j < sm.off.size(); ++j) {
-          if (U(gen) < fraction) {
-            const size_t a = (size_t)std::floor(U(gen) * (float)sm.off.size());
-            const size_t b = (size_t)std::floor(U(gen) * (float)sm.off.size());
-            std::iter_swap(sm.off.begin() + a, sm.off.begin() + b);
-          }
-        }
-      }
-      if (sm.off.size() > 1) {
-        sm.seed = U(gen);
-        smp.push_back(std::move(sm));
-      }
-    }
-    // scan: every sample on every TI, in waves
-    const size_t ns = smp.size();
-    std::vector<float> allErr(ns * (size_t)nTI * K);
-    std::vector<unsigned> allPos(ns * (size_t)nTI * K);
+    si = 0;
+    // draws of the next wave's samples run on the host while the GPU scans
+    // the current wave (one TI); with several TIs each wave visits them in turn
+    draw_until((size_t)waveMax);
     const auto ts = clk::now();
-    for (int t = 0; t < nTI && !rc; ++t) {
-      for (size_t w0 = 0; w0 < ns && !rc; w0 += waveMax) {
-        const int nn = (int)std::min<size_t>(waveMax, ns - w0);
-        nbStart.assign(nn + 1, 0);
-        offs.clear();
-        uvals.clear();
-        d0.assign(nn, 0.f);
-        seeds.assign(nn, 0.f);
-        exP.assign(nn, 0xFFFFFFFFu);
-        exR.assign(nn, 0.f);
-        for (int q = 0; q < nn; ++q) {
-          const Sample& sm = smp[w0 + q];
-          const Ker& kq = kers[(sm.setting / nCfg) % nKer];
-          const int nb = (int)sm.off.size();
-          nbStart[q + 1] = nbStart[q] + nb;
-          // include/quantileSamplingModule.hpp:368-457 with this sample's
-          // kernel: u = w, w x, w x^2 per sub-variable (0 outside the
-          // kernel or for NaN), delta0 = -sum w x^2 over the continuous
-          // variables at their original index
-          unsigned centre = 0;
-          for (int i = D - 1; i >= 0; --i) centre = centre * kq.dims[i] + kq.dims[i] / 2;
-          ubuf.assign((size_t)nb * nSub, 0.f);
-          std::vector<unsigned> kidx(nb);
-          std::vector<char> kin(nb);
-          for (int j = 0; j < nb; ++j) {
-            kin[j] = index_with_delta(D, kq.dims, centre, sm.off[j].data(), &kidx[j]);
-            for (int i = 0; i < D; ++i) offs.push_back(sm.off[j][i]);
-            for (int B = 0; B < nSub; ++B) {
-              const int e = subVarOf[B];
-              const float x = sm.val[(size_t)j * nExp + e];
-              const float w = kq.w[(size_t)kidx[j] * nExp + (e % nExp)];
-              float r = 0.f;
-              if (kin[j] && !std::isnan(x)) r = convTypes[B] == 0 ? w * 1.f : convTypes[B] == 2 ? w * x * x : w * x;
-              ubuf[(size_t)j * nSub + B] = r;
-            }
-          }
-          uvals.insert(uvals.end(), ubuf.begin(), ubuf.end());
-          float delta0 = 0.f;
-          for (int v = 0; v < nVar; ++v) {
-            if (rq->varTypes[v] != 0) continue;
-            for (int j = 0; j < nb; ++j) {
-              const float x = sm.val[(size_t)j * nExp + v];
-              if (kin[j] && !std::isnan(x)) delta0 += -1.f * (kq.w[(size_t)kidx[j] * nExp + (v % nExp)] * x * x);
-            }
-          }
-          d0[q] = delta0;
-          seeds[q] = sm.seed;
-          if (sm.tiIndex == t) {
-            exP[q] = sm.cell;
-            exR[q] = radius;
-          }
+    double hostInScan = 0.0;
+    for (size_t w0 = 0; w0 < smp.size() && !rc; ) {
+      const int nn = (int)std::min<size_t>(waveMax, smp.size() - w0);
+      for (int t = 0; t < nTI && !rc; ++t) {
+        rc = launch(w0, nn, t);
+        if (!rc && t == nTI - 1) {
+          const double before = drawMs;
+          draw_until(w0 + nn + (size_t)waveMax);
+          hostInScan += drawMs - before;
         }
-        oErr.resize((size_t)nn * K);
-        oPos.resize((size_t)nn * K);
-        rc = qsmi_distribution_wave(tis[t].dev, nn, nbStart.data(), offs.data(), uvals.data(), d0.data(), seeds.data(),
-                                    (float)K, exP.data(), exR.data(), oErr.data(), oPos.data());
-        if (rc) break;
-        for (int q = 0; q < nn; ++q)
-          for (int i = 0; i < K; ++i) {
-            allErr[((w0 + q) * nTI + t) * K + i] = oErr[(size_t)q * K + i];
-            allPos[((w0 + q) * nTI + t) * K + i] = oPos[(size_t)q * K + i];
-          }
-        waves++;
+        if (!rc) rc = collect(w0, nn, t);
       }
+      w0 += nn;
     }
-    scanMs += std::chrono::duration<double, std::milli>(clk::now() - ts).count();
+    scanMs += std::chrono::duration<double, std::milli>(clk::now() - ts).count() - hostInScan;
     if (rc) break;
+    const size_t ns = smp.size();
     // accumulate in setting order (include/calibration.hpp:206-246)
     for (size_t q = 0; q < ns; ++q) {
       const Sample& sm = smp[q];
-      const size_t si = (size_t)sm.setting;
-      const int den = (int)(si / ((size_t)nCfg * nKer));
-      const unsigned n0 = nSamp[si * K];
-      const float errBefore = (float)(cumErr[si * K] / n0);
+      const size_t sx = (size_t)sm.setting;
+      const int den = (int)(sx / ((size_t)nCfg * nKer));
+      const unsigned n0 = nSamp[sx * K];
+      const float errBefore = (float)(cumErr[sx * K] / n0);
       // merge the TIs' lists (each best first): descending error (its tie
       // key when a tie quantum is set), earlier TI first on ties
       std::vector<std::pair<float, std::pair<int, unsigned>>> m;
@@ -401,12 +454,12 @@ extern "C" int qsmi_calibration(const qsmi_calib_request* rq, qsmi_calib_result*
         // variable-interleaved array (variable 0 for one variable)
         const float x = tis[m[i].second.first].data[m[i].second.second];
         const double err = std::pow(std::fabs(x - ref), rq->power);
-        cumErr[si * K + i] += err;
-        cumSq[si * K + i] += err * err;
-        nSamp[si * K + i]++;
+        cumErr[sx * K + i] += err;
+        cumSq[sx * K + i] += err * err;
+        nSamp[sx * K + i]++;
       }
       if (logged < rq->logCap) {
-        rq->logSetting[logged] = (int)si;
+        rq->logSetting[logged] = (int)sx;
         rq->logCell[logged * 2] = (unsigned)sm.tiIndex;
         rq->logCell[logged * 2 + 1] = sm.cell;
         for (int i = 0; i < K; ++i) {
@@ -417,10 +470,10 @@ extern "C" int qsmi_calibration(const qsmi_calib_request* rq, qsmi_calib_result*
         logged++;
       }
       dists++;
-      const float errAfter = (float)(cumErr[si * K] / nSamp[si * K]);
+      const float errAfter = (float)(cumErr[sx * K] / nSamp[sx * K]);
       if (errBefore > errAfter && best[den] > errAfter) {
         best[den] = errAfter;
-        devBest[den] = (float)std::sqrt(cumSq[si * K] / nSamp[si * K] - (double)(errAfter * errAfter));
+        devBest[den] = (float)std::sqrt(cumSq[sx * K] / nSamp[sx * K] - (double)(errAfter * errAfter));
       }
     }
     iter = (int)it + 1;
@@ -438,7 +491,10 @@ extern "C" int qsmi_calibration(const qsmi_calib_request* rq, qsmi_calib_result*
   rs->distributions = dists;
   rs->iterations = iter;
   rs->waves = waves;
-  rs->ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+  rs->ms = std::chrono::duration<double, std::milli>(clk::now() - t1).count();
   rs->scanMs = scanMs;
+  rs->drawMs = drawMs;
+  for (int i = 0; i < 4; ++i) rs->stageMs[i] = stage[i];
+  rs->setupMs = std::chrono::duration<double, std::milli>(t1 - t0).count();
   return QSMI_OK;
 }

@@ -119,6 +119,15 @@ int row_blocks(const qsmi_ti* t);
 long long c_stride(const qsmi_ti* t);
 int grow_wave_bufs(qsmi_device* dev, WaveBufs* b, int nNodes, int total, int K);
 void free_wave_bufs(WaveBufs* b);
+// One wave in three steps (qsmi_scan_wave / qsmi_distribution_wave are
+// stage + run + read): host preparation into pinned staging and upload,
+// the asynchronous launches, the blocking read-back. exPos/exRadius: the
+// optional verbatim disk per node (UINT32_MAX: none).
+int stage_wave(qsmi_device* dev, WaveBufs* b, int nNodes, const int* nbStart, const int* offsets,
+               const float* values, const float* delta0, const float* seeds, float k,
+               const unsigned* exPos = nullptr, const float* exRadius = nullptr);
+int run_wave(qsmi_device* dev, WaveBufs* b);
+int read_wave(qsmi_device* dev, WaveBufs* b, float* outErrors, unsigned* outPositions, unsigned* outChosen);
 // spectrum stage (+ dim1 inverse for 3-D) and rows stage of nNodes nodes
 // (returns a qsmi status: tap tables may be allocated)
 // count (device, optional; 1000x1000 path only): nodes [0, *count) of the

@@ -406,6 +406,7 @@ typedef struct {
   int* logSetting;            /* logCap */
   unsigned* logCell;          /* logCap x 2: TI index, cell */
   unsigned* logPos;           /* logCap x maxK x 2: TI index, position (UINT32_MAX past the valid ones) */
+  int stageTiming;            /* 1: time every wave's kernels with HIP events (stageMs) */
 } qsmi_calib_request;
 
 typedef struct {
@@ -415,8 +416,13 @@ typedef struct {
   long long distributions;    /* out: samples scored */
   int iterations;             /* out */
   int waves;                  /* out */
-  double ms;                  /* out: wall time of the loop */
-  double scanMs;              /* out: of which in the GPU waves */
+  double ms;                  /* out: wall time of the loop (set-up excluded) */
+  double scanMs;              /* out: of which waiting for the GPU waves */
+  double drawMs;              /* out: host draws (partly overlapped with the waves) */
+  double setupMs;             /* out: TI spectra, devices, offsets */
+  double stageMs[4];          /* out: with stageTiming, device time per stage summed
+                                 over the waves: spectrum, dim1 inverse (3-D),
+                                 rows + select, merge */
 } qsmi_calib_result;
 
 int qsmi_calibration(const qsmi_calib_request* req, qsmi_calib_result* res);
