@@ -303,7 +303,7 @@ class Wave:
 
 def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dims=None, gpu=0,
             wave_size=256, tie_quantum=-1.0, stage_timing=False, record=0, lp=None, lp_shortlist=0,
-            lp_fallback_every=0, lp_fallback_mse=0.0, lp_audit=False, lp_min_cover=0.0):
+            lp_fallback_every=0, lp_fallback_mse=0.0, lp_audit=False, lp_min_cover=0.0, no_verbatim=False):
     """In-memory QuickSampling (qsmi_qs_run_in_memory).
 
     Arrays are flat in G2S DataImage order (dimension 0 fastest, variable
@@ -340,6 +340,7 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
     rq.waveSize = int(wave_size)
     rq.tieQuantum = float(tie_quantum)
     rq.stageTiming = int(bool(stage_timing))
+    rq.noVerbatim = int(bool(no_verbatim))
     rec = None
     if record:
         rec = (np.zeros((int(record), 80, 2), np.int32), np.zeros((int(record), 80), np.float32),

@@ -183,6 +183,7 @@ class QsRequest(C.Structure):
         ("lpFallbackMse", C.c_float),
         ("lpAudit", C.c_int),
         ("lpMinCover", C.c_float),
+        ("noVerbatim", C.c_int),
     ]
 
 

@@ -526,6 +526,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     R.subOf[B] = subVarOf[B];
   }
   R.stageTiming = rq->stageTiming != 0;
+  R.noVerbatim = rq->noVerbatim != 0;
   std::vector<int> orderI(order.begin(), order.end());
   void *dOrder = nullptr, *dSeeds = nullptr, *dKer = nullptr, *dCat = nullptr, *dDi = nullptr, *dIdx = nullptr,
        *dTi = nullptr;

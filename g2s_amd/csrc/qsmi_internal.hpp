@@ -158,6 +158,7 @@ struct ResidentLoop {
   int conv[16], subOf[16];
   std::vector<int> waveStart;  // wave w = order[waveStart[w] .. waveStart[w+1])
   bool stageTiming = false;
+  bool noVerbatim = false;     // -nV: no candidate at the continuation of the nearest neighbour's copy
   const int* dOrder;
   const unsigned* dPathCells;
   const float* dPathSeeds;

@@ -40,6 +40,8 @@ struct GatherParams {
   int trueMismatch;
   float quantum;
   unsigned tiCells;
+  int noVerbatim;  // 1: exclude the position that continues the nearest neighbour's copy (-nV)
+  unsigned f[3];   // TI FFT grid (the reference's index space)
 };
 
 __device__ __forceinline__ uint32_t seed_bits_dev(float seed) {
@@ -229,6 +231,53 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
     }
     ni.nb = cnt;
     ni.delta0 = (P.trueMismatch && !P.cross) ? delta0 : 0.f;
+    if (P.noVerbatim) {
+      // include/simulation.hpp:567-580: the TI position the nearest
+      // neighbour was copied from, moved back by its offset (wrapped, its
+      // validity ignored), then QuantileSamplingModule's
+      // setValueInErrorArray(cvtPositionToIndex(.)) (include/
+      // quantileSamplingModule.hpp:566-571): the map entry at the index of
+      // that position, i.e. position cvtIndexToPosition(index), both maps
+      // with the node's bounding-box minimum in unsigned arithmetic
+      // (src/CPUThreadDevice.cpp:257-284)
+      const int* d1 = offs + (long long)idx[1] * D;
+      long long t1 = 0, mul = 1;
+      for (int d = 0; d < D; ++d) {
+        t1 += (long long)(cc[d] + d1[d]) * mul;
+        mul *= P.sd[d];
+      }
+      unsigned rem = index[t1];
+      unsigned v[3] = {0, 0, 0};
+      for (int d = 0; d < D; ++d) {
+        int c = (int)(rem % (unsigned)P.s[d]) - d1[d];
+        c = ((c % P.s[d]) + P.s[d]) % P.s[d];
+        v[d] = (unsigned)c;
+        rem /= (unsigned)P.s[d];
+      }
+      unsigned vpos = 0;
+      for (int d = D - 1; d >= 0; --d) vpos = vpos * (unsigned)P.s[d] + v[d];
+      unsigned S = 1, N = 1;
+      for (int d = 0; d < D; ++d) {
+        S *= (unsigned)P.s[d];
+        N *= P.f[d];
+      }
+      unsigned idxv = 0, div = S;
+      for (int d = D - 1; d >= 0; --d) {
+        const unsigned f = P.f[d], sd = (unsigned)P.s[d];
+        div /= sd;
+        idxv = idxv * f + (f - (vpos / div + (unsigned)mn[d]) % sd - 1);
+      }
+      unsigned pos = 0;
+      div = N;
+      for (int d = D - 1; d >= 0; --d) {
+        const unsigned f = P.f[d], sd = (unsigned)P.s[d];
+        div /= f;
+        pos = pos * sd + ((f - (idxv / div) % f - (unsigned)mn[d] - 1) + sd) % sd;
+      }
+      ni.exOn = 1;
+      ni.exPos = pos;
+      ni.exR2 = 0.f;
+    }
     // the count layer is a sum of neighbour weights (0 or >= the smallest
     // one): below half of it the FFT value is noise around zero
     ni.cmin = (P.cross && wmin < INFINITY) ? 0.5f * wmin : 0.f;
@@ -341,6 +390,9 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   for (int a = 0; a < P.V; ++a) P.act[a] = dev->active[a];
   for (int q = 0; q < t->nSub * t->nSub; ++q) P.alpha[q] = dev->alpha[q];
   P.cross = dev->cross ? 1 : 0;
+  P.noVerbatim = R.noVerbatim ? 1 : 0;
+  for (int d = 0; d < 3; ++d) P.f[d] = d < D ? (unsigned)t->f[d] : 1u;
+  if (R.noVerbatim && R.lp) return fail(QSMI_ERR_UNSUPPORTED, "learned proposal with the verbatim exclusion");
   if (dev->cross) {
     if (R.lp) return fail(QSMI_ERR_UNSUPPORTED, "learned proposal: TI with NaN unsupported");
     for (int q = 0; q < t->nSub * t->nSub; ++q) P.alphaX[q] = dev->alphaX[q];

@@ -272,6 +272,10 @@ typedef struct {
   float lpMinCover;        /* exact scan when fewer than this fraction of a
                               node's neighbours lie on the key stencil (the
                               encoders cannot see the others; <= 0 -> 0.5) */
+  int noVerbatim;          /* -nV (src/qs.cpp:711-715): per node, no candidate at
+                              the TI position that continues the nearest
+                              neighbour's copy (include/simulation.hpp:567-580);
+                              not with the learned proposal */
 } qsmi_qs_request;
 
 typedef struct {

@@ -62,6 +62,7 @@ class OrcSim(C.Structure):
         ("mode", C.c_int),
         ("quantum", C.c_float),
         ("cross", C.c_int),
+        ("noVerbatim", C.c_int),
     ]
 
 
@@ -110,6 +111,7 @@ def lib():
     _sig(L, "orc_sim_node", C.c_uint, [C.c_void_p, f32p, C.c_uint, C.c_int, f32p, u32p, C.POINTER(C.c_int)])
     _sig(L, "orc_sim_run", None, [C.c_void_p, f32p, u32p])
     _sig(L, "orc_sim_node_map", C.c_int, [C.c_void_p, f32p, C.c_uint, f64p, C.c_void_p])
+    _sig(L, "orc_sim_set_index", None, [C.c_void_p, C.c_void_p])
     _sig(L, "orc_calib_offsets", C.c_int, [C.c_int, u32p, i32p, C.c_int])
     _sig(L, "orc_exclude_disk", None, [C.c_int, u32p, f32p, C.c_uint, C.c_float])
     _sig(L, "orc_calibration", C.c_int, [C.c_int, C.c_int, u32p, C.c_void_p, C.c_int, i32p, C.c_int, u32p, C.c_void_p,
@@ -254,7 +256,7 @@ class Sim:
     """Serial QS simulation / per-node lockstep checker over the oracle."""
 
     def __init__(self, ti, tidims, types, ncat, catvals, simdims, kdims, kernel_exp, offs, n, k, path, seeds,
-                 begin, mode=0, quantum=0.0, cross=False):
+                 begin, mode=0, quantum=0.0, cross=False, no_verbatim=False):
         self._keep = []
 
         def keep(a):
@@ -288,7 +290,9 @@ class Sim:
         s.mode = int(mode)
         s.quantum = float(quantum)
         s.cross = int(bool(cross))
+        s.noVerbatim = int(bool(no_verbatim))
         self.spec = s
+        self._index = None
         self.nvar = len(types)
         self.K = int(np.ceil(k))
         self.h = lib().orc_sim_create(C.byref(s))
@@ -309,6 +313,11 @@ class Sim:
         c = lib().orc_sim_gather(self.h, i, out)
         return out[:c].copy()
 
+    def set_index(self, idx):
+        """The importDataIndex state -nV reads (kept alive here)."""
+        self._index = np.ascontiguousarray(idx, np.uint32)
+        lib().orc_sim_set_index(self.h, self._index.ctypes.data)
+
     def node(self, di, i):
         v = np.zeros(self.K, np.float32)
         p = np.zeros(self.K, np.uint32)
@@ -325,7 +334,7 @@ class Sim:
 
     def run(self, di):
         di = f32(di).reshape(-1).copy()
-        idx = np.full(self.ncells, 0xFFFFFFFF, np.uint32)
+        idx = np.zeros(self.ncells, np.uint32)  # hard data keep 0, as the reference's empty index image
         lib().orc_sim_run(self.h, di, idx)
         return di, idx
 

@@ -56,6 +56,8 @@ void orc_ti_spectra_f32(int D, const unsigned* f, int nSub, const float* imgs, f
 void orc_ti_spectra_f64(int D, const unsigned* f, int nSub, const float* imgs, double* out);
 void orc_fkst(const float* data, unsigned N, unsigned short k, float seed, float* out, unsigned* pos);
 unsigned orc_index_to_position(int D, const unsigned* f, const unsigned* s, const int* mn, unsigned index);
+unsigned orc_position_to_index(int D, const unsigned* f, const unsigned* s, const int* mn, unsigned position);
+int orc_index_with_delta(int D, const unsigned* dims, unsigned position, const int* delta, unsigned* location);
 unsigned orc_seed_from_unit_float(float seed);
 uint32_t orc_tie_hash(uint32_t seedbits, uint32_t position);
 void orc_bbox(int D, int nb, const int* off, int* mn, int* mx);
@@ -83,6 +85,7 @@ struct OrcSim {
   int mode;       // 0: direct FP64 + device rule; 1: refflow FP32 + fKst; 2: refflow FP64 + fKst
   float quantum;  // tie quantum of the device rule (0: none)
   int cross;      // 1: TI with NaN, cross-measurement form (mode 0 only)
+  int noVerbatim; // 1: -nV, no candidate at the continuation of neighbour 1's copy
 };
 
 struct OrcSimCtx {
@@ -101,6 +104,7 @@ struct OrcSimCtx {
   std::vector<float> catflat;
   std::vector<int> typesOrig;
   std::vector<unsigned> posterior;
+  const unsigned* index = nullptr;  // importDataIndex state for -nV (orc_sim_set_index)
 };
 
 void* orc_sim_create(const OrcSim* p) {
@@ -318,6 +322,35 @@ static void cross_map(OrcSimCtx* c, int nb, const std::vector<int>& off, const s
 // (value, TI position) pairs sorted best first, and returns the TI position
 // chosen by the draw (or the marginal draw when only the centre is a
 // neighbour, simulation.hpp:586-606). *nbOut = neighbour count.
+// -nV (include/simulation.hpp:567-580): the TI position the nearest
+// neighbour (entry 1 of the neighbour list) was copied from, moved back by
+// its offset with DataImage::indexWithDelta (wrapped, validity ignored), as
+// the reference index QuantileSamplingModule hands to setValueInErrorArray
+// (include/quantileSamplingModule.hpp:566-571): cvtPositionToIndex with the
+// node's bounding box.
+static unsigned verbatim_index(const OrcSimCtx* c, unsigned pathIdx, const std::vector<int>& off, int nb) {
+  const OrcSim& p = c->p;
+  const int D = c->D;
+  unsigned cell = p.path[p.beginPath + pathIdx];
+  size_t rem = cell, t = 0, mul = 1;
+  for (int i = 0; i < D; ++i) {
+    int cc = int(rem % c->sd[i]);
+    rem /= c->sd[i];
+    t += (size_t)(cc + off[(size_t)1 * D + i]) * mul;
+    mul *= c->sd[i];
+  }
+  const unsigned from = c->index[t];
+  int back[3] = {0, 0, 0};
+  for (int i = 0; i < D; ++i) back[i] = -off[(size_t)1 * D + i];
+  unsigned vpos = 0;
+  orc_index_with_delta(D, c->s, from, back, &vpos);
+  int mn[3], mx[3];
+  orc_bbox(D, nb, off.data(), mn, mx);
+  return orc_position_to_index(D, c->f, c->s, mn, vpos);
+}
+
+void orc_sim_set_index(void* h, const unsigned* index) { static_cast<OrcSimCtx*>(h)->index = index; }
+
 unsigned orc_sim_node(void* h, const float* di, unsigned pathIdx, int K, float* outVal, unsigned* outPos,
                       int* nbOut) {
   auto* c = static_cast<OrcSimCtx*>(h);
@@ -357,6 +390,12 @@ unsigned orc_sim_node(void* h, const float* di, unsigned pathIdx, int K, float* 
     else
       orc_map_direct_f64(D, c->s, c->nSub, c->Y.data(), c->alpha.data(), nb, off.data(), u.data(), (double)delta0, 0,
                          map.data());
+    if (p.noVerbatim && c->index) {
+      // the index's entry in position space
+      int mn[3], mx[3];
+      orc_bbox(D, nb, off.data(), mn, mx);
+      map[orc_index_to_position(D, c->f, c->s, mn, verbatim_index(c, pathIdx, off, nb))] = -INFINITY;
+    }
     // device rule on the exact map
     uint32_t sb = orc_seed_from_unit_float(seed);
     struct Cd {
@@ -394,6 +433,7 @@ unsigned orc_sim_node(void* h, const float* di, unsigned pathIdx, int K, float* 
     else
       orc_map_refflow_f64(D, c->s, c->f, c->nSub, c->Yhat64.data(), c->alpha.data(), nb, off.data(), u.data(),
                           delta0, 0, 1, map.data());
+    if (p.noVerbatim && c->index) map[verbatim_index(c, pathIdx, off, nb)] = -INFINITY;  // index space
     std::vector<unsigned> pos(K);
     orc_fkst(map.data(), (unsigned)c->F, (unsigned short)K, seed, outVal, pos.data());
     int mn[3], mx[3];
@@ -471,6 +511,8 @@ void orc_sim_run(void* h, float* di, unsigned* idxOut) {
   int K = (int)std::ceil(p.k);
   std::vector<float> v(K);
   std::vector<unsigned> pos(K);
+  const unsigned* keep = c->index;
+  c->index = idxOut;  // the run's importDataIndex (hard data 0, as the reference's empty copy)
   for (unsigned i = 0; i + p.beginPath < p.pathLen; ++i) {
     unsigned cell = p.path[p.beginPath + i];
     int nb;
@@ -480,6 +522,7 @@ void orc_sim_run(void* h, float* di, unsigned* idxOut) {
     const float* tiv = p.cross ? c->tiX.data() : p.ti;
     for (int q = 0; q < p.nVar; ++q) di[(size_t)cell * p.nVar + q] = tiv[(size_t)tpos * p.nVar + q];
   }
+  c->index = keep;
 }
 
 }  // extern "C"

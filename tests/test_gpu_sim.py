@@ -32,7 +32,7 @@ def default_kernel(ti_dims, nexp):
 
 
 def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatch_frac=0.01, nodes=None,
-             weights=None, wave_size=64, threads=8, cross=False):
+             weights=None, wave_size=64, threads=8, cross=False, no_verbatim=False):
     """nodes: path-relative node indices to check (None: every node);
     weights: one kernel weight per variable (None: the default uniform 1)."""
     types = list(types)
@@ -42,7 +42,7 @@ def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatc
         kd0 = [d // 2 + 1 for d in ti_dims]
         gkw = dict(kernel=np.tile(np.asarray(weights, np.float32), int(np.prod(kd0))), kernel_dims=kd0)
     sim, idx, st = g2s_amd.qs(ti, ti_dims, di0, sim_dims, types, n, k, seed, wave_size=wave_size,
-                              tie_quantum=quantum, **gkw)
+                              tie_quantum=quantum, no_verbatim=no_verbatim, **gkw)
     _log(f"  gpu run: {st}")
     # oracle set-up identical to the reference's qs.cpp
     ncat, catvals = [], []
@@ -67,7 +67,8 @@ def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatc
     gp, gs, gb = g2s_amd.path(informed, seed)
     assert np.array_equal(p, gp) and np.array_equal(s, gs) and b == gb
     orc = oracle.Sim(ti, ti_dims, types, ncat, catvals, sim_dims, kd, kern, offs, n, k, p, s, b, mode=0,
-                     quantum=quantum, cross=cross)
+                     quantum=quantum, cross=cross, no_verbatim=no_verbatim)
+    orc.set_index(idx)  # -nV reads the positions the neighbours were copied from
     K = int(np.ceil(k))
     simf = sim.reshape(-1, nvar)
     # hard data kept
@@ -108,6 +109,68 @@ def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatc
          f"tolerance {worst_gap:.3f})")
     assert frac <= max_mismatch_frac
     return sim, idx, st
+
+
+def _verbatim_hits(orc, ti_dims, sim_dims, offs, p, b, idx):
+    """Nodes whose chosen position is the one -nV removes (the TI position
+    neighbour 1 was copied from, moved back by its offset with wrap-around,
+    through the pinned index maps with the node's bounding box)."""
+    L = oracle.lib()
+    f = oracle.fft_size(ti_dims)
+    s = oracle.u32(ti_dims)
+    hits = total = 0
+    for i in range(len(p) - b):
+        g = orc.gather(i)
+        if len(g) < 2:
+            continue
+        o = offs[g]
+        c = np.array(np.unravel_index(int(p[b + i]), tuple(reversed(sim_dims)))[::-1])
+        t1 = int(np.ravel_multi_index(tuple((c + o[1])[::-1]), tuple(reversed(sim_dims))))
+        src = np.array(np.unravel_index(int(idx[t1]), tuple(reversed(ti_dims)))[::-1])
+        v = (src - o[1]) % np.array(ti_dims)
+        vpos = int(np.ravel_multi_index(tuple(v[::-1]), tuple(reversed(ti_dims))))
+        mn = np.minimum(o.min(0), 0).astype(np.int32)
+        excl = L.orc_index_to_position(len(ti_dims), f, s, mn, L.orc_position_to_index(len(ti_dims), f, s, mn, vpos))
+        hits += int(idx[p[b + i]] == excl)
+        total += 1
+    return hits, total
+
+
+@pytest.mark.parametrize("kind", ["continuous", "categorical_hard_data"])
+def test_sim_no_verbatim(kind):
+    """-nV: lockstep with the oracle's serial loop (which removes the
+    position in its own index maps), and no node takes its verbatim
+    position; the plain run on the same TI takes it at some nodes."""
+    if kind == "continuous":
+        g = synth.grf((40, 44), 4.0, seed=31)
+        ti, dims = synth.to_g2s(np.exp(g))
+        types, n, k, q = [0], 12, 1.5, 0.0
+        sdims = (36, 30)
+        di = np.full(int(np.prod(sdims)), np.nan, np.float32)
+    else:
+        g = synth.grf((44, 40), 5.0, seed=32)
+        ti, dims = synth.to_g2s(np.digitize(g, np.quantile(g, [0.4, 0.75])).astype(np.float32))
+        types, n, k, q = [1], 10, 1.2, float(2.0 ** (np.ceil(np.log2(10)) - 16))
+        sdims = (32, 34)
+        rng = np.random.default_rng(9)
+        di = np.full(int(np.prod(sdims)), np.nan, np.float32)
+        pick = rng.choice(di.size, di.size // 10, replace=False)
+        di[pick] = ti.reshape(-1)[rng.integers(0, ti.size, pick.size)]
+    _log(f"-nV {kind}")
+    sim, idx, st = lockstep(ti, dims, di, sdims, types, n, k, 5, q, no_verbatim=True)
+    kd, kern = default_kernel(dims, 1 if types[0] == 0 else len(np.unique(ti)))
+    offs = oracle.offsets(kd)
+    informed = (~np.isnan(di)).astype(np.uint8)
+    p, s_, b = oracle.path(informed, 5)
+    orc = oracle.Sim(ti, dims, types, [] if types[0] == 0 else [len(np.unique(ti))],
+                     [] if types[0] == 0 else np.unique(ti).tolist(), sdims, kd, kern, offs, n, k, p, s_, b)
+    hits, total = _verbatim_hits(orc, dims, sdims, offs, p, b, idx)
+    _log(f"  verbatim positions taken with -nV: {hits} of {total}")
+    assert hits == 0 and total > 100
+    _, idxp, _ = g2s_amd.qs(ti, dims, di, sdims, types, n, k, 5, wave_size=64, tie_quantum=q)
+    hp, tp = _verbatim_hits(orc, dims, sdims, offs, p, b, idxp)
+    _log(f"  without -nV: {hp} of {tp}")
+    assert hp > 0
 
 
 def test_sim_continuous_2d():

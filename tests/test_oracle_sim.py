@@ -17,7 +17,7 @@ import oracle
 from g2s_amd import synth
 
 
-def setup(shape_ti=(30, 34), shape_sim=(20, 18), n=9, seed=3, hard_frac=0.0, mode=0):
+def setup(shape_ti=(30, 34), shape_sim=(20, 18), n=9, seed=3, hard_frac=0.0, mode=0, no_verbatim=False):
     g = synth.grf(shape_ti, 3.0, seed=seed)
     ti, dims = synth.to_g2s(np.exp(g))
     sdims = tuple(reversed(shape_sim))
@@ -32,7 +32,7 @@ def setup(shape_ti=(30, 34), shape_sim=(20, 18), n=9, seed=3, hard_frac=0.0, mod
     kd = [d // 2 + 1 for d in dims]
     offs = oracle.offsets(kd)
     sim = oracle.Sim(ti, dims, [0], [], [], sdims, kd, np.ones(int(np.prod(kd)), np.float32), offs, n, 1.5, p, s, b,
-                     mode=mode)
+                     mode=mode, no_verbatim=no_verbatim)
     return sim, ti, dims, sdims, di, p, s, b, offs, kd
 
 
@@ -84,3 +84,53 @@ def test_reference_dataflow_matches_closed_form_realisation():
     sim2, *_ = setup(mode=2)
     out2, idx2 = sim2.run(di)
     assert np.array_equal(idx0, idx2)
+
+
+def _verbatim_positions(sim, ti_dims, sdims, offs, p, b, idx):
+    """Per path node with a neighbour: the position -nV removes, from its
+    definition (include/simulation.hpp:567-580 then
+    include/quantileSamplingModule.hpp:566-571): the TI position neighbour 1
+    was copied from, moved back by neighbour 1's offset with wrap-around,
+    through cvtPositionToIndex and back (the pinned index maps, bounding box
+    of the node's offsets)."""
+    L = oracle.lib()
+    f = oracle.fft_size(ti_dims)
+    s = oracle.u32(ti_dims)
+    out = {}
+    for i in range(len(p) - b):
+        g = sim.gather(i)
+        if len(g) < 2:
+            continue
+        o = offs[g]
+        cell = int(p[b + i])
+        c = np.array(np.unravel_index(cell, tuple(reversed(sdims)))[::-1])
+        t1 = int(np.ravel_multi_index(tuple((c + o[1])[::-1]), tuple(reversed(sdims))))
+        src = np.array(np.unravel_index(int(idx[t1]), tuple(reversed(ti_dims)))[::-1])
+        v = (src - o[1]) % np.array(ti_dims)
+        vpos = int(np.ravel_multi_index(tuple(v[::-1]), tuple(reversed(ti_dims))))
+        mn = np.minimum(o.min(0), 0).astype(np.int32)
+        index = L.orc_position_to_index(len(ti_dims), f, s, mn, vpos)
+        out[i] = L.orc_index_to_position(len(ti_dims), f, s, mn, index)
+    return out
+
+
+@pytest.mark.parametrize("hard", [0.0, 0.15])
+def test_no_verbatim(hard):
+    """-nV: the closed-form map with the exclusion in position space and the
+    reference dataflow with it in index space give the same realisation; no
+    node takes its verbatim position; without -nV some nodes do (small TI,
+    so verbatim copies are common)."""
+    sim0, ti, dims, sdims, di, p, s, b, offs, kd = setup(hard_frac=hard, mode=0, no_verbatim=True)
+    out0, idx0 = sim0.run(di)
+    sim2, *_ = setup(hard_frac=hard, mode=2, no_verbatim=True)
+    out2, idx2 = sim2.run(di)
+    assert np.array_equal(idx0, idx2)
+    vb = _verbatim_positions(sim0, dims, sdims, offs, p, b, idx0)
+    assert len(vb) > 100
+    assert all(idx0[p[b + i]] != v for i, v in vb.items())
+    plain, *_ = setup(hard_frac=hard, mode=0)
+    outp, idxp = plain.run(di)
+    vbp = _verbatim_positions(plain, dims, sdims, offs, p, b, idxp)
+    hits = sum(idxp[p[b + i]] == v for i, v in vbp.items())
+    assert hits > 0
+    assert not np.array_equal(idx0, idxp)
