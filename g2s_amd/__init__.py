@@ -313,7 +313,15 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
     record > 0: stats["pairs"] = (offsets [m,80,2], values [m,80], best [m])
     of the first `record` path nodes (2-D, one continuous variable).
     lp: a LearnedProposal with keys built for this TI (learned-proposal
-    form; the stats then carry its counters)."""
+    form; the stats then carry its counters).
+    ti may be a list of arrays of one extent (several training images, qs
+    -ti a b ...); index is then position * nTI + TI."""
+    tis = None
+    if isinstance(ti, (list, tuple)):
+        tis = [_f32(t).reshape(-1) for t in ti]
+        ti = tis[0]
+        if len(tis) == 1:
+            tis = None
     ti = _f32(ti).reshape(-1)
     di = _f32(di).reshape(-1)
     var_types = _i32(var_types)
@@ -341,6 +349,11 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
     rq.tieQuantum = float(tie_quantum)
     rq.stageTiming = int(bool(stage_timing))
     rq.noVerbatim = int(bool(no_verbatim))
+    tptr = None
+    if tis is not None:
+        tptr = (C.c_void_p * len(tis))(*[t.ctypes.data for t in tis])
+        rq.nTI = len(tis)
+        rq.tis = C.cast(tptr, C.c_void_p)
     rec = None
     if record:
         rec = (np.zeros((int(record), 80, 2), np.int32), np.zeros((int(record), 80), np.float32),

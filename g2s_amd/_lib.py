@@ -184,6 +184,8 @@ class QsRequest(C.Structure):
         ("lpAudit", C.c_int),
         ("lpMinCover", C.c_float),
         ("noVerbatim", C.c_int),
+        ("nTI", C.c_int),
+        ("tis", C.c_void_p),
     ]
 
 

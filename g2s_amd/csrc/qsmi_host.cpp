@@ -269,7 +269,10 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   if (!rq || !rs || !rs->simulation || !rs->index) return hfail(QSMI_ERR_ARG, "null argument");
   const int D = rq->ndim;
   if (D < 1 || D > 3) return hfail(QSMI_ERR_UNSUPPORTED, "ndim must be 1, 2 or 3");
-  if (rq->nVar < 1 || !rq->varTypes || !rq->ti || !rq->di) return hfail(QSMI_ERR_ARG, "invalid variables");
+  const int nTI = rq->nTI > 1 ? rq->nTI : 1;
+  if (rq->nVar < 1 || !rq->varTypes || !rq->di || (nTI == 1 && !rq->ti) || (nTI > 1 && !rq->tis))
+    return hfail(QSMI_ERR_ARG, "invalid variables");
+  if (nTI > 8) return hfail(QSMI_ERR_UNSUPPORTED, "at most 8 training images");
   if (rq->n < 1 || !(rq->k > 0.f) || std::ceil(rq->k) > 16.f) return hfail(QSMI_ERR_UNSUPPORTED, "need n >= 1, 0 < k <= 16");
   const auto t0 = clk::now();
   const int nVar = rq->nVar;
@@ -281,10 +284,26 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     S *= s[i];
     SD *= sd[i];
   }
+  // several TIs: one array, TI after TI (the order qs walks them in for
+  // categories, src/qs.cpp:1498-1525, and the marginal draw)
+  std::vector<float> tiCat;
+  const float* tiIn = rq->ti;
+  if (nTI > 1) {
+    if (rq->lp || rq->recordCap > 0 || rq->noVerbatim)
+      return hfail(QSMI_ERR_UNSUPPORTED, "several TIs: not with the learned proposal, the record or -nV");
+    tiCat.resize((size_t)nTI * S * nVar);
+    for (int t = 0; t < nTI; ++t) {
+      if (!rq->tis[t]) return hfail(QSMI_ERR_ARG, "null training image");
+      std::memcpy(tiCat.data() + (size_t)t * S * nVar, rq->tis[t], S * nVar * sizeof(float));
+    }
+    for (float x : tiCat)
+      if (std::isnan(x)) return hfail(QSMI_ERR_UNSUPPORTED, "several TIs with NaN (cross-measurement form)");
+    tiIn = tiCat.data();
+  }
   // a TI with NaN runs in the cross-measurement form (src/qs.cpp:1436-1466):
   // cells with any NaN variable lose all of them
   bool cross = false;
-  for (size_t c = 0; c < S * nVar && !cross; ++c) cross = std::isnan(rq->ti[c]);
+  for (size_t c = 0; c < S * nVar && !cross; ++c) cross = std::isnan(tiIn[c]);
   std::vector<float> tiX;
   std::vector<uint8_t> centreOk;
   if (cross) {
@@ -295,7 +314,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
                      "value past each neighbour's expanded values (include/simulation.hpp:512-537 against "
                      "include/DataImage.hpp:681-692); continuous variables only");
     if (rq->lp || rq->recordCap > 0) return hfail(QSMI_ERR_UNSUPPORTED, "learned proposal: TI with NaN unsupported");
-    tiX.assign(rq->ti, rq->ti + S * nVar);
+    tiX.assign(tiIn, tiIn + S * nVar);
     centreOk.assign(S, 1);
     for (size_t c = 0; c < S; ++c) {
       bool nan = false;
@@ -306,7 +325,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
       }
     }
   }
-  const float* tiData = cross ? tiX.data() : rq->ti;
+  const float* tiData = cross ? tiX.data() : tiIn;
   if ((rq->lp || rq->recordCap > 0) && !(D == 2 && nVar == 1 && rq->varTypes[0] == 0))
     return hfail(QSMI_ERR_UNSUPPORTED, "learned proposal and pair record: 2-D, one continuous variable");
   if (rq->recordCap > 0 && (!rq->recOffsets || !rq->recValues || !rq->recBest))
@@ -319,7 +338,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     if (rq->varTypes[v] != 0) {
       std::vector<float> cv(4096);
       int cnt = 0;
-      int rc = qsmi_categories(S, nVar, rq->ti, v, cv.data(), 4096, &cnt);
+      int rc = qsmi_categories(S * nTI, nVar, tiIn, v, cv.data(), 4096, &cnt);
       if (rc) return rc;
       ncat.push_back(cnt);
       catvals.insert(catvals.end(), cv.begin(), cv.begin() + cnt);
@@ -383,14 +402,33 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   if (cross)
     qsmi_ti_subimages_x(D, s, nVar, rq->varTypes, tiData, sub.data());
   else
-    qsmi_ti_subimages(D, s, nVar, rq->varTypes, rq->ti, ncat.data(), catvals.data(), sub.data());
+    qsmi_ti_subimages(D, s, nVar, rq->varTypes, tiIn, ncat.data(), catvals.data(), sub.data());
   qsmi_ti* ti = nullptr;
   qsmi_device* dev = nullptr;
   int rc = qsmi_ti_create(rq->gpu, D, s, nullptr, nSub, sub.data(), &ti);
   if (rc) return rc;
   rc = qsmi_device_create(ti, alpha, 0, &dev);
   if (!rc && cross) rc = qsmi::set_cross(dev, alphaX, centreOk.data());
+  // the other TIs: spectra and a device each
+  std::vector<qsmi_ti*> xti;
+  std::vector<qsmi_device*> xdev;
+  for (int t = 1; t < nTI && !rc; ++t) {
+    qsmi_ti_subimages(D, s, nVar, rq->varTypes, tiIn + (size_t)t * S * nVar, ncat.data(), catvals.data(), sub.data());
+    qsmi_ti* tt = nullptr;
+    qsmi_device* dd = nullptr;
+    rc = qsmi_ti_create(rq->gpu, D, s, nullptr, nSub, sub.data(), &tt);
+    if (!rc) xti.push_back(tt);
+    if (!rc) rc = qsmi_device_create(tt, alpha, 0, &dd);
+    if (!rc) xdev.push_back(dd);
+  }
+  auto destroy_extra = [&]() {
+    for (qsmi_device* d : xdev) qsmi_device_destroy(d);
+    for (qsmi_ti* x : xti) qsmi_ti_destroy(x);
+    xdev.clear();
+    xti.clear();
+  };
   if (rc) {
+    destroy_extra();
     if (dev) qsmi_device_destroy(dev);
     qsmi_ti_destroy(ti);
     return rc;
@@ -406,6 +444,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     }
   }
   qsmi_set_tie_quantum(dev, quantum);
+  for (qsmi_device* d : xdev) qsmi_set_tie_quantum(d, quantum);
   // output state
   float* di = rs->simulation;
   std::memcpy(di, rq->di, SD * nVar * sizeof(float));
@@ -537,7 +576,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   if (!rc) rc = dev_upload(catv.data(), catv.size() * sizeof(float), &dCat);
   if (!rc) rc = dev_upload(di, SD * nVar * sizeof(float), &dDi);
   if (!rc) rc = dev_upload(rs->index, SD * sizeof(unsigned), &dIdx);
-  if (!rc) rc = dev_upload(tiData, S * nVar * sizeof(float), &dTi);
+  if (!rc) rc = dev_upload(tiData, (size_t)nTI * S * nVar * sizeof(float), &dTi);
   const size_t recCap = rq->recordCap > 0 ? std::min<size_t>((size_t)rq->recordCap, nPath) : 0;
   void *dRecOff = nullptr, *dRecVal = nullptr, *dRecBest = nullptr;
   if (!rc && recCap) {
@@ -583,6 +622,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     R.dDi = (float*)dDi;
     R.dIndex = (uint32_t*)dIdx;
     R.dTi = (const float*)dTi;
+    R.extraDevs = xdev;
     rc = qsmi::run_resident_waves(dev, R, stageMs, &scanned, &waves);
   }
   if (!rc) {
@@ -605,6 +645,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   rs->phaseMs[2] = std::chrono::duration<double, std::milli>(tUp - tLv).count();
   rs->phaseMs[3] = std::chrono::duration<double, std::milli>(tLoop - tUp).count();
   const auto t2 = clk::now();
+  destroy_extra();
   qsmi_device_destroy(dev);
   qsmi_ti_destroy(ti);
   if (rc) {

@@ -145,6 +145,9 @@ void launch_rows(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, cons
 void launch_merge_draw(const ScanGeom& g, const NodeInfo* info, const Cand* cand, int nNodes, int candBlocks,
                        int rowsPerBlock, int K, float kf, float* outVal, uint32_t* outPos, uint32_t* chosen,
                        float* simVals, const float* tiVals, int nVar, uint32_t* indexOut, hipStream_t st);
+void launch_merge_ti(const NodeInfo* info, int nN, int nTI, int K, float kf, const float* val, const uint32_t* pos,
+                     long long stride, float quantum, float* simVals, const float* tiVals, int nVar, uint32_t tiCells,
+                     uint32_t* indexOut, hipStream_t st);
 
 // Inputs of the device-resident path loop (qsmi_sim.hip). Device pointers
 // are indexed by path position after the hard data (i in [0, nPath)).
@@ -159,6 +162,7 @@ struct ResidentLoop {
   std::vector<int> waveStart;  // wave w = order[waveStart[w] .. waveStart[w+1])
   bool stageTiming = false;
   bool noVerbatim = false;     // -nV: no candidate at the continuation of the nearest neighbour's copy
+  std::vector<qsmi_device*> extraDevs;  // TIs 1.. of a several-TI run (same extent as this device's)
   const int* dOrder;
   const unsigned* dPathCells;
   const float* dPathSeeds;

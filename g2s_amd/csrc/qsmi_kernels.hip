@@ -515,6 +515,57 @@ __global__ void __launch_bounds__(256) k_merge_draw(ScanGeom g, const NodeInfo* 
   }
 }
 
+// Several TIs: per node, the TIs' K best lists (val/pos [t][node][K], TI
+// positions) merged as QuantileSamplingModule::sample does
+// (include/quantileSamplingModule.hpp:90-126: every TI's k best sorted
+// together by value, rank floor(seed k ceil(nTI/k) / nTI)), the device rule
+// ordering them with positions encoded as position * nTI + TI. The rank is
+// clamped to the last valid candidate. Writes the value and importDataIndex.
+// One thread per node.
+__global__ void k_merge_ti(const NodeInfo* __restrict__ info, int nN, int nTI, int K, float kf,
+                           const float* __restrict__ val, const uint32_t* __restrict__ pos, long long stride,
+                           float quantum, float* __restrict__ simVals, const float* __restrict__ tiVals, int nVar,
+                           uint32_t tiCells, uint32_t* __restrict__ indexOut) {
+  const int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= nN) return;
+  const NodeInfo ni = info[node];
+  if (ni.nb == 0) return;  // drawn from the marginal by the gather stage
+  int r = (int)floorf(ni.seed * kf * (ceilf((float)nTI / kf) / (float)nTI));
+  if (r < 0) r = 0;
+  Cand prev = worst_cand(), pick = worst_cand();
+  bool any = false;
+  for (int q = 0; q <= r; ++q) {
+    Cand best = worst_cand();
+    bool found = false;
+    for (int t = 0; t < nTI; ++t)
+      for (int i = 0; i < K; ++i) {
+        const long long o = t * stride + (long long)node * K + i;
+        const uint32_t p = pos[o];
+        const float v = val[o];
+        if (p == 0xFFFFFFFFu || !(v > -INFINITY)) continue;
+        Cand c;
+        c.val = v;
+        c.key = (quantum > 0.f) ? rintf(v * (1.f / quantum)) : v;
+        c.pos = p * (uint32_t)nTI + (uint32_t)t;
+        c.h = tie_hash(ni.seedbits, c.pos);
+        if (q > 0 && !cand_better(prev, c)) continue;
+        if (!found || cand_better(c, best)) {
+          best = c;
+          found = true;
+        }
+      }
+    if (!found) break;
+    prev = best;
+    pick = best;
+    any = true;
+  }
+  if (!any) return;
+  const uint32_t t = pick.pos % (uint32_t)nTI, p = pick.pos / (uint32_t)nTI;
+  for (int v = 0; v < nVar; ++v)
+    simVals[(long long)ni.cell * nVar + v] = tiVals[((long long)t * tiCells + p) * nVar + v];
+  indexOut[ni.cell] = pick.pos;
+}
+
 __global__ void k_fill(float* __restrict__ p, long long n, float v) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -678,6 +729,13 @@ void launch_fill_disk(float* map, const int* s, uint32_t o, float r2, float v, h
   }
   hipLaunchKernelGGL(k_fill_disk, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, map, s[0], s[1], o, r2, bl[0],
                      bl[1], bl[2], bn[0], bn[1], total, v);
+}
+
+void launch_merge_ti(const NodeInfo* info, int nN, int nTI, int K, float kf, const float* val, const uint32_t* pos,
+                     long long stride, float quantum, float* simVals, const float* tiVals, int nVar, uint32_t tiCells,
+                     uint32_t* indexOut, hipStream_t st) {
+  hipLaunchKernelGGL(k_merge_ti, dim3((unsigned)((nN + 63) / 64)), dim3(64), 0, st, info, nN, nTI, K, kf, val, pos,
+                     stride, quantum, simVals, tiVals, nVar, tiCells, indexOut);
 }
 
 void launch_fill(float* p, long long n, float v, hipStream_t st) {

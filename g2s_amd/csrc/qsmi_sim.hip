@@ -42,6 +42,7 @@ struct GatherParams {
   unsigned tiCells;
   int noVerbatim;  // 1: exclude the position that continues the nearest neighbour's copy (-nV)
   unsigned f[3];   // TI FFT grid (the reference's index space)
+  int nTI;         // training images (tiVals holds them one after the other)
 };
 
 __device__ __forceinline__ uint32_t seed_bits_dev(float seed) {
@@ -84,7 +85,8 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
   if (cnt <= 1) {
     // only the cell itself: draw from the marginal (simulation.hpp:586-606)
     if (lane == 0) {
-      unsigned position = (unsigned)(int)floorf(seed * (float)P.tiCells);
+      // over all TIs in order: TI = position / cells (one extent for all)
+      unsigned position = (unsigned)(int)floorf(seed * (float)(P.tiCells * (unsigned)P.nTI));
       if (P.cross && !centreOk[position]) {
         // incomplete cell: rank among the complete cells (simulation.hpp:
         // 617-654; rank 0 keeps cell 0 as the reference's walk does)
@@ -92,7 +94,7 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
         position = r == 0 ? 0u : complete[r - 1];
       }
       for (int v = 0; v < P.nVar; ++v) di[(long long)cell * P.nVar + v] = tiVals[(long long)position * P.nVar + v];
-      index[cell] = position;
+      index[cell] = (position % P.tiCells) * (unsigned)P.nTI + position / P.tiCells;
       ni.nb = 0;
       ni.delta0 = 0.f;
       for (int d = 0; d < 3; ++d) {
@@ -402,6 +404,10 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   P.scale = dev->trueMismatch ? (float)(1.0 / (double)t->N) : 1.f;
   P.quantum = dev->quantum;
   P.tiCells = (unsigned)t->S;
+  const int nTI = 1 + (int)R.extraDevs.size();
+  P.nTI = nTI;
+  if (nTI > 1 && (R.lp || dev->cross || R.noVerbatim || R.recCap > 0))
+    return fail(QSMI_ERR_UNSUPPORTED, "several TIs: not with the learned proposal, NaN TIs, -nV or the record");
 
   const int K = (int)std::ceil(R.k);
   int maxWave = 0;
@@ -416,6 +422,36 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   if (R.lp && (rc = lp_prepare(R.lp, t, std::max(maxWave, 1), K, R.lpShortlist, dev->active.data(),
                                (int)dev->active.size(), dev->circular)))
     return rc;
+  // several TIs: every extra device works on this device's stream and
+  // wave inputs; per-TI K best lists [t][node][K] for the cross-TI merge
+  std::vector<hipStream_t> keepStreams;
+  float* dValAll = nullptr;
+  uint32_t* dPosAll = nullptr;
+  const long long tiStride = (long long)std::max(maxWave, 1) * K;
+  if (nTI > 1) {
+    for (qsmi_device* x : R.extraDevs) {
+      if (!x->wb) x->wb = new WaveBufs();
+      if ((rc = grow_wave_bufs(x, x->wb, std::max(maxWave, 1), std::max(maxWave, 1) * R.n, K))) return rc;
+      if (x->ti->fast1k && (rc = ensure_tap_tables(x, std::max(maxWave, 1)))) return rc;
+    }
+    if (hipMalloc(&dValAll, (size_t)nTI * tiStride * sizeof(float)) != hipSuccess ||
+        hipMalloc(&dPosAll, (size_t)nTI * tiStride * sizeof(uint32_t)) != hipSuccess) {
+      (void)hipFree(dValAll);
+      return fail(QSMI_ERR_NOMEM, "several TIs: candidate lists");
+    }
+    for (qsmi_device* x : R.extraDevs) {
+      keepStreams.push_back(x->stream);
+      x->stream = st;
+    }
+  }
+  auto restore = [&]() {
+    for (size_t i = 0; i < keepStreams.size(); ++i) R.extraDevs[i]->stream = keepStreams[i];
+    keepStreams.clear();
+    if (dValAll) (void)hipFree(dValAll);
+    if (dPosAll) (void)hipFree(dPosAll);
+    dValAll = nullptr;
+    dPosAll = nullptr;
+  };
   ScanGeom g = geom_of(dev);
   // rows stage: the cross form puts one row (both layers) in each FFT
   const int rpb = dev->cross ? (1 << t->lognv[0]) : rows_per_block(t);
@@ -466,6 +502,24 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
       if (ev) (void)hipEventRecord(ev[3], st);
       launch_merge_draw(g, b->dInfo, lp_cand(R.lp), nN, 1, 0, K, R.k, b->dOutVal, b->dOutPos, nullptr, R.dDi, R.dTi,
                         R.nVar, R.dIndex, st);
+    } else if (nTI > 1) {
+      // every TI's scan and K best, then the merge across TIs and the draw
+      for (int tt = 0; tt < nTI; ++tt) {
+        qsmi_device* dv = tt == 0 ? dev : R.extraDevs[tt - 1];
+        WaveBufs* xb = tt == 0 ? b : dv->wb;
+        const ScanGeom gt = geom_of(dv);
+        if ((rc = launch_spectrum(dv, gt, nN, b->dInfo, b->dOff, b->dVal, xb->dC))) {
+          restore();
+          return rc;
+        }
+        launch_rows(dv, gt, b->dInfo, xb->dC, nN, rb, K, xb->dCand, nullptr, nullptr, 0);
+        launch_merge_draw(gt, b->dInfo, xb->dCand, nN, rb, rpb, K, R.k, dValAll + tt * tiStride,
+                          dPosAll + tt * tiStride, nullptr, nullptr, nullptr, R.nVar, nullptr, st);
+      }
+      if (ev) (void)hipEventRecord(ev[2], st);
+      if (ev) (void)hipEventRecord(ev[3], st);
+      launch_merge_ti(b->dInfo, nN, nTI, K, R.k, dValAll, dPosAll, tiStride, dev->quantum, R.dDi, R.dTi, R.nVar,
+                      (uint32_t)t->S, R.dIndex, st);
     } else {
       if ((rc = launch_spectrum(dev, g, layers * nN, b->dInfo, b->dOff, b->dVal, b->dC))) return rc;
       if (ev) (void)hipEventRecord(ev[2], st);
@@ -483,8 +537,13 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   }
   dev->timing = savedTiming;
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(QSMI_ERR_HIP, std::string("resident waves: ") + hipGetErrorString(e));
+  if (e != hipSuccess) {
+    (void)hipStreamSynchronize(st);
+    restore();
+    return fail(QSMI_ERR_HIP, std::string("resident waves: ") + hipGetErrorString(e));
+  }
   e = hipStreamSynchronize(st);
+  restore();
   if (e != hipSuccess) return fail(QSMI_ERR_HIP, std::string("resident waves: ") + hipGetErrorString(e));
   double stage[4] = {0, 0, 0, 0};
   if (timing) {

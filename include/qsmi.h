@@ -276,6 +276,15 @@ typedef struct {
                               the TI position that continues the nearest
                               neighbour's copy (include/simulation.hpp:567-580);
                               not with the learned proposal */
+  /* several training images (qs -ti a b ...): nTI >= 2 takes tis[0..nTI-1],
+     each prod(tiSize) x nVar (one extent for all), and ignores ti. Per node
+     every TI's k best (QuantileSamplingModule::determineDistribution), merged
+     by value and drawn with rank floor(seed k ceil(nTI/k) / nTI)
+     (include/quantileSamplingModule.hpp:90-126); index = position * nTI + TI;
+     the marginal draw spans the TIs in order (include/simulation.hpp:586-606).
+     Not with NaN TIs, the learned proposal, the pair record or -nV. */
+  int nTI;
+  const float* const* tis;
 } qsmi_qs_request;
 
 typedef struct {

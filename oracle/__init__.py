@@ -63,6 +63,7 @@ class OrcSim(C.Structure):
         ("quantum", C.c_float),
         ("cross", C.c_int),
         ("noVerbatim", C.c_int),
+        ("nTI", C.c_int),
     ]
 
 
@@ -256,7 +257,7 @@ class Sim:
     """Serial QS simulation / per-node lockstep checker over the oracle."""
 
     def __init__(self, ti, tidims, types, ncat, catvals, simdims, kdims, kernel_exp, offs, n, k, path, seeds,
-                 begin, mode=0, quantum=0.0, cross=False, no_verbatim=False):
+                 begin, mode=0, quantum=0.0, cross=False, no_verbatim=False, n_ti=1):
         self._keep = []
 
         def keep(a):
@@ -291,6 +292,7 @@ class Sim:
         s.quantum = float(quantum)
         s.cross = int(bool(cross))
         s.noVerbatim = int(bool(no_verbatim))
+        s.nTI = int(n_ti)  # ti then holds the n_ti images one after the other
         self.spec = s
         self._index = None
         self.nvar = len(types)

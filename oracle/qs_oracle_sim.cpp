@@ -57,6 +57,8 @@ void orc_ti_spectra_f64(int D, const unsigned* f, int nSub, const float* imgs, d
 void orc_fkst(const float* data, unsigned N, unsigned short k, float seed, float* out, unsigned* pos);
 unsigned orc_index_to_position(int D, const unsigned* f, const unsigned* s, const int* mn, unsigned index);
 unsigned orc_position_to_index(int D, const unsigned* f, const unsigned* s, const int* mn, unsigned position);
+int orc_select_device(int D, const unsigned* s, const float* map, const int* lo, const int* hi, int K, float seed,
+                      float quantum, float* outVal, unsigned* outPos);
 int orc_index_with_delta(int D, const unsigned* dims, unsigned position, const int* delta, unsigned* location);
 unsigned orc_seed_from_unit_float(float seed);
 uint32_t orc_tie_hash(uint32_t seedbits, uint32_t position);
@@ -86,6 +88,7 @@ struct OrcSim {
   float quantum;  // tie quantum of the device rule (0: none)
   int cross;      // 1: TI with NaN, cross-measurement form (mode 0 only)
   int noVerbatim; // 1: -nV, no candidate at the continuation of neighbour 1's copy
+  int nTI;        // training images in `ti`, one after the other (<= 1: one; mode 0 only when > 1)
 };
 
 struct OrcSimCtx {
@@ -105,6 +108,8 @@ struct OrcSimCtx {
   std::vector<int> typesOrig;
   std::vector<unsigned> posterior;
   const unsigned* index = nullptr;  // importDataIndex state for -nV (orc_sim_set_index)
+  int nTI = 1;
+  std::vector<std::vector<float>> Yx;  // sub-images of TIs 1.. (several TIs)
 };
 
 void* orc_sim_create(const OrcSim* p) {
@@ -162,6 +167,13 @@ void* orc_sim_create(const OrcSim* p) {
     // unreflected sub-images on the TI grid
     c->Y.resize((size_t)c->nSub * c->S);
     orc_ti_subimages(p->D, c->s, c->s, p->nVar, p->types, p->ti, p->ncat, p->catvals, 0, c->Y.data());
+    c->nTI = p->nTI > 1 ? p->nTI : 1;
+    for (int t = 1; t < c->nTI; ++t) {
+      c->Yx.emplace_back((size_t)c->nSub * c->S);
+      orc_ti_subimages(p->D, c->s, c->s, p->nVar, p->types, p->ti + (size_t)t * c->S * p->nVar, p->ncat, p->catvals,
+                       0, c->Yx.back().data());
+    }
+    if (c->nTI > 1) c->p.mode = 0;
   }
   if (p->mode == 1 || p->mode == 2) {
     std::vector<float> refl((size_t)c->nSub * c->F);
@@ -351,6 +363,63 @@ static unsigned verbatim_index(const OrcSimCtx* c, unsigned pathIdx, const std::
 
 void orc_sim_set_index(void* h, const unsigned* index) { static_cast<OrcSimCtx*>(h)->index = index; }
 
+// Several TIs (include/quantileSamplingModule.hpp:90-126 with
+// determineDistribution :478-577): every TI's K best by the device rule on
+// its FP64 map, then all of them ordered together — value (or its tie key)
+// descending, then the hash and the position of the encoded index
+// position * nTI + TI — and the rank floor(seed k ceil(nTI/k) / nTI)
+// (float arithmetic) taken, clamped to the last valid entry. Returns the
+// encoded index; outVal/outPos get the merged K best (encoded).
+static unsigned node_multi_ti(OrcSimCtx* c, int nb, const std::vector<int>& off, const std::vector<float>& u,
+                              float delta0, float seed, int K, float* outVal, unsigned* outPos) {
+  const OrcSim& p = c->p;
+  const int D = c->D;
+  const unsigned nT = (unsigned)c->nTI;
+  const uint32_t sb = orc_seed_from_unit_float(seed);
+  struct Cd {
+    float key;
+    uint32_t h, pos;
+    float val;
+  };
+  auto better = [](const Cd& a, const Cd& b) {
+    if (a.key != b.key) return a.key > b.key;
+    if (a.h != b.h) return a.h > b.h;
+    return a.pos < b.pos;
+  };
+  std::vector<Cd> all;
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int i = 0; i < D; ++i) hi[i] = int(c->s[i]) - 1;
+  for (unsigned t = 0; t < nT; ++t) {
+    std::vector<double> m64(c->S);
+    orc_map_direct_f64(D, c->s, c->nSub, t == 0 ? c->Y.data() : c->Yx[t - 1].data(), c->alpha.data(), nb, off.data(),
+                       u.data(), (double)delta0, 0, m64.data());
+    std::vector<float> m(c->S);
+    for (size_t q = 0; q < c->S; ++q) m[q] = (float)m64[q];
+    std::vector<float> v(K);
+    std::vector<unsigned> ps(K);
+    orc_select_device(D, c->s, m.data(), lo, hi, K, seed, p.quantum, v.data(), ps.data());
+    for (int i = 0; i < K; ++i) {
+      if (ps[i] == UINT_MAX || !(v[i] > -INFINITY)) continue;
+      Cd cd;
+      cd.val = v[i];
+      cd.key = p.quantum > 0.f ? std::rint(v[i] * (1.f / p.quantum)) : v[i];
+      cd.pos = ps[i] * nT + t;
+      cd.h = orc_tie_hash(sb, cd.pos);
+      all.push_back(cd);
+    }
+  }
+  std::sort(all.begin(), all.end(), better);
+  for (int i = 0; i < K; ++i) {
+    outVal[i] = i < (int)all.size() ? all[i].val : -INFINITY;
+    outPos[i] = i < (int)all.size() ? all[i].pos : UINT_MAX;
+  }
+  if (all.empty()) return UINT_MAX;
+  int r = (int)std::floor(seed * p.k * (std::ceil((float)nT / p.k) / (float)nT));
+  if (r < 0) r = 0;
+  if (r > (int)all.size() - 1) r = (int)all.size() - 1;
+  return all[r].pos;
+}
+
 unsigned orc_sim_node(void* h, const float* di, unsigned pathIdx, int K, float* outVal, unsigned* outPos,
                       int* nbOut) {
   auto* c = static_cast<OrcSimCtx*>(h);
@@ -367,7 +436,12 @@ unsigned orc_sim_node(void* h, const float* di, unsigned pathIdx, int K, float* 
   int nb = node_inputs(c, di, pathIdx, off, u, &delta0, nullptr);
   *nbOut = nb;
   if (nb <= 1) {
-    // marginal draw: position = floor(seed * cells) in float
+    // marginal draw: position = floor(seed * cells) in float, over all TIs in
+    // order (include/simulation.hpp:586-606); several TIs: the encoded index
+    if (c->nTI > 1) {
+      const unsigned q = (unsigned)int(std::floor(seed * (float)((unsigned)c->S * (unsigned)c->nTI)));
+      return (q % (unsigned)c->S) * (unsigned)c->nTI + q / (unsigned)c->S;
+    }
     unsigned position = (unsigned)int(std::floor(seed * (float)(unsigned)c->S));
     if (p.cross && !c->centreOk[position]) {
       // simulation.hpp:617-654: rank among the complete cells, the first
@@ -383,6 +457,7 @@ unsigned orc_sim_node(void* h, const float* di, unsigned pathIdx, int K, float* 
     }
     return position;
   }
+  if (c->nTI > 1) return node_multi_ti(c, nb, off, u, delta0, seed, K, outVal, outPos);
   if (p.mode == 0) {
     std::vector<double> map(c->S);
     if (p.cross)
@@ -518,9 +593,10 @@ void orc_sim_run(void* h, float* di, unsigned* idxOut) {
     int nb;
     unsigned tpos = orc_sim_node(h, di, i, K, v.data(), pos.data(), &nb);
     if (tpos == UINT_MAX) continue;
-    idxOut[cell] = tpos;  // importDataIndex = index*nTI + TI with nTI = 1
+    idxOut[cell] = tpos;  // importDataIndex = index*nTI + TI
     const float* tiv = p.cross ? c->tiX.data() : p.ti;
-    for (int q = 0; q < p.nVar; ++q) di[(size_t)cell * p.nVar + q] = tiv[(size_t)tpos * p.nVar + q];
+    const size_t src = (size_t)(tpos % (unsigned)c->nTI) * c->S + tpos / (unsigned)c->nTI;
+    for (int q = 0; q < p.nVar; ++q) di[(size_t)cell * p.nVar + q] = tiv[src * p.nVar + q];
   }
   c->index = keep;
 }

@@ -255,3 +255,52 @@ def test_sim_nan_ti_two_continuous_hard_data():
     di[pick] = tiok[rng.integers(0, tiok.shape[0], size=pick.size)]
     _log("two continuous variables, TI with NaN, hard data")
     lockstep(ti.reshape(-1), dims, di.reshape(-1), sdims, [0, 0], 12, 1.5, 4, 0.0, cross=True)
+
+
+def test_sim_several_tis():
+    """Several TIs: lockstep with the oracle's merge across TIs (each TI's K
+    best, ordered together, rank floor(seed k ceil(nTI/k) / nTI)); a TI that
+    never matches changes nothing but the index encoding; three copies of a
+    TI at k = 1.5 give the one-TI realisation at k = 1."""
+    ga = synth.grf((40, 44), 4.0, seed=41)
+    gb = synth.grf((40, 44), 4.0, seed=42)
+    a, dims = synth.to_g2s(np.exp(ga))
+    bb, _ = synth.to_g2s(np.exp(gb))
+    a, bb = a.reshape(-1), bb.reshape(-1)
+    sdims = (36, 30)
+    cells = int(np.prod(sdims))
+    rng = np.random.default_rng(6)
+    di = np.full(cells, np.nan, np.float32)
+    pick = rng.choice(cells, cells // 20, replace=False)
+    di[pick] = a[rng.integers(0, a.size, pick.size)]
+    _log("several TIs: 2 x 40x44 continuous, sim 36x30, 5 % hard data, n=12, k=1.5")
+    sim, idx, st = g2s_amd.qs([a, bb], dims, di, sdims, [0], 12, 1.5, 4, wave_size=64)
+    kd = [d // 2 + 1 for d in dims]
+    offs = oracle.offsets(kd)
+    informed = (~np.isnan(di)).astype(np.uint8)
+    p, s_, b = oracle.path(informed, 4)
+    orc = oracle.Sim(np.concatenate([a, bb]), dims, [0], [], [], sdims, kd, np.ones(int(np.prod(kd)), np.float32),
+                     offs, 12, 1.5, p, s_, b, n_ti=2)
+    free = p[b:]
+    both = np.concatenate([a, bb])
+    assert np.array_equal(sim[free], both[(idx[free] % 2) * a.size + idx[free] // 2])
+    mism = sum(int(orc.node(sim, i)[0] != idx[p[b + i]]) for i in range(len(free)))
+    used = np.bincount(idx[free] % 2, minlength=2)
+    _log(f"  lockstep mismatches {mism} of {len(free)}; nodes per TI {used.tolist()}")
+    assert mism <= len(free) // 100
+    assert used.min() > 0
+    far = np.full_like(a, 1e4)
+    s1, i1, _ = g2s_amd.qs(a, dims, di, sdims, [0], 12, 1.5, 4, wave_size=64)
+    # (the generic kernels sum taps that share a row with LDS atomics, so two
+    # runs may differ in the last bits; a near tie may then flip a node)
+    for order, t in (([a, far], 0), ([far, a], 1)):
+        s2, i2, _ = g2s_amd.qs(order, dims, di, sdims, [0], 12, 1.5, 4, wave_size=64)
+        assert np.all(i2[free] % 2 == t)
+        same2 = float(np.mean(i2[free] == i1[free] * 2 + t))
+        _log(f"  far TI at {t}: {same2:.4f} of the nodes identical to the one-TI run")
+        assert same2 >= 0.98
+    sk1, ik1, _ = g2s_amd.qs(a, dims, di, sdims, [0], 12, 1.0, 4, wave_size=64)
+    s3, i3, _ = g2s_amd.qs([a, a, a], dims, di, sdims, [0], 12, 1.5, 4, wave_size=64)
+    same = float(np.mean(i3[free] // 3 == ik1[free]))
+    _log(f"  three copies at k=1.5 vs one at k=1: {same:.4f} of the nodes identical")
+    assert same >= 0.98

@@ -134,3 +134,56 @@ def test_no_verbatim(hard):
     hits = sum(idxp[p[b + i]] == v for i, v in vbp.items())
     assert hits > 0
     assert not np.array_equal(idx0, idxp)
+
+
+def _sim_multi(tis, dims, sdims, di, n, k, seed):
+    kd = [d // 2 + 1 for d in dims]
+    offs = oracle.offsets(kd)
+    informed = (~np.isnan(di)).astype(np.uint8)
+    p, s, b = oracle.path(informed, seed)
+    cat = np.concatenate([np.asarray(t, np.float32).reshape(-1) for t in tis])
+    sim = oracle.Sim(cat, dims, [0], [], [], sdims, kd, np.ones(int(np.prod(kd)), np.float32), offs, n, k, p, s, b,
+                     n_ti=len(tis))
+    return sim, p, s, b
+
+
+def test_several_tis():
+    """Several TIs (include/quantileSamplingModule.hpp:90-126,
+    include/simulation.hpp:586-606): a second TI that can never match leaves
+    the realisation unchanged, with index = position * nTI + TI (in either
+    order); three copies of a TI at k = 1.5 take rank floor(seed) = 0, i.e.
+    the one-TI realisation at k = 1; the marginal draw spans the TIs in
+    order."""
+    g = synth.grf((30, 34), 3.0, seed=3)
+    a, dims = synth.to_g2s(np.exp(g))
+    a = a.reshape(-1)
+    far = np.full_like(a, 1e4)
+    sdims = (20, 18)
+    cells = int(np.prod(sdims))
+    rng = np.random.default_rng(3)
+    di = np.full(cells, np.nan, np.float32)
+    pick = rng.choice(cells, cells // 20, replace=False)
+    di[pick] = a[rng.integers(0, a.size, pick.size)]
+    one, p, s, b = _sim_multi([a], dims, sdims, di, 9, 1.5, 3)
+    out1, idx1 = one.run(di)
+    free = p[b:]
+    for order, t in (([a, far], 0), ([far, a], 1)):
+        two, *_ = _sim_multi(order, dims, sdims, di, 9, 1.5, 3)
+        out2, idx2 = two.run(di)
+        assert np.array_equal(out2, out1)
+        assert np.array_equal(idx2[free], idx1[free] * 2 + t)
+    k1, *_ = _sim_multi([a], dims, sdims, di, 9, 1.0, 3)
+    outk1, idxk1 = k1.run(di)
+    three, *_ = _sim_multi([a, a, a], dims, sdims, di, 9, 1.5, 3)
+    out3, idx3 = three.run(di)
+    assert np.array_equal(out3, outk1)
+    assert np.array_equal(idx3[free] // 3, idxk1[free])
+    assert len(set((idx3[free] % 3).tolist())) == 3  # the copies all get picked
+    # marginal draw over the TIs in order: the first path node of an
+    # unconditional run has only the centre
+    di0 = np.full(cells, np.nan, np.float32)
+    two, p0, s0, b0 = _sim_multi([a, far], dims, sdims, di0, 9, 1.5, 5)
+    chosen, _, _, nb = two.node(di0, 0)
+    assert nb == 1
+    q = int(np.floor(np.float32(s0[b0]) * np.float32(2 * a.size)))
+    assert chosen == (q % a.size) * 2 + q // a.size
