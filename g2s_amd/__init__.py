@@ -303,7 +303,8 @@ class Wave:
 
 def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dims=None, gpu=0,
             wave_size=256, tie_quantum=-1.0, stage_timing=False, record=0, lp=None, lp_shortlist=0,
-            lp_fallback_every=0, lp_fallback_mse=0.0, lp_audit=False, lp_min_cover=0.0, no_verbatim=False):
+            lp_fallback_every=0, lp_fallback_mse=0.0, lp_audit=False, lp_min_cover=0.0, no_verbatim=False,
+            n_image=None, k_image=None):
     """In-memory QuickSampling (qsmi_qs_run_in_memory).
 
     Arrays are flat in G2S DataImage order (dimension 0 fastest, variable
@@ -315,7 +316,8 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
     lp: a LearnedProposal with keys built for this TI (learned-proposal
     form; the stats then carry its counters).
     ti may be a list of arrays of one extent (several training images, qs
-    -ti a b ...); index is then position * nTI + TI."""
+    -ti a b ...); index is then position * nTI + TI. n_image / k_image: per
+    simulation cell n and k (qs -ni / -kvi)."""
     tis = None
     if isinstance(ti, (list, tuple)):
         tis = [_f32(t).reshape(-1) for t in ti]
@@ -349,6 +351,13 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
     rq.tieQuantum = float(tie_quantum)
     rq.stageTiming = int(bool(stage_timing))
     rq.noVerbatim = int(bool(no_verbatim))
+    nimg = kimg = None
+    if n_image is not None:
+        nimg = _f32(n_image).reshape(-1)
+        rq.nImage = nimg.ctypes.data
+    if k_image is not None:
+        kimg = _f32(k_image).reshape(-1)
+        rq.kImage = kimg.ctypes.data
     tptr = None
     if tis is not None:
         tptr = (C.c_void_p * len(tis))(*[t.ctypes.data for t in tis])

@@ -186,6 +186,8 @@ class QsRequest(C.Structure):
         ("noVerbatim", C.c_int),
         ("nTI", C.c_int),
         ("tis", C.c_void_p),
+        ("nImage", C.c_void_p),
+        ("kImage", C.c_void_p),
     ]
 
 

@@ -273,7 +273,10 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   if (rq->nVar < 1 || !rq->varTypes || !rq->di || (nTI == 1 && !rq->ti) || (nTI > 1 && !rq->tis))
     return hfail(QSMI_ERR_ARG, "invalid variables");
   if (nTI > 8) return hfail(QSMI_ERR_UNSUPPORTED, "at most 8 training images");
-  if (rq->n < 1 || !(rq->k > 0.f) || std::ceil(rq->k) > 16.f) return hfail(QSMI_ERR_UNSUPPORTED, "need n >= 1, 0 < k <= 16");
+  if ((!rq->nImage && rq->n < 1) || !(rq->k > 0.f) || std::ceil(rq->k) > 16.f)
+    return hfail(QSMI_ERR_UNSUPPORTED, "need n >= 1, 0 < k <= 16");
+  if ((rq->nImage || rq->kImage) && (rq->lp || rq->recordCap > 0))
+    return hfail(QSMI_ERR_UNSUPPORTED, "n / k images: not with the learned proposal or the record");
   const auto t0 = clk::now();
   const int nVar = rq->nVar;
   unsigned s[3] = {1, 1, 1}, sd[3] = {1, 1, 1};
@@ -283,6 +286,21 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     sd[i] = rq->simSize[i];
     S *= s[i];
     SD *= sd[i];
+  }
+  // n and k images (include/simulation.hpp:337-357): the largest n sizes
+  // the neighbour lists, the largest k the candidate lists
+  int nMax = rq->n;
+  float kMax = rq->k;
+  if (rq->nImage) {
+    nMax = 1;
+    for (size_t c = 0; c < SD; ++c)
+      if (std::isfinite(rq->nImage[c]) && rq->nImage[c] < 1e6f) nMax = std::max(nMax, (int)rq->nImage[c]);
+    if (nMax > 4096) return hfail(QSMI_ERR_UNSUPPORTED, "n image: values above 4096");
+  }
+  if (rq->kImage) {
+    for (size_t c = 0; c < SD; ++c)
+      if (rq->kImage[c] > kMax) kMax = rq->kImage[c];
+    if (!(std::ceil(kMax) <= 16.f)) return hfail(QSMI_ERR_UNSUPPORTED, "k image: values above 16");
   }
   // several TIs: one array, TI after TI (the order qs walks them in for
   // categories, src/qs.cpp:1498-1525, and the marginal draw)
@@ -439,7 +457,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   if (quantum < 0.f) {
     quantum = 0.f;
     if (!anyCont && maxw > 0.f) {
-      int e2 = (int)std::ceil(std::log2((double)rq->n * (double)maxw));
+      int e2 = (int)std::ceil(std::log2((double)nMax * (double)maxw));
       quantum = std::ldexp(1.f, e2 - 16);
     }
   }
@@ -468,7 +486,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   };
   // path index of every cell, -1 for hard data (the posterior order of
   // src/qs.cpp:1615-1641: hard data first, then the path)
-  const int n = rq->n;
+  const int n = nMax;
   std::vector<int> postS(SD, -1);
   for (size_t i = 0; i < nPath; ++i) postS[path[begin + i]] = (int)i;
   // neighbour sets of every node (simulation.hpp:447-511), centre first, on
@@ -480,9 +498,23 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   if (!rc) rc = dev_upload(nullptr, nPath * (size_t)n * sizeof(int), &dNb);
   if (!rc) rc = dev_upload(nullptr, nPath * (size_t)n * sizeof(int), &dPj);
   if (!rc) rc = dev_upload(nullptr, nPath * sizeof(int), &dCnt);
+  void *dNNode = nullptr, *dPathK = nullptr;
+  if (rq->nImage) {
+    std::vector<int> nNode(nPath);
+    for (size_t i = 0; i < nPath; ++i) {
+      const float v = rq->nImage[path[begin + i]];
+      nNode[i] = (std::isfinite(v) && v > 0.f && v < 1e6f) ? (int)v : 0;
+    }
+    if (!rc) rc = dev_upload(nNode.data(), nPath * sizeof(int), &dNNode);
+  }
+  if (rq->kImage) {
+    std::vector<float> kNode(nPath);
+    for (size_t i = 0; i < nPath; ++i) kNode[i] = rq->kImage[path[begin + i]];
+    if (!rc) rc = dev_upload(kNode.data(), nPath * sizeof(float), &dPathK);
+  }
   if (!rc)
     rc = qsmi::nb_search_device(D, sd, nPath, (const unsigned*)dCells, (const int*)dPostS, (const int*)dOffs, nOff, n,
-                                (int*)dNb, (int*)dPj, (int*)dCnt, dev->stream);
+                                (int*)dNb, (int*)dPj, (int*)dCnt, dev->stream, (const int*)dNNode);
   // learned proposal: which nodes reach the stencil coverage it needs
   const float lpMinCover = rq->lpMinCover > 0.f ? rq->lpMinCover : 0.5f;
   std::vector<unsigned char> lpOk;
@@ -542,6 +574,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   R.nExp = nExp;
   R.n = n;
   R.k = rq->k;
+  R.kMax = kMax;
   for (int v = 0; v < nVar && v < 8; ++v) R.types[v] = rq->varTypes[v];
   {
     int e = 0, ci = 0, off = 0;
@@ -623,6 +656,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     R.dIndex = (uint32_t*)dIdx;
     R.dTi = (const float*)dTi;
     R.extraDevs = xdev;
+    R.dPathK = (const float*)dPathK;
     rc = qsmi::run_resident_waves(dev, R, stageMs, &scanned, &waves);
   }
   if (!rc) {

@@ -163,6 +163,8 @@ struct ResidentLoop {
   bool stageTiming = false;
   bool noVerbatim = false;     // -nV: no candidate at the continuation of the nearest neighbour's copy
   std::vector<qsmi_device*> extraDevs;  // TIs 1.. of a several-TI run (same extent as this device's)
+  const float* dPathK = nullptr;        // per path node k from a k image (nullptr: the run's k)
+  float kMax = 0.f;                     // largest k of the run (candidate list length ceil(kMax))
   const int* dOrder;
   const unsigned* dPathCells;
   const float* dPathSeeds;
@@ -213,7 +215,8 @@ void lp_record(int nN, int w0, const int* order, const NodeInfo* info, const int
 
 // neighbour sets of every path node on the device (stream-ordered)
 int nb_search_device(int D, const unsigned* sd, size_t nPath, const unsigned* dCells, const int* dPostS,
-                     const int* dOffs, int nOff, int n, int* dNbIdx, int* dNbPj, int* dNbCnt, hipStream_t st);
+                     const int* dOffs, int nOff, int n, int* dNbIdx, int* dNbPj, int* dNbCnt, hipStream_t st,
+                     const int* dNNode = nullptr);
 
 // stageMs: [0] spectrum (+ dim1 inverse), [1] gather, [2] rows + select,
 // [3] merge + draw + write; filled when R.stageTiming.

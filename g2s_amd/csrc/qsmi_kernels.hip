@@ -500,8 +500,11 @@ __global__ void __launch_bounds__(256) k_merge_draw(ScanGeom g, const NodeInfo* 
     outPos[(long long)node * K + threadIdx.x] = best[threadIdx.x].pos;
   }
   if (threadIdx.x == 0 && (chosen || simVals)) {
-    // QuantileSamplingModule::sample rank, one TI: floor(seed * k * (ceil(1/k)/1))
-    int r = (int)floorf(ni.seed * kf * (ceilf(1.f / kf) / 1.f));
+    // QuantileSamplingModule::sample rank, one TI: floor(seed * k * (ceil(1/k)/1)),
+    // k from the node's k image value when given (include/simulation.hpp:355-357)
+    const float kk = ni.kLocal > 0.f ? ni.kLocal : kf;
+    int r = (int)floorf(ni.seed * kk * (ceilf(1.f / kk) / 1.f));
+    if (r > (int)ceilf(kk) - 1) r = (int)ceilf(kk) - 1;  // the node's list holds ceil(k)
     if (r > K - 1) r = K - 1;
     if (r < 0) r = 0;
     const uint32_t p = best[r].pos;
@@ -530,7 +533,8 @@ __global__ void k_merge_ti(const NodeInfo* __restrict__ info, int nN, int nTI, i
   if (node >= nN) return;
   const NodeInfo ni = info[node];
   if (ni.nb == 0) return;  // drawn from the marginal by the gather stage
-  int r = (int)floorf(ni.seed * kf * (ceilf((float)nTI / kf) / (float)nTI));
+  const float kk = ni.kLocal > 0.f ? ni.kLocal : kf;
+  int r = (int)floorf(ni.seed * kk * (ceilf((float)nTI / kk) / (float)nTI));
   if (r < 0) r = 0;
   Cand prev = worst_cand(), pick = worst_cand();
   bool any = false;

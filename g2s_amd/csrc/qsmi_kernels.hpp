@@ -53,6 +53,7 @@ struct NodeInfo {
   int exOn;           // 1: no candidate within the disk below (verbatim exclusion)
   uint32_t exPos;     // disk centre, TI position
   float exR2;         // squared radius: positions p with sum_d (p_d - o_d)^2 <= exR2
+  float kLocal;       // this node's k from a k image (0: the run's k)
 };
 
 // Verbatim exclusion test of TI position (c0, c1, c2) for a node.

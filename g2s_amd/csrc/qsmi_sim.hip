@@ -63,7 +63,8 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
                                                const float* __restrict__ tiVals, NodeInfo* __restrict__ info,
                                                int* __restrict__ nbOff, float* __restrict__ nbVal,
                                                const uint8_t* __restrict__ centreOk,
-                                               const uint32_t* __restrict__ complete, uint32_t nComplete) {
+                                               const uint32_t* __restrict__ complete, uint32_t nComplete,
+                                               const float* __restrict__ pathK) {
   const int q = blockIdx.x;
   const int lane = threadIdx.x;
   const int i = order[w0 + q];
@@ -82,6 +83,7 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
   ni.exOn = 0;
   ni.exPos = 0;
   ni.exR2 = 0.f;
+  ni.kLocal = pathK ? pathK[i] : 0.f;
   if (cnt <= 1) {
     // only the cell itself: draw from the marginal (simulation.hpp:586-606)
     if (lane == 0) {
@@ -304,7 +306,7 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
 __global__ void __launch_bounds__(64) k_nbsearch(int D, int sd0, int sd1, int sd2, const unsigned* __restrict__ cells,
                                                 const int* __restrict__ postS, const int* __restrict__ offs, int nOff,
                                                 int n, int* __restrict__ nbIdx, int* __restrict__ nbPj,
-                                                int* __restrict__ nbCnt) {
+                                                int* __restrict__ nbCnt, const int* __restrict__ nNode) {
   const long long i = blockIdx.x;
   const int lane = threadIdx.x;
   const int sd[3] = {sd0, sd1, sd2};
@@ -318,8 +320,10 @@ __global__ void __launch_bounds__(64) k_nbsearch(int D, int sd0, int sd1, int sd
   }
   int* out = nbIdx + i * n;
   int* outP = nbPj + i * n;
+  // this node's count from an n image (include/simulation.hpp:337-343), at most n
+  const int nI = nNode ? min(max(nNode[i], 0), n) : n;
   int cnt = 0;
-  for (int base = 0; base < nOff && cnt < n; base += 64) {
+  for (int base = 0; base < nOff && cnt < nI; base += 64) {
     const int o = base + lane;
     bool hit = false;
     int pj = -1;
@@ -339,20 +343,21 @@ __global__ void __launch_bounds__(64) k_nbsearch(int D, int sd0, int sd1, int sd
     }
     const unsigned long long m = __ballot(hit);
     const int before = __popcll(m & ((1ull << lane) - 1ull));
-    if (hit && cnt + before < n) {
+    if (hit && cnt + before < nI) {
       out[cnt + before] = o;
       outP[cnt + before] = pj;
     }
     cnt += __popcll(m);
   }
-  if (lane == 0) nbCnt[i] = min(cnt, n);
+  if (lane == 0) nbCnt[i] = min(cnt, nI);
 }
 
 int nb_search_device(int D, const unsigned* sd, size_t nPath, const unsigned* dCells, const int* dPostS,
-                     const int* dOffs, int nOff, int n, int* dNbIdx, int* dNbPj, int* dNbCnt, hipStream_t st) {
+                     const int* dOffs, int nOff, int n, int* dNbIdx, int* dNbPj, int* dNbCnt, hipStream_t st,
+                     const int* dNNode) {
   if (nPath == 0) return QSMI_OK;
   hipLaunchKernelGGL(k_nbsearch, dim3((unsigned)nPath), dim3(64), 0, st, D, (int)sd[0], (int)sd[1], (int)sd[2], dCells,
-                     dPostS, dOffs, nOff, n, dNbIdx, dNbPj, dNbCnt);
+                     dPostS, dOffs, nOff, n, dNbIdx, dNbPj, dNbCnt, dNNode);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(QSMI_ERR_HIP, std::string("neighbour search: ") + hipGetErrorString(e));
   return QSMI_OK;
@@ -406,10 +411,12 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   P.tiCells = (unsigned)t->S;
   const int nTI = 1 + (int)R.extraDevs.size();
   P.nTI = nTI;
+  if (R.dPathK && (R.lp || R.recCap > 0))
+    return fail(QSMI_ERR_UNSUPPORTED, "k image: not with the learned proposal or the record");
   if (nTI > 1 && (R.lp || dev->cross || R.noVerbatim || R.recCap > 0))
     return fail(QSMI_ERR_UNSUPPORTED, "several TIs: not with the learned proposal, NaN TIs, -nV or the record");
 
-  const int K = (int)std::ceil(R.k);
+  const int K = (int)std::ceil(std::max(R.k, R.kMax));
   int maxWave = 0;
   for (size_t w = 0; w + 1 < R.waveStart.size(); ++w) maxWave = std::max(maxWave, R.waveStart[w + 1] - R.waveStart[w]);
   if (!dev->wb) dev->wb = new WaveBufs();
@@ -481,7 +488,7 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
     if (ev) (void)hipEventRecord(ev[0], st);
     hipLaunchKernelGGL(k_gather, dim3(nN), dim3(64), gatherLds, st, P, w0, nN, R.dOrder, R.dPathCells, R.dPathSeeds,
                        R.dNbIdx, R.dNbCnt, R.dOffs, R.dKernel, R.dCatvals, R.dDi, R.dIndex, R.dTi, b->dInfo, b->dOff,
-                       b->dVal, dev->dCentreOk, dev->dComplete, dev->nComplete);
+                       b->dVal, dev->dCentreOk, dev->dComplete, dev->nComplete, R.dPathK);
     if (ev) (void)hipEventRecord(ev[1], st);
     const bool useLp = R.lp && (R.waveLp.empty() || R.waveLp[w]);
     if (R.lp && !useLp) R.lpSkipped += nN;

@@ -285,6 +285,12 @@ typedef struct {
      Not with NaN TIs, the learned proposal, the pair record or -nV. */
   int nTI;
   const float* const* tis;
+  /* per simulation cell (prod(simSize) floats, NULL: none), as qs -ni / -kvi
+     (include/simulation.hpp:337-357): the cell's n (centre included; n
+     itself is then ignored) and its k (values <= 0: the run's k; ceil <= 16).
+     Not with the learned proposal or the pair record. */
+  const float* nImage;
+  const float* kImage;
 } qsmi_qs_request;
 
 typedef struct {

@@ -64,6 +64,8 @@ class OrcSim(C.Structure):
         ("cross", C.c_int),
         ("noVerbatim", C.c_int),
         ("nTI", C.c_int),
+        ("nImage", C.c_void_p),
+        ("kImage", C.c_void_p),
     ]
 
 
@@ -257,7 +259,8 @@ class Sim:
     """Serial QS simulation / per-node lockstep checker over the oracle."""
 
     def __init__(self, ti, tidims, types, ncat, catvals, simdims, kdims, kernel_exp, offs, n, k, path, seeds,
-                 begin, mode=0, quantum=0.0, cross=False, no_verbatim=False, n_ti=1):
+                 begin, mode=0, quantum=0.0, cross=False, no_verbatim=False, n_ti=1, n_image=None,
+                 k_image=None):
         self._keep = []
 
         def keep(a):
@@ -293,6 +296,10 @@ class Sim:
         s.cross = int(bool(cross))
         s.noVerbatim = int(bool(no_verbatim))
         s.nTI = int(n_ti)  # ti then holds the n_ti images one after the other
+        if n_image is not None:  # per cell n; n must be the largest value
+            s.nImage = keep(f32(n_image).reshape(-1))
+        if k_image is not None:
+            s.kImage = keep(f32(k_image).reshape(-1))
         self.spec = s
         self._index = None
         self.nvar = len(types)

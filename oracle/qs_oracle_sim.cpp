@@ -89,6 +89,8 @@ struct OrcSim {
   int cross;      // 1: TI with NaN, cross-measurement form (mode 0 only)
   int noVerbatim; // 1: -nV, no candidate at the continuation of neighbour 1's copy
   int nTI;        // training images in `ti`, one after the other (<= 1: one; mode 0 only when > 1)
+  const float* nImage;  // per simulation cell n (nullptr: n); n must then be the largest value
+  const float* kImage;  // per simulation cell k (nullptr, or values <= 0: k)
 };
 
 struct OrcSimCtx {
@@ -219,7 +221,13 @@ int orc_sim_gather(void* h, unsigned pathIdx, int* outIdx) {
     cc[i] = int(rem % c->sd[i]);
     rem /= c->sd[i];
   }
-  for (int o = 0; o < p.nOffsets && cnt < p.n; ++o) {
+  // include/simulation.hpp:337-343: the cell's n from the n image
+  int nCell = p.n;
+  if (p.nImage) {
+    const float v = p.nImage[cell];
+    nCell = (std::isfinite(v) && v > 0.f && v < 1e6f) ? std::min((int)v, p.n) : 0;
+  }
+  for (int o = 0; o < p.nOffsets && cnt < nCell; ++o) {
     bool inside = true;
     size_t t = 0, mul = 1;
     for (int i = 0; i < c->D; ++i) {
@@ -371,7 +379,7 @@ void orc_sim_set_index(void* h, const unsigned* index) { static_cast<OrcSimCtx*>
 // (float arithmetic) taken, clamped to the last valid entry. Returns the
 // encoded index; outVal/outPos get the merged K best (encoded).
 static unsigned node_multi_ti(OrcSimCtx* c, int nb, const std::vector<int>& off, const std::vector<float>& u,
-                              float delta0, float seed, int K, float* outVal, unsigned* outPos) {
+                              float delta0, float seed, int K, float kLoc, float* outVal, unsigned* outPos) {
   const OrcSim& p = c->p;
   const int D = c->D;
   const unsigned nT = (unsigned)c->nTI;
@@ -414,15 +422,40 @@ static unsigned node_multi_ti(OrcSimCtx* c, int nb, const std::vector<int>& off,
     outPos[i] = i < (int)all.size() ? all[i].pos : UINT_MAX;
   }
   if (all.empty()) return UINT_MAX;
-  int r = (int)std::floor(seed * p.k * (std::ceil((float)nT / p.k) / (float)nT));
+  int r = (int)std::floor(seed * kLoc * (std::ceil((float)nT / kLoc) / (float)nT));
   if (r < 0) r = 0;
   if (r > (int)all.size() - 1) r = (int)all.size() - 1;
   return all[r].pos;
 }
 
+static unsigned node_impl(OrcSimCtx* c, const float* di, unsigned pathIdx, int K, float kLoc, float* outVal,
+                          unsigned* outPos, int* nbOut);
+
 unsigned orc_sim_node(void* h, const float* di, unsigned pathIdx, int K, float* outVal, unsigned* outPos,
                       int* nbOut) {
   auto* c = static_cast<OrcSimCtx*>(h);
+  const OrcSim& p = c->p;
+  // include/simulation.hpp:353-357: the cell's k from the k image (values
+  // <= 0 leave the run's k, quantileSamplingModule.hpp:92-93); the k best
+  // lists hold ceil(k) entries, whatever K the caller reads back
+  float kLoc = p.k;
+  if (p.kImage) {
+    const float v = p.kImage[p.path[p.beginPath + pathIdx]];
+    if (v > 0.f) kLoc = v;
+  }
+  const int KL = (int)std::ceil(kLoc);
+  std::vector<float> v(KL);
+  std::vector<unsigned> ps(KL);
+  const unsigned r = node_impl(c, di, pathIdx, KL, kLoc, v.data(), ps.data(), nbOut);
+  for (int i = 0; i < K; ++i) {
+    outVal[i] = i < KL ? v[i] : -INFINITY;
+    outPos[i] = i < KL ? ps[i] : UINT_MAX;
+  }
+  return r;
+}
+
+static unsigned node_impl(OrcSimCtx* c, const float* di, unsigned pathIdx, int K, float kLoc, float* outVal,
+                          unsigned* outPos, int* nbOut) {
   const OrcSim& p = c->p;
   const int D = c->D;
   float seed = p.seeds[p.beginPath + pathIdx];
@@ -457,7 +490,7 @@ unsigned orc_sim_node(void* h, const float* di, unsigned pathIdx, int K, float* 
     }
     return position;
   }
-  if (c->nTI > 1) return node_multi_ti(c, nb, off, u, delta0, seed, K, outVal, outPos);
+  if (c->nTI > 1) return node_multi_ti(c, nb, off, u, delta0, seed, K, kLoc, outVal, outPos);
   if (p.mode == 0) {
     std::vector<double> map(c->S);
     if (p.cross)
@@ -517,7 +550,7 @@ unsigned orc_sim_node(void* h, const float* di, unsigned pathIdx, int K, float* 
       outPos[i] = (pos[i] == UINT_MAX) ? UINT_MAX : orc_index_to_position(D, c->f, c->s, mn, pos[i]);
   }
   // QSM::sample — sort the candidates (already best first), rank by the seed
-  unsigned r = orc_draw_rank(seed, p.k, 1);
+  unsigned r = orc_draw_rank(seed, kLoc, 1);
   if ((int)r >= K) r = K - 1;
   return outPos[r];
 }

@@ -32,7 +32,7 @@ def default_kernel(ti_dims, nexp):
 
 
 def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatch_frac=0.01, nodes=None,
-             weights=None, wave_size=64, threads=8, cross=False, no_verbatim=False):
+             weights=None, wave_size=64, threads=8, cross=False, no_verbatim=False, n_image=None, k_image=None):
     """nodes: path-relative node indices to check (None: every node);
     weights: one kernel weight per variable (None: the default uniform 1)."""
     types = list(types)
@@ -42,7 +42,7 @@ def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatc
         kd0 = [d // 2 + 1 for d in ti_dims]
         gkw = dict(kernel=np.tile(np.asarray(weights, np.float32), int(np.prod(kd0))), kernel_dims=kd0)
     sim, idx, st = g2s_amd.qs(ti, ti_dims, di0, sim_dims, types, n, k, seed, wave_size=wave_size,
-                              tie_quantum=quantum, no_verbatim=no_verbatim, **gkw)
+                              tie_quantum=quantum, no_verbatim=no_verbatim, n_image=n_image, k_image=k_image, **gkw)
     _log(f"  gpu run: {st}")
     # oracle set-up identical to the reference's qs.cpp
     ncat, catvals = [], []
@@ -67,7 +67,7 @@ def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatc
     gp, gs, gb = g2s_amd.path(informed, seed)
     assert np.array_equal(p, gp) and np.array_equal(s, gs) and b == gb
     orc = oracle.Sim(ti, ti_dims, types, ncat, catvals, sim_dims, kd, kern, offs, n, k, p, s, b, mode=0,
-                     quantum=quantum, cross=cross, no_verbatim=no_verbatim)
+                     quantum=quantum, cross=cross, no_verbatim=no_verbatim, n_image=n_image, k_image=k_image)
     orc.set_index(idx)  # -nV reads the positions the neighbours were copied from
     K = int(np.ceil(k))
     simf = sim.reshape(-1, nvar)
@@ -304,3 +304,24 @@ def test_sim_several_tis():
     same = float(np.mean(i3[free] // 3 == ik1[free]))
     _log(f"  three copies at k=1.5 vs one at k=1: {same:.4f} of the nodes identical")
     assert same >= 0.98
+
+
+@pytest.mark.parametrize("kind", ["continuous", "categorical"])
+def test_sim_n_and_k_images(kind):
+    """qs -ni / -kvi: per-cell n and k, lockstep with the oracle's serial loop."""
+    if kind == "continuous":
+        g = synth.grf((48, 56), 4.0, seed=51)
+        ti, dims = synth.to_g2s(np.exp(g))
+        types, q = [0], 0.0
+    else:
+        g = synth.grf((50, 50), 5.0, seed=52)
+        ti, dims = synth.to_g2s((g > np.quantile(g, 0.6)).astype(np.float32))
+        types, q = [1], float(2.0 ** (np.ceil(np.log2(20)) - 16))
+    sdims = (40, 36)
+    cells = int(np.prod(sdims))
+    rng = np.random.default_rng(7)
+    nimg = rng.choice([4, 9, 20], cells).astype(np.float32)
+    kimg = rng.choice([1.0, 2.0, 3.5, 0.0], cells).astype(np.float32)
+    di = np.full(cells, np.nan, np.float32)
+    _log(f"n / k images {kind}: n in (4, 9, 20), k in (1, 2, 3.5, run's 1.5)")
+    lockstep(ti, dims, di, sdims, types, 20, 1.5, 8, q, n_image=nimg, k_image=kimg)

@@ -187,3 +187,36 @@ def test_several_tis():
     assert nb == 1
     q = int(np.floor(np.float32(s0[b0]) * np.float32(2 * a.size)))
     assert chosen == (q % a.size) * 2 + q // a.size
+
+
+def test_n_and_k_images():
+    """qs -ni / -kvi (include/simulation.hpp:337-357): constant images equal
+    the plain run; a cell's neighbour list is the prefix of length n(cell) of
+    the list a larger n gives; a cell with k = 1 takes its best candidate, a
+    cell with k = 1.5 the rank floor(1.5 seed)."""
+    sim, ti, dims, sdims, di, p, s, b, offs, kd = setup(hard_frac=0.1, n=12)
+    cells = int(np.prod(sdims))
+    out0, idx0 = sim.run(di)
+
+    def make(n_image=None, k_image=None, n=12, k=1.5):
+        return oracle.Sim(ti, dims, [0], [], [], sdims, kd, np.ones(int(np.prod(kd)), np.float32), offs, n, k, p, s,
+                          b, n_image=n_image, k_image=k_image)
+
+    outc, idxc = make(np.full(cells, 12, np.float32), np.full(cells, 1.5, np.float32)).run(di)
+    assert np.array_equal(idxc, idx0) and np.array_equal(outc, out0)
+    rng = np.random.default_rng(4)
+    nimg = rng.choice([3, 7, 12], cells).astype(np.float32)
+    kimg = rng.choice([1.0, 1.5, 0.0], cells).astype(np.float32)  # 0: the run's k
+    mixed = make(nimg, kimg)
+    outm, idxm = mixed.run(di)
+    for i in range(0, len(p) - b, 5):
+        cell = p[b + i]
+        g_small, g_big = mixed.gather(i), sim.gather(i)
+        assert len(g_small) == min(int(nimg[cell]), len(g_big))
+        assert np.array_equal(g_small, g_big[: len(g_small)])
+        chosen, v, pos, nb = mixed.node(outm, i)
+        if nb > 1:
+            kk = 1.5 if kimg[cell] <= 0 else float(kimg[cell])
+            r = int(np.floor(np.float32(s[b + i]) * np.float32(kk)))
+            assert chosen == pos[min(r, int(np.ceil(kk)) - 1)]
+            assert chosen == idxm[cell]
