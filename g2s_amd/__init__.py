@@ -304,7 +304,7 @@ class Wave:
 def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dims=None, gpu=0,
             wave_size=256, tie_quantum=-1.0, stage_timing=False, record=0, lp=None, lp_shortlist=0,
             lp_fallback_every=0, lp_fallback_mse=0.0, lp_audit=False, lp_min_cover=0.0, no_verbatim=False,
-            n_image=None, k_image=None):
+            n_image=None, k_image=None, kernels=None, kernels_dims=None, kernel_index_image=None):
     """In-memory QuickSampling (qsmi_qs_run_in_memory).
 
     Arrays are flat in G2S DataImage order (dimension 0 fastest, variable
@@ -317,7 +317,8 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
     form; the stats then carry its counters).
     ti may be a list of arrays of one extent (several training images, qs
     -ti a b ...); index is then position * nTI + TI. n_image / k_image: per
-    simulation cell n and k (qs -ni / -kvi)."""
+    simulation cell n and k (qs -ni / -kvi). kernels / kernels_dims /
+    kernel_index_image: several kernels, one chosen per cell (qs -ki ... -kii)."""
     tis = None
     if isinstance(ti, (list, tuple)):
         tis = [_f32(t).reshape(-1) for t in ti]
@@ -358,6 +359,17 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
     if k_image is not None:
         kimg = _f32(k_image).reshape(-1)
         rq.kImage = kimg.ctypes.data
+    kptr = kimgi = ksz = None
+    if kernels is not None and len(kernels) > 1:
+        kernels = [_f32(x).reshape(-1) for x in kernels]
+        kptr = (C.c_void_p * len(kernels))(*[x.ctypes.data for x in kernels])
+        ksz = np.ones((len(kernels), 3), np.uint32)
+        ksz[:, :D] = np.asarray(kernels_dims, np.uint32)
+        kimgi = _f32(kernel_index_image).reshape(-1)
+        rq.nKernels = len(kernels)
+        rq.kernels = C.cast(kptr, C.c_void_p)
+        rq.kernelSizes = ksz.ctypes.data
+        rq.kernelIndexImage = kimgi.ctypes.data
     tptr = None
     if tis is not None:
         tptr = (C.c_void_p * len(tis))(*[t.ctypes.data for t in tis])

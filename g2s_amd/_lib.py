@@ -188,6 +188,10 @@ class QsRequest(C.Structure):
         ("tis", C.c_void_p),
         ("nImage", C.c_void_p),
         ("kImage", C.c_void_p),
+        ("nKernels", C.c_int),
+        ("kernelSizes", C.c_void_p),
+        ("kernels", C.c_void_p),
+        ("kernelIndexImage", C.c_void_p),
     ]
 
 

@@ -373,13 +373,21 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   }
   // kernel (src/qs.cpp:1010-1042) expanded per one-hot variable
   // (DataImage::offsetKernel4categories, include/DataImage.hpp:482-508)
+  const int nKer = rq->nKernels >= 2 ? rq->nKernels : 1;
+  if (nKer > 8) return hfail(QSMI_ERR_UNSUPPORTED, "at most 8 kernels");
+  if (nKer > 1 && (!rq->kernelSizes || !rq->kernels || !rq->kernelIndexImage))
+    return hfail(QSMI_ERR_ARG, "several kernels need their sizes, data and a kernel-index image");
+  if (nKer > 1 && (rq->lp || rq->recordCap > 0))
+    return hfail(QSMI_ERR_UNSUPPORTED, "kernel-index image: not with the learned proposal or the record");
+  const float* kernel0 = nKer > 1 ? rq->kernels[0] : rq->kernel;
+  const unsigned* ksize0 = nKer > 1 ? rq->kernelSizes : rq->kernelSize;
   KernelImg ker;
   ker.D = D;
   ker.nch = nExp;
-  bool defaultKernel = (rq->kernel == nullptr);
+  bool defaultKernel = (kernel0 == nullptr);
   size_t KC = 1;
   for (int i = 0; i < D; ++i) {
-    ker.dims[i] = defaultKernel ? std::min(s[i] / 2 + 1, s[i] / 2 + 1) : rq->kernelSize[i];
+    ker.dims[i] = defaultKernel ? std::min(s[i] / 2 + 1, s[i] / 2 + 1) : ksize0[i];
     if (ker.dims[i] < 1) return hfail(QSMI_ERR_ARG, "zero kernel extent");
     KC *= ker.dims[i];
   }
@@ -388,7 +396,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     int e = 0;
     for (int v = 0; v < nVar; ++v)
       for (int q = 0; q < factor[v]; ++q, ++e)
-        for (size_t c = 0; c < KC; ++c) ker.data[c * nExp + e] = rq->kernel[c * nVar + v];
+        for (size_t c = 0; c < KC; ++c) ker.data[c * nExp + e] = kernel0[c * nVar + v];
   }
   float maxw = 0.f;
   for (float w : ker.data)
@@ -402,6 +410,41 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     int rc = qsmi_offsets(D, ker.dims, nExp, defaultKernel ? nullptr : ker.data.data(), offs.data(), (int)cap, &nOff);
     if (rc) return rc;
   }
+  // the other kernels: expanded data after kernel 0's, offsets after its
+  // offsets (src/qs.cpp:1055-1137 builds one sorted list per kernel)
+  std::vector<int> offStart(nKer, 0), offCount(nKer, nOff);
+  unsigned kdK[8][3] = {};
+  long long kerOff[8] = {};
+  for (int i = 0; i < 3; ++i) kdK[0][i] = i < D ? ker.dims[i] : 1;
+  offs.resize((size_t)nOff * D);
+  for (int q = 1; q < nKer; ++q) {
+    if (!rq->kernels[q]) return hfail(QSMI_ERR_ARG, "null kernel");
+    size_t kc = 1;
+    for (int i = 0; i < 3; ++i) {
+      kdK[q][i] = i < D ? rq->kernelSizes[q * 3 + i] : 1;
+      if (kdK[q][i] < 1) return hfail(QSMI_ERR_ARG, "zero kernel extent");
+      kc *= kdK[q][i];
+    }
+    std::vector<float> kq(kc * nExp);
+    int e = 0;
+    for (int v = 0; v < nVar; ++v)
+      for (int f = 0; f < factor[v]; ++f, ++e)
+        for (size_t c = 0; c < kc; ++c) kq[c * nExp + e] = rq->kernels[q][c * nVar + v];
+    for (float w : kq)
+      if (std::isfinite(w)) maxw = std::max(maxw, std::fabs(w));
+    kerOff[q] = (long long)ker.data.size();
+    ker.data.insert(ker.data.end(), kq.begin(), kq.end());
+    size_t capq = 1;
+    for (int i = 0; i < D; ++i) capq *= (size_t)(2 * (((int)kdK[q][i] + 1) / 2) - 1);
+    std::vector<int> oq(capq * D);
+    int nq = 0;
+    int rc = qsmi_offsets(D, kdK[q], nExp, kq.data(), oq.data(), (int)capq, &nq);
+    if (rc) return rc;
+    offStart[q] = (int)(offs.size() / D);
+    offCount[q] = nq;
+    offs.insert(offs.end(), oq.begin(), oq.begin() + (size_t)nq * D);
+  }
+  if (nKer > 1) nOff = (int)(offs.size() / D);
   // path and seeds (src/qs.cpp:1159-1258)
   std::vector<unsigned char> informed(SD, 0);
   for (size_t c = 0; c < SD; ++c) {
@@ -498,7 +541,18 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   if (!rc) rc = dev_upload(nullptr, nPath * (size_t)n * sizeof(int), &dNb);
   if (!rc) rc = dev_upload(nullptr, nPath * (size_t)n * sizeof(int), &dPj);
   if (!rc) rc = dev_upload(nullptr, nPath * sizeof(int), &dCnt);
-  void *dNNode = nullptr, *dPathK = nullptr;
+  void *dNNode = nullptr, *dPathK = nullptr, *dNodeKer = nullptr, *dOffStart = nullptr, *dOffCount = nullptr;
+  if (nKer > 1) {
+    std::vector<int> nk(nPath);
+    for (size_t i = 0; i < nPath; ++i) {
+      const float v = rq->kernelIndexImage[path[begin + i]];
+      if (!(v >= 0.f && v < (float)nKer)) rc = hfail(QSMI_ERR_ARG, "kernel-index image: value outside [0, nKernels)");
+      nk[i] = rc ? 0 : (int)v;
+    }
+    if (!rc) rc = dev_upload(nk.data(), nPath * sizeof(int), &dNodeKer);
+    if (!rc) rc = dev_upload(offStart.data(), nKer * sizeof(int), &dOffStart);
+    if (!rc) rc = dev_upload(offCount.data(), nKer * sizeof(int), &dOffCount);
+  }
   if (rq->nImage) {
     std::vector<int> nNode(nPath);
     for (size_t i = 0; i < nPath; ++i) {
@@ -514,7 +568,8 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   }
   if (!rc)
     rc = qsmi::nb_search_device(D, sd, nPath, (const unsigned*)dCells, (const int*)dPostS, (const int*)dOffs, nOff, n,
-                                (int*)dNb, (int*)dPj, (int*)dCnt, dev->stream, (const int*)dNNode);
+                                (int*)dNb, (int*)dPj, (int*)dCnt, dev->stream, (const int*)dNNode,
+                                (const int*)dNodeKer, (const int*)dOffStart, (const int*)dOffCount);
   // learned proposal: which nodes reach the stencil coverage it needs
   const float lpMinCover = rq->lpMinCover > 0.f ? rq->lpMinCover : 0.5f;
   std::vector<unsigned char> lpOk;
@@ -575,6 +630,11 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   R.n = n;
   R.k = rq->k;
   R.kMax = kMax;
+  R.nKer = nKer;
+  for (int q = 0; q < nKer; ++q) {
+    for (int i = 0; i < 3; ++i) R.kdK[q][i] = (int)kdK[q][i];
+    R.kerOff[q] = kerOff[q];
+  }
   for (int v = 0; v < nVar && v < 8; ++v) R.types[v] = rq->varTypes[v];
   {
     int e = 0, ci = 0, off = 0;
@@ -657,6 +717,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     R.dTi = (const float*)dTi;
     R.extraDevs = xdev;
     R.dPathK = (const float*)dPathK;
+    R.dNodeKer = (const int*)dNodeKer;
     rc = qsmi::run_resident_waves(dev, R, stageMs, &scanned, &waves);
   }
   if (!rc) {

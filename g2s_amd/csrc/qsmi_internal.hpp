@@ -165,6 +165,11 @@ struct ResidentLoop {
   std::vector<qsmi_device*> extraDevs;  // TIs 1.. of a several-TI run (same extent as this device's)
   const float* dPathK = nullptr;        // per path node k from a k image (nullptr: the run's k)
   float kMax = 0.f;                     // largest k of the run (candidate list length ceil(kMax))
+  // kernel-index image: per path node kernel, per kernel its grid and offset
+  const int* dNodeKer = nullptr;
+  int nKer = 1;
+  int kdK[8][3] = {};
+  long long kerOff[8] = {};
   const int* dOrder;
   const unsigned* dPathCells;
   const float* dPathSeeds;
@@ -216,7 +221,8 @@ void lp_record(int nN, int w0, const int* order, const NodeInfo* info, const int
 // neighbour sets of every path node on the device (stream-ordered)
 int nb_search_device(int D, const unsigned* sd, size_t nPath, const unsigned* dCells, const int* dPostS,
                      const int* dOffs, int nOff, int n, int* dNbIdx, int* dNbPj, int* dNbCnt, hipStream_t st,
-                     const int* dNNode = nullptr);
+                     const int* dNNode = nullptr, const int* dNodeKer = nullptr, const int* dOffStart = nullptr,
+                     const int* dOffCount = nullptr);
 
 // stageMs: [0] spectrum (+ dim1 inverse), [1] gather, [2] rows + select,
 // [3] merge + draw + write; filled when R.stageTiming.

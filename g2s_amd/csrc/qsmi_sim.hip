@@ -43,6 +43,8 @@ struct GatherParams {
   int noVerbatim;  // 1: exclude the position that continues the nearest neighbour's copy (-nV)
   unsigned f[3];   // TI FFT grid (the reference's index space)
   int nTI;         // training images (tiVals holds them one after the other)
+  int kdK[8][3];        // per kernel of a kernel-index image: its grid (kernel 0: kd)
+  long long kerOff[8];  // its offset in the kernel array (floats)
 };
 
 __device__ __forceinline__ uint32_t seed_bits_dev(float seed) {
@@ -64,7 +66,7 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
                                                int* __restrict__ nbOff, float* __restrict__ nbVal,
                                                const uint8_t* __restrict__ centreOk,
                                                const uint32_t* __restrict__ complete, uint32_t nComplete,
-                                               const float* __restrict__ pathK) {
+                                               const float* __restrict__ pathK, const int* __restrict__ nodeKer) {
   const int q = blockIdx.x;
   const int lane = threadIdx.x;
   const int i = order[w0 + q];
@@ -72,6 +74,12 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
   const float seed = pathSeeds[i];
   const int cnt = nbCnt[i];
   const int D = P.D;
+  // the node's kernel (include/simulation.hpp:348-351: a kernel-index image
+  // picks one per cell), centre dims/2 (quantileSamplingModule.hpp:362-366)
+  const int kk = nodeKer ? nodeKer[i] : 0;
+  const int kdv[3] = {P.kdK[kk][0], P.kdK[kk][1], P.kdK[kk][2]};
+  const int kctr[3] = {kdv[0] / 2, kdv[1] / 2, kdv[2] / 2};
+  const float* __restrict__ kn = kern + P.kerOff[kk];
   NodeInfo ni;
   ni.cell = cell;
   ni.seed = seed;
@@ -132,17 +140,17 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
       mx[d] = max(mx[d], dd[d]);
     }
     for (int d = D - 1; d >= 0; --d) {  // DataImage::indexWithDelta from the kernel centre
-      int c = P.kctr[d] + dd[d];
-      inK &= (c >= 0 && c < P.kd[d]);
-      c = ((c % P.kd[d]) + P.kd[d]) % P.kd[d];
-      kc = kc * P.kd[d] + c;
+      int c = kctr[d] + dd[d];
+      inK &= (c >= 0 && c < kdv[d]);
+      c = ((c % kdv[d]) + kdv[d]) % kdv[d];
+      kc = kc * kdv[d] + c;
     }
     const float* x = di + t * P.nVar;
     for (int d = 0; d < D; ++d) nbOff[((long long)q * P.n + j) * D + d] = dd[d];
     if (P.cross) {
       for (int d = 0; d < D; ++d) nbOff[((long long)(nN + q) * P.n + j) * D + d] = dd[d];
       for (int v = 0; v < P.nVar; ++v) {
-        const float w = kern[kc * P.nExp + v];
+        const float w = kn[kc * P.nExp + v];
         if (inK && !isnan(x[v]) && w > 0.f) wmin = fminf(wmin, w);
       }
     }
@@ -159,7 +167,7 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
           const int e = P.subOf[B];
           const float xv = x[P.expVar[e]];
           const float xe = (P.expCat[e] < 0) ? xv : ((xv == catvals[P.expCat[e]]) ? 1.f : 0.f);
-          const float w = kern[kc * P.nExp + e];
+          const float w = kn[kc * P.nExp + e];
           const float u = (inK && !isnan(xe)) ? (P.conv[B] == 0 ? w * 1.f : P.conv[B] == 2 ? w * xe * xe : w * xe) : 0.f;
           acc += al * u;
         }
@@ -193,10 +201,10 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
       mul *= P.sd[d];
     }
     for (int d = D - 1; d >= 0; --d) {
-      int c = P.kctr[d] + dd[d];
-      inK &= (c >= 0 && c < P.kd[d]);
-      c = ((c % P.kd[d]) + P.kd[d]) % P.kd[d];
-      kc = kc * P.kd[d] + c;
+      int c = kctr[d] + dd[d];
+      inK &= (c >= 0 && c < kdv[d]);
+      c = ((c % kdv[d]) + kdv[d]) % kdv[d];
+      kc = kc * kdv[d] + c;
     }
     for (int v = 0; v < P.nVar; ++v) {
       float tv = 0.f;
@@ -206,7 +214,7 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
         const float xv = di[t * P.nVar + P.expVar[e]];
         const float xe = (P.expCat[e] < 0) ? xv : ((xv == catvals[P.expCat[e]]) ? 1.f : 0.f);
         if (inK && !isnan(xe)) {
-          tv = -1.f * (kern[kc * P.nExp + (v % P.nExp)] * xe * xe);
+          tv = -1.f * (kn[kc * P.nExp + (v % P.nExp)] * xe * xe);
           h = true;
         }
       }
@@ -306,7 +314,9 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
 __global__ void __launch_bounds__(64) k_nbsearch(int D, int sd0, int sd1, int sd2, const unsigned* __restrict__ cells,
                                                 const int* __restrict__ postS, const int* __restrict__ offs, int nOff,
                                                 int n, int* __restrict__ nbIdx, int* __restrict__ nbPj,
-                                                int* __restrict__ nbCnt, const int* __restrict__ nNode) {
+                                                int* __restrict__ nbCnt, const int* __restrict__ nNode,
+                                                const int* __restrict__ nodeKer, const int* __restrict__ offStart,
+                                                const int* __restrict__ offCount) {
   const long long i = blockIdx.x;
   const int lane = threadIdx.x;
   const int sd[3] = {sd0, sd1, sd2};
@@ -322,12 +332,15 @@ __global__ void __launch_bounds__(64) k_nbsearch(int D, int sd0, int sd1, int sd
   int* outP = nbPj + i * n;
   // this node's count from an n image (include/simulation.hpp:337-343), at most n
   const int nI = nNode ? min(max(nNode[i], 0), n) : n;
+  // the node's kernel's offset list (kernel-index image), indices global
+  const int o0 = nodeKer ? offStart[nodeKer[i]] : 0;
+  const int nO = nodeKer ? offCount[nodeKer[i]] : nOff;
   int cnt = 0;
-  for (int base = 0; base < nOff && cnt < nI; base += 64) {
-    const int o = base + lane;
+  for (int base = 0; base < nO && cnt < nI; base += 64) {
+    const int o = o0 + base + lane;
     bool hit = false;
     int pj = -1;
-    if (o < nOff) {
+    if (base + lane < nO) {
       bool inside = true;
       long long t = 0, mul = 1;
       for (int d = 0; d < D; ++d) {
@@ -354,10 +367,10 @@ __global__ void __launch_bounds__(64) k_nbsearch(int D, int sd0, int sd1, int sd
 
 int nb_search_device(int D, const unsigned* sd, size_t nPath, const unsigned* dCells, const int* dPostS,
                      const int* dOffs, int nOff, int n, int* dNbIdx, int* dNbPj, int* dNbCnt, hipStream_t st,
-                     const int* dNNode) {
+                     const int* dNNode, const int* dNodeKer, const int* dOffStart, const int* dOffCount) {
   if (nPath == 0) return QSMI_OK;
   hipLaunchKernelGGL(k_nbsearch, dim3((unsigned)nPath), dim3(64), 0, st, D, (int)sd[0], (int)sd[1], (int)sd[2], dCells,
-                     dPostS, dOffs, nOff, n, dNbIdx, dNbPj, dNbCnt, dNNode);
+                     dPostS, dOffs, nOff, n, dNbIdx, dNbPj, dNbCnt, dNNode, dNodeKer, dOffStart, dOffCount);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(QSMI_ERR_HIP, std::string("neighbour search: ") + hipGetErrorString(e));
   return QSMI_OK;
@@ -378,6 +391,7 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
     P.sd[d] = d < D ? R.sd[d] : 1;
     P.kd[d] = d < D ? R.kd[d] : 1;
     P.kctr[d] = P.kd[d] / 2;
+    P.kdK[0][d] = P.kd[d];
     P.s[d] = t->s[d];
   }
   P.nVar = R.nVar;
@@ -411,6 +425,13 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   P.tiCells = (unsigned)t->S;
   const int nTI = 1 + (int)R.extraDevs.size();
   P.nTI = nTI;
+  if (R.nKer > 8) return fail(QSMI_ERR_UNSUPPORTED, "at most 8 kernels");
+  for (int q = 1; q < R.nKer; ++q) {
+    for (int d = 0; d < 3; ++d) P.kdK[q][d] = d < D ? R.kdK[q][d] : 1;
+    P.kerOff[q] = R.kerOff[q];
+  }
+  if (R.dNodeKer && (R.lp || R.recCap > 0))
+    return fail(QSMI_ERR_UNSUPPORTED, "kernel-index image: not with the learned proposal or the record");
   if (R.dPathK && (R.lp || R.recCap > 0))
     return fail(QSMI_ERR_UNSUPPORTED, "k image: not with the learned proposal or the record");
   if (nTI > 1 && (R.lp || dev->cross || R.noVerbatim || R.recCap > 0))
@@ -488,7 +509,7 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
     if (ev) (void)hipEventRecord(ev[0], st);
     hipLaunchKernelGGL(k_gather, dim3(nN), dim3(64), gatherLds, st, P, w0, nN, R.dOrder, R.dPathCells, R.dPathSeeds,
                        R.dNbIdx, R.dNbCnt, R.dOffs, R.dKernel, R.dCatvals, R.dDi, R.dIndex, R.dTi, b->dInfo, b->dOff,
-                       b->dVal, dev->dCentreOk, dev->dComplete, dev->nComplete, R.dPathK);
+                       b->dVal, dev->dCentreOk, dev->dComplete, dev->nComplete, R.dPathK, R.dNodeKer);
     if (ev) (void)hipEventRecord(ev[1], st);
     const bool useLp = R.lp && (R.waveLp.empty() || R.waveLp[w]);
     if (R.lp && !useLp) R.lpSkipped += nN;

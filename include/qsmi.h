@@ -291,6 +291,16 @@ typedef struct {
      Not with the learned proposal or the pair record. */
   const float* nImage;
   const float* kImage;
+  /* several kernels chosen per cell (qs -ki a b ... -kii image,
+     include/simulation.hpp:348-351, src/qs.cpp:1055-1137): nKernels >= 2
+     takes kernels[q] (prod(kernelSizes[3q..3q+2]) x nVar) in place of
+     kernel/kernelSize, each with its own offset order, and
+     kernelIndexImage (prod(simSize) floats) the kernel of every simulated
+     cell. Not with the learned proposal or the pair record. */
+  int nKernels;
+  const unsigned* kernelSizes;
+  const float* const* kernels;
+  const float* kernelIndexImage;
 } qsmi_qs_request;
 
 typedef struct {

@@ -325,3 +325,53 @@ def test_sim_n_and_k_images(kind):
     di = np.full(cells, np.nan, np.float32)
     _log(f"n / k images {kind}: n in (4, 9, 20), k in (1, 2, 3.5, run's 1.5)")
     lockstep(ti, dims, di, sdims, types, 20, 1.5, 8, q, n_image=nimg, k_image=kimg)
+
+
+def test_sim_kernel_index_image():
+    """qs -ki a b -kii image: each cell's kernel (weights and its own offset
+    order); lockstep with the oracle node by node (one oracle per kernel,
+    the node's kernel deciding), and an index image of zeros gives the
+    one-kernel run."""
+    g = synth.grf((48, 56), 4.0, seed=61)
+    ti, dims = synth.to_g2s(np.exp(g))
+    ti = ti.reshape(-1)
+    sdims = (40, 36)
+    cells = int(np.prod(sdims))
+    kd = [(21, 21), (15, 11)]
+    k0 = np.ones(21 * 21, np.float32)
+    yy, xx = np.mgrid[-5:6, -7:8]
+    k1 = np.exp(-np.sqrt(xx ** 2 + yy ** 2) / 4.0).astype(np.float32).reshape(-1)
+    rng = np.random.default_rng(11)
+    kii = (rng.random(cells) < 0.5).astype(np.float32)
+    di = np.full(cells, np.nan, np.float32)
+    pick = rng.choice(cells, cells // 25, replace=False)
+    di[pick] = ti[rng.integers(0, ti.size, pick.size)]
+    n, k, seed = 16, 1.5, 12
+    _log("kernel-index image: 21x21 uniform / 15x11 exponential, random per cell")
+    sim, idx, st = g2s_amd.qs(ti, dims, di, sdims, [0], n, k, seed, wave_size=64, kernels=[k0, k1], kernels_dims=kd,
+                              kernel_index_image=kii)
+    informed = (~np.isnan(di)).astype(np.uint8)
+    p, s_, b = oracle.path(informed, seed)
+    sims = [oracle.Sim(ti, dims, [0], [], [], sdims, kdq, kq, oracle.offsets(kdq, kq), n, k, p, s_, b)
+            for kdq, kq in zip(kd, (k0, k1))]
+    mism, worst = 0, 0.0
+    for i in range(len(p) - b):
+        cell = p[b + i]
+        o = sims[int(kii[cell])]
+        chosen, _, _, nb = o.node(sim, i)
+        if chosen == idx[cell]:
+            continue
+        mism += 1
+        assert nb > 1, f"marginal draw differs at node {i}"
+        m, sc, _ = o.node_map(sim, i)
+        a, gq = int(chosen), int(idx[cell])
+        gap = abs(m[a] - m[gq])
+        bound = MAP_RTOL * (sc[a] + sc[gq])
+        assert gap <= bound, f"node {i}: gpu {gq} oracle {a} gap {gap} > {bound}"
+        worst = max(worst, gap / bound)
+    _log(f"  lockstep mismatches {mism} of {len(p) - b} (largest gap / tolerance {worst:.3f})")
+    assert mism <= (len(p) - b) // 100
+    s0, i0, _ = g2s_amd.qs(ti, dims, di, sdims, [0], n, k, seed, wave_size=64, kernel=k0, kernel_dims=kd[0])
+    sz, iz, _ = g2s_amd.qs(ti, dims, di, sdims, [0], n, k, seed, wave_size=64, kernels=[k0, k1], kernels_dims=kd,
+                           kernel_index_image=np.zeros(cells, np.float32))
+    assert np.array_equal(iz, i0) and np.array_equal(sz, s0, equal_nan=True)
