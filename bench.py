@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--sim-size", type=int, default=0, help="simulation extent when smaller than the TI (profiling)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 256 MiB device copy (HBM counter calibration under rocprofv3 --pmc)")
-    ap.add_argument("--wave", type=int, default=256, help="max path nodes per wave")
+    ap.add_argument("--wave", type=int, default=512, help="max path nodes per wave (the learned form: 256)")
     ap.add_argument("--cpu-sample", type=int, default=48, help="path nodes of the CPU baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--learned", type=int, default=1,

@@ -302,7 +302,7 @@ class Wave:
 
 
 def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dims=None, gpu=0,
-            wave_size=256, tie_quantum=-1.0, stage_timing=False, record=0, lp=None, lp_shortlist=0,
+            wave_size=0, tie_quantum=-1.0, stage_timing=False, record=0, lp=None, lp_shortlist=0,
             lp_fallback_every=0, lp_fallback_mse=0.0, lp_audit=False, lp_min_cover=0.0, no_verbatim=False,
             n_image=None, k_image=None, kernels=None, kernels_dims=None, kernel_index_image=None):
     """In-memory QuickSampling (qsmi_qs_run_in_memory).

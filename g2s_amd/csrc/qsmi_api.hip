@@ -252,7 +252,8 @@ int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeI
   if (t->fast1k) {
     int rc = ensure_tap_tables(dev, nNodes);
     if (rc) return rc;
-    const int npb = std::max(1, std::min(8, nNodes / 8));
+    // nodes per pencil block: 3 measured best over 1..16 (256-node waves)
+    const int npb = std::max(1, std::min(3, nNodes / 8));
     launch_pencil1k(g, t->dYT, t->dTw[0], nNodes, npb, info, off, val, dev->dTap, dev->dTapOf, C, count, st);
   } else if (count) {
     return fail(QSMI_ERR_UNSUPPORTED, "a device node count is supported on the 1000x1000 path only");

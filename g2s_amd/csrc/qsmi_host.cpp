@@ -307,6 +307,8 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   std::vector<float> tiCat;
   const float* tiIn = rq->ti;
   if (nTI > 1) {
+    if ((unsigned long long)S * nTI > 0xFFFFFFFFull)
+      return hfail(QSMI_ERR_UNSUPPORTED, "several TIs: cells x nTI must fit the 32-bit index (position * nTI + TI)");
     if (rq->lp || rq->recordCap > 0 || rq->noVerbatim)
       return hfail(QSMI_ERR_UNSUPPORTED, "several TIs: not with the learned proposal, the record or -nV");
     tiCat.resize((size_t)nTI * S * nVar);
@@ -613,7 +615,9 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     std::vector<size_t> fillp(lvStart.begin(), lvStart.end() - 1);
     for (size_t i = 0; i < nPath; ++i) order[fillp[level[i]]++] = (unsigned)i;
   }
-  const int waveSize = rq->waveSize > 0 ? rq->waveSize : 256;
+  // 512 nodes per wave measured best on the 1000x1000 path (fewer launches;
+  // 768 no better); the learned proposal keeps its queries in LDS: 256
+  const int waveSize = rq->waveSize > 0 ? rq->waveSize : (rq->lp ? 256 : 512);
   const auto tLv = clk::now();
   // waves: each level cut into chunks of at most waveSize nodes
   qsmi::ResidentLoop R;

@@ -248,7 +248,7 @@ typedef struct {
   int n;                   /* -n, centre included */
   float k;                 /* -k */
   unsigned seed;           /* -s */
-  int waveSize;            /* max nodes per wave, 0 -> 256 */
+  int waveSize;            /* max nodes per wave, 0 -> 512 (256 with the learned proposal) */
   float tieQuantum;        /* < 0 automatic: 0 with a continuous variable,
                               2^(ceil(log2(n*max|w|))-16) otherwise */
   int stageTiming;         /* 1: time every wave's kernels with HIP events (stageMs) */
