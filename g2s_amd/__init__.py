@@ -304,7 +304,8 @@ class Wave:
 def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dims=None, gpu=0,
             wave_size=0, tie_quantum=-1.0, stage_timing=False, record=0, lp=None, lp_shortlist=0,
             lp_fallback_every=0, lp_fallback_mse=0.0, lp_audit=False, lp_min_cover=0.0, no_verbatim=False,
-            n_image=None, k_image=None, kernels=None, kernels_dims=None, kernel_index_image=None):
+            n_image=None, k_image=None, kernels=None, kernels_dims=None, kernel_index_image=None,
+            circular_sim=False):
     """In-memory QuickSampling (qsmi_qs_run_in_memory).
 
     Arrays are flat in G2S DataImage order (dimension 0 fastest, variable
@@ -352,6 +353,7 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
     rq.tieQuantum = float(tie_quantum)
     rq.stageTiming = int(bool(stage_timing))
     rq.noVerbatim = int(bool(no_verbatim))
+    rq.circularSim = int(bool(circular_sim))
     nimg = kimg = None
     if n_image is not None:
         nimg = _f32(n_image).reshape(-1)

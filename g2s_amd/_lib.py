@@ -192,6 +192,7 @@ class QsRequest(C.Structure):
         ("kernelSizes", C.c_void_p),
         ("kernels", C.c_void_p),
         ("kernelIndexImage", C.c_void_p),
+        ("circularSim", C.c_int),
     ]
 
 

@@ -275,6 +275,8 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   if (nTI > 8) return hfail(QSMI_ERR_UNSUPPORTED, "at most 8 training images");
   if ((!rq->nImage && rq->n < 1) || !(rq->k > 0.f) || std::ceil(rq->k) > 16.f)
     return hfail(QSMI_ERR_UNSUPPORTED, "need n >= 1, 0 < k <= 16");
+  if (rq->circularSim && (rq->lp || rq->recordCap > 0))
+    return hfail(QSMI_ERR_UNSUPPORTED, "circular simulation: not with the learned proposal or the record");
   if ((rq->nImage || rq->kImage) && (rq->lp || rq->recordCap > 0))
     return hfail(QSMI_ERR_UNSUPPORTED, "n / k images: not with the learned proposal or the record");
   const auto t0 = clk::now();
@@ -571,7 +573,8 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   if (!rc)
     rc = qsmi::nb_search_device(D, sd, nPath, (const unsigned*)dCells, (const int*)dPostS, (const int*)dOffs, nOff, n,
                                 (int*)dNb, (int*)dPj, (int*)dCnt, dev->stream, (const int*)dNNode,
-                                (const int*)dNodeKer, (const int*)dOffStart, (const int*)dOffCount);
+                                (const int*)dNodeKer, (const int*)dOffStart, (const int*)dOffCount,
+                                rq->circularSim ? 1 : 0);
   // learned proposal: which nodes reach the stencil coverage it needs
   const float lpMinCover = rq->lpMinCover > 0.f ? rq->lpMinCover : 0.5f;
   std::vector<unsigned char> lpOk;
@@ -663,6 +666,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   }
   R.stageTiming = rq->stageTiming != 0;
   R.noVerbatim = rq->noVerbatim != 0;
+  R.circSim = rq->circularSim != 0;
   std::vector<int> orderI(order.begin(), order.end());
   void *dOrder = nullptr, *dSeeds = nullptr, *dKer = nullptr, *dCat = nullptr, *dDi = nullptr, *dIdx = nullptr,
        *dTi = nullptr;

@@ -167,6 +167,7 @@ struct ResidentLoop {
   float kMax = 0.f;                     // largest k of the run (candidate list length ceil(kMax))
   // kernel-index image: per path node kernel, per kernel its grid and offset
   const int* dNodeKer = nullptr;
+  bool circSim = false;  // -csim: the simulation grid wraps around
   int nKer = 1;
   int kdK[8][3] = {};
   long long kerOff[8] = {};
@@ -222,7 +223,7 @@ void lp_record(int nN, int w0, const int* order, const NodeInfo* info, const int
 int nb_search_device(int D, const unsigned* sd, size_t nPath, const unsigned* dCells, const int* dPostS,
                      const int* dOffs, int nOff, int n, int* dNbIdx, int* dNbPj, int* dNbCnt, hipStream_t st,
                      const int* dNNode = nullptr, const int* dNodeKer = nullptr, const int* dOffStart = nullptr,
-                     const int* dOffCount = nullptr);
+                     const int* dOffCount = nullptr, int circSim = 0);
 
 // stageMs: [0] spectrum (+ dim1 inverse), [1] gather, [2] rows + select,
 // [3] merge + draw + write; filled when R.stageTiming.

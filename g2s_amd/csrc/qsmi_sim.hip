@@ -43,6 +43,7 @@ struct GatherParams {
   int noVerbatim;  // 1: exclude the position that continues the nearest neighbour's copy (-nV)
   unsigned f[3];   // TI FFT grid (the reference's index space)
   int nTI;         // training images (tiVals holds them one after the other)
+  int circSim;          // 1: the simulation grid wraps around (-csim)
   int kdK[8][3];        // per kernel of a kernel-index image: its grid (kernel 0: kd)
   long long kerOff[8];  // its offset in the kernel array (floats)
 };
@@ -133,7 +134,8 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
     bool inK = true;
     long long kc = 0;
     for (int d = 0; d < D; ++d) {
-      const int x = cc[d] + dd[d];
+      int x = cc[d] + dd[d];
+      if (P.circSim) x = ((x % P.sd[d]) + P.sd[d]) % P.sd[d];
       t += (long long)x * mul;
       mul *= P.sd[d];
       mn[d] = min(mn[d], dd[d]);
@@ -197,7 +199,9 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
     long long t = 0, mul = 1, kc = 0;
     bool inK = true;
     for (int d = 0; d < D; ++d) {
-      t += (long long)(cc[d] + dd[d]) * mul;
+      int x = cc[d] + dd[d];
+      if (P.circSim) x = ((x % P.sd[d]) + P.sd[d]) % P.sd[d];
+      t += (long long)x * mul;
       mul *= P.sd[d];
     }
     for (int d = D - 1; d >= 0; --d) {
@@ -255,7 +259,9 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
       const int* d1 = offs + (long long)idx[1] * D;
       long long t1 = 0, mul = 1;
       for (int d = 0; d < D; ++d) {
-        t1 += (long long)(cc[d] + d1[d]) * mul;
+        int x = cc[d] + d1[d];
+        if (P.circSim) x = ((x % P.sd[d]) + P.sd[d]) % P.sd[d];
+        t1 += (long long)x * mul;
         mul *= P.sd[d];
       }
       unsigned rem = index[t1];
@@ -316,7 +322,7 @@ __global__ void __launch_bounds__(64) k_nbsearch(int D, int sd0, int sd1, int sd
                                                 int n, int* __restrict__ nbIdx, int* __restrict__ nbPj,
                                                 int* __restrict__ nbCnt, const int* __restrict__ nNode,
                                                 const int* __restrict__ nodeKer, const int* __restrict__ offStart,
-                                                const int* __restrict__ offCount) {
+                                                const int* __restrict__ offCount, int circSim) {
   const long long i = blockIdx.x;
   const int lane = threadIdx.x;
   const int sd[3] = {sd0, sd1, sd2};
@@ -344,7 +350,9 @@ __global__ void __launch_bounds__(64) k_nbsearch(int D, int sd0, int sd1, int sd
       bool inside = true;
       long long t = 0, mul = 1;
       for (int d = 0; d < D; ++d) {
-        const int x = cc[d] + offs[(long long)o * D + d];
+        int x = cc[d] + offs[(long long)o * D + d];
+        // include/simulation.hpp:394: `|| circularSim` takes the wrapped cell
+        if (circSim) x = ((x % sd[d]) + sd[d]) % sd[d];
         inside &= (x >= 0 && x < sd[d]);
         t += (long long)x * mul;
         mul *= sd[d];
@@ -367,10 +375,11 @@ __global__ void __launch_bounds__(64) k_nbsearch(int D, int sd0, int sd1, int sd
 
 int nb_search_device(int D, const unsigned* sd, size_t nPath, const unsigned* dCells, const int* dPostS,
                      const int* dOffs, int nOff, int n, int* dNbIdx, int* dNbPj, int* dNbCnt, hipStream_t st,
-                     const int* dNNode, const int* dNodeKer, const int* dOffStart, const int* dOffCount) {
+                     const int* dNNode, const int* dNodeKer, const int* dOffStart, const int* dOffCount,
+                     int circSim) {
   if (nPath == 0) return QSMI_OK;
   hipLaunchKernelGGL(k_nbsearch, dim3((unsigned)nPath), dim3(64), 0, st, D, (int)sd[0], (int)sd[1], (int)sd[2], dCells,
-                     dPostS, dOffs, nOff, n, dNbIdx, dNbPj, dNbCnt, dNNode, dNodeKer, dOffStart, dOffCount);
+                     dPostS, dOffs, nOff, n, dNbIdx, dNbPj, dNbCnt, dNNode, dNodeKer, dOffStart, dOffCount, circSim);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(QSMI_ERR_HIP, std::string("neighbour search: ") + hipGetErrorString(e));
   return QSMI_OK;
@@ -412,6 +421,7 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   for (int q = 0; q < t->nSub * t->nSub; ++q) P.alpha[q] = dev->alpha[q];
   P.cross = dev->cross ? 1 : 0;
   P.noVerbatim = R.noVerbatim ? 1 : 0;
+  P.circSim = R.circSim ? 1 : 0;
   for (int d = 0; d < 3; ++d) P.f[d] = d < D ? (unsigned)t->f[d] : 1u;
   if (R.noVerbatim && R.lp) return fail(QSMI_ERR_UNSUPPORTED, "learned proposal with the verbatim exclusion");
   if (dev->cross) {

@@ -301,6 +301,8 @@ typedef struct {
   const unsigned* kernelSizes;
   const float* const* kernels;
   const float* kernelIndexImage;
+  int circularSim;         /* -csim (src/qs.cpp:797-801): neighbours wrap around the
+                              simulation grid (include/simulation.hpp:394) */
 } qsmi_qs_request;
 
 typedef struct {

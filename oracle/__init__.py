@@ -66,6 +66,7 @@ class OrcSim(C.Structure):
         ("nTI", C.c_int),
         ("nImage", C.c_void_p),
         ("kImage", C.c_void_p),
+        ("circularSim", C.c_int),
     ]
 
 
@@ -260,7 +261,7 @@ class Sim:
 
     def __init__(self, ti, tidims, types, ncat, catvals, simdims, kdims, kernel_exp, offs, n, k, path, seeds,
                  begin, mode=0, quantum=0.0, cross=False, no_verbatim=False, n_ti=1, n_image=None,
-                 k_image=None):
+                 k_image=None, circular_sim=False):
         self._keep = []
 
         def keep(a):
@@ -296,6 +297,7 @@ class Sim:
         s.cross = int(bool(cross))
         s.noVerbatim = int(bool(no_verbatim))
         s.nTI = int(n_ti)  # ti then holds the n_ti images one after the other
+        s.circularSim = int(bool(circular_sim))
         if n_image is not None:  # per cell n; n must be the largest value
             s.nImage = keep(f32(n_image).reshape(-1))
         if k_image is not None:

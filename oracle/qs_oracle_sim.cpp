@@ -91,6 +91,7 @@ struct OrcSim {
   int nTI;        // training images in `ti`, one after the other (<= 1: one; mode 0 only when > 1)
   const float* nImage;  // per simulation cell n (nullptr: n); n must then be the largest value
   const float* kImage;  // per simulation cell k (nullptr, or values <= 0: k)
+  int circularSim;      // 1: -csim, neighbour cells wrap around the simulation grid
 };
 
 struct OrcSimCtx {
@@ -113,6 +114,10 @@ struct OrcSimCtx {
   int nTI = 1;
   std::vector<std::vector<float>> Yx;  // sub-images of TIs 1.. (several TIs)
 };
+
+// include/simulation.hpp:394: with circularSim the wrapped cell of
+// DataImage::indexWithDelta is taken
+static inline int wrap_if(int x, int n, int circ) { return circ ? ((x % n) + n) % n : x; }
 
 void* orc_sim_create(const OrcSim* p) {
   auto* c = new OrcSimCtx();
@@ -231,7 +236,7 @@ int orc_sim_gather(void* h, unsigned pathIdx, int* outIdx) {
     bool inside = true;
     size_t t = 0, mul = 1;
     for (int i = 0; i < c->D; ++i) {
-      int x = cc[i] + p.offsets[o * c->D + i];
+      int x = wrap_if(cc[i] + p.offsets[o * c->D + i], int(c->sd[i]), p.circularSim);
       inside &= (x >= 0 && x < int(c->sd[i]));
       t += (size_t)x * mul;
       mul *= c->sd[i];
@@ -265,7 +270,7 @@ static int node_inputs(OrcSimCtx* c, const float* di, unsigned pathIdx, std::vec
     for (int i = 0; i < D; ++i) {
       int d = p.offsets[idx[j] * D + i];
       off[(size_t)j * D + i] = d;
-      t += (size_t)(cc[i] + d) * mul;
+      t += (size_t)wrap_if(cc[i] + d, int(c->sd[i]), p.circularSim) * mul;
       mul *= c->sd[i];
     }
     // one-hot expansion — simulation.hpp:512-537. The node's own cell is
@@ -356,7 +361,7 @@ static unsigned verbatim_index(const OrcSimCtx* c, unsigned pathIdx, const std::
   for (int i = 0; i < D; ++i) {
     int cc = int(rem % c->sd[i]);
     rem /= c->sd[i];
-    t += (size_t)(cc + off[(size_t)1 * D + i]) * mul;
+    t += (size_t)wrap_if(cc + off[(size_t)1 * D + i], int(c->sd[i]), p.circularSim) * mul;
     mul *= c->sd[i];
   }
   const unsigned from = c->index[t];

@@ -32,7 +32,8 @@ def default_kernel(ti_dims, nexp):
 
 
 def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatch_frac=0.01, nodes=None,
-             weights=None, wave_size=64, threads=8, cross=False, no_verbatim=False, n_image=None, k_image=None):
+             weights=None, wave_size=64, threads=8, cross=False, no_verbatim=False, n_image=None, k_image=None,
+             circular_sim=False):
     """nodes: path-relative node indices to check (None: every node);
     weights: one kernel weight per variable (None: the default uniform 1)."""
     types = list(types)
@@ -42,7 +43,8 @@ def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatc
         kd0 = [d // 2 + 1 for d in ti_dims]
         gkw = dict(kernel=np.tile(np.asarray(weights, np.float32), int(np.prod(kd0))), kernel_dims=kd0)
     sim, idx, st = g2s_amd.qs(ti, ti_dims, di0, sim_dims, types, n, k, seed, wave_size=wave_size,
-                              tie_quantum=quantum, no_verbatim=no_verbatim, n_image=n_image, k_image=k_image, **gkw)
+                              tie_quantum=quantum, no_verbatim=no_verbatim, n_image=n_image, k_image=k_image,
+                              circular_sim=circular_sim, **gkw)
     _log(f"  gpu run: {st}")
     # oracle set-up identical to the reference's qs.cpp
     ncat, catvals = [], []
@@ -67,7 +69,8 @@ def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatc
     gp, gs, gb = g2s_amd.path(informed, seed)
     assert np.array_equal(p, gp) and np.array_equal(s, gs) and b == gb
     orc = oracle.Sim(ti, ti_dims, types, ncat, catvals, sim_dims, kd, kern, offs, n, k, p, s, b, mode=0,
-                     quantum=quantum, cross=cross, no_verbatim=no_verbatim, n_image=n_image, k_image=k_image)
+                     quantum=quantum, cross=cross, no_verbatim=no_verbatim, n_image=n_image, k_image=k_image,
+                     circular_sim=circular_sim)
     orc.set_index(idx)  # -nV reads the positions the neighbours were copied from
     K = int(np.ceil(k))
     simf = sim.reshape(-1, nvar)
@@ -375,3 +378,21 @@ def test_sim_kernel_index_image():
     sz, iz, _ = g2s_amd.qs(ti, dims, di, sdims, [0], n, k, seed, wave_size=64, kernels=[k0, k1], kernels_dims=kd,
                            kernel_index_image=np.zeros(cells, np.float32))
     assert np.array_equal(iz, i0) and np.array_equal(sz, s0, equal_nan=True)
+
+
+@pytest.mark.parametrize("kind", ["continuous", "categorical_no_verbatim"])
+def test_sim_circular_simulation(kind):
+    """qs -csim: neighbours wrap around the simulation grid; lockstep with
+    the oracle's serial loop (with -nV on top in the categorical case)."""
+    if kind == "continuous":
+        g = synth.grf((48, 56), 4.0, seed=71)
+        ti, dims = synth.to_g2s(np.exp(g))
+        types, q, nv = [0], 0.0, False
+    else:
+        g = synth.grf((50, 50), 5.0, seed=72)
+        ti, dims = synth.to_g2s((g > np.quantile(g, 0.6)).astype(np.float32))
+        types, q, nv = [1], float(2.0 ** (np.ceil(np.log2(15)) - 16)), True
+    sdims = (40, 36)
+    di = np.full(int(np.prod(sdims)), np.nan, np.float32)
+    _log(f"circular simulation {kind}")
+    lockstep(ti, dims, di, sdims, types, 15, 1.5, 9, q, circular_sim=True, no_verbatim=nv)

@@ -220,3 +220,43 @@ def test_n_and_k_images():
             r = int(np.floor(np.float32(s[b + i]) * np.float32(kk)))
             assert chosen == pos[min(r, int(np.ceil(kk)) - 1)]
             assert chosen == idxm[cell]
+
+
+def test_circular_simulation_gather():
+    """qs -csim (include/simulation.hpp:394): offsets walk in list order and
+    take the wrapped cell, so every node's neighbour list is, by brute force,
+    the first n offsets whose wrapped cell is informed before the node; edge
+    nodes reach across the border, and the realisation differs from the plain
+    one."""
+    g = synth.grf((30, 34), 3.0, seed=3)
+    ti, dims = synth.to_g2s(np.exp(g))
+    sdims = (20, 18)
+    cells = int(np.prod(sdims))
+    di = np.full(cells, np.nan, np.float32)
+    informed = np.zeros(cells, np.uint8)
+    p, s, b = oracle.path(informed, 3)
+    kd = [d // 2 + 1 for d in dims]
+    offs = oracle.offsets(kd)
+    kw = dict(n=9, k=1.5)
+    circ = oracle.Sim(ti, dims, [0], [], [], sdims, kd, np.ones(int(np.prod(kd)), np.float32), offs, kw["n"], kw["k"],
+                      p, s, b, circular_sim=True)
+    post = circ.posterior()
+    crossed = 0
+    for i in range(0, len(p) - b, 3):
+        c = np.array(np.unravel_index(int(p[b + i]), tuple(reversed(sdims)))[::-1])
+        want = []
+        for j, o in enumerate(offs):
+            t = (c + o) % np.array(sdims)
+            if post[int(np.ravel_multi_index(tuple(t[::-1]), tuple(reversed(sdims))))] <= i:
+                want.append(j)
+                if len(want) == kw["n"]:
+                    break
+        got = circ.gather(i)
+        assert got.tolist() == want, i
+        crossed += int(np.any((c + offs[got] < 0) | (c + offs[got] >= np.array(sdims))))
+    assert crossed > 0
+    plain = oracle.Sim(ti, dims, [0], [], [], sdims, kd, np.ones(int(np.prod(kd)), np.float32), offs, kw["n"],
+                       kw["k"], p, s, b)
+    _, i0 = plain.run(di)
+    _, i1 = circ.run(di)
+    assert not np.array_equal(i0, i1)
