@@ -193,6 +193,7 @@ class QsRequest(C.Structure):
         ("kernels", C.c_void_p),
         ("kernelIndexImage", C.c_void_p),
         ("circularSim", C.c_int),
+        ("circularTI", C.c_int),
     ]
 
 

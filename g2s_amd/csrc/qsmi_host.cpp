@@ -275,6 +275,8 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   if (nTI > 8) return hfail(QSMI_ERR_UNSUPPORTED, "at most 8 training images");
   if ((!rq->nImage && rq->n < 1) || !(rq->k > 0.f) || std::ceil(rq->k) > 16.f)
     return hfail(QSMI_ERR_UNSUPPORTED, "need n >= 1, 0 < k <= 16");
+  if (rq->circularTI && (rq->lp || rq->recordCap > 0))
+    return hfail(QSMI_ERR_UNSUPPORTED, "circular TI: not with the learned proposal or the record");
   if (rq->circularSim && (rq->lp || rq->recordCap > 0))
     return hfail(QSMI_ERR_UNSUPPORTED, "circular simulation: not with the learned proposal or the record");
   if ((rq->nImage || rq->kImage) && (rq->lp || rq->recordCap > 0))
@@ -472,7 +474,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   qsmi_device* dev = nullptr;
   int rc = qsmi_ti_create(rq->gpu, D, s, nullptr, nSub, sub.data(), &ti);
   if (rc) return rc;
-  rc = qsmi_device_create(ti, alpha, 0, &dev);
+  rc = qsmi_device_create(ti, alpha, rq->circularTI ? 1 : 0, &dev);
   if (!rc && cross) rc = qsmi::set_cross(dev, alphaX, centreOk.data());
   // the other TIs: spectra and a device each
   std::vector<qsmi_ti*> xti;
@@ -483,7 +485,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     qsmi_device* dd = nullptr;
     rc = qsmi_ti_create(rq->gpu, D, s, nullptr, nSub, sub.data(), &tt);
     if (!rc) xti.push_back(tt);
-    if (!rc) rc = qsmi_device_create(tt, alpha, 0, &dd);
+    if (!rc) rc = qsmi_device_create(tt, alpha, rq->circularTI ? 1 : 0, &dd);
     if (!rc) xdev.push_back(dd);
   }
   auto destroy_extra = [&]() {

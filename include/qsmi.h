@@ -303,6 +303,9 @@ typedef struct {
   const float* kernelIndexImage;
   int circularSim;         /* -csim (src/qs.cpp:797-801): neighbours wrap around the
                               simulation grid (include/simulation.hpp:394) */
+  int circularTI;          /* -cti (src/qs.cpp:791-795): the devices' circularTI, every TI
+                              position valid, neighbourhoods wrapped over the FFT grid
+                              (the TI grid where its extents are FFT sizes) */
 } qsmi_qs_request;
 
 typedef struct {

@@ -92,6 +92,7 @@ struct OrcSim {
   const float* nImage;  // per simulation cell n (nullptr: n); n must then be the largest value
   const float* kImage;  // per simulation cell k (nullptr, or values <= 0: k)
   int circularSim;      // 1: -csim, neighbour cells wrap around the simulation grid
+  int circularTI;       // 1: -cti, every TI position valid, neighbourhoods wrapped
 };
 
 struct OrcSimCtx {
@@ -405,7 +406,7 @@ static unsigned node_multi_ti(OrcSimCtx* c, int nb, const std::vector<int>& off,
   for (unsigned t = 0; t < nT; ++t) {
     std::vector<double> m64(c->S);
     orc_map_direct_f64(D, c->s, c->nSub, t == 0 ? c->Y.data() : c->Yx[t - 1].data(), c->alpha.data(), nb, off.data(),
-                       u.data(), (double)delta0, 0, m64.data());
+                       u.data(), (double)delta0, c->p.circularTI, m64.data());
     std::vector<float> m(c->S);
     for (size_t q = 0; q < c->S; ++q) m[q] = (float)m64[q];
     std::vector<float> v(K);
@@ -501,7 +502,7 @@ static unsigned node_impl(OrcSimCtx* c, const float* di, unsigned pathIdx, int K
     if (p.cross)
       cross_map(c, nb, off, u, map.data(), nullptr);
     else
-      orc_map_direct_f64(D, c->s, c->nSub, c->Y.data(), c->alpha.data(), nb, off.data(), u.data(), (double)delta0, 0,
+      orc_map_direct_f64(D, c->s, c->nSub, c->Y.data(), c->alpha.data(), nb, off.data(), u.data(), (double)delta0, c->p.circularTI,
                          map.data());
     if (p.noVerbatim && c->index) {
       // the index's entry in position space
@@ -542,10 +543,10 @@ static unsigned node_impl(OrcSimCtx* c, const float* di, unsigned pathIdx, int K
     std::vector<float> map(c->F);
     if (p.mode == 1)
       orc_map_refflow_f32(D, c->s, c->f, c->nSub, c->Yhat32.data(), c->alpha.data(), nb, off.data(), u.data(),
-                          delta0, 0, 1, map.data());
+                          delta0, c->p.circularTI, 1, map.data());
     else
       orc_map_refflow_f64(D, c->s, c->f, c->nSub, c->Yhat64.data(), c->alpha.data(), nb, off.data(), u.data(),
-                          delta0, 0, 1, map.data());
+                          delta0, c->p.circularTI, 1, map.data());
     if (p.noVerbatim && c->index) map[verbatim_index(c, pathIdx, off, nb)] = -INFINITY;  // index space
     std::vector<unsigned> pos(K);
     orc_fkst(map.data(), (unsigned)c->F, (unsigned short)K, seed, outVal, pos.data());
@@ -576,7 +577,7 @@ int orc_sim_node_map(void* h, const float* di, unsigned pathIdx, double* map, do
     cross_map(c, nb, off, u, map, scale);
     return nb;
   }
-  orc_map_direct_f64(D, c->s, c->nSub, c->Y.data(), c->alpha.data(), nb, off.data(), u.data(), (double)delta0, 0,
+  orc_map_direct_f64(D, c->s, c->nSub, c->Y.data(), c->alpha.data(), nb, off.data(), u.data(), (double)delta0, c->p.circularTI,
                      map);
   if (!scale) return nb;
   std::fill(scale, scale + c->S, 0.0);

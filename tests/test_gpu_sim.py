@@ -33,7 +33,7 @@ def default_kernel(ti_dims, nexp):
 
 def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatch_frac=0.01, nodes=None,
              weights=None, wave_size=64, threads=8, cross=False, no_verbatim=False, n_image=None, k_image=None,
-             circular_sim=False):
+             circular_sim=False, circular_ti=False):
     """nodes: path-relative node indices to check (None: every node);
     weights: one kernel weight per variable (None: the default uniform 1)."""
     types = list(types)
@@ -44,7 +44,7 @@ def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatc
         gkw = dict(kernel=np.tile(np.asarray(weights, np.float32), int(np.prod(kd0))), kernel_dims=kd0)
     sim, idx, st = g2s_amd.qs(ti, ti_dims, di0, sim_dims, types, n, k, seed, wave_size=wave_size,
                               tie_quantum=quantum, no_verbatim=no_verbatim, n_image=n_image, k_image=k_image,
-                              circular_sim=circular_sim, **gkw)
+                              circular_sim=circular_sim, circular_ti=circular_ti, **gkw)
     _log(f"  gpu run: {st}")
     # oracle set-up identical to the reference's qs.cpp
     ncat, catvals = [], []
@@ -70,7 +70,7 @@ def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, max_mismatc
     assert np.array_equal(p, gp) and np.array_equal(s, gs) and b == gb
     orc = oracle.Sim(ti, ti_dims, types, ncat, catvals, sim_dims, kd, kern, offs, n, k, p, s, b, mode=0,
                      quantum=quantum, cross=cross, no_verbatim=no_verbatim, n_image=n_image, k_image=k_image,
-                     circular_sim=circular_sim)
+                     circular_sim=circular_sim, circular_ti=circular_ti)
     orc.set_index(idx)  # -nV reads the positions the neighbours were copied from
     K = int(np.ceil(k))
     simf = sim.reshape(-1, nvar)
@@ -396,3 +396,24 @@ def test_sim_circular_simulation(kind):
     di = np.full(int(np.prod(sdims)), np.nan, np.float32)
     _log(f"circular simulation {kind}")
     lockstep(ti, dims, di, sdims, types, 15, 1.5, 9, q, circular_sim=True, no_verbatim=nv)
+
+
+@pytest.mark.parametrize("kind", ["continuous", "categorical"])
+def test_sim_circular_ti(kind):
+    """qs -cti: every TI position valid, neighbourhoods wrapped around the
+    TI (extents that are FFT sizes, so the FFT grid is the TI grid);
+    lockstep with the oracle's serial loop on the wrapped closed-form map."""
+    if kind == "continuous":
+        g = synth.grf((48, 56), 4.0, seed=81)
+        ti, dims = synth.to_g2s(np.exp(g))
+        types, q = [0], 0.0
+    else:
+        g = synth.grf((50, 50), 5.0, seed=82)
+        ti, dims = synth.to_g2s((g > np.quantile(g, 0.6)).astype(np.float32))
+        types, q = [1], float(2.0 ** (np.ceil(np.log2(15)) - 16))
+    assert all(g2s_amd.fft_size(d) == d for d in dims)
+    sdims = (40, 36)
+    di = np.full(int(np.prod(sdims)), np.nan, np.float32)
+    _log(f"circular TI {kind}")
+    sim, idx, st = lockstep(ti, dims, di, sdims, types, 15, 1.5, 10, q, circular_ti=True)
+    assert idx.max() < int(np.prod(dims))
