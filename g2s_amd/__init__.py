@@ -27,6 +27,7 @@ __all__ = [
     "coef_matrix",
     "ti_subimages",
     "offsets",
+    "kernel_generate",
     "path",
     "TrainingImage",
     "AcceleratorDevice",
@@ -104,6 +105,15 @@ def offsets(kernel_size, kernel=None, knvar=1):
         kp = kernel.ctypes.data
     check(lib().qsmi_offsets(len(ks), ks, knvar, kp, out, cap, C.byref(cnt)))
     return out[: cnt.value * len(ks)].reshape(cnt.value, len(ks)).copy()
+
+
+def kernel_generate(dims, types, weights, alphas):
+    """DataImage::genearteKernel: prod(dims) x nVar kernel (types 0 uniform,
+    2 exponential, 7 gaussian per variable)."""
+    types = _i32(types)
+    out = np.zeros(int(np.prod(dims)) * types.size, np.float32)
+    check(lib().qsmi_kernel_generate(len(dims), _u32(dims), types.size, types, _f32(weights), _f32(alphas), out))
+    return out
 
 
 def path(informed, seed):

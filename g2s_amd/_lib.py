@@ -54,6 +54,7 @@ SIGNATURES = {
     "qsmi_coef_matrix": (C.c_int, [C.c_int, i32p, i32p, f32p, i32p, i32p, C.POINTER(C.c_int)]),
     "qsmi_ti_subimages": (C.c_int, [C.c_int, u32p, C.c_int, i32p, f32p, i32p, f32p, f32p]),
     "qsmi_offsets": (C.c_int, [C.c_int, u32p, C.c_int, vp, i32p, C.c_int, C.POINTER(C.c_int)]),
+    "qsmi_kernel_generate": (C.c_int, [C.c_int, u32p, C.c_int, i32p, f32p, f32p, f32p]),
     "qsmi_path": (C.c_int, [C.c_size_t, u8p, C.c_uint, u32p, f32p, C.POINTER(C.c_uint)]),
     "qsmi_qs_run_in_memory": (C.c_int, [vp, vp]),
     "qsmi_lp_create": (C.c_int, [C.c_int, C.c_uint, C.POINTER(vp)]),

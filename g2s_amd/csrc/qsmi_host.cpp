@@ -197,6 +197,47 @@ int qsmi_ti_subimages_x(int ndim, const unsigned* s, int nVar, const int* types,
   return QSMI_OK;
 }
 
+// DataImage::genearteKernel (include/DataImage.hpp:433-480), qs -kernel /
+// -ks / -alpha (src/qs.cpp:1010-1042): per variable j a uniform weight
+// (type 0), exp(-alpha_j * distance) (type 2) or the unit normal density at
+// alpha_j * distance (type 7) over a grid centred at (dims - 1) / 2; other
+// types leave zeros, as the reference's empty cases do. The distance is
+// formed in float and the exponentials in double, as the reference's calls
+// of the global exp / sqrt on float arguments do.
+int qsmi_kernel_generate(int ndim, const unsigned* dims, int nVar, const int* types, const float* weight,
+                         const float* alphas, float* out) {
+  if (ndim < 1 || ndim > 3 || nVar < 1 || !dims || !types || !weight || !alphas || !out)
+    return hfail(QSMI_ERR_ARG, "invalid argument");
+  size_t cells = 1;
+  for (int i = 0; i < ndim; ++i) {
+    if (dims[i] < 1) return hfail(QSMI_ERR_ARG, "zero kernel extent");
+    cells *= dims[i];
+  }
+  for (size_t c = 0; c < cells; ++c) {
+    float d2 = 0.f;  // squared distance to the centre, in float
+    size_t rem = c;
+    for (int i = 0; i < ndim; ++i) {
+      const int off = std::abs((int)(rem % dims[i]) - ((int)dims[i] - 1) / 2);
+      rem /= dims[i];
+      d2 += (float)(off * off);
+    }
+    for (int j = 0; j < nVar; ++j) {
+      float v = 0.f;
+      if (types[j] == 0) {
+        v = weight[j];
+      } else if (types[j] == 2) {
+        const float dist = (float)::sqrt((double)d2);
+        v = (float)::exp((double)(-alphas[j] * dist));
+      } else if (types[j] == 7) {
+        const float a2 = d2 * alphas[j] * alphas[j];
+        v = (float)(1.0 / ::sqrt(2.0 * M_PI) * ::exp((double)(-a2 / 2.f)));
+      }
+      out[c * nVar + j] = v;
+    }
+  }
+  return QSMI_OK;
+}
+
 int qsmi_offsets(int ndim, const unsigned* ks, int kNVar, const float* kernel, int* out, int capacity, int* count) {
   if (!ks || !out || !count || ndim < 1 || ndim > 3) return hfail(QSMI_ERR_ARG, "invalid argument");
   // generation order of src/qs.cpp:1055-1084: dimension by dimension, each

@@ -210,6 +210,14 @@ int qsmi_coef_matrix_x(int nVar, const int* types, float* alpha, float* alphaX, 
                        int* nSub);
 int qsmi_ti_subimages_x(int ndim, const unsigned* srcSize, int nVar, const int* types, const float* ti, float* out);
 
+/* Replaces DataImage::genearteKernel (include/DataImage.hpp:433-480), the
+ * kernel qs builds from -kernel / -ks / -alpha (src/qs.cpp:1010-1042):
+ * out = prod(dims) x nVar; types[j]: 0 uniform (weight[j]), 2 exponential
+ * exp(-alphas[j] d), 7 gaussian (unit normal density at alphas[j] d), d the
+ * distance to the centre (dims-1)/2; other types give zeros. */
+int qsmi_kernel_generate(int ndim, const unsigned* dims, int nVar, const int* types, const float* weight,
+                         const float* alphas, float* out);
+
 /* Sorted neighbour offset list of src/qs.cpp:1055-1137: every offset of the
  * kernel box (half-width (size+1)/2-1), NaN-kernel offsets removed
  * (kernel may be NULL), std::sort by increasing |d|^2 (libstdc++, so ties

@@ -176,3 +176,17 @@ def test_index_with_delta_live():
         b = np.zeros(1, np.uint32)
         assert L.orc_index_with_delta(D, dims, pos, delta, a) == R.ref_index_with_delta(D, dims, pos, delta, b)
         assert a[0] == b[0]
+
+
+def test_product_kernel_generation_golden():
+    """qsmi_kernel_generate (the library's CPU set-up) against the vectors the
+    compiled reference's genearteKernel wrote."""
+    import g2s_amd
+
+    ki = 0
+    while f"kern_{ki}_dims" in GOLD:
+        dims = GOLD[f"kern_{ki}_dims"]
+        out = g2s_amd.kernel_generate(dims, GOLD[f"kern_{ki}_types"], GOLD[f"kern_{ki}_w"], GOLD[f"kern_{ki}_a"])
+        assert np.array_equal(out, GOLD[f"kern_{ki}_out"]), ki
+        ki += 1
+    assert ki >= 2
