@@ -315,7 +315,7 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
             wave_size=0, tie_quantum=-1.0, stage_timing=False, record=0, lp=None, lp_shortlist=0,
             lp_fallback_every=0, lp_fallback_mse=0.0, lp_audit=False, lp_min_cover=0.0, no_verbatim=False,
             n_image=None, k_image=None, kernels=None, kernels_dims=None, kernel_index_image=None,
-            circular_sim=False, circular_ti=False):
+            circular_sim=False, circular_ti=False, ti_index_image=None):
     """In-memory QuickSampling (qsmi_qs_run_in_memory).
 
     Arrays are flat in G2S DataImage order (dimension 0 fastest, variable
@@ -365,6 +365,10 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
     rq.noVerbatim = int(bool(no_verbatim))
     rq.circularSim = int(bool(circular_sim))
     rq.circularTI = int(bool(circular_ti))
+    tiimg = None
+    if ti_index_image is not None:
+        tiimg = _f32(ti_index_image).reshape(-1)
+        rq.tiIndexImage = tiimg.ctypes.data
     nimg = kimg = None
     if n_image is not None:
         nimg = _f32(n_image).reshape(-1)

@@ -194,6 +194,7 @@ class QsRequest(C.Structure):
         ("kernels", C.c_void_p),
         ("kernelIndexImage", C.c_void_p),
         ("circularSim", C.c_int),
+        ("tiIndexImage", C.c_void_p),
         ("circularTI", C.c_int),
     ]
 

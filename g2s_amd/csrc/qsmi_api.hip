@@ -206,6 +206,7 @@ void prep_node(const qsmi_device* dev, int nb, const int* off, const float* u, f
   ni->exPos = 0;
   ni->exR2 = 0.f;
   ni->kLocal = 0.f;
+  ni->tiReq = -1;
   for (int j = 0; j < nb; ++j) {
     for (int i = 0; i < D; ++i) offOut[j * D + i] = off[j * D + i];
     for (int a = 0; a < V; ++a) {

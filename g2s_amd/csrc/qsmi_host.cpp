@@ -589,6 +589,17 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   if (!rc) rc = dev_upload(nullptr, nPath * (size_t)n * sizeof(int), &dPj);
   if (!rc) rc = dev_upload(nullptr, nPath * sizeof(int), &dCnt);
   void *dNNode = nullptr, *dPathK = nullptr, *dNodeKer = nullptr, *dOffStart = nullptr, *dOffCount = nullptr;
+  void* dNodeTi = nullptr;
+  if (rq->tiIndexImage) {
+    if (nTI < 2) rc = hfail(QSMI_ERR_ARG, "a TI-index image needs several TIs");
+    std::vector<int> nt(nPath);
+    for (size_t i = 0; i < nPath && !rc; ++i) {
+      const float v = rq->tiIndexImage[path[begin + i]];
+      if (!(v >= 0.f && v < (float)nTI)) rc = hfail(QSMI_ERR_ARG, "TI-index image: value outside [0, nTI)");
+      nt[i] = rc ? 0 : (int)v;
+    }
+    if (!rc) rc = dev_upload(nt.data(), nPath * sizeof(int), &dNodeTi);
+  }
   if (nKer > 1) {
     std::vector<int> nk(nPath);
     for (size_t i = 0; i < nPath; ++i) {
@@ -769,6 +780,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     R.extraDevs = xdev;
     R.dPathK = (const float*)dPathK;
     R.dNodeKer = (const int*)dNodeKer;
+    R.dNodeTi = (const int*)dNodeTi;
     rc = qsmi::run_resident_waves(dev, R, stageMs, &scanned, &waves);
   }
   if (!rc) {

@@ -168,6 +168,7 @@ struct ResidentLoop {
   // kernel-index image: per path node kernel, per kernel its grid and offset
   const int* dNodeKer = nullptr;
   bool circSim = false;  // -csim: the simulation grid wraps around
+  const int* dNodeTi = nullptr;  // per path node TI of a TI-index image (several TIs)
   int nKer = 1;
   int kdK[8][3] = {};
   long long kerOff[8] = {};

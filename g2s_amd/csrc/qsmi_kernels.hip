@@ -541,7 +541,10 @@ __global__ void k_merge_ti(const NodeInfo* __restrict__ info, int nN, int nTI, i
   for (int q = 0; q <= r; ++q) {
     Cand best = worst_cand();
     bool found = false;
-    for (int t = 0; t < nTI; ++t)
+    for (int t = 0; t < nTI; ++t) {
+      // a TI-index image restricts the node to one TI (include/
+      // quantileSamplingModule.hpp:460-464: only that device is updated)
+      if (ni.tiReq >= 0 && t != ni.tiReq) continue;
       for (int i = 0; i < K; ++i) {
         const long long o = t * stride + (long long)node * K + i;
         const uint32_t p = pos[o];
@@ -558,6 +561,7 @@ __global__ void k_merge_ti(const NodeInfo* __restrict__ info, int nN, int nTI, i
           found = true;
         }
       }
+    }
     if (!found) break;
     prev = best;
     pick = best;

@@ -54,6 +54,7 @@ struct NodeInfo {
   uint32_t exPos;     // disk centre, TI position
   float exR2;         // squared radius: positions p with sum_d (p_d - o_d)^2 <= exR2
   float kLocal;       // this node's k from a k image (0: the run's k)
+  int tiReq;          // several TIs: the only TI this node samples (-1: all; a TI-index image)
 };
 
 // Verbatim exclusion test of TI position (c0, c1, c2) for a node.

@@ -67,7 +67,8 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
                                                int* __restrict__ nbOff, float* __restrict__ nbVal,
                                                const uint8_t* __restrict__ centreOk,
                                                const uint32_t* __restrict__ complete, uint32_t nComplete,
-                                               const float* __restrict__ pathK, const int* __restrict__ nodeKer) {
+                                               const float* __restrict__ pathK, const int* __restrict__ nodeKer,
+                                               const int* __restrict__ nodeTi) {
   const int q = blockIdx.x;
   const int lane = threadIdx.x;
   const int i = order[w0 + q];
@@ -93,6 +94,7 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
   ni.exPos = 0;
   ni.exR2 = 0.f;
   ni.kLocal = pathK ? pathK[i] : 0.f;
+  ni.tiReq = nodeTi ? nodeTi[i] : -1;
   if (cnt <= 1) {
     // only the cell itself: draw from the marginal (simulation.hpp:586-606)
     if (lane == 0) {
@@ -519,7 +521,7 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
     if (ev) (void)hipEventRecord(ev[0], st);
     hipLaunchKernelGGL(k_gather, dim3(nN), dim3(64), gatherLds, st, P, w0, nN, R.dOrder, R.dPathCells, R.dPathSeeds,
                        R.dNbIdx, R.dNbCnt, R.dOffs, R.dKernel, R.dCatvals, R.dDi, R.dIndex, R.dTi, b->dInfo, b->dOff,
-                       b->dVal, dev->dCentreOk, dev->dComplete, dev->nComplete, R.dPathK, R.dNodeKer);
+                       b->dVal, dev->dCentreOk, dev->dComplete, dev->nComplete, R.dPathK, R.dNodeKer, R.dNodeTi);
     if (ev) (void)hipEventRecord(ev[1], st);
     const bool useLp = R.lp && (R.waveLp.empty() || R.waveLp[w]);
     if (R.lp && !useLp) R.lpSkipped += nN;

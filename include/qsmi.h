@@ -311,6 +311,8 @@ typedef struct {
   const float* kernelIndexImage;
   int circularSim;         /* -csim (src/qs.cpp:797-801): neighbours wrap around the
                               simulation grid (include/simulation.hpp:394) */
+  const float* tiIndexImage; /* -ii with several TIs (include/simulation.hpp:560): per
+                              simulation cell the only TI its scan samples (NULL: all) */
   int circularTI;          /* -cti (src/qs.cpp:791-795): the devices' circularTI, every TI
                               position valid, neighbourhoods wrapped over the FFT grid
                               (the TI grid where its extents are FFT sizes) */

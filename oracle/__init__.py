@@ -68,6 +68,7 @@ class OrcSim(C.Structure):
         ("kImage", C.c_void_p),
         ("circularSim", C.c_int),
         ("circularTI", C.c_int),
+        ("tiImage", C.c_void_p),
     ]
 
 
@@ -262,7 +263,7 @@ class Sim:
 
     def __init__(self, ti, tidims, types, ncat, catvals, simdims, kdims, kernel_exp, offs, n, k, path, seeds,
                  begin, mode=0, quantum=0.0, cross=False, no_verbatim=False, n_ti=1, n_image=None,
-                 k_image=None, circular_sim=False, circular_ti=False):
+                 k_image=None, circular_sim=False, circular_ti=False, ti_image=None):
         self._keep = []
 
         def keep(a):
@@ -300,6 +301,8 @@ class Sim:
         s.nTI = int(n_ti)  # ti then holds the n_ti images one after the other
         s.circularSim = int(bool(circular_sim))
         s.circularTI = int(bool(circular_ti))
+        if ti_image is not None:
+            s.tiImage = keep(f32(ti_image).reshape(-1))
         if n_image is not None:  # per cell n; n must be the largest value
             s.nImage = keep(f32(n_image).reshape(-1))
         if k_image is not None:

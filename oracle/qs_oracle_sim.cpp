@@ -93,6 +93,7 @@ struct OrcSim {
   const float* kImage;  // per simulation cell k (nullptr, or values <= 0: k)
   int circularSim;      // 1: -csim, neighbour cells wrap around the simulation grid
   int circularTI;       // 1: -cti, every TI position valid, neighbourhoods wrapped
+  const float* tiImage; // several TIs: per simulation cell the only TI sampled (nullptr: all)
 };
 
 struct OrcSimCtx {
@@ -385,7 +386,8 @@ void orc_sim_set_index(void* h, const unsigned* index) { static_cast<OrcSimCtx*>
 // (float arithmetic) taken, clamped to the last valid entry. Returns the
 // encoded index; outVal/outPos get the merged K best (encoded).
 static unsigned node_multi_ti(OrcSimCtx* c, int nb, const std::vector<int>& off, const std::vector<float>& u,
-                              float delta0, float seed, int K, float kLoc, float* outVal, unsigned* outPos) {
+                              float delta0, float seed, int K, float kLoc, int tiReq, float* outVal,
+                              unsigned* outPos) {
   const OrcSim& p = c->p;
   const int D = c->D;
   const unsigned nT = (unsigned)c->nTI;
@@ -404,6 +406,9 @@ static unsigned node_multi_ti(OrcSimCtx* c, int nb, const std::vector<int>& off,
   int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   for (int i = 0; i < D; ++i) hi[i] = int(c->s[i]) - 1;
   for (unsigned t = 0; t < nT; ++t) {
+    // a TI-index image: only that TI's device is updated
+    // (include/quantileSamplingModule.hpp:460-464)
+    if (tiReq >= 0 && (int)t != tiReq) continue;
     std::vector<double> m64(c->S);
     orc_map_direct_f64(D, c->s, c->nSub, t == 0 ? c->Y.data() : c->Yx[t - 1].data(), c->alpha.data(), nb, off.data(),
                        u.data(), (double)delta0, c->p.circularTI, m64.data());
@@ -496,7 +501,10 @@ static unsigned node_impl(OrcSimCtx* c, const float* di, unsigned pathIdx, int K
     }
     return position;
   }
-  if (c->nTI > 1) return node_multi_ti(c, nb, off, u, delta0, seed, K, kLoc, outVal, outPos);
+  if (c->nTI > 1) {
+    const int tiReq = p.tiImage ? (int)p.tiImage[p.path[p.beginPath + pathIdx]] : -1;
+    return node_multi_ti(c, nb, off, u, delta0, seed, K, kLoc, tiReq, outVal, outPos);
+  }
   if (p.mode == 0) {
     std::vector<double> map(c->S);
     if (p.cross)

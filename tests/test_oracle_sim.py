@@ -260,3 +260,37 @@ def test_circular_simulation_gather():
     _, i0 = plain.run(di)
     _, i1 = circ.run(di)
     assert not np.array_equal(i0, i1)
+
+
+def test_ti_index_image():
+    """qs -ii with several TIs (include/simulation.hpp:560,
+    include/quantileSamplingModule.hpp:460-464): a cell scans only its TI.
+    An image of ones over [A, B] gives the one-TI realisation on B (index
+    position * 2 + 1); a mixed image takes every node from its own TI."""
+    g = synth.grf((30, 34), 3.0, seed=3)
+    a, dims = synth.to_g2s(np.exp(g))
+    gb = synth.grf((30, 34), 3.0, seed=4)
+    bb, _ = synth.to_g2s(np.exp(gb))
+    a, bb = a.reshape(-1), bb.reshape(-1)
+    sdims = (20, 18)
+    cells = int(np.prod(sdims))
+    rng = np.random.default_rng(5)
+    di = np.full(cells, np.nan, np.float32)
+    pick = rng.choice(cells, cells // 20, replace=False)
+    di[pick] = a[rng.integers(0, a.size, pick.size)]
+    kd = [d // 2 + 1 for d in dims]
+    offs = oracle.offsets(kd)
+    p, s, b = oracle.path((~np.isnan(di)).astype(np.uint8), 3)
+
+    def make(tis, ti_image=None):
+        cat = np.concatenate(tis)
+        return oracle.Sim(cat, dims, [0], [], [], sdims, kd, np.ones(int(np.prod(kd)), np.float32), offs, 9, 1.5, p,
+                          s, b, n_ti=len(tis), ti_image=ti_image)
+
+    free = p[b:]
+    _, i1 = make([bb]).run(di)
+    out2, i2 = make([a, bb], np.ones(cells, np.float32)).run(di)
+    assert np.array_equal(i2[free], i1[free] * 2 + 1)
+    mixed = rng.integers(0, 2, cells).astype(np.float32)
+    _, im = make([a, bb], mixed).run(di)
+    assert np.array_equal(im[free] % 2, mixed[free].astype(np.uint32))
