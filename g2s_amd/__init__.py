@@ -315,7 +315,7 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
             wave_size=0, tie_quantum=-1.0, stage_timing=False, record=0, lp=None, lp_shortlist=0,
             lp_fallback_every=0, lp_fallback_mse=0.0, lp_audit=False, lp_min_cover=0.0, no_verbatim=False,
             n_image=None, k_image=None, kernels=None, kernels_dims=None, kernel_index_image=None,
-            circular_sim=False, circular_ti=False, ti_index_image=None):
+            circular_sim=False, circular_ti=False, ti_index_image=None, ti_dims_all=None):
     """In-memory QuickSampling (qsmi_qs_run_in_memory).
 
     Arrays are flat in G2S DataImage order (dimension 0 fastest, variable
@@ -387,11 +387,15 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
         rq.kernels = C.cast(kptr, C.c_void_p)
         rq.kernelSizes = ksz.ctypes.data
         rq.kernelIndexImage = kimgi.ctypes.data
-    tptr = None
+    tptr = tsz = None
     if tis is not None:
         tptr = (C.c_void_p * len(tis))(*[t.ctypes.data for t in tis])
         rq.nTI = len(tis)
         rq.tis = C.cast(tptr, C.c_void_p)
+        if ti_dims_all is not None:
+            tsz = np.ones((len(tis), 3), np.uint32)
+            tsz[:, :D] = np.asarray(ti_dims_all, np.uint32)
+            rq.tiSizes = tsz.ctypes.data
     rec = None
     if record:
         rec = (np.zeros((int(record), 80, 2), np.int32), np.zeros((int(record), 80), np.float32),

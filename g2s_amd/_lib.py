@@ -196,6 +196,7 @@ class QsRequest(C.Structure):
         ("circularSim", C.c_int),
         ("tiIndexImage", C.c_void_p),
         ("circularTI", C.c_int),
+        ("tiSizes", C.c_void_p),
     ]
 
 

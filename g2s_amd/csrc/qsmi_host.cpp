@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <climits>
@@ -351,15 +352,31 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   // categories, src/qs.cpp:1498-1525, and the marginal draw)
   std::vector<float> tiCat;
   const float* tiIn = rq->ti;
+  // extents and cell offsets of the TIs in the concatenation
+  std::vector<std::array<unsigned, 3>> sT(nTI);
+  std::vector<size_t> cumT(nTI + 1, 0);
+  for (int t = 0; t < nTI; ++t) {
+    size_t c = 1;
+    for (int i = 0; i < 3; ++i) {
+      sT[t][i] = i < D ? ((nTI > 1 && rq->tiSizes) ? rq->tiSizes[t * 3 + i] : s[i]) : 1u;
+      if (sT[t][i] < 1) return hfail(QSMI_ERR_ARG, "zero TI extent");
+      c *= sT[t][i];
+    }
+    cumT[t + 1] = cumT[t] + c;
+  }
+  for (int i = 0; i < D; ++i)
+    if (sT[0][i] != s[i]) return hfail(QSMI_ERR_ARG, "tiSizes[0..2] must equal tiSize");
   if (nTI > 1) {
-    if ((unsigned long long)S * nTI > 0xFFFFFFFFull)
+    size_t maxCells = 0;
+    for (int t = 0; t < nTI; ++t) maxCells = std::max(maxCells, cumT[t + 1] - cumT[t]);
+    if ((unsigned long long)maxCells * nTI > 0xFFFFFFFFull || cumT[nTI] > 0xFFFFFFFFull)
       return hfail(QSMI_ERR_UNSUPPORTED, "several TIs: cells x nTI must fit the 32-bit index (position * nTI + TI)");
     if (rq->lp || rq->recordCap > 0 || rq->noVerbatim)
       return hfail(QSMI_ERR_UNSUPPORTED, "several TIs: not with the learned proposal, the record or -nV");
-    tiCat.resize((size_t)nTI * S * nVar);
+    tiCat.resize(cumT[nTI] * nVar);
     for (int t = 0; t < nTI; ++t) {
       if (!rq->tis[t]) return hfail(QSMI_ERR_ARG, "null training image");
-      std::memcpy(tiCat.data() + (size_t)t * S * nVar, rq->tis[t], S * nVar * sizeof(float));
+      std::memcpy(tiCat.data() + cumT[t] * nVar, rq->tis[t], (cumT[t + 1] - cumT[t]) * nVar * sizeof(float));
     }
     for (float x : tiCat)
       if (std::isnan(x)) return hfail(QSMI_ERR_UNSUPPORTED, "several TIs with NaN (cross-measurement form)");
@@ -403,7 +420,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     if (rq->varTypes[v] != 0) {
       std::vector<float> cv(4096);
       int cnt = 0;
-      int rc = qsmi_categories(S * nTI, nVar, tiIn, v, cv.data(), 4096, &cnt);
+      int rc = qsmi_categories(cumT[nTI], nVar, tiIn, v, cv.data(), 4096, &cnt);
       if (rc) return rc;
       ncat.push_back(cnt);
       catvals.insert(catvals.end(), cv.begin(), cv.begin() + cnt);
@@ -521,10 +538,12 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   std::vector<qsmi_ti*> xti;
   std::vector<qsmi_device*> xdev;
   for (int t = 1; t < nTI && !rc; ++t) {
-    qsmi_ti_subimages(D, s, nVar, rq->varTypes, tiIn + (size_t)t * S * nVar, ncat.data(), catvals.data(), sub.data());
+    std::vector<float> subT((size_t)nSub * (cumT[t + 1] - cumT[t]));
+    qsmi_ti_subimages(D, sT[t].data(), nVar, rq->varTypes, tiIn + cumT[t] * nVar, ncat.data(), catvals.data(),
+                      subT.data());
     qsmi_ti* tt = nullptr;
     qsmi_device* dd = nullptr;
-    rc = qsmi_ti_create(rq->gpu, D, s, nullptr, nSub, sub.data(), &tt);
+    rc = qsmi_ti_create(rq->gpu, D, sT[t].data(), nullptr, nSub, subT.data(), &tt);
     if (!rc) xti.push_back(tt);
     if (!rc) rc = qsmi_device_create(tt, alpha, rq->circularTI ? 1 : 0, &dd);
     if (!rc) xdev.push_back(dd);
@@ -731,7 +750,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   if (!rc) rc = dev_upload(catv.data(), catv.size() * sizeof(float), &dCat);
   if (!rc) rc = dev_upload(di, SD * nVar * sizeof(float), &dDi);
   if (!rc) rc = dev_upload(rs->index, SD * sizeof(unsigned), &dIdx);
-  if (!rc) rc = dev_upload(tiData, (size_t)nTI * S * nVar * sizeof(float), &dTi);
+  if (!rc) rc = dev_upload(tiData, cumT[nTI] * nVar * sizeof(float), &dTi);
   const size_t recCap = rq->recordCap > 0 ? std::min<size_t>((size_t)rq->recordCap, nPath) : 0;
   void *dRecOff = nullptr, *dRecVal = nullptr, *dRecBest = nullptr;
   if (!rc && recCap) {
@@ -778,6 +797,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     R.dIndex = (uint32_t*)dIdx;
     R.dTi = (const float*)dTi;
     R.extraDevs = xdev;
+    for (int t = 0; t <= nTI; ++t) R.tiCum[t] = (unsigned)cumT[t];
     R.dPathK = (const float*)dPathK;
     R.dNodeKer = (const int*)dNodeKer;
     R.dNodeTi = (const int*)dNodeTi;

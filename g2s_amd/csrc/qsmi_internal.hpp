@@ -146,8 +146,10 @@ void launch_merge_draw(const ScanGeom& g, const NodeInfo* info, const Cand* cand
                        int rowsPerBlock, int K, float kf, float* outVal, uint32_t* outPos, uint32_t* chosen,
                        float* simVals, const float* tiVals, int nVar, uint32_t* indexOut, hipStream_t st);
 void launch_merge_ti(const NodeInfo* info, int nN, int nTI, int K, float kf, const float* val, const uint32_t* pos,
-                     long long stride, float quantum, float* simVals, const float* tiVals, int nVar, uint32_t tiCells,
+                     long long stride, float quantum, float* simVals, const float* tiVals, int nVar, TiCum cum,
                      uint32_t* indexOut, hipStream_t st);
+void launch_info_shift(const NodeInfo* in, NodeInfo* out, int nN, int dh0, int dh1, int dh2, float scale,
+                       hipStream_t st);
 
 // Inputs of the device-resident path loop (qsmi_sim.hip). Device pointers
 // are indexed by path position after the hard data (i in [0, nPath)).
@@ -169,6 +171,7 @@ struct ResidentLoop {
   const int* dNodeKer = nullptr;
   bool circSim = false;  // -csim: the simulation grid wraps around
   const int* dNodeTi = nullptr;  // per path node TI of a TI-index image (several TIs)
+  unsigned tiCum[9] = {};         // several TIs: first cell of each in the concatenation, then the total
   int nKer = 1;
   int kdK[8][3] = {};
   long long kerOff[8] = {};

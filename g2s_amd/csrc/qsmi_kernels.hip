@@ -528,7 +528,7 @@ __global__ void __launch_bounds__(256) k_merge_draw(ScanGeom g, const NodeInfo* 
 __global__ void k_merge_ti(const NodeInfo* __restrict__ info, int nN, int nTI, int K, float kf,
                            const float* __restrict__ val, const uint32_t* __restrict__ pos, long long stride,
                            float quantum, float* __restrict__ simVals, const float* __restrict__ tiVals, int nVar,
-                           uint32_t tiCells, uint32_t* __restrict__ indexOut) {
+                           TiCum cum, uint32_t* __restrict__ indexOut) {
   const int node = blockIdx.x * blockDim.x + threadIdx.x;
   if (node >= nN) return;
   const NodeInfo ni = info[node];
@@ -570,8 +570,27 @@ __global__ void k_merge_ti(const NodeInfo* __restrict__ info, int nN, int nTI, i
   if (!any) return;
   const uint32_t t = pick.pos % (uint32_t)nTI, p = pick.pos / (uint32_t)nTI;
   for (int v = 0; v < nVar; ++v)
-    simVals[(long long)ni.cell * nVar + v] = tiVals[((long long)t * tiCells + p) * nVar + v];
+    simVals[(long long)ni.cell * nVar + v] = tiVals[((long long)cum.c[t] + p) * nVar + v];
   indexOut[ni.cell] = pick.pos;
+}
+
+// Several TIs of their own extents: a node on another TI. The lower bound
+// -min of the valid box depends on the offsets only; the upper bound
+// s - 1 - max moves with the extent (dh = s_t - s_0), and the map scale is
+// that TI's 1 / prod(FFT extent). Marginal nodes (nb == 0) keep their empty
+// box; circular TIs span the whole extent.
+__global__ void k_info_shift(const NodeInfo* __restrict__ in, NodeInfo* __restrict__ out, int nN, int dh0, int dh1,
+                             int dh2, float scale) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nN) return;
+  NodeInfo ni = in[q];
+  if (ni.nb > 0) {
+    ni.hi[0] += dh0;
+    ni.hi[1] += dh1;
+    ni.hi[2] += dh2;
+  }
+  ni.scale = scale;
+  out[q] = ni;
 }
 
 __global__ void k_fill(float* __restrict__ p, long long n, float v) {
@@ -740,10 +759,16 @@ void launch_fill_disk(float* map, const int* s, uint32_t o, float r2, float v, h
 }
 
 void launch_merge_ti(const NodeInfo* info, int nN, int nTI, int K, float kf, const float* val, const uint32_t* pos,
-                     long long stride, float quantum, float* simVals, const float* tiVals, int nVar, uint32_t tiCells,
+                     long long stride, float quantum, float* simVals, const float* tiVals, int nVar, TiCum cum,
                      uint32_t* indexOut, hipStream_t st) {
   hipLaunchKernelGGL(k_merge_ti, dim3((unsigned)((nN + 63) / 64)), dim3(64), 0, st, info, nN, nTI, K, kf, val, pos,
-                     stride, quantum, simVals, tiVals, nVar, tiCells, indexOut);
+                     stride, quantum, simVals, tiVals, nVar, cum, indexOut);
+}
+
+void launch_info_shift(const NodeInfo* in, NodeInfo* out, int nN, int dh0, int dh1, int dh2, float scale,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_info_shift, dim3((unsigned)((nN + 63) / 64)), dim3(64), 0, st, in, out, nN, dh0, dh1, dh2,
+                     scale);
 }
 
 void launch_fill(float* p, long long n, float v, hipStream_t st) {

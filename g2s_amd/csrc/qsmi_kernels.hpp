@@ -57,6 +57,11 @@ struct NodeInfo {
   int tiReq;          // several TIs: the only TI this node samples (-1: all; a TI-index image)
 };
 
+// Several TIs: the first cell of each TI in their concatenation, then the total.
+struct TiCum {
+  unsigned c[9];
+};
+
 // Verbatim exclusion test of TI position (c0, c1, c2) for a node.
 __device__ __forceinline__ bool excluded(const NodeInfo& ni, int s0, int s1, int c0, int c1, int c2) {
   const int o0 = (int)(ni.exPos % (uint32_t)s0);

@@ -43,6 +43,7 @@ struct GatherParams {
   int noVerbatim;  // 1: exclude the position that continues the nearest neighbour's copy (-nV)
   unsigned f[3];   // TI FFT grid (the reference's index space)
   int nTI;         // training images (tiVals holds them one after the other)
+  unsigned tiCum[9];  // first cell of each TI in tiVals, then the total
   int circSim;          // 1: the simulation grid wraps around (-csim)
   int kdK[8][3];        // per kernel of a kernel-index image: its grid (kernel 0: kd)
   long long kerOff[8];  // its offset in the kernel array (floats)
@@ -98,8 +99,9 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
   if (cnt <= 1) {
     // only the cell itself: draw from the marginal (simulation.hpp:586-606)
     if (lane == 0) {
-      // over all TIs in order: TI = position / cells (one extent for all)
-      unsigned position = (unsigned)(int)floorf(seed * (float)(P.tiCells * (unsigned)P.nTI));
+      // over all TIs in order (include/simulation.hpp:586-606): the TI whose
+      // cells hold the position
+      unsigned position = (unsigned)(int)floorf(seed * (float)P.tiCum[P.nTI]);
       if (P.cross && !centreOk[position]) {
         // incomplete cell: rank among the complete cells (simulation.hpp:
         // 617-654; rank 0 keeps cell 0 as the reference's walk does)
@@ -107,7 +109,9 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
         position = r == 0 ? 0u : complete[r - 1];
       }
       for (int v = 0; v < P.nVar; ++v) di[(long long)cell * P.nVar + v] = tiVals[(long long)position * P.nVar + v];
-      index[cell] = (position % P.tiCells) * (unsigned)P.nTI + position / P.tiCells;
+      int tt = 0;
+      while (tt + 1 < P.nTI && position >= P.tiCum[tt + 1]) ++tt;
+      index[cell] = (position - P.tiCum[tt]) * (unsigned)P.nTI + (unsigned)tt;
       ni.nb = 0;
       ni.delta0 = 0.f;
       for (int d = 0; d < 3; ++d) {
@@ -437,6 +441,11 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   P.tiCells = (unsigned)t->S;
   const int nTI = 1 + (int)R.extraDevs.size();
   P.nTI = nTI;
+  if (nTI > 8) return fail(QSMI_ERR_UNSUPPORTED, "at most 8 training images");
+  for (int q = 0; q <= 8; ++q) P.tiCum[q] = q == 0 ? 0u : (q <= nTI ? R.tiCum[q] : R.tiCum[nTI]);
+  if (nTI == 1) P.tiCum[1] = (unsigned)t->S;
+  TiCum cum;
+  for (int q = 0; q < 9; ++q) cum.c[q] = P.tiCum[q];
   if (R.nKer > 8) return fail(QSMI_ERR_UNSUPPORTED, "at most 8 kernels");
   for (int q = 1; q < R.nKer; ++q) {
     for (int d = 0; d < 3; ++d) P.kdK[q][d] = d < D ? R.kdK[q][d] : 1;
@@ -548,18 +557,28 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
         qsmi_device* dv = tt == 0 ? dev : R.extraDevs[tt - 1];
         WaveBufs* xb = tt == 0 ? b : dv->wb;
         const ScanGeom gt = geom_of(dv);
-        if ((rc = launch_spectrum(dv, gt, nN, b->dInfo, b->dOff, b->dVal, xb->dC))) {
+        // the node boxes on this TI's extent
+        const NodeInfo* it = b->dInfo;
+        if (tt > 0 && (dv->ti->s[0] != t->s[0] || dv->ti->s[1] != t->s[1] || dv->ti->s[2] != t->s[2])) {
+          const float sc = dv->trueMismatch ? (float)(1.0 / (double)dv->ti->N) : 1.f;
+          launch_info_shift(b->dInfo, xb->dInfo, nN, dv->ti->s[0] - t->s[0], dv->ti->s[1] - t->s[1],
+                            dv->ti->s[2] - t->s[2], sc, st);
+          it = xb->dInfo;
+        }
+        const int rbt = tt == 0 ? rb : row_blocks(dv->ti);
+        const int rpbt = tt == 0 ? rpb : rows_per_block(dv->ti);
+        if ((rc = launch_spectrum(dv, gt, nN, it, b->dOff, b->dVal, xb->dC))) {
           restore();
           return rc;
         }
-        launch_rows(dv, gt, b->dInfo, xb->dC, nN, rb, K, xb->dCand, nullptr, nullptr, 0);
-        launch_merge_draw(gt, b->dInfo, xb->dCand, nN, rb, rpb, K, R.k, dValAll + tt * tiStride,
+        launch_rows(dv, gt, it, xb->dC, nN, rbt, K, xb->dCand, nullptr, nullptr, 0);
+        launch_merge_draw(gt, it, xb->dCand, nN, rbt, rpbt, K, R.k, dValAll + tt * tiStride,
                           dPosAll + tt * tiStride, nullptr, nullptr, nullptr, R.nVar, nullptr, st);
       }
       if (ev) (void)hipEventRecord(ev[2], st);
       if (ev) (void)hipEventRecord(ev[3], st);
       launch_merge_ti(b->dInfo, nN, nTI, K, R.k, dValAll, dPosAll, tiStride, dev->quantum, R.dDi, R.dTi, R.nVar,
-                      (uint32_t)t->S, R.dIndex, st);
+                      cum, R.dIndex, st);
     } else {
       if ((rc = launch_spectrum(dev, g, layers * nN, b->dInfo, b->dOff, b->dVal, b->dC))) return rc;
       if (ev) (void)hipEventRecord(ev[2], st);

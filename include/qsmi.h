@@ -285,7 +285,7 @@ typedef struct {
                               neighbour's copy (include/simulation.hpp:567-580);
                               not with the learned proposal */
   /* several training images (qs -ti a b ...): nTI >= 2 takes tis[0..nTI-1],
-     each prod(tiSize) x nVar (one extent for all), and ignores ti. Per node
+     each prod(its extent) x nVar (tiSizes, or tiSize for all), and ignores ti. Per node
      every TI's k best (QuantileSamplingModule::determineDistribution), merged
      by value and drawn with rank floor(seed k ceil(nTI/k) / nTI)
      (include/quantileSamplingModule.hpp:90-126); index = position * nTI + TI;
@@ -316,6 +316,8 @@ typedef struct {
   int circularTI;          /* -cti (src/qs.cpp:791-795): the devices' circularTI, every TI
                               position valid, neighbourhoods wrapped over the FFT grid
                               (the TI grid where its extents are FFT sizes) */
+  const unsigned* tiSizes; /* several TIs of their own extents: nTI x 3 (entry 0 = tiSize);
+                              NULL: tiSize for all */
 } qsmi_qs_request;
 
 typedef struct {

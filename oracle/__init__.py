@@ -69,6 +69,7 @@ class OrcSim(C.Structure):
         ("circularSim", C.c_int),
         ("circularTI", C.c_int),
         ("tiImage", C.c_void_p),
+        ("tiDimsAll", C.c_void_p),
     ]
 
 
@@ -263,7 +264,7 @@ class Sim:
 
     def __init__(self, ti, tidims, types, ncat, catvals, simdims, kdims, kernel_exp, offs, n, k, path, seeds,
                  begin, mode=0, quantum=0.0, cross=False, no_verbatim=False, n_ti=1, n_image=None,
-                 k_image=None, circular_sim=False, circular_ti=False, ti_image=None):
+                 k_image=None, circular_sim=False, circular_ti=False, ti_image=None, ti_dims_all=None):
         self._keep = []
 
         def keep(a):
@@ -303,6 +304,10 @@ class Sim:
         s.circularTI = int(bool(circular_ti))
         if ti_image is not None:
             s.tiImage = keep(f32(ti_image).reshape(-1))
+        if ti_dims_all is not None:  # several TIs of their own extents, nTI x D
+            tda = np.ones((len(ti_dims_all), 3), np.uint32)
+            tda[:, :D] = np.asarray(ti_dims_all, np.uint32)
+            s.tiDimsAll = keep(tda.reshape(-1))
         if n_image is not None:  # per cell n; n must be the largest value
             s.nImage = keep(f32(n_image).reshape(-1))
         if k_image is not None:

@@ -21,6 +21,7 @@
 // compensateMissingData, src/CPUThreadAcceleratorDevice.cpp:414-425), and
 // the marginal draw skips incomplete cells (simulation.hpp:586-654).
 #include <algorithm>
+#include <array>
 #include <climits>
 #include <cmath>
 #include <complex>
@@ -94,6 +95,7 @@ struct OrcSim {
   int circularSim;      // 1: -csim, neighbour cells wrap around the simulation grid
   int circularTI;       // 1: -cti, every TI position valid, neighbourhoods wrapped
   const float* tiImage; // several TIs: per simulation cell the only TI sampled (nullptr: all)
+  const unsigned* tiDimsAll;  // several TIs: nTI x 3 extents (nullptr: tiDims for all)
 };
 
 struct OrcSimCtx {
@@ -115,6 +117,8 @@ struct OrcSimCtx {
   const unsigned* index = nullptr;  // importDataIndex state for -nV (orc_sim_set_index)
   int nTI = 1;
   std::vector<std::vector<float>> Yx;  // sub-images of TIs 1.. (several TIs)
+  std::vector<std::array<unsigned, 3>> sT;  // extents of the TIs
+  std::vector<size_t> cumT;                 // first cell of each TI, then the total
 };
 
 // include/simulation.hpp:394: with circularSim the wrapped cell of
@@ -177,10 +181,21 @@ void* orc_sim_create(const OrcSim* p) {
     c->Y.resize((size_t)c->nSub * c->S);
     orc_ti_subimages(p->D, c->s, c->s, p->nVar, p->types, p->ti, p->ncat, p->catvals, 0, c->Y.data());
     c->nTI = p->nTI > 1 ? p->nTI : 1;
+    c->sT.resize(c->nTI);
+    c->cumT.assign(c->nTI + 1, 0);
+    for (int t = 0; t < c->nTI; ++t) {
+      size_t cells = 1;
+      for (int i = 0; i < 3; ++i) {
+        c->sT[t][i] = (t > 0 && p->tiDimsAll && i < p->D) ? p->tiDimsAll[t * 3 + i] : c->s[i];
+        cells *= c->sT[t][i];
+      }
+      c->cumT[t + 1] = c->cumT[t] + cells;
+    }
     for (int t = 1; t < c->nTI; ++t) {
-      c->Yx.emplace_back((size_t)c->nSub * c->S);
-      orc_ti_subimages(p->D, c->s, c->s, p->nVar, p->types, p->ti + (size_t)t * c->S * p->nVar, p->ncat, p->catvals,
-                       0, c->Yx.back().data());
+      const size_t St = c->cumT[t + 1] - c->cumT[t];
+      c->Yx.emplace_back((size_t)c->nSub * St);
+      orc_ti_subimages(p->D, c->sT[t].data(), c->sT[t].data(), p->nVar, p->types, p->ti + c->cumT[t] * p->nVar,
+                       p->ncat, p->catvals, 0, c->Yx.back().data());
     }
     if (c->nTI > 1) c->p.mode = 0;
   }
@@ -403,20 +418,22 @@ static unsigned node_multi_ti(OrcSimCtx* c, int nb, const std::vector<int>& off,
     return a.pos < b.pos;
   };
   std::vector<Cd> all;
-  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-  for (int i = 0; i < D; ++i) hi[i] = int(c->s[i]) - 1;
   for (unsigned t = 0; t < nT; ++t) {
     // a TI-index image: only that TI's device is updated
     // (include/quantileSamplingModule.hpp:460-464)
     if (tiReq >= 0 && (int)t != tiReq) continue;
-    std::vector<double> m64(c->S);
-    orc_map_direct_f64(D, c->s, c->nSub, t == 0 ? c->Y.data() : c->Yx[t - 1].data(), c->alpha.data(), nb, off.data(),
+    const unsigned* st = c->sT[t].data();
+    const size_t St = c->cumT[t + 1] - c->cumT[t];
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int i = 0; i < D; ++i) hi[i] = int(st[i]) - 1;
+    std::vector<double> m64(St);
+    orc_map_direct_f64(D, st, c->nSub, t == 0 ? c->Y.data() : c->Yx[t - 1].data(), c->alpha.data(), nb, off.data(),
                        u.data(), (double)delta0, c->p.circularTI, m64.data());
-    std::vector<float> m(c->S);
-    for (size_t q = 0; q < c->S; ++q) m[q] = (float)m64[q];
+    std::vector<float> m(St);
+    for (size_t q = 0; q < St; ++q) m[q] = (float)m64[q];
     std::vector<float> v(K);
     std::vector<unsigned> ps(K);
-    orc_select_device(D, c->s, m.data(), lo, hi, K, seed, p.quantum, v.data(), ps.data());
+    orc_select_device(D, st, m.data(), lo, hi, K, seed, p.quantum, v.data(), ps.data());
     for (int i = 0; i < K; ++i) {
       if (ps[i] == UINT_MAX || !(v[i] > -INFINITY)) continue;
       Cd cd;
@@ -483,8 +500,10 @@ static unsigned node_impl(OrcSimCtx* c, const float* di, unsigned pathIdx, int K
     // marginal draw: position = floor(seed * cells) in float, over all TIs in
     // order (include/simulation.hpp:586-606); several TIs: the encoded index
     if (c->nTI > 1) {
-      const unsigned q = (unsigned)int(std::floor(seed * (float)((unsigned)c->S * (unsigned)c->nTI)));
-      return (q % (unsigned)c->S) * (unsigned)c->nTI + q / (unsigned)c->S;
+      const unsigned q = (unsigned)int(std::floor(seed * (float)(unsigned)c->cumT[c->nTI]));
+      unsigned t = 0;
+      while ((int)t + 1 < c->nTI && q >= c->cumT[t + 1]) ++t;
+      return (q - (unsigned)c->cumT[t]) * (unsigned)c->nTI + t;
     }
     unsigned position = (unsigned)int(std::floor(seed * (float)(unsigned)c->S));
     if (p.cross && !c->centreOk[position]) {
@@ -642,7 +661,7 @@ void orc_sim_run(void* h, float* di, unsigned* idxOut) {
     if (tpos == UINT_MAX) continue;
     idxOut[cell] = tpos;  // importDataIndex = index*nTI + TI
     const float* tiv = p.cross ? c->tiX.data() : p.ti;
-    const size_t src = (size_t)(tpos % (unsigned)c->nTI) * c->S + tpos / (unsigned)c->nTI;
+    const size_t src = (c->nTI > 1 ? c->cumT[tpos % (unsigned)c->nTI] : 0) + tpos / (unsigned)c->nTI;
     for (int q = 0; q < p.nVar; ++q) di[(size_t)cell * p.nVar + q] = tiv[src * p.nVar + q];
   }
   c->index = keep;

@@ -452,3 +452,40 @@ def test_sim_ti_index_image():
     same = float(np.mean(i2[free] == i1[free] * 2 + 1))
     _log(f"  image of ones vs the one-TI run on B: {same:.4f} of the nodes identical")
     assert same >= 0.98
+
+
+def test_sim_several_tis_of_their_own_extents():
+    """Several TIs of different extents: lockstep with the oracle (each TI's
+    boxes on its own extent), and a never-matching second TI of another
+    extent leaves the one-TI realisation unchanged."""
+    ga = synth.grf((40, 44), 4.0, seed=95)
+    gb = synth.grf((30, 36), 4.0, seed=96)
+    a, dims = synth.to_g2s(np.exp(ga))
+    bb, dims_b = synth.to_g2s(np.exp(gb))
+    a, bb = a.reshape(-1), bb.reshape(-1)
+    sdims = (36, 30)
+    cells = int(np.prod(sdims))
+    rng = np.random.default_rng(9)
+    di = np.full(cells, np.nan, np.float32)
+    pick = rng.choice(cells, cells // 20, replace=False)
+    di[pick] = a[rng.integers(0, a.size, pick.size)]
+    _log(f"several TIs of their own extents: {dims} and {dims_b}")
+    sim, idx, _ = g2s_amd.qs([a, bb], dims, di, sdims, [0], 12, 1.5, 4, wave_size=64, ti_dims_all=[dims, dims_b])
+    kd = [d // 2 + 1 for d in dims]
+    offs = oracle.offsets(kd)
+    p, s_, b = oracle.path((~np.isnan(di)).astype(np.uint8), 4)
+    orc = oracle.Sim(np.concatenate([a, bb]), dims, [0], [], [], sdims, kd, np.ones(int(np.prod(kd)), np.float32),
+                     offs, 12, 1.5, p, s_, b, n_ti=2, ti_dims_all=[dims, dims_b])
+    free = p[b:]
+    src = np.where(idx[free] % 2 == 0, a[np.minimum(idx[free] // 2, a.size - 1)], bb[np.minimum(idx[free] // 2, bb.size - 1)])
+    assert np.array_equal(sim[free], src)
+    mism = sum(int(orc.node(sim, i)[0] != idx[p[b + i]]) for i in range(len(free)))
+    used = np.bincount(idx[free] % 2, minlength=2)
+    _log(f"  lockstep mismatches {mism} of {len(free)}; nodes per TI {used.tolist()}")
+    assert mism <= len(free) // 100 and used.min() > 0
+    far = np.full(30 * 36, 1e4, np.float32)
+    s1, i1, _ = g2s_amd.qs(a, dims, di, sdims, [0], 12, 1.5, 4, wave_size=64)
+    s2, i2, _ = g2s_amd.qs([a, far], dims, di, sdims, [0], 12, 1.5, 4, wave_size=64, ti_dims_all=[dims, (30, 36)])
+    same = float(np.mean(i2[free] == i1[free] * 2))
+    _log(f"  far TI of another extent: {same:.4f} of the nodes identical to the one-TI run")
+    assert same >= 0.98

@@ -294,3 +294,47 @@ def test_ti_index_image():
     mixed = rng.integers(0, 2, cells).astype(np.float32)
     _, im = make([a, bb], mixed).run(di)
     assert np.array_equal(im[free] % 2, mixed[free].astype(np.uint32))
+
+
+def test_several_tis_of_their_own_extents():
+    """Several TIs of different extents: a never-matching second TI of
+    another extent leaves the realisation unchanged (index position * 2);
+    cells forced onto a smaller TI by a TI-index image take its values; the
+    marginal draw walks the TIs' cell counts in order."""
+    g = synth.grf((30, 34), 3.0, seed=3)
+    a, dims = synth.to_g2s(np.exp(g))
+    gb = synth.grf((22, 20), 3.0, seed=6)
+    bb, dims_b = synth.to_g2s(np.exp(gb))
+    a, bb = a.reshape(-1), bb.reshape(-1)
+    far = np.full(25 * 27, 1e4, np.float32)
+    sdims = (20, 18)
+    cells = int(np.prod(sdims))
+    rng = np.random.default_rng(5)
+    di = np.full(cells, np.nan, np.float32)
+    pick = rng.choice(cells, cells // 20, replace=False)
+    di[pick] = a[rng.integers(0, a.size, pick.size)]
+    kd = [d // 2 + 1 for d in dims]
+    offs = oracle.offsets(kd)
+    p, s, b = oracle.path((~np.isnan(di)).astype(np.uint8), 3)
+    free = p[b:]
+
+    def make(tis, all_dims, **kw):
+        return oracle.Sim(np.concatenate(tis), dims, [0], [], [], sdims, kd, np.ones(int(np.prod(kd)), np.float32),
+                          offs, 9, 1.5, p, s, b, n_ti=len(tis), ti_dims_all=all_dims, **kw)
+
+    _, i1 = oracle.Sim(a, dims, [0], [], [], sdims, kd, np.ones(int(np.prod(kd)), np.float32), offs, 9, 1.5, p, s,
+                       b).run(di)
+    _, i2 = make([a, far], [dims, (25, 27)]).run(di)
+    assert np.array_equal(i2[free], i1[free] * 2)
+    img = rng.integers(0, 2, cells).astype(np.float32)
+    out3, i3 = make([a, bb], [dims, dims_b], ti_image=img).run(di)
+    onb = free[img[free] == 1]
+    assert np.all(i3[onb] % 2 == 1) and np.all(i3[onb] // 2 < bb.size)
+    assert np.array_equal(out3[onb], bb[i3[onb] // 2])
+    di0 = np.full(cells, np.nan, np.float32)
+    p0, s0, b0 = oracle.path(np.zeros(cells, np.uint8), 7)
+    m = oracle.Sim(np.concatenate([a, bb]), dims, [0], [], [], sdims, kd, np.ones(int(np.prod(kd)), np.float32), offs,
+                   9, 1.5, p0, s0, b0, n_ti=2, ti_dims_all=[dims, dims_b])
+    chosen, _, _, nb = m.node(di0, 0)
+    q = int(np.floor(np.float32(s0[b0]) * np.float32(a.size + bb.size)))
+    assert nb == 1 and chosen == ((q * 2) if q < a.size else ((q - a.size) * 2 + 1))
