@@ -142,6 +142,26 @@ class TrainingImage:
         self._fd = fd
         self.h = h
 
+    @classmethod
+    def from_reflected(cls, images, dims, fft_dims, gpu=0):
+        """From the sub-images as G2S's SharedMemoryManager hands them to
+        allocAndInitSharedMemory: one point-reflected FFT-grid image each
+        (qsmi_ti_create_from_reflected)."""
+        dims = _u32(dims)
+        fd = _u32(fft_dims)
+        imgs = [_f32(im).reshape(-1) for im in images]
+        ptrs = (C.c_void_p * len(imgs))(*[im.ctypes.data for im in imgs])
+        self = cls.__new__(cls)
+        self.dims = tuple(int(d) for d in dims)
+        self.nsub = len(imgs)
+        self.ndim = len(dims)
+        h = C.c_void_p()
+        check(lib().qsmi_ti_create_from_reflected(gpu, len(dims), dims, fd, len(imgs), C.cast(ptrs, C.c_void_p),
+                                                  C.byref(h)))
+        self._fd = fd
+        self.h = h
+        return self
+
     def close(self):
         if getattr(self, "h", None):
             lib().qsmi_ti_destroy(self.h)
@@ -149,6 +169,18 @@ class TrainingImage:
 
     def __del__(self):
         self.close()
+
+
+def subimages_from_reflected(images, dims, fft_dims):
+    """Unreflected TI-grid sub-images (nSub, prod(dims)) from G2S's reflected
+    FFT-grid layout (qsmi_subimages_from_reflected, host only)."""
+    dims = _u32(dims)
+    fd = _u32(fft_dims)
+    imgs = [_f32(im).reshape(-1) for im in images]
+    ptrs = (C.c_void_p * len(imgs))(*[im.ctypes.data for im in imgs])
+    out = np.zeros(len(imgs) * int(np.prod(dims)), np.float32)
+    check(lib().qsmi_subimages_from_reflected(len(dims), dims, fd, len(imgs), C.cast(ptrs, C.c_void_p), out))
+    return out.reshape(len(imgs), -1)
 
 
 class AcceleratorDevice:

@@ -26,6 +26,8 @@ SIGNATURES = {
     "qsmi_get_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "qsmi_fft_size": (C.c_uint, [C.c_uint]),
     "qsmi_ti_create": (C.c_int, [C.c_int, C.c_int, u32p, vp, C.c_int, f32p, C.POINTER(vp)]),
+    "qsmi_subimages_from_reflected": (C.c_int, [C.c_int, u32p, u32p, C.c_int, vp, f32p]),
+    "qsmi_ti_create_from_reflected": (C.c_int, [C.c_int, C.c_int, u32p, u32p, C.c_int, vp, C.POINTER(vp)]),
     "qsmi_ti_destroy": (C.c_int, [vp]),
     "qsmi_device_create": (C.c_int, [vp, f32p, C.c_int, C.POINTER(vp)]),
     "qsmi_device_destroy": (C.c_int, [vp]),

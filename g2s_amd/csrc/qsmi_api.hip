@@ -23,7 +23,7 @@ void launch_colfft(int dir, float2* base, const FftPlan& plan, const float2* tw,
                    int planesPerNode, int validDim, hipStream_t st);
 void launch_pencil(const ScanGeom& g, const FftPlan& planP, int lognv, const float2* Yhat, const float2* tw0,
                    const float2* tw1, const float2* twP, int nNodes, int npb, const NodeInfo* info, const int* nbOff,
-                   const float* nbVal, float2* C, hipStream_t st);
+                   const float* nbVal, uint32_t* ord, float2* C, hipStream_t st);
 void launch_direct1d(const ScanGeom& g, const float2* Yhat, const float2* tw0, int nNodes, const NodeInfo* info,
                      const int* nbOff, const float* nbVal, float2* C, hipStream_t st);
 void launch_rows_select(const ScanGeom& g, const FftPlan& plan0, int lognv, const float2* tw0, const NodeInfo* info,
@@ -36,7 +36,7 @@ void launch_fill_disk(float* map, const int* s, uint32_t o, float r2, float v, h
 long long ct1k_stride(int h0);
 void launch_transpose_spectra(const float2* Y, float2* YT, int h0, int f1, int nA, hipStream_t st);
 void launch_pencil1k(const ScanGeom& g, const float2* YT, const float2* tw1k, int nNodes, int npb,
-                     const NodeInfo* info, const int* nbOff, const float* nbVal, TapInfo* taps, uint8_t* tapOf,
+                     const NodeInfo* info, const int* nbOff, const float* nbVal, TapInfo* taps, uint16_t* tapPos,
                      float2* CT, const int* count, hipStream_t st);
 int rows1k_tiles();
 hipError_t set_rows1k_lds();
@@ -207,6 +207,7 @@ void prep_node(const qsmi_device* dev, int nb, const int* off, const float* u, f
   ni->exR2 = 0.f;
   ni->kLocal = 0.f;
   ni->tiReq = -1;
+  std::vector<double> sumAbs(V, 0.0);
   for (int j = 0; j < nb; ++j) {
     for (int i = 0; i < D; ++i) offOut[j * D + i] = off[j * D + i];
     for (int a = 0; a < V; ++a) {
@@ -214,7 +215,14 @@ void prep_node(const qsmi_device* dev, int nb, const int* off, const float* u, f
       float acc = 0.f;
       for (int B = 0; B < nSub; ++B) acc += dev->alpha[A + B * nSub] * u[j * nSub + B];
       valOut[j * V + a] = acc;
+      sumAbs[a] += std::fabs((double)acc);
     }
+  }
+  ni->eps = 0.f;
+  if (dev->quantum == 0.f && !dev->cross && nb > 0) {
+    double e = 0.0;
+    for (int a = 0; a < V; ++a) e += (double)t->rms[dev->active[a]] * sumAbs[a];
+    ni->eps = (float)((double)kMapErrC * (double)ni->scale * (double)t->N * e);
   }
 }
 
@@ -234,19 +242,20 @@ int ensure_tap_tables(qsmi_device* dev, int nNodes) {
   if (dev->capTap >= (size_t)nNodes) return QSMI_OK;
   // grown rarely: hipFree waits for queued work that may still read them
   if (dev->dTap) (void)hipFree(dev->dTap);
-  if (dev->dTapOf) (void)hipFree(dev->dTapOf);
+  if (dev->dTapPos) (void)hipFree(dev->dTapPos);
   dev->dTap = nullptr;
-  dev->dTapOf = nullptr;
+  dev->dTapPos = nullptr;
   dev->capTap = 0;
   const size_t cap = std::max<size_t>((size_t)nNodes, 256);
-  if (hipMalloc(&dev->dTap, cap * sizeof(TapInfo)) != hipSuccess || hipMalloc(&dev->dTapOf, cap * 1000) != hipSuccess)
+  if (hipMalloc(&dev->dTap, cap * sizeof(TapInfo)) != hipSuccess ||
+      hipMalloc(&dev->dTapPos, cap * kTapNbMax * sizeof(uint16_t)) != hipSuccess)
     return fail(QSMI_ERR_NOMEM, "tap tables: out of device memory");
   dev->capTap = cap;
   return QSMI_OK;
 }
 
 int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeInfo* info, const int* off,
-                    const float* val, float2* C, const int* count) {
+                    const float* val, uint32_t* ord, float2* C, const int* count) {
   qsmi_ti* t = dev->ti;
   hipStream_t st = dev->stream;
   if (dev->timing) hipEventRecord(dev->ev[0], st);
@@ -255,15 +264,15 @@ int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeI
     if (rc) return rc;
     // nodes per pencil block: 3 measured best over 1..16 (256-node waves)
     const int npb = std::max(1, std::min(3, nNodes / 8));
-    launch_pencil1k(g, t->dYT, t->dTw[0], nNodes, npb, info, off, val, dev->dTap, dev->dTapOf, C, count, st);
+    launch_pencil1k(g, t->dYT, t->dTw[0], nNodes, npb, info, off, val, dev->dTap, dev->dTapPos, C, count, st);
   } else if (count) {
     return fail(QSMI_ERR_UNSUPPORTED, "a device node count is supported on the 1000x1000 path only");
   } else if (t->D == 1) {
     launch_direct1d(g, t->dYhat, t->dTw[0], nNodes, info, off, val, C, st);
   } else {
     const int P = t->D - 1;
-    launch_pencil(g, t->plan[P], t->lognv[P], t->dYhat, t->dTw[0], t->dTw[1], t->dTw[P], nNodes, 1, info, off, val, C,
-                  st);
+    launch_pencil(g, t->plan[P], t->lognv[P], t->dYhat, t->dTw[0], t->dTw[1], t->dTw[P], nNodes, 1, info, off, val,
+                  ord, C, st);
   }
   if (dev->timing) hipEventRecord(dev->ev[1], st);
   if (t->D == 3) {
@@ -421,7 +430,15 @@ int qsmi_ti_create(int gpu, int ndim, const unsigned* srcSize, const unsigned* f
         float* dst = pad.data() + (size_t)A * t->N + ((size_t)c2 * t->f[1] + c1) * t->f[0];
         memcpy(dst, src, sizeof(float) * t->s[0]);
       }
-  // spatial copy on the TI grid (exact rescoring of proposed positions)
+  // root mean square of every sub-image over the FFT grid: the scale of the
+  // FP32 map error bound
+  for (int A = 0; A < nSub; ++A) {
+    double acc = 0.0;
+    const float* src = subImages + (size_t)A * t->S;
+    for (long long q = 0; q < t->S; ++q) acc += (double)src[q] * (double)src[q];
+    t->rms[A] = (float)std::sqrt(acc / (double)t->N);
+  }
+  // spatial copy on the TI grid (exact rescoring of band and proposed positions)
   TCHK(dalloc(&t->dSub, (size_t)nSub * t->S));
   TCHK(hipMemcpy(t->dSub, subImages, (size_t)nSub * t->S * sizeof(float), hipMemcpyHostToDevice));
   float* dReal = nullptr;
@@ -509,16 +526,18 @@ int qsmi_device_destroy(qsmi_device* d) {
   hipFree(d->dMap);
   hipFree(d->dC);
   hipFree(d->dTap);
-  hipFree(d->dTapOf);
+  hipFree(d->dTapPos);
   hipFree(d->dCand);
   hipFree(d->dInfo);
   hipFree(d->dOff);
   hipFree(d->dVal);
+  hipFree(d->dOrd);
   hipFree(d->dOutVal);
   hipFree(d->dOutPos);
   hipFree(d->dChosen);
   hipFree(d->dCentreOk);
   hipFree(d->dComplete);
+  band_free(&d->band);
   if (d->wb) {
     free_wave_bufs(d->wb);
     delete d->wb;
@@ -580,6 +599,7 @@ int qsmi_candidate_for_pattern(qsmi_device* dev, int nb, const int* offsets, con
   if ((rc = ensure(&dev->dInfo, &dev->capInfo, 1))) return rc;
   if ((rc = ensure(&dev->dOff, &dev->capOff, off.size()))) return rc;
   if ((rc = ensure(&dev->dVal, &dev->capVal, val.size()))) return rc;
+  if ((rc = ensure(&dev->dOrd, &dev->capOrd, (size_t)nb))) return rc;
   if ((rc = ensure(&dev->dC, &dev->capC, (size_t)c_stride(t)))) return rc;
   hipStream_t st = dev->stream;
   HIPCHK(hipMemcpyAsync(dev->dInfo, &ni, sizeof(ni), hipMemcpyHostToDevice, st));
@@ -587,12 +607,14 @@ int qsmi_candidate_for_pattern(qsmi_device* dev, int nb, const int* offsets, con
   HIPCHK(hipMemcpyAsync(dev->dVal, val.data(), val.size() * sizeof(float), hipMemcpyHostToDevice, st));
   launch_fill(dev->dMap, t->S, -INFINITY, st);
   ScanGeom g = geom_of(dev);
-  if ((rc = launch_spectrum(dev, g, 1, dev->dInfo, dev->dOff, dev->dVal, dev->dC))) return rc;
+  if ((rc = launch_spectrum(dev, g, 1, dev->dInfo, dev->dOff, dev->dVal, dev->dOrd, dev->dC))) return rc;
   launch_rows(dev, g, dev->dInfo, dev->dC, 1, row_blocks(t), 1, nullptr, dev->dMap);
   HIPCHK(hipGetLastError());
   // the host arrays above are pageable and go out of scope: finish the copies
   HIPCHK(hipStreamSynchronize(st));
   dev->mapValid = true;
+  dev->mapEdited = false;
+  dev->lastNode = ni;
   if (computed) *computed = 1;
   return QSMI_OK;
 }
@@ -610,8 +632,9 @@ int qsmi_search_k_biggest(qsmi_device* dev, float* errors, unsigned* encodedPosi
   if ((rc = ensure(&dev->dOutVal, &dev->capOut, (size_t)K))) return rc;
   if ((rc = ensure(&dev->dOutPos, &dev->capOutPos, (size_t)K))) return rc;
   hipStream_t st = dev->stream;
-  NodeInfo ni;
-  memset(&ni, 0, sizeof(ni));
+  // the last scanned node with this call's seed (its neighbours stay in
+  // dOff / dVal for the exact selection)
+  NodeInfo ni = dev->lastNode;
   ni.seed = seed;
   ni.seedbits = seed_from_unit_float(seed);
   ni.quantum = dev->quantum;
@@ -619,8 +642,12 @@ int qsmi_search_k_biggest(qsmi_device* dev, float* errors, unsigned* encodedPosi
   HIPCHK(hipMemcpyAsync(dev->dInfo, &ni, sizeof(ni), hipMemcpyHostToDevice, st));
   launch_select_map(dev->dMap, t->S, K, dev->quantum, ni.seedbits, dev->dCand, chunks, st);
   ScanGeom g = geom_of(dev);
-  launch_merge_draw(g, dev->dInfo, dev->dCand, 1, chunks, 0, K, (float)K, dev->dOutVal, dev->dOutPos, nullptr,
+  launch_merge_draw(g, dev->dInfo, dev->dCand, 1, chunks, K, 0, K, (float)K, dev->dOutVal, dev->dOutPos, nullptr,
                     nullptr, nullptr, 0, nullptr, st);
+  // exact selection of the FP32 map's band unless a value was set into the
+  // map from outside (exclusions by -inf keep it)
+  if (ni.eps > 0.f && !dev->mapEdited && exact_selection_on(dev))
+    if ((rc = band_select_map(dev, dev->dInfo, ni.nb, K, ni.eps, dev->dOutVal, dev->dOutPos, &dev->band))) return rc;
   HIPCHK(hipGetLastError());
   std::vector<uint32_t> pos(K);
   HIPCHK(hipMemcpyAsync(errors, dev->dOutVal, K * sizeof(float), hipMemcpyDeviceToHost, st));
@@ -694,6 +721,7 @@ int qsmi_set_value_in_error_array(qsmi_device* dev, unsigned index, float value)
   if (pos >= (unsigned)dev->ti->S) return fail(QSMI_ERR_ARG, "index maps outside the TI");
   HIPCHK(hipMemcpyAsync(dev->dMap + pos, &value, sizeof(float), hipMemcpyHostToDevice, dev->stream));
   HIPCHK(hipStreamSynchronize(dev->stream));
+  if (!(value == -INFINITY)) dev->mapEdited = true;
   return QSMI_OK;
 }
 
@@ -708,14 +736,17 @@ void free_wave_bufs(WaveBufs* b) {
   (void)hipFree(b->dInfo);
   (void)hipFree(b->dOff);
   (void)hipFree(b->dVal);
+  (void)hipFree(b->dOrd);
   (void)hipFree(b->dC);
   (void)hipFree(b->dCand);
+  (void)hipFree(b->dCandEx);
   (void)hipFree(b->dOutVal);
   (void)hipFree(b->dOutPos);
   (void)hipFree(b->dChosen);
   (void)hipHostFree(b->hInfo);
   (void)hipHostFree(b->hOff);
   (void)hipHostFree(b->hVal);
+  band_free(&b->band);
   *b = WaveBufs();
 }
 
@@ -730,7 +761,9 @@ int grow_wave_bufs(qsmi_device* dev, WaveBufs* b, int nNodes, int total, int K) 
     (void)hipFree(b->dOutPos);
     (void)hipFree(b->dChosen);
     (void)hipFree(b->dCand);
+    (void)hipFree(b->dCandEx);
     (void)hipHostFree(b->hInfo);
+    b->dCandEx = nullptr;
     b->dInfo = nullptr;
     b->dOutVal = nullptr;
     b->dOutPos = nullptr;
@@ -744,6 +777,7 @@ int grow_wave_bufs(qsmi_device* dev, WaveBufs* b, int nNodes, int total, int K) 
     HIPCHK(dalloc(&b->dOutPos, n * kMaxK));
     HIPCHK(dalloc(&b->dChosen, n));
     HIPCHK(dalloc(&b->dCand, n * rb * kMaxK));
+    HIPCHK(dalloc(&b->dCandEx, n * kMaxK));
     HIPCHK(hipHostMalloc((void**)&b->hInfo, n * sizeof(NodeInfo), hipHostMallocDefault));
     b->capNodes = n;
     b->capCand = n * rb * kMaxK;
@@ -761,15 +795,18 @@ int grow_wave_bufs(qsmi_device* dev, WaveBufs* b, int nNodes, int total, int K) 
     size_t n = std::max<size_t>(std::max(total, 1), b->capNb * 3 / 2);
     (void)hipFree(b->dOff);
     (void)hipFree(b->dVal);
+    (void)hipFree(b->dOrd);
     (void)hipHostFree(b->hOff);
     (void)hipHostFree(b->hVal);
     b->dOff = nullptr;
     b->dVal = nullptr;
+    b->dOrd = nullptr;
     b->hOff = nullptr;
     b->hVal = nullptr;
     b->capNb = 0;
     HIPCHK(dalloc(&b->dOff, n * 3));
     HIPCHK(dalloc(&b->dVal, n * V));
+    HIPCHK(dalloc(&b->dOrd, n));
     HIPCHK(hipHostMalloc((void**)&b->hOff, n * 3 * sizeof(int), hipHostMallocDefault));
     HIPCHK(hipHostMalloc((void**)&b->hVal, n * V * sizeof(float), hipHostMallocDefault));
     b->capNb = n;
@@ -789,8 +826,11 @@ int stage_wave(qsmi_device* dev, WaveBufs* b, int nNodes, const int* nbStart, co
   const int D = t->D, nSub = t->nSub, V = (int)dev->active.size();
   const int total = nbStart[nNodes];
   if (nbStart[0] != 0 || total < 0) return fail(QSMI_ERR_ARG, "nbStart must start at 0");
-  for (int i = 0; i < nNodes; ++i)
+  int nbMax = 0;
+  for (int i = 0; i < nNodes; ++i) {
     if (nbStart[i + 1] < nbStart[i]) return fail(QSMI_ERR_ARG, "nbStart must be non-decreasing");
+    nbMax = std::max(nbMax, nbStart[i + 1] - nbStart[i]);
+  }
   if (total > 0 && (!offsets || !values)) return fail(QSMI_ERR_ARG, "null neighbour arrays");
   int rc = grow_wave_bufs(dev, b, nNodes, total, K);
   if (rc) return rc;
@@ -817,6 +857,7 @@ int stage_wave(qsmi_device* dev, WaveBufs* b, int nNodes, const int* nbStart, co
   b->nNodes = nNodes;
   b->K = K;
   b->k = k;
+  b->nbMax = nbMax;
   b->rowBlocks = row_blocks(t);
   return QSMI_OK;
 }
@@ -825,12 +866,13 @@ int run_wave(qsmi_device* dev, WaveBufs* b) {
   qsmi_ti* t = dev->ti;
   hipStream_t st = dev->stream;
   ScanGeom g = geom_of(dev);
-  int rc = launch_spectrum(dev, g, b->nNodes, b->dInfo, b->dOff, b->dVal, b->dC);
+  int rc = launch_spectrum(dev, g, b->nNodes, b->dInfo, b->dOff, b->dVal, b->dOrd, b->dC);
   if (rc) return rc;
   launch_rows(dev, g, b->dInfo, b->dC, b->nNodes, b->rowBlocks, b->K, b->dCand, nullptr);
   if (dev->timing) (void)hipEventRecord(dev->ev[3], st);
-  launch_merge_draw(g, b->dInfo, b->dCand, b->nNodes, b->rowBlocks, rows_per_block(t), b->K, b->k, b->dOutVal,
-                    b->dOutPos, b->dChosen, nullptr, nullptr, 0, nullptr, st);
+  rc = band_select(dev, g, b->dInfo, b->dCand, b->nNodes, b->K, b->k, b->dC, b->dOff, b->dVal, b->nbMax, b->dOutVal,
+                   b->dOutPos, nullptr, b->dChosen, nullptr, nullptr, 0, nullptr, &b->band);
+  if (rc) return rc;
   if (dev->timing) (void)hipEventRecord(dev->ev[4], st);
   HIPCHK(hipGetLastError());
   return QSMI_OK;
@@ -929,6 +971,7 @@ int qsmi_set_value_in_error_array_with_radius(qsmi_device* dev, unsigned index, 
   launch_fill_disk(dev->dMap, dev->ti->s, pos, radius * radius, value, dev->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(dev->stream));
+  if (!(value == -INFINITY)) dev->mapEdited = true;
   return QSMI_OK;
 }
 

@@ -1,0 +1,542 @@
+// Exact selection of the K best positions of a node (gfx950).
+//
+// The FP32 maps of the scan carry an absolute rounding error that is set by
+// the norms of the TI sub-images and of the neighbour values, not by the
+// local magnitude of the mismatch: on a heavy-tailed TI (the 1000^2
+// log-normal workload) the error is ~1e-2 everywhere, larger than the gap
+// between the best positions of a low-valued neighbourhood. The selection
+// therefore never trusts FP32 values near the K-th best:
+//
+//   1. merge (k_merge_band): per node, the K best FP32 candidates of the
+//      rows stage give T, the K-th best value; the band is every position
+//      whose FP32 value is >= theta = T - 2 eps (eps: the node's error
+//      bound, NodeInfo::eps). Every position whose exact value reaches the
+//      exact K-th best lies in the band (its FP32 value is at least its
+//      exact value - eps, and the exact K-th best is at least T - eps).
+//      Each rows group (wavefront or block) emitted its M best; a group
+//      whose M-th candidate is still in the band may hold more band
+//      positions, so its rows block is queued for a second pass; the
+//      candidates of the other groups in the band go to the node's list.
+//   2. second pass (k_rows1k_collect / k_rows_collect): the queued rows
+//      blocks again, appending every position >= theta.
+//   3. rescoring (k_band_rescore): the exact value of every listed position
+//      in FP64, E = delta0 + sum_j sum_a v_ja Y_a(p + d_j) summed in that
+//      order with separately rounded products and sums (no fused
+//      multiply-add), and the K best by the device rule on those values,
+//      then the rank draw and the simulation write.
+//   4. full fallback (k_rescore_full + k_rescore_full_merge): a node whose
+//      band overflows the list capacity is rescored over its whole valid
+//      box in FP64.
+// Nodes with eps == 0 (categorical-only TIs with a tie quantum, TIs with
+// NaN) keep the FP32 selection and finish in step 1.
+//
+// Replaces the selection of the reference's scan (fKst over the FP32 map,
+// src/CPUThreadAcceleratorDevice.cpp:427-468, include/fKb.hpp:907-954) with
+// an exact one; the map itself is src/CPUThreadAcceleratorDevice.cpp:251-357.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "qsmi_internal.hpp"
+#include "qsmi_select.hpp"
+
+namespace qsmi {
+
+int cands_per_group(int K);
+int rows1k_blocks();
+int rows1k_waves_per_block();
+int rows1k_tiles();
+void launch_rows1k_collect(const ScanGeom& g, const float2* tw1k, const NodeInfo* info, const float2* CT,
+                           const int2* queue, const int* qcnt, const float* theta, uint32_t* bandPos, int* bandCnt,
+                           int maxItems, hipStream_t st);
+void launch_rows_collect(const ScanGeom& g, const FftPlan& plan0, int lognv, const float2* tw0, const NodeInfo* info,
+                         const float2* C, const int2* queue, const int* qcnt, const float* theta, uint32_t* bandPos,
+                         int* bandCnt, int maxItems, hipStream_t st);
+
+namespace {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int kFullChunks = 64;  // blocks per node of the full fallback
+constexpr int kFullSlots = 8;    // nodes of the full fallback in flight
+
+// Device view of the band buffers.
+struct BandDev {
+  uint32_t* pos;   // [node][kBandCap]
+  int* cnt;        // [node]: listed positions (-1: node finished by the merge)
+  float* theta;    // [node]
+  int2* queue;     // (node, rows block) items of the second pass
+  int* qcnt;       // [0] queue items, [1] nodes of the full fallback
+  int* fullq;      // [item] node
+  Cand64* part;    // [item][kFullChunks][K]
+};
+
+BandDev dev_view(const BandBufs& b) {
+  BandDev d;
+  d.pos = b.pos;
+  d.cnt = b.cnt;
+  d.theta = b.theta;
+  d.queue = b.queue;
+  d.qcnt = b.qcnt;
+  d.fullq = b.fullq;
+  d.part = b.part;
+  return d;
+}
+
+// Exact value of position p (FP64, the oracle's summation order).
+__device__ __forceinline__ double exact_value(const ScanGeom& g, const NodeInfo& ni, const int* __restrict__ off,
+                                              const double* __restrict__ v, const float* __restrict__ Y, long long S,
+                                              uint32_t p, double nscale) {
+  const int D = g.D, V = g.V;
+  int c[3] = {0, 0, 0};
+  {
+    uint32_t rem = p;
+    for (int d = 0; d < D; ++d) {
+      c[d] = (int)(rem % (uint32_t)g.s[d]);
+      rem /= (uint32_t)g.s[d];
+    }
+  }
+  double acc = (double)ni.delta0;
+  for (int j = 0; j < ni.nb; ++j) {
+    long long t = 0, mul = 1;
+    for (int d = 0; d < D; ++d) {
+      int x = c[d] + off[j * D + d];
+      if (g.circular) x = ((x % g.s[d]) + g.s[d]) % g.s[d];
+      t += (long long)x * mul;
+      mul *= g.s[d];
+    }
+    for (int a = 0; a < V; ++a)
+      acc = __dadd_rn(acc, __dmul_rn(v[j * V + a], (double)Y[(long long)g.A[a] * S + t]));
+  }
+  return nscale == 1.0 ? acc : acc * nscale;
+}
+
+// The node's neighbour offsets and alpha-combined values in LDS:
+// [nb * D] int, then [nb * V] double (8-byte aligned).
+__device__ __forceinline__ void stage_neighbours(const ScanGeom& g, const NodeInfo& ni, const int* __restrict__ nbOff,
+                                                 const float* __restrict__ nbVal, char* sm, int** offOut,
+                                                 double** valOut) {
+  const int D = g.D, V = g.V, nb = ni.nb;
+  int* so = reinterpret_cast<int*>(sm);
+  double* sv = reinterpret_cast<double*>(sm + (((size_t)nb * D * sizeof(int) + 7) & ~(size_t)7));
+  for (int t = threadIdx.x; t < nb * D; t += blockDim.x) so[t] = nbOff[(long long)ni.nbStart * D + t];
+  for (int t = threadIdx.x; t < nb * V; t += blockDim.x) sv[t] = (double)nbVal[(long long)ni.nbStart * V + t];
+  *offOut = so;
+  *valOut = sv;
+}
+
+__global__ void k_band_reset(int* __restrict__ qcnt, int* __restrict__ cnt0) {
+  if (threadIdx.x < 2) qcnt[threadIdx.x] = 0;
+  if (threadIdx.x == 0 && cnt0) *cnt0 = 0;
+}
+
+// Step 1. One 256-thread block per node. cand: [node][candGroups][M];
+// groups of one rows block are consecutive (wpb per block); rowsPerBlock > 0
+// (generic rows kernel): only the blocks covering the node's valid rows
+// wrote candidates (one group each).
+template <int KT>
+__global__ void __launch_bounds__(256) k_merge_band(const NodeInfo* __restrict__ info, const Cand* __restrict__ cand,
+                                                    int candGroups, int M, int wpb, int rowsPerBlock, int K, float kf,
+                                                    float* __restrict__ outVal, uint32_t* __restrict__ outPos,
+                                                    Cand* __restrict__ outCand, uint32_t* __restrict__ chosen,
+                                                    float* __restrict__ simVals, const float* __restrict__ tiVals,
+                                                    int nVar, uint32_t* __restrict__ indexOut, BandDev bb,
+                                                    const int* __restrict__ count) {
+  __shared__ Cand red[16];
+  __shared__ Cand best[kMaxK];
+  __shared__ uint32_t bpos[kMaxK];
+  __shared__ uint32_t flags[kMaxBandBlocks / 32];
+  __shared__ int lcnt;
+  const int node = blockIdx.x, tid = threadIdx.x;
+  if (count && node >= *count) return;
+  const NodeInfo ni = info[node];
+  if (simVals && ni.nb == 0) {  // drawn from the marginal by the gather stage
+    if (tid == 0) bb.cnt[node] = -1;
+    return;
+  }
+  int nGroups = candGroups, nBlocks = candGroups / wpb;
+  if (rowsPerBlock > 0) {
+    const int n1v = ni.hi[1] - ni.lo[1] + 1;
+    const int n2v = ni.hi[2] - ni.lo[2] + 1;
+    const int n0v = ni.hi[0] - ni.lo[0] + 1;
+    const long long nrv = (n1v > 0 && n2v > 0 && n0v > 0) ? (long long)n1v * n2v : 0;
+    nGroups = (int)((nrv + rowsPerBlock - 1) / rowsPerBlock);
+    nBlocks = nGroups;
+  }
+  TopK<KT> tk;
+  tk.init();
+  const Cand* cn = cand + (long long)node * candGroups * M;
+  for (int i = tid; i < nGroups * M; i += blockDim.x) tk.push(cn[i]);
+  block_topk<KT>(tk, K, best, red);
+  if (!(ni.eps > 0.f)) {  // FP32 selection
+    if (tid < K) {
+      outVal[(long long)node * K + tid] = best[tid].val;
+      outPos[(long long)node * K + tid] = best[tid].pos;
+      if (outCand) outCand[(long long)node * K + tid] = best[tid];
+      bpos[tid] = best[tid].pos;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      bb.cnt[node] = -1;
+      if (chosen || simVals) draw_write(ni, bpos, K, kf, chosen, node, simVals, tiVals, nVar, indexOut);
+    }
+    return;
+  }
+  // the band: FP32 values >= theta (every position when fewer than K are valid)
+  const float th = (best[K - 1].pos == kNone) ? -INFINITY : best[K - 1].val - 2.f * ni.eps * 1.0009765625f;
+  for (int i = tid; i < kMaxBandBlocks / 32; i += blockDim.x) flags[i] = 0u;
+  if (tid == 0) lcnt = 0;
+  __syncthreads();
+  for (int gi = tid; gi < nGroups; gi += blockDim.x) {
+    const Cand last = cn[(long long)gi * M + M - 1];
+    if (last.pos != kNone && last.val >= th) {
+      const int rb = gi / wpb;
+      atomicOr(&flags[rb >> 5], 1u << (rb & 31));
+    }
+  }
+  __syncthreads();
+  uint32_t* out = bb.pos + (long long)node * kBandCap;
+  for (int i = tid; i < nGroups * M; i += blockDim.x) {
+    const int rb = (i / M) / wpb;
+    if ((flags[rb >> 5] >> (rb & 31)) & 1u) continue;
+    const Cand c = cn[i];
+    if (c.pos != kNone && c.val >= th) {
+      const int k = atomicAdd(&lcnt, 1);
+      if (k < kBandCap) out[k] = c.pos;
+    }
+  }
+  for (int rb = tid; rb < nBlocks; rb += blockDim.x) {
+    if ((flags[rb >> 5] >> (rb & 31)) & 1u) {
+      const int q = atomicAdd(&bb.qcnt[0], 1);  // capacity: every (node, block) of the wave
+      bb.queue[q] = make_int2(node, rb);
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    bb.cnt[node] = lcnt;
+    bb.theta[node] = th;
+  }
+}
+
+// Step 3. One 256-thread block per node; dynamic LDS: the node's neighbours.
+template <int KT>
+__global__ void __launch_bounds__(256) k_band_rescore(ScanGeom g, const NodeInfo* __restrict__ info,
+                                                      const int* __restrict__ nbOff, const float* __restrict__ nbVal,
+                                                      const float* __restrict__ Y, long long S, double nscale, int K,
+                                                      float kf, float* __restrict__ outVal,
+                                                      uint32_t* __restrict__ outPos, Cand* __restrict__ outCand,
+                                                      uint32_t* __restrict__ chosen, float* __restrict__ simVals,
+                                                      const float* __restrict__ tiVals, int nVar,
+                                                      uint32_t* __restrict__ indexOut, BandDev bb,
+                                                      const int* __restrict__ count) {
+  extern __shared__ char sm[];
+  __shared__ Cand64 red[4];
+  __shared__ Cand64 best[kMaxK];
+  __shared__ uint32_t bpos[kMaxK];
+  const int node = blockIdx.x, tid = threadIdx.x;
+  if (count && node >= *count) return;
+  const int cnt = bb.cnt[node];
+  if (cnt < 0) return;
+  const NodeInfo ni = info[node];
+  if (cnt > kBandCap) {  // the list overflowed: whole valid box
+    if (tid == 0) {
+      const int q = atomicAdd(&bb.qcnt[1], 1);
+      bb.fullq[q] = node;
+    }
+    return;
+  }
+  int* so;
+  double* sv;
+  stage_neighbours(g, ni, nbOff, nbVal, sm, &so, &sv);
+  __syncthreads();
+  TopK64<KT> tk;
+  tk.init();
+  const uint32_t* lst = bb.pos + (long long)node * kBandCap;
+  for (int i = tid; i < cnt; i += blockDim.x) {
+    const uint32_t p = lst[i];
+    Cand64 c;
+    c.val = exact_value(g, ni, so, sv, Y, S, p, nscale);
+    c.h = tie_hash(ni.seedbits, p);
+    c.pos = p;
+    tk.push(c);
+  }
+  block_topk64<KT>(tk, K, best, red);
+  if (tid < K) {
+    const Cand64 c = best[tid];
+    outVal[(long long)node * K + tid] = (float)c.val;
+    outPos[(long long)node * K + tid] = c.pos;
+    if (outCand) {
+      Cand w;
+      w.key = (float)c.val;
+      w.h = c.h;
+      w.pos = c.pos;
+      w.val = (float)c.val;
+      outCand[(long long)node * K + tid] = w;
+    }
+    bpos[tid] = c.pos;
+  }
+  __syncthreads();
+  if (tid == 0 && (chosen || simVals)) draw_write(ni, bpos, K, kf, chosen, node, simVals, tiVals, nVar, indexOut);
+}
+
+// Step 4a. grid (kFullChunks, kFullSlots): chunk x of every full-fallback
+// node, in FP64, K best per chunk into part.
+template <int KT>
+__global__ void __launch_bounds__(256) k_rescore_full(ScanGeom g, const NodeInfo* __restrict__ info,
+                                                      const int* __restrict__ nbOff, const float* __restrict__ nbVal,
+                                                      const float* __restrict__ Y, long long S, double nscale, int K,
+                                                      BandDev bb) {
+  extern __shared__ char sm[];
+  __shared__ Cand64 red[4];
+  const int items = bb.qcnt[1];
+  for (int it = blockIdx.y; it < items; it += gridDim.y) {
+    const int node = bb.fullq[it];
+    const NodeInfo ni = info[node];
+    int* so;
+    double* sv;
+    stage_neighbours(g, ni, nbOff, nbVal, sm, &so, &sv);
+    __syncthreads();
+    const long long n0 = ni.hi[0] - ni.lo[0] + 1, n1 = ni.hi[1] - ni.lo[1] + 1, n2 = ni.hi[2] - ni.lo[2] + 1;
+    const long long total = (n0 > 0 && n1 > 0 && n2 > 0) ? n0 * n1 * n2 : 0;
+    const long long per = (total + kFullChunks - 1) / kFullChunks;
+    const long long beg = (long long)blockIdx.x * per, end = min(total, beg + per);
+    TopK64<KT> tk;
+    tk.init();
+    for (long long i = beg + threadIdx.x; i < end; i += blockDim.x) {
+      const int c0 = ni.lo[0] + (int)(i % n0);
+      const int c1 = ni.lo[1] + (int)((i / n0) % n1);
+      const int c2 = ni.lo[2] + (int)(i / (n0 * n1));
+      if (ni.exOn && excluded(ni, g.s[0], g.s[1], c0, c1, c2)) continue;
+      const uint32_t p = (uint32_t)(c0 + (long long)g.s[0] * (c1 + (long long)g.s[1] * c2));
+      Cand64 c;
+      c.val = exact_value(g, ni, so, sv, Y, S, p, nscale);
+      c.h = tie_hash(ni.seedbits, p);
+      c.pos = p;
+      tk.push(c);
+    }
+    block_topk64<KT>(tk, K, bb.part + ((long long)it * kFullChunks + blockIdx.x) * K, red);
+  }
+}
+
+// Step 4b. kFullSlots blocks: per full-fallback node, the K best of its
+// chunks, outputs and draw.
+template <int KT>
+__global__ void __launch_bounds__(256) k_rescore_full_merge(const NodeInfo* __restrict__ info, int K, float kf,
+                                                            float* __restrict__ outVal, uint32_t* __restrict__ outPos,
+                                                            Cand* __restrict__ outCand, uint32_t* __restrict__ chosen,
+                                                            float* __restrict__ simVals,
+                                                            const float* __restrict__ tiVals, int nVar,
+                                                            uint32_t* __restrict__ indexOut, BandDev bb) {
+  __shared__ Cand64 red[4];
+  __shared__ Cand64 best[kMaxK];
+  __shared__ uint32_t bpos[kMaxK];
+  const int items = bb.qcnt[1];
+  const int tid = threadIdx.x;
+  for (int it = blockIdx.x; it < items; it += gridDim.x) {
+    const int node = bb.fullq[it];
+    const NodeInfo ni = info[node];
+    TopK64<KT> tk;
+    tk.init();
+    const Cand64* pp = bb.part + (long long)it * kFullChunks * K;
+    for (int i = tid; i < kFullChunks * K; i += blockDim.x) tk.push(pp[i]);
+    block_topk64<KT>(tk, K, best, red);
+    if (tid < K) {
+      const Cand64 c = best[tid];
+      outVal[(long long)node * K + tid] = (float)c.val;
+      outPos[(long long)node * K + tid] = c.pos;
+      if (outCand) {
+        Cand w;
+        w.key = (float)c.val;
+        w.h = c.h;
+        w.pos = c.pos;
+        w.val = (float)c.val;
+        outCand[(long long)node * K + tid] = w;
+      }
+      bpos[tid] = c.pos;
+    }
+    __syncthreads();
+    if (tid == 0 && (chosen || simVals)) draw_write(ni, bpos, K, kf, chosen, node, simVals, tiVals, nVar, indexOut);
+    __syncthreads();
+  }
+}
+
+// Per-node API: the band of a materialised map (position space, -inf
+// outside the valid box and at excluded positions) from the FP32 K-th best
+// outVal[K-1]; node 0 of the band buffers.
+__global__ void __launch_bounds__(256) k_map_collect(const float* __restrict__ map, long long S,
+                                                     const float* __restrict__ outVal,
+                                                     const uint32_t* __restrict__ outPos, int K, float eps,
+                                                     BandDev bb) {
+  const int lane = threadIdx.x & 63;
+  const float th = (outPos[K - 1] == kNone) ? -INFINITY : outVal[K - 1] - 2.f * eps * 1.0009765625f;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long n = ((S + 63) / 64) * 64;  // every lane of a wavefront runs the same rounds
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i - lane < n; i += stride) {
+    const bool ok = i < S && map[i] >= th && map[i] > -INFINITY;
+    const unsigned long long m = __ballot(ok);
+    if (m == 0ull) continue;
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&bb.cnt[0], __popcll(m));
+    base = __shfl(base, leader);
+    if (ok) {
+      const int idx = base + __popcll(m & ((1ull << lane) - 1ull));
+      if (idx < kBandCap) bb.pos[idx] = (uint32_t)i;
+    }
+  }
+}
+
+int kt_of(int K) { return K <= 2 ? 2 : K <= 4 ? 4 : K <= 8 ? 8 : 16; }
+
+size_t neighbour_lds(const ScanGeom& g, int nbMax) {
+  return (((size_t)nbMax * g.D * sizeof(int) + 7) & ~(size_t)7) + (size_t)nbMax * g.V * sizeof(double);
+}
+
+}  // namespace
+
+bool exact_selection_on(const qsmi_device* dev) {
+  if (dev->quantum != 0.f || dev->cross) return false;
+  const qsmi_ti* t = dev->ti;
+  const int blocks = t->fast1k ? rows1k_blocks() : row_blocks(t);
+  return blocks <= kMaxBandBlocks;
+}
+
+void band_free(BandBufs* b) {
+  (void)hipFree(b->pos);
+  (void)hipFree(b->cnt);
+  (void)hipFree(b->theta);
+  (void)hipFree(b->queue);
+  (void)hipFree(b->qcnt);
+  (void)hipFree(b->fullq);
+  (void)hipFree(b->part);
+  *b = BandBufs();
+}
+
+int band_grow(BandBufs* b, int nNodes, int blocksPerNode, int K) {
+  const size_t n = (size_t)std::max(nNodes, 1);
+  const size_t q = n * (size_t)std::max(blocksPerNode, 1);
+  if (n <= b->capNodes && q <= b->capQueue && K <= b->K) return QSMI_OK;
+  const size_t nn = std::max(n, b->capNodes);
+  const size_t qq = std::max(q, b->capQueue);
+  const int kk = std::max(K, b->K);
+  band_free(b);
+  if (hipMalloc(&b->pos, nn * kBandCap * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&b->cnt, nn * sizeof(int)) != hipSuccess || hipMalloc(&b->theta, nn * sizeof(float)) != hipSuccess ||
+      hipMalloc(&b->queue, qq * sizeof(int2)) != hipSuccess || hipMalloc(&b->qcnt, 2 * sizeof(int)) != hipSuccess ||
+      hipMalloc(&b->fullq, nn * sizeof(int)) != hipSuccess ||
+      hipMalloc(&b->part, nn * kFullChunks * (size_t)kk * sizeof(Cand64)) != hipSuccess) {
+    band_free(b);
+    return fail(QSMI_ERR_NOMEM, "exact selection buffers: out of device memory");
+  }
+  b->capNodes = nn;
+  b->capQueue = qq;
+  b->K = kk;
+  return QSMI_OK;
+}
+
+int band_select(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const Cand* cand, int nNodes, int K,
+                float kf, const float2* C, const int* nbOff, const float* nbVal, int nbMax, float* outVal,
+                uint32_t* outPos, Cand* outCand, uint32_t* chosen, float* simVals, const float* tiVals, int nVar,
+                uint32_t* indexOut, BandBufs* bb, const int* count) {
+  qsmi_ti* t = dev->ti;
+  hipStream_t st = dev->stream;
+  const int M = cands_per_group(K);
+  int candGroups, wpb, rpb, blocks;
+  if (t->fast1k) {
+    candGroups = rows1k_tiles();
+    wpb = rows1k_waves_per_block();
+    rpb = 0;
+    blocks = rows1k_blocks();
+  } else {
+    candGroups = row_blocks(t);
+    wpb = 1;
+    rpb = rows_per_block(t);
+    blocks = candGroups;
+  }
+  if (dev->cross) {  // one row per transform in the cross-measurement form (no exact selection)
+    rpb = 1 << t->lognv[0];
+    candGroups = (int)(((long long)t->s[1] * t->s[2] + rpb - 1) / rpb);
+    blocks = candGroups;
+  }
+  const bool exact = exact_selection_on(dev);
+  int rc = band_grow(bb, nNodes, exact ? blocks : 1, K);
+  if (rc) return rc;
+  const BandDev bd = dev_view(*bb);
+  hipLaunchKernelGGL(k_band_reset, dim3(1), dim3(64), 0, st, bb->qcnt, (int*)nullptr);
+  const int KT = kt_of(K);
+#define QSMI_BAND_MERGE(KT_)                                                                                       \
+  hipLaunchKernelGGL(k_merge_band<KT_>, dim3((unsigned)nNodes), dim3(256), 0, st, info, cand, candGroups, M, wpb,  \
+                     rpb, K, kf, outVal, outPos, outCand, chosen, simVals, tiVals, nVar, indexOut, bd, count)
+  switch (KT) {
+    case 2: QSMI_BAND_MERGE(2); break;
+    case 4: QSMI_BAND_MERGE(4); break;
+    case 8: QSMI_BAND_MERGE(8); break;
+    default: QSMI_BAND_MERGE(16); break;
+  }
+#undef QSMI_BAND_MERGE
+  if (!exact) return QSMI_OK;
+  const int maxItems = nNodes * blocks;
+  if (t->fast1k)
+    launch_rows1k_collect(g, t->dTw[0], info, C, bb->queue, bb->qcnt, bb->theta, bb->pos, bb->cnt, maxItems, st);
+  else
+    launch_rows_collect(g, t->plan[0], t->lognv[0], t->dTw[0], info, C, bb->queue, bb->qcnt, bb->theta, bb->pos,
+                        bb->cnt, maxItems, st);
+  const double nscale = dev->trueMismatch ? 1.0 : (double)t->N;
+  const size_t lds = neighbour_lds(g, nbMax);
+  if (lds > 48 * 1024) return fail(QSMI_ERR_UNSUPPORTED, "exact selection: n x (variables) too large for LDS");
+#define QSMI_BAND_RESCORE(KT_)                                                                                        \
+  hipLaunchKernelGGL(k_band_rescore<KT_>, dim3((unsigned)nNodes), dim3(256), lds, st, g, info, nbOff, nbVal, t->dSub, \
+                     t->S, nscale, K, kf, outVal, outPos, outCand, chosen, simVals, tiVals, nVar, indexOut, bd,       \
+                     count);                                                                                          \
+  hipLaunchKernelGGL(k_rescore_full<KT_>, dim3(kFullChunks, kFullSlots), dim3(256), lds, st, g, info, nbOff, nbVal,  \
+                     t->dSub, t->S, nscale, K, bd);                                                                   \
+  hipLaunchKernelGGL(k_rescore_full_merge<KT_>, dim3(kFullSlots), dim3(256), 0, st, info, K, kf, outVal, outPos,     \
+                     outCand, chosen, simVals, tiVals, nVar, indexOut, bd)
+  switch (KT) {
+    case 2: QSMI_BAND_RESCORE(2); break;
+    case 4: QSMI_BAND_RESCORE(4); break;
+    case 8: QSMI_BAND_RESCORE(8); break;
+    default: QSMI_BAND_RESCORE(16); break;
+  }
+#undef QSMI_BAND_RESCORE
+  return QSMI_OK;
+}
+
+int band_select_map(qsmi_device* dev, const NodeInfo* info, int nb, int K, float eps, float* outVal,
+                    uint32_t* outPos, BandBufs* bb) {
+  qsmi_ti* t = dev->ti;
+  hipStream_t st = dev->stream;
+  int rc = band_grow(bb, 1, 1, K);
+  if (rc) return rc;
+  const BandDev bd = dev_view(*bb);
+  const ScanGeom g = geom_of(dev);
+  hipLaunchKernelGGL(k_band_reset, dim3(1), dim3(64), 0, st, bb->qcnt, bb->cnt);
+  const long long S = t->S;
+  const int blocks = (int)std::min<long long>(1024, (S + 255) / 256);
+  hipLaunchKernelGGL(k_map_collect, dim3((unsigned)std::max(blocks, 1)), dim3(256), 0, st, dev->dMap, S, outVal,
+                     outPos, K, eps, bd);
+  const double nscale = dev->trueMismatch ? 1.0 : (double)t->N;
+  const size_t lds = neighbour_lds(g, nb);
+  if (lds > 48 * 1024) return fail(QSMI_ERR_UNSUPPORTED, "exact selection: n x (variables) too large for LDS");
+  const float kf = (float)K;
+#define QSMI_MAP_RESCORE(KT_)                                                                                         \
+  hipLaunchKernelGGL(k_band_rescore<KT_>, dim3(1), dim3(256), lds, st, g, info, dev->dOff, dev->dVal, t->dSub, S,     \
+                     nscale, K, kf, outVal, outPos, (Cand*)nullptr, (uint32_t*)nullptr, (float*)nullptr,              \
+                     (const float*)nullptr, 0, (uint32_t*)nullptr, bd, (const int*)nullptr);                          \
+  hipLaunchKernelGGL(k_rescore_full<KT_>, dim3(kFullChunks, kFullSlots), dim3(256), lds, st, g, info, dev->dOff,     \
+                     dev->dVal, t->dSub, S, nscale, K, bd);                                                           \
+  hipLaunchKernelGGL(k_rescore_full_merge<KT_>, dim3(kFullSlots), dim3(256), 0, st, info, K, kf, outVal, outPos,     \
+                     (Cand*)nullptr, (uint32_t*)nullptr, (float*)nullptr, (const float*)nullptr, 0,                   \
+                     (uint32_t*)nullptr, bd)
+  switch (kt_of(K)) {
+    case 2: QSMI_MAP_RESCORE(2); break;
+    case 4: QSMI_MAP_RESCORE(4); break;
+    case 8: QSMI_MAP_RESCORE(8); break;
+    default: QSMI_MAP_RESCORE(16); break;
+  }
+#undef QSMI_MAP_RESCORE
+  return QSMI_OK;
+}
+
+}  // namespace qsmi

@@ -23,6 +23,8 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "qsmi_kernels.hpp"
 #include "qsmi_select.hpp"
 
@@ -90,18 +92,60 @@ __device__ __forceinline__ void fft1000(float2* buf, float2 (&u)[10], const floa
   fft1000<DIR>(buf, u, tw, j, j < 100, fromLds);
 }
 
+// Row pair p of one node's CT: Hermitian completion of the half spectrum
+// (h0 = 501: columns k = j + 100 q >= 501 are conj(Z[1000 - k]); the
+// imaginary parts of columns 0 and 500 do not enter a c2r), then one
+// two-for-one c2r along dim0. On exit u[q] holds position c0 = j + 100 q of
+// row 2p (x) and row 2p+1 (y). Every thread of the block calls it (barriers
+// inside); va / vb: the rows are valid, act = va || vb.
+template <int G>
+__device__ __forceinline__ void rows1k_pair(const float2* __restrict__ Cn, int p, int h0, bool va, bool vb, bool act,
+                                            bool gin, int grp, int j, float2 (&buf)[G][L], const float2* twl,
+                                            float2 (&u)[10]) {
+  if (act) {
+    const float4* rowp = reinterpret_cast<const float4*>(Cn + (long long)p * h0 * TR);
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+      const int k = j + 100 * q;
+      const bool cj = (q >= 6) || (q == 5 && j > 0);
+      const int kk = cj ? (L - k) : k;
+      const float4 ab = rowp[kk];  // (row 2p, row 2p+1) at column kk
+      float2 A = va ? make_float2(ab.x, ab.y) : make_float2(0.f, 0.f);
+      float2 B = vb ? make_float2(ab.z, ab.w) : make_float2(0.f, 0.f);
+      if ((q == 0 || q == 5) && j == 0) {
+        A.y = 0.f;
+        B.y = 0.f;
+      }
+      if (cj) {
+        A.y = -A.y;
+        B.y = -B.y;
+      }
+      u[q] = make_float2(A.x - B.y, A.y + B.x);
+    }
+  }
+  fft1000<+1>(buf[gin ? grp : 0], u, twl, j, act, false);
+}
+
 }  // namespace f1k
 
 // ---------------------------------------------------------------------------
 // Distinct neighbour rows of each node (one 64-thread block per node):
-// TapInfo with the rows ordered by (row mod 10, row), and tapOf[node][row] =
-// tap index of every row that holds a neighbour.
+// TapInfo with the rows ordered by (row mod 10, row), and the neighbours'
+// slots in tap order: tapPos[node][j] = start[tap of j] + (number of earlier
+// neighbours in the same tap), so the pencil can sum every tap's
+// contributions in neighbour-list order (a fixed summation order, hence
+// reproducible FP32 maps). T = 0 (full transform) above kTapMax rows or
+// kTapNbMax neighbours.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(64) k_taps1k(const NodeInfo* __restrict__ info, const int* __restrict__ nbOff,
-                                              TapInfo* __restrict__ taps, uint8_t* __restrict__ tapOf,
+                                              TapInfo* __restrict__ taps, uint16_t* __restrict__ tapPos,
                                               const int* __restrict__ nodeCount) {
   using namespace f1k;
   __shared__ int flag[L];
+  __shared__ short tix[L];           // tap index of a present row
+  __shared__ short tapJ[kTapNbMax];  // tap of neighbour j
+  __shared__ int cntT[kTapMax];
+  __shared__ int startL[kTapMax + 1];
   const int node = blockIdx.x, lane = threadIdx.x;
   if (nodeCount && node >= *nodeCount) return;
   const NodeInfo ni = info[node];
@@ -114,7 +158,6 @@ __global__ void __launch_bounds__(64) k_taps1k(const NodeInfo* __restrict__ info
   }
   __syncthreads();
   TapInfo* tp = taps + node;
-  uint8_t* map = tapOf + (long long)node * L;
   int count = 0;
   if (lane == 0) tp->cls[0] = 0;
   for (int c = 0; c < 10; ++c) {
@@ -126,13 +169,43 @@ __global__ void __launch_bounds__(64) k_taps1k(const NodeInfo* __restrict__ info
       const int before = __popcll(mask & ((1ull << lane) - 1ull));
       if (present && count + before < kTapMax) {
         tp->row[count + before] = row;
-        map[row] = (uint8_t)(count + before);
+        tix[row] = (short)(count + before);
       }
       count += __popcll(mask);
     }
     if (lane == 0) tp->cls[c + 1] = count;
   }
-  if (lane == 0) tp->T = count <= kTapMax ? count : 0;
+  const int T = (count <= kTapMax && ni.nb <= kTapNbMax) ? count : 0;
+  if (lane == 0) tp->T = T;
+  if (T == 0) return;  // block-uniform
+  for (int t = lane; t < T; t += 64) cntT[t] = 0;
+  __syncthreads();
+  for (int t = lane; t < ni.nb; t += 64) {
+    int r = (-nbOff[(ni.nbStart + t) * 2 + 1]) % L;
+    if (r < 0) r += L;
+    const short tt = tix[r];
+    tapJ[t] = tt;
+    atomicAdd(&cntT[tt], 1);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    int acc = 0;
+    for (int t = 0; t < T; ++t) {
+      startL[t] = acc;
+      tp->start[t] = acc;
+      acc += cntT[t];
+    }
+    startL[T] = acc;
+    tp->start[T] = acc;
+  }
+  __syncthreads();
+  uint16_t* out = tapPos + (long long)node * kTapNbMax;
+  for (int t = lane; t < ni.nb; t += 64) {
+    const short tt = tapJ[t];
+    int r = 0;
+    for (int q = 0; q < t; ++q) r += (tapJ[q] == tt) ? 1 : 0;
+    out[t] = (uint16_t)(startL[tt] + r);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -155,13 +228,17 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) k
                                                  const float2* __restrict__ tw1k, int nNodes, int npb,
                                                  const NodeInfo* __restrict__ info, const int* __restrict__ nbOff,
                                                  const float* __restrict__ nbVal, const TapInfo* __restrict__ taps,
-                                                 const uint8_t* __restrict__ tapOf, float2* __restrict__ CT,
+                                                 const uint16_t* __restrict__ tapPos, float2* __restrict__ CT,
                                                  long long ctStride, const int* __restrict__ count, int chunks) {
   using namespace f1k;
-  __shared__ float2 sb[L];            // scatter / exchange buffer
+  constexpr int kChunk = 512;         // neighbours per pass of the full-transform column build
+  __shared__ float2 sb[L];            // contributions in tap order / column / exchange buffer
   __shared__ float2 twl[L];           // exp(-2 pi i t / 1000)
   __shared__ float2 G[2 * kTapMax];   // [tap][sub-variable of the pair]
   __shared__ int rowL[kTapMax];
+  __shared__ int startL[kTapMax + 1];
+  __shared__ float2 cl[kChunk];       // full-transform path: contributions in list order
+  __shared__ short rl[kChunk];        // and their rows
   const int lane = threadIdx.x;  // butterfly index j for lane < 100
   const int h0 = g.h0;  // 501
   const int slice = (h0 + NXCD - 1) / NXCD;
@@ -183,7 +260,8 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) k
   if (nodeBeg >= nodeEnd) return;
   const int V = g.V;
   const bool act = lane < 100;
-  for (int i = lane; i < L; i += 128) twl[i] = tw1k[i];  // first read after a barrier below
+  for (int i = lane; i < L; i += 128) twl[i] = tw1k[i];
+  __syncthreads();
   for (int node = nodeBeg; node < nodeEnd; ++node) {
     const NodeInfo ni = info[node];
     if (ni.lo[1] > ni.hi[1] || ni.lo[0] > ni.hi[0]) continue;
@@ -193,31 +271,39 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) k
     const TapInfo* tp = taps + node;
     const int T = tp->T;
     if (T > 0) {
-      const uint8_t* tmap = tapOf + (long long)node * L;
+      const uint16_t* tpos = tapPos + (long long)node * kTapNbMax;
+      if (lane < T) rowL[lane] = tp->row[lane];
+      if (lane <= T) startL[lane] = tp->start[lane];
       for (int a0 = 0; a0 < V; a0 += 2) {
         const bool two = a0 + 1 < V;
-        for (int i = lane; i < 2 * T; i += 128) G[i] = make_float2(0.f, 0.f);
-        if (lane < T) rowL[lane] = tp->row[lane];
-        __syncthreads();
+        // every neighbour's value * exp(+2 pi i k0 d0 / 1000) into its slot in
+        // tap order (sb as [slot][sub-variable of the pair]) ...
         for (int t = lane; t < ni.nb; t += 128) {
           const int jj = ni.nbStart + t;
           const float v0 = nbVal[(long long)jj * V + a0];
           const float v1 = two ? nbVal[(long long)jj * V + a0 + 1] : 0.f;
-          if (v0 == 0.f && v1 == 0.f) continue;
           const int d0 = nbOff[jj * 2 + 0];
-          const int d1 = nbOff[jj * 2 + 1];
           int e = (k0 * d0) % L;
           if (e < 0) e += L;
           const float2 w = twl[e];
-          int q = (-d1) % L;
-          if (q < 0) q += L;
-          const int tap = tmap[q];
-          atomicAdd(&G[2 * tap].x, v0 * w.x);
-          atomicAdd(&G[2 * tap].y, -v0 * w.y);
-          if (two) {
-            atomicAdd(&G[2 * tap + 1].x, v1 * w.x);
-            atomicAdd(&G[2 * tap + 1].y, -v1 * w.y);
+          const int slot = tpos[t];
+          sb[2 * slot] = make_float2(v0 * w.x, -v0 * w.y);
+          sb[2 * slot + 1] = make_float2(v1 * w.x, -v1 * w.y);
+        }
+        __syncthreads();
+        // ... then one lane per tap sums its row's neighbours in list order
+        if (lane < T) {
+          float2 g0 = make_float2(0.f, 0.f), g1 = make_float2(0.f, 0.f);
+          const int e1 = startL[lane + 1];
+          for (int i = startL[lane]; i < e1; ++i) {
+            const float2 c0 = sb[2 * i], c1 = sb[2 * i + 1];
+            g0.x += c0.x;
+            g0.y += c0.y;
+            g1.x += c1.x;
+            g1.y += c1.y;
           }
+          G[2 * lane] = g0;
+          G[2 * lane + 1] = g1;
         }
         __syncthreads();
         if (act) {
@@ -271,24 +357,46 @@ __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(3))) k
       }
     } else {
       for (int a = 0; a < V; ++a) {
-        // zero the scatter buffer, then the sparse neighbour column:
-        // value * exp(+2 pi i k0 d0 / 1000) at row (-d1) mod 1000
-        for (int i = lane; i < L; i += 128) sb[i] = make_float2(0.f, 0.f);
-        __syncthreads();
-        for (int t = lane; t < ni.nb; t += 128) {
-          const int jj = ni.nbStart + t;
-          const float v = nbVal[(long long)jj * V + a];
-          if (v == 0.f) continue;
-          const int d0 = nbOff[jj * 2 + 0];
-          const int d1 = nbOff[jj * 2 + 1];
-          int e = (k0 * d0) % L;
-          if (e < 0) e += L;
-          const float2 w = twl[e];
-          int q = (-d1) % L;
-          if (q < 0) q += L;
-          atomicAdd(&sb[q].x, v * w.x);
-          atomicAdd(&sb[q].y, -v * w.y);
+        // the sparse neighbour column: value * exp(+2 pi i k0 d0 / 1000) at
+        // row (-d1) mod 1000; lane l owns rows l + 128 i and sums their
+        // neighbours in list order
+        float2 rs[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) rs[i] = make_float2(0.f, 0.f);
+        for (int c0 = 0; c0 < ni.nb; c0 += kChunk) {
+          const int cn = min(kChunk, ni.nb - c0);
+          for (int t = lane; t < cn; t += 128) {
+            const int jj = ni.nbStart + c0 + t;
+            const float v = nbVal[(long long)jj * V + a];
+            const int d0 = nbOff[jj * 2 + 0];
+            const int d1 = nbOff[jj * 2 + 1];
+            int e = (k0 * d0) % L;
+            if (e < 0) e += L;
+            const float2 w = twl[e];
+            int q = (-d1) % L;
+            if (q < 0) q += L;
+            cl[t] = make_float2(v * w.x, -v * w.y);
+            rl[t] = (short)q;
+          }
+          __syncthreads();
+          for (int t = 0; t < cn; ++t) {
+            const int q = rl[t];
+            if ((q & 127) == lane) {
+              const float2 c = cl[t];
+              const int i = q >> 7;
+#pragma unroll
+              for (int k = 0; k < 8; ++k)
+                if (k == i) {
+                  rs[k].x += c.x;
+                  rs[k].y += c.y;
+                }
+            }
+          }
+          __syncthreads();
         }
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+          if (lane + 128 * i < L) sb[lane + 128 * i] = rs[i];
         __syncthreads();
         float2 u[10];
         fft1000<-1>(sb, u, twl, lane, true);
@@ -351,7 +459,7 @@ __global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k(ScanGeom 
   const int h0 = g.h0;
   // one candidate group of K per wavefront of the block
   const int wavesPerBlock = blockDim.x >> 6;
-  Cand* out = cand ? cand + ((long long)node * candBlocks + (long long)blockIdx.x * wavesPerBlock) * K : nullptr;
+  Cand* out = cand ? cand + ((long long)node * candBlocks + (long long)blockIdx.x * wavesPerBlock) * KT : nullptr;
   const int p = blockIdx.x * G + grp;  // this group's pair
   const int ca = 2 * p;
   const bool gin = grp < G && p < L / 2;
@@ -360,7 +468,7 @@ __global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k(ScanGeom 
   // block-uniform exit when none of the block's rows is valid
   const int rlo = max(ni.lo[1], 2 * blockIdx.x * G), rhi = min(ni.hi[1], 2 * (blockIdx.x * G + G) - 1);
   if (rlo > rhi || ni.lo[0] > ni.hi[0]) {
-    if (out && tid < K * wavesPerBlock) {
+    if (out && tid < KT * wavesPerBlock) {
       Cand w;
       w.key = -INFINITY;
       w.h = 0u;
@@ -373,32 +481,7 @@ __global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k(ScanGeom 
   for (int i = tid; i < L; i += blockDim.x) twl[i] = tw1k[i];  // first read after the FFT's first barrier
   const bool act = va || vb;  // this thread's group has a valid row (j < 100 by construction)
   float2 u[10];
-  if (act) {
-    const float4* rowp =
-        reinterpret_cast<const float4*>(CT + (long long)node * ctStride + (long long)p * h0 * TR);
-    // Hermitian completion of the half spectrum (h0 = 501): columns
-    // k = j + 100 q >= 501 are conj(Z[1000 - k]); the imaginary parts of
-    // columns 0 and 500 do not enter a c2r
-#pragma unroll
-    for (int q = 0; q < 10; ++q) {
-      const int k = j + 100 * q;
-      const bool cj = (q >= 6) || (q == 5 && j > 0);
-      const int kk = cj ? (L - k) : k;
-      const float4 ab = rowp[kk];  // (row 2p, row 2p+1) at column kk
-      float2 A = va ? make_float2(ab.x, ab.y) : make_float2(0.f, 0.f);
-      float2 B = vb ? make_float2(ab.z, ab.w) : make_float2(0.f, 0.f);
-      if ((q == 0 || q == 5) && j == 0) {
-        A.y = 0.f;
-        B.y = 0.f;
-      }
-      if (cj) {
-        A.y = -A.y;
-        B.y = -B.y;
-      }
-      u[q] = make_float2(A.x - B.y, A.y + B.x);
-    }
-  }
-  fft1000<+1>(buf[gin ? grp : 0], u, twl, j, act, false);
+  f1k::rows1k_pair<G>(CT + (long long)node * ctStride, p, h0, va, vb, act, gin, grp, j, buf, twl, u);
   const float scale = ni.scale, d0 = ni.delta0, quantum = ni.quantum;
   const int s0 = g.s[0];
   const int lo0 = ni.lo[0], hi0 = ni.hi[0];
@@ -457,8 +540,10 @@ __global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k(ScanGeom 
       m[i] = hi;
     }
   }
+  // KT candidates per wavefront (the exact selection needs more than K to
+  // see whether a wavefront holds further positions near the K-th best);
   // tau2 also excludes the -inf of invalid positions
-  const float tau2 = fmaxf(wave_kth_key<KT>(m, K), -3.402823466e38f);
+  const float tau2 = fmaxf(wave_kth_key<KT>(m, KT), -3.402823466e38f);
   // pass 2: exact device rule among the keys >= tau (normally K per block)
   RankList<KT> rl;
   rl.init();
@@ -470,7 +555,67 @@ __global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k(ScanGeom 
       rl.insert(key[e], tie_hash(ni.seedbits, pos), pos, (w ? u[q].y : u[q].x) * scale + d0);
     }
   }
-  wave_extract<KT>(rl, K, out + (threadIdx.x >> 6) * K);
+  wave_extract<KT>(rl, KT, out + (threadIdx.x >> 6) * KT);
+}
+
+// ---------------------------------------------------------------------------
+// Exact selection, second pass on the 1000 x 1000 path: for every queued
+// (node, rows block) whose candidates may hide more positions within the
+// node's band, the block's rows again (same FFT and epilogue as k_rows1k),
+// appending every valid position whose FP32 value is >= theta[node] to the
+// node's band list. Persistent grid over the device queue count.
+// ---------------------------------------------------------------------------
+template <int G>
+__global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k_collect(
+    ScanGeom g, const float2* __restrict__ tw1k, const NodeInfo* __restrict__ info, const float2* __restrict__ CT,
+    long long ctStride, const int2* __restrict__ queue, const int* __restrict__ qcnt,
+    const float* __restrict__ theta, uint32_t* __restrict__ bandPos, int* __restrict__ bandCnt) {
+  using namespace f1k;
+  __shared__ float2 buf[G][L];
+  __shared__ float2 twl[L];
+  const int tid = threadIdx.x;
+  const int grp = tid / 100;
+  const int j = tid - 100 * grp;
+  const int lane = tid & 63;
+  const int h0 = g.h0, s0 = g.s[0];
+  const int items = *qcnt;
+  if ((int)blockIdx.x >= items) return;
+  for (int i = tid; i < L; i += blockDim.x) twl[i] = tw1k[i];  // first read after the FFT's first barrier
+  for (int it = blockIdx.x; it < items; it += gridDim.x) {
+    const int2 qi = queue[it];
+    const int node = qi.x, bx = qi.y;
+    const NodeInfo ni = info[node];
+    const int p = bx * G + grp;
+    const int ca = 2 * p;
+    const bool gin = grp < G && p < L / 2;
+    const bool va = gin && (ca >= ni.lo[1] && ca <= ni.hi[1]);
+    const bool vb = gin && (ca + 1 >= ni.lo[1] && ca + 1 <= ni.hi[1]);
+    const bool act = va || vb;
+    float2 u[10];
+    rows1k_pair<G>(CT + (long long)node * ctStride, p, h0, va, vb, act, gin, grp, j, buf, twl, u);
+    const float th = theta[node];
+    const float scale = ni.scale, d0 = ni.delta0;
+    const int lo0 = ni.lo[0], hi0 = ni.hi[0];
+    uint32_t* out = bandPos + (long long)node * kBandCap;
+#pragma unroll
+    for (int e = 0; e < 20; ++e) {
+      const int q = e >> 1, w = e & 1;
+      const int c0 = j + 100 * q;
+      const float val = (w ? u[q].y : u[q].x) * scale + d0;
+      bool ok = act && (w ? vb : va) && c0 >= lo0 && c0 <= hi0 && val >= th;
+      if (ok && ni.exOn && excluded(ni, s0, g.s[1], c0, ca + w, 0)) ok = false;
+      const unsigned long long m = __ballot(ok);
+      if (m == 0ull) continue;
+      const int leader = __ffsll((long long)m) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(&bandCnt[node], __popcll(m));
+      base = __shfl(base, leader);
+      if (ok) {
+        const int idx = base + __popcll(m & ((1ull << lane) - 1ull));
+        if (idx < kBandCap) out[idx] = (uint32_t)(c0 + s0 * (ca + w));
+      }
+    }
+  }
 }
 
 // YT[A][k0][k1] <- Yhat[A][k1][k0] (once per TI)
@@ -502,15 +647,17 @@ void launch_transpose_spectra(const float2* Y, float2* YT, int h0, int f1, int n
 }
 
 void launch_pencil1k(const ScanGeom& g, const float2* YT, const float2* tw1k, int nNodes, int npb,
-                     const NodeInfo* info, const int* nbOff, const float* nbVal, TapInfo* taps, uint8_t* tapOf,
+                     const NodeInfo* info, const int* nbOff, const float* nbVal, TapInfo* taps, uint16_t* tapPos,
                      float2* CT, const int* count, hipStream_t st) {
-  hipLaunchKernelGGL(k_taps1k, dim3((unsigned)nNodes), dim3(64), 0, st, info, nbOff, taps, tapOf, count);
+  hipLaunchKernelGGL(k_taps1k, dim3((unsigned)nNodes), dim3(64), 0, st, info, nbOff, taps, tapPos, count);
   const int slice = (g.h0 + f1k::NXCD - 1) / f1k::NXCD;
   const int chunks = (nNodes + npb - 1) / npb;
   const long long blocks = (long long)f1k::NXCD * slice * chunks;
   hipLaunchKernelGGL(k_pencil1k, dim3((unsigned)blocks), dim3(128), 0, st, g, YT, tw1k, nNodes, npb, info, nbOff, nbVal,
-                     taps, tapOf, CT, ct1k_stride(g.h0), count, chunks);
+                     taps, tapPos, CT, ct1k_stride(g.h0), count, chunks);
 }
+
+int cands_per_group(int K);
 
 // row pairs per rows block: 5 pairs -> 512 threads, 98 % of the lanes busy
 constexpr int kRowsG = 5;
@@ -520,6 +667,17 @@ constexpr int kRowsWaves = (100 * kRowsG + 63) / 64;
 
 // candidate groups per node: one per wavefront of every rows block
 int rows1k_tiles() { return kRowsBlocks * kRowsWaves; }
+int rows1k_blocks() { return kRowsBlocks; }
+int rows1k_waves_per_block() { return kRowsWaves; }
+
+void launch_rows1k_collect(const ScanGeom& g, const float2* tw1k, const NodeInfo* info, const float2* CT,
+                           const int2* queue, const int* qcnt, const float* theta, uint32_t* bandPos, int* bandCnt,
+                           int maxItems, hipStream_t st) {
+  // persistent grid: at most one block per possible item, at most 2048
+  const int blocks = std::max(1, std::min(maxItems, 2048));
+  hipLaunchKernelGGL((k_rows1k_collect<kRowsG>), dim3((unsigned)blocks), dim3(64 * kRowsWaves), 0, st, g, tw1k, info,
+                     CT, ct1k_stride(g.h0), queue, qcnt, theta, bandPos, bandCnt);
+}
 
 hipError_t set_rows1k_lds() { return hipSuccess; }
 
@@ -529,13 +687,14 @@ void launch_rows1k(const ScanGeom& g, const float2* tw1k, const NodeInfo* info, 
   dim3 grid((unsigned)blocks, (unsigned)nNodes);
   dim3 block(64 * ((100 * kRowsG + 63) / 64));
   const long long cs = ct1k_stride(g.h0);
-  if (K <= 2)
+  const int M = cands_per_group(K);
+  if (M <= 2)
     hipLaunchKernelGGL((k_rows1k<2, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map,
                        count);
-  else if (K <= 4)
+  else if (M <= 4)
     hipLaunchKernelGGL((k_rows1k<4, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map,
                        count);
-  else if (K <= 8)
+  else if (M <= 8)
     hipLaunchKernelGGL((k_rows1k<8, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map,
                        count);
   else

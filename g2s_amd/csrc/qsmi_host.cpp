@@ -152,6 +152,61 @@ int qsmi_ti_subimages(int ndim, const unsigned* s, int nVar, const int* types, c
   return QSMI_OK;
 }
 
+// The point reflection of DataImage::convertInput4Xcorr (include/
+// DataImage.hpp:538-599) undone: TI cell c of sub-image A sits at
+// N - 1 - flat_fft(c) of the reflected image, N = prod(fftSize).
+int qsmi_subimages_from_reflected(int ndim, const unsigned* srcSize, const unsigned* fftSize, int nSub,
+                                  const float* const* reflected, float* out) {
+  if (!srcSize || !fftSize || !reflected || !out || ndim < 1 || ndim > 3 || nSub < 1)
+    return hfail(QSMI_ERR_ARG, "invalid argument");
+  unsigned s[3] = {1, 1, 1}, f[3] = {1, 1, 1};
+  for (int i = 0; i < ndim; ++i) {
+    s[i] = srcSize[i];
+    f[i] = fftSize[i];
+    if (s[i] < 1 || f[i] < s[i]) return hfail(QSMI_ERR_ARG, "fftSize must be >= srcSize >= 1");
+  }
+  const size_t S = (size_t)s[0] * s[1] * s[2];
+  const size_t N = (size_t)f[0] * f[1] * f[2];
+  for (int A = 0; A < nSub; ++A) {
+    const float* r = reflected[A];
+    if (!r) return hfail(QSMI_ERR_ARG, "null sub-image pointer");
+    float* o = out + (size_t)A * S;
+    size_t inside = 0;
+    for (unsigned c2 = 0; c2 < s[2]; ++c2)
+      for (unsigned c1 = 0; c1 < s[1]; ++c1)
+        for (unsigned c0 = 0; c0 < s[0]; ++c0) {
+          const size_t flat = ((size_t)c2 * f[1] + c1) * f[0] + c0;
+          o[((size_t)c2 * s[1] + c1) * s[0] + c0] = r[N - 1 - flat];
+          inside++;
+        }
+    // the padding (reflected cells outside the TI box) must be zero
+    if (inside != N) {
+      for (size_t q = 0; q < N; ++q) {
+        const size_t flat = N - 1 - q;
+        const unsigned c0 = (unsigned)(flat % f[0]), c1 = (unsigned)((flat / f[0]) % f[1]),
+                       c2 = (unsigned)(flat / ((size_t)f[0] * f[1]));
+        if (c0 < s[0] && c1 < s[1] && c2 < s[2]) continue;
+        if (r[q] != 0.f)
+          return hfail(QSMI_ERR_ARG, "sub-image not in the reflected FFT-grid layout (non-zero padding)");
+      }
+    }
+  }
+  return QSMI_OK;
+}
+
+int qsmi_ti_create_from_reflected(int gpu, int ndim, const unsigned* srcSize, const unsigned* fftSize, int nSub,
+                                  const float* const* reflected, qsmi_ti** out) {
+  if (!out) return hfail(QSMI_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (!srcSize || !fftSize || ndim < 1 || ndim > 3 || nSub < 1) return hfail(QSMI_ERR_ARG, "invalid argument");
+  size_t S = 1;
+  for (int i = 0; i < ndim; ++i) S *= srcSize[i];
+  std::vector<float> sub((size_t)nSub * S);
+  int rc = qsmi_subimages_from_reflected(ndim, srcSize, fftSize, nSub, reflected, sub.data());
+  if (rc) return rc;
+  return qsmi_ti_create(gpu, ndim, srcSize, fftSize, nSub, sub.data(), out);
+}
+
 int qsmi_coef_matrix_x(int nVar, const int* types, float* alpha, float* alphaX, int* convTypes, int* subVarOf,
                        int* nSubOut) {
   if (!types || !alpha || !alphaX || !convTypes || !subVarOf || !nSubOut) return hfail(QSMI_ERR_ARG, "null argument");

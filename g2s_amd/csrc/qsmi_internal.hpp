@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -26,15 +27,29 @@ struct qsmi_ti {
   float2* dYhat = nullptr;
   float2* dYT = nullptr;  // column-contiguous copy (1000 x 1000 fast path)
   float* dSub = nullptr;  // the sub-images on the TI grid, nSub x S
+  float rms[qsmi::kMaxSub] = {};  // root mean square of each sub-image over the FFT grid
   bool fast1k = false;
   float2* dTw[3] = {nullptr, nullptr, nullptr};
   FftPlan plan[3];
   int lognv[3] = {0, 0, 0};
   hipStream_t stream = nullptr;
-  int refs = 0;
+  std::atomic<int> refs{0};  // devices over this TI (created from several host threads at once)
 };
 
 struct WaveBufs;
+
+// Buffers of the exact selection (qsmi_band.hip), grown on demand.
+struct BandBufs {
+  size_t capNodes = 0, capQueue = 0;
+  int K = 0;
+  uint32_t* pos = nullptr;       // [node][kBandCap] positions to rescore
+  int* cnt = nullptr;            // [node]
+  float* theta = nullptr;        // [node] lower end of the band
+  int2* queue = nullptr;         // second-pass items (node, rows block)
+  int* qcnt = nullptr;           // [2]: queue items, full-fallback nodes
+  int* fullq = nullptr;          // full-fallback nodes
+  qsmi::Cand64* part = nullptr;  // full-fallback chunk results
+};
 
 struct qsmi_device {
   qsmi_ti* ti = nullptr;
@@ -59,7 +74,7 @@ struct qsmi_device {
   float2* dC = nullptr;
   size_t capC = 0;
   qsmi::TapInfo* dTap = nullptr;  // 1000^2 path: distinct neighbour rows per node
-  uint8_t* dTapOf = nullptr;   // [node][1000] tap index of a row
+  uint16_t* dTapPos = nullptr;  // [node][kTapNbMax] slot of each neighbour in tap order
   size_t capTap = 0;
   Cand* dCand = nullptr;
   size_t capCand = 0;
@@ -69,6 +84,8 @@ struct qsmi_device {
   size_t capOff = 0;
   float* dVal = nullptr;
   size_t capVal = 0;
+  uint32_t* dOrd = nullptr;  // neighbours in pencil-row order (generic spectrum stage)
+  size_t capOrd = 0;
   float* dOutVal = nullptr;
   uint32_t* dOutPos = nullptr;
   uint32_t* dChosen = nullptr;
@@ -81,6 +98,11 @@ struct qsmi_device {
   float stageMs[4] = {0, 0, 0, 0};
   WaveBufs* wb = nullptr;  // workspace of qsmi_scan_wave
   std::vector<hipEvent_t> evPool;  // per-wave stage events of the resident loop
+  // per-node API: the last node (exact selection of its map) and whether the
+  // map holds a value set from outside (then the FP32 selection is kept)
+  NodeInfo lastNode{};
+  bool mapEdited = false;
+  BandBufs band;
 };
 
 // Device buffers of one wave (grown on demand) and pinned host staging.
@@ -89,16 +111,19 @@ struct WaveBufs {
   NodeInfo* dInfo = nullptr;
   int* dOff = nullptr;
   float* dVal = nullptr;
+  uint32_t* dOrd = nullptr;   // neighbours in pencil-row order (generic spectrum stage)
   float2* dC = nullptr;
   Cand* dCand = nullptr;
+  Cand* dCandEx = nullptr;    // [node][K] exact K best as candidates (learned proposal)
   float* dOutVal = nullptr;
   uint32_t* dOutPos = nullptr;
   uint32_t* dChosen = nullptr;
   NodeInfo* hInfo = nullptr;  // pinned
   int* hOff = nullptr;        // pinned
   float* hVal = nullptr;      // pinned
-  int nNodes = 0, K = 0, rowBlocks = 0;
+  int nNodes = 0, K = 0, rowBlocks = 0, nbMax = 0;
   float k = 0.f;
+  BandBufs band;
 };
 
 struct qsmi_wave {
@@ -132,8 +157,9 @@ int read_wave(qsmi_device* dev, WaveBufs* b, float* outErrors, unsigned* outPosi
 // (returns a qsmi status: tap tables may be allocated)
 // count (device, optional; 1000x1000 path only): nodes [0, *count) of the
 // nNodes are scanned, the rest skipped at no cost
+// ord: work space of one uint32 per neighbour (the generic path sorts them)
 int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeInfo* info, const int* off,
-                    const float* val, float2* C, const int* count = nullptr);
+                    const float* val, uint32_t* ord, float2* C, const int* count = nullptr);
 // tap tables of the 1000^2 spectrum stage for nNodes nodes per launch
 int ensure_tap_tables(qsmi_device* dev, int nNodes);
 // crossOff > 0 (cross-measurement form, generic path): node q's count layer
@@ -142,14 +168,38 @@ void launch_rows(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, cons
                  int rowBlocks, int K, Cand* cand, float* map, const int* count = nullptr, int crossOff = 0);
 // merge + draw; when simVals != nullptr also writes the drawn TI values and
 // the index image of each node's cell (NodeInfo::cell)
+// M: candidates per block
 void launch_merge_draw(const ScanGeom& g, const NodeInfo* info, const Cand* cand, int nNodes, int candBlocks,
-                       int rowsPerBlock, int K, float kf, float* outVal, uint32_t* outPos, uint32_t* chosen,
+                       int M, int rowsPerBlock, int K, float kf, float* outVal, uint32_t* outPos, uint32_t* chosen,
                        float* simVals, const float* tiVals, int nVar, uint32_t* indexOut, hipStream_t st);
+// candidates per rows group (wavefront or block) for ceil(k) = K
+int cands_per_group(int K);
+
+// Exact selection (qsmi_band.hip). exact_selection_on: the device's maps
+// take it (no tie quantum, no NaN TI, a rows grid it can track).
+bool exact_selection_on(const qsmi_device* dev);
+int band_grow(BandBufs* b, int nNodes, int blocksPerNode, int K);
+void band_free(BandBufs* b);
+// After the rows stage of nNodes nodes (cand from launch_rows, C the
+// spectrum-stage buffer the rows stage read): the K best of every node by the
+// exact selection (nodes with eps == 0: by their FP32 candidates) into
+// outVal/outPos [node][K] (and outCand when given), then the rank draw into
+// chosen / the simulation write when given. nbMax: largest neighbour count.
+// count (device, optional): only nodes [0, *count) (compacted lists).
+int band_select(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const Cand* cand, int nNodes, int K,
+                float kf, const float2* C, const int* nbOff, const float* nbVal, int nbMax, float* outVal,
+                uint32_t* outPos, Cand* outCand, uint32_t* chosen, float* simVals, const float* tiVals, int nVar,
+                uint32_t* indexOut, BandBufs* bb, const int* count = nullptr);
+// Per-node API: exact K best of the device's materialised map (dev->dMap)
+// for the node in info[0] (neighbours in dev->dOff / dev->dVal), given its
+// FP32 K best in outVal/outPos (overwritten).
+int band_select_map(qsmi_device* dev, const NodeInfo* info, int nb, int K, float eps, float* outVal,
+                    uint32_t* outPos, BandBufs* bb);
 void launch_merge_ti(const NodeInfo* info, int nN, int nTI, int K, float kf, const float* val, const uint32_t* pos,
                      long long stride, float quantum, float* simVals, const float* tiVals, int nVar, TiCum cum,
                      uint32_t* indexOut, hipStream_t st);
 void launch_info_shift(const NodeInfo* in, NodeInfo* out, int nN, int dh0, int dh1, int dh2, float scale,
-                       hipStream_t st);
+                       float epsFactor, hipStream_t st);
 
 // Inputs of the device-resident path loop (qsmi_sim.hip). Device pointers
 // are indexed by path position after the hard data (i in [0, nPath)).

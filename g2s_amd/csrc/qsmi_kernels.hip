@@ -29,88 +29,9 @@
 #include <stdint.h>
 
 #include "qsmi_kernels.hpp"
+#include "qsmi_select.hpp"
 
 namespace qsmi {
-
-// ---------------------------------------------------------------------------
-// top-K of the device rule
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ Cand worst_cand() {
-  Cand c;
-  c.key = -INFINITY;
-  c.h = 0u;
-  c.pos = 0xFFFFFFFFu;
-  c.val = -INFINITY;
-  return c;
-}
-
-template <int KT>
-struct TopK {
-  Cand c[KT];
-  __device__ __forceinline__ void init() {
-#pragma unroll
-    for (int i = 0; i < KT; ++i) c[i] = worst_cand();
-  }
-  __device__ __forceinline__ void push(Cand cur) {
-    if (!cand_better(cur, c[KT - 1])) return;
-#pragma unroll
-    for (int i = 0; i < KT; ++i) {
-      if (cand_better(cur, c[i])) {
-        Cand t = c[i];
-        c[i] = cur;
-        cur = t;
-      }
-    }
-  }
-  // value fast path: the hash is only computed when the key can enter
-  __device__ __forceinline__ void push_value(float val, float quantum, uint32_t pos, uint32_t seedbits) {
-    float key = (quantum > 0.f) ? rintf(val * (1.f / quantum)) : val;
-    if (!(key >= c[KT - 1].key)) return;  // also rejects NaN
-    Cand cd;
-    cd.key = key;
-    cd.val = val;
-    cd.pos = pos;
-    cd.h = tie_hash(seedbits, pos);
-    push(cd);
-  }
-  __device__ __forceinline__ void pop() {
-#pragma unroll
-    for (int i = 0; i < KT - 1; ++i) c[i] = c[i + 1];
-    c[KT - 1] = worst_cand();
-  }
-};
-
-__device__ __forceinline__ Cand shfl_xor_cand(const Cand& a, int off) {
-  Cand o;
-  o.key = __shfl_xor(a.key, off);
-  o.h = __shfl_xor(a.h, off);
-  o.pos = __shfl_xor(a.pos, off);
-  o.val = __shfl_xor(a.val, off);
-  return o;
-}
-
-// Block-wide K best of the per-thread lists, best first, into out[0..K).
-// blockDim.x must be a multiple of 64. Ends with a barrier.
-template <int KT>
-__device__ void block_topk(TopK<KT>& t, int K, Cand* out, Cand* red) {
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
-  for (int it = 0; it < K; ++it) {
-    Cand best = t.c[0];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      Cand o = shfl_xor_cand(best, off);
-      if (cand_better(o, best)) best = o;
-    }
-    if (lane == 0) red[wave] = best;
-    __syncthreads();
-    Cand w = red[0];
-    for (int i = 1; i < nwaves; ++i)
-      if (cand_better(red[i], w)) w = red[i];
-    if (tid == 0) out[it] = w;
-    if (t.c[0].pos == w.pos && t.c[0].pos != 0xFFFFFFFFu) t.pop();
-    __syncthreads();
-  }
-}
 
 // ---------------------------------------------------------------------------
 // TI spectra, once per run: rows r2c (two real rows per complex FFT)
@@ -205,6 +126,7 @@ __global__ void __launch_bounds__(kPencilThreads) k_pencil(ScanGeom g, FftPlan p
                                                            const NodeInfo* __restrict__ info,
                                                            const int* __restrict__ nbOff,
                                                            const float* __restrict__ nbVal,
+                                                           const uint32_t* __restrict__ ord,
                                                            float2* __restrict__ C) {
   extern __shared__ float2 smem[];
   const int LP = planP.L, nvec = 1 << lognv, tid = threadIdx.x, nthr = blockDim.x;
@@ -235,36 +157,41 @@ __global__ void __launch_bounds__(kPencilThreads) k_pencil(ScanGeom g, FftPlan p
     for (int a = 0; a < g.V; ++a) {
       for (int e = tid; e < elems; e += nthr) bufA[e] = make_float2(0.f, 0.f);
       __syncthreads();
-      // sparse scatter of the neighbour kernel column(s)
+      // the sparse neighbour kernel column(s): the neighbours sorted by
+      // their row along the pencil dimension (k_nb_sort); the head of each
+      // row's run sums the run in neighbour-list order (fixed order)
+      const uint32_t* op = ord + ni.nbStart;
       for (int t = tid; t < ni.nb * nvec; t += nthr) {
-        const int j = t >> lognv, v = t & (nvec - 1);
+        const int i = t >> lognv, v = t & (nvec - 1);
+        const int len = (int)(op[i] >> 16);
+        if (len == 0) continue;
         const int k0 = k0base + v;
         if (k0 >= g.h0) continue;
-        const int jj = ni.nbStart + j;
-        const float val = nbVal[(long long)jj * g.V + a];
-        if (val == 0.f) continue;
-        const int d0 = nbOff[jj * D + 0];
-        int t0 = (int)(((long long)k0 * d0) % f0);
-        if (t0 < 0) t0 += f0;
-        float2 ph = tw0g[t0];
-        ph.y = -ph.y;  // exp(+2 pi i k0 d0 / f0)
-        int dP;
-        if (D == 3) {
-          const int d1 = nbOff[jj * D + 1];
-          int t1 = (int)(((long long)k1 * d1) % f1);
-          if (t1 < 0) t1 += f1;
-          float2 p1 = tw1g[t1];
-          p1.y = -p1.y;
-          ph = cmul(ph, p1);
-          dP = nbOff[jj * D + 2];
-        } else {
-          dP = nbOff[jj * D + 1];
+        float2 acc = make_float2(0.f, 0.f);
+        int dP = 0;
+        for (int r = 0; r < len; ++r) {
+          const int jj = ni.nbStart + (int)(op[i + r] & 0xFFFFu);
+          const float val = nbVal[(long long)jj * g.V + a];
+          const int d0 = nbOff[jj * D + 0];
+          int t0 = (int)(((long long)k0 * d0) % f0);
+          if (t0 < 0) t0 += f0;
+          float2 ph = tw0g[t0];
+          ph.y = -ph.y;  // exp(+2 pi i k0 d0 / f0)
+          if (D == 3) {
+            const int d1 = nbOff[jj * D + 1];
+            int t1 = (int)(((long long)k1 * d1) % f1);
+            if (t1 < 0) t1 += f1;
+            float2 p1 = tw1g[t1];
+            p1.y = -p1.y;
+            ph = cmul(ph, p1);
+          }
+          dP = nbOff[jj * D + P];
+          acc.x += val * ph.x;
+          acc.y += val * ph.y;
         }
         int q = (-dP) % fP;
         if (q < 0) q += fP;
-        float* dst = reinterpret_cast<float*>(&bufA[(q << lognv) + v]);
-        atomicAdd(dst, val * ph.x);
-        atomicAdd(dst + 1, val * ph.y);
+        bufA[(q << lognv) + v] = acc;
       }
       __syncthreads();
       float2* X = stockham<-1>(bufA, bufB, twP, planP, lognv, tid, nthr);
@@ -302,6 +229,60 @@ __global__ void __launch_bounds__(kPencilThreads) k_pencil(ScanGeom g, FftPlan p
 }
 
 // ---------------------------------------------------------------------------
+// Neighbours of each node sorted by their row along the pencil dimension P
+// (key (-d_P) mod f_P, ties in list order), for the fixed-order column build
+// of k_pencil: ord[nbStart + i] = j | (run length << 16) at the head of each
+// run of equal keys (0 elsewhere). One 64-thread block per node.
+// ---------------------------------------------------------------------------
+constexpr int kSortLds = 2048;
+__global__ void __launch_bounds__(64) k_nb_sort(const NodeInfo* __restrict__ info, const int* __restrict__ nbOff,
+                                               int D, int fP, uint32_t* __restrict__ ord) {
+  __shared__ int key[kSortLds];
+  __shared__ int jAt[kSortLds];
+  const NodeInfo ni = info[blockIdx.x];
+  const int nb = ni.nb, lane = threadIdx.x;
+  if (nb <= 0) return;
+  auto keyOf = [&](int j) {
+    int q = (-nbOff[(long long)(ni.nbStart + j) * D + (D - 1)]) % fP;
+    return q < 0 ? q + fP : q;
+  };
+  const bool lds = nb <= kSortLds;
+  if (lds)
+    for (int j = lane; j < nb; j += 64) key[j] = keyOf(j);
+  __syncthreads();
+  uint32_t* out = ord + ni.nbStart;
+  for (int j = lane; j < nb; j += 64) {
+    const int kj = lds ? key[j] : keyOf(j);
+    int pos = 0;
+    for (int q = 0; q < nb; ++q) {
+      const int kq = lds ? key[q] : keyOf(q);
+      pos += (kq < kj || (kq == kj && q < j)) ? 1 : 0;
+    }
+    out[pos] = (uint32_t)j;
+    if (lds) jAt[pos] = j;
+  }
+  __syncthreads();
+  // run lengths at the heads
+  for (int i = lane; i < nb; i += 64) {
+    const int ji = lds ? jAt[i] : (int)(out[i] & 0xFFFFu);
+    const int ki = lds ? key[ji] : keyOf(ji);
+    bool head = true;
+    if (i > 0) {
+      const int jp = lds ? jAt[i - 1] : (int)(out[i - 1] & 0xFFFFu);
+      head = (lds ? key[jp] : keyOf(jp)) != ki;
+    }
+    if (!head) continue;
+    int len = 1;
+    while (i + len < nb) {
+      const int jn = lds ? jAt[i + len] : (int)(out[i + len] & 0xFFFFu);
+      if ((lds ? key[jn] : keyOf(jn)) != ki) break;
+      ++len;
+    }
+    out[i] = (uint32_t)ji | ((uint32_t)len << 16);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 1-D nodes: the spectrum directly, M(k0) = sum_A Yhat_A(k0) sum_j v_jA e^{+2pi i k0 d_j/f0}
 // ---------------------------------------------------------------------------
 __global__ void k_direct1d(ScanGeom g, const float2* __restrict__ Yhat, const float2* __restrict__ tw0g,
@@ -330,6 +311,53 @@ __global__ void k_direct1d(ScanGeom g, const float2* __restrict__ Yhat, const fl
     acc.y += y.x * x.y + y.y * x.x;
   }
   C[(long long)node * g.H + k0] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Rows of one rows block (valid rows q0 .. q0 + 2 nvec - 1 of the node, or
+// nvec rows in the cross-measurement form): Hermitian completion and one
+// two-for-one c2r along dim0 per row pair (Z = A + i B). smem: twiddles
+// (already loaded, first read after a barrier), then two L x nvec buffers.
+// Returns the LDS buffer holding the transform ([c0][v]). Every thread of
+// the block calls it (barriers inside).
+// ---------------------------------------------------------------------------
+__device__ float2* rows_block_c2r(const ScanGeom& g, const FftPlan& plan0, int lognv, const NodeInfo& ni,
+                                  const float2* __restrict__ Cn, const float2* __restrict__ Cx, bool cross,
+                                  long long q0, long long nrv, int n1v, float2* smem) {
+  const int L = plan0.L, nvec = 1 << lognv, tid = threadIdx.x, nthr = blockDim.x;
+  float2* tw = smem;
+  float2* a = tw + L;
+  float2* b = a + L * nvec;
+  const int h0 = g.h0, f1 = g.f[1];
+  for (int e = tid; e < L * nvec; e += nthr) {
+    const int k = e >> lognv, v = e & (nvec - 1);
+    const long long qa = q0 + (cross ? v : 2 * v), qb = cross ? qa : qa + 1;
+    const bool conjk = (k >= h0);
+    const int kk = conjk ? (L - k) : k;
+    float2 A = make_float2(0.f, 0.f), B = make_float2(0.f, 0.f);
+    if (qa < nrv) {
+      const int c1 = ni.lo[1] + (int)(qa % n1v);
+      const int c2 = ni.lo[2] + (int)(qa / n1v);
+      A = Cn[((long long)c2 * f1 + c1) * h0 + kk];
+    }
+    if (qb < nrv) {
+      const int c1 = ni.lo[1] + (int)(qb % n1v);
+      const int c2 = ni.lo[2] + (int)(qb / n1v);
+      B = Cx[((long long)c2 * f1 + c1) * h0 + kk];
+    }
+    // a Hermitian c2r ignores the imaginary part of the DC and Nyquist bins
+    if (kk == 0 || 2 * kk == L) {
+      A.y = 0.f;
+      B.y = 0.f;
+    }
+    if (conjk) {
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    a[e] = make_float2(A.x - B.y, A.y + B.x);
+  }
+  __syncthreads();
+  return stockham<+1>(a, b, tw, plan0, lognv, tid, nthr);
 }
 
 // ---------------------------------------------------------------------------
@@ -363,42 +391,10 @@ __global__ void __launch_bounds__(256) k_rows_select(ScanGeom g, FftPlan plan0, 
   const long long q0 = (long long)blockIdx.x * (cross ? 1 : 2) * nvec;
   if (q0 >= nrv) return;
   float2* tw = smem;
-  float2* a = tw + L;
-  float2* b = a + L * nvec;
   for (int i = tid; i < L; i += nthr) tw[i] = tw0g[i];
-  const int h0 = g.h0, f1 = g.f[1];
   const float2* Cn = C + (long long)node * g.H;
   const float2* Cx = cross ? C + (long long)(node + crossOff) * g.H : Cn;
-  // load Z = Aext + i Bext
-  for (int e = tid; e < L * nvec; e += nthr) {
-    const int k = e >> lognv, v = e & (nvec - 1);
-    const long long qa = q0 + (cross ? v : 2 * v), qb = cross ? qa : qa + 1;
-    const bool conjk = (k >= h0);
-    const int kk = conjk ? (L - k) : k;
-    float2 A = make_float2(0.f, 0.f), B = make_float2(0.f, 0.f);
-    if (qa < nrv) {
-      const int c1 = ni.lo[1] + (int)(qa % n1v);
-      const int c2 = ni.lo[2] + (int)(qa / n1v);
-      A = Cn[((long long)c2 * f1 + c1) * h0 + kk];
-    }
-    if (qb < nrv) {
-      const int c1 = ni.lo[1] + (int)(qb % n1v);
-      const int c2 = ni.lo[2] + (int)(qb / n1v);
-      B = Cx[((long long)c2 * f1 + c1) * h0 + kk];
-    }
-    // a Hermitian c2r ignores the imaginary part of the DC and Nyquist bins
-    if (kk == 0 || 2 * kk == L) {
-      A.y = 0.f;
-      B.y = 0.f;
-    }
-    if (conjk) {
-      A.y = -A.y;
-      B.y = -B.y;
-    }
-    a[e] = make_float2(A.x - B.y, A.y + B.x);
-  }
-  __syncthreads();
-  float2* z = stockham<+1>(a, b, tw, plan0, lognv, tid, nthr);
+  float2* z = rows_block_c2r(g, plan0, lognv, ni, Cn, Cx, cross, q0, nrv, n1v, smem);
   TopK<KT> tk;
   tk.init();
   const float scale = ni.scale, d0 = ni.delta0;
@@ -441,7 +437,78 @@ __global__ void __launch_bounds__(256) k_rows_select(ScanGeom g, FftPlan plan0, 
     }
   }
   if (cand) {
-    block_topk<KT>(tk, K, cand + ((long long)node * candBlocks + blockIdx.x) * K, red);
+    // KT candidates per block (the exact selection needs more than K to
+    // see whether a block holds further positions near the K-th best)
+    block_topk<KT>(tk, KT, cand + ((long long)node * candBlocks + blockIdx.x) * KT, red);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Exact selection, second pass on the generic path: for every queued
+// (node, rows block) the block's rows again (same transform and epilogue as
+// k_rows_select, never the cross-measurement form), appending every valid
+// position whose FP32 value is >= theta[node] to the node's band list.
+// Persistent grid over the device queue count.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_rows_collect(ScanGeom g, FftPlan plan0, int lognv,
+                                                      const float2* __restrict__ tw0g,
+                                                      const NodeInfo* __restrict__ info,
+                                                      const float2* __restrict__ C, const int2* __restrict__ queue,
+                                                      const int* __restrict__ qcnt, const float* __restrict__ theta,
+                                                      uint32_t* __restrict__ bandPos, int* __restrict__ bandCnt) {
+  extern __shared__ float2 smem[];
+  const int L = plan0.L, nvec = 1 << lognv, tid = threadIdx.x, nthr = blockDim.x, lane = tid & 63;
+  const int items = *qcnt;
+  if ((int)blockIdx.x >= items) return;
+  for (int i = tid; i < L; i += nthr) smem[i] = tw0g[i];  // first read after a barrier
+  const int s0 = g.s[0], s1 = g.s[1];
+  const int per = (L * nvec + nthr - 1) / nthr;
+  for (int it = blockIdx.x; it < items; it += gridDim.x) {
+    const int2 qi = queue[it];
+    const int node = qi.x;
+    const NodeInfo ni = info[node];
+    const int n1v = ni.hi[1] - ni.lo[1] + 1;
+    const int n2v = ni.hi[2] - ni.lo[2] + 1;
+    const int n0v = ni.hi[0] - ni.lo[0] + 1;
+    const long long nrv = (n1v > 0 && n2v > 0 && n0v > 0) ? (long long)n1v * n2v : 0;
+    const long long q0 = (long long)qi.y * 2 * nvec;
+    const float2* Cn = C + (long long)node * g.H;
+    float2* z = rows_block_c2r(g, plan0, lognv, ni, Cn, Cn, false, q0, nrv, n1v, smem);
+    const float th = theta[node];
+    const float scale = ni.scale, d0 = ni.delta0;
+    uint32_t* out = bandPos + (long long)node * kBandCap;
+    // every lane runs the same number of rounds (ballots below)
+    for (int r = 0; r < per; ++r) {
+      const int e = tid + r * nthr;
+#pragma unroll
+      for (int w = 0; w < 2; ++w) {
+        bool ok = false;
+        uint32_t pos = 0;
+        if (e < L * nvec) {
+          const int c0 = e >> lognv, v = e & (nvec - 1);
+          const long long q = q0 + 2 * v + w;
+          if (c0 >= ni.lo[0] && c0 <= ni.hi[0] && q < nrv) {
+            const int c1 = ni.lo[1] + (int)(q % n1v);
+            const int c2 = ni.lo[2] + (int)(q / n1v);
+            const float2 zz = z[e];
+            const float val = (w == 0 ? zz.x : zz.y) * scale + d0;
+            pos = (uint32_t)(c0 + s0 * (c1 + s1 * c2));
+            ok = val >= th && !(ni.exOn && excluded(ni, s0, s1, c0, c1, c2));
+          }
+        }
+        const unsigned long long m = __ballot(ok);
+        if (m == 0ull) continue;
+        const int leader = __ffsll((long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&bandCnt[node], __popcll(m));
+        base = __shfl(base, leader);
+        if (ok) {
+          const int idx = base + __popcll(m & ((1ull << lane) - 1ull));
+          if (idx < kBandCap) out[idx] = pos;
+        }
+      }
+    }
+    __syncthreads();  // the next item's transform reuses the LDS buffers
   }
 }
 
@@ -472,13 +539,14 @@ __global__ void __launch_bounds__(256) k_select_map(const float* __restrict__ ma
 // ---------------------------------------------------------------------------
 template <int KT>
 __global__ void __launch_bounds__(256) k_merge_draw(ScanGeom g, const NodeInfo* __restrict__ info,
-                                                    const Cand* __restrict__ cand, int candBlocks,
+                                                    const Cand* __restrict__ cand, int candBlocks, int M,
                                                     int rowsPerBlock, int K, float kf, float* __restrict__ outVal,
                                                     uint32_t* __restrict__ outPos, uint32_t* __restrict__ chosen,
                                                     float* __restrict__ simVals, const float* __restrict__ tiVals,
                                                     int nVar, uint32_t* __restrict__ indexOut) {
   __shared__ Cand red[16];
   __shared__ Cand best[kMaxK];
+  __shared__ uint32_t bpos[kMaxK];
   const int node = blockIdx.x;
   const NodeInfo ni = info[node];
   if (simVals && ni.nb == 0) return;  // drawn from the marginal by the gather stage
@@ -492,30 +560,16 @@ __global__ void __launch_bounds__(256) k_merge_draw(ScanGeom g, const NodeInfo* 
   }
   TopK<KT> tk;
   tk.init();
-  const Cand* cn = cand + (long long)node * candBlocks * K;
-  for (int i = threadIdx.x; i < nblk * K; i += blockDim.x) tk.push(cn[i]);
+  const Cand* cn = cand + (long long)node * candBlocks * M;
+  for (int i = threadIdx.x; i < nblk * M; i += blockDim.x) tk.push(cn[i]);
   block_topk<KT>(tk, K, best, red);
   if (threadIdx.x < K) {
     outVal[(long long)node * K + threadIdx.x] = best[threadIdx.x].val;
     outPos[(long long)node * K + threadIdx.x] = best[threadIdx.x].pos;
+    bpos[threadIdx.x] = best[threadIdx.x].pos;
   }
-  if (threadIdx.x == 0 && (chosen || simVals)) {
-    // QuantileSamplingModule::sample rank, one TI: floor(seed * k * (ceil(1/k)/1)),
-    // k from the node's k image value when given (include/simulation.hpp:355-357)
-    const float kk = ni.kLocal > 0.f ? ni.kLocal : kf;
-    int r = (int)floorf(ni.seed * kk * (ceilf(1.f / kk) / 1.f));
-    if (r > (int)ceilf(kk) - 1) r = (int)ceilf(kk) - 1;  // the node's list holds ceil(k)
-    if (r > K - 1) r = K - 1;
-    if (r < 0) r = 0;
-    const uint32_t p = best[r].pos;
-    if (chosen) chosen[node] = p;
-    // simulation write (include/simulation.hpp:658-666): the TI value of the
-    // chosen position and importDataIndex = index * nTI + TI (one TI)
-    if (simVals && p != 0xFFFFFFFFu) {
-      for (int v = 0; v < nVar; ++v) simVals[(long long)ni.cell * nVar + v] = tiVals[(long long)p * nVar + v];
-      indexOut[ni.cell] = p;
-    }
-  }
+  __syncthreads();
+  if (threadIdx.x == 0 && (chosen || simVals)) draw_write(ni, bpos, K, kf, chosen, node, simVals, tiVals, nVar, indexOut);
 }
 
 // Several TIs: per node, the TIs' K best lists (val/pos [t][node][K], TI
@@ -580,7 +634,7 @@ __global__ void k_merge_ti(const NodeInfo* __restrict__ info, int nN, int nTI, i
 // that TI's 1 / prod(FFT extent). Marginal nodes (nb == 0) keep their empty
 // box; circular TIs span the whole extent.
 __global__ void k_info_shift(const NodeInfo* __restrict__ in, NodeInfo* __restrict__ out, int nN, int dh0, int dh1,
-                             int dh2, float scale) {
+                             int dh2, float scale, float epsFactor) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nN) return;
   NodeInfo ni = in[q];
@@ -590,6 +644,7 @@ __global__ void k_info_shift(const NodeInfo* __restrict__ in, NodeInfo* __restri
     ni.hi[2] += dh2;
   }
   ni.scale = scale;
+  ni.eps *= epsFactor;
   out[q] = ni;
 }
 
@@ -623,8 +678,8 @@ template __global__ void k_colfft<+1>(float2*, FftPlan, const float2*, int, long
   template __global__ void k_rows_select<KT>(ScanGeom, FftPlan, int, const float2*, const NodeInfo*,              \
                                              const float2*, int, Cand*, int, float*, int, const uint8_t*);        \
   template __global__ void k_select_map<KT>(const float*, long long, int, float, uint32_t, Cand*);                \
-  template __global__ void k_merge_draw<KT>(ScanGeom, const NodeInfo*, const Cand*, int, int, int, float, float*, \
-                                            uint32_t*, uint32_t*, float*, const float*, int, uint32_t*);
+  template __global__ void k_merge_draw<KT>(ScanGeom, const NodeInfo*, const Cand*, int, int, int, int, float,     \
+                                            float*, uint32_t*, uint32_t*, float*, const float*, int, uint32_t*);
 QSMI_INST_K(2)
 QSMI_INST_K(4)
 QSMI_INST_K(8)
@@ -649,7 +704,7 @@ hipError_t set_lds_limits(int device, size_t* limitOut) {
   *limitOut = (size_t)lim;
   const void* fns[] = {(const void*)k_pencil,         (const void*)k_r2c_rows,        (const void*)k_colfft<-1>,
                        (const void*)k_colfft<+1>,     (const void*)k_rows_select<2>,  (const void*)k_rows_select<4>,
-                       (const void*)k_rows_select<8>, (const void*)k_rows_select<16>};
+                       (const void*)k_rows_select<8>, (const void*)k_rows_select<16>, (const void*)k_rows_collect};
   for (const void* f : fns) {
     e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
     if (e != hipSuccess) return e;
@@ -682,12 +737,13 @@ void launch_colfft(int dir, float2* base, const FftPlan& plan, const float2* tw,
 
 void launch_pencil(const ScanGeom& g, const FftPlan& planP, int lognv, const float2* Yhat, const float2* tw0,
                    const float2* tw1, const float2* twP, int nNodes, int npb, const NodeInfo* info, const int* nbOff,
-                   const float* nbVal, float2* C, hipStream_t st) {
+                   const float* nbVal, uint32_t* ord, float2* C, hipStream_t st) {
   const int nvec = 1 << lognv;
   const int nblk0 = (g.h0 + nvec - 1) / nvec;
+  hipLaunchKernelGGL(k_nb_sort, dim3((unsigned)nNodes), dim3(64), 0, st, info, nbOff, g.D, g.f[g.D - 1], ord);
   dim3 grid((unsigned)(nblk0 * (g.D == 3 ? g.f[1] : 1)), (unsigned)((nNodes + npb - 1) / npb));
   hipLaunchKernelGGL(k_pencil, grid, dim3(kPencilThreads), fft_lds_bytes(planP.L, nvec), st, g, planP, lognv, nblk0,
-                     Yhat, tw0, tw1, twP, nNodes, npb, info, nbOff, nbVal, C);
+                     Yhat, tw0, tw1, twP, nNodes, npb, info, nbOff, nbVal, ord, C);
 }
 
 void launch_direct1d(const ScanGeom& g, const float2* Yhat, const float2* tw0, int nNodes, const NodeInfo* info,
@@ -698,6 +754,9 @@ void launch_direct1d(const ScanGeom& g, const float2* Yhat, const float2* tw0, i
 
 static int kt_for(int K) { return K <= 2 ? 2 : K <= 4 ? 4 : K <= 8 ? 8 : 16; }
 
+// candidates per rows block / wavefront group for ceil(k) = K
+int cands_per_group(int K) { return kt_for(K + 1); }
+
 void launch_rows_select(const ScanGeom& g, const FftPlan& plan0, int lognv, const float2* tw0, const NodeInfo* info,
                         const float2* C, int nNodes, int rowBlocks, int K, Cand* cand, float* map, hipStream_t st,
                         int crossOff, const uint8_t* centreOk) {
@@ -707,13 +766,22 @@ void launch_rows_select(const ScanGeom& g, const FftPlan& plan0, int lognv, cons
 #define QSMI_ROWS(KT)                                                                                        \
   hipLaunchKernelGGL(k_rows_select<KT>, grid, dim3(256), lds, st, g, plan0, lognv, tw0, info, C, K, cand, rowBlocks, \
                      map, crossOff, centreOk)
-  switch (kt_for(K)) {
+  switch (cands_per_group(K)) {
     case 2: QSMI_ROWS(2); break;
     case 4: QSMI_ROWS(4); break;
     case 8: QSMI_ROWS(8); break;
     default: QSMI_ROWS(16); break;
   }
 #undef QSMI_ROWS
+}
+
+void launch_rows_collect(const ScanGeom& g, const FftPlan& plan0, int lognv, const float2* tw0, const NodeInfo* info,
+                         const float2* C, const int2* queue, const int* qcnt, const float* theta, uint32_t* bandPos,
+                         int* bandCnt, int maxItems, hipStream_t st) {
+  const int nvec = 1 << lognv;
+  const int blocks = std::max(1, std::min(maxItems, 2048));
+  hipLaunchKernelGGL(k_rows_collect, dim3((unsigned)blocks), dim3(256), fft_lds_bytes(plan0.L, nvec), st, g, plan0,
+                     lognv, tw0, info, C, queue, qcnt, theta, bandPos, bandCnt);
 }
 
 void launch_select_map(const float* map, long long S, int K, float quantum, uint32_t seedbits, Cand* cand, int chunks,
@@ -727,11 +795,11 @@ void launch_select_map(const float* map, long long S, int K, float quantum, uint
 }
 
 void launch_merge_draw(const ScanGeom& g, const NodeInfo* info, const Cand* cand, int nNodes, int candBlocks,
-                       int rowsPerBlock, int K, float kf, float* outVal, uint32_t* outPos, uint32_t* chosen,
+                       int M, int rowsPerBlock, int K, float kf, float* outVal, uint32_t* outPos, uint32_t* chosen,
                        float* simVals, const float* tiVals, int nVar, uint32_t* indexOut, hipStream_t st) {
-#define QSMI_MERGE(KT)                                                                                              \
-  hipLaunchKernelGGL(k_merge_draw<KT>, dim3(nNodes), dim3(256), 0, st, g, info, cand, candBlocks, rowsPerBlock, K, \
-                     kf, outVal, outPos, chosen, simVals, tiVals, nVar, indexOut)
+#define QSMI_MERGE(KT)                                                                                               \
+  hipLaunchKernelGGL(k_merge_draw<KT>, dim3(nNodes), dim3(256), 0, st, g, info, cand, candBlocks, M, rowsPerBlock, \
+                     K, kf, outVal, outPos, chosen, simVals, tiVals, nVar, indexOut)
   switch (kt_for(K)) {
     case 2: QSMI_MERGE(2); break;
     case 4: QSMI_MERGE(4); break;
@@ -766,9 +834,9 @@ void launch_merge_ti(const NodeInfo* info, int nN, int nTI, int K, float kf, con
 }
 
 void launch_info_shift(const NodeInfo* in, NodeInfo* out, int nN, int dh0, int dh1, int dh2, float scale,
-                       hipStream_t st) {
+                       float epsFactor, hipStream_t st) {
   hipLaunchKernelGGL(k_info_shift, dim3((unsigned)((nN + 63) / 64)), dim3(64), 0, st, in, out, nN, dh0, dh1, dh2,
-                     scale);
+                     scale, epsFactor);
 }
 
 void launch_fill(float* p, long long n, float v, hipStream_t st) {

@@ -29,10 +29,12 @@ struct ScanGeom {
 // 1000), ordered by (row mod 10, row), for the sparse column transform.
 // T = 0 when the node has more than kTapMax distinct rows (full FFT used).
 constexpr int kTapMax = 48;
+constexpr int kTapNbMax = 500;  // neighbours of a node on the tap path (slots of one 8 KiB LDS column)
 struct TapInfo {
   int T;
-  int cls[11];        // rows of class c: [cls[c], cls[c+1])
+  int cls[11];              // rows of class c: [cls[c], cls[c+1])
   int row[kTapMax];
+  int start[kTapMax + 1];   // tap t's neighbours: slots [start[t], start[t+1]) in tap order
 };
 
 // Per path node of a wave. Valid TI centres: lo[i] <= c_i <= hi[i].
@@ -55,6 +57,9 @@ struct NodeInfo {
   float exR2;         // squared radius: positions p with sum_d (p_d - o_d)^2 <= exR2
   float kLocal;       // this node's k from a k image (0: the run's k)
   int tiReq;          // several TIs: the only TI this node samples (-1: all; a TI-index image)
+  float eps;          // bound of the FP32 map error of this node, in map units; > 0 switches the
+                      // exact selection on (every position within 2 eps of the K-th best FP32
+                      // value is rescored in FP64 before the device rule picks the K best)
 };
 
 // Several TIs: the first cell of each TI in their concatenation, then the total.
@@ -97,5 +102,32 @@ __host__ __device__ __forceinline__ uint32_t fmix32(uint32_t h) {
 __host__ __device__ __forceinline__ uint32_t tie_hash(uint32_t seedbits, uint32_t pos) {
   return fmix32(pos * 0x9E3779B1u + seedbits);
 }
+
+// Candidate of the exact selection: FP64 value, the same device rule
+// (value desc, hash desc, position asc).
+struct Cand64 {
+  double val;
+  uint32_t h;
+  uint32_t pos;
+};
+
+__device__ __forceinline__ bool cand64_better(const Cand64& a, const Cand64& b) {
+  if (a.val != b.val) return a.val > b.val;
+  if (a.h != b.h) return a.h > b.h;
+  return a.pos < b.pos;
+}
+
+// FP32 map error bound per node: eps = kMapErrC * (scale * N) *
+// sum_a rms(Y_a) * sum_j |v_ja|, with rms(Y_a) the root mean square of TI
+// sub-image a over the FFT grid (zero padding included). The FFT rounding
+// error of a correlation is proportional to the norms of its two operands;
+// the constant holds the measured worst case with a margin.
+constexpr float kMapErrC = 4e-5f;
+
+// Exact selection work space per wave node: positions to rescore.
+constexpr int kBandCap = 2048;
+// rows blocks per node the exact selection tracks (larger grids keep the
+// FP32 selection)
+constexpr int kMaxBandBlocks = 8192;
 
 }  // namespace qsmi

@@ -10,6 +10,86 @@
 namespace qsmi {
 
 // ---------------------------------------------------------------------------
+// top-K of the device rule
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ Cand worst_cand() {
+  Cand c;
+  c.key = -INFINITY;
+  c.h = 0u;
+  c.pos = 0xFFFFFFFFu;
+  c.val = -INFINITY;
+  return c;
+}
+
+template <int KT>
+struct TopK {
+  Cand c[KT];
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int i = 0; i < KT; ++i) c[i] = worst_cand();
+  }
+  __device__ __forceinline__ void push(Cand cur) {
+    if (!cand_better(cur, c[KT - 1])) return;
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+      if (cand_better(cur, c[i])) {
+        Cand t = c[i];
+        c[i] = cur;
+        cur = t;
+      }
+    }
+  }
+  // value fast path: the hash is only computed when the key can enter
+  __device__ __forceinline__ void push_value(float val, float quantum, uint32_t pos, uint32_t seedbits) {
+    float key = (quantum > 0.f) ? rintf(val * (1.f / quantum)) : val;
+    if (!(key >= c[KT - 1].key)) return;  // also rejects NaN
+    Cand cd;
+    cd.key = key;
+    cd.val = val;
+    cd.pos = pos;
+    cd.h = tie_hash(seedbits, pos);
+    push(cd);
+  }
+  __device__ __forceinline__ void pop() {
+#pragma unroll
+    for (int i = 0; i < KT - 1; ++i) c[i] = c[i + 1];
+    c[KT - 1] = worst_cand();
+  }
+};
+
+__device__ __forceinline__ Cand shfl_xor_cand(const Cand& a, int off) {
+  Cand o;
+  o.key = __shfl_xor(a.key, off);
+  o.h = __shfl_xor(a.h, off);
+  o.pos = __shfl_xor(a.pos, off);
+  o.val = __shfl_xor(a.val, off);
+  return o;
+}
+
+// Block-wide K best of the per-thread lists, best first, into out[0..K).
+// blockDim.x must be a multiple of 64. Ends with a barrier.
+template <int KT>
+__device__ void block_topk(TopK<KT>& t, int K, Cand* out, Cand* red) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+  for (int it = 0; it < K; ++it) {
+    Cand best = t.c[0];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      Cand o = shfl_xor_cand(best, off);
+      if (cand_better(o, best)) best = o;
+    }
+    if (lane == 0) red[wave] = best;
+    __syncthreads();
+    Cand w = red[0];
+    for (int i = 1; i < nwaves; ++i)
+      if (cand_better(red[i], w)) w = red[i];
+    if (tid == 0) out[it] = w;
+    if (t.c[0].pos == w.pos && t.c[0].pos != 0xFFFFFFFFu) t.pop();
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Block top-K of the device rule in two passes (blockDim a multiple of 64,
 // at most 8 wavefronts):
 //  1. every thread keeps its KT largest keys (floats only); the block finds
@@ -237,6 +317,96 @@ __device__ void wave_extract(RankList<KT>& rl, int K, Cand* out) {
       }
       out[it] = w;
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Top-K of the exact selection (FP64 values, device rule).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ Cand64 worst_cand64() {
+  Cand64 c;
+  c.val = -INFINITY;
+  c.h = 0u;
+  c.pos = 0xFFFFFFFFu;
+  return c;
+}
+
+template <int KT>
+struct TopK64 {
+  Cand64 c[KT];
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int i = 0; i < KT; ++i) c[i] = worst_cand64();
+  }
+  __device__ __forceinline__ void push(Cand64 cur) {
+    if (!cand64_better(cur, c[KT - 1])) return;
+#pragma unroll
+    for (int i = 0; i < KT; ++i) {
+      if (cand64_better(cur, c[i])) {
+        Cand64 t = c[i];
+        c[i] = cur;
+        cur = t;
+      }
+    }
+  }
+  __device__ __forceinline__ void pop() {
+#pragma unroll
+    for (int i = 0; i < KT - 1; ++i) c[i] = c[i + 1];
+    c[KT - 1] = worst_cand64();
+  }
+};
+
+__device__ __forceinline__ Cand64 shfl_xor_cand64(const Cand64& a, int off) {
+  Cand64 o;
+  o.val = __shfl_xor(a.val, off);
+  o.h = __shfl_xor(a.h, off);
+  o.pos = __shfl_xor(a.pos, off);
+  return o;
+}
+
+// Block-wide K best, best first, into out[0..K) (LDS or global); red holds
+// one entry per wavefront. blockDim.x a multiple of 64. Ends with a barrier.
+template <int KT>
+__device__ void block_topk64(TopK64<KT>& t, int K, Cand64* out, Cand64* red) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+  for (int it = 0; it < K; ++it) {
+    Cand64 best = t.c[0];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      Cand64 o = shfl_xor_cand64(best, off);
+      if (cand64_better(o, best)) best = o;
+    }
+    if (lane == 0) red[wave] = best;
+    __syncthreads();
+    Cand64 w = red[0];
+    for (int i = 1; i < nwaves; ++i)
+      if (cand64_better(red[i], w)) w = red[i];
+    if (tid == 0) out[it] = w;
+    if (t.c[0].pos == w.pos && t.c[0].pos != 0xFFFFFFFFu) t.pop();
+    __syncthreads();
+  }
+}
+
+// QuantileSamplingModule::sample's rank for one TI (include/
+// quantileSamplingModule.hpp:101-113): floor(seed * k * (ceil(1/k)/1)), k
+// from the node's k image when given (include/simulation.hpp:355-357),
+// clamped to the node's list; then the simulation write (include/
+// simulation.hpp:658-666): the TI value at the chosen position and
+// importDataIndex = index * nTI + TI (one TI). pos: the node's K best,
+// best first (0xFFFFFFFF: none).
+__device__ __forceinline__ void draw_write(const NodeInfo& ni, const uint32_t* pos, int K, float kf,
+                                           uint32_t* chosen, int node, float* simVals, const float* tiVals,
+                                           int nVar, uint32_t* indexOut) {
+  const float kk = ni.kLocal > 0.f ? ni.kLocal : kf;
+  int r = (int)floorf(ni.seed * kk * (ceilf(1.f / kk) / 1.f));
+  if (r > (int)ceilf(kk) - 1) r = (int)ceilf(kk) - 1;  // the node's list holds ceil(k)
+  if (r > K - 1) r = K - 1;
+  if (r < 0) r = 0;
+  const uint32_t p = pos[r];
+  if (chosen) chosen[node] = p;
+  if (simVals && p != 0xFFFFFFFFu) {
+    for (int v = 0; v < nVar; ++v) simVals[(long long)ni.cell * nVar + v] = tiVals[(long long)p * nVar + v];
+    indexOut[ni.cell] = p;
   }
 }
 

@@ -47,6 +47,8 @@ struct GatherParams {
   int circSim;          // 1: the simulation grid wraps around (-csim)
   int kdK[8][3];        // per kernel of a kernel-index image: its grid (kernel 0: kd)
   long long kerOff[8];  // its offset in the kernel array (floats)
+  float rms[16];        // per active sub-variable: rms of its TI sub-image (FFT grid)
+  float epsScale;       // kMapErrC * scale * N; 0: exact selection off (tie quantum, NaN TI)
 };
 
 __device__ __forceinline__ uint32_t seed_bits_dev(float seed) {
@@ -96,6 +98,7 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
   ni.exR2 = 0.f;
   ni.kLocal = pathK ? pathK[i] : 0.f;
   ni.tiReq = nodeTi ? nodeTi[i] : -1;
+  ni.eps = 0.f;
   if (cnt <= 1) {
     // only the cell itself: draw from the marginal (simulation.hpp:586-606)
     if (lane == 0) {
@@ -134,6 +137,7 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
   const int* idx = nbIdx + (long long)i * P.n;
   int mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
   float wmin = INFINITY;  // smallest weight of a neighbour value (cross form)
+  float eAcc = 0.f;       // sum_a rms_a |v_ja| over this lane's neighbours (map error bound)
   for (int j = lane; j < cnt; j += 64) {
     const int* dd = offs + (long long)idx[j] * D;
     long long t = 0, mul = 1;
@@ -180,6 +184,7 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
           acc += al * u;
         }
         nbVal[row * P.V + a] = acc;
+        if (layer == 0) eAcc += P.rms[a] * fabsf(acc);
       }
     }
   }
@@ -193,6 +198,8 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) wmin = fminf(wmin, __shfl_xor(wmin, off));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) eAcc += __shfl_xor(eAcc, off);
   // delta0 in the reference's order: variable by variable, neighbour by
   // neighbour, float accumulation of -w x^2 read at expanded index v. The
   // terms are formed in parallel (one lane per neighbour) and summed by lane
@@ -253,6 +260,7 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
     }
     ni.nb = cnt;
     ni.delta0 = (P.trueMismatch && !P.cross) ? delta0 : 0.f;
+    ni.eps = P.epsScale * eAcc;
     if (P.noVerbatim) {
       // include/simulation.hpp:567-580: the TI position the nearest
       // neighbour was copied from, moved back by its offset (wrapped, its
@@ -437,6 +445,10 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   P.circular = dev->circular;
   P.trueMismatch = dev->trueMismatch;
   P.scale = dev->trueMismatch ? (float)(1.0 / (double)t->N) : 1.f;
+  for (int a = 0; a < P.V; ++a) P.rms[a] = t->rms[dev->active[a]];
+  P.epsScale = (dev->quantum == 0.f && !dev->cross)
+                   ? (float)((double)kMapErrC * (double)P.scale * (double)t->N)
+                   : 0.f;
   P.quantum = dev->quantum;
   P.tiCells = (unsigned)t->S;
   const int nTI = 1 + (int)R.extraDevs.size();
@@ -544,13 +556,17 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
       const bool compact = t->fast1k;
       const NodeInfo* infoEx = compact ? lp_info_compact(R.lp) : lp_info_ex(R.lp);
       const int* exCount = compact ? lp_compact_count(R.lp) : nullptr;
-      if ((rc = launch_spectrum(dev, g, nN, infoEx, b->dOff, b->dVal, b->dC, exCount))) return rc;
+      if ((rc = launch_spectrum(dev, g, nN, infoEx, b->dOff, b->dVal, b->dOrd, b->dC, exCount))) return rc;
       if (ev) (void)hipEventRecord(ev[2], st);
       launch_rows(dev, g, infoEx, b->dC, nN, rb, K, b->dCand, nullptr, exCount);
-      lp_combine(R.lp, nN, compact ? b->dInfo : infoEx, b->dCand, compact, rb, rpb, K, R.lpAudit, st);
+      // the scanned nodes' K best by the exact selection, as candidate lists
+      if ((rc = band_select(dev, g, infoEx, b->dCand, nN, K, R.k, b->dC, b->dOff, b->dVal, R.n, b->dOutVal,
+                            b->dOutPos, b->dCandEx, nullptr, nullptr, nullptr, 0, nullptr, &b->band, exCount)))
+        return rc;
+      lp_combine(R.lp, nN, compact ? b->dInfo : infoEx, b->dCandEx, compact, 1, 0, K, R.lpAudit, st);
       if (ev) (void)hipEventRecord(ev[3], st);
-      launch_merge_draw(g, b->dInfo, lp_cand(R.lp), nN, 1, 0, K, R.k, b->dOutVal, b->dOutPos, nullptr, R.dDi, R.dTi,
-                        R.nVar, R.dIndex, st);
+      launch_merge_draw(g, b->dInfo, lp_cand(R.lp), nN, 1, K, 0, K, R.k, b->dOutVal, b->dOutPos, nullptr, R.dDi,
+                        R.dTi, R.nVar, R.dIndex, st);
     } else if (nTI > 1) {
       // every TI's scan and K best, then the merge across TIs and the draw
       for (int tt = 0; tt < nTI; ++tt) {
@@ -559,33 +575,44 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
         const ScanGeom gt = geom_of(dv);
         // the node boxes on this TI's extent
         const NodeInfo* it = b->dInfo;
-        if (tt > 0 && (dv->ti->s[0] != t->s[0] || dv->ti->s[1] != t->s[1] || dv->ti->s[2] != t->s[2])) {
+        if (tt > 0) {
+          // this TI's boxes, map scale and error bound: the bound scales
+          // with the TI's sub-image norms (largest ratio over the active
+          // sub-variables) and with scale * N
           const float sc = dv->trueMismatch ? (float)(1.0 / (double)dv->ti->N) : 1.f;
+          double ratio = 0.0;
+          for (int A : dev->active)
+            ratio = std::max(ratio, t->rms[A] > 0.f ? (double)dv->ti->rms[A] / (double)t->rms[A] : 1e30);
+          if (!dv->trueMismatch) ratio *= (double)dv->ti->N / (double)t->N;
           launch_info_shift(b->dInfo, xb->dInfo, nN, dv->ti->s[0] - t->s[0], dv->ti->s[1] - t->s[1],
-                            dv->ti->s[2] - t->s[2], sc, st);
+                            dv->ti->s[2] - t->s[2], sc, (float)std::min(ratio, 1e30), st);
           it = xb->dInfo;
         }
         const int rbt = tt == 0 ? rb : row_blocks(dv->ti);
-        const int rpbt = tt == 0 ? rpb : rows_per_block(dv->ti);
-        if ((rc = launch_spectrum(dv, gt, nN, it, b->dOff, b->dVal, xb->dC))) {
+        if ((rc = launch_spectrum(dv, gt, nN, it, b->dOff, b->dVal, xb->dOrd, xb->dC))) {
           restore();
           return rc;
         }
         launch_rows(dv, gt, it, xb->dC, nN, rbt, K, xb->dCand, nullptr, nullptr, 0);
-        launch_merge_draw(gt, it, xb->dCand, nN, rbt, rpbt, K, R.k, dValAll + tt * tiStride,
-                          dPosAll + tt * tiStride, nullptr, nullptr, nullptr, R.nVar, nullptr, st);
+        if ((rc = band_select(dv, gt, it, xb->dCand, nN, K, R.k, xb->dC, b->dOff, b->dVal, R.n,
+                              dValAll + tt * tiStride, dPosAll + tt * tiStride, nullptr, nullptr, nullptr, nullptr,
+                              R.nVar, nullptr, &xb->band))) {
+          restore();
+          return rc;
+        }
       }
       if (ev) (void)hipEventRecord(ev[2], st);
       if (ev) (void)hipEventRecord(ev[3], st);
       launch_merge_ti(b->dInfo, nN, nTI, K, R.k, dValAll, dPosAll, tiStride, dev->quantum, R.dDi, R.dTi, R.nVar,
                       cum, R.dIndex, st);
     } else {
-      if ((rc = launch_spectrum(dev, g, layers * nN, b->dInfo, b->dOff, b->dVal, b->dC))) return rc;
+      if ((rc = launch_spectrum(dev, g, layers * nN, b->dInfo, b->dOff, b->dVal, b->dOrd, b->dC))) return rc;
       if (ev) (void)hipEventRecord(ev[2], st);
       launch_rows(dev, g, b->dInfo, b->dC, nN, rb, K, b->dCand, nullptr, nullptr, dev->cross ? nN : 0);
       if (ev) (void)hipEventRecord(ev[3], st);
-      launch_merge_draw(g, b->dInfo, b->dCand, nN, rb, rpb, K, R.k, b->dOutVal, b->dOutPos, nullptr, R.dDi, R.dTi,
-                        R.nVar, R.dIndex, st);
+      if ((rc = band_select(dev, g, b->dInfo, b->dCand, nN, K, R.k, b->dC, b->dOff, b->dVal, R.n, b->dOutVal,
+                            b->dOutPos, nullptr, nullptr, R.dDi, R.dTi, R.nVar, R.dIndex, &b->band)))
+        return rc;
     }
     if (R.recCap > 0)
       lp_record(nN, w0, R.dOrder, b->dInfo, b->dOff, R.dDi, R.nVar, R.sd[0], b->dOutPos, K, R.recCap, R.dRecOff,

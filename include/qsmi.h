@@ -63,6 +63,22 @@ unsigned qsmi_fft_size(unsigned src);
  * Every FFT extent must be <= 4096. The spectra stay in HBM until destroy. */
 int qsmi_ti_create(int gpu, int ndim, const unsigned* srcSize, const unsigned* fftSize, int nSub,
                    const float* subImages, qsmi_ti** out);
+/* The sub-images in the layout G2S's SharedMemoryManager hands to
+ * AcceleratorDevice::allocAndInitSharedMemory (src/sharedMemoryManager.cpp:
+ * 103-108, fed by src/qs.cpp:1544-1550 from DataImage::convertInput4Xcorr,
+ * include/DataImage.hpp:538-599): one host pointer per sub-image, each
+ * prod(fftSize) floats, point-reflected on the FFT grid (TI cell c at
+ * prod(fftSize) - 1 - flat_fft(c), zero elsewhere). Writes the unreflected
+ * TI-grid images qsmi_ti_create takes (nSub x prod(srcSize)). Returns
+ * QSMI_ERR_ARG when an image holds a non-zero value outside the reflected
+ * TI box (not this layout). Host only. */
+int qsmi_subimages_from_reflected(int ndim, const unsigned* srcSize, const unsigned* fftSize, int nSub,
+                                  const float* const* reflected, float* out);
+/* qsmi_ti_create from that layout: what an AcceleratorDevice subclass's
+ * allocAndInitSharedMemory calls with the arguments it receives
+ * (include/AcceleratorDevice.hpp:46). fftSize is required. */
+int qsmi_ti_create_from_reflected(int gpu, int ndim, const unsigned* srcSize, const unsigned* fftSize, int nSub,
+                                  const float* const* reflected, qsmi_ti** out);
 /* Replaces AcceleratorDevice::freeSharedMemory (include/AcceleratorDevice.hpp:47). */
 int qsmi_ti_destroy(qsmi_ti* ti);
 
@@ -100,7 +116,15 @@ int qsmi_candidate_for_pattern(qsmi_device* dev, int nNeighbors, const int* offs
 
 /* Replaces AcceleratorDevice::searchKBiggest (include/AcceleratorDevice.hpp:68):
  * the extendK best entries of the current map, best first, as values and
- * encoded indices (reference index space). extendK <= 16. */
+ * encoded indices (reference index space). extendK <= 16.
+ * Selection: the device rule (value desc, then a seeded hash of the position,
+ * see qsmi_set_tie_quantum). Without a tie quantum it is exact: every map
+ * position whose FP32 value lies within 2 eps of the extendK-th best
+ * (eps = 4e-5 * sum_a rms(Y_a) * sum_j |v_ja|, the bound of the FP32 map
+ * error: rms over the FFT grid of TI sub-image a, v_ja the neighbours'
+ * alpha-combined values) is rescored in FP64 and the extendK best of those
+ * values are returned (as floats). After a finite value was set with
+ * qsmi_set_value_in_error_array(_with_radius) the FP32 map is used as is. */
 int qsmi_search_k_biggest(qsmi_device* dev, float* errors, unsigned* encodedPosition, unsigned extendK, float seed);
 
 /* cvtIndexToPosition / cvtPositionToIndex (include/AcceleratorDevice.hpp:57-58),

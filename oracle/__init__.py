@@ -96,6 +96,8 @@ def lib():
     _sig(L, "orc_seed_from_unit_float", C.c_uint, [C.c_float])
     _sig(L, "orc_tie_hash", C.c_uint32, [C.c_uint32, C.c_uint32])
     _sig(L, "orc_select_device", C.c_int, [C.c_int, u32p, f32p, i32p, i32p, C.c_int, C.c_float, C.c_float, f32p, u32p])
+    _sig(L, "orc_select_device_f64", C.c_int,
+         [C.c_int, u32p, f64p, i32p, i32p, C.c_int, C.c_float, C.c_float, f64p, u32p])
     _sig(L, "orc_draw_rank", C.c_uint, [C.c_float, C.c_float, C.c_uint])
     _sig(L, "orc_index_to_position", C.c_uint, [C.c_int, u32p, u32p, i32p, C.c_uint])
     _sig(L, "orc_position_to_index", C.c_uint, [C.c_int, u32p, u32p, i32p, C.c_uint])
@@ -118,6 +120,7 @@ def lib():
     _sig(L, "orc_sim_node", C.c_uint, [C.c_void_p, f32p, C.c_uint, C.c_int, f32p, u32p, C.POINTER(C.c_int)])
     _sig(L, "orc_sim_run", None, [C.c_void_p, f32p, u32p])
     _sig(L, "orc_sim_node_map", C.c_int, [C.c_void_p, f32p, C.c_uint, f64p, C.c_void_p])
+    _sig(L, "orc_sim_node_norm", C.c_double, [C.c_void_p, f32p, C.c_uint])
     _sig(L, "orc_sim_set_index", None, [C.c_void_p, C.c_void_p])
     _sig(L, "orc_calib_offsets", C.c_int, [C.c_int, u32p, i32p, C.c_int])
     _sig(L, "orc_exclude_disk", None, [C.c_int, u32p, f32p, C.c_uint, C.c_float])
@@ -259,6 +262,16 @@ def select_device(dims, m, lo, hi, K, seed, quantum=0.0):
     return outv, outp, n
 
 
+def select_device_f64(dims, m, lo, hi, K, seed, quantum=0.0):
+    """The device rule on an FP64 map (the exact selection's result)."""
+    dims = u32(dims)
+    m = np.ascontiguousarray(m, np.float64)
+    outv = np.zeros(K, np.float64)
+    outp = np.zeros(K, np.uint32)
+    n = lib().orc_select_device_f64(len(dims), dims, m, i32(lo), i32(hi), K, seed, quantum, outv, outp)
+    return outv, outp, n
+
+
 class Sim:
     """Serial QS simulation / per-node lockstep checker over the oracle."""
 
@@ -352,6 +365,10 @@ class Sim:
         sc = np.zeros_like(m) if with_scale else None
         nb = lib().orc_sim_node_map(self.h, f32(di).reshape(-1), i, m, None if sc is None else sc.ctypes.data)
         return m, sc, nb
+
+    def node_norm(self, di, i):
+        """sum_A rms(Y_A) sum_j |v_jA| of node i: the scale of its FP32 map error."""
+        return float(lib().orc_sim_node_norm(self.h, f32(di).reshape(-1), i))
 
     def run(self, di):
         di = f32(di).reshape(-1).copy()

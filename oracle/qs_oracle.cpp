@@ -508,6 +508,56 @@ int orc_select_device(int D, const unsigned* s, const float* map, const int* lo,
   return (int)best.size();
 }
 
+// The device rule on an FP64 map (keys = the FP64 values, or rint(value /
+// quantum) with a tie quantum): the selection the library's exact selection
+// makes. Same conventions as orc_select_device.
+int orc_select_device_f64(int D, const unsigned* s, const double* map, const int* lo, const int* hi, int K,
+                          float seed, float quantum, double* outVal, unsigned* outPos) {
+  uint32_t sb = seed_from_unit_float(seed);
+  struct Cd {
+    double key;
+    uint32_t h, pos;
+    double val;
+  };
+  auto better = [](const Cd& a, const Cd& b) {
+    if (a.key != b.key) return a.key > b.key;
+    if (a.h != b.h) return a.h > b.h;
+    return a.pos < b.pos;
+  };
+  std::vector<Cd> best;
+  size_t S = 1;
+  for (int i = 0; i < D; ++i) S *= s[i];
+  for (size_t c = 0; c < S; ++c) {
+    size_t rem = c;
+    bool ok = true;
+    for (int i = 0; i < D; ++i) {
+      int ci = int(rem % s[i]);
+      rem /= s[i];
+      ok &= (ci >= lo[i] && ci <= hi[i]);
+    }
+    const double v = map[c];
+    if (!ok || !std::isfinite(v)) continue;
+    Cd cd{quantum > 0.f ? std::rint(v / (double)quantum) : v, orc_tie_hash(sb, (uint32_t)c), (uint32_t)c, v};
+    if ((int)best.size() < K) {
+      best.push_back(cd);
+      std::sort(best.begin(), best.end(), better);
+    } else if (better(cd, best.back())) {
+      best.back() = cd;
+      std::sort(best.begin(), best.end(), better);
+    }
+  }
+  for (int i = 0; i < K; ++i) {
+    if (i < (int)best.size()) {
+      outVal[i] = best[i].val;
+      outPos[i] = best[i].pos;
+    } else {
+      outVal[i] = -INFINITY;
+      outPos[i] = UINT_MAX;
+    }
+  }
+  return (int)best.size();
+}
+
 // ---------------------------------------------------------------------------
 // a11. Candidate draw — include/quantileSamplingModule.hpp:101-113:
 // rank = floor(seed * k * (ceil(nTI/k)/nTI)), computed in float like the

@@ -5,11 +5,12 @@
 // QuantileSamplingModule::distribution call (include/quantileSamplingModule.
 // hpp:128-169, 342-577) evaluated by brute force: the FP64 direct map of the
 // sample (orc_map_direct_f64), the verbatim disk set to -inf, then the
-// device selection rule (orc_select_device). Written as the reference's
-// single-threaded loop, with one stated change: the skip test of an
-// iteration reads the running best per density as it stood at the start of
-// that iteration (the order the GPU batch needs; the reference's parallel
-// loop leaves the order of these updates to its OpenMP schedule).
+// device selection rule on the FP64 maps (orc_select_device_f64). Written
+// as the reference's single-threaded loop, with one stated change: the skip
+// test of an iteration reads the running best per density as it stood at
+// the start of that iteration (the order the GPU batch needs; the
+// reference's parallel loop leaves the order of these updates to its OpenMP
+// schedule).
 //
 // Pinned by tests/test_oracle_calib.py: offsets order (every box offset
 // once, non-decreasing distance, std::sort's order of equal distances
@@ -34,6 +35,8 @@ float orc_convert_neighbors(int D, int nb, const int* off, int nExp, const float
                             const unsigned* kdims, const float* kernel, float* u);
 void orc_map_direct_f64(int D, const unsigned* s, int nSub, const float* Y, const float* alpha, int nb,
                         const int* off, const float* u, double delta0, int circular, double* out);
+int orc_select_device_f64(int D, const unsigned* s, const double* map, const int* lo, const int* hi, int K,
+                          float seed, float quantum, double* outVal, unsigned* outPos);
 int orc_select_device(int D, const unsigned* s, const float* map, const int* lo, const int* hi, int K, float seed,
                       float quantum, float* outVal, unsigned* outPos);
 int orc_index_with_delta(int D, const unsigned* dims, unsigned position, const int* delta, unsigned* location);
@@ -304,17 +307,21 @@ int orc_calibration(int D, int nTI, const unsigned* tiSize, const float* const* 
           std::vector<double> m64(S);
           orc_map_direct_f64(D, tiSize + t * 3, nSub, Y[t].data(), alpha.data(), nb, off.data(), u.data(), delta0, 0,
                              m64.data());
-          std::vector<float> m(S);
-          for (size_t c = 0; c < S; ++c) m[c] = (float)m64[c];
-          if (t == tiIndex) orc_exclude_disk(D, tiSize + t * 3, m.data(), currentCell, radius);
+          if (t == tiIndex) {  // the disk, marked on a float copy, applied to the FP64 map
+            std::vector<float> mark(S, 0.f);
+            orc_exclude_disk(D, tiSize + t * 3, mark.data(), currentCell, radius);
+            for (size_t c = 0; c < S; ++c)
+              if (mark[c] == -INFINITY) m64[c] = -INFINITY;
+          }
           int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
           for (int d = 0; d < D; ++d) hi[d] = int(tiSize[t * 3 + d]) - 1;
-          std::vector<float> ov(maxK_);
+          std::vector<double> ov(maxK_);
           std::vector<unsigned> op(maxK_);
-          orc_select_device(D, tiSize + t * 3, m.data(), lo, hi, maxK_, sseed, quantum, ov.data(), op.data());
+          orc_select_device_f64(D, tiSize + t * 3, m64.data(), lo, hi, maxK_, sseed, quantum, ov.data(), op.data());
           for (int i = 0; i < maxK_; ++i)
             if (op[i] != UINT_MAX) {
-              errs.push_back(quantum > 0.f ? std::rint(ov[i] / quantum) : ov[i]);
+              const float v = (float)ov[i];
+              errs.push_back(quantum > 0.f ? std::rint(v / quantum) : v);
               where.push_back({t, op[i]});
             }
         }

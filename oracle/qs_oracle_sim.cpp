@@ -58,6 +58,8 @@ void orc_ti_spectra_f64(int D, const unsigned* f, int nSub, const float* imgs, d
 void orc_fkst(const float* data, unsigned N, unsigned short k, float seed, float* out, unsigned* pos);
 unsigned orc_index_to_position(int D, const unsigned* f, const unsigned* s, const int* mn, unsigned index);
 unsigned orc_position_to_index(int D, const unsigned* f, const unsigned* s, const int* mn, unsigned position);
+int orc_select_device_f64(int D, const unsigned* s, const double* map, const int* lo, const int* hi, int K,
+                          float seed, float quantum, double* outVal, unsigned* outPos);
 int orc_select_device(int D, const unsigned* s, const float* map, const int* lo, const int* hi, int K, float seed,
                       float quantum, float* outVal, unsigned* outPos);
 int orc_index_with_delta(int D, const unsigned* dims, unsigned position, const int* delta, unsigned* location);
@@ -429,16 +431,17 @@ static unsigned node_multi_ti(OrcSimCtx* c, int nb, const std::vector<int>& off,
     std::vector<double> m64(St);
     orc_map_direct_f64(D, st, c->nSub, t == 0 ? c->Y.data() : c->Yx[t - 1].data(), c->alpha.data(), nb, off.data(),
                        u.data(), (double)delta0, c->p.circularTI, m64.data());
-    std::vector<float> m(St);
-    for (size_t q = 0; q < St; ++q) m[q] = (float)m64[q];
-    std::vector<float> v(K);
+    // every TI's K best by the device rule on its FP64 map, then the merge
+    // of their values (as float) across TIs
+    std::vector<double> v64(K);
     std::vector<unsigned> ps(K);
-    orc_select_device(D, st, m.data(), lo, hi, K, seed, p.quantum, v.data(), ps.data());
+    orc_select_device_f64(D, st, m64.data(), lo, hi, K, seed, p.quantum, v64.data(), ps.data());
     for (int i = 0; i < K; ++i) {
-      if (ps[i] == UINT_MAX || !(v[i] > -INFINITY)) continue;
+      const float vi = (float)v64[i];
+      if (ps[i] == UINT_MAX || !(vi > -INFINITY)) continue;
       Cd cd;
-      cd.val = v[i];
-      cd.key = p.quantum > 0.f ? std::rint(v[i] * (1.f / p.quantum)) : v[i];
+      cd.val = vi;
+      cd.key = p.quantum > 0.f ? std::rint(vi * (1.f / p.quantum)) : vi;
       cd.pos = ps[i] * nT + t;
       cd.h = orc_tie_hash(sb, cd.pos);
       all.push_back(cd);
@@ -643,6 +646,31 @@ int orc_sim_node_map(void* h, const float* di, unsigned pathIdx, double* map, do
     }
   }
   return nb;
+}
+
+// Norm that scales the FP32 map error of a node (test criterion of the GPU
+// maps): sum_A rms(Y_A) * sum_j |v_jA|, v_jA = sum_B alpha_AB u_jB, rms over
+// the FFT grid (zero padding included). Single TI, plain form.
+double orc_sim_node_norm(void* h, const float* di, unsigned pathIdx) {
+  auto* c = static_cast<OrcSimCtx*>(h);
+  std::vector<int> off;
+  std::vector<float> u;
+  float delta0 = 0.f;
+  int nb = node_inputs(c, di, pathIdx, off, u, &delta0, nullptr);
+  double total = 0.0;
+  for (int A = 0; A < c->nSub; ++A) {
+    double ss = 0.0;
+    for (size_t q = 0; q < c->S; ++q) ss += (double)c->Y[(size_t)A * c->S + q] * (double)c->Y[(size_t)A * c->S + q];
+    const double rms = std::sqrt(ss / (double)c->F);
+    double sv = 0.0;
+    for (int j = 0; j < nb; ++j) {
+      double v = 0.0;
+      for (int B = 0; B < c->nSub; ++B) v += (double)c->alpha[A + B * c->nSub] * (double)u[(size_t)j * c->nSub + B];
+      sv += std::fabs(v);
+    }
+    total += rms * sv;
+  }
+  return total;
 }
 
 // Serial simulation: fills di (cells x nVar) and idx (importDataIndex).

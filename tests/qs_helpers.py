@@ -7,9 +7,18 @@ import numpy as np
 
 import oracle
 
-# |E_gpu - E_oracle| <= MAP_RTOL * s(p), s(p) = sum_j w_j (x_j^2 + y(p+d_j)^2)
-# (continuous) or 2 sum_j w_j (categorical): the magnitude of the cancelling
-# terms of the FP32 FFT correlation.
+# The library's statement of its FP32 map error, restated: every value of a
+# node's map is within eps = ERR_C * sum_a rms(Y_a) * sum_j |v_ja| of the exact
+# (FP64) value, rms over the FFT grid of TI sub-image a, v_ja = sum_B
+# alpha_aB u_jB the neighbours' alpha-combined converted values. The FFT
+# rounding error of a correlation scales with the norms of its two operands,
+# not with the local magnitude of the mismatch.
+ERR_C = 4e-5
+
+# Relative size of the map error against s(p) = sum_j w_j (x_j^2 +
+# y(p+d_j)^2) (continuous) or 2 sum_j w_j (categorical), the magnitude of the
+# cancelling terms: reported, and the bound of the cross-measurement form
+# (TIs with NaN), whose selection stays in FP32.
 MAP_RTOL = 1e-4
 
 
@@ -101,6 +110,17 @@ class Problem:
                         d0 = np.float32(d0 + np.float32(-1.0) * np.float32(w[j] * xv[j] * xv[j]))
         return u, float(d0)
 
+    def sub_rms(self):
+        """rms of every TI sub-image over the FFT grid (zero padding included)."""
+        n_fft = int(np.prod([oracle.lib().orc_fft_size_1d(d) for d in self.dims]))
+        y = self.sub.reshape(self.nsub, -1).astype(np.float64)
+        return np.sqrt((y * y).sum(axis=1) / n_fft)
+
+    def eps_bound(self, u):
+        """The FP32 map error bound of a node with converted values u (nb, nsub)."""
+        v = u.astype(np.float64) @ self.alpha.astype(np.float64).T
+        return ERR_C * float(np.sum(self.sub_rms() * np.abs(v).sum(axis=0)))
+
     def oracle_map(self, off, u, d0, circular=False):
         return oracle.map_direct(self.dims, self.sub, self.alpha, off, u, d0, circular)
 
@@ -141,6 +161,43 @@ def neighbourhood(problem, rng, nb, radius, centre=True):
     if centre:
         x[0, :] = np.nan
     return o.astype(np.int32), x.astype(np.float32)
+
+
+def neighbourhood_low(problem, rng, nb, q_lo=0.02, q_hi=0.10, keep=0.9, noise=0.05, var=0):
+    """The hard case of an FP32 map on a heavy-tailed TI: a dense
+    neighbourhood (the nb-1 nearest offsets, each kept with probability
+    `keep`, then the next ones) copied, with multiplicative noise, from a TI
+    cell whose value lies between the q_lo and q_hi quantiles of variable
+    `var`. Its best matches sit in low-valued TI regions, where the FP32
+    error set by the TI's global norms is largest against the mismatch gaps.
+    Returns offsets (nb, D) with the centre first and raw values (centre NaN)."""
+    D = problem.D
+    dims = np.asarray(problem.dims)
+    vals = problem.ti[:, var]
+    lo, hi = np.quantile(vals, [q_lo, q_hi])
+    offs = oracle.offsets([2 * 9 + 1] * D)
+    rad = np.abs(offs).max(axis=1)
+    for _ in range(1000):
+        cands = np.nonzero((vals >= lo) & (vals <= hi))[0]
+        src = int(cands[rng.integers(len(cands))])
+        c = np.array(np.unravel_index(src, tuple(reversed(problem.dims)))[::-1])
+        inside = np.all((c[None, :] + offs >= 0) & (c[None, :] + offs < dims[None, :]), axis=1)
+        sel = [0]
+        for j in range(1, offs.shape[0]):
+            if len(sel) == nb:
+                break
+            if inside[j] and rng.random() < keep:
+                sel.append(j)
+        if len(sel) == nb and rad[sel].max() < 9:
+            break
+    o = offs[sel].astype(np.int32)
+    t = c[None, :] + o
+    flat = np.ravel_multi_index(tuple(t[:, ::-1].T), tuple(reversed(problem.dims)))
+    x = problem.ti[flat].astype(np.float64)
+    x *= 1.0 + noise * rng.standard_normal(x.shape)
+    x = x.astype(np.float32)
+    x[0, :] = np.nan
+    return o, x
 
 
 def valid_box(dims, off, circular=False):

@@ -3,10 +3,12 @@ checked node by node against the oracle on a seeded sample of path nodes.
 
 Every node of the sample is recomputed by the oracle from the GPU's final
 realisation restricted to what the serial loop would have seen (FP64 direct
-map, device selection rule, the reference's draw); the GPU's choice must be
-the oracle's except for near ties within the FP32 map tolerance (see
-test_gpu_sim.lockstep). Sample sizes keep the oracle's CPU time to about a
-minute per workload.
+map, device selection rule, the reference's draw). Configs 2 and 4 run the
+exact selection: the GPU's choice must be the oracle's at every sampled node.
+Configs 1 and 3 (categorical-only TIs, tie quantum) keep the FP32 selection:
+a differing node is accepted only when the two FP64 values are within
+quantum + 2 eps (see test_gpu_sim.lockstep). Sample sizes keep the oracle's
+CPU time to about a minute per workload.
 
   config 1: TI 250x250 binary facies, sim 200x200, n=50, k=1.2
   config 2: TI 1000x1000 exp(GRF), sim 1000x1000, n=80, k=1.5 (the benchmark)

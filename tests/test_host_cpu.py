@@ -114,3 +114,37 @@ def test_argument_errors_without_gpu():
     assert L.qsmi_device_destroy(None) == 0 and L.qsmi_ti_destroy(None) == 0
     with pytest.raises(g2s_amd.QsmiError):
         g2s_amd.offsets((3, 3, 3, 3))
+
+
+def test_reflected_layout_golden():
+    """The point-reflected FFT-grid sub-images the compiled reference's
+    DataImage::convertInput4Xcorr writes (golden vectors; what
+    SharedMemoryManager hands to allocAndInitSharedMemory) -> the TI-grid
+    layout of qsmi_ti_create: equal to the oracle's unreflected images.
+    TI-grid images placed without the reflection are rejected."""
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_vectors.npz"))
+    ci = 0
+    while f"conv_{ci}_dims" in gold:
+        g = {k[len(f"conv_{ci}_"):]: gold[k] for k in gold.files if k.startswith(f"conv_{ci}_")}
+        dims, fdims, types = g["dims"], g["fdims"], g["types"]
+        ncat = list(g["ncat"]) if np.any(types != 0) else []
+        catv = list(g["catvals"]) if np.any(types != 0) else []
+        F = int(np.prod(fdims))
+        imgs = g["images"].reshape(-1, F)
+        got = g2s_amd.subimages_from_reflected(list(imgs), dims, fdims)
+        ref = oracle.ti_subimages(g["data"], dims, types, ncat, catv)
+        assert np.array_equal(got, ref), ci
+        if types[0] == 0:
+            # y^2 of cell 0 placed at FFT-grid index 0 is, read as reflected,
+            # the last FFT-grid cell: outside the TI box when fftSize > srcSize
+            bad = np.zeros_like(imgs)
+            S = int(np.prod(dims))
+            for q in range(S):
+                c = np.unravel_index(q, tuple(reversed(dims)))[::-1]
+                flat = int(np.ravel_multi_index(tuple(reversed(c)), tuple(reversed(fdims))))
+                bad[:, flat] = ref[:, q]
+            assert ref[0, 0] != 0
+            with pytest.raises(g2s_amd.QsmiError):
+                g2s_amd.subimages_from_reflected(list(bad), dims, fdims)
+        ci += 1
+    assert ci >= 4
