@@ -452,7 +452,9 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
     stats = dict(setup_ms=rs.setupMs, path_loop_ms=rs.pathLoopMs, scan_ms=rs.scanMs, spectrum_ms=rs.stageMs[0],
                  gather_ms=rs.stageMs[1], rows_ms=rs.stageMs[2], merge_ms=rs.stageMs[3], nodes=rs.nodes,
                  scanned_nodes=rs.scannedNodes, waves=rs.waves, search_ms=rs.phaseMs[0], levels_ms=rs.phaseMs[1],
-                 upload_ms=rs.phaseMs[2], device_loop_ms=rs.phaseMs[3])
+                 upload_ms=rs.phaseMs[2], device_loop_ms=rs.phaseMs[3], exact_nodes=rs.exactNodes,
+                 band_positions=rs.bandPositions, second_pass_blocks=rs.secondPassBlocks,
+                 full_rescore_nodes=rs.fullRescoreNodes)
     if rec is not None:
         m = min(int(record), int(rs.nodes))
         stats["pairs"] = (rec[0][:m], rec[1][:m], rec[2][:m])

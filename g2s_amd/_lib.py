@@ -221,6 +221,10 @@ class QsResult(C.Structure):
         ("lpAuditNodes", C.c_longlong),
         ("lpAuditHits", C.c_longlong),
         ("lpAuditBestHits", C.c_longlong),
+        ("exactNodes", C.c_longlong),
+        ("bandPositions", C.c_longlong),
+        ("secondPassBlocks", C.c_longlong),
+        ("fullRescoreNodes", C.c_longlong),
     ]
 
 

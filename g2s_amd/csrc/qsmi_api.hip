@@ -28,7 +28,7 @@ void launch_direct1d(const ScanGeom& g, const float2* Yhat, const float2* tw0, i
                      const int* nbOff, const float* nbVal, float2* C, hipStream_t st);
 void launch_rows_select(const ScanGeom& g, const FftPlan& plan0, int lognv, const float2* tw0, const NodeInfo* info,
                         const float2* C, int nNodes, int rowBlocks, int K, Cand* cand, float* map, hipStream_t st,
-                        int crossOff = 0, const uint8_t* centreOk = nullptr);
+                        int crossOff = 0, const uint8_t* centreOk = nullptr, int* gcnt = nullptr);
 void launch_select_map(const float* map, long long S, int K, float quantum, uint32_t seedbits, Cand* cand, int chunks,
                        hipStream_t st);
 void launch_fill(float* p, long long n, float v, hipStream_t st);
@@ -41,7 +41,7 @@ void launch_pencil1k(const ScanGeom& g, const float2* YT, const float2* tw1k, in
 int rows1k_tiles();
 hipError_t set_rows1k_lds();
 void launch_rows1k(const ScanGeom& g, const float2* tw1k, const NodeInfo* info, const float2* CT, int nNodes, int K,
-                   Cand* cand, float* map, const int* count, hipStream_t st);
+                   Cand* cand, float* map, const int* count, int* gcnt, hipStream_t st);
 }  // namespace qsmi
 
 using namespace qsmi;
@@ -285,13 +285,13 @@ int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeI
 }
 
 void launch_rows(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const float2* C, int nNodes, int rowBlocks,
-                 int K, Cand* cand, float* map, const int* count, int crossOff) {
+                 int K, Cand* cand, int* gcnt, float* map, const int* count, int crossOff) {
   qsmi_ti* t = dev->ti;
   if (t->fast1k)
-    launch_rows1k(g, t->dTw[0], info, C, nNodes, K, cand, map, count, dev->stream);
+    launch_rows1k(g, t->dTw[0], info, C, nNodes, K, cand, map, count, gcnt, dev->stream);
   else
     launch_rows_select(g, t->plan[0], t->lognv[0], t->dTw[0], info, C, nNodes, rowBlocks, K, cand, map, dev->stream,
-                       crossOff, dev->dCentreOk);
+                       crossOff, dev->dCentreOk, gcnt);
 }
 
 int set_cross(qsmi_device* dev, const float* alphaX, const uint8_t* centreOk) {
@@ -608,7 +608,7 @@ int qsmi_candidate_for_pattern(qsmi_device* dev, int nb, const int* offsets, con
   launch_fill(dev->dMap, t->S, -INFINITY, st);
   ScanGeom g = geom_of(dev);
   if ((rc = launch_spectrum(dev, g, 1, dev->dInfo, dev->dOff, dev->dVal, dev->dOrd, dev->dC))) return rc;
-  launch_rows(dev, g, dev->dInfo, dev->dC, 1, row_blocks(t), 1, nullptr, dev->dMap);
+  launch_rows(dev, g, dev->dInfo, dev->dC, 1, row_blocks(t), 1, nullptr, nullptr, dev->dMap);
   HIPCHK(hipGetLastError());
   // the host arrays above are pageable and go out of scope: finish the copies
   HIPCHK(hipStreamSynchronize(st));
@@ -740,6 +740,7 @@ void free_wave_bufs(WaveBufs* b) {
   (void)hipFree(b->dC);
   (void)hipFree(b->dCand);
   (void)hipFree(b->dCandEx);
+  (void)hipFree(b->dGcnt);
   (void)hipFree(b->dOutVal);
   (void)hipFree(b->dOutPos);
   (void)hipFree(b->dChosen);
@@ -762,8 +763,10 @@ int grow_wave_bufs(qsmi_device* dev, WaveBufs* b, int nNodes, int total, int K) 
     (void)hipFree(b->dChosen);
     (void)hipFree(b->dCand);
     (void)hipFree(b->dCandEx);
+    (void)hipFree(b->dGcnt);
     (void)hipHostFree(b->hInfo);
     b->dCandEx = nullptr;
+    b->dGcnt = nullptr;
     b->dInfo = nullptr;
     b->dOutVal = nullptr;
     b->dOutPos = nullptr;
@@ -778,6 +781,7 @@ int grow_wave_bufs(qsmi_device* dev, WaveBufs* b, int nNodes, int total, int K) 
     HIPCHK(dalloc(&b->dChosen, n));
     HIPCHK(dalloc(&b->dCand, n * rb * kMaxK));
     HIPCHK(dalloc(&b->dCandEx, n * kMaxK));
+    HIPCHK(dalloc(&b->dGcnt, n * rb));
     HIPCHK(hipHostMalloc((void**)&b->hInfo, n * sizeof(NodeInfo), hipHostMallocDefault));
     b->capNodes = n;
     b->capCand = n * rb * kMaxK;
@@ -868,10 +872,10 @@ int run_wave(qsmi_device* dev, WaveBufs* b) {
   ScanGeom g = geom_of(dev);
   int rc = launch_spectrum(dev, g, b->nNodes, b->dInfo, b->dOff, b->dVal, b->dOrd, b->dC);
   if (rc) return rc;
-  launch_rows(dev, g, b->dInfo, b->dC, b->nNodes, b->rowBlocks, b->K, b->dCand, nullptr);
+  launch_rows(dev, g, b->dInfo, b->dC, b->nNodes, b->rowBlocks, b->K, b->dCand, b->dGcnt, nullptr);
   if (dev->timing) (void)hipEventRecord(dev->ev[3], st);
-  rc = band_select(dev, g, b->dInfo, b->dCand, b->nNodes, b->K, b->k, b->dC, b->dOff, b->dVal, b->nbMax, b->dOutVal,
-                   b->dOutPos, nullptr, b->dChosen, nullptr, nullptr, 0, nullptr, &b->band);
+  rc = band_select(dev, g, b->dInfo, b->dCand, b->dGcnt, b->nNodes, b->K, b->k, b->dC, b->dOff, b->dVal, b->nbMax,
+                   b->dOutVal, b->dOutPos, nullptr, b->dChosen, nullptr, nullptr, 0, nullptr, &b->band);
   if (rc) return rc;
   if (dev->timing) (void)hipEventRecord(dev->ev[4], st);
   HIPCHK(hipGetLastError());

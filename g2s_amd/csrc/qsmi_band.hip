@@ -13,17 +13,20 @@
 //      bound, NodeInfo::eps). Every position whose exact value reaches the
 //      exact K-th best lies in the band (its FP32 value is at least its
 //      exact value - eps, and the exact K-th best is at least T - eps).
-//      Each rows group (wavefront or block) emitted its M best; a group
-//      whose M-th candidate is still in the band may hold more band
-//      positions, so its rows block is queued for a second pass; the
-//      candidates of the other groups in the band go to the node's list.
+//      Each rows group (wavefront or block) emitted its K best and the
+//      number of its values within 2 eps of its own K-th best; a group whose
+//      K-th candidate is in the band and which holds more than K such values
+//      may hold band positions it did not emit, so its rows block is queued
+//      for a second pass; the candidates of the other groups in the band go
+//      to the node's list.
 //   2. second pass (k_rows1k_collect / k_rows_collect): the queued rows
 //      blocks again, appending every position >= theta.
-//   3. rescoring (k_band_rescore): the exact value of every listed position
-//      in FP64, E = delta0 + sum_j sum_a v_ja Y_a(p + d_j) summed in that
-//      order with separately rounded products and sums (no fused
-//      multiply-add), and the K best by the device rule on those values,
-//      then the rank draw and the simulation write.
+//   3. rescoring (k_band_rescore, kBandChunks blocks per node, then
+//      k_band_final): the exact value of every listed position in FP64,
+//      E = delta0 + sum_j sum_a v_ja Y_a(p + d_j) summed in that order with
+//      separately rounded products and sums (no fused multiply-add), and the
+//      K best by the device rule on those values, then the rank draw and the
+//      simulation write.
 //   4. full fallback (k_rescore_full + k_rescore_full_merge): a node whose
 //      band overflows the list capacity is rescored over its whole valid
 //      box in FP64.
@@ -44,7 +47,6 @@
 
 namespace qsmi {
 
-int cands_per_group(int K);
 int rows1k_blocks();
 int rows1k_waves_per_block();
 int rows1k_tiles();
@@ -60,6 +62,7 @@ namespace {
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr int kFullChunks = 64;  // blocks per node of the full fallback
 constexpr int kFullSlots = 8;    // nodes of the full fallback in flight
+constexpr int kBandChunks = 8;   // rescoring blocks per node (listed positions interleaved)
 
 // Device view of the band buffers.
 struct BandDev {
@@ -70,6 +73,8 @@ struct BandDev {
   int* qcnt;       // [0] queue items, [1] nodes of the full fallback
   int* fullq;      // [item] node
   Cand64* part;    // [item][kFullChunks][K]
+  Cand64* bpart;   // [node][kBandChunks][K] rescoring partial results
+  unsigned long long* stats;  // exact nodes, listed positions, second-pass blocks, full-fallback nodes
 };
 
 BandDev dev_view(const BandBufs& b) {
@@ -81,6 +86,8 @@ BandDev dev_view(const BandBufs& b) {
   d.qcnt = b.qcnt;
   d.fullq = b.fullq;
   d.part = b.part;
+  d.bpart = b.bpart;
+  d.stats = b.stats;
   return d;
 }
 
@@ -131,13 +138,15 @@ __global__ void k_band_reset(int* __restrict__ qcnt, int* __restrict__ cnt0) {
   if (threadIdx.x == 0 && cnt0) *cnt0 = 0;
 }
 
-// Step 1. One 256-thread block per node. cand: [node][candGroups][M];
+// Step 1. One 256-thread block per node. cand: [node][candGroups][K], gcnt:
+// [node][candGroups];
 // groups of one rows block are consecutive (wpb per block); rowsPerBlock > 0
 // (generic rows kernel): only the blocks covering the node's valid rows
 // wrote candidates (one group each).
 template <int KT>
 __global__ void __launch_bounds__(256) k_merge_band(const NodeInfo* __restrict__ info, const Cand* __restrict__ cand,
-                                                    int candGroups, int M, int wpb, int rowsPerBlock, int K, float kf,
+                                                    const int* __restrict__ gcnt, int candGroups, int wpb,
+                                                    int rowsPerBlock, int K, float kf,
                                                     float* __restrict__ outVal, uint32_t* __restrict__ outPos,
                                                     Cand* __restrict__ outCand, uint32_t* __restrict__ chosen,
                                                     float* __restrict__ simVals, const float* __restrict__ tiVals,
@@ -166,8 +175,9 @@ __global__ void __launch_bounds__(256) k_merge_band(const NodeInfo* __restrict__
   }
   TopK<KT> tk;
   tk.init();
-  const Cand* cn = cand + (long long)node * candGroups * M;
-  for (int i = tid; i < nGroups * M; i += blockDim.x) tk.push(cn[i]);
+  const Cand* cn = cand + (long long)node * candGroups * K;
+  const int* gc = gcnt + (long long)node * candGroups;
+  for (int i = tid; i < nGroups * K; i += blockDim.x) tk.push(cn[i]);
   block_topk<KT>(tk, K, best, red);
   if (!(ni.eps > 0.f)) {  // FP32 selection
     if (tid < K) {
@@ -188,17 +198,20 @@ __global__ void __launch_bounds__(256) k_merge_band(const NodeInfo* __restrict__
   for (int i = tid; i < kMaxBandBlocks / 32; i += blockDim.x) flags[i] = 0u;
   if (tid == 0) lcnt = 0;
   __syncthreads();
+  // a group may hide band positions when its K-th candidate is in the band
+  // and more than K of its values lie within 2 eps of that K-th value (the
+  // band lies inside that range: theta >= its K-th value - 2 eps)
   for (int gi = tid; gi < nGroups; gi += blockDim.x) {
-    const Cand last = cn[(long long)gi * M + M - 1];
-    if (last.pos != kNone && last.val >= th) {
+    const Cand last = cn[(long long)gi * K + K - 1];
+    if (last.pos != kNone && last.val >= th && gc[gi] > K) {
       const int rb = gi / wpb;
       atomicOr(&flags[rb >> 5], 1u << (rb & 31));
     }
   }
   __syncthreads();
   uint32_t* out = bb.pos + (long long)node * kBandCap;
-  for (int i = tid; i < nGroups * M; i += blockDim.x) {
-    const int rb = (i / M) / wpb;
+  for (int i = tid; i < nGroups * K; i += blockDim.x) {
+    const int rb = (i / K) / wpb;
     if ((flags[rb >> 5] >> (rb & 31)) & 1u) continue;
     const Cand c = cn[i];
     if (c.pos != kNone && c.val >= th) {
@@ -206,61 +219,88 @@ __global__ void __launch_bounds__(256) k_merge_band(const NodeInfo* __restrict__
       if (k < kBandCap) out[k] = c.pos;
     }
   }
+  int nq = 0;
   for (int rb = tid; rb < nBlocks; rb += blockDim.x) {
     if ((flags[rb >> 5] >> (rb & 31)) & 1u) {
       const int q = atomicAdd(&bb.qcnt[0], 1);  // capacity: every (node, block) of the wave
       bb.queue[q] = make_int2(node, rb);
+      ++nq;
     }
   }
+  if (nq) atomicAdd(&bb.stats[2], (unsigned long long)nq);
   __syncthreads();
   if (tid == 0) {
     bb.cnt[node] = lcnt;
     bb.theta[node] = th;
+    atomicAdd(&bb.stats[0], 1ull);
   }
 }
 
-// Step 3. One 256-thread block per node; dynamic LDS: the node's neighbours.
+// Step 3a. grid (node, kBandChunks), 256 threads; dynamic LDS: the node's
+// neighbours. Chunk c scores listed positions c*256 + t + j*kBandChunks*256
+// and writes its K best to bpart[node][c].
 template <int KT>
 __global__ void __launch_bounds__(256) k_band_rescore(ScanGeom g, const NodeInfo* __restrict__ info,
                                                       const int* __restrict__ nbOff, const float* __restrict__ nbVal,
                                                       const float* __restrict__ Y, long long S, double nscale, int K,
-                                                      float kf, float* __restrict__ outVal,
-                                                      uint32_t* __restrict__ outPos, Cand* __restrict__ outCand,
-                                                      uint32_t* __restrict__ chosen, float* __restrict__ simVals,
-                                                      const float* __restrict__ tiVals, int nVar,
-                                                      uint32_t* __restrict__ indexOut, BandDev bb,
-                                                      const int* __restrict__ count) {
+                                                      BandDev bb, const int* __restrict__ count) {
   extern __shared__ char sm[];
+  __shared__ Cand64 red[4];
+  const int node = blockIdx.x, chunk = blockIdx.y, tid = threadIdx.x;
+  if (count && node >= *count) return;
+  const int cnt = bb.cnt[node];
+  if (cnt < 0) return;
+  Cand64* out = bb.bpart + ((long long)node * kBandChunks + chunk) * K;
+  if (cnt > kBandCap) {  // the list overflowed: whole valid box (step 4)
+    if (chunk == 0 && tid == 0) {
+      const int q = atomicAdd(&bb.qcnt[1], 1);
+      bb.fullq[q] = node;
+      atomicAdd(&bb.stats[3], 1ull);
+    }
+    return;
+  }
+  if (chunk == 0 && tid == 0) atomicAdd(&bb.stats[1], (unsigned long long)cnt);
+  TopK64<KT> tk;
+  tk.init();
+  if (chunk * 256 < cnt) {  // block-uniform
+    const NodeInfo ni = info[node];
+    int* so;
+    double* sv;
+    stage_neighbours(g, ni, nbOff, nbVal, sm, &so, &sv);
+    __syncthreads();
+    const uint32_t* lst = bb.pos + (long long)node * kBandCap;
+    for (int i = chunk * 256 + tid; i < cnt; i += kBandChunks * 256) {
+      const uint32_t p = lst[i];
+      Cand64 c;
+      c.val = exact_value(g, ni, so, sv, Y, S, p, nscale);
+      c.h = tie_hash(ni.seedbits, p);
+      c.pos = p;
+      tk.push(c);
+    }
+  }
+  block_topk64<KT>(tk, K, out, red);
+}
+
+// Step 3b. One block per node: the K best of its chunks, outputs and draw.
+template <int KT>
+__global__ void __launch_bounds__(256) k_band_final(const NodeInfo* __restrict__ info, int K, float kf,
+                                                    float* __restrict__ outVal, uint32_t* __restrict__ outPos,
+                                                    Cand* __restrict__ outCand, uint32_t* __restrict__ chosen,
+                                                    float* __restrict__ simVals, const float* __restrict__ tiVals,
+                                                    int nVar, uint32_t* __restrict__ indexOut, BandDev bb,
+                                                    const int* __restrict__ count) {
   __shared__ Cand64 red[4];
   __shared__ Cand64 best[kMaxK];
   __shared__ uint32_t bpos[kMaxK];
   const int node = blockIdx.x, tid = threadIdx.x;
   if (count && node >= *count) return;
   const int cnt = bb.cnt[node];
-  if (cnt < 0) return;
+  if (cnt < 0 || cnt > kBandCap) return;  // finished by the merge / the full fallback
   const NodeInfo ni = info[node];
-  if (cnt > kBandCap) {  // the list overflowed: whole valid box
-    if (tid == 0) {
-      const int q = atomicAdd(&bb.qcnt[1], 1);
-      bb.fullq[q] = node;
-    }
-    return;
-  }
-  int* so;
-  double* sv;
-  stage_neighbours(g, ni, nbOff, nbVal, sm, &so, &sv);
-  __syncthreads();
   TopK64<KT> tk;
   tk.init();
-  const uint32_t* lst = bb.pos + (long long)node * kBandCap;
-  for (int i = tid; i < cnt; i += blockDim.x) {
-    const uint32_t p = lst[i];
-    Cand64 c;
-    c.val = exact_value(g, ni, so, sv, Y, S, p, nscale);
-    c.h = tie_hash(ni.seedbits, p);
-    c.pos = p;
-    tk.push(c);
-  }
+  const Cand64* pp = bb.bpart + (long long)node * kBandChunks * K;
+  for (int i = tid; i < kBandChunks * K; i += blockDim.x) tk.push(pp[i]);
   block_topk64<KT>(tk, K, best, red);
   if (tid < K) {
     const Cand64 c = best[tid];
@@ -410,6 +450,8 @@ void band_free(BandBufs* b) {
   (void)hipFree(b->qcnt);
   (void)hipFree(b->fullq);
   (void)hipFree(b->part);
+  (void)hipFree(b->bpart);
+  (void)hipFree(b->stats);
   *b = BandBufs();
 }
 
@@ -420,12 +462,19 @@ int band_grow(BandBufs* b, int nNodes, int blocksPerNode, int K) {
   const size_t nn = std::max(n, b->capNodes);
   const size_t qq = std::max(q, b->capQueue);
   const int kk = std::max(K, b->K);
+  // grown between launches of one stream: work queued on the old buffers
+  // finishes first (hipFree waits for it)
+  unsigned long long keep[4] = {0, 0, 0, 0};
+  if (b->stats) (void)hipMemcpy(keep, b->stats, sizeof(keep), hipMemcpyDeviceToHost);
   band_free(b);
   if (hipMalloc(&b->pos, nn * kBandCap * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&b->cnt, nn * sizeof(int)) != hipSuccess || hipMalloc(&b->theta, nn * sizeof(float)) != hipSuccess ||
       hipMalloc(&b->queue, qq * sizeof(int2)) != hipSuccess || hipMalloc(&b->qcnt, 2 * sizeof(int)) != hipSuccess ||
       hipMalloc(&b->fullq, nn * sizeof(int)) != hipSuccess ||
-      hipMalloc(&b->part, nn * kFullChunks * (size_t)kk * sizeof(Cand64)) != hipSuccess) {
+      hipMalloc(&b->part, nn * kFullChunks * (size_t)kk * sizeof(Cand64)) != hipSuccess ||
+      hipMalloc(&b->bpart, nn * kBandChunks * (size_t)kk * sizeof(Cand64)) != hipSuccess ||
+      hipMalloc(&b->stats, sizeof(keep)) != hipSuccess ||
+      hipMemcpy(b->stats, keep, sizeof(keep), hipMemcpyHostToDevice) != hipSuccess) {
     band_free(b);
     return fail(QSMI_ERR_NOMEM, "exact selection buffers: out of device memory");
   }
@@ -435,13 +484,40 @@ int band_grow(BandBufs* b, int nNodes, int blocksPerNode, int K) {
   return QSMI_OK;
 }
 
-int band_select(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const Cand* cand, int nNodes, int K,
-                float kf, const float2* C, const int* nbOff, const float* nbVal, int nbMax, float* outVal,
-                uint32_t* outPos, Cand* outCand, uint32_t* chosen, float* simVals, const float* tiVals, int nVar,
-                uint32_t* indexOut, BandBufs* bb, const int* count) {
+int band_read_stats(const BandBufs* b, long long out[4]) {
+  unsigned long long v[4] = {0, 0, 0, 0};
+  if (b->stats && hipMemcpy(v, b->stats, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess)
+    return fail(QSMI_ERR_HIP, "exact selection statistics");
+  for (int i = 0; i < 4; ++i) out[i] = (long long)v[i];
+  return QSMI_OK;
+}
+
+namespace {
+
+// steps 3 and 4 for nNodes nodes whose lists are in bb
+template <int KT>
+void launch_rescore(const ScanGeom& g, const NodeInfo* info, const int* nbOff, const float* nbVal, const qsmi_ti* t,
+                    double nscale, int nNodes, int K, float kf, float* outVal, uint32_t* outPos, Cand* outCand,
+                    uint32_t* chosen, float* simVals, const float* tiVals, int nVar, uint32_t* indexOut,
+                    const BandDev& bd, const int* count, size_t lds, hipStream_t st) {
+  hipLaunchKernelGGL(k_band_rescore<KT>, dim3((unsigned)nNodes, kBandChunks), dim3(256), lds, st, g, info, nbOff,
+                     nbVal, t->dSub, t->S, nscale, K, bd, count);
+  hipLaunchKernelGGL(k_band_final<KT>, dim3((unsigned)nNodes), dim3(256), 0, st, info, K, kf, outVal, outPos, outCand,
+                     chosen, simVals, tiVals, nVar, indexOut, bd, count);
+  hipLaunchKernelGGL(k_rescore_full<KT>, dim3(kFullChunks, kFullSlots), dim3(256), lds, st, g, info, nbOff, nbVal,
+                     t->dSub, t->S, nscale, K, bd);
+  hipLaunchKernelGGL(k_rescore_full_merge<KT>, dim3(kFullSlots), dim3(256), 0, st, info, K, kf, outVal, outPos,
+                     outCand, chosen, simVals, tiVals, nVar, indexOut, bd);
+}
+
+}  // namespace
+
+int band_select(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const Cand* cand, const int* gcnt,
+                int nNodes, int K, float kf, const float2* C, const int* nbOff, const float* nbVal, int nbMax,
+                float* outVal, uint32_t* outPos, Cand* outCand, uint32_t* chosen, float* simVals,
+                const float* tiVals, int nVar, uint32_t* indexOut, BandBufs* bb, const int* count) {
   qsmi_ti* t = dev->ti;
   hipStream_t st = dev->stream;
-  const int M = cands_per_group(K);
   int candGroups, wpb, rpb, blocks;
   if (t->fast1k) {
     candGroups = rows1k_tiles();
@@ -465,9 +541,9 @@ int band_select(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const
   const BandDev bd = dev_view(*bb);
   hipLaunchKernelGGL(k_band_reset, dim3(1), dim3(64), 0, st, bb->qcnt, (int*)nullptr);
   const int KT = kt_of(K);
-#define QSMI_BAND_MERGE(KT_)                                                                                       \
-  hipLaunchKernelGGL(k_merge_band<KT_>, dim3((unsigned)nNodes), dim3(256), 0, st, info, cand, candGroups, M, wpb,  \
-                     rpb, K, kf, outVal, outPos, outCand, chosen, simVals, tiVals, nVar, indexOut, bd, count)
+#define QSMI_BAND_MERGE(KT_)                                                                                      \
+  hipLaunchKernelGGL(k_merge_band<KT_>, dim3((unsigned)nNodes), dim3(256), 0, st, info, cand, gcnt, candGroups, \
+                     wpb, rpb, K, kf, outVal, outPos, outCand, chosen, simVals, tiVals, nVar, indexOut, bd, count)
   switch (KT) {
     case 2: QSMI_BAND_MERGE(2); break;
     case 4: QSMI_BAND_MERGE(4); break;
@@ -485,21 +561,16 @@ int band_select(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const
   const double nscale = dev->trueMismatch ? 1.0 : (double)t->N;
   const size_t lds = neighbour_lds(g, nbMax);
   if (lds > 48 * 1024) return fail(QSMI_ERR_UNSUPPORTED, "exact selection: n x (variables) too large for LDS");
-#define QSMI_BAND_RESCORE(KT_)                                                                                        \
-  hipLaunchKernelGGL(k_band_rescore<KT_>, dim3((unsigned)nNodes), dim3(256), lds, st, g, info, nbOff, nbVal, t->dSub, \
-                     t->S, nscale, K, kf, outVal, outPos, outCand, chosen, simVals, tiVals, nVar, indexOut, bd,       \
-                     count);                                                                                          \
-  hipLaunchKernelGGL(k_rescore_full<KT_>, dim3(kFullChunks, kFullSlots), dim3(256), lds, st, g, info, nbOff, nbVal,  \
-                     t->dSub, t->S, nscale, K, bd);                                                                   \
-  hipLaunchKernelGGL(k_rescore_full_merge<KT_>, dim3(kFullSlots), dim3(256), 0, st, info, K, kf, outVal, outPos,     \
-                     outCand, chosen, simVals, tiVals, nVar, indexOut, bd)
   switch (KT) {
-    case 2: QSMI_BAND_RESCORE(2); break;
-    case 4: QSMI_BAND_RESCORE(4); break;
-    case 8: QSMI_BAND_RESCORE(8); break;
-    default: QSMI_BAND_RESCORE(16); break;
+    case 2: launch_rescore<2>(g, info, nbOff, nbVal, t, nscale, nNodes, K, kf, outVal, outPos, outCand, chosen, simVals,
+                              tiVals, nVar, indexOut, bd, count, lds, st); break;
+    case 4: launch_rescore<4>(g, info, nbOff, nbVal, t, nscale, nNodes, K, kf, outVal, outPos, outCand, chosen, simVals,
+                              tiVals, nVar, indexOut, bd, count, lds, st); break;
+    case 8: launch_rescore<8>(g, info, nbOff, nbVal, t, nscale, nNodes, K, kf, outVal, outPos, outCand, chosen, simVals,
+                              tiVals, nVar, indexOut, bd, count, lds, st); break;
+    default: launch_rescore<16>(g, info, nbOff, nbVal, t, nscale, nNodes, K, kf, outVal, outPos, outCand, chosen,
+                                simVals, tiVals, nVar, indexOut, bd, count, lds, st); break;
   }
-#undef QSMI_BAND_RESCORE
   return QSMI_OK;
 }
 
@@ -520,22 +591,16 @@ int band_select_map(qsmi_device* dev, const NodeInfo* info, int nb, int K, float
   const size_t lds = neighbour_lds(g, nb);
   if (lds > 48 * 1024) return fail(QSMI_ERR_UNSUPPORTED, "exact selection: n x (variables) too large for LDS");
   const float kf = (float)K;
-#define QSMI_MAP_RESCORE(KT_)                                                                                         \
-  hipLaunchKernelGGL(k_band_rescore<KT_>, dim3(1), dim3(256), lds, st, g, info, dev->dOff, dev->dVal, t->dSub, S,     \
-                     nscale, K, kf, outVal, outPos, (Cand*)nullptr, (uint32_t*)nullptr, (float*)nullptr,              \
-                     (const float*)nullptr, 0, (uint32_t*)nullptr, bd, (const int*)nullptr);                          \
-  hipLaunchKernelGGL(k_rescore_full<KT_>, dim3(kFullChunks, kFullSlots), dim3(256), lds, st, g, info, dev->dOff,     \
-                     dev->dVal, t->dSub, S, nscale, K, bd);                                                           \
-  hipLaunchKernelGGL(k_rescore_full_merge<KT_>, dim3(kFullSlots), dim3(256), 0, st, info, K, kf, outVal, outPos,     \
-                     (Cand*)nullptr, (uint32_t*)nullptr, (float*)nullptr, (const float*)nullptr, 0,                   \
-                     (uint32_t*)nullptr, bd)
   switch (kt_of(K)) {
-    case 2: QSMI_MAP_RESCORE(2); break;
-    case 4: QSMI_MAP_RESCORE(4); break;
-    case 8: QSMI_MAP_RESCORE(8); break;
-    default: QSMI_MAP_RESCORE(16); break;
+    case 2: launch_rescore<2>(g, info, dev->dOff, dev->dVal, t, nscale, 1, K, kf, outVal, outPos, nullptr, nullptr,
+                              nullptr, nullptr, 0, nullptr, bd, nullptr, lds, st); break;
+    case 4: launch_rescore<4>(g, info, dev->dOff, dev->dVal, t, nscale, 1, K, kf, outVal, outPos, nullptr, nullptr,
+                              nullptr, nullptr, 0, nullptr, bd, nullptr, lds, st); break;
+    case 8: launch_rescore<8>(g, info, dev->dOff, dev->dVal, t, nscale, 1, K, kf, outVal, outPos, nullptr, nullptr,
+                              nullptr, nullptr, 0, nullptr, bd, nullptr, lds, st); break;
+    default: launch_rescore<16>(g, info, dev->dOff, dev->dVal, t, nscale, 1, K, kf, outVal, outPos, nullptr, nullptr,
+                                nullptr, nullptr, 0, nullptr, bd, nullptr, lds, st); break;
   }
-#undef QSMI_MAP_RESCORE
   return QSMI_OK;
 }
 

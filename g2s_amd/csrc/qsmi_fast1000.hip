@@ -446,7 +446,8 @@ __global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k(ScanGeom 
                                                                      long long ctStride, int K,
                                                                      Cand* __restrict__ cand, int candBlocks,
                                                                      float* __restrict__ map,
-                                                                     const int* __restrict__ count) {
+                                                                     const int* __restrict__ count,
+                                                                     int* __restrict__ gcnt) {
   using namespace f1k;
   __shared__ float2 buf[G][L];
   __shared__ float2 twl[L];
@@ -459,7 +460,7 @@ __global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k(ScanGeom 
   const int h0 = g.h0;
   // one candidate group of K per wavefront of the block
   const int wavesPerBlock = blockDim.x >> 6;
-  Cand* out = cand ? cand + ((long long)node * candBlocks + (long long)blockIdx.x * wavesPerBlock) * KT : nullptr;
+  Cand* out = cand ? cand + ((long long)node * candBlocks + (long long)blockIdx.x * wavesPerBlock) * K : nullptr;
   const int p = blockIdx.x * G + grp;  // this group's pair
   const int ca = 2 * p;
   const bool gin = grp < G && p < L / 2;
@@ -468,7 +469,7 @@ __global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k(ScanGeom 
   // block-uniform exit when none of the block's rows is valid
   const int rlo = max(ni.lo[1], 2 * blockIdx.x * G), rhi = min(ni.hi[1], 2 * (blockIdx.x * G + G) - 1);
   if (rlo > rhi || ni.lo[0] > ni.hi[0]) {
-    if (out && tid < KT * wavesPerBlock) {
+    if (out && tid < K * wavesPerBlock) {
       Cand w;
       w.key = -INFINITY;
       w.h = 0u;
@@ -540,10 +541,21 @@ __global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k(ScanGeom 
       m[i] = hi;
     }
   }
-  // KT candidates per wavefront (the exact selection needs more than K to
-  // see whether a wavefront holds further positions near the K-th best);
   // tau2 also excludes the -inf of invalid positions
-  const float tau2 = fmaxf(wave_kth_key<KT>(m, KT), -3.402823466e38f);
+  const float tau2 = fmaxf(wave_kth_key<KT>(m, K), -3.402823466e38f);
+  if (gcnt && ni.eps > 0.f && quantum == 0.f) {
+    // exact selection: how many of the wavefront's values lie within 2 eps
+    // of its K-th best (more than K: the node's band may reach beyond the
+    // wavefront's candidates)
+    const float th = (tau2 <= -3.402823466e38f) ? -INFINITY : tau2 - 2.f * ni.eps * 1.0009765625f;
+    int c = 0;
+#pragma unroll
+    for (int e = 0; e < 20; ++e) c += (key[e] >= th) ? 1 : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+    if ((tid & 63) == 0)
+      gcnt[(long long)node * candBlocks + (long long)blockIdx.x * wavesPerBlock + (tid >> 6)] = c;
+  }
   // pass 2: exact device rule among the keys >= tau (normally K per block)
   RankList<KT> rl;
   rl.init();
@@ -555,7 +567,7 @@ __global__ void __launch_bounds__(64 * ((100 * G + 63) / 64)) k_rows1k(ScanGeom 
       rl.insert(key[e], tie_hash(ni.seedbits, pos), pos, (w ? u[q].y : u[q].x) * scale + d0);
     }
   }
-  wave_extract<KT>(rl, KT, out + (threadIdx.x >> 6) * KT);
+  wave_extract<KT>(rl, K, out + (threadIdx.x >> 6) * K);
 }
 
 // ---------------------------------------------------------------------------
@@ -657,8 +669,6 @@ void launch_pencil1k(const ScanGeom& g, const float2* YT, const float2* tw1k, in
                      taps, tapPos, CT, ct1k_stride(g.h0), count, chunks);
 }
 
-int cands_per_group(int K);
-
 // row pairs per rows block: 5 pairs -> 512 threads, 98 % of the lanes busy
 constexpr int kRowsG = 5;
 
@@ -682,24 +692,23 @@ void launch_rows1k_collect(const ScanGeom& g, const float2* tw1k, const NodeInfo
 hipError_t set_rows1k_lds() { return hipSuccess; }
 
 void launch_rows1k(const ScanGeom& g, const float2* tw1k, const NodeInfo* info, const float2* CT, int nNodes, int K,
-                   Cand* cand, float* map, const int* count, hipStream_t st) {
+                   Cand* cand, float* map, const int* count, int* gcnt, hipStream_t st) {
   const int blocks = kRowsBlocks, groups = rows1k_tiles();
   dim3 grid((unsigned)blocks, (unsigned)nNodes);
   dim3 block(64 * ((100 * kRowsG + 63) / 64));
   const long long cs = ct1k_stride(g.h0);
-  const int M = cands_per_group(K);
-  if (M <= 2)
-    hipLaunchKernelGGL((k_rows1k<2, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map,
-                       count);
-  else if (M <= 4)
-    hipLaunchKernelGGL((k_rows1k<4, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map,
-                       count);
-  else if (M <= 8)
-    hipLaunchKernelGGL((k_rows1k<8, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map,
-                       count);
+  if (K <= 2)
+    hipLaunchKernelGGL((k_rows1k<2, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map, count,
+                       gcnt);
+  else if (K <= 4)
+    hipLaunchKernelGGL((k_rows1k<4, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map, count,
+                       gcnt);
+  else if (K <= 8)
+    hipLaunchKernelGGL((k_rows1k<8, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map, count,
+                       gcnt);
   else
     hipLaunchKernelGGL((k_rows1k<16, kRowsG>), grid, block, 0, st, g, tw1k, info, CT, cs, K, cand, groups, map,
-                       count);
+                       count, gcnt);
 }
 
 }  // namespace qsmi

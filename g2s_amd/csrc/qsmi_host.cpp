@@ -837,6 +837,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   }
   const auto tUp = clk::now();
   double stageMs[4] = {0, 0, 0, 0};
+  long long bandStats[4] = {0, 0, 0, 0};
   long long scanned = 0;
   int waves = 0;
   if (!rc) {
@@ -856,7 +857,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     R.dPathK = (const float*)dPathK;
     R.dNodeKer = (const int*)dNodeKer;
     R.dNodeTi = (const int*)dNodeTi;
-    rc = qsmi::run_resident_waves(dev, R, stageMs, &scanned, &waves);
+    rc = qsmi::run_resident_waves(dev, R, stageMs, &scanned, &waves, bandStats);
   }
   if (!rc) {
     hipError_t e = hipMemcpy(di, dDi, SD * nVar * sizeof(float), hipMemcpyDeviceToHost);
@@ -898,6 +899,10 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   rs->lpAuditNodes = lpc[4];
   rs->lpAuditHits = lpc[5];
   rs->lpAuditBestHits = lpc[6];
+  rs->exactNodes = bandStats[0];
+  rs->bandPositions = bandStats[1];
+  rs->secondPassBlocks = bandStats[2];
+  rs->fullRescoreNodes = bandStats[3];
   return QSMI_OK;
 }
 

@@ -49,6 +49,8 @@ struct BandBufs {
   int* qcnt = nullptr;           // [2]: queue items, full-fallback nodes
   int* fullq = nullptr;          // full-fallback nodes
   qsmi::Cand64* part = nullptr;  // full-fallback chunk results
+  qsmi::Cand64* bpart = nullptr; // rescoring chunk results
+  unsigned long long* stats = nullptr;  // exact nodes, listed positions, second-pass blocks, full-fallback nodes
 };
 
 struct qsmi_device {
@@ -115,6 +117,7 @@ struct WaveBufs {
   float2* dC = nullptr;
   Cand* dCand = nullptr;
   Cand* dCandEx = nullptr;    // [node][K] exact K best as candidates (learned proposal)
+  int* dGcnt = nullptr;       // [node][group] values within 2 eps of the group's K-th best
   float* dOutVal = nullptr;
   uint32_t* dOutPos = nullptr;
   uint32_t* dChosen = nullptr;
@@ -164,29 +167,35 @@ int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeI
 int ensure_tap_tables(qsmi_device* dev, int nNodes);
 // crossOff > 0 (cross-measurement form, generic path): node q's count layer
 // is node q + crossOff of C; rowBlocks then covers one row per FFT
+// cand: [node][group][K] (a group is a wavefront of the 1000^2 rows kernel,
+// a block of the generic one); gcnt (optional): [node][group] count of the
+// group's values within 2 eps of its K-th best (exact-selection nodes)
 void launch_rows(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const float2* C, int nNodes,
-                 int rowBlocks, int K, Cand* cand, float* map, const int* count = nullptr, int crossOff = 0);
+                 int rowBlocks, int K, Cand* cand, int* gcnt, float* map, const int* count = nullptr,
+                 int crossOff = 0);
 // merge + draw; when simVals != nullptr also writes the drawn TI values and
 // the index image of each node's cell (NodeInfo::cell)
 // M: candidates per block
 void launch_merge_draw(const ScanGeom& g, const NodeInfo* info, const Cand* cand, int nNodes, int candBlocks,
                        int M, int rowsPerBlock, int K, float kf, float* outVal, uint32_t* outPos, uint32_t* chosen,
                        float* simVals, const float* tiVals, int nVar, uint32_t* indexOut, hipStream_t st);
-// candidates per rows group (wavefront or block) for ceil(k) = K
-int cands_per_group(int K);
 
 // Exact selection (qsmi_band.hip). exact_selection_on: the device's maps
 // take it (no tie quantum, no NaN TI, a rows grid it can track).
 bool exact_selection_on(const qsmi_device* dev);
 int band_grow(BandBufs* b, int nNodes, int blocksPerNode, int K);
 void band_free(BandBufs* b);
+// counters accumulated since the buffers were made: exact-selection nodes,
+// listed (rescored) positions, second-pass rows blocks, full-fallback nodes
+int band_read_stats(const BandBufs* b, long long out[4]);
 // After the rows stage of nNodes nodes (cand from launch_rows, C the
 // spectrum-stage buffer the rows stage read): the K best of every node by the
 // exact selection (nodes with eps == 0: by their FP32 candidates) into
 // outVal/outPos [node][K] (and outCand when given), then the rank draw into
 // chosen / the simulation write when given. nbMax: largest neighbour count.
 // count (device, optional): only nodes [0, *count) (compacted lists).
-int band_select(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const Cand* cand, int nNodes, int K,
+int band_select(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const Cand* cand, const int* gcnt,
+                int nNodes, int K,
                 float kf, const float2* C, const int* nbOff, const float* nbVal, int nbMax, float* outVal,
                 uint32_t* outPos, Cand* outCand, uint32_t* chosen, float* simVals, const float* tiVals, int nVar,
                 uint32_t* indexOut, BandBufs* bb, const int* count = nullptr);
@@ -281,7 +290,10 @@ int nb_search_device(int D, const unsigned* sd, size_t nPath, const unsigned* dC
 
 // stageMs: [0] spectrum (+ dim1 inverse), [1] gather, [2] rows + select,
 // [3] merge + draw + write; filled when R.stageTiming.
-int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMs, long long* scanned, int* waves);
+// bandStats (optional): the exact selection's counters over the run
+// (band_read_stats, summed over the TIs)
+int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMs, long long* scanned, int* waves,
+                       long long* bandStats = nullptr);
 // cross-measurement form: the count layer's coefficients and the TI cells'
 // completeness (forces the generic kernels)
 int set_cross(qsmi_device* dev, const float* alphaX, const uint8_t* centreOk);

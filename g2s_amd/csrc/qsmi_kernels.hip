@@ -374,9 +374,10 @@ __global__ void __launch_bounds__(256) k_rows_select(ScanGeom g, FftPlan plan0, 
                                                      const NodeInfo* __restrict__ info,
                                                      const float2* __restrict__ C, int K, Cand* __restrict__ cand,
                                                      int candBlocks, float* __restrict__ map, int crossOff,
-                                                     const uint8_t* __restrict__ centreOk) {
+                                                     const uint8_t* __restrict__ centreOk, int* __restrict__ gcnt) {
   extern __shared__ float2 smem[];
   __shared__ Cand red[16];
+  __shared__ int cred[16];
   const int L = plan0.L, nvec = 1 << lognv, tid = threadIdx.x, nthr = blockDim.x;
   const int node = blockIdx.y;
   const NodeInfo ni = info[node];
@@ -437,9 +438,36 @@ __global__ void __launch_bounds__(256) k_rows_select(ScanGeom g, FftPlan plan0, 
     }
   }
   if (cand) {
-    // KT candidates per block (the exact selection needs more than K to
-    // see whether a block holds further positions near the K-th best)
-    block_topk<KT>(tk, KT, cand + ((long long)node * candBlocks + blockIdx.x) * KT, red);
+    const Cand kth = block_topk<KT>(tk, K, cand + ((long long)node * candBlocks + blockIdx.x) * K, red);
+    if (gcnt && !cross && ni.eps > 0.f && ni.quantum == 0.f) {
+      // exact selection: how many of the block's values lie within 2 eps of
+      // its K-th best (more than K: the band may reach beyond its candidates)
+      const float th = (kth.pos == 0xFFFFFFFFu) ? -INFINITY : kth.val - 2.f * ni.eps * 1.0009765625f;
+      int c = 0;
+      for (int e = tid; e < L * nvec; e += nthr) {
+        const int c0 = e >> lognv, v = e & (nvec - 1);
+        if (c0 < ni.lo[0] || c0 > ni.hi[0]) continue;
+        const float2 zz = z[e];
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+          const long long q = q0 + 2 * v + w;
+          if (q >= nrv) continue;
+          const int c1 = ni.lo[1] + (int)(q % n1v);
+          const int c2 = ni.lo[2] + (int)(q / n1v);
+          const float val = (w == 0 ? zz.x : zz.y) * scale + d0;
+          if (val >= th && !(ni.exOn && excluded(ni, s0, s1, c0, c1, c2))) ++c;
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+      if ((tid & 63) == 0) cred[tid >> 6] = c;
+      __syncthreads();
+      if (tid == 0) {
+        int tot = 0;
+        for (int i = 0; i < (nthr >> 6); ++i) tot += cred[i];
+        gcnt[(long long)node * candBlocks + blockIdx.x] = tot;
+      }
+    }
   }
 }
 
@@ -676,7 +704,7 @@ template __global__ void k_colfft<+1>(float2*, FftPlan, const float2*, int, long
                                       const NodeInfo*, int, int);
 #define QSMI_INST_K(KT)                                                                                            \
   template __global__ void k_rows_select<KT>(ScanGeom, FftPlan, int, const float2*, const NodeInfo*,              \
-                                             const float2*, int, Cand*, int, float*, int, const uint8_t*);        \
+                                             const float2*, int, Cand*, int, float*, int, const uint8_t*, int*);  \
   template __global__ void k_select_map<KT>(const float*, long long, int, float, uint32_t, Cand*);                \
   template __global__ void k_merge_draw<KT>(ScanGeom, const NodeInfo*, const Cand*, int, int, int, int, float,     \
                                             float*, uint32_t*, uint32_t*, float*, const float*, int, uint32_t*);
@@ -754,19 +782,17 @@ void launch_direct1d(const ScanGeom& g, const float2* Yhat, const float2* tw0, i
 
 static int kt_for(int K) { return K <= 2 ? 2 : K <= 4 ? 4 : K <= 8 ? 8 : 16; }
 
-// candidates per rows block / wavefront group for ceil(k) = K
-int cands_per_group(int K) { return kt_for(K + 1); }
 
 void launch_rows_select(const ScanGeom& g, const FftPlan& plan0, int lognv, const float2* tw0, const NodeInfo* info,
                         const float2* C, int nNodes, int rowBlocks, int K, Cand* cand, float* map, hipStream_t st,
-                        int crossOff, const uint8_t* centreOk) {
+                        int crossOff, const uint8_t* centreOk, int* gcnt) {
   const int nvec = 1 << lognv;
   dim3 grid((unsigned)rowBlocks, (unsigned)nNodes);
   size_t lds = fft_lds_bytes(plan0.L, nvec);
 #define QSMI_ROWS(KT)                                                                                        \
   hipLaunchKernelGGL(k_rows_select<KT>, grid, dim3(256), lds, st, g, plan0, lognv, tw0, info, C, K, cand, rowBlocks, \
-                     map, crossOff, centreOk)
-  switch (cands_per_group(K)) {
+                     map, crossOff, centreOk, gcnt)
+  switch (kt_for(K)) {
     case 2: QSMI_ROWS(2); break;
     case 4: QSMI_ROWS(4); break;
     case 8: QSMI_ROWS(8); break;

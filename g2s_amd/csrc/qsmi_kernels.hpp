@@ -125,7 +125,7 @@ __device__ __forceinline__ bool cand64_better(const Cand64& a, const Cand64& b) 
 constexpr float kMapErrC = 4e-5f;
 
 // Exact selection work space per wave node: positions to rescore.
-constexpr int kBandCap = 2048;
+constexpr int kBandCap = 8192;
 // rows blocks per node the exact selection tracks (larger grids keep the
 // FP32 selection)
 constexpr int kMaxBandBlocks = 8192;

@@ -67,10 +67,12 @@ __device__ __forceinline__ Cand shfl_xor_cand(const Cand& a, int off) {
 }
 
 // Block-wide K best of the per-thread lists, best first, into out[0..K).
-// blockDim.x must be a multiple of 64. Ends with a barrier.
+// blockDim.x must be a multiple of 64. Ends with a barrier. Returns the K-th
+// best (the worst candidate when fewer than K exist) in every thread.
 template <int KT>
-__device__ void block_topk(TopK<KT>& t, int K, Cand* out, Cand* red) {
+__device__ Cand block_topk(TopK<KT>& t, int K, Cand* out, Cand* red) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwaves = blockDim.x >> 6;
+  Cand last = worst_cand();
   for (int it = 0; it < K; ++it) {
     Cand best = t.c[0];
 #pragma unroll
@@ -85,8 +87,10 @@ __device__ void block_topk(TopK<KT>& t, int K, Cand* out, Cand* red) {
       if (cand_better(red[i], w)) w = red[i];
     if (tid == 0) out[it] = w;
     if (t.c[0].pos == w.pos && t.c[0].pos != 0xFFFFFFFFu) t.pop();
+    last = w;
     __syncthreads();
   }
+  return last;
 }
 
 // ---------------------------------------------------------------------------

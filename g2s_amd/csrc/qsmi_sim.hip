@@ -400,7 +400,7 @@ int nb_search_device(int D, const unsigned* sd, size_t nPath, const unsigned* dC
 }
 
 int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsOut, long long* scannedOut,
-                       int* wavesOut) {
+                       int* wavesOut, long long* bandStats) {
   qsmi_ti* t = dev->ti;
   hipStream_t st = dev->stream;
   const int D = t->D;
@@ -558,9 +558,9 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
       const int* exCount = compact ? lp_compact_count(R.lp) : nullptr;
       if ((rc = launch_spectrum(dev, g, nN, infoEx, b->dOff, b->dVal, b->dOrd, b->dC, exCount))) return rc;
       if (ev) (void)hipEventRecord(ev[2], st);
-      launch_rows(dev, g, infoEx, b->dC, nN, rb, K, b->dCand, nullptr, exCount);
+      launch_rows(dev, g, infoEx, b->dC, nN, rb, K, b->dCand, b->dGcnt, nullptr, exCount);
       // the scanned nodes' K best by the exact selection, as candidate lists
-      if ((rc = band_select(dev, g, infoEx, b->dCand, nN, K, R.k, b->dC, b->dOff, b->dVal, R.n, b->dOutVal,
+      if ((rc = band_select(dev, g, infoEx, b->dCand, b->dGcnt, nN, K, R.k, b->dC, b->dOff, b->dVal, R.n, b->dOutVal,
                             b->dOutPos, b->dCandEx, nullptr, nullptr, nullptr, 0, nullptr, &b->band, exCount)))
         return rc;
       lp_combine(R.lp, nN, compact ? b->dInfo : infoEx, b->dCandEx, compact, 1, 0, K, R.lpAudit, st);
@@ -593,8 +593,8 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
           restore();
           return rc;
         }
-        launch_rows(dv, gt, it, xb->dC, nN, rbt, K, xb->dCand, nullptr, nullptr, 0);
-        if ((rc = band_select(dv, gt, it, xb->dCand, nN, K, R.k, xb->dC, b->dOff, b->dVal, R.n,
+        launch_rows(dv, gt, it, xb->dC, nN, rbt, K, xb->dCand, xb->dGcnt, nullptr, nullptr, 0);
+        if ((rc = band_select(dv, gt, it, xb->dCand, xb->dGcnt, nN, K, R.k, xb->dC, b->dOff, b->dVal, R.n,
                               dValAll + tt * tiStride, dPosAll + tt * tiStride, nullptr, nullptr, nullptr, nullptr,
                               R.nVar, nullptr, &xb->band))) {
           restore();
@@ -608,9 +608,9 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
     } else {
       if ((rc = launch_spectrum(dev, g, layers * nN, b->dInfo, b->dOff, b->dVal, b->dOrd, b->dC))) return rc;
       if (ev) (void)hipEventRecord(ev[2], st);
-      launch_rows(dev, g, b->dInfo, b->dC, nN, rb, K, b->dCand, nullptr, nullptr, dev->cross ? nN : 0);
+      launch_rows(dev, g, b->dInfo, b->dC, nN, rb, K, b->dCand, b->dGcnt, nullptr, nullptr, dev->cross ? nN : 0);
       if (ev) (void)hipEventRecord(ev[3], st);
-      if ((rc = band_select(dev, g, b->dInfo, b->dCand, nN, K, R.k, b->dC, b->dOff, b->dVal, R.n, b->dOutVal,
+      if ((rc = band_select(dev, g, b->dInfo, b->dCand, b->dGcnt, nN, K, R.k, b->dC, b->dOff, b->dVal, R.n, b->dOutVal,
                             b->dOutPos, nullptr, nullptr, R.dDi, R.dTi, R.nVar, R.dIndex, &b->band)))
         return rc;
     }
@@ -629,6 +629,22 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
     return fail(QSMI_ERR_HIP, std::string("resident waves: ") + hipGetErrorString(e));
   }
   e = hipStreamSynchronize(st);
+  if (e == hipSuccess && bandStats) {
+    for (int q = 0; q < 4; ++q) bandStats[q] = 0;
+    long long one[4];
+    if ((rc = band_read_stats(&b->band, one))) {
+      restore();
+      return rc;
+    }
+    for (int q = 0; q < 4; ++q) bandStats[q] += one[q];
+    for (qsmi_device* x : R.extraDevs) {
+      if ((rc = band_read_stats(&x->wb->band, one))) {
+        restore();
+        return rc;
+      }
+      for (int q = 0; q < 4; ++q) bandStats[q] += one[q];
+    }
+  }
   restore();
   if (e != hipSuccess) return fail(QSMI_ERR_HIP, std::string("resident waves: ") + hipGetErrorString(e));
   double stage[4] = {0, 0, 0, 0};

@@ -365,6 +365,10 @@ typedef struct {
   long long lpAuditNodes;  /* out: audited proposal nodes */
   long long lpAuditHits;   /* out: exact k-best positions found in the proposals */
   long long lpAuditBestHits; /* out: audited nodes whose exact best position was proposed */
+  long long exactNodes;      /* out: nodes selected by the exact selection (FP64 rescoring) */
+  long long bandPositions;   /* out: positions rescored in FP64 over those nodes */
+  long long secondPassBlocks; /* out: rows blocks transformed again to list their band */
+  long long fullRescoreNodes; /* out: nodes rescored over their whole valid box */
 } qsmi_qs_result;
 
 int qsmi_qs_run_in_memory(const qsmi_qs_request* req, qsmi_qs_result* res);
