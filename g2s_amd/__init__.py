@@ -233,6 +233,14 @@ class AcceleratorDevice:
         check(lib().qsmi_search_k_biggest(self.h, err, pos, k, float(seed)))
         return err, pos
 
+    def search_k_biggest_noalloc(self, k, seed):
+        """search_k_biggest without any device allocation (the library's
+        per-GPU scratch array; calls on one GPU must not overlap)."""
+        err = np.zeros(k, np.float32)
+        pos = np.zeros(k, np.uint32)
+        check(lib().qsmi_search_k_biggest_noalloc(self.h, err, pos, k, float(seed)))
+        return err, pos
+
     def cvt_index_to_position(self, index):
         out = C.c_uint(0)
         check(lib().qsmi_cvt_index_to_position(self.h, int(index), C.byref(out)))

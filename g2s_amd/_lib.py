@@ -35,6 +35,7 @@ SIGNATURES = {
     "qsmi_set_tie_quantum": (C.c_int, [vp, C.c_float]),
     "qsmi_candidate_for_pattern": (C.c_int, [vp, C.c_int, i32p, f32p, f32p, C.POINTER(C.c_int)]),
     "qsmi_search_k_biggest": (C.c_int, [vp, f32p, u32p, C.c_uint, C.c_float]),
+    "qsmi_search_k_biggest_noalloc": (C.c_int, [vp, f32p, u32p, C.c_uint, C.c_float]),
     "qsmi_cvt_index_to_position": (C.c_int, [vp, C.c_uint, C.POINTER(C.c_uint)]),
     "qsmi_cvt_position_to_index": (C.c_int, [vp, C.c_uint, C.POINTER(C.c_uint)]),
     "qsmi_get_array_size": (C.c_int, [vp, C.POINTER(C.c_uint)]),

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -131,7 +132,37 @@ hipError_t dalloc(T** p, size_t count) {
 }  // namespace qsmi
 
 
+// Scratch of qsmi_search_k_biggest_noalloc: one node's candidate lists, its
+// K best, its node record and the exact selection's buffers. Every GPU the
+// library runs on holds one copy until the process ends; its address on a
+// GPU is taken once (hipGetSymbolAddress with that GPU current, in
+// qsmi_ti_create) and used for every access from the host.
+constexpr size_t kScratchBytes = 256 * 1024;
+constexpr int kScratchChunks = 512;  // blocks of the FP32 top-K over the map
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace qsmi {
+
+namespace {
+std::mutex g_scratchMu;
+std::vector<void*> g_scratchPtr;  // per GPU ordinal
+}  // namespace
+
+// The scratch address on GPU `gpu`, which must be the current device.
+int scratch_address(int gpu, void** out) {
+  std::lock_guard<std::mutex> lock(g_scratchMu);
+  if (gpu < 0) return fail(QSMI_ERR_ARG, "negative GPU ordinal");
+  if ((size_t)gpu >= g_scratchPtr.size()) g_scratchPtr.resize((size_t)gpu + 1, nullptr);
+  if (!g_scratchPtr[gpu]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(QSMI_ERR_HIP, std::string("scratch address: ") + hipGetErrorString(e));
+    g_scratchPtr[gpu] = p;
+  }
+  *out = g_scratchPtr[gpu];
+  return QSMI_OK;
+}
 
 ScanGeom geom_of(const qsmi_device* dev) {
   const qsmi_ti* t = dev->ti;
@@ -390,6 +421,11 @@ int qsmi_ti_create(int gpu, int ndim, const unsigned* srcSize, const unsigned* f
   if (e != hipSuccess) {
     delete t;
     return fail(QSMI_ERR_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+  }
+  void* scratchPtr = nullptr;  // once per GPU, with the GPU current
+  if (scratch_address(gpu, &scratchPtr) != QSMI_OK) {
+    delete t;
+    return QSMI_ERR_HIP;
   }
   size_t ldsLimit = 0;
   e = set_lds_limits(gpu, &ldsLimit);
@@ -652,6 +688,65 @@ int qsmi_search_k_biggest(qsmi_device* dev, float* errors, unsigned* encodedPosi
   std::vector<uint32_t> pos(K);
   HIPCHK(hipMemcpyAsync(errors, dev->dOutVal, K * sizeof(float), hipMemcpyDeviceToHost, st));
   HIPCHK(hipMemcpyAsync(pos.data(), dev->dOutPos, K * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  for (int i = 0; i < K; ++i) {
+    if (pos[i] == 0xFFFFFFFFu) {
+      encodedPosition[i] = UINT_MAX;
+      continue;
+    }
+    unsigned idx;
+    qsmi_cvt_position_to_index(dev, pos[i], &idx);
+    encodedPosition[i] = idx;
+  }
+  return QSMI_OK;
+}
+
+int qsmi_search_k_biggest_noalloc(qsmi_device* dev, float* errors, unsigned* encodedPosition, unsigned extendK,
+                                  float seed) {
+  int rc = check_dev(dev);
+  if (rc) return rc;
+  if (!errors || !encodedPosition) return fail(QSMI_ERR_ARG, "null argument");
+  if (extendK < 1 || extendK > (unsigned)kMaxK) return fail(QSMI_ERR_UNSUPPORTED, "extendK must be in [1, 16]");
+  if (!dev->mapValid) return fail(QSMI_ERR_ARG, "no map: call qsmi_candidate_for_pattern first");
+  qsmi_ti* t = dev->ti;
+  const int K = (int)extendK;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t candBytes = al((size_t)kScratchChunks * K * sizeof(Cand));
+  const size_t need = candBytes + 2 * al((size_t)K * 4) + al(sizeof(NodeInfo)) + band_scratch_bytes(K);
+  if (need > kScratchBytes)
+    return fail(QSMI_ERR_UNSUPPORTED, "extendK " + std::to_string(K) + " needs " + std::to_string(need) +
+                                          " bytes of scratch, above the " + std::to_string(kScratchBytes) + " it holds");
+  void* base = nullptr;
+  if ((rc = scratch_address(t->gpu, &base))) return rc;
+  char* p = static_cast<char*>(base);
+  Cand* cand = reinterpret_cast<Cand*>(p);
+  p += candBytes;
+  float* outVal = reinterpret_cast<float*>(p);
+  p += al((size_t)K * 4);
+  uint32_t* outPos = reinterpret_cast<uint32_t*>(p);
+  p += al((size_t)K * 4);
+  NodeInfo* info = reinterpret_cast<NodeInfo*>(p);
+  p += al(sizeof(NodeInfo));
+  BandBufs band;
+  band_view(&band, p, K);
+  hipStream_t st = dev->stream;
+  // the last scanned node with this call's seed (its neighbours stay in
+  // dOff / dVal for the exact selection); the copy completes before return
+  NodeInfo ni = dev->lastNode;
+  ni.seed = seed;
+  ni.seedbits = seed_from_unit_float(seed);
+  ni.quantum = dev->quantum;
+  HIPCHK(hipMemcpyAsync(info, &ni, sizeof(ni), hipMemcpyHostToDevice, st));
+  launch_select_map(dev->dMap, t->S, K, dev->quantum, ni.seedbits, cand, kScratchChunks, st);
+  ScanGeom g = geom_of(dev);
+  launch_merge_draw(g, info, cand, 1, kScratchChunks, K, 0, K, (float)K, outVal, outPos, nullptr, nullptr, nullptr,
+                    0, nullptr, st);
+  if (ni.eps > 0.f && !dev->mapEdited && exact_selection_on(dev))
+    if ((rc = band_select_map(dev, info, ni.nb, K, ni.eps, outVal, outPos, &band))) return rc;
+  HIPCHK(hipGetLastError());
+  uint32_t pos[kMaxK];
+  HIPCHK(hipMemcpyAsync(errors, outVal, K * sizeof(float), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(pos, outPos, K * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
   for (int i = 0; i < K; ++i) {
     if (pos[i] == 0xFFFFFFFFu) {

@@ -484,6 +484,41 @@ int band_grow(BandBufs* b, int nNodes, int blocksPerNode, int K) {
   return QSMI_OK;
 }
 
+// Buffers for one node carved from memory the caller owns (base aligned to
+// 256 bytes): the layout band_scratch_bytes(K) sizes. band_grow then finds
+// them large enough and allocates nothing.
+size_t band_scratch_bytes(int K) {
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  return al(kBandCap * sizeof(uint32_t)) + 4 * al(sizeof(int)) + al(sizeof(float)) + al(sizeof(int2)) +
+         al((size_t)kFullChunks * K * sizeof(Cand64)) + al((size_t)kBandChunks * K * sizeof(Cand64)) +
+         al(4 * sizeof(unsigned long long));
+}
+
+void band_view(BandBufs* b, void* base, int K) {
+  auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  char* p = static_cast<char*>(base);
+  b->pos = reinterpret_cast<uint32_t*>(p);
+  p += al(kBandCap * sizeof(uint32_t));
+  b->cnt = reinterpret_cast<int*>(p);
+  p += al(sizeof(int));
+  b->theta = reinterpret_cast<float*>(p);
+  p += al(sizeof(float));
+  b->queue = reinterpret_cast<int2*>(p);
+  p += al(sizeof(int2));
+  b->qcnt = reinterpret_cast<int*>(p);
+  p += al(sizeof(int));
+  b->fullq = reinterpret_cast<int*>(p);
+  p += al(sizeof(int));
+  b->part = reinterpret_cast<Cand64*>(p);
+  p += al((size_t)kFullChunks * K * sizeof(Cand64));
+  b->bpart = reinterpret_cast<Cand64*>(p);
+  p += al((size_t)kBandChunks * K * sizeof(Cand64));
+  b->stats = reinterpret_cast<unsigned long long*>(p);
+  b->capNodes = 1;
+  b->capQueue = 1;
+  b->K = K;
+}
+
 int band_read_stats(const BandBufs* b, long long out[4]) {
   unsigned long long v[4] = {0, 0, 0, 0};
   if (b->stats && hipMemcpy(v, b->stats, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess)

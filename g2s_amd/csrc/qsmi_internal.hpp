@@ -188,6 +188,10 @@ void band_free(BandBufs* b);
 // counters accumulated since the buffers were made: exact-selection nodes,
 // listed (rescored) positions, second-pass rows blocks, full-fallback nodes
 int band_read_stats(const BandBufs* b, long long out[4]);
+// One node's buffers inside caller memory (no allocation): bytes needed for
+// ceil(k) = K, and the view over base (256-byte aligned)
+size_t band_scratch_bytes(int K);
+void band_view(BandBufs* b, void* base, int K);
 // After the rows stage of nNodes nodes (cand from launch_rows, C the
 // spectrum-stage buffer the rows stage read): the K best of every node by the
 // exact selection (nodes with eps == 0: by their FP32 candidates) into

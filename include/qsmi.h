@@ -127,6 +127,18 @@ int qsmi_candidate_for_pattern(qsmi_device* dev, int nNeighbors, const int* offs
  * qsmi_set_value_in_error_array(_with_radius) the FP32 map is used as is. */
 int qsmi_search_k_biggest(qsmi_device* dev, float* errors, unsigned* encodedPosition, unsigned extendK, float seed);
 
+/* No counterpart in the reference: qsmi_search_k_biggest for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid it). Same arguments and results; it calls no
+ * allocation routine: the candidate lists, the K best and the exact
+ * selection's buffers live in a 256 KiB scratch array the library holds on
+ * every GPU it runs on (enough for every extendK <= 16; larger requests fail
+ * with QSMI_ERR_UNSUPPORTED). Every call on a GPU shares that array, so calls
+ * that overlap on one GPU (several devices' streams, several host threads)
+ * race: serialise them. */
+int qsmi_search_k_biggest_noalloc(qsmi_device* dev, float* errors, unsigned* encodedPosition, unsigned extendK,
+                                  float seed);
+
 /* cvtIndexToPosition / cvtPositionToIndex (include/AcceleratorDevice.hpp:57-58),
  * using the bounding box of the last candidate_for_pattern call. */
 int qsmi_cvt_index_to_position(qsmi_device* dev, unsigned index, unsigned* position);
