@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 256 MiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     ap.add_argument("--wave", type=int, default=512, help="max path nodes per wave (the learned form: 256)")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="scan-kernel launch parameter for the TIs of the run (qsmi_set_default_tuning)")
     ap.add_argument("--cpu-sample", type=int, default=48, help="path nodes of the CPU baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--learned", type=int, default=1,
@@ -217,6 +219,9 @@ def main():
     import g2s_amd
     from g2s_amd import synth
 
+    for kv in args.tune:
+        key, val = kv.split("=")
+        g2s_amd.set_default_tuning(key, int(val))
     cfg = synth.config2_continuous_2d(args.size)
     if args.sim_size:
         cfg["sim_dims"] = (args.sim_size, args.sim_size)

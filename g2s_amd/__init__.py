@@ -30,6 +30,7 @@ __all__ = [
     "kernel_generate",
     "path",
     "TrainingImage",
+    "set_default_tuning",
     "AcceleratorDevice",
     "qs",
     "calibration",
@@ -162,6 +163,15 @@ class TrainingImage:
         self.h = h
         return self
 
+    def set_tuning(self, key, value):
+        """Launch parameter of the scan kernels (qsmi_ti_set_tuning)."""
+        check(lib().qsmi_ti_set_tuning(self.h, key.encode(), int(value)))
+
+    def get_tuning(self, key):
+        v = C.c_int(0)
+        check(lib().qsmi_ti_get_tuning(self.h, key.encode(), C.byref(v)))
+        return v.value
+
     def close(self):
         if getattr(self, "h", None):
             lib().qsmi_ti_destroy(self.h)
@@ -169,6 +179,12 @@ class TrainingImage:
 
     def __del__(self):
         self.close()
+
+
+def set_default_tuning(key, value):
+    """Launch parameter every TI created afterwards starts with
+    (qsmi_set_default_tuning); keys as TrainingImage.set_tuning."""
+    check(lib().qsmi_set_default_tuning(key.encode(), int(value)))
 
 
 def subimages_from_reflected(images, dims, fft_dims):

@@ -29,6 +29,9 @@ SIGNATURES = {
     "qsmi_subimages_from_reflected": (C.c_int, [C.c_int, u32p, u32p, C.c_int, vp, f32p]),
     "qsmi_ti_create_from_reflected": (C.c_int, [C.c_int, C.c_int, u32p, u32p, C.c_int, vp, C.POINTER(vp)]),
     "qsmi_ti_destroy": (C.c_int, [vp]),
+    "qsmi_ti_set_tuning": (C.c_int, [vp, C.c_char_p, C.c_int]),
+    "qsmi_ti_get_tuning": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_int)]),
+    "qsmi_set_default_tuning": (C.c_int, [C.c_char_p, C.c_int]),
     "qsmi_device_create": (C.c_int, [vp, f32p, C.c_int, C.POINTER(vp)]),
     "qsmi_device_destroy": (C.c_int, [vp]),
     "qsmi_set_true_mismatch": (C.c_int, [vp, C.c_int]),

@@ -56,6 +56,16 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
+namespace {
+std::mutex g_tune_mu;
+ScanTune g_default_tune;  // copied into every TI at creation (qsmi_set_default_tuning)
+}  // namespace
+
+ScanTune default_tune() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return g_default_tune;
+}
+
 #define HIPCHK(expr)                                                                                  \
   do {                                                                                                \
     hipError_t _e = (expr);                                                                           \
@@ -293,8 +303,7 @@ int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeI
   if (t->fast1k) {
     int rc = ensure_tap_tables(dev, nNodes);
     if (rc) return rc;
-    // nodes per pencil block: 3 measured best over 1..16 (256-node waves)
-    const int npb = std::max(1, std::min(3, nNodes / 8));
+    const int npb = std::max(1, std::min(t->tune.pencilNpb, nNodes / 8));
     launch_pencil1k(g, t->dYT, t->dTw[0], nNodes, npb, info, off, val, dev->dTap, dev->dTapPos, C, count, st);
   } else if (count) {
     return fail(QSMI_ERR_UNSUPPORTED, "a device node count is supported on the 1000x1000 path only");
@@ -383,6 +392,7 @@ int qsmi_ti_create(int gpu, int ndim, const unsigned* srcSize, const unsigned* f
   if (ndim < 1 || ndim > 3) return fail(QSMI_ERR_UNSUPPORTED, "ndim must be 1, 2 or 3");
   if (nSub < 1 || nSub > kMaxSub) return fail(QSMI_ERR_UNSUPPORTED, "nSub must be in [1, 16]");
   auto* t = new qsmi_ti();
+  t->tune = default_tune();
   t->gpu = gpu;
   t->D = ndim;
   for (int i = 0; i < ndim; ++i) {
@@ -522,6 +532,52 @@ int qsmi_ti_destroy(qsmi_ti* t) {
     if (t->dTw[i]) hipFree(t->dTw[i]);
   if (t->stream) hipStreamDestroy(t->stream);
   delete t;
+  return QSMI_OK;
+}
+
+namespace {
+struct TuneKey {
+  const char* name;
+  int ScanTune::*field;
+  int lo, hi;
+};
+const TuneKey kTuneKeys[] = {
+    {"pencil1k.npb", &ScanTune::pencilNpb, 1, 16},
+};
+const TuneKey* find_tune_key(const char* key) {
+  if (!key) return nullptr;
+  for (const TuneKey& k : kTuneKeys)
+    if (std::strcmp(k.name, key) == 0) return &k;
+  return nullptr;
+}
+}  // namespace
+
+namespace {
+int set_tune_field(ScanTune* tune, const char* key, int value) {
+  const TuneKey* k = find_tune_key(key);
+  if (!k) return fail(QSMI_ERR_ARG, std::string("unknown tuning key: ") + (key ? key : "(null)"));
+  if (value < k->lo || value > k->hi)
+    return fail(QSMI_ERR_ARG, std::string("tuning value out of range for ") + k->name);
+  tune->*(k->field) = value;
+  return QSMI_OK;
+}
+}  // namespace
+
+int qsmi_ti_set_tuning(qsmi_ti* ti, const char* key, int value) {
+  if (!ti) return fail(QSMI_ERR_ARG, "null TI");
+  return set_tune_field(&ti->tune, key, value);
+}
+
+int qsmi_set_default_tuning(const char* key, int value) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return set_tune_field(&g_default_tune, key, value);
+}
+
+int qsmi_ti_get_tuning(const qsmi_ti* ti, const char* key, int* value) {
+  if (!ti || !value) return fail(QSMI_ERR_ARG, "null argument");
+  const TuneKey* k = find_tune_key(key);
+  if (!k) return fail(QSMI_ERR_ARG, std::string("unknown tuning key: ") + (key ? key : "(null)"));
+  *value = ti->tune.*(k->field);
   return QSMI_OK;
 }
 

@@ -108,27 +108,27 @@ __device__ __forceinline__ void dft8(float2* u) {
   }
 }
 
+// Prime-factor (Good-Thomas) 10 = 2 x 5: 2 and 5 are coprime, so with the
+// input map n = (5 n1 + 2 n2) mod 10 and the output map k = (5 k1 + 6 k2)
+// mod 10 the transform is two length-5 DFTs and five 2-point butterflies with
+// no twiddle factors in between: W10^(nk) = W2^(n1 k1) W5^(n2 k2).
 template <int DIR>
 __device__ __forceinline__ void dft10(float2* u) {
-  // W10^k = cos(pi k/5) + DIR*i*sin(pi k/5)
-  const float cs[5] = {1.f, 0.80901699437494742410f, 0.30901699437494742410f, -0.30901699437494742410f,
-                       -0.80901699437494742410f};
-  const float sn[5] = {0.f, 0.58778525229247312917f, 0.95105651629515357212f, 0.95105651629515357212f,
-                       0.58778525229247312917f};
-  float2 e[5] = {u[0], u[2], u[4], u[6], u[8]};
-  float2 o[5] = {u[1], u[3], u[5], u[7], u[9]};
-  dft5<DIR>(e);
-  dft5<DIR>(o);
-#pragma unroll
-  for (int k = 1; k < 5; ++k) {
-    float2 w = make_float2(cs[k], (float)DIR * sn[k]);
-    o[k] = cmul(o[k], w);
-  }
-#pragma unroll
-  for (int k = 0; k < 5; ++k) {
-    u[k] = cadd(e[k], o[k]);
-    u[k + 5] = csub(e[k], o[k]);
-  }
+  float2 a[5] = {u[0], u[2], u[4], u[6], u[8]};  // n1 = 0: n = 2 n2
+  float2 b[5] = {u[5], u[7], u[9], u[1], u[3]};  // n1 = 1: n = 5 + 2 n2 mod 10
+  dft5<DIR>(a);
+  dft5<DIR>(b);
+  // k2 -> outputs (6 k2) mod 10 (k1 = 0) and (5 + 6 k2) mod 10 (k1 = 1)
+  u[0] = cadd(a[0], b[0]);
+  u[5] = csub(a[0], b[0]);
+  u[6] = cadd(a[1], b[1]);
+  u[1] = csub(a[1], b[1]);
+  u[2] = cadd(a[2], b[2]);
+  u[7] = csub(a[2], b[2]);
+  u[8] = cadd(a[3], b[3]);
+  u[3] = csub(a[3], b[3]);
+  u[4] = cadd(a[4], b[4]);
+  u[9] = csub(a[4], b[4]);
 }
 
 // generic odd prime radix through the length-L twiddle table (R | L)

@@ -16,6 +16,12 @@ using qsmi::FftPlan;
 using qsmi::NodeInfo;
 using qsmi::ScanGeom;
 
+// Launch parameters of the scan kernels for one TI geometry: defaults, set
+// by qsmi_ti_set_tuning, or chosen by measurement (qsmi_ti_autotune).
+struct ScanTune {
+  int pencilNpb = 3;  // 1000 x 1000 pencil: nodes per block
+};
+
 struct qsmi_ti {
   int gpu = 0;
   int D = 0;
@@ -34,6 +40,7 @@ struct qsmi_ti {
   int lognv[3] = {0, 0, 0};
   hipStream_t stream = nullptr;
   std::atomic<int> refs{0};  // devices over this TI (created from several host threads at once)
+  ScanTune tune;
 };
 
 struct WaveBufs;
@@ -139,6 +146,7 @@ namespace qsmi {
 
 // error channel (thread-local message + code)
 int fail(int code, const std::string& msg);
+ScanTune default_tune();
 unsigned seed_from_unit_float(float seed);
 
 ScanGeom geom_of(const qsmi_device* dev);

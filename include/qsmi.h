@@ -82,6 +82,18 @@ int qsmi_ti_create_from_reflected(int gpu, int ndim, const unsigned* srcSize, co
 /* Replaces AcceleratorDevice::freeSharedMemory (include/AcceleratorDevice.hpp:47). */
 int qsmi_ti_destroy(qsmi_ti* ti);
 
+/* Launch parameters of the scan kernels over one TI (no counterpart in the
+ * reference, whose FFTW/cuFFT plans choose their own; its planner call is
+ * src/CPUThreadDevice.cpp:99-141). Keys:
+ *   "pencil1k.npb"  path nodes per pencil block on that path: 1..16
+ * Results do not depend on them (every choice sums in the same order).
+ * QSMI_ERR_ARG for an unknown key or a value out of range. */
+int qsmi_ti_set_tuning(qsmi_ti* ti, const char* key, int value);
+int qsmi_ti_get_tuning(const qsmi_ti* ti, const char* key, int* value);
+/* The value every TI created afterwards starts with (also the TIs the
+ * in-memory simulation entry points build). Thread-safe. */
+int qsmi_set_default_tuning(const char* key, int value);
+
 /* Replaces the c_NvidiaGPUAcceleratorDevice factory
  * (src/NvidiaGPUAcceleratorDevice.cu:229-232) for one coefficient layer
  * (no cross-measurement layer): alpha is the OperationMatrix of
