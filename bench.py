@@ -20,7 +20,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-PROFILE = "profiles/k_pencil1k__qs_2d_continuous_1000.md"
+PROFILE = "profiles/k_pencil1024__qs_2d_continuous_1000.md"
 
 
 def measured_bytes_per_node(kernel):
@@ -179,7 +179,7 @@ def calibration_leg(cfg, iters, cpu_sample):
     nb = g2s_amd.default_neighbour_configs((80,))
     g2s_amd.calibration([ti], dims, [0], [ker], [kd], nb_configs=nb[:4], max_iter=1, seed=4)  # warm
     r = g2s_amd.calibration([ti], dims, [0], [ker], [kd], nb_configs=nb, max_iter=iters, seed=5, stage_timing=True)
-    # the dominant kernel is the exact scan's spectrum stage (k_pencil1k):
+    # the dominant kernel is the exact scan's spectrum stage (k_taps1024 + k_pencil1024):
     # the same algorithmic bytes per scanned sample as per path node
     V, S, N = 2, (dims[0] // 2 + 1) * dims[1], dims[0] * dims[1]
     a_node = V * S * 8 + 2 * N * 4
@@ -187,7 +187,7 @@ def calibration_leg(cfg, iters, cpu_sample):
     if r["spectrum_ms"] > 0:
         achieved = a_node * r["distributions"] / (r["spectrum_ms"] / 1e3) / 1e9
         roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
-                    traffic=None, kernel="k_pencil1k", avg_launch_us=r["spectrum_ms"] * 1e3 / max(1, r["waves"]),
+                    traffic=None, kernel="k_pencil1024", avg_launch_us=r["spectrum_ms"] * 1e3 / max(1, r["waves"]),
                     samples_per_launch=r["distributions"] / max(1, r["waves"]), algorithmic_bytes_per_sample=a_node)
     out = dict(value=r["distributions"] / (r["ms"] / 1e3), unit="samples/s", samples=r["distributions"],
                iterations=r["iterations"], waves=r["waves"], ms=r["ms"], scan_ms=r["scan_ms"],
@@ -270,10 +270,10 @@ def main():
     if pencil_ms > 0:
         achieved = a_node * st["scanned_nodes"] / (pencil_ms / 1e3) / 1e9
         nodes_per_launch = st["scanned_nodes"] / max(1, st["waves"])
-        per_node = measured_bytes_per_node("k_pencil1k") if dims == (1000, 1000) else None
+        per_node = measured_bytes_per_node("k_pencil1024") if dims == (1000, 1000) else None
         roof = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s", frac=achieved / HBM_PEAK_GBS,
                     traffic=(per_node * nodes_per_launch if per_node else None),
-                    kernel="k_pencil1k", avg_launch_us=pencil_ms * 1e3 / max(1, st["waves"]),
+                    kernel="k_pencil1024", avg_launch_us=pencil_ms * 1e3 / max(1, st["waves"]),
                     nodes_per_launch=nodes_per_launch, algorithmic_bytes_per_node=a_node,
                     traffic_source=(PROFILE if per_node else None))
     learned = learned_form(cfg, args, ms) if args.learned else None

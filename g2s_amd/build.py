@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqsmi.so")
 
-SOURCES = ["qsmi_kernels.hip", "qsmi_fast1000.hip", "qsmi_band.hip", "qsmi_sim.hip", "qsmi_api.hip", "qsmi_lp.hip",
+SOURCES = ["qsmi_kernels.hip", "qsmi_scan1024.hip", "qsmi_band.hip", "qsmi_sim.hip", "qsmi_api.hip", "qsmi_lp.hip",
            "qsmi_host.cpp", "qsmi_calib.cpp", "qsmi_bgrid.cpp"]
 HEADERS = ["qsmi_fft.hpp", "qsmi_kernels.hpp", "qsmi_internal.hpp", "qsmi_select.hpp"]
 

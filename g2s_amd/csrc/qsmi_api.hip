@@ -34,15 +34,15 @@ void launch_select_map(const float* map, long long S, int K, float quantum, uint
                        hipStream_t st);
 void launch_fill(float* p, long long n, float v, hipStream_t st);
 void launch_fill_disk(float* map, const int* s, uint32_t o, float r2, float v, hipStream_t st);
-long long ct1k_stride(int h0);
-void launch_transpose_spectra(const float2* Y, float2* YT, int h0, int f1, int nA, hipStream_t st);
-void launch_pencil1k(const ScanGeom& g, const float2* YT, const float2* tw1k, int nNodes, int npb,
-                     const NodeInfo* info, const int* nbOff, const float* nbVal, TapInfo* taps, uint16_t* tapPos,
-                     float2* CT, const int* count, hipStream_t st);
-int rows1k_tiles();
-hipError_t set_rows1k_lds();
-void launch_rows1k(const ScanGeom& g, const float2* tw1k, const NodeInfo* info, const float2* CT, int nNodes, int K,
-                   Cand* cand, float* map, const int* count, int* gcnt, hipStream_t st);
+long long ct1024_stride();
+int rows1024_groups();
+hipError_t spectra1024(const float* dSub, int s0, int s1, long long S, int nSub, const double2* twd, float2* YT,
+                       hipStream_t st);
+void launch_pencil1024(const ScanGeom& g, const float2* YT, const float2* tw, int nNodes, const NodeInfo* info,
+                       const int* nbOff, const float* nbVal, TapInfo1024* taps, uint16_t* tapD0, float* tapVal,
+                       float2* CT, const int* count, hipStream_t st);
+void launch_rows1024(const ScanGeom& g, const float2* tw, const NodeInfo* info, const float2* CT, int nNodes, int K,
+                     bool quant, Cand* cand, float* map, const int* count, int* gcnt, hipStream_t st);
 }  // namespace qsmi
 
 using namespace qsmi;
@@ -193,18 +193,34 @@ ScanGeom geom_of(const qsmi_device* dev) {
   return g;
 }
 
+// the device scans on the 1024 x 1024 grid (wavefront transforms)
+bool use_w1k(const qsmi_device* dev) { return dev->ti->w1k && !dev->circular && !dev->cross; }
+
+// cells of the grid the device's transforms run on
+double scan_cells(const qsmi_device* dev) { return use_w1k(dev) ? 1024.0 * 1024.0 : (double)dev->ti->N; }
+
+// map = raw * scale + delta0 for the unnormalised correlation raw on the scan
+// grid: the true mismatch (trueMismatch) or N (E - delta0) with N the cells
+// of the TI's FFT grid (the reference's unnormalised CPU map)
+float map_scale(const qsmi_device* dev) {
+  return (float)((dev->trueMismatch ? 1.0 : (double)dev->ti->N) / scan_cells(dev));
+}
+
 // rows per candidate block; 0: the merge reads every block (tiles write -inf
 // candidates when they hold no valid row)
-int rows_per_block(const qsmi_ti* t) { return t->fast1k ? 0 : (2 << t->lognv[0]); }
+int rows_per_block(const qsmi_device* dev) { return use_w1k(dev) ? 0 : (2 << dev->ti->lognv[0]); }
 
-int row_blocks(const qsmi_ti* t) {
-  if (t->fast1k) return rows1k_tiles();
+int row_blocks(const qsmi_device* dev) {
+  if (use_w1k(dev)) return rows1024_groups();
+  const qsmi_ti* t = dev->ti;
   long long rows = (long long)t->s[1] * t->s[2];
-  return (int)((rows + rows_per_block(t) - 1) / rows_per_block(t));
+  return (int)((rows + rows_per_block(dev) - 1) / rows_per_block(dev));
 }
 
 // spectrum-stage intermediate per node (float2 elements)
-long long c_stride(const qsmi_ti* t) { return t->fast1k ? std::max<long long>(t->H, ct1k_stride(t->h0)) : t->H; }
+long long c_stride(const qsmi_device* dev) {
+  return use_w1k(dev) ? std::max<long long>(dev->ti->H, ct1024_stride()) : dev->ti->H;
+}
 
 // Node preparation on the host: bounding box, valid range, alpha-combined
 // values v_ja = sum_B alpha(A_a, B) u_jB.
@@ -236,7 +252,7 @@ void prep_node(const qsmi_device* dev, int nb, const int* off, const float* u, f
     }
   ni->nbStart = nbStart;
   ni->nb = nb;
-  ni->scale = dev->trueMismatch ? (float)(1.0 / (double)t->N) : 1.f;
+  ni->scale = map_scale(dev);
   ni->delta0 = dev->trueMismatch ? delta0 : 0.f;
   ni->seed = seed;
   ni->seedbits = seed_from_unit_float(seed);
@@ -263,7 +279,7 @@ void prep_node(const qsmi_device* dev, int nb, const int* off, const float* u, f
   if (dev->quantum == 0.f && !dev->cross && nb > 0) {
     double e = 0.0;
     for (int a = 0; a < V; ++a) e += (double)t->rms[dev->active[a]] * sumAbs[a];
-    ni->eps = (float)((double)kMapErrC * (double)ni->scale * (double)t->N * e);
+    ni->eps = (float)((double)kMapErrC * (double)ni->scale * scan_cells(dev) * e);
   }
 }
 
@@ -279,19 +295,25 @@ int ensure(T** p, size_t* cap, size_t need) {
 }
 
 // Launch the spectrum stage and the dim1 inverse (3-D) for nNodes nodes.
-int ensure_tap_tables(qsmi_device* dev, int nNodes) {
-  if (dev->capTap >= (size_t)nNodes) return QSMI_OK;
+int ensure_tap_tables(qsmi_device* dev, int nNodes, int V) {
+  if (dev->capTap >= (size_t)nNodes && dev->capTapV >= V) return QSMI_OK;
   // grown rarely: hipFree waits for queued work that may still read them
   if (dev->dTap) (void)hipFree(dev->dTap);
-  if (dev->dTapPos) (void)hipFree(dev->dTapPos);
+  if (dev->dTapD0) (void)hipFree(dev->dTapD0);
+  if (dev->dTapVal) (void)hipFree(dev->dTapVal);
   dev->dTap = nullptr;
-  dev->dTapPos = nullptr;
+  dev->dTapD0 = nullptr;
+  dev->dTapVal = nullptr;
   dev->capTap = 0;
+  dev->capTapV = 0;
   const size_t cap = std::max<size_t>((size_t)nNodes, 256);
-  if (hipMalloc(&dev->dTap, cap * sizeof(TapInfo)) != hipSuccess ||
-      hipMalloc(&dev->dTapPos, cap * kTapNbMax * sizeof(uint16_t)) != hipSuccess)
+  const int capV = std::max(V, 2);
+  if (hipMalloc(&dev->dTap, cap * sizeof(TapInfo1024)) != hipSuccess ||
+      hipMalloc(&dev->dTapD0, cap * kTapNbMax * sizeof(uint16_t)) != hipSuccess ||
+      hipMalloc(&dev->dTapVal, cap * kTapNbMax * capV * sizeof(float)) != hipSuccess)
     return fail(QSMI_ERR_NOMEM, "tap tables: out of device memory");
   dev->capTap = cap;
+  dev->capTapV = capV;
   return QSMI_OK;
 }
 
@@ -300,19 +322,19 @@ int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeI
   qsmi_ti* t = dev->ti;
   hipStream_t st = dev->stream;
   if (dev->timing) hipEventRecord(dev->ev[0], st);
-  if (t->fast1k) {
-    int rc = ensure_tap_tables(dev, nNodes);
+  if (use_w1k(dev)) {
+    int rc = ensure_tap_tables(dev, nNodes, g.V);
     if (rc) return rc;
-    const int npb = std::max(1, std::min(t->tune.pencilNpb, nNodes / 8));
-    launch_pencil1k(g, t->dYT, t->dTw[0], nNodes, npb, info, off, val, dev->dTap, dev->dTapPos, C, count, st);
+    launch_pencil1024(g, t->dYT, t->dTw1024, nNodes, info, off, val, dev->dTap, dev->dTapD0, dev->dTapVal, C, count,
+                      st);
   } else if (count) {
-    return fail(QSMI_ERR_UNSUPPORTED, "a device node count is supported on the 1000x1000 path only");
+    return fail(QSMI_ERR_UNSUPPORTED, "a device node count is supported on the 1024-grid path only");
   } else if (t->D == 1) {
     launch_direct1d(g, t->dYhat, t->dTw[0], nNodes, info, off, val, C, st);
   } else {
     const int P = t->D - 1;
-    launch_pencil(g, t->plan[P], t->lognv[P], t->dYhat, t->dTw[0], t->dTw[1], t->dTw[P], nNodes, 1, info, off, val,
-                  ord, C, st);
+    launch_pencil(g, t->plan[P], t->lognv[P], t->dYhat, t->dTw[0], t->dTw[1], t->dTw[P], nNodes, t->tune.pencilNpb,
+                  info, off, val, ord, C, st);
   }
   if (dev->timing) hipEventRecord(dev->ev[1], st);
   if (t->D == 3) {
@@ -327,8 +349,8 @@ int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeI
 void launch_rows(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const float2* C, int nNodes, int rowBlocks,
                  int K, Cand* cand, int* gcnt, float* map, const int* count, int crossOff) {
   qsmi_ti* t = dev->ti;
-  if (t->fast1k)
-    launch_rows1k(g, t->dTw[0], info, C, nNodes, K, cand, map, count, gcnt, dev->stream);
+  if (use_w1k(dev))
+    launch_rows1024(g, t->dTw1024, info, C, nNodes, K, dev->quantum > 0.f, cand, map, count, gcnt, dev->stream);
   else
     launch_rows_select(g, t->plan[0], t->lognv[0], t->dTw[0], info, C, nNodes, rowBlocks, K, cand, map, dev->stream,
                        crossOff, dev->dCentreOk, gcnt);
@@ -354,8 +376,7 @@ int set_cross(qsmi_device* dev, const float* alphaX, const uint8_t* centreOk) {
   HIPCHK(hipMemcpy(dev->dCentreOk, centreOk, (size_t)t->S, hipMemcpyHostToDevice));
   if (!complete.empty())
     HIPCHK(hipMemcpy(dev->dComplete, complete.data(), complete.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-  t->fast1k = false;  // the two-layer rows stage exists on the generic path
-  dev->cross = true;
+  dev->cross = true;  // the two-layer rows stage exists on the generic path (use_w1k is false)
   return QSMI_OK;
 }
 
@@ -506,12 +527,24 @@ int qsmi_ti_create(int gpu, int ndim, const unsigned* srcSize, const unsigned* f
                   t->stream);
     TCHK(hipGetLastError());
   }
-  t->fast1k = (ndim == 2 && t->f[0] == 1000 && t->f[1] == 1000);
-  if (t->fast1k) {
-    TCHK(set_rows1k_lds());
-    TCHK(dalloc(&t->dYT, (size_t)nSub * t->H));
-    launch_transpose_spectra(t->dYhat, t->dYT, t->h0, t->f[1], nSub, t->stream);
-    TCHK(hipGetLastError());
+  t->w1k = (ndim == 2 && t->s[0] > 512 && t->s[1] > 512 && t->s[0] <= 1024 && t->s[1] <= 1024);
+  if (t->w1k) {
+    std::vector<float2> tw(1024);
+    std::vector<double2> twd(1024);
+    for (int i = 0; i < 1024; ++i) {
+      const double a = -2.0 * M_PI * (double)i / 1024.0;
+      twd[i] = make_double2(std::cos(a), std::sin(a));
+      tw[i] = make_float2((float)twd[i].x, (float)twd[i].y);
+    }
+    TCHK(dalloc(&t->dTw1024, 1024));
+    TCHK(hipMemcpy(t->dTw1024, tw.data(), 1024 * sizeof(float2), hipMemcpyHostToDevice));
+    double2* dTwd = nullptr;
+    TCHK(hipMalloc(&dTwd, 1024 * sizeof(double2)));
+    TCHK(hipMemcpy(dTwd, twd.data(), 1024 * sizeof(double2), hipMemcpyHostToDevice));
+    TCHK(dalloc(&t->dYT, (size_t)nSub * 513 * 1024));
+    const hipError_t se = spectra1024(t->dSub, t->s[0], t->s[1], t->S, nSub, dTwd, t->dYT, t->stream);
+    (void)hipFree(dTwd);
+    TCHK(se);
   }
   TCHK(hipStreamSynchronize(t->stream));
   (void)hipFree(dReal);
@@ -527,6 +560,7 @@ int qsmi_ti_destroy(qsmi_ti* t) {
   if (t->stream) hipStreamSynchronize(t->stream);
   if (t->dYhat) (void)hipFree(t->dYhat);
   if (t->dYT) (void)hipFree(t->dYT);
+  if (t->dTw1024) (void)hipFree(t->dTw1024);
   if (t->dSub) (void)hipFree(t->dSub);
   for (int i = 0; i < 3; ++i)
     if (t->dTw[i]) hipFree(t->dTw[i]);
@@ -542,7 +576,7 @@ struct TuneKey {
   int lo, hi;
 };
 const TuneKey kTuneKeys[] = {
-    {"pencil1k.npb", &ScanTune::pencilNpb, 1, 16},
+    {"pencil.npb", &ScanTune::pencilNpb, 1, 16},
 };
 const TuneKey* find_tune_key(const char* key) {
   if (!key) return nullptr;
@@ -618,7 +652,8 @@ int qsmi_device_destroy(qsmi_device* d) {
   hipFree(d->dMap);
   hipFree(d->dC);
   hipFree(d->dTap);
-  hipFree(d->dTapPos);
+  hipFree(d->dTapD0);
+  hipFree(d->dTapVal);
   hipFree(d->dCand);
   hipFree(d->dInfo);
   hipFree(d->dOff);
@@ -692,7 +727,7 @@ int qsmi_candidate_for_pattern(qsmi_device* dev, int nb, const int* offsets, con
   if ((rc = ensure(&dev->dOff, &dev->capOff, off.size()))) return rc;
   if ((rc = ensure(&dev->dVal, &dev->capVal, val.size()))) return rc;
   if ((rc = ensure(&dev->dOrd, &dev->capOrd, (size_t)nb))) return rc;
-  if ((rc = ensure(&dev->dC, &dev->capC, (size_t)c_stride(t)))) return rc;
+  if ((rc = ensure(&dev->dC, &dev->capC, (size_t)c_stride(dev)))) return rc;
   hipStream_t st = dev->stream;
   HIPCHK(hipMemcpyAsync(dev->dInfo, &ni, sizeof(ni), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(dev->dOff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, st));
@@ -700,7 +735,7 @@ int qsmi_candidate_for_pattern(qsmi_device* dev, int nb, const int* offsets, con
   launch_fill(dev->dMap, t->S, -INFINITY, st);
   ScanGeom g = geom_of(dev);
   if ((rc = launch_spectrum(dev, g, 1, dev->dInfo, dev->dOff, dev->dVal, dev->dOrd, dev->dC))) return rc;
-  launch_rows(dev, g, dev->dInfo, dev->dC, 1, row_blocks(t), 1, nullptr, nullptr, dev->dMap);
+  launch_rows(dev, g, dev->dInfo, dev->dC, 1, row_blocks(dev), 1, nullptr, nullptr, dev->dMap);
   HIPCHK(hipGetLastError());
   // the host arrays above are pageable and go out of scope: finish the copies
   HIPCHK(hipStreamSynchronize(st));
@@ -905,7 +940,7 @@ void free_wave_bufs(WaveBufs* b) {
 int grow_wave_bufs(qsmi_device* dev, WaveBufs* b, int nNodes, int total, int K) {
   qsmi_ti* t = dev->ti;
   const int V = std::max((int)dev->active.size(), 1);
-  const int rb = row_blocks(t);
+  const int rb = row_blocks(dev);
   if ((size_t)nNodes > b->capNodes || (size_t)nNodes * rb * K > b->capCand) {
     size_t n = std::max<size_t>(nNodes, b->capNodes);
     (void)hipFree(b->dInfo);
@@ -937,11 +972,11 @@ int grow_wave_bufs(qsmi_device* dev, WaveBufs* b, int nNodes, int total, int K) 
     b->capNodes = n;
     b->capCand = n * rb * kMaxK;
   }
-  if ((size_t)nNodes * c_stride(t) > b->capC) {
+  if ((size_t)nNodes * c_stride(dev) > b->capC) {
     (void)hipFree(b->dC);
     b->dC = nullptr;
     b->capC = 0;
-    size_t n = (size_t)nNodes * c_stride(t);
+    size_t n = (size_t)nNodes * c_stride(dev);
     hipError_t e = dalloc(&b->dC, n);
     if (e != hipSuccess) return fail(QSMI_ERR_NOMEM, std::string("wave spectrum buffer: ") + hipGetErrorString(e));
     b->capC = n;
@@ -1013,7 +1048,7 @@ int stage_wave(qsmi_device* dev, WaveBufs* b, int nNodes, const int* nbStart, co
   b->K = K;
   b->k = k;
   b->nbMax = nbMax;
-  b->rowBlocks = row_blocks(t);
+  b->rowBlocks = row_blocks(dev);
   return QSMI_OK;
 }
 

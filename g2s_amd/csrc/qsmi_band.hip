@@ -19,7 +19,7 @@
 //      may hold band positions it did not emit, so its rows block is queued
 //      for a second pass; the candidates of the other groups in the band go
 //      to the node's list.
-//   2. second pass (k_rows1k_collect / k_rows_collect): the queued rows
+//   2. second pass (k_rows1024_collect / k_rows_collect): the queued rows
 //      blocks again, appending every position >= theta.
 //   3. rescoring (k_band_rescore, kBandChunks blocks per node, then
 //      k_band_final): the exact value of every listed position in FP64,
@@ -47,10 +47,8 @@
 
 namespace qsmi {
 
-int rows1k_blocks();
-int rows1k_waves_per_block();
-int rows1k_tiles();
-void launch_rows1k_collect(const ScanGeom& g, const float2* tw1k, const NodeInfo* info, const float2* CT,
+int rows1024_groups();
+void launch_rows1024_collect(const ScanGeom& g, const float2* tw, const NodeInfo* info, const float2* CT,
                            const int2* queue, const int* qcnt, const float* theta, uint32_t* bandPos, int* bandCnt,
                            int maxItems, hipStream_t st);
 void launch_rows_collect(const ScanGeom& g, const FftPlan& plan0, int lognv, const float2* tw0, const NodeInfo* info,
@@ -438,7 +436,7 @@ size_t neighbour_lds(const ScanGeom& g, int nbMax) {
 bool exact_selection_on(const qsmi_device* dev) {
   if (dev->quantum != 0.f || dev->cross) return false;
   const qsmi_ti* t = dev->ti;
-  const int blocks = t->fast1k ? rows1k_blocks() : row_blocks(t);
+  const int blocks = row_blocks(dev);
   return blocks <= kMaxBandBlocks;
 }
 
@@ -554,15 +552,15 @@ int band_select(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const
   qsmi_ti* t = dev->ti;
   hipStream_t st = dev->stream;
   int candGroups, wpb, rpb, blocks;
-  if (t->fast1k) {
-    candGroups = rows1k_tiles();
-    wpb = rows1k_waves_per_block();
-    rpb = 0;
-    blocks = rows1k_blocks();
-  } else {
-    candGroups = row_blocks(t);
+  if (use_w1k(dev)) {  // one candidate group per row-pair wavefront
+    candGroups = rows1024_groups();
     wpb = 1;
-    rpb = rows_per_block(t);
+    rpb = 0;
+    blocks = candGroups;
+  } else {
+    candGroups = row_blocks(dev);
+    wpb = 1;
+    rpb = rows_per_block(dev);
     blocks = candGroups;
   }
   if (dev->cross) {  // one row per transform in the cross-measurement form (no exact selection)
@@ -588,8 +586,8 @@ int band_select(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const
 #undef QSMI_BAND_MERGE
   if (!exact) return QSMI_OK;
   const int maxItems = nNodes * blocks;
-  if (t->fast1k)
-    launch_rows1k_collect(g, t->dTw[0], info, C, bb->queue, bb->qcnt, bb->theta, bb->pos, bb->cnt, maxItems, st);
+  if (use_w1k(dev))
+    launch_rows1024_collect(g, t->dTw1024, info, C, bb->queue, bb->qcnt, bb->theta, bb->pos, bb->cnt, maxItems, st);
   else
     launch_rows_collect(g, t->plan[0], t->lognv[0], t->dTw[0], info, C, bb->queue, bb->qcnt, bb->theta, bb->pos,
                         bb->cnt, maxItems, st);

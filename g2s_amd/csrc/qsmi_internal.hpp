@@ -19,7 +19,7 @@ using qsmi::ScanGeom;
 // Launch parameters of the scan kernels for one TI geometry: defaults, set
 // by qsmi_ti_set_tuning, or chosen by measurement (qsmi_ti_autotune).
 struct ScanTune {
-  int pencilNpb = 3;  // 1000 x 1000 pencil: nodes per block
+  int pencilNpb = 1;  // generic pencil: path nodes per block
 };
 
 struct qsmi_ti {
@@ -31,10 +31,13 @@ struct qsmi_ti {
   long long S = 1, N = 1, H = 1;
   int nSub = 0;
   float2* dYhat = nullptr;
-  float2* dYT = nullptr;  // column-contiguous copy (1000 x 1000 fast path)
+  float2* dYT = nullptr;  // spectra on the 1024 x 1024 scan grid, [A][k0 < 513][k1] (w1k)
   float* dSub = nullptr;  // the sub-images on the TI grid, nSub x S
   float rms[qsmi::kMaxSub] = {};  // root mean square of each sub-image over the FFT grid
-  bool fast1k = false;
+  // 2-D TIs of extent 513..1024: non-circular scans run on a 1024 x 1024
+  // grid, one wavefront per transform (qsmi_scan1024.hip)
+  bool w1k = false;
+  float2* dTw1024 = nullptr;  // exp(-2 pi i t / 1024)
   float2* dTw[3] = {nullptr, nullptr, nullptr};
   FftPlan plan[3];
   int lognv[3] = {0, 0, 0};
@@ -82,9 +85,11 @@ struct qsmi_device {
   // workspace
   float2* dC = nullptr;
   size_t capC = 0;
-  qsmi::TapInfo* dTap = nullptr;  // 1000^2 path: distinct neighbour rows per node
-  uint16_t* dTapPos = nullptr;  // [node][kTapNbMax] slot of each neighbour in tap order
+  qsmi::TapInfo1024* dTap = nullptr;  // 1024-grid path: distinct neighbour rows per node
+  uint16_t* dTapD0 = nullptr;  // [node][kTapNbMax] d0 mod 1024 of each slot of the tap order
+  float* dTapVal = nullptr;    // [node][kTapNbMax][V] values of each slot
   size_t capTap = 0;
+  int capTapV = 0;
   Cand* dCand = nullptr;
   size_t capCand = 0;
   NodeInfo* dInfo = nullptr;
@@ -150,9 +155,12 @@ ScanTune default_tune();
 unsigned seed_from_unit_float(float seed);
 
 ScanGeom geom_of(const qsmi_device* dev);
-int rows_per_block(const qsmi_ti* t);
-int row_blocks(const qsmi_ti* t);
-long long c_stride(const qsmi_ti* t);
+bool use_w1k(const qsmi_device* dev);
+double scan_cells(const qsmi_device* dev);
+float map_scale(const qsmi_device* dev);
+int rows_per_block(const qsmi_device* dev);
+int row_blocks(const qsmi_device* dev);
+long long c_stride(const qsmi_device* dev);
 int grow_wave_bufs(qsmi_device* dev, WaveBufs* b, int nNodes, int total, int K);
 void free_wave_bufs(WaveBufs* b);
 // One wave in three steps (qsmi_scan_wave / qsmi_distribution_wave are
@@ -172,7 +180,7 @@ int read_wave(qsmi_device* dev, WaveBufs* b, float* outErrors, unsigned* outPosi
 int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeInfo* info, const int* off,
                     const float* val, uint32_t* ord, float2* C, const int* count = nullptr);
 // tap tables of the 1000^2 spectrum stage for nNodes nodes per launch
-int ensure_tap_tables(qsmi_device* dev, int nNodes);
+int ensure_tap_tables(qsmi_device* dev, int nNodes, int V);
 // crossOff > 0 (cross-measurement form, generic path): node q's count layer
 // is node q + crossOff of C; rowBlocks then covers one row per FFT
 // cand: [node][group][K] (a group is a wavefront of the 1000^2 rows kernel,

@@ -25,16 +25,19 @@ struct ScanGeom {
   long long H;  // half-spectrum size h0*f1*f2
 };
 
-// Distinct neighbour rows of one node on a 1000-row grid (row = (-d1) mod
-// 1000), ordered by (row mod 10, row), for the sparse column transform.
-// T = 0 when the node has more than kTapMax distinct rows (full FFT used).
+// Distinct neighbour rows ("taps") of one node on the 1024-row scan grid
+// (row = (-d1) mod 1024), ordered by (row mod 16, row), and the node's
+// neighbours in tap order, for the sparse column transform. T = 0 when the
+// node has more than kTapNbMax neighbours (no lists); up to kTapMax taps the
+// column transform is the tap sum, beyond it a full FFT of the column the
+// taps fill.
 constexpr int kTapMax = 48;
-constexpr int kTapNbMax = 500;  // neighbours of a node on the tap path (slots of one 8 KiB LDS column)
-struct TapInfo {
+constexpr int kTapNbMax = 500;
+struct TapInfo1024 {
   int T;
-  int cls[11];              // rows of class c: [cls[c], cls[c+1])
-  int row[kTapMax];
-  int start[kTapMax + 1];   // tap t's neighbours: slots [start[t], start[t+1]) in tap order
+  int cls[17];                     // taps of class c: [cls[c], cls[c+1])
+  uint16_t row[kTapNbMax];
+  uint16_t start[kTapNbMax + 1];   // tap t's neighbours: slots [start[t], start[t+1]) in tap order
 };
 
 // Per path node of a wave. Valid TI centres: lo[i] <= c_i <= hi[i].

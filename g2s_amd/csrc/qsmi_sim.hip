@@ -444,10 +444,10 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   }
   P.circular = dev->circular;
   P.trueMismatch = dev->trueMismatch;
-  P.scale = dev->trueMismatch ? (float)(1.0 / (double)t->N) : 1.f;
+  P.scale = map_scale(dev);
   for (int a = 0; a < P.V; ++a) P.rms[a] = t->rms[dev->active[a]];
   P.epsScale = (dev->quantum == 0.f && !dev->cross)
-                   ? (float)((double)kMapErrC * (double)P.scale * (double)t->N)
+                   ? (float)((double)kMapErrC * (double)P.scale * scan_cells(dev))
                    : 0.f;
   P.quantum = dev->quantum;
   P.tiCells = (unsigned)t->S;
@@ -479,7 +479,7 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   const int layers = dev->cross ? 2 : 1;
   int rc = grow_wave_bufs(dev, b, layers * std::max(maxWave, 1), layers * std::max(maxWave, 1) * R.n, K);
   if (rc) return rc;
-  if (t->fast1k && (rc = ensure_tap_tables(dev, std::max(maxWave, 1)))) return rc;
+  if (use_w1k(dev) && (rc = ensure_tap_tables(dev, std::max(maxWave, 1), geom_of(dev).V))) return rc;
   if (R.lp && (rc = lp_prepare(R.lp, t, std::max(maxWave, 1), K, R.lpShortlist, dev->active.data(),
                                (int)dev->active.size(), dev->circular)))
     return rc;
@@ -493,7 +493,7 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
     for (qsmi_device* x : R.extraDevs) {
       if (!x->wb) x->wb = new WaveBufs();
       if ((rc = grow_wave_bufs(x, x->wb, std::max(maxWave, 1), std::max(maxWave, 1) * R.n, K))) return rc;
-      if (x->ti->fast1k && (rc = ensure_tap_tables(x, std::max(maxWave, 1)))) return rc;
+      if (use_w1k(x) && (rc = ensure_tap_tables(x, std::max(maxWave, 1), geom_of(x).V))) return rc;
     }
     if (hipMalloc(&dValAll, (size_t)nTI * tiStride * sizeof(float)) != hipSuccess ||
         hipMalloc(&dPosAll, (size_t)nTI * tiStride * sizeof(uint32_t)) != hipSuccess) {
@@ -515,8 +515,8 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   };
   ScanGeom g = geom_of(dev);
   // rows stage: the cross form puts one row (both layers) in each FFT
-  const int rpb = dev->cross ? (1 << t->lognv[0]) : rows_per_block(t);
-  const int rb = dev->cross ? (int)(((long long)t->s[1] * t->s[2] + rpb - 1) / rpb) : row_blocks(t);
+  const int rpb = dev->cross ? (1 << t->lognv[0]) : rows_per_block(dev);
+  const int rb = dev->cross ? (int)(((long long)t->s[1] * t->s[2] + rpb - 1) / rpb) : row_blocks(dev);
   const bool savedTiming = dev->timing;
   dev->timing = false;  // the per-call events of launch_spectrum are not used here
   const size_t nWaves = R.waveStart.size() > 0 ? R.waveStart.size() - 1 : 0;
@@ -550,10 +550,10 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
       // learned proposal for every node, the exact scan for the routed ones
       lp_propose(R.lp, t, P.V, nN, b->dInfo, b->dOff, b->dVal, R.dDi, R.nVar, R.sd[0], K, R.lpMinCover, st);
       lp_flags(R.lp, nN, w0, R.dOrder, b->dInfo, K, R.lpEvery, R.lpMse, R.lpMinCover, R.lpAudit, st);
-      // the nodes to scan: compacted to the front on the 1000x1000 path (the
+      // the nodes to scan: compacted to the front on the 1024-grid path (the
       // kernels read the count on the device), else the whole wave with the
       // other nodes' boxes emptied
-      const bool compact = t->fast1k;
+      const bool compact = use_w1k(dev);
       const NodeInfo* infoEx = compact ? lp_info_compact(R.lp) : lp_info_ex(R.lp);
       const int* exCount = compact ? lp_compact_count(R.lp) : nullptr;
       if ((rc = launch_spectrum(dev, g, nN, infoEx, b->dOff, b->dVal, b->dOrd, b->dC, exCount))) return rc;
@@ -579,7 +579,7 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
           // this TI's boxes, map scale and error bound: the bound scales
           // with the TI's sub-image norms (largest ratio over the active
           // sub-variables) and with scale * N
-          const float sc = dv->trueMismatch ? (float)(1.0 / (double)dv->ti->N) : 1.f;
+          const float sc = map_scale(dv);
           double ratio = 0.0;
           for (int A : dev->active)
             ratio = std::max(ratio, t->rms[A] > 0.f ? (double)dv->ti->rms[A] / (double)t->rms[A] : 1e30);
@@ -588,7 +588,7 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
                             dv->ti->s[2] - t->s[2], sc, (float)std::min(ratio, 1e30), st);
           it = xb->dInfo;
         }
-        const int rbt = tt == 0 ? rb : row_blocks(dv->ti);
+        const int rbt = tt == 0 ? rb : row_blocks(dv);
         if ((rc = launch_spectrum(dv, gt, nN, it, b->dOff, b->dVal, xb->dOrd, xb->dC))) {
           restore();
           return rc;

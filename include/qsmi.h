@@ -85,7 +85,8 @@ int qsmi_ti_destroy(qsmi_ti* ti);
 /* Launch parameters of the scan kernels over one TI (no counterpart in the
  * reference, whose FFTW/cuFFT plans choose their own; its planner call is
  * src/CPUThreadDevice.cpp:99-141). Keys:
- *   "pencil1k.npb"  path nodes per pencil block on that path: 1..16
+ *   "pencil.npb"  path nodes per block of the pencil kernel of TIs scanned
+ *                 on their own FFT grid (3-D, smaller than 513 or circular): 1..16
  * Results do not depend on them (every choice sums in the same order).
  * QSMI_ERR_ARG for an unknown key or a value out of range. */
 int qsmi_ti_set_tuning(qsmi_ti* ti, const char* key, int value);
