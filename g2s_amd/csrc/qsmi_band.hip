@@ -117,6 +117,48 @@ __device__ __forceinline__ double exact_value(const ScanGeom& g, const NodeInfo&
   return nscale == 1.0 ? acc : acc * nscale;
 }
 
+// The same value for 2-D, non-circular scans with V = VV <= 2 active
+// sub-variables: the same sum in the same order, with the TI loads of eight
+// neighbours issued ahead of their products (the position is valid, so every
+// p + d_j lies inside the TI; the last batch re-reads neighbour nb - 1).
+template <int VV>
+__device__ __forceinline__ double exact_value_2d(const ScanGeom& g, const NodeInfo& ni, const int* __restrict__ off,
+                                                 const double* __restrict__ v, const float* __restrict__ Y,
+                                                 long long S, uint32_t p, double nscale) {
+  const int s0 = g.s[0];
+  const int c0 = (int)(p % (uint32_t)s0), c1 = (int)(p / (uint32_t)s0);
+  const float* Y0 = Y + (long long)g.A[0] * S;
+  const float* Y1 = Y + (long long)g.A[VV - 1] * S;
+  const int nb = ni.nb;
+  double acc = (double)ni.delta0;
+  for (int j0 = 0; j0 < nb; j0 += 8) {
+    float y0[8], y1[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int j = min(j0 + u, nb - 1);
+      const long long t = (long long)(c0 + off[2 * j]) + (long long)(c1 + off[2 * j + 1]) * s0;
+      y0[u] = Y0[t];
+      if (VV == 2) y1[u] = Y1[t];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (j0 + u < nb) {
+        acc = __dadd_rn(acc, __dmul_rn(v[(j0 + u) * VV], (double)y0[u]));
+        if (VV == 2) acc = __dadd_rn(acc, __dmul_rn(v[(j0 + u) * VV + 1], (double)y1[u]));
+      }
+    }
+  }
+  return nscale == 1.0 ? acc : acc * nscale;
+}
+
+__device__ __forceinline__ double exact_value_any(const ScanGeom& g, const NodeInfo& ni, const int* __restrict__ off,
+                                                  const double* __restrict__ v, const float* __restrict__ Y,
+                                                  long long S, uint32_t p, double nscale) {
+  if (g.D == 2 && !g.circular && g.V == 2) return exact_value_2d<2>(g, ni, off, v, Y, S, p, nscale);
+  if (g.D == 2 && !g.circular && g.V == 1) return exact_value_2d<1>(g, ni, off, v, Y, S, p, nscale);
+  return exact_value(g, ni, off, v, Y, S, p, nscale);
+}
+
 // The node's neighbour offsets and alpha-combined values in LDS:
 // [nb * D] int, then [nb * V] double (8-byte aligned).
 __device__ __forceinline__ void stage_neighbours(const ScanGeom& g, const NodeInfo& ni, const int* __restrict__ nbOff,
@@ -238,7 +280,7 @@ __global__ void __launch_bounds__(256) k_merge_band(const NodeInfo* __restrict__
 // neighbours. Chunk c scores listed positions c*256 + t + j*kBandChunks*256
 // and writes its K best to bpart[node][c].
 template <int KT>
-__global__ void __launch_bounds__(256) k_band_rescore(ScanGeom g, const NodeInfo* __restrict__ info,
+__global__ void __launch_bounds__(64) k_band_rescore(ScanGeom g, const NodeInfo* __restrict__ info,
                                                       const int* __restrict__ nbOff, const float* __restrict__ nbVal,
                                                       const float* __restrict__ Y, long long S, double nscale, int K,
                                                       BandDev bb, const int* __restrict__ count) {
@@ -260,17 +302,17 @@ __global__ void __launch_bounds__(256) k_band_rescore(ScanGeom g, const NodeInfo
   if (chunk == 0 && tid == 0) atomicAdd(&bb.stats[1], (unsigned long long)cnt);
   TopK64<KT> tk;
   tk.init();
-  if (chunk * 256 < cnt) {  // block-uniform
+  if (chunk * 64 < cnt) {  // block-uniform
     const NodeInfo ni = info[node];
     int* so;
     double* sv;
     stage_neighbours(g, ni, nbOff, nbVal, sm, &so, &sv);
     __syncthreads();
     const uint32_t* lst = bb.pos + (long long)node * kBandCap;
-    for (int i = chunk * 256 + tid; i < cnt; i += kBandChunks * 256) {
+    for (int i = chunk * 64 + tid; i < cnt; i += kBandChunks * 64) {
       const uint32_t p = lst[i];
       Cand64 c;
-      c.val = exact_value(g, ni, so, sv, Y, S, p, nscale);
+      c.val = exact_value_any(g, ni, so, sv, Y, S, p, nscale);
       c.h = tie_hash(ni.seedbits, p);
       c.pos = p;
       tk.push(c);
@@ -348,7 +390,7 @@ __global__ void __launch_bounds__(256) k_rescore_full(ScanGeom g, const NodeInfo
       if (ni.exOn && excluded(ni, g.s[0], g.s[1], c0, c1, c2)) continue;
       const uint32_t p = (uint32_t)(c0 + (long long)g.s[0] * (c1 + (long long)g.s[1] * c2));
       Cand64 c;
-      c.val = exact_value(g, ni, so, sv, Y, S, p, nscale);
+      c.val = exact_value_any(g, ni, so, sv, Y, S, p, nscale);
       c.h = tie_hash(ni.seedbits, p);
       c.pos = p;
       tk.push(c);
@@ -533,7 +575,7 @@ void launch_rescore(const ScanGeom& g, const NodeInfo* info, const int* nbOff, c
                     double nscale, int nNodes, int K, float kf, float* outVal, uint32_t* outPos, Cand* outCand,
                     uint32_t* chosen, float* simVals, const float* tiVals, int nVar, uint32_t* indexOut,
                     const BandDev& bd, const int* count, size_t lds, hipStream_t st) {
-  hipLaunchKernelGGL(k_band_rescore<KT>, dim3((unsigned)nNodes, kBandChunks), dim3(256), lds, st, g, info, nbOff,
+  hipLaunchKernelGGL(k_band_rescore<KT>, dim3((unsigned)nNodes, kBandChunks), dim3(64), lds, st, g, info, nbOff,
                      nbVal, t->dSub, t->S, nscale, K, bd, count);
   hipLaunchKernelGGL(k_band_final<KT>, dim3((unsigned)nNodes), dim3(256), 0, st, info, K, kf, outVal, outPos, outCand,
                      chosen, simVals, tiVals, nVar, indexOut, bd, count);

@@ -41,7 +41,8 @@ constexpr int F = 1024;
 constexpr int H0 = 513;             // half spectrum along dim0
 constexpr int NXCD = 8;
 constexpr int BUF = 1088;           // padded exchange buffer (float2)
-constexpr int GSLOT = 2 * kTapMax;  // tap sums of two sub-variables
+constexpr int kTapPass = 3;        // taps per class on the tap path
+constexpr int GSLOT = 2 * 16 * kTapPass;  // tap sums of two sub-variables per slot (k, c)
 // LDS float2 per pencil wavefront; = 2 mod 16 so that the wavefronts'
 // buffers start 4 banks apart (the block's transposed store reads the same
 // offset of every buffer)
@@ -303,6 +304,9 @@ __global__ void __launch_bounds__(64 * NWP) k_pencil1024(
     const int* __restrict__ count) {
   __shared__ float2 twl[F];
   __shared__ float2 wbuf[NWP][WSTRIDE];
+  // the node's tap twiddles W(t, l), shared by the block's columns, at slot
+  // (k, c) of the k-th tap of class c: row (16 k + c) of 64 lanes
+  __shared__ float2 wtab[kTapPass * 16 * 64];
   load_twiddles(twl, tw);
   const int l = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform (scalar) from here on
@@ -329,18 +333,32 @@ __global__ void __launch_bounds__(64 * NWP) k_pencil1024(
     int lx = l;
     asm volatile("" : "+v"(lx));
     lx &= 63;  // the range is known again
+    const TapInfo1024* tp = taps + node;
+    const int T = tp->T;  // block-uniform
+    // the tap path: at most kTapMax taps and kTapPass of them in any class
+    int kmax = 0;
+    if (T > 0 && T <= kTapMax) {
+#pragma unroll
+      for (int c = 0; c < 16; ++c) kmax = max(kmax, tp->cls[c + 1] - tp->cls[c]);
+    }
+    const bool tapPath = kmax > 0 && kmax <= kTapPass;  // block-uniform
+    if (tapPath) {
+      // W(t, l) = exp(-2 pi i l r_t / 1024): the same for every column of the node
+      for (int i = threadIdx.x; i < T * 64; i += 64 * NWP) {
+        const int t = i >> 6, r = tp->row[t], c = r & 15;
+        wtab[((t - tp->cls[c]) * 16 + c) * 64 + (i & 63)] = twl[__umul24((uint32_t)(i & 63), (uint32_t)r) & (F - 1)];
+      }
+      __syncthreads();
+    }
     if (colOk) {
       float2 M[16];
 #pragma unroll
       for (int m = 0; m < 16; ++m) M[m] = make_float2(0.f, 0.f);
-      const TapInfo1024* tp = taps + node;
-      const int T = tp->T;
       const int nb = ni.nb;
-      if (T > 0 && T <= kTapMax) {
+      if (tapPath) {
         const uint16_t* td = tapD0 + (long long)node * kTapNbMax;
         const float* tv = tapVal + (long long)node * kTapNbMax * V;
-        // tap t's row lives in lane t (T <= kTapMax < 64), read by readlane
-        const int myRow = lx < T ? (int)tp->row[lx] : 0;
+        const float2* wl = wtab + lx;
         for (int a0 = 0; a0 < V; a0 += 2) {
           const bool two = a0 + 1 < V;
           // every slot's contribution v exp(+2 pi i k0 d0 / 1024) into buf ...
@@ -352,6 +370,8 @@ __global__ void __launch_bounds__(64 * NWP) k_pencil1024(
             buf[2 * s] = make_float2(v0 * wv.x, -v0 * wv.y);
             buf[2 * s + 1] = make_float2(v1 * wv.x, -v1 * wv.y);
           }
+          // the first pass's slots of classes without taps add zero
+          if (lx < 32) gbuf[lx] = make_float2(0.f, 0.f);
           wave_sync();
           // ... then lane t sums tap t's slots in list order
           if (lx < T) {
@@ -365,39 +385,72 @@ __global__ void __launch_bounds__(64 * NWP) k_pencil1024(
               g1.x += c1.x;
               g1.y += c1.y;
             }
-            gbuf[2 * lx] = g0;
-            gbuf[2 * lx + 1] = g1;
+            const int c = tp->row[lx] & 15;  // slot (k, c) of tap lx
+            const int sl = (lx - tp->cls[c]) * 16 + c;
+            gbuf[2 * sl] = g0;
+            gbuf[2 * sl + 1] = g1;
           }
           wave_sync();
-          for (int h = 0; h < (two ? 2 : 1); ++h) {
-            // this sub-variable's TI spectrum column, loaded before the tap MACs
-            const float2* Yh = YT + ((long long)g.A[a0 + h] * H0 + k0) * F + lx;
-            float2 y[16];
+          // both sub-variables' class sums share each twiddle read; the
+          // first TI spectrum column is loaded before the tap MACs
+          const float2* Y0 = YT + ((long long)g.A[a0] * H0 + k0) * F + lx;
+          float2 y[16];
   #pragma unroll
-            for (int m = 0; m < 16; ++m) y[m] = Yh[64 * m];
-            float2 S[16];
+          for (int m = 0; m < 16; ++m) y[m] = Y0[64 * m];
+          float2 S0[16], S1[16];
+  #pragma unroll
+          for (int c = 0; c < 16; ++c) {
+            S0[c] = make_float2(0.f, 0.f);
+            S1[c] = make_float2(0.f, 0.f);
+          }
+          const float4* G4 = reinterpret_cast<const float4*>(gbuf);  // slot (k, c): (G0, G1)
+          int cn[16];  // taps per class (uniform)
+  #pragma unroll
+          for (int c = 0; c < 16; ++c) cn[c] = tp->cls[c + 1] - tp->cls[c];
+          // pass k adds the k-th tap of every class: classes are independent
+          // accumulators, so a pass's twiddle and tap-sum reads are issued
+          // together (constant offsets from one base); each class still adds
+          // its taps in order
+          for (int k = 0; k < kmax; ++k) {
+            const float2* wk = wl + k * 16 * 64;
+            const float4* gk = G4 + k * 16;
   #pragma unroll
             for (int c = 0; c < 16; ++c) {
-              float2 sc = make_float2(0.f, 0.f);
-              const int te = tp->cls[c + 1];  // uniform
-  #pragma unroll 2
-              for (int t = tp->cls[c]; t < te; ++t) {
-                const float2 wv = twl[__umul24((uint32_t)lx, (uint32_t)__builtin_amdgcn_readlane(myRow, t)) & (F - 1)];
-                const float2 ga = gbuf[2 * t + h];
-                sc.x = fmaf(ga.x, wv.x, sc.x);
-                sc.x = fmaf(-ga.y, wv.y, sc.x);
-                sc.y = fmaf(ga.x, wv.y, sc.y);
-                sc.y = fmaf(ga.y, wv.x, sc.y);
+              // pass 0 takes every class (branch-free, its reads issued
+              // together; empty slots hold G = 0 and an unused finite W)
+              if (k == 0 || k < cn[c]) {  // uniform
+                const float2 wv = wk[c * 64];
+                const float4 gg = gk[c];
+                S0[c].x = fmaf(gg.x, wv.x, S0[c].x);
+                S0[c].x = fmaf(-gg.y, wv.y, S0[c].x);
+                S0[c].y = fmaf(gg.x, wv.y, S0[c].y);
+                S0[c].y = fmaf(gg.y, wv.x, S0[c].y);
+                S1[c].x = fmaf(gg.z, wv.x, S1[c].x);
+                S1[c].x = fmaf(-gg.w, wv.y, S1[c].x);
+                S1[c].y = fmaf(gg.z, wv.y, S1[c].y);
+                S1[c].y = fmaf(gg.w, wv.x, S1[c].y);
               }
-              S[c] = sc;
             }
-            dft16<-1>(S);
+          }
+          dft16<-1>(S0);
+  #pragma unroll
+          for (int m = 0; m < 16; ++m) {
+            M[m].x = fmaf(y[m].x, S0[m].x, M[m].x);
+            M[m].x = fmaf(-y[m].y, S0[m].y, M[m].x);
+            M[m].y = fmaf(y[m].x, S0[m].y, M[m].y);
+            M[m].y = fmaf(y[m].y, S0[m].x, M[m].y);
+          }
+          if (two) {
+            const float2* Y1 = YT + ((long long)g.A[a0 + 1] * H0 + k0) * F + lx;
+  #pragma unroll
+            for (int m = 0; m < 16; ++m) y[m] = Y1[64 * m];
+            dft16<-1>(S1);
   #pragma unroll
             for (int m = 0; m < 16; ++m) {
-              M[m].x = fmaf(y[m].x, S[m].x, M[m].x);
-              M[m].x = fmaf(-y[m].y, S[m].y, M[m].x);
-              M[m].y = fmaf(y[m].x, S[m].y, M[m].y);
-              M[m].y = fmaf(y[m].y, S[m].x, M[m].y);
+              M[m].x = fmaf(y[m].x, S1[m].x, M[m].x);
+              M[m].x = fmaf(-y[m].y, S1[m].y, M[m].x);
+              M[m].y = fmaf(y[m].x, S1[m].y, M[m].y);
+              M[m].y = fmaf(y[m].y, S1[m].x, M[m].y);
             }
           }
         }
