@@ -32,12 +32,15 @@ struct ScanGeom {
 // column transform is the tap sum, beyond it a full FFT of the column the
 // taps fill.
 constexpr int kTapMax = 48;
+constexpr int kTapPass = 3;      // most taps of one class on the tap path
 constexpr int kTapNbMax = 500;
 struct TapInfo1024 {
   int T;
+  int passes;                      // tap path: the most taps of one class (<= kTapPass); 0: full transform
   int cls[17];                     // taps of class c: [cls[c], cls[c+1])
   uint16_t row[kTapNbMax];
   uint16_t start[kTapNbMax + 1];   // tap t's neighbours: slots [start[t], start[t+1]) in tap order
+  uint8_t slot[kTapMax];           // tap path: tap t is the k-th of class c: slot 16 k + c
 };
 
 // Per path node of a wave. Valid TI centres: lo[i] <= c_i <= hi[i].

@@ -41,7 +41,6 @@ constexpr int F = 1024;
 constexpr int H0 = 513;             // half spectrum along dim0
 constexpr int NXCD = 8;
 constexpr int BUF = 1088;           // padded exchange buffer (float2)
-constexpr int kTapPass = 3;        // taps per class on the tap path
 constexpr int GSLOT = 2 * 16 * kTapPass;  // tap sums of two sub-variables per slot (k, c)
 // LDS float2 per pencil wavefront; = 2 mod 16 so that the wavefronts'
 // buffers start 4 banks apart (the block's transposed store reads the same
@@ -226,14 +225,17 @@ __global__ void __launch_bounds__(64) k_taps1024(const NodeInfo* __restrict__ in
   const NodeInfo ni = info[node];
   TapInfo1024* tp = taps + node;
   if (ni.nb > kTapNbMax || ni.nb <= 0) {  // block-uniform
-    if (lane == 0) tp->T = 0;
+    if (lane == 0) {
+      tp->T = 0;
+      tp->passes = 0;
+    }
     return;
   }
   for (int i = lane; i < F; i += 64) flag[i] = 0;
   __syncthreads();
   for (int t = lane; t < ni.nb; t += 64) flag[(-nbOff[(ni.nbStart + t) * 2 + 1]) & (F - 1)] = 1;
   __syncthreads();
-  int count = 0;
+  int count = 0, kmax = 0;
   if (lane == 0) tp->cls[0] = 0;
   for (int c = 0; c < 16; ++c) {
     const int row = c + 16 * lane;  // the 64 rows of class c
@@ -243,12 +245,18 @@ __global__ void __launch_bounds__(64) k_taps1024(const NodeInfo* __restrict__ in
     if (present) {  // count + before < nb <= kTapNbMax
       tp->row[count + before] = (uint16_t)row;
       tix[row] = (short)(count + before);
+      // tap path slot: the before-th tap of class c
+      if (count + before < kTapMax && before < kTapPass) tp->slot[count + before] = (uint8_t)(16 * before + c);
     }
     count += __popcll(mask);
+    kmax = max(kmax, __popcll(mask));
     if (lane == 0) tp->cls[c + 1] = count;
   }
   const int T = count;
-  if (lane == 0) tp->T = T;
+  if (lane == 0) {
+    tp->T = T;
+    tp->passes = (T <= kTapMax && kmax <= kTapPass) ? kmax : 0;
+  }
   for (int t = lane; t < T; t += 64) cntT[t] = 0;
   __syncthreads();
   for (int t = lane; t < ni.nb; t += 64) {
@@ -324,9 +332,6 @@ __global__ void __launch_bounds__(64 * NWP) k_pencil1024(
     const int node = (int)(it / ngx);
     const int cg = xcd + NXCD * (int)(it - (long long)node * ngx);
     const int k0 = cg * NWP + w;
-    const NodeInfo ni = info[node];
-    if (ni.lo[1] > ni.hi[1] || ni.lo[0] > ni.hi[0]) continue;  // block-uniform
-    const bool colOk = k0 < H0;  // wave-uniform; idle waves still take the barriers
     // the lane index, opaque to the compiler per item: lane-dependent
     // addresses are recomputed per item instead of held (spilled) across
     // the item loop
@@ -334,19 +339,29 @@ __global__ void __launch_bounds__(64 * NWP) k_pencil1024(
     asm volatile("" : "+v"(lx));
     lx &= 63;  // the range is known again
     const TapInfo1024* tp = taps + node;
-    const int T = tp->T;  // block-uniform
-    // the tap path: at most kTapMax taps and kTapPass of them in any class
-    int kmax = 0;
-    if (T > 0 && T <= kTapMax) {
-#pragma unroll
-      for (int c = 0; c < 16; ++c) kmax = max(kmax, tp->cls[c + 1] - tp->cls[c]);
-    }
-    const bool tapPath = kmax > 0 && kmax <= kTapPass;  // block-uniform
+    const uint16_t* td = tapD0 + (long long)node * kTapNbMax;
+    const float* tv = tapVal + (long long)node * kTapNbMax * V;
+    // one round of loads for the item: every address below is known without
+    // the others' values (lists are read in full up to 128 slots and 64
+    // taps; unused entries are ignored)
+    const NodeInfo ni = info[node];
+    const int T = tp->T;        // block-uniform
+    const int kmax = tp->passes;
+    const int myRow = tp->row[lx];
+    const int mySlot = tp->slot[lx < kTapMax ? lx : 0];
+    const int myS0 = tp->start[lx], myS1 = tp->start[lx + 1];
+    const uint32_t sdA = td[lx], sdB = td[lx + 64];
+    const float svA0 = tv[lx * V], svA1 = tv[lx * V + (V > 1 ? 1 : 0)];
+    const float svB0 = tv[(lx + 64) * V], svB1 = tv[(lx + 64) * V + (V > 1 ? 1 : 0)];
+    if (ni.lo[1] > ni.hi[1] || ni.lo[0] > ni.hi[0]) continue;  // block-uniform
+    const bool colOk = k0 < H0;  // wave-uniform; idle waves still take the barriers
+    const bool tapPath = kmax > 0;  // block-uniform
     if (tapPath) {
-      // W(t, l) = exp(-2 pi i l r_t / 1024): the same for every column of the node
-      for (int i = threadIdx.x; i < T * 64; i += 64 * NWP) {
-        const int t = i >> 6, r = tp->row[t], c = r & 15;
-        wtab[((t - tp->cls[c]) * 16 + c) * 64 + (i & 63)] = twl[__umul24((uint32_t)(i & 63), (uint32_t)r) & (F - 1)];
+      // W(t, l) = exp(-2 pi i l r_t / 1024), the same for every column of the
+      // node: wave w builds the slots of taps w, w + NWP, ...
+      for (int t = w; t < T; t += NWP) {
+        const int r = __builtin_amdgcn_readlane(myRow, t), sl = __builtin_amdgcn_readlane(mySlot, t);
+        wtab[sl * 64 + lx] = twl[__umul24((uint32_t)lx, (uint32_t)r) & (F - 1)];
       }
       __syncthreads();
     }
@@ -356,17 +371,26 @@ __global__ void __launch_bounds__(64 * NWP) k_pencil1024(
       for (int m = 0; m < 16; ++m) M[m] = make_float2(0.f, 0.f);
       const int nb = ni.nb;
       if (tapPath) {
-        const uint16_t* td = tapD0 + (long long)node * kTapNbMax;
-        const float* tv = tapVal + (long long)node * kTapNbMax * V;
         const float2* wl = wtab + lx;
         for (int a0 = 0; a0 < V; a0 += 2) {
           const bool two = a0 + 1 < V;
           // every slot's contribution v exp(+2 pi i k0 d0 / 1024) into buf ...
           wave_sync();
           for (int s = lx; s < nb; s += 64) {
-            const float2 wv = twl[__umul24((uint32_t)k0, (uint32_t)td[s]) & (F - 1)];
-            const float v0 = tv[(long long)s * V + a0];
-            const float v1 = two ? tv[(long long)s * V + a0 + 1] : 0.f;
+            uint32_t d0m;
+            float v0, v1;
+            if (a0 == 0 && s < 128) {  // the prefetched slots
+              const bool A = s < 64;
+              d0m = A ? sdA : sdB;
+              v0 = A ? svA0 : svB0;
+              v1 = A ? svA1 : svB1;
+            } else {
+              d0m = td[s];
+              v0 = tv[(long long)s * V + a0];
+              v1 = tv[(long long)s * V + a0 + (two ? 1 : 0)];
+            }
+            if (!two) v1 = 0.f;
+            const float2 wv = twl[__umul24((uint32_t)k0, d0m) & (F - 1)];
             buf[2 * s] = make_float2(v0 * wv.x, -v0 * wv.y);
             buf[2 * s + 1] = make_float2(v1 * wv.x, -v1 * wv.y);
           }
@@ -376,19 +400,16 @@ __global__ void __launch_bounds__(64 * NWP) k_pencil1024(
           // ... then lane t sums tap t's slots in list order
           if (lx < T) {
             float2 g0 = make_float2(0.f, 0.f), g1 = make_float2(0.f, 0.f);
-            const int e1 = tp->start[lx + 1];
   #pragma unroll 4
-            for (int s = tp->start[lx]; s < e1; ++s) {
+            for (int s = myS0; s < myS1; ++s) {
               const float2 c0 = buf[2 * s], c1 = buf[2 * s + 1];
               g0.x += c0.x;
               g0.y += c0.y;
               g1.x += c1.x;
               g1.y += c1.y;
             }
-            const int c = tp->row[lx] & 15;  // slot (k, c) of tap lx
-            const int sl = (lx - tp->cls[c]) * 16 + c;
-            gbuf[2 * sl] = g0;
-            gbuf[2 * sl + 1] = g1;
+            gbuf[2 * mySlot] = g0;
+            gbuf[2 * mySlot + 1] = g1;
           }
           wave_sync();
           // both sub-variables' class sums share each twiddle read; the
@@ -455,10 +476,8 @@ __global__ void __launch_bounds__(64 * NWP) k_pencil1024(
           }
         }
       } else if (T > 0) {
-        // more than kTapMax rows: per sub-variable, the row sums scattered
-        // into a zeroed column, then a full forward transform
-        const uint16_t* td = tapD0 + (long long)node * kTapNbMax;
-        const float* tv = tapVal + (long long)node * kTapNbMax * V;
+        // more rows (or more than kTapPass in a class): per sub-variable, the
+        // row sums scattered into a zeroed column, then a full forward transform
         for (int a = 0; a < V; ++a) {
           wave_sync();
           for (int s = lx; s < nb; s += 64) {
