@@ -60,7 +60,8 @@ namespace {
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr int kFullChunks = 64;  // blocks per node of the full fallback
 constexpr int kFullSlots = 8;    // nodes of the full fallback in flight
-constexpr int kBandChunks = 8;   // rescoring blocks per node (listed positions interleaved)
+constexpr int kBandChunks = 32;  // rescoring wavefronts per node (listed positions interleaved; a
+                                 // crowded band's node does not hold back the launch)
 
 // Device view of the band buffers.
 struct BandDev {
@@ -276,9 +277,9 @@ __global__ void __launch_bounds__(256) k_merge_band(const NodeInfo* __restrict__
   }
 }
 
-// Step 3a. grid (node, kBandChunks), 256 threads; dynamic LDS: the node's
-// neighbours. Chunk c scores listed positions c*256 + t + j*kBandChunks*256
-// and writes its K best to bpart[node][c].
+// Step 3a. grid (node, kBandChunks), one wavefront each; dynamic LDS: the
+// node's neighbours. Chunk c scores listed positions c*64 + t +
+// j*kBandChunks*64 and writes its K best to bpart[node][c].
 template <int KT>
 __global__ void __launch_bounds__(64) k_band_rescore(ScanGeom g, const NodeInfo* __restrict__ info,
                                                       const int* __restrict__ nbOff, const float* __restrict__ nbVal,
