@@ -148,7 +148,7 @@ hipError_t dalloc(T** p, size_t count) {
 // GPU is taken once (hipGetSymbolAddress with that GPU current, in
 // qsmi_ti_create) and used for every access from the host.
 constexpr size_t kScratchBytes = 256 * 1024;
-constexpr int kScratchChunks = 512;  // blocks of the FP32 top-K over the map
+constexpr int kScratchChunks = 256;  // blocks of the FP32 top-K over the map
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace qsmi {

@@ -21,15 +21,16 @@
 //      to the node's list.
 //   2. second pass (k_rows1024_collect / k_rows_collect): the queued rows
 //      blocks again, appending every position >= theta.
-//   3. rescoring (k_band_rescore, kBandChunks blocks per node, then
-//      k_band_final): the exact value of every listed position in FP64,
-//      E = delta0 + sum_j sum_a v_ja Y_a(p + d_j) summed in that order with
-//      separately rounded products and sums (no fused multiply-add), and the
-//      K best by the device rule on those values, then the rank draw and the
-//      simulation write.
-//   4. full fallback (k_rescore_full + k_rescore_full_merge): a node whose
-//      band overflows the list capacity is rescored over its whole valid
-//      box in FP64.
+//   3. rescoring (k_band_rescore): the exact value of every listed
+//      position in FP64, E = delta0 + sum_j sum_a v_ja Y_a(p + d_j) summed in
+//      that order with separately rounded products and sums (no fused
+//      multiply-add). The lists of all nodes of the call are cut into tiles
+//      of 64 positions spread over one grid, so a crowded node's list is
+//      scored by as many wavefronts as it has tiles. A node whose band
+//      overflows the list capacity (full fallback) is rescored over its whole
+//      valid box instead, in kFullTiles tiles that keep their K best.
+//   4. k_band_final: per node, the K best by the device rule on the FP64
+//      values, then the rank draw and the simulation write.
 // Nodes with eps == 0 (categorical-only TIs with a tie quantum, TIs with
 // NaN) keep the FP32 selection and finish in step 1.
 //
@@ -58,10 +59,8 @@ void launch_rows_collect(const ScanGeom& g, const FftPlan& plan0, int lognv, con
 namespace {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
-constexpr int kFullChunks = 64;  // blocks per node of the full fallback
-constexpr int kFullSlots = 8;    // nodes of the full fallback in flight
-constexpr int kBandChunks = 32;  // rescoring wavefronts per node (listed positions interleaved; a
-                                 // crowded band's node does not hold back the launch)
+constexpr int kFullTiles = 128;  // rescoring tiles of a full-fallback node (slices of its valid box)
+constexpr int kSegNodes = 2048;  // nodes per rescoring launch (their tile prefix sums are held in LDS)
 
 // Device view of the band buffers.
 struct BandDev {
@@ -69,10 +68,9 @@ struct BandDev {
   int* cnt;        // [node]: listed positions (-1: node finished by the merge)
   float* theta;    // [node]
   int2* queue;     // (node, rows block) items of the second pass
-  int* qcnt;       // [0] queue items, [1] nodes of the full fallback
-  int* fullq;      // [item] node
-  Cand64* part;    // [item][kFullChunks][K]
-  Cand64* bpart;   // [node][kBandChunks][K] rescoring partial results
+  int* qcnt;       // [0] queue items (zero between calls)
+  double* val;     // [node][kBandCap] FP64 values of the listed positions
+  Cand64* part;    // [node][kFullTiles][K] full-fallback tile results
   unsigned long long* stats;  // exact nodes, listed positions, second-pass blocks, full-fallback nodes
 };
 
@@ -83,9 +81,8 @@ BandDev dev_view(const BandBufs& b) {
   d.theta = b.theta;
   d.queue = b.queue;
   d.qcnt = b.qcnt;
-  d.fullq = b.fullq;
+  d.val = b.val;
   d.part = b.part;
-  d.bpart = b.bpart;
   d.stats = b.stats;
   return d;
 }
@@ -119,9 +116,39 @@ __device__ __forceinline__ double exact_value(const ScanGeom& g, const NodeInfo&
 }
 
 // The same value for 2-D, non-circular scans with V = VV <= 2 active
-// sub-variables: the same sum in the same order, with the TI loads of eight
-// neighbours issued ahead of their products (the position is valid, so every
-// p + d_j lies inside the TI; the last batch re-reads neighbour nb - 1).
+// sub-variables: the same sum in the same order, with the TI loads of the
+// next kExactBatch neighbours in flight while the products of the current
+// ones are summed (the position is valid, so every p + d_j lies inside the
+// TI; loads past the last neighbour re-read neighbour nb - 1 and are not
+// summed).
+constexpr int kExactBatch = 16;
+
+template <int VV>
+__device__ __forceinline__ void exact_load(const int* __restrict__ off, const float* __restrict__ Y0,
+                                           const float* __restrict__ Y1, int c0, int c1, int s0, int nb, int j0,
+                                           float (&y0)[kExactBatch], float (&y1)[kExactBatch]) {
+#pragma unroll
+  for (int u = 0; u < kExactBatch; ++u) {
+    const int j = min(j0 + u, nb - 1);
+    const long long t = (long long)(c0 + off[2 * j]) + (long long)(c1 + off[2 * j + 1]) * s0;
+    y0[u] = Y0[t];
+    if (VV == 2) y1[u] = Y1[t];
+  }
+}
+
+template <int VV>
+__device__ __forceinline__ void exact_sum(const double* __restrict__ v, int nb, int j0,
+                                          const float (&y0)[kExactBatch], const float (&y1)[kExactBatch],
+                                          double& acc) {
+#pragma unroll
+  for (int u = 0; u < kExactBatch; ++u) {
+    if (j0 + u < nb) {
+      acc = __dadd_rn(acc, __dmul_rn(v[(j0 + u) * VV], (double)y0[u]));
+      if (VV == 2) acc = __dadd_rn(acc, __dmul_rn(v[(j0 + u) * VV + 1], (double)y1[u]));
+    }
+  }
+}
+
 template <int VV>
 __device__ __forceinline__ double exact_value_2d(const ScanGeom& g, const NodeInfo& ni, const int* __restrict__ off,
                                                  const double* __restrict__ v, const float* __restrict__ Y,
@@ -132,21 +159,14 @@ __device__ __forceinline__ double exact_value_2d(const ScanGeom& g, const NodeIn
   const float* Y1 = Y + (long long)g.A[VV - 1] * S;
   const int nb = ni.nb;
   double acc = (double)ni.delta0;
-  for (int j0 = 0; j0 < nb; j0 += 8) {
-    float y0[8], y1[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int j = min(j0 + u, nb - 1);
-      const long long t = (long long)(c0 + off[2 * j]) + (long long)(c1 + off[2 * j + 1]) * s0;
-      y0[u] = Y0[t];
-      if (VV == 2) y1[u] = Y1[t];
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      if (j0 + u < nb) {
-        acc = __dadd_rn(acc, __dmul_rn(v[(j0 + u) * VV], (double)y0[u]));
-        if (VV == 2) acc = __dadd_rn(acc, __dmul_rn(v[(j0 + u) * VV + 1], (double)y1[u]));
-      }
+  if (nb > 0) {
+    float a0[kExactBatch], a1[kExactBatch], b0[kExactBatch], b1[kExactBatch];
+    exact_load<VV>(off, Y0, Y1, c0, c1, s0, nb, 0, a0, a1);
+    for (int j0 = 0; j0 < nb; j0 += 2 * kExactBatch) {
+      exact_load<VV>(off, Y0, Y1, c0, c1, s0, nb, j0 + kExactBatch, b0, b1);
+      exact_sum<VV>(v, nb, j0, a0, a1, acc);
+      exact_load<VV>(off, Y0, Y1, c0, c1, s0, nb, j0 + 2 * kExactBatch, a0, a1);
+      exact_sum<VV>(v, nb, j0 + kExactBatch, b0, b1, acc);
     }
   }
   return nscale == 1.0 ? acc : acc * nscale;
@@ -192,7 +212,7 @@ __global__ void __launch_bounds__(256) k_merge_band(const NodeInfo* __restrict__
                                                     Cand* __restrict__ outCand, uint32_t* __restrict__ chosen,
                                                     float* __restrict__ simVals, const float* __restrict__ tiVals,
                                                     int nVar, uint32_t* __restrict__ indexOut, BandDev bb,
-                                                    const int* __restrict__ count) {
+                                                    int exactOn, const int* __restrict__ count) {
   __shared__ Cand red[16];
   __shared__ Cand best[kMaxK];
   __shared__ uint32_t bpos[kMaxK];
@@ -220,7 +240,7 @@ __global__ void __launch_bounds__(256) k_merge_band(const NodeInfo* __restrict__
   const int* gc = gcnt + (long long)node * candGroups;
   for (int i = tid; i < nGroups * K; i += blockDim.x) tk.push(cn[i]);
   block_topk<KT>(tk, K, best, red);
-  if (!(ni.eps > 0.f)) {  // FP32 selection
+  if (!exactOn || !(ni.eps > 0.f)) {  // FP32 selection
     if (tid < K) {
       outVal[(long long)node * K + tid] = best[tid].val;
       outPos[(long long)node * K + tid] = best[tid].pos;
@@ -277,52 +297,101 @@ __global__ void __launch_bounds__(256) k_merge_band(const NodeInfo* __restrict__
   }
 }
 
-// Step 3a. grid (node, kBandChunks), one wavefront each; dynamic LDS: the
-// node's neighbours. Chunk c scores listed positions c*64 + t +
-// j*kBandChunks*64 and writes its K best to bpart[node][c].
+__device__ __forceinline__ int band_tiles(int cnt) {
+  return cnt < 0 ? 0 : cnt > kBandCap ? kFullTiles : (cnt + 63) >> 6;
+}
+
+// Step 3. Nodes [node0, node0 + nSeg) (nSeg <= kSegNodes), one wavefront
+// per block. Every block sums the nodes' tile counts in LDS, then takes tiles
+// blockIdx.x, blockIdx.x + gridDim.x, ...: a list tile scores 64 listed
+// positions into val, a full-fallback tile scores a slice of the node's valid
+// box and writes its K best to part. Dynamic LDS: the tile node's neighbours.
 template <int KT>
 __global__ void __launch_bounds__(64) k_band_rescore(ScanGeom g, const NodeInfo* __restrict__ info,
                                                       const int* __restrict__ nbOff, const float* __restrict__ nbVal,
                                                       const float* __restrict__ Y, long long S, double nscale, int K,
-                                                      BandDev bb, const int* __restrict__ count) {
+                                                      int node0, int nSeg, BandDev bb, const int* __restrict__ count) {
   extern __shared__ char sm[];
-  __shared__ Cand64 red[4];
-  const int node = blockIdx.x, chunk = blockIdx.y, tid = threadIdx.x;
-  if (count && node >= *count) return;
-  const int cnt = bb.cnt[node];
-  if (cnt < 0) return;
-  Cand64* out = bb.bpart + ((long long)node * kBandChunks + chunk) * K;
-  if (cnt > kBandCap) {  // the list overflowed: whole valid box (step 4)
-    if (chunk == 0 && tid == 0) {
-      const int q = atomicAdd(&bb.qcnt[1], 1);
-      bb.fullq[q] = node;
-      atomicAdd(&bb.stats[3], 1ull);
-    }
-    return;
+  __shared__ int pre[kSegNodes + 1];
+  __shared__ Cand64 red[1];
+  const int tid = threadIdx.x;
+  int nLive = nSeg;
+  if (count) nLive = min(nSeg, max(0, *count - node0));
+  if (nLive <= 0) return;
+  // exclusive prefix of the tile counts: thread tid owns a run of nodes
+  const int per = (nLive + 63) >> 6;
+  const int r0 = min(nLive, tid * per), r1 = min(nLive, r0 + per);
+  int run = 0;
+  for (int i = r0; i < r1; ++i) run += band_tiles(bb.cnt[node0 + i]);
+  int incl = run;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o);
+    if (tid >= o) incl += y;
   }
-  if (chunk == 0 && tid == 0) atomicAdd(&bb.stats[1], (unsigned long long)cnt);
-  TopK64<KT> tk;
-  tk.init();
-  if (chunk * 64 < cnt) {  // block-uniform
+  int acc = incl - run;
+  for (int i = r0; i < r1; ++i) {
+    pre[i] = acc;
+    acc += band_tiles(bb.cnt[node0 + i]);
+  }
+  if (tid == 63) pre[nLive] = incl;
+  __syncthreads();
+  const int total = pre[nLive];
+  int cur = -1;
+  int* so = nullptr;
+  double* sv = nullptr;
+  for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    // the node holding the tile: the last with pre[n] <= tile
+    int lo = 0, hi = nLive - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (pre[mid] <= tile) lo = mid;
+      else hi = mid - 1;
+    }
+    const int node = node0 + lo, lt = tile - pre[lo];
     const NodeInfo ni = info[node];
-    int* so;
-    double* sv;
-    stage_neighbours(g, ni, nbOff, nbVal, sm, &so, &sv);
-    __syncthreads();
-    const uint32_t* lst = bb.pos + (long long)node * kBandCap;
-    for (int i = chunk * 64 + tid; i < cnt; i += kBandChunks * 64) {
-      const uint32_t p = lst[i];
-      Cand64 c;
-      c.val = exact_value_any(g, ni, so, sv, Y, S, p, nscale);
-      c.h = tie_hash(ni.seedbits, p);
-      c.pos = p;
-      tk.push(c);
+    if (node != cur) {
+      __syncthreads();  // the previous node's neighbours are read no more
+      stage_neighbours(g, ni, nbOff, nbVal, sm, &so, &sv);
+      __syncthreads();
+      cur = node;
+    }
+    const int cnt = bb.cnt[node];
+    if (cnt <= kBandCap) {
+      const int i = lt * 64 + tid;
+      if (i < cnt) {
+        const uint32_t p = bb.pos[(long long)node * kBandCap + i];
+        bb.val[(long long)node * kBandCap + i] = exact_value_any(g, ni, so, sv, Y, S, p, nscale);
+      }
+    } else {  // the list overflowed: slice lt of the whole valid box
+      const long long n0 = ni.hi[0] - ni.lo[0] + 1, n1 = ni.hi[1] - ni.lo[1] + 1, n2 = ni.hi[2] - ni.lo[2] + 1;
+      const long long box = (n0 > 0 && n1 > 0 && n2 > 0) ? n0 * n1 * n2 : 0;
+      const long long len = (box + kFullTiles - 1) / kFullTiles;
+      const long long beg = (long long)lt * len, end = min(box, beg + len);
+      TopK64<KT> tk;
+      tk.init();
+      for (long long i = beg + tid; i < end; i += 64) {
+        const int c0 = ni.lo[0] + (int)(i % n0);
+        const int c1 = ni.lo[1] + (int)((i / n0) % n1);
+        const int c2 = ni.lo[2] + (int)(i / (n0 * n1));
+        if (ni.exOn && excluded(ni, g.s[0], g.s[1], c0, c1, c2)) continue;
+        const uint32_t p = (uint32_t)(c0 + (long long)g.s[0] * (c1 + (long long)g.s[1] * c2));
+        Cand64 c;
+        c.val = exact_value_any(g, ni, so, sv, Y, S, p, nscale);
+        c.h = tie_hash(ni.seedbits, p);
+        c.pos = p;
+        tk.push(c);
+      }
+      block_topk64<KT>(tk, K, bb.part + ((long long)node * kFullTiles + lt) * K, red);
     }
   }
-  block_topk64<KT>(tk, K, out, red);
 }
 
-// Step 3b. One block per node: the K best of its chunks, outputs and draw.
+// Step 4. One block per node: the K best of its listed positions' FP64
+// values (of its tiles' K best after a full fallback) by the device rule,
+// outputs, the draw and the simulation write. Block 0 clears the second-pass
+// queue count for the next call (the collect kernels read it before this
+// launch).
 template <int KT>
 __global__ void __launch_bounds__(256) k_band_final(const NodeInfo* __restrict__ info, int K, float kf,
                                                     float* __restrict__ outVal, uint32_t* __restrict__ outPos,
@@ -334,14 +403,29 @@ __global__ void __launch_bounds__(256) k_band_final(const NodeInfo* __restrict__
   __shared__ Cand64 best[kMaxK];
   __shared__ uint32_t bpos[kMaxK];
   const int node = blockIdx.x, tid = threadIdx.x;
+  if (node == 0 && tid == 0) bb.qcnt[0] = 0;
   if (count && node >= *count) return;
   const int cnt = bb.cnt[node];
-  if (cnt < 0 || cnt > kBandCap) return;  // finished by the merge / the full fallback
+  if (cnt < 0) return;  // finished by the merge
   const NodeInfo ni = info[node];
   TopK64<KT> tk;
   tk.init();
-  const Cand64* pp = bb.bpart + (long long)node * kBandChunks * K;
-  for (int i = tid; i < kBandChunks * K; i += blockDim.x) tk.push(pp[i]);
+  if (cnt <= kBandCap) {
+    const uint32_t* lst = bb.pos + (long long)node * kBandCap;
+    const double* vl = bb.val + (long long)node * kBandCap;
+    for (int i = tid; i < cnt; i += blockDim.x) {
+      Cand64 c;
+      c.pos = lst[i];
+      c.val = vl[i];
+      c.h = tie_hash(ni.seedbits, c.pos);
+      tk.push(c);
+    }
+    if (tid == 0) atomicAdd(&bb.stats[1], (unsigned long long)cnt);
+  } else {
+    const Cand64* pp = bb.part + (long long)node * kFullTiles * K;
+    for (int i = tid; i < kFullTiles * K; i += blockDim.x) tk.push(pp[i]);
+    if (tid == 0) atomicAdd(&bb.stats[3], 1ull);
+  }
   block_topk64<KT>(tk, K, best, red);
   if (tid < K) {
     const Cand64 c = best[tid];
@@ -359,87 +443,6 @@ __global__ void __launch_bounds__(256) k_band_final(const NodeInfo* __restrict__
   }
   __syncthreads();
   if (tid == 0 && (chosen || simVals)) draw_write(ni, bpos, K, kf, chosen, node, simVals, tiVals, nVar, indexOut);
-}
-
-// Step 4a. grid (kFullChunks, kFullSlots): chunk x of every full-fallback
-// node, in FP64, K best per chunk into part.
-template <int KT>
-__global__ void __launch_bounds__(256) k_rescore_full(ScanGeom g, const NodeInfo* __restrict__ info,
-                                                      const int* __restrict__ nbOff, const float* __restrict__ nbVal,
-                                                      const float* __restrict__ Y, long long S, double nscale, int K,
-                                                      BandDev bb) {
-  extern __shared__ char sm[];
-  __shared__ Cand64 red[4];
-  const int items = bb.qcnt[1];
-  for (int it = blockIdx.y; it < items; it += gridDim.y) {
-    const int node = bb.fullq[it];
-    const NodeInfo ni = info[node];
-    int* so;
-    double* sv;
-    stage_neighbours(g, ni, nbOff, nbVal, sm, &so, &sv);
-    __syncthreads();
-    const long long n0 = ni.hi[0] - ni.lo[0] + 1, n1 = ni.hi[1] - ni.lo[1] + 1, n2 = ni.hi[2] - ni.lo[2] + 1;
-    const long long total = (n0 > 0 && n1 > 0 && n2 > 0) ? n0 * n1 * n2 : 0;
-    const long long per = (total + kFullChunks - 1) / kFullChunks;
-    const long long beg = (long long)blockIdx.x * per, end = min(total, beg + per);
-    TopK64<KT> tk;
-    tk.init();
-    for (long long i = beg + threadIdx.x; i < end; i += blockDim.x) {
-      const int c0 = ni.lo[0] + (int)(i % n0);
-      const int c1 = ni.lo[1] + (int)((i / n0) % n1);
-      const int c2 = ni.lo[2] + (int)(i / (n0 * n1));
-      if (ni.exOn && excluded(ni, g.s[0], g.s[1], c0, c1, c2)) continue;
-      const uint32_t p = (uint32_t)(c0 + (long long)g.s[0] * (c1 + (long long)g.s[1] * c2));
-      Cand64 c;
-      c.val = exact_value_any(g, ni, so, sv, Y, S, p, nscale);
-      c.h = tie_hash(ni.seedbits, p);
-      c.pos = p;
-      tk.push(c);
-    }
-    block_topk64<KT>(tk, K, bb.part + ((long long)it * kFullChunks + blockIdx.x) * K, red);
-  }
-}
-
-// Step 4b. kFullSlots blocks: per full-fallback node, the K best of its
-// chunks, outputs and draw.
-template <int KT>
-__global__ void __launch_bounds__(256) k_rescore_full_merge(const NodeInfo* __restrict__ info, int K, float kf,
-                                                            float* __restrict__ outVal, uint32_t* __restrict__ outPos,
-                                                            Cand* __restrict__ outCand, uint32_t* __restrict__ chosen,
-                                                            float* __restrict__ simVals,
-                                                            const float* __restrict__ tiVals, int nVar,
-                                                            uint32_t* __restrict__ indexOut, BandDev bb) {
-  __shared__ Cand64 red[4];
-  __shared__ Cand64 best[kMaxK];
-  __shared__ uint32_t bpos[kMaxK];
-  const int items = bb.qcnt[1];
-  const int tid = threadIdx.x;
-  for (int it = blockIdx.x; it < items; it += gridDim.x) {
-    const int node = bb.fullq[it];
-    const NodeInfo ni = info[node];
-    TopK64<KT> tk;
-    tk.init();
-    const Cand64* pp = bb.part + (long long)it * kFullChunks * K;
-    for (int i = tid; i < kFullChunks * K; i += blockDim.x) tk.push(pp[i]);
-    block_topk64<KT>(tk, K, best, red);
-    if (tid < K) {
-      const Cand64 c = best[tid];
-      outVal[(long long)node * K + tid] = (float)c.val;
-      outPos[(long long)node * K + tid] = c.pos;
-      if (outCand) {
-        Cand w;
-        w.key = (float)c.val;
-        w.h = c.h;
-        w.pos = c.pos;
-        w.val = (float)c.val;
-        outCand[(long long)node * K + tid] = w;
-      }
-      bpos[tid] = c.pos;
-    }
-    __syncthreads();
-    if (tid == 0 && (chosen || simVals)) draw_write(ni, bpos, K, kf, chosen, node, simVals, tiVals, nVar, indexOut);
-    __syncthreads();
-  }
 }
 
 // Per-node API: the band of a materialised map (position space, -inf
@@ -489,9 +492,8 @@ void band_free(BandBufs* b) {
   (void)hipFree(b->theta);
   (void)hipFree(b->queue);
   (void)hipFree(b->qcnt);
-  (void)hipFree(b->fullq);
+  (void)hipFree(b->val);
   (void)hipFree(b->part);
-  (void)hipFree(b->bpart);
   (void)hipFree(b->stats);
   *b = BandBufs();
 }
@@ -504,18 +506,23 @@ int band_grow(BandBufs* b, int nNodes, int blocksPerNode, int K) {
   const size_t qq = std::max(q, b->capQueue);
   const int kk = std::max(K, b->K);
   // grown between launches of one stream: work queued on the old buffers
-  // finishes first (hipFree waits for it)
+  // finishes first (the callers' streams do not synchronise with the null
+  // stream, so wait for the device before the statistics are read back)
   unsigned long long keep[4] = {0, 0, 0, 0};
-  if (b->stats) (void)hipMemcpy(keep, b->stats, sizeof(keep), hipMemcpyDeviceToHost);
+  const int zero[2] = {0, 0};  // the queue count starts at zero; k_band_final clears it after each call
+  if (b->stats) {
+    (void)hipDeviceSynchronize();
+    (void)hipMemcpy(keep, b->stats, sizeof(keep), hipMemcpyDeviceToHost);
+  }
   band_free(b);
   if (hipMalloc(&b->pos, nn * kBandCap * sizeof(uint32_t)) != hipSuccess ||
       hipMalloc(&b->cnt, nn * sizeof(int)) != hipSuccess || hipMalloc(&b->theta, nn * sizeof(float)) != hipSuccess ||
       hipMalloc(&b->queue, qq * sizeof(int2)) != hipSuccess || hipMalloc(&b->qcnt, 2 * sizeof(int)) != hipSuccess ||
-      hipMalloc(&b->fullq, nn * sizeof(int)) != hipSuccess ||
-      hipMalloc(&b->part, nn * kFullChunks * (size_t)kk * sizeof(Cand64)) != hipSuccess ||
-      hipMalloc(&b->bpart, nn * kBandChunks * (size_t)kk * sizeof(Cand64)) != hipSuccess ||
+      hipMalloc(&b->val, nn * kBandCap * sizeof(double)) != hipSuccess ||
+      hipMalloc(&b->part, nn * kFullTiles * (size_t)kk * sizeof(Cand64)) != hipSuccess ||
       hipMalloc(&b->stats, sizeof(keep)) != hipSuccess ||
-      hipMemcpy(b->stats, keep, sizeof(keep), hipMemcpyHostToDevice) != hipSuccess) {
+      hipMemcpy(b->stats, keep, sizeof(keep), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->qcnt, zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess) {
     band_free(b);
     return fail(QSMI_ERR_NOMEM, "exact selection buffers: out of device memory");
   }
@@ -530,8 +537,8 @@ int band_grow(BandBufs* b, int nNodes, int blocksPerNode, int K) {
 // them large enough and allocates nothing.
 size_t band_scratch_bytes(int K) {
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  return al(kBandCap * sizeof(uint32_t)) + 4 * al(sizeof(int)) + al(sizeof(float)) + al(sizeof(int2)) +
-         al((size_t)kFullChunks * K * sizeof(Cand64)) + al((size_t)kBandChunks * K * sizeof(Cand64)) +
+  return al(kBandCap * sizeof(uint32_t)) + 2 * al(sizeof(int)) + al(sizeof(float)) + al(sizeof(int2)) +
+         al(kBandCap * sizeof(double)) + al((size_t)kFullTiles * K * sizeof(Cand64)) +
          al(4 * sizeof(unsigned long long));
 }
 
@@ -548,12 +555,10 @@ void band_view(BandBufs* b, void* base, int K) {
   p += al(sizeof(int2));
   b->qcnt = reinterpret_cast<int*>(p);
   p += al(sizeof(int));
-  b->fullq = reinterpret_cast<int*>(p);
-  p += al(sizeof(int));
+  b->val = reinterpret_cast<double*>(p);
+  p += al(kBandCap * sizeof(double));
   b->part = reinterpret_cast<Cand64*>(p);
-  p += al((size_t)kFullChunks * K * sizeof(Cand64));
-  b->bpart = reinterpret_cast<Cand64*>(p);
-  p += al((size_t)kBandChunks * K * sizeof(Cand64));
+  p += al((size_t)kFullTiles * K * sizeof(Cand64));
   b->stats = reinterpret_cast<unsigned long long*>(p);
   b->capNodes = 1;
   b->capQueue = 1;
@@ -576,14 +581,16 @@ void launch_rescore(const ScanGeom& g, const NodeInfo* info, const int* nbOff, c
                     double nscale, int nNodes, int K, float kf, float* outVal, uint32_t* outPos, Cand* outCand,
                     uint32_t* chosen, float* simVals, const float* tiVals, int nVar, uint32_t* indexOut,
                     const BandDev& bd, const int* count, size_t lds, hipStream_t st) {
-  hipLaunchKernelGGL(k_band_rescore<KT>, dim3((unsigned)nNodes, kBandChunks), dim3(64), lds, st, g, info, nbOff,
-                     nbVal, t->dSub, t->S, nscale, K, bd, count);
+  for (int n0 = 0; n0 < nNodes; n0 += kSegNodes) {
+    const int nSeg = std::min(kSegNodes, nNodes - n0);
+    // about two tiles per node (most lists hold under 64 positions); blocks
+    // beyond the tile total return after the prefix
+    const int grid = std::min(8192, 2 * nSeg + 512);
+    hipLaunchKernelGGL(k_band_rescore<KT>, dim3((unsigned)grid), dim3(64), lds, st, g, info, nbOff, nbVal, t->dSub,
+                       t->S, nscale, K, n0, nSeg, bd, count);
+  }
   hipLaunchKernelGGL(k_band_final<KT>, dim3((unsigned)nNodes), dim3(256), 0, st, info, K, kf, outVal, outPos, outCand,
                      chosen, simVals, tiVals, nVar, indexOut, bd, count);
-  hipLaunchKernelGGL(k_rescore_full<KT>, dim3(kFullChunks, kFullSlots), dim3(256), lds, st, g, info, nbOff, nbVal,
-                     t->dSub, t->S, nscale, K, bd);
-  hipLaunchKernelGGL(k_rescore_full_merge<KT>, dim3(kFullSlots), dim3(256), 0, st, info, K, kf, outVal, outPos,
-                     outCand, chosen, simVals, tiVals, nVar, indexOut, bd);
 }
 
 }  // namespace
@@ -615,11 +622,11 @@ int band_select(qsmi_device* dev, const ScanGeom& g, const NodeInfo* info, const
   int rc = band_grow(bb, nNodes, exact ? blocks : 1, K);
   if (rc) return rc;
   const BandDev bd = dev_view(*bb);
-  hipLaunchKernelGGL(k_band_reset, dim3(1), dim3(64), 0, st, bb->qcnt, (int*)nullptr);
   const int KT = kt_of(K);
 #define QSMI_BAND_MERGE(KT_)                                                                                      \
   hipLaunchKernelGGL(k_merge_band<KT_>, dim3((unsigned)nNodes), dim3(256), 0, st, info, cand, gcnt, candGroups, \
-                     wpb, rpb, K, kf, outVal, outPos, outCand, chosen, simVals, tiVals, nVar, indexOut, bd, count)
+                     wpb, rpb, K, kf, outVal, outPos, outCand, chosen, simVals, tiVals, nVar, indexOut, bd,       \
+                     exact ? 1 : 0, count)
   switch (KT) {
     case 2: QSMI_BAND_MERGE(2); break;
     case 4: QSMI_BAND_MERGE(4); break;

@@ -56,10 +56,9 @@ struct BandBufs {
   int* cnt = nullptr;            // [node]
   float* theta = nullptr;        // [node] lower end of the band
   int2* queue = nullptr;         // second-pass items (node, rows block)
-  int* qcnt = nullptr;           // [2]: queue items, full-fallback nodes
-  int* fullq = nullptr;          // full-fallback nodes
-  qsmi::Cand64* part = nullptr;  // full-fallback chunk results
-  qsmi::Cand64* bpart = nullptr; // rescoring chunk results
+  int* qcnt = nullptr;           // [2]: queue items (zero between calls), unused
+  double* val = nullptr;         // [node][kBandCap] FP64 values of the listed positions
+  qsmi::Cand64* part = nullptr;  // [node][kFullTiles][K] full-fallback tile results
   unsigned long long* stats = nullptr;  // exact nodes, listed positions, second-pass blocks, full-fallback nodes
 };
 
