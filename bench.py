@@ -11,6 +11,7 @@ Prints one JSON line (rank 0). Single GPU only.
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -46,11 +47,15 @@ def parse():
     ap.add_argument("--sim-size", type=int, default=0, help="simulation extent when smaller than the TI (profiling)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="also run a plain 256 MiB device copy (HBM counter calibration under rocprofv3 --pmc)")
-    ap.add_argument("--wave", type=int, default=512, help="max path nodes per wave (the learned form: 256)")
+    ap.add_argument("--wave", type=int, default=256,
+                    help="max path nodes per wave: the workload batches 256 path nodes per launch")
+    ap.add_argument("--alt-wave", type=int, default=512,
+                    help="also time one realisation with waves of at most this many nodes (0: skip)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="scan-kernel launch parameter for the TIs of the run (qsmi_set_default_tuning)")
-    ap.add_argument("--cpu-sample", type=int, default=48, help="path nodes of the CPU baseline sample (0: skip)")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-sample", type=int, default=-1,
+                    help="path nodes of the CPU baseline sample (-1: four per thread, 0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: nproc)")
     ap.add_argument("--learned", type=int, default=1,
                     help="also measure the learned-proposal form (1) or not (0)")
     ap.add_argument("--lp-train-sets", type=int, default=8,
@@ -127,6 +132,42 @@ def learned_form(cfg, args, exact_ms):
     return out
 
 
+def host_threads():
+    """What nproc prints: the CPUs this process may run on, capped by
+    OMP_NUM_THREADS / OMP_THREAD_LIMIT when set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    for var in ("OMP_NUM_THREADS", "OMP_THREAD_LIMIT"):
+        try:
+            v = int(os.environ.get(var, "").split(",")[0])
+            if v > 0:
+                n = min(n, v) if var == "OMP_THREAD_LIMIT" else v
+        except ValueError:
+            pass
+    return max(1, n)
+
+
+def cpu_model():
+    """The host CPU as lscpu names it (model name, sockets x cores x threads)."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    name = info.get("Model name")
+    if not name:
+        try:
+            with open("/proc/cpuinfo") as fh:
+                name = next((l.split(":", 1)[1].strip() for l in fh if l.startswith("model name")), "unknown")
+        except OSError:
+            name = "unknown"
+    return dict(model=name, sockets=info.get("Socket(s)"), cores_per_socket=info.get("Core(s) per socket"),
+                threads_per_core=info.get("Thread(s) per core"), online=info.get("On-line CPU(s) list"),
+                affinity_cpus=len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None)
+
+
 def cpu_baseline(cfg, sim, threads, sample):
     """The oracle in reference-dataflow mode (FP32 FFT scan + fKst select,
     src/CPUThreadDevice.cpp:349-490 restated) timed on this host for a
@@ -159,7 +200,7 @@ def cpu_baseline(cfg, sim, threads, sample):
         t0 = time.time()
         res = list(ex.map(one, nodes))
         dt = time.time() - t0
-    return dict(value=len(nodes) / dt, unit="nodes/s", cores=threads, kind="port",
+    return dict(value=len(nodes) / dt, unit="nodes/s", cores=threads, kind="port", host=cpu_model(),
                 sample=(f"{len(nodes)} path nodes {start}..{start + len(nodes) - 1} of the same 1000^2 "
                         f"realisation, oracle reference-dataflow FP32 FFT + fKst, {threads} threads, "
                         f"own mixed-radix CPU FFT (no FFTW on the host); spectra setup {setup:.1f}s untimed"))
@@ -196,7 +237,7 @@ def calibration_leg(cfg, iters, cpu_sample):
                config=(f"auto_qs calibration, TI {dims[0]}x{dims[1]} exp(GRF), uniform kernel {kd[0]}x{kd[1]}, "
                        f"{nb.shape[0]} neighbour counts x 7 densities, maxK 1, power 2, {iters} iterations, "
                        f"exclusion radius {(dims[0] * dims[1] * 0.005) ** 0.5:.1f}"))
-    if cpu_sample > 0:
+    if cpu_sample != 0:
         import oracle
 
         oracle.build()
@@ -205,7 +246,7 @@ def calibration_leg(cfg, iters, cpu_sample):
         c = oracle.calibration([ti], dims, [0], [ker], [kd], sub, g2s_amd.DEFAULT_DENSITIES, 1, 2.0, 1, 1000, 0.0, 5)
         dt = time.perf_counter() - t0
         out["cpu_baseline"] = dict(value=c["distributions"] / dt, unit="samples/s",
-                                   cores=int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), kind="port",
+                                   cores=host_threads(), kind="port",
                                    sample=(f"{c['distributions']} samples: one iteration over neighbour counts "
                                            f"10, 40, 79 x 7 densities, oracle FP64 brute-force maps (OpenMP)"))
     return out
@@ -275,15 +316,26 @@ def main():
                     traffic=(per_node * nodes_per_launch if per_node else None),
                     kernel="k_pencil1024", avg_launch_us=pencil_ms * 1e3 / max(1, st["waves"]),
                     nodes_per_launch=nodes_per_launch, algorithmic_bytes_per_node=a_node,
-                    traffic_source=(PROFILE if per_node else None))
+                    traffic_source=(PROFILE if per_node else None),
+                    note=("the algorithmic bytes count a 4-byte map per TI cell written and read once; the "
+                          "pencil and rows kernels never materialise that map (column transforms pass between "
+                          "them), so achieved may exceed the HBM peak: the pencil is bound by VALU issue, and "
+                          "traffic is its measured HBM bytes per launch"))
+    alt = None
+    if args.alt_wave > 0 and args.alt_wave != args.wave:
+        _, _, sa = g2s_amd.qs(ti, dims, di, sdims, cfg["types"], n, k, seed=cfg["seed"], wave_size=args.alt_wave,
+                              stage_timing=True)
+        alt = dict(wave=args.alt_wave, value=sa["nodes"] / (sa["path_loop_ms"] / 1e3), ms_per_step=sa["path_loop_ms"],
+                   waves=sa["waves"], spectrum_ms=sa["spectrum_ms"], rows_ms=sa["rows_ms"], merge_ms=sa["merge_ms"])
+        print(f"[bench] waves <= {args.alt_wave}: {alt}", file=sys.stderr, flush=True)
     learned = learned_form(cfg, args, ms) if args.learned else None
     calib = calibration_leg(cfg, args.calib_iters, args.cpu_sample) if args.calib_iters > 0 else None
     if calib:
         print(f"[bench] calibration: {calib}", file=sys.stderr, flush=True)
-    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    threads = args.cpu_threads or host_threads()
     cpu = None
-    if args.cpu_sample > 0:
-        cpu = cpu_baseline(cfg, sim, threads, args.cpu_sample)
+    if args.cpu_sample != 0:
+        cpu = cpu_baseline(cfg, sim, threads, args.cpu_sample if args.cpu_sample > 0 else 4 * threads)
     out = {
         "metric": "QS nodes/sec at 1000^2 TI n=80 k=1.5",
         "value": value,
@@ -305,6 +357,7 @@ def main():
         "cpu_baseline": cpu,
         "learned_proposal": learned,
         "calibration": calib,
+        "alt_wave": alt,
         "stats": st,
     }
     print(json.dumps(out))

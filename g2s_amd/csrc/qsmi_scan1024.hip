@@ -49,12 +49,45 @@ constexpr int WSTRIDE = BUF + GSLOT + 2;
 constexpr int NW = 16;              // rows wavefronts per block (one block per CU, 4 per SIMD)
 constexpr int NWP = 12;             // pencil wavefronts per block = columns per group (3 per SIMD)
 constexpr int NGRP = (H0 + NWP - 1) / NWP;  // 43 column groups
+constexpr int GX_BASE = NGRP / NXCD;   // groups of every node kept on one XCD (5)
+constexpr int GX_EXTRA = NGRP % NXCD;  // the node's last groups, on rotating XCDs (3)
 constexpr int CT_PAIR = 520 * 2;    // float2 per row pair: 513 columns padded to 64-byte multiples
 constexpr long long CT_STRIDE = (long long)(F / 2) * CT_PAIR;  // float2 per node
 static_assert(WSTRIDE % 16 == 2, "buffer bank offset");
 static_assert((NWP * 16) % 64 == 0 && (CT_PAIR * 8) % 64 == 0, "64-byte aligned group stores");
 
 __device__ __forceinline__ int pidx(int e) { return e + (e >> 4); }
+
+// Pencil items of XCD x over a chunk of NXCD consecutive nodes: its r-th
+// (r < NGRP) is node offset o, column group cg. A node's groups x + 8 i
+// (i < GX_BASE) always go to XCD x, its last GX_EXTRA groups (8 GX_BASE + e)
+// to XCD (o + e) mod 8, so every XCD takes NGRP items per chunk (no XCD waits
+// for one that holds an extra group of every node) and reads a fixed eighth
+// of the TI spectra plus the last groups' columns.
+__device__ __forceinline__ void pencil_item(int x, int r, int& o, int& cg) {
+  o = 0;
+  for (; o < NXCD - 1; ++o) {
+    int ne = 0;
+    for (int e = 0; e < GX_EXTRA; ++e) ne += ((o + e) % NXCD) == x ? 1 : 0;
+    if (r < GX_BASE + ne) break;
+    r -= GX_BASE + ne;
+  }
+  if (r < GX_BASE) {
+    cg = x + NXCD * r;
+    return;
+  }
+  r -= GX_BASE;
+  cg = NXCD * GX_BASE;
+  for (int e = 0; e < GX_EXTRA; ++e) {
+    if (((o + e) % NXCD) == x) {
+      if (r == 0) {
+        cg = NXCD * GX_BASE + e;
+        return;
+      }
+      --r;
+    }
+  }
+}
 
 // orders this wavefront's LDS accesses across lanes (no hardware barrier)
 __device__ __forceinline__ void wave_sync() {
@@ -320,17 +353,19 @@ __global__ void __launch_bounds__(64 * NWP) k_pencil1024(
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform (scalar) from here on
   float2* buf = wbuf[w];
   float2* gbuf = wbuf[w] + BUF;
-  // block items: (node, column group cg); the groups of XCD x are
-  // cg = x + 8 i, so each XCD reads 1/8 of the TI spectra
+  // block items: (node, column group cg), NGRP per XCD and chunk of NXCD
+  // nodes (pencil_item)
   const int xcd = blockIdx.x % NXCD;
   const int blocksX = gridDim.x / NXCD, bX = blockIdx.x / NXCD;
-  const int ngx = (NGRP - xcd + NXCD - 1) / NXCD;
   const int nAct = count ? min(*count, nNodes) : nNodes;
-  const long long items = (long long)nAct * ngx;
+  const long long items = (long long)((nAct + NXCD - 1) / NXCD) * NGRP;
   const int V = g.V;
   for (long long it = bX; it < items; it += blocksX) {  // block-uniform
-    const int node = (int)(it / ngx);
-    const int cg = xcd + NXCD * (int)(it - (long long)node * ngx);
+    const int ch = (int)(it / NGRP);
+    int o, cg;
+    pencil_item(xcd, (int)(it - (long long)ch * NGRP), o, cg);
+    const int node = ch * NXCD + o;
+    if (node >= nAct) continue;  // block-uniform (the last chunk)
     const int k0 = cg * NWP + w;
     // the lane index, opaque to the compiler per item: lane-dependent
     // addresses are recomputed per item instead of held (spilled) across
@@ -899,7 +934,7 @@ void launch_pencil1024(const ScanGeom& g, const float2* YT, const float2* tw, in
   hipLaunchKernelGGL(k_taps1024, dim3((unsigned)nNodes), dim3(64), 0, st, info, nbOff, nbVal, g.V, taps, tapD0, tapVal,
                      count);
   // enough blocks for the XCD's items, at most one block per CU
-  const long long itemsX = (long long)nNodes * ((NGRP + NXCD - 1) / NXCD);
+  const long long itemsX = (long long)((nNodes + NXCD - 1) / NXCD) * NGRP;
   const int perX = (int)std::min<long long>(w1k_blocks() / NXCD, itemsX);
   hipLaunchKernelGGL(k_pencil1024, dim3((unsigned)(std::max(perX, 1) * NXCD)), dim3(64 * NWP), 0, st, g, YT, tw, info,
                      nbOff, nbVal, taps, tapD0, tapVal, CT, nNodes, count);
