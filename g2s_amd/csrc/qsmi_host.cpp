@@ -713,30 +713,42 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
       rc = qsmi::lp_cover_all(rq->lp, (int)nPath, n, (const int*)dNb, (const int*)dCnt, (const int*)dOffs, lpMinCover,
                               (unsigned char*)dOk, dev->stream);
   }
-  std::vector<int> nbPj(nPath * (size_t)n);
-  std::vector<int> nbCnt(nPath, 0);
+  // dependency levels (1 + the deepest level among a node's path neighbours)
+  // on the device; only the levels come back
+  std::vector<int> level(nPath, 0);
+  void *dLevel = nullptr, *dLvErr = nullptr;
+  if (!rc) rc = dev_upload(nullptr, nPath * sizeof(int), &dLevel);
+  if (!rc) rc = dev_upload(nullptr, sizeof(int), &dLvErr);
+  if (!rc) rc = qsmi::levels_device(nPath, n, (const int*)dPj, (const int*)dCnt, (int*)dLevel, (int*)dLvErr,
+                                    dev->stream);
+  int lvErr = 0;
   if (!rc) {
     hipError_t e = hipStreamSynchronize(dev->stream);
-    if (e == hipSuccess) e = hipMemcpy(nbCnt.data(), dCnt, nPath * sizeof(int), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(nbPj.data(), dPj, nbPj.size() * sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && nPath) e = hipMemcpy(level.data(), dLevel, nPath * sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&lvErr, dLvErr, sizeof(int), hipMemcpyDeviceToHost);
     if (e == hipSuccess && dOk) {
       lpOk.resize(nPath);
       e = hipMemcpy(lpOk.data(), dOk, nPath, hipMemcpyDeviceToHost);
     }
     if (e != hipSuccess) rc = hfail(QSMI_ERR_HIP, std::string("neighbour search: ") + hipGetErrorString(e));
   }
-  const auto tNb = clk::now();
-  // dependency levels: 1 + the deepest level among a node's path neighbours
-  std::vector<int> level(nPath, 0);
-  int maxLevel = 0;
-  for (size_t i = 0; i < nPath; ++i) {
-    int lv = 0;
-    const int* pj = nbPj.data() + i * (size_t)n;
-    for (int j = 0; j < nbCnt[i]; ++j)
-      if (pj[j] >= 0 && (size_t)pj[j] != i) lv = std::max(lv, level[pj[j]] + 1);
-    level[i] = lv;
-    maxLevel = std::max(maxLevel, lv);
+  if (!rc && lvErr) {
+    // a device wait hit its bound: the levels again on the host, in path order
+    std::vector<int> nbPj(nPath * (size_t)n), nbCnt(nPath, 0);
+    hipError_t e = hipMemcpy(nbCnt.data(), dCnt, nPath * sizeof(int), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(nbPj.data(), dPj, nbPj.size() * sizeof(int), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hfail(QSMI_ERR_HIP, std::string("neighbour search: ") + hipGetErrorString(e));
+    for (size_t i = 0; i < nPath && !rc; ++i) {
+      int lv = 0;
+      const int* pj = nbPj.data() + i * (size_t)n;
+      for (int j = 0; j < nbCnt[i]; ++j)
+        if (pj[j] >= 0 && (size_t)pj[j] < i) lv = std::max(lv, level[pj[j]] + 1);
+      level[i] = lv;
+    }
   }
+  const auto tNb = clk::now();
+  int maxLevel = 0;
+  for (size_t i = 0; i < nPath; ++i) maxLevel = std::max(maxLevel, level[i]);
   // nodes ordered by (level, path order)
   std::vector<size_t> lvStart(maxLevel + 2, 0);
   for (size_t i = 0; i < nPath; ++i) lvStart[level[i] + 1]++;

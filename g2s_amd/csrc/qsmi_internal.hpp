@@ -302,6 +302,10 @@ void lp_record(int nN, int w0, const int* order, const NodeInfo* info, const int
                hipStream_t st);
 
 // neighbour sets of every path node on the device (stream-ordered)
+// dependency levels of the path nodes from their neighbours' path indices
+// (level[i] = 1 + max level of i's path neighbours); *dErr = 1 when a wait
+// hit its bound (the levels are then not valid)
+int levels_device(size_t nPath, int n, const int* dNbPj, const int* dNbCnt, int* dLevel, int* dErr, hipStream_t st);
 int nb_search_device(int D, const unsigned* sd, size_t nPath, const unsigned* dCells, const int* dPostS,
                      const int* dOffs, int nOff, int n, int* dNbIdx, int* dNbPj, int* dNbCnt, hipStream_t st,
                      const int* dNNode = nullptr, const int* dNodeKer = nullptr, const int* dOffStart = nullptr,

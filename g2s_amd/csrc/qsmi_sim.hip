@@ -399,6 +399,59 @@ int nb_search_device(int D, const unsigned* sd, size_t nPath, const unsigned* dC
   return QSMI_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Dependency levels of the path nodes: 1 + the deepest level among a node's
+// path neighbours (hard data and the cell itself do not count), one
+// wavefront per node. A node's neighbours precede it on the path, so node i
+// waits only for levels of nodes < i; workgroups start in index order on
+// every XCD, so the lowest unfinished node is always resident with its
+// neighbours' levels written. Levels start at -1 (hipMemsetAsync before the
+// launch) and are read and written by device-scope atomics (coherent across
+// the XCDs' L2s), relaxed: nothing else is published with them. The wait is
+// bounded: a neighbour level still unwritten after
+// kLevelPolls polls sets *err and counts as 0 (the caller then recomputes the
+// levels on the host), so every wavefront ends.
+// ---------------------------------------------------------------------------
+constexpr int kLevelPolls = 1 << 21;  // x s_sleep(4) (256 cycles): ~0.2 s, far above any real wait
+
+__global__ void __launch_bounds__(64) k_levels(int n, const int* __restrict__ nbPj, const int* __restrict__ nbCnt,
+                                               int* level, int* err) {
+  const int i = blockIdx.x, lane = threadIdx.x;
+  const int cnt = nbCnt[i];
+  int lv = 0;
+  for (int j = lane; j < cnt; j += 64) {
+    const int pj = nbPj[(long long)i * n + j];
+    if (pj >= 0 && pj < i) {
+      int v = __hip_atomic_load(&level[pj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int polls = 0; v < 0; ++polls) {
+        if (polls >= kLevelPolls) {
+          __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          v = 0;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(4);
+        v = __hip_atomic_load(&level[pj], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      lv = max(lv, v + 1);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) lv = max(lv, __shfl_xor(lv, off));
+  if (lane == 0) __hip_atomic_store(&level[i], lv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+int levels_device(size_t nPath, int n, const int* dNbPj, const int* dNbCnt, int* dLevel, int* dErr, hipStream_t st) {
+  if (nPath == 0) return QSMI_OK;
+  if (nPath > 0x7fffffffull) return fail(QSMI_ERR_UNSUPPORTED, "path longer than 2^31 nodes");
+  hipError_t e = hipMemsetAsync(dLevel, 0xFF, nPath * sizeof(int), st);
+  if (e == hipSuccess) e = hipMemsetAsync(dErr, 0, sizeof(int), st);
+  if (e != hipSuccess) return fail(QSMI_ERR_HIP, std::string("path levels: ") + hipGetErrorString(e));
+  hipLaunchKernelGGL(k_levels, dim3((unsigned)nPath), dim3(64), 0, st, n, dNbPj, dNbCnt, dLevel, dErr);
+  e = hipGetLastError();
+  if (e != hipSuccess) return fail(QSMI_ERR_HIP, std::string("path levels: ") + hipGetErrorString(e));
+  return QSMI_OK;
+}
+
 int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsOut, long long* scannedOut,
                        int* wavesOut, long long* bandStats) {
   qsmi_ti* t = dev->ti;
