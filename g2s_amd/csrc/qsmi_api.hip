@@ -669,6 +669,15 @@ int qsmi_device_destroy(qsmi_device* d) {
     free_wave_bufs(d->wb);
     delete d->wb;
   }
+  if (d->side.stream) hipStreamSynchronize(d->side.stream);
+  if (d->side.wb) {
+    free_wave_bufs(d->side.wb);
+    delete d->side.wb;
+  }
+  hipFree(d->side.dTap);
+  hipFree(d->side.dTapD0);
+  hipFree(d->side.dTapVal);
+  if (d->side.stream) hipStreamDestroy(d->side.stream);
   for (int i = 0; i < 5; ++i)
     if (d->ev[i]) (void)hipEventDestroy(d->ev[i]);
   for (hipEvent_t ev : d->evPool) (void)hipEventDestroy(ev);

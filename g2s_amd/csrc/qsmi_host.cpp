@@ -766,8 +766,10 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   qsmi::ResidentLoop R;
   R.waveStart.push_back(0);
   for (int l = 0; l <= maxLevel; ++l)
-    for (size_t w0 = lvStart[l]; w0 < lvStart[l + 1]; w0 += waveSize)
+    for (size_t w0 = lvStart[l]; w0 < lvStart[l + 1]; w0 += waveSize) {
       R.waveStart.push_back((int)std::min(lvStart[l + 1], w0 + waveSize));
+      R.waveLevel.push_back(l);
+    }
   for (int d = 0; d < 3; ++d) {
     R.sd[d] = (int)sd[d];
     R.kd[d] = (int)ker.dims[d];

@@ -116,6 +116,19 @@ struct qsmi_device {
   NodeInfo lastNode{};
   bool mapEdited = false;
   BandBufs band;
+  // second stream of the resident loop (consecutive waves of one dependency
+  // level alternate between stream and side.stream), with its own wave
+  // buffers and tap tables; swapped into the fields above while one of its
+  // waves is enqueued
+  struct SideLane {
+    hipStream_t stream = nullptr;
+    WaveBufs* wb = nullptr;
+    qsmi::TapInfo1024* dTap = nullptr;
+    uint16_t* dTapD0 = nullptr;
+    float* dTapVal = nullptr;
+    size_t capTap = 0;
+    int capTapV = 0;
+  } side;
 };
 
 // Device buffers of one wave (grown on demand) and pinned host staging.
@@ -240,6 +253,7 @@ struct ResidentLoop {
   int expVar[16], expCat[16];
   int conv[16], subOf[16];
   std::vector<int> waveStart;  // wave w = order[waveStart[w] .. waveStart[w+1])
+  std::vector<int> waveLevel;  // dependency level of wave w (empty: one stream)
   bool stageTiming = false;
   bool noVerbatim = false;     // -nV: no candidate at the continuation of the nearest neighbour's copy
   std::vector<qsmi_device*> extraDevs;  // TIs 1.. of a several-TI run (same extent as this device's)

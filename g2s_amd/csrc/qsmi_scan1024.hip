@@ -348,7 +348,11 @@ __global__ void __launch_bounds__(64 * NWP) k_pencil1024(
   // the node's tap twiddles W(t, l), shared by the block's columns, at slot
   // (k, c) of the k-th tap of class c: row (16 k + c) of 64 lanes
   __shared__ float2 wtab[kTapPass * 16 * 64];
-  load_twiddles(twl, tw);
+  // the first pass reads every class's slot: a class without taps multiplies
+  // its zero tap sum by whatever its slot holds, which must be finite (LDS
+  // is not cleared between kernels; later items leave earlier twiddles)
+  for (int i = threadIdx.x; i < 16 * 64; i += blockDim.x) wtab[i] = make_float2(0.f, 0.f);
+  load_twiddles(twl, tw);  // ends with a barrier
   const int l = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform (scalar) from here on
   float2* buf = wbuf[w];

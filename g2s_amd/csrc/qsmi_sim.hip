@@ -452,6 +452,40 @@ int levels_device(size_t nPath, int n, const int* dNbPj, const int* dNbCnt, int*
   return QSMI_OK;
 }
 
+namespace {
+
+// Swaps a device's stream, wave buffers and tap tables with its side lane's
+// for the lifetime of the object (when on).
+struct SideSwap {
+  qsmi_device* d;
+  bool on;
+  SideSwap(qsmi_device* dev, bool enable) : d(dev), on(enable) {
+    if (on) flip();
+  }
+  ~SideSwap() {
+    if (on) flip();
+  }
+  void flip() {
+    std::swap(d->stream, d->side.stream);
+    std::swap(d->wb, d->side.wb);
+    std::swap(d->dTap, d->side.dTap);
+    std::swap(d->dTapD0, d->side.dTapD0);
+    std::swap(d->dTapVal, d->side.dTapVal);
+    std::swap(d->capTap, d->side.capTap);
+    std::swap(d->capTapV, d->side.capTapV);
+  }
+};
+
+struct LaneEvents {
+  hipEvent_t e[2] = {nullptr, nullptr};
+  ~LaneEvents() {
+    for (hipEvent_t x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+};
+
+}  // namespace
+
 int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsOut, long long* scannedOut,
                        int* wavesOut, long long* bandStats) {
   qsmi_ti* t = dev->ti;
@@ -588,9 +622,62 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   }
   long long scanned = 0;
   int waves = 0;
+  // Two streams: no node of a level has a neighbour in the same level, so
+  // consecutive waves of one level are independent; they alternate between
+  // the device's stream (lane 0) and its side stream (lane 1), each with its
+  // own wave buffers. At a level change each lane waits for the other's
+  // earlier work.
+  const bool twoLanes = !R.lp && nTI == 1 && R.recCap <= 0 && R.waveLevel.size() == nWaves && nWaves > 1;
+  LaneEvents lev;
+  if (twoLanes) {
+    if (!dev->side.stream && hipStreamCreateWithFlags(&dev->side.stream, hipStreamNonBlocking) != hipSuccess) {
+      dev->timing = savedTiming;
+      return fail(QSMI_ERR_HIP, "resident waves: side stream");
+    }
+    if (!dev->side.wb) dev->side.wb = new WaveBufs();
+    {
+      SideSwap sw(dev, true);
+      rc = grow_wave_bufs(dev, dev->wb, layers * std::max(maxWave, 1), layers * std::max(maxWave, 1) * R.n, K);
+      if (!rc && use_w1k(dev)) rc = ensure_tap_tables(dev, std::max(maxWave, 1), geom_of(dev).V);
+    }
+    for (int q = 0; q < 2 && !rc; ++q)
+      if (hipEventCreateWithFlags(&lev.e[q], hipEventDisableTiming) != hipSuccess)
+        rc = fail(QSMI_ERR_HIP, "resident waves: lane events");
+    if (rc) {
+      dev->timing = savedTiming;
+      return rc;
+    }
+  }
+  int prevLevel = -1, lane = 0;
+  bool used1 = false, need1 = false;
   for (size_t w = 0; w < nWaves; ++w) {
     const int w0 = R.waveStart[w], nN = R.waveStart[w + 1] - w0;
     if (nN <= 0) continue;
+    if (twoLanes) {
+      if (R.waveLevel[w] != prevLevel) {  // a new level starts on lane 0
+        if (prevLevel >= 0) {
+          (void)hipEventRecord(lev.e[0], dev->stream);
+          if (used1) {
+            (void)hipEventRecord(lev.e[1], dev->side.stream);
+            (void)hipStreamWaitEvent(dev->stream, lev.e[1], 0);
+          }
+          need1 = true;  // lane 1 waits for lane 0's earlier levels before its next wave
+        }
+        prevLevel = R.waveLevel[w];
+        lane = 0;
+        used1 = false;
+      } else {
+        lane ^= 1;
+      }
+      if (lane == 1) {
+        if (need1) (void)hipStreamWaitEvent(dev->side.stream, lev.e[0], 0);
+        need1 = false;
+        used1 = true;
+      }
+    }
+    SideSwap sideSwap(dev, twoLanes && lane == 1);
+    hipStream_t st = dev->stream;  // this wave's lane
+    WaveBufs* b = dev->wb;
     hipEvent_t* ev = timing ? &dev->evPool[w * 5] : nullptr;
     if (ev) (void)hipEventRecord(ev[0], st);
     hipLaunchKernelGGL(k_gather, dim3(nN), dim3(64), gatherLds, st, P, w0, nN, R.dOrder, R.dPathCells, R.dPathSeeds,
@@ -678,10 +765,12 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     (void)hipStreamSynchronize(st);
+    if (twoLanes) (void)hipStreamSynchronize(dev->side.stream);
     restore();
     return fail(QSMI_ERR_HIP, std::string("resident waves: ") + hipGetErrorString(e));
   }
   e = hipStreamSynchronize(st);
+  if (e == hipSuccess && twoLanes) e = hipStreamSynchronize(dev->side.stream);
   if (e == hipSuccess && bandStats) {
     for (int q = 0; q < 4; ++q) bandStats[q] = 0;
     long long one[4];
@@ -690,6 +779,13 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
       return rc;
     }
     for (int q = 0; q < 4; ++q) bandStats[q] += one[q];
+    if (twoLanes) {
+      if ((rc = band_read_stats(&dev->side.wb->band, one))) {
+        restore();
+        return rc;
+      }
+      for (int q = 0; q < 4; ++q) bandStats[q] += one[q];
+    }
     for (qsmi_device* x : R.extraDevs) {
       if ((rc = band_read_stats(&x->wb->band, one))) {
         restore();
