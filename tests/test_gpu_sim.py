@@ -509,3 +509,29 @@ def test_sim_bit_reproducible(kind):
     s1, i1, _ = g2s_amd.qs(ti, dims, di, sdims, types, n, k, 3, wave_size=256, tie_quantum=q)
     s2, i2, _ = g2s_amd.qs(ti, dims, di, sdims, types, n, k, 3, wave_size=256, tie_quantum=q)
     assert s1.tobytes() == s2.tobytes() and i1.tobytes() == i2.tobytes()
+
+
+@pytest.mark.parametrize("kind", ["continuous_1000", "categorical_generic"])
+def test_sim_wave_size_invariant(kind):
+    """The realisation does not depend on how path nodes are packed into
+    waves: 16-node waves (several waves per dependency level, alternating
+    between the loop's two streams), 256-node waves, and one wave per level
+    (one stream) give identical bytes."""
+    if kind == "continuous_1000":
+        w = synth.config2_continuous_2d()
+        ti, dims, types, n, k, q = w["ti"], w["ti_dims"], [0], 80, 1.5, 0.0
+        sdims = (200, 160)
+    else:
+        g = synth.grf((50, 50), 5.0, seed=22, anis=(3.0,), angle_deg=30.0)
+        ti, dims = synth.to_g2s((g > np.quantile(g, 0.72)).astype(np.float32))
+        types, n, k, q = [1], 15, 1.2, float(2.0 ** (np.ceil(np.log2(15)) - 16))
+        sdims = (60, 50)
+    di = np.full(int(np.prod(sdims)), np.nan, np.float32)
+    runs = []
+    for ws in (16, 256, 1 << 20):
+        _log(f"wave-size invariance {kind}: waves of <= {ws} nodes")
+        s, i, st = g2s_amd.qs(ti, dims, di, sdims, types, n, k, 5, wave_size=ws, tie_quantum=q)
+        runs.append((s.tobytes(), i.tobytes(), st["waves"]))
+    assert runs[0][2] > runs[2][2] and runs[1][2] >= runs[2][2]
+    for s, i, _ in runs[1:]:
+        assert s == runs[0][0] and i == runs[0][1]
