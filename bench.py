@@ -320,7 +320,9 @@ def main():
                     note=("the algorithmic bytes count a 4-byte map per TI cell written and read once; the "
                           "pencil and rows kernels never materialise that map (column transforms pass between "
                           "them), so achieved may exceed the HBM peak: the pencil is bound by VALU issue, and "
-                          "traffic is its measured HBM bytes per launch"))
+                          "traffic is its measured HBM bytes per launch. Consecutive waves of one dependency level "
+                          "run on two streams, so a launch's duration includes time shared with the other "
+                          "stream's kernels"))
     alt = None
     if args.alt_wave > 0 and args.alt_wave != args.wave:
         _, _, sa = g2s_amd.qs(ti, dims, di, sdims, cfg["types"], n, k, seed=cfg["seed"], wave_size=args.alt_wave,
