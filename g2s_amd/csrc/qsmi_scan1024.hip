@@ -49,45 +49,19 @@ constexpr int WSTRIDE = BUF + GSLOT + 2;
 constexpr int NW = 16;              // rows wavefronts per block (one block per CU, 4 per SIMD)
 constexpr int NWP = 12;             // pencil wavefronts per block = columns per group (3 per SIMD)
 constexpr int NGRP = (H0 + NWP - 1) / NWP;  // 43 column groups
-constexpr int GX_BASE = NGRP / NXCD;   // groups of every node kept on one XCD (5)
-constexpr int GX_EXTRA = NGRP % NXCD;  // the node's last groups, on rotating XCDs (3)
+// Column groups of a node on XCD x: x + 8 i (i < GX_BASE), so the XCD's L2
+// holds a fixed eighth of the TI spectra, and the last GX_EXTRA groups
+// 8 GX_BASE + e on XCD (node + node / 8 + e) mod 8: over every 8 nodes
+// n = 8 c .. 8 c + 7 each XCD takes GX_EXTRA of them, so all XCDs get the
+// same number of groups.
+constexpr int GX_BASE = NGRP / NXCD;   // 5
+constexpr int GX_EXTRA = NGRP % NXCD;  // 3
 constexpr int CT_PAIR = 520 * 2;    // float2 per row pair: 513 columns padded to 64-byte multiples
 constexpr long long CT_STRIDE = (long long)(F / 2) * CT_PAIR;  // float2 per node
 static_assert(WSTRIDE % 16 == 2, "buffer bank offset");
 static_assert((NWP * 16) % 64 == 0 && (CT_PAIR * 8) % 64 == 0, "64-byte aligned group stores");
 
 __device__ __forceinline__ int pidx(int e) { return e + (e >> 4); }
-
-// Pencil items of XCD x over a chunk of NXCD consecutive nodes: its r-th
-// (r < NGRP) is node offset o, column group cg. A node's groups x + 8 i
-// (i < GX_BASE) always go to XCD x, its last GX_EXTRA groups (8 GX_BASE + e)
-// to XCD (o + e) mod 8, so every XCD takes NGRP items per chunk (no XCD waits
-// for one that holds an extra group of every node) and reads a fixed eighth
-// of the TI spectra plus the last groups' columns.
-__device__ __forceinline__ void pencil_item(int x, int r, int& o, int& cg) {
-  o = 0;
-  for (; o < NXCD - 1; ++o) {
-    int ne = 0;
-    for (int e = 0; e < GX_EXTRA; ++e) ne += ((o + e) % NXCD) == x ? 1 : 0;
-    if (r < GX_BASE + ne) break;
-    r -= GX_BASE + ne;
-  }
-  if (r < GX_BASE) {
-    cg = x + NXCD * r;
-    return;
-  }
-  r -= GX_BASE;
-  cg = NXCD * GX_BASE;
-  for (int e = 0; e < GX_EXTRA; ++e) {
-    if (((o + e) % NXCD) == x) {
-      if (r == 0) {
-        cg = NXCD * GX_BASE + e;
-        return;
-      }
-      --r;
-    }
-  }
-}
 
 // orders this wavefront's LDS accesses across lanes (no hardware barrier)
 __device__ __forceinline__ void wave_sync() {
@@ -357,30 +331,24 @@ __global__ void __launch_bounds__(64 * NWP) k_pencil1024(
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave-uniform (scalar) from here on
   float2* buf = wbuf[w];
   float2* gbuf = wbuf[w] + BUF;
-  // block items: (node, column group cg), NGRP per XCD and chunk of NXCD
-  // nodes (pencil_item)
+  // a block takes whole nodes (node bX, bX + blocksX, ... of its XCD): the
+  // node's lists and tap twiddles are loaded and built once for all of the
+  // XCD's column groups of the node
   const int xcd = blockIdx.x % NXCD;
   const int blocksX = gridDim.x / NXCD, bX = blockIdx.x / NXCD;
   const int nAct = count ? min(*count, nNodes) : nNodes;
-  const long long items = (long long)((nAct + NXCD - 1) / NXCD) * NGRP;
   const int V = g.V;
-  for (long long it = bX; it < items; it += blocksX) {  // block-uniform
-    const int ch = (int)(it / NGRP);
-    int o, cg;
-    pencil_item(xcd, (int)(it - (long long)ch * NGRP), o, cg);
-    const int node = ch * NXCD + o;
-    if (node >= nAct) continue;  // block-uniform (the last chunk)
-    const int k0 = cg * NWP + w;
-    // the lane index, opaque to the compiler per item: lane-dependent
-    // addresses are recomputed per item instead of held (spilled) across
-    // the item loop
+  for (int node = bX; node < nAct; node += blocksX) {  // block-uniform
+    // the lane index, opaque to the compiler per node and per group:
+    // lane-dependent addresses are recomputed instead of held (spilled)
+    // across the loops
     int lx = l;
     asm volatile("" : "+v"(lx));
     lx &= 63;  // the range is known again
     const TapInfo1024* tp = taps + node;
     const uint16_t* td = tapD0 + (long long)node * kTapNbMax;
     const float* tv = tapVal + (long long)node * kTapNbMax * V;
-    // one round of loads for the item: every address below is known without
+    // one round of loads for the node: every address below is known without
     // the others' values (lists are read in full up to 128 slots and 64
     // taps; unused entries are ignored)
     const NodeInfo ni = info[node];
@@ -393,17 +361,30 @@ __global__ void __launch_bounds__(64 * NWP) k_pencil1024(
     const float svA0 = tv[lx * V], svA1 = tv[lx * V + (V > 1 ? 1 : 0)];
     const float svB0 = tv[(lx + 64) * V], svB1 = tv[(lx + 64) * V + (V > 1 ? 1 : 0)];
     if (ni.lo[1] > ni.hi[1] || ni.lo[0] > ni.hi[0]) continue;  // block-uniform
-    const bool colOk = k0 < H0;  // wave-uniform; idle waves still take the barriers
     const bool tapPath = kmax > 0;  // block-uniform
     if (tapPath) {
       // W(t, l) = exp(-2 pi i l r_t / 1024), the same for every column of the
-      // node: wave w builds the slots of taps w, w + NWP, ...
+      // node: wave w builds the slots of taps w, w + NWP, ... (the previous
+      // node's last store phase ended with a barrier)
       for (int t = w; t < T; t += NWP) {
         const int r = __builtin_amdgcn_readlane(myRow, t), sl = __builtin_amdgcn_readlane(mySlot, t);
         wtab[sl * 64 + lx] = twl[__umul24((uint32_t)lx, (uint32_t)r) & (F - 1)];
       }
       __syncthreads();
     }
+    const int lxn = lx;
+    for (int gi = 0; gi < GX_BASE + GX_EXTRA; ++gi) {  // block-uniform
+    int cg = xcd + NXCD * gi;
+    if (gi >= GX_BASE) {  // one of the node's last groups: on this XCD when its turn
+      const int e = gi - GX_BASE;
+      if ((node + node / NXCD + e) % NXCD != xcd) continue;
+      cg = NXCD * GX_BASE + e;
+    }
+    int lx = lxn;
+    asm volatile("" : "+v"(lx));
+    lx &= 63;
+    const int k0 = cg * NWP + w;
+    const bool colOk = k0 < H0;  // wave-uniform; idle waves still take the barriers
     if (colOk) {
       float2 M[16];
 #pragma unroll
@@ -620,6 +601,7 @@ __global__ void __launch_bounds__(64 * NWP) k_pencil1024(
       }
     }
     __syncthreads();
+    }  // column groups
   }
 }
 
@@ -937,9 +919,8 @@ void launch_pencil1024(const ScanGeom& g, const float2* YT, const float2* tw, in
                        float2* CT, const int* count, hipStream_t st) {
   hipLaunchKernelGGL(k_taps1024, dim3((unsigned)nNodes), dim3(64), 0, st, info, nbOff, nbVal, g.V, taps, tapD0, tapVal,
                      count);
-  // enough blocks for the XCD's items, at most one block per CU
-  const long long itemsX = (long long)((nNodes + NXCD - 1) / NXCD) * NGRP;
-  const int perX = (int)std::min<long long>(w1k_blocks() / NXCD, itemsX);
+  // at most one block per CU and per node of the XCD
+  const int perX = std::min(w1k_blocks() / NXCD, nNodes);
   hipLaunchKernelGGL(k_pencil1024, dim3((unsigned)(std::max(perX, 1) * NXCD)), dim3(64 * NWP), 0, st, g, YT, tw, info,
                      nbOff, nbVal, taps, tapD0, tapVal, CT, nNodes, count);
 }
