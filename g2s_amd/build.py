@@ -14,7 +14,7 @@ LIB = os.path.join(HERE, "libqsmi.so")
 
 SOURCES = ["qsmi_kernels.hip", "qsmi_scan1024.hip", "qsmi_band.hip", "qsmi_sim.hip", "qsmi_api.hip", "qsmi_lp.hip",
            "qsmi_host.cpp", "qsmi_calib.cpp", "qsmi_bgrid.cpp"]
-HEADERS = ["qsmi_fft.hpp", "qsmi_kernels.hpp", "qsmi_internal.hpp", "qsmi_select.hpp"]
+HEADERS = ["qsmi_fft.hpp", "qsmi_kernels.hpp", "qsmi_internal.hpp", "qsmi_select.hpp", "qsmi_taps.hpp"]
 
 FLAGS = [
     "-O3",

@@ -40,7 +40,7 @@ hipError_t spectra1024(const float* dSub, int s0, int s1, long long S, int nSub,
                        hipStream_t st);
 void launch_pencil1024(const ScanGeom& g, const float2* YT, const float2* tw, int nNodes, const NodeInfo* info,
                        const int* nbOff, const float* nbVal, TapInfo1024* taps, uint16_t* tapD0, float* tapVal,
-                       float2* CT, const int* count, hipStream_t st);
+                       float2* CT, const int* count, bool tapsBuilt, hipStream_t st);
 void launch_rows1024(const ScanGeom& g, const float2* tw, const NodeInfo* info, const float2* CT, int nNodes, int K,
                      bool quant, Cand* cand, float* map, const int* count, int* gcnt, hipStream_t st);
 }  // namespace qsmi
@@ -318,7 +318,7 @@ int ensure_tap_tables(qsmi_device* dev, int nNodes, int V) {
 }
 
 int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeInfo* info, const int* off,
-                    const float* val, uint32_t* ord, float2* C, const int* count) {
+                    const float* val, uint32_t* ord, float2* C, const int* count, bool tapsBuilt) {
   qsmi_ti* t = dev->ti;
   hipStream_t st = dev->stream;
   if (dev->timing) hipEventRecord(dev->ev[0], st);
@@ -326,7 +326,7 @@ int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeI
     int rc = ensure_tap_tables(dev, nNodes, g.V);
     if (rc) return rc;
     launch_pencil1024(g, t->dYT, t->dTw1024, nNodes, info, off, val, dev->dTap, dev->dTapD0, dev->dTapVal, C, count,
-                      st);
+                      tapsBuilt, st);
   } else if (count) {
     return fail(QSMI_ERR_UNSUPPORTED, "a device node count is supported on the 1024-grid path only");
   } else if (t->D == 1) {

@@ -189,8 +189,10 @@ int read_wave(qsmi_device* dev, WaveBufs* b, float* outErrors, unsigned* outPosi
 // count (device, optional; 1000x1000 path only): nodes [0, *count) of the
 // nNodes are scanned, the rest skipped at no cost
 // ord: work space of one uint32 per neighbour (the generic path sorts them)
+// tapsBuilt: the wave's tap lists are already in the device's tap tables
+// (the path loop's gather builds them)
 int launch_spectrum(qsmi_device* dev, const ScanGeom& g, int nNodes, const NodeInfo* info, const int* off,
-                    const float* val, uint32_t* ord, float2* C, const int* count = nullptr);
+                    const float* val, uint32_t* ord, float2* C, const int* count = nullptr, bool tapsBuilt = false);
 // tap tables of the 1000^2 spectrum stage for nNodes nodes per launch
 int ensure_tap_tables(qsmi_device* dev, int nNodes, int V);
 // crossOff > 0 (cross-measurement form, generic path): node q's count layer

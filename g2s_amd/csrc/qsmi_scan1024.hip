@@ -33,6 +33,7 @@
 
 #include "qsmi_kernels.hpp"
 #include "qsmi_select.hpp"
+#include "qsmi_taps.hpp"
 
 namespace qsmi {
 namespace w1k {
@@ -211,89 +212,16 @@ __device__ __forceinline__ void load_twiddles(V2* twl, const V2* tw) {
 using namespace w1k;
 
 // ---------------------------------------------------------------------------
-// Distinct neighbour rows ("taps", row = (-d1) mod 1024) of each node, one
-// wavefront per node: TapInfo1024 with the rows ordered by (row mod 16, row)
-// and every tap's slot range; the slots hold the neighbours in tap order (a
-// tap's neighbours in neighbour-list order, so every tap sum has a fixed
-// summation order), each with d0 mod 1024 (tapD0) and its V values (tapVal).
-// T = 0 (no lists) above kTapNbMax neighbours.
+// Tap lists of each node (build_taps1024), one wavefront per node.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(64) k_taps1024(const NodeInfo* __restrict__ info, const int* __restrict__ nbOff,
                                                 const float* __restrict__ nbVal, int V, TapInfo1024* __restrict__ taps,
                                                 uint16_t* __restrict__ tapD0, float* __restrict__ tapVal,
                                                 const int* __restrict__ nodeCount) {
-  __shared__ uint8_t flag[F];
-  __shared__ short tix[F];            // tap index of a present row
-  __shared__ short tapJ[kTapNbMax];   // tap of neighbour j
-  __shared__ int cntT[kTapNbMax];
-  __shared__ int startL[kTapNbMax + 1];
-  const int node = blockIdx.x, lane = threadIdx.x;
+  const int node = blockIdx.x;
   if (nodeCount && node >= *nodeCount) return;
   const NodeInfo ni = info[node];
-  TapInfo1024* tp = taps + node;
-  if (ni.nb > kTapNbMax || ni.nb <= 0) {  // block-uniform
-    if (lane == 0) {
-      tp->T = 0;
-      tp->passes = 0;
-    }
-    return;
-  }
-  for (int i = lane; i < F; i += 64) flag[i] = 0;
-  __syncthreads();
-  for (int t = lane; t < ni.nb; t += 64) flag[(-nbOff[(ni.nbStart + t) * 2 + 1]) & (F - 1)] = 1;
-  __syncthreads();
-  int count = 0, kmax = 0;
-  if (lane == 0) tp->cls[0] = 0;
-  for (int c = 0; c < 16; ++c) {
-    const int row = c + 16 * lane;  // the 64 rows of class c
-    const bool present = flag[row] != 0;
-    const unsigned long long mask = __ballot(present);
-    const int before = __popcll(mask & ((1ull << lane) - 1ull));
-    if (present) {  // count + before < nb <= kTapNbMax
-      tp->row[count + before] = (uint16_t)row;
-      tix[row] = (short)(count + before);
-      // tap path slot: the before-th tap of class c
-      if (count + before < kTapMax && before < kTapPass) tp->slot[count + before] = (uint8_t)(16 * before + c);
-    }
-    count += __popcll(mask);
-    kmax = max(kmax, __popcll(mask));
-    if (lane == 0) tp->cls[c + 1] = count;
-  }
-  const int T = count;
-  if (lane == 0) {
-    tp->T = T;
-    tp->passes = (T <= kTapMax && kmax <= kTapPass) ? kmax : 0;
-  }
-  for (int t = lane; t < T; t += 64) cntT[t] = 0;
-  __syncthreads();
-  for (int t = lane; t < ni.nb; t += 64) {
-    const short tt = tix[(-nbOff[(ni.nbStart + t) * 2 + 1]) & (F - 1)];
-    tapJ[t] = tt;
-    atomicAdd(&cntT[tt], 1);
-  }
-  __syncthreads();
-  if (lane == 0) {
-    int acc = 0;
-    for (int t = 0; t < T; ++t) {
-      startL[t] = acc;
-      tp->start[t] = (uint16_t)acc;
-      acc += cntT[t];
-    }
-    startL[T] = acc;
-    tp->start[T] = (uint16_t)acc;
-  }
-  __syncthreads();
-  uint16_t* od = tapD0 + (long long)node * kTapNbMax;
-  float* ov = tapVal + (long long)node * kTapNbMax * V;
-  for (int t = lane; t < ni.nb; t += 64) {
-    const short tt = tapJ[t];
-    int r = 0;
-    for (int q = 0; q < t; ++q) r += (tapJ[q] == tt) ? 1 : 0;
-    const int slot = startL[tt] + r;
-    const long long jj = ni.nbStart + t;
-    od[slot] = (uint16_t)(nbOff[jj * 2 + 0] & (F - 1));
-    for (int a = 0; a < V; ++a) ov[(long long)slot * V + a] = nbVal[jj * V + a];
-  }
+  build_taps1024(node, ni.nb, ni.nbStart, nbOff, nbVal, V, taps, tapD0, tapVal);
 }
 
 // ---------------------------------------------------------------------------
@@ -916,9 +844,10 @@ hipError_t spectra1024(const float* dSub, int s0, int s1, long long S, int nSub,
 
 void launch_pencil1024(const ScanGeom& g, const float2* YT, const float2* tw, int nNodes, const NodeInfo* info,
                        const int* nbOff, const float* nbVal, TapInfo1024* taps, uint16_t* tapD0, float* tapVal,
-                       float2* CT, const int* count, hipStream_t st) {
-  hipLaunchKernelGGL(k_taps1024, dim3((unsigned)nNodes), dim3(64), 0, st, info, nbOff, nbVal, g.V, taps, tapD0, tapVal,
-                     count);
+                       float2* CT, const int* count, bool tapsBuilt, hipStream_t st) {
+  if (!tapsBuilt)
+    hipLaunchKernelGGL(k_taps1024, dim3((unsigned)nNodes), dim3(64), 0, st, info, nbOff, nbVal, g.V, taps, tapD0,
+                       tapVal, count);
   // at most one block per CU and per node of the XCD
   const int perX = std::min(w1k_blocks() / NXCD, nNodes);
   hipLaunchKernelGGL(k_pencil1024, dim3((unsigned)(std::max(perX, 1) * NXCD)), dim3(64 * NWP), 0, st, g, YT, tw, info,

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "qsmi_internal.hpp"
+#include "qsmi_taps.hpp"
 
 namespace qsmi {
 
@@ -59,7 +60,9 @@ __device__ __forceinline__ uint32_t seed_bits_dev(float seed) {
 
 // One wavefront per node of the wave (order[w0 + q]). In the cross-
 // measurement form node q's count layer is written as node nN + q (same
-// neighbours, the count-layer values, an offset copy of its own).
+// neighbours, the count-layer values, an offset copy of its own). With taps
+// (1024-grid scans) the wavefront then builds the node's tap lists
+// (build_taps1024) from the neighbours it wrote.
 __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, const int* __restrict__ order,
                                                const unsigned* __restrict__ pathCells,
                                                const float* __restrict__ pathSeeds, const int* __restrict__ nbIdx,
@@ -71,7 +74,8 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
                                                const uint8_t* __restrict__ centreOk,
                                                const uint32_t* __restrict__ complete, uint32_t nComplete,
                                                const float* __restrict__ pathK, const int* __restrict__ nodeKer,
-                                               const int* __restrict__ nodeTi) {
+                                               const int* __restrict__ nodeTi, TapInfo1024* __restrict__ taps,
+                                               uint16_t* __restrict__ tapD0, float* __restrict__ tapVal) {
   const int q = blockIdx.x;
   const int lane = threadIdx.x;
   const int i = order[w0 + q];
@@ -124,6 +128,7 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
       info[q] = ni;
       if (P.cross) info[nN + q] = ni;
     }
+    if (taps) build_taps1024(q, 0, q * P.n, nbOff, nbVal, P.V, taps, tapD0, tapVal);  // no lists
     return;
   }
   int cc[3] = {0, 0, 0};
@@ -319,6 +324,10 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
       ni.delta0 = 0.f;
       info[nN + q] = ni;
     }
+  }
+  if (taps) {
+    __syncthreads();  // this wavefront's neighbour offsets and values are written
+    build_taps1024(q, cnt, q * P.n, nbOff, nbVal, P.V, taps, tapD0, tapVal);
   }
 }
 
@@ -680,9 +689,13 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
     WaveBufs* b = dev->wb;
     hipEvent_t* ev = timing ? &dev->evPool[w * 5] : nullptr;
     if (ev) (void)hipEventRecord(ev[0], st);
+    // plain 1024-grid waves: the gather also builds the tap lists
+    const bool fuseTaps = use_w1k(dev) && !R.lp && nTI == 1 && !dev->cross;
     hipLaunchKernelGGL(k_gather, dim3(nN), dim3(64), gatherLds, st, P, w0, nN, R.dOrder, R.dPathCells, R.dPathSeeds,
                        R.dNbIdx, R.dNbCnt, R.dOffs, R.dKernel, R.dCatvals, R.dDi, R.dIndex, R.dTi, b->dInfo, b->dOff,
-                       b->dVal, dev->dCentreOk, dev->dComplete, dev->nComplete, R.dPathK, R.dNodeKer, R.dNodeTi);
+                       b->dVal, dev->dCentreOk, dev->dComplete, dev->nComplete, R.dPathK, R.dNodeKer, R.dNodeTi,
+                       fuseTaps ? dev->dTap : nullptr, fuseTaps ? dev->dTapD0 : nullptr,
+                       fuseTaps ? dev->dTapVal : nullptr);
     if (ev) (void)hipEventRecord(ev[1], st);
     const bool useLp = R.lp && (R.waveLp.empty() || R.waveLp[w]);
     if (R.lp && !useLp) R.lpSkipped += nN;
@@ -746,7 +759,8 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
       launch_merge_ti(b->dInfo, nN, nTI, K, R.k, dValAll, dPosAll, tiStride, dev->quantum, R.dDi, R.dTi, R.nVar,
                       cum, R.dIndex, st);
     } else {
-      if ((rc = launch_spectrum(dev, g, layers * nN, b->dInfo, b->dOff, b->dVal, b->dOrd, b->dC))) return rc;
+      if ((rc = launch_spectrum(dev, g, layers * nN, b->dInfo, b->dOff, b->dVal, b->dOrd, b->dC, nullptr, fuseTaps)))
+        return rc;
       if (ev) (void)hipEventRecord(ev[2], st);
       launch_rows(dev, g, b->dInfo, b->dC, nN, rb, K, b->dCand, b->dGcnt, nullptr, nullptr, dev->cross ? nN : 0);
       if (ev) (void)hipEventRecord(ev[3], st);
