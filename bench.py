@@ -49,6 +49,9 @@ def parse():
                     help="also run a plain 256 MiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     ap.add_argument("--wave", type=int, default=256,
                     help="max path nodes per wave: the workload batches 256 path nodes per launch")
+    ap.add_argument("--stage-every", type=int, default=32,
+                    help="HIP events around the stages of every k-th wave of the timed steps (1: every wave; "
+                         "events cost ~1.5%% of the loop when recorded for every wave)")
     ap.add_argument("--alt-wave", type=int, default=512,
                     help="also time one realisation with waves of at most this many nodes (0: skip)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
@@ -291,7 +294,7 @@ def main():
     for s in range(args.steps):
         t0 = time.perf_counter()
         sim, idx, st = g2s_amd.qs(ti, dims, di, sdims, cfg["types"], n, k, seed=cfg["seed"] + s, wave_size=args.wave,
-                                  stage_timing=True)
+                                  stage_timing=args.stage_every)
         wall = time.perf_counter() - t0
         times.append(st["path_loop_ms"])
         stats.append(st)
@@ -317,6 +320,8 @@ def main():
                     kernel="k_pencil1024", avg_launch_us=pencil_ms * 1e3 / max(1, st["waves"]),
                     nodes_per_launch=nodes_per_launch, algorithmic_bytes_per_node=a_node,
                     traffic_source=(PROFILE if per_node else None),
+                    launch_timing=(f"HIP events on the stream of every {args.stage_every}-th wave of the timed "
+                                   "step (k_taps work included with the gather)"),
                     note=("the algorithmic bytes count a 4-byte map per TI cell written and read once; the "
                           "pencil and rows kernels never materialise that map (column transforms pass between "
                           "them), so achieved may exceed the HBM peak: the pencil is bound by VALU issue, and "
@@ -326,7 +331,7 @@ def main():
     alt = None
     if args.alt_wave > 0 and args.alt_wave != args.wave:
         _, _, sa = g2s_amd.qs(ti, dims, di, sdims, cfg["types"], n, k, seed=cfg["seed"], wave_size=args.alt_wave,
-                              stage_timing=True)
+                              stage_timing=args.stage_every)
         alt = dict(wave=args.alt_wave, value=sa["nodes"] / (sa["path_loop_ms"] / 1e3), ms_per_step=sa["path_loop_ms"],
                    waves=sa["waves"], spectrum_ms=sa["spectrum_ms"], rows_ms=sa["rows_ms"], merge_ms=sa["merge_ms"])
         print(f"[bench] waves <= {args.alt_wave}: {alt}", file=sys.stderr, flush=True)

@@ -378,6 +378,8 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
     fastest of all): ti has prod(ti_dims)*nVar values, di prod(sim_dims)*nVar
     (NaN = simulate). Returns (simulation, index, stats dict).
 
+    stage_timing: True (or 1) times every wave's stages with HIP events,
+    k > 1 every k-th wave (stage totals scaled to all waves), False off.
     record > 0: stats["pairs"] = (offsets [m,80,2], values [m,80], best [m])
     of the first `record` path nodes (2-D, one continuous variable).
     lp: a LearnedProposal with keys built for this TI (learned-proposal
@@ -417,7 +419,7 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
     rq.seed = int(seed)
     rq.waveSize = int(wave_size)
     rq.tieQuantum = float(tie_quantum)
-    rq.stageTiming = int(bool(stage_timing))
+    rq.stageTiming = int(stage_timing)
     rq.noVerbatim = int(bool(no_verbatim))
     rq.circularSim = int(bool(circular_sim))
     rq.circularTI = int(bool(circular_ti))

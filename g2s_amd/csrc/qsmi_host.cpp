@@ -806,7 +806,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     R.conv[B] = convTypes[B];
     R.subOf[B] = subVarOf[B];
   }
-  R.stageTiming = rq->stageTiming != 0;
+  R.stageTiming = std::max(0, rq->stageTiming);
   R.noVerbatim = rq->noVerbatim != 0;
   R.circSim = rq->circularSim != 0;
   std::vector<int> orderI(order.begin(), order.end());

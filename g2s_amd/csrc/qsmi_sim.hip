@@ -616,9 +616,10 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   const bool savedTiming = dev->timing;
   dev->timing = false;  // the per-call events of launch_spectrum are not used here
   const size_t nWaves = R.waveStart.size() > 0 ? R.waveStart.size() - 1 : 0;
-  const bool timing = R.stageTiming;
+  const bool timing = R.stageTiming > 0;
+  const size_t every = (size_t)std::max(1, R.stageTiming);  // timed waves: w % every == 0
   if (timing) {
-    const size_t need = nWaves * 5;
+    const size_t need = ((nWaves + every - 1) / every) * 5;
     while (dev->evPool.size() < need) {
       hipEvent_t ev;
       hipError_t e = hipEventCreate(&ev);
@@ -687,7 +688,7 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
     SideSwap sideSwap(dev, twoLanes && lane == 1);
     hipStream_t st = dev->stream;  // this wave's lane
     WaveBufs* b = dev->wb;
-    hipEvent_t* ev = timing ? &dev->evPool[w * 5] : nullptr;
+    hipEvent_t* ev = (timing && w % every == 0) ? &dev->evPool[(w / every) * 5] : nullptr;
     if (ev) (void)hipEventRecord(ev[0], st);
     // plain 1024-grid waves: the gather also builds the tap lists
     const bool fuseTaps = use_w1k(dev) && !R.lp && nTI == 1 && !dev->cross;
@@ -812,9 +813,13 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   if (e != hipSuccess) return fail(QSMI_ERR_HIP, std::string("resident waves: ") + hipGetErrorString(e));
   double stage[4] = {0, 0, 0, 0};
   if (timing) {
+    size_t timed = 0, all = 0;
     for (size_t w = 0; w < nWaves; ++w) {
       if (R.waveStart[w + 1] - R.waveStart[w] <= 0) continue;
-      hipEvent_t* ev = &dev->evPool[w * 5];
+      ++all;
+      if (w % every) continue;
+      ++timed;
+      hipEvent_t* ev = &dev->evPool[(w / every) * 5];
       float ms[4] = {0, 0, 0, 0};
       for (int q = 0; q < 4; ++q) (void)hipEventElapsedTime(&ms[q], ev[q], ev[q + 1]);
       stage[1] += ms[0];
@@ -822,6 +827,8 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
       stage[2] += ms[2];
       stage[3] += ms[3];
     }
+    // the timed waves stand for all of them
+    if (timed) for (int q = 0; q < 4; ++q) stage[q] *= (double)all / (double)timed;
   }
   for (int q = 0; q < 4; ++q) stageMsOut[q] = stage[q];
   *scannedOut = scanned;

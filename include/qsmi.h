@@ -308,7 +308,8 @@ typedef struct {
   int waveSize;            /* max nodes per wave, 0 -> 512 (256 with the learned proposal) */
   float tieQuantum;        /* < 0 automatic: 0 with a continuous variable,
                               2^(ceil(log2(n*max|w|))-16) otherwise */
-  int stageTiming;         /* 1: time every wave's kernels with HIP events (stageMs) */
+  int stageTiming;         /* k >= 1: time the kernels of every k-th wave with HIP
+                              events (stageMs, scaled to all waves); 0: off */
   /* optional training-pair record (2-D, one continuous variable, exact scan
      only): for path node i (path order, hard data excluded) with i <
      recordCap, its neighbours 1..n-1 (the cell itself excluded, nearest
@@ -376,7 +377,8 @@ typedef struct {
   double pathLoopMs;       /* out: neighbour search + waves + writes (the reference's timed loop) */
   double scanMs;           /* out: device time of the scan stages inside the loop */
   double stageMs[4];       /* out: per stage: [0] spectrum, [1] gather, [2] rows +
-                              select, [3] merge + draw + write (stageTiming) */
+                              select, [3] merge + draw + write (stageTiming; with
+                              k > 1 the timed waves' sum x waves / timed waves) */
   long long nodes;         /* out: simulated path nodes */
   long long scannedNodes;  /* out: nodes that went through the scan (the rest
                               drew from the marginal) */
