@@ -762,14 +762,18 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   // 768 no better); the learned proposal keeps its queries in LDS: 256
   const int waveSize = rq->waveSize > 0 ? rq->waveSize : (rq->lp ? 256 : 512);
   const auto tLv = clk::now();
-  // waves: each level cut into chunks of at most waveSize nodes
+  // waves: each level cut into the fewest waves of at most waveSize nodes,
+  // of equal sizes (the waves of a level run side by side on two streams)
   qsmi::ResidentLoop R;
   R.waveStart.push_back(0);
-  for (int l = 0; l <= maxLevel; ++l)
-    for (size_t w0 = lvStart[l]; w0 < lvStart[l + 1]; w0 += waveSize) {
-      R.waveStart.push_back((int)std::min(lvStart[l + 1], w0 + waveSize));
+  for (int l = 0; l <= maxLevel; ++l) {
+    const size_t n = lvStart[l + 1] - lvStart[l];
+    const size_t nw = (n + (size_t)waveSize - 1) / (size_t)waveSize;
+    for (size_t q = 1; q <= nw; ++q) {
+      R.waveStart.push_back((int)(lvStart[l] + n * q / nw));
       R.waveLevel.push_back(l);
     }
+  }
   for (int d = 0; d < 3; ++d) {
     R.sd[d] = (int)sd[d];
     R.kd[d] = (int)ker.dims[d];
