@@ -41,6 +41,9 @@ def measured_bytes_per_node(kernel):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", type=int, default=2, choices=(1, 2, 3, 4),
+                    help="2: the benchmark (1000^2 continuous); 1, 3, 4: the other parity workloads, timed "
+                         "alone (path loop, loop roofline; no learned form, calibration or CPU leg)")
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--size", type=int, default=1000, help="TI and simulation extent (default 1000)")
@@ -255,6 +258,67 @@ def calibration_leg(cfg, iters, cpu_sample):
     return out
 
 
+# The other parity workloads: (synth constructor, tie quantum, per-variable
+# kernel weights, algorithmic bytes per node V*S*8 + 2*N*4 with V active
+# spectra, S the half spectrum and N the FFT grid of the TI)
+OTHER_WORKLOADS = {
+    1: ("config1_categorical_2d", "auto", None, 2 * 126 * 250 * 8 + 2 * 250 * 250 * 4),
+    3: ("config3_categorical_3d", "auto", None, 3 * 91 * 150 * 120 * 8 + 2 * 180 * 150 * 120 * 4),
+    4: ("config4_multivariate_2d", 0.0, (0.5, 0.5), 6 * 251 * 500 * 8 + 2 * 500 * 500 * 4),
+}
+
+
+def other_workload(args):
+    """One parity workload at its full simulation grid (--sim-size shrinks
+    it): warmup, timed realisations, nodes/s and the loop's fraction of the
+    A_node roofline."""
+    import g2s_amd
+    from g2s_amd import synth
+
+    name, quantum, weights, a_node = OTHER_WORKLOADS[args.workload]
+    cfg = getattr(synth, name)()
+    ti, dims, sdims = cfg["ti"], cfg["ti_dims"], tuple(cfg["sim_dims"])
+    if args.sim_size:
+        sdims = tuple([args.sim_size] * len(sdims))
+    nvar = len(cfg["types"])
+    di = cfg.get("di")
+    if di is None or args.sim_size:
+        di = np.full(int(np.prod(sdims)) * nvar, np.nan, np.float32)
+    q = -1.0 if quantum == "auto" else quantum
+    kw = {}
+    if weights is not None:
+        kd = [d // 2 + 1 for d in dims]
+        kw = dict(kernel=np.tile(np.asarray(weights, np.float32), int(np.prod(kd))), kernel_dims=kd)
+    print(f"[bench] workload {args.workload}: TI {dims} sim {sdims} n={cfg['n']} k={cfg['k']}", file=sys.stderr,
+          flush=True)
+    for w in range(args.warmup):
+        g2s_amd.qs(ti, dims, di, sdims, cfg["types"], cfg["n"], cfg["k"], seed=1000 + w, wave_size=args.wave,
+                   tie_quantum=q, **kw)
+    times, st = [], None
+    for s in range(args.steps):
+        _, _, st = g2s_amd.qs(ti, dims, di, sdims, cfg["types"], cfg["n"], cfg["k"], seed=cfg["seed"] + s,
+                              wave_size=args.wave, tie_quantum=q, stage_timing=args.stage_every, **kw)
+        times.append(st["path_loop_ms"])
+        print(f"[bench] step {s}: {st}", file=sys.stderr, flush=True)
+    ms = float(np.mean(times))
+    value = st["nodes"] / (ms / 1e3)
+    return {
+        "metric": f"QS nodes/sec, parity workload {args.workload}",
+        "value": value,
+        "unit": "nodes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "config": {"workload": f"{name}: TI {dims}, sim {sdims}, n={cfg['n']}, k={cfg['k']}, waves<={args.wave}",
+                   "nodes_per_step": st["nodes"]},
+        "algorithmic_bytes_per_node": a_node,
+        "hbm_roofline_frac_whole_loop": a_node * value / (HBM_PEAK_GBS * 1e9),
+        "stats": st,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -266,6 +330,9 @@ def main():
     for kv in args.tune:
         key, val = kv.split("=")
         g2s_amd.set_default_tuning(key, int(val))
+    if args.workload != 2:
+        print(json.dumps(other_workload(args)))
+        return
     cfg = synth.config2_continuous_2d(args.size)
     if args.sim_size:
         cfg["sim_dims"] = (args.sim_size, args.sim_size)

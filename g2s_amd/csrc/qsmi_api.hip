@@ -527,7 +527,10 @@ int qsmi_ti_create(int gpu, int ndim, const unsigned* srcSize, const unsigned* f
                   t->stream);
     TCHK(hipGetLastError());
   }
-  t->w1k = (ndim == 2 && t->s[0] > 512 && t->s[1] > 512 && t->s[0] <= 1024 && t->s[1] <= 1024);
+  // the 1024 x 1024 grid holds any 2-D TI of extents up to 1024 (valid
+  // positions never wrap); below grid1024Min the TI's own FFT grid is used
+  const int g1k = t->tune.grid1024Min;
+  t->w1k = (ndim == 2 && t->s[0] >= g1k && t->s[1] >= g1k && t->s[0] <= 1024 && t->s[1] <= 1024);
   if (t->w1k) {
     std::vector<float2> tw(1024);
     std::vector<double2> twd(1024);
@@ -577,6 +580,7 @@ struct TuneKey {
 };
 const TuneKey kTuneKeys[] = {
     {"pencil.npb", &ScanTune::pencilNpb, 1, 16},
+    {"scan.grid1024_min", &ScanTune::grid1024Min, 2, 1025},
 };
 const TuneKey* find_tune_key(const char* key) {
   if (!key) return nullptr;

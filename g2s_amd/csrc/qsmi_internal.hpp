@@ -19,7 +19,9 @@ using qsmi::ScanGeom;
 // Launch parameters of the scan kernels for one TI geometry: defaults, set
 // by qsmi_ti_set_tuning, or chosen by measurement (qsmi_ti_autotune).
 struct ScanTune {
-  int pencilNpb = 1;  // generic pencil: path nodes per block
+  int pencilNpb = 1;       // generic pencil: path nodes per block
+  int grid1024Min = 320;   // 2-D TIs whose extents are all in [grid1024Min, 1024] scan on the 1024 x 1024
+                           // grid (read when the TI is created)
 };
 
 struct qsmi_ti {

@@ -87,7 +87,14 @@ int qsmi_ti_destroy(qsmi_ti* ti);
  * src/CPUThreadDevice.cpp:99-141). Keys:
  *   "pencil.npb"  path nodes per block of the pencil kernel of TIs scanned
  *                 on their own FFT grid (3-D, smaller than 513 or circular): 1..16
- * Results do not depend on them (every choice sums in the same order).
+ *   "scan.grid1024_min"  2-D TIs (not circular, no NaN) whose extents are
+ *                 all in [value, 1024] scan on a 1024 x 1024 grid instead of
+ *                 their own FFT grid: 2..1025 (1025: never), default 320
+ *                 (a 500 x 500 TI with six spectra: 2.5x the own-grid rate);
+ *                 read when the TI is created (set it with
+ *                 qsmi_set_default_tuning before qsmi_ti_create)
+ * Results do not depend on them up to the FP32 map error the exact selection
+ * absorbs (pencil.npb: every choice sums in the same order).
  * QSMI_ERR_ARG for an unknown key or a value out of range. */
 int qsmi_ti_set_tuning(qsmi_ti* ti, const char* key, int value);
 int qsmi_ti_get_tuning(const qsmi_ti* ti, const char* key, int* value);
