@@ -90,24 +90,29 @@ unsigned size_rule(unsigned src) {
   return src;
 }
 
-// Radix plan of one FFT length; prefers 10, 8, 4 (fewest LDS passes).
-bool make_plan(int L, FftPlan* p) {
+// Radix plan of one FFT length; prefers 10, 8, 4 (fewest LDS passes), no
+// radix above radixMax unless the length leaves no other factor.
+bool make_plan(int L, FftPlan* p, int radixMax) {
   p->L = L;
   p->nst = 0;
   int m = L;
   static const int pref[] = {10, 8, 4, 5, 3, 2, 7, 11, 13};
   while (m > 1) {
-    bool ok = false;
-    for (int r : pref) {
-      if (m % r == 0) {
-        if (p->nst >= kMaxStages) return false;
-        p->radix[p->nst++] = r;
-        m /= r;
-        ok = true;
+    int pick = 0;
+    for (int r : pref)
+      if (r <= radixMax && m % r == 0) {
+        pick = r;
         break;
       }
-    }
-    if (!ok) return false;
+    if (!pick)
+      for (int r : pref)
+        if (m % r == 0) {
+          pick = r;
+          break;
+        }
+    if (!pick || p->nst >= kMaxStages) return false;
+    p->radix[p->nst++] = pick;
+    m /= pick;
   }
   if (L == 1) {  // a length-1 transform is the identity: one radix-1 stage is not needed
     p->nst = 0;
@@ -115,10 +120,11 @@ bool make_plan(int L, FftPlan* p) {
   return true;
 }
 
-// largest power-of-two vector count with L*nvec <= kMaxVecElems and LDS <= 64 KiB
-int choose_lognv(int L) {
+// largest power-of-two vector count with L*nvec <= kMaxVecElems and LDS <= 64 KiB,
+// at most 2^lognvMax
+int choose_lognv(int L, int lognvMax) {
   int lognv = 0;
-  while (true) {
+  while (lognv < lognvMax) {
     int nv = 1 << (lognv + 1);
     if ((long long)L * nv > kMaxVecElems) break;
     if ((size_t)(L + 2 * L * nv) * sizeof(float2) > 64 * 1024) break;
@@ -442,11 +448,11 @@ int qsmi_ti_create(int gpu, int ndim, const unsigned* srcSize, const unsigned* f
   t->H = (long long)t->h0 * t->f[1] * t->f[2];
   t->nSub = nSub;
   for (int i = 0; i < 3; ++i) {
-    if (!make_plan(t->f[i], &t->plan[i])) {
+    if (!make_plan(t->f[i], &t->plan[i], t->tune.radixMax)) {
       delete t;
       return fail(QSMI_ERR_UNSUPPORTED, "FFT extent with a prime factor above 13");
     }
-    t->lognv[i] = choose_lognv(t->f[i]);
+    t->lognv[i] = choose_lognv(t->f[i], t->tune.lognvMax);
   }
   hipError_t e = hipSetDevice(gpu);
   if (e != hipSuccess) {
@@ -581,6 +587,8 @@ struct TuneKey {
 const TuneKey kTuneKeys[] = {
     {"pencil.npb", &ScanTune::pencilNpb, 1, 16},
     {"scan.grid1024_min", &ScanTune::grid1024Min, 2, 1025},
+    {"fft.radix_max", &ScanTune::radixMax, 2, 13},
+    {"fft.lognv_max", &ScanTune::lognvMax, 0, 6},
 };
 const TuneKey* find_tune_key(const char* key) {
   if (!key) return nullptr;

@@ -22,6 +22,8 @@ struct ScanTune {
   int pencilNpb = 1;       // generic pencil: path nodes per block
   int grid1024Min = 320;   // 2-D TIs whose extents are all in [grid1024Min, 1024] scan on the 1024 x 1024
                            // grid (read when the TI is created)
+  int radixMax = 13;       // largest radix of the own-grid FFT plans (read when the TI is created)
+  int lognvMax = 6;        // at most 2^lognvMax transforms per own-grid FFT block (read at creation)
 };
 
 struct qsmi_ti {
