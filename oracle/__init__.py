@@ -124,6 +124,7 @@ def lib():
     _sig(L, "orc_sim_set_index", None, [C.c_void_p, C.c_void_p])
     _sig(L, "orc_calib_offsets", C.c_int, [C.c_int, u32p, i32p, C.c_int])
     _sig(L, "orc_exclude_disk", None, [C.c_int, u32p, f32p, C.c_uint, C.c_float])
+    _sig(L, "orc_marginal_cell", C.c_uint, [C.c_void_p, C.c_size_t, C.c_int, C.c_float])
     _sig(L, "orc_calibration", C.c_int, [C.c_int, C.c_int, u32p, C.c_void_p, C.c_int, i32p, C.c_int, u32p, C.c_void_p,
                                          C.c_int, C.c_int, u32p, C.c_int, f32p, C.c_int, C.c_float, C.c_uint,
                                          C.c_uint, C.c_float, C.c_uint, C.c_float, f32p, f32p, u32p, C.c_int, i32p,
@@ -147,6 +148,7 @@ def ref():
     _sig(R, "ref_coef_matrix", C.c_int, [C.c_int, i32p, i32p, f32p, f32p, i32p, i32p, f32p])
     _sig(R, "ref_generate_kernel", None, [C.c_int, u32p, C.c_int, i32p, f32p, f32p, f32p])
     _sig(R, "ref_kernel_expand", C.c_int, [C.c_int, u32p, C.c_int, f32p, i32p, f32p])
+    _sig(R, "ref_offsets", C.c_int, [C.c_int, u32p, C.c_int, C.c_void_p, i32p])
     _sig(R, "ref_index_with_delta", C.c_int, [C.c_int, u32p, C.c_uint, i32p, u32p])
     _sig(R, "ref_complex_add", None, [f32p, f32p, f32p, C.c_float, C.c_uint])
     _sig(R, "ref_bgrid_serialize", C.c_size_t, [C.c_int, u32p, C.c_int, i32p, C.c_int, f32p, C.c_void_p, C.c_size_t])
@@ -229,6 +231,13 @@ def offsets(kdims, kernel=None, knvar=1):
         kp = kernel.ctypes.data
     n = lib().orc_offsets(len(kdims), kdims, knvar, kp, out)
     return out[: n * len(kdims)].reshape(n, len(kdims)).copy()
+
+
+def marginal_cell(complete, nvar, seed):
+    """The marginal draw's TI cell for `seed` (one TI, complete: 1 per cell
+    without NaN)."""
+    c = np.ascontiguousarray(complete, np.uint8)
+    return int(lib().orc_marginal_cell(c.ctypes.data, c.size, int(nvar), float(seed)))
 
 
 def path(informed, seed):

@@ -364,6 +364,25 @@ static void cross_map(OrcSimCtx* c, int nb, const std::vector<int>& off, const s
 
 // One node's scan and draw given the DI state. Writes the K best
 // (value, TI position) pairs sorted best first, and returns the TI position
+// The marginal draw of one TI (include/simulation.hpp:586-654): cell
+// floor(seed * cells); where that cell has a NaN variable, the walk over the
+// complete cells: rank r = floor(seed * (C / nVar)) * nVar with C the number
+// of complete cells, and the first cell whose running count of complete
+// cells reaches r (r = 0 stops at cell 0, complete or not).
+extern "C" unsigned orc_marginal_cell(const uint8_t* complete, size_t S, int nVar, float seed) {
+  const unsigned position = (unsigned)int(std::floor(seed * (float)(unsigned)S));
+  if (complete[position]) return position;
+  unsigned total = 0;
+  for (size_t q = 0; q < S; ++q) total += complete[q];
+  const unsigned r = (unsigned)int(std::floor(seed * (float)(total / (unsigned)nVar))) * (unsigned)nVar;
+  unsigned run = 0;
+  for (size_t q = 0; q < S; ++q) {
+    run += complete[q];
+    if (r <= run) return (unsigned)q;
+  }
+  return position;
+}
+
 // chosen by the draw (or the marginal draw when only the centre is a
 // neighbour, simulation.hpp:586-606). *nbOut = neighbour count.
 // -nV (include/simulation.hpp:567-580): the TI position the nearest
@@ -508,20 +527,8 @@ static unsigned node_impl(OrcSimCtx* c, const float* di, unsigned pathIdx, int K
       while ((int)t + 1 < c->nTI && q >= c->cumT[t + 1]) ++t;
       return (q - (unsigned)c->cumT[t]) * (unsigned)c->nTI + t;
     }
-    unsigned position = (unsigned)int(std::floor(seed * (float)(unsigned)c->S));
-    if (p.cross && !c->centreOk[position]) {
-      // simulation.hpp:617-654: rank among the complete cells, the first
-      // cell whose running count reaches it
-      unsigned total = 0;
-      for (size_t q = 0; q < c->S; ++q) total += c->centreOk[q];
-      const unsigned r = (unsigned)int(std::floor(seed * (float)(total / (unsigned)p.nVar))) * (unsigned)p.nVar;
-      unsigned run = 0;
-      for (size_t q = 0; q < c->S; ++q) {
-        run += c->centreOk[q];
-        if (r <= run) return (unsigned)q;
-      }
-    }
-    return position;
+    if (p.cross) return orc_marginal_cell(c->centreOk.data(), c->S, p.nVar, seed);
+    return (unsigned)int(std::floor(seed * (float)(unsigned)c->S));
   }
   if (c->nTI > 1) {
     const int tiReq = p.tiImage ? (int)p.tiImage[p.path[p.beginPath + pathIdx]] : -1;

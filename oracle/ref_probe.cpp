@@ -12,9 +12,12 @@
 //   DataImage::indexWithDelta         include/DataImage.hpp:343-376
 //   g2s::complexAddAlphaxCxD          include/complexMulti.hpp:58-67,303-331
 //   the same two with the cross-measurement layer (TIs with NaN)
+//   DataImage::distance2ToCenter      include/DataImage.hpp:521-530 (the
+//                                     offset sort key of src/qs.cpp:1055-1137)
 //   DataImage::serialize              include/DataImage.hpp:143-170
 //   DataImage::validateSerializedData include/DataImage.hpp:172-224
 // The glue below only marshals plain arrays in and out.
+#include <algorithm>
 #include <climits>
 #include <cstring>
 #include <cmath>
@@ -127,6 +130,61 @@ void ref_generate_kernel(int D, const unsigned* dims, int nVar, const int* ktype
   g2s::DataImage k = g2s::DataImage::genearteKernel(kt, ms, w, a);
   size_t n = k.dataSize();
   for (size_t i = 0; i < n; ++i) out[i] = k._data[i];
+}
+
+// Neighbour offsets of one kernel in qs.cpp's order (src/qs.cpp:1055-1137,
+// Euclidean search distance). Only the candidate generation is restated
+// here (every offset with |o_i| < (k_i + 1) / 2; per dimension the previous
+// list with o_i = 0, then with o_i = +1, -1, +2, -2, ...). The sort key image
+// (minus the squared distance to the centre), the centre, the NaN test and
+// the comparator's lookups are the compiled DataImage's (emptyCopy,
+// distance2ToCenter, indexWithDelta), sorted by std::sort like qs.cpp.
+// kernel: cells x kNVar (nullptr: no NaN removal). Returns the count.
+int ref_offsets(int D, const unsigned* kdims, int kNVar, const float* kernel, int* out) {
+  size_t cells = 1;
+  for (int i = 0; i < D; ++i) cells *= kdims[i];
+  std::vector<float> kdata = kernel ? std::vector<float>(kernel, kernel + cells * kNVar)
+                                    : std::vector<float>(cells * kNVar, 1.f);
+  std::vector<int> types(kNVar, 0);
+  g2s::DataImage kim = make_image(D, kdims, kNVar, types.data(), kdata.data());
+  std::vector<std::vector<int>> offs(1);
+  for (int i = 0; i < D; ++i) {
+    const int half = ((int)kdims[i] + 1) / 2;
+    std::vector<int> order = {0};  // the blocks' o_i: 0, +1, -1, +2, -2, ...
+    for (int j = 1; j < half; ++j) {
+      order.push_back(j);
+      order.push_back(-j);
+    }
+    std::vector<std::vector<int>> grown;
+    for (int o : order)
+      for (std::vector<int> v : offs) {
+        v.push_back(o);
+        grown.push_back(v);
+      }
+    offs.swap(grown);
+  }
+  g2s::DataImage w = kim.emptyCopy(true);
+  for (unsigned i = 0; i < w.dataSize(); ++i) w._data[i] = -w.distance2ToCenter(i);
+  unsigned centre = 0;
+  for (int i = D - 1; i >= 0; --i) centre = centre * w._dims[i] + (w._dims[i] - 1) / 2;
+  if (kernel) {
+    std::vector<std::vector<int>> kept;
+    for (auto& v : offs) {
+      unsigned at;
+      kim.indexWithDelta(at, centre, v);
+      if (!std::isnan(kim._data[at * (unsigned)kNVar])) kept.push_back(v);
+    }
+    offs.swap(kept);
+  }
+  std::sort(offs.begin(), offs.end(), [&](std::vector<int>& a, std::vector<int>& b) {
+    unsigned la, lb;
+    w.indexWithDelta(la, centre, a);
+    w.indexWithDelta(lb, centre, b);
+    return w._data[la] > w._data[lb];
+  });
+  for (size_t k = 0; k < offs.size(); ++k)
+    for (int i = 0; i < D; ++i) out[k * D + i] = offs[k][i];
+  return (int)offs.size();
 }
 
 int ref_kernel_expand(int D, const unsigned* dims, int nVarIn, const float* kin, const int* factor, float* out) {

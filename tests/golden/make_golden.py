@@ -20,6 +20,13 @@ and tests/golden/bgrid_vectors.npz:
                    size field or the length changed, and
                    DataImage::validateSerializedData's verdict on each
 
+and tests/golden/offsets_vectors.npz:
+
+  off_*            the neighbour offsets of a kernel in qs.cpp's order (the
+                   sort key image and its lookups from the compiled
+                   DataImage, std::sort), with and without NaN kernel cells,
+                   including the 501 x 501 kernel of the 1000^2 benchmark
+
 and tests/golden/cross_vectors.npz (TIs with NaN, cross-measurement form):
 
   cx_coef_*        DataImage::generateCoefMatrix4Xcorr with forXMeasurement
@@ -43,6 +50,7 @@ import oracle  # noqa: E402
 OUT = os.path.join(HERE, "ref_vectors.npz")
 OUT_BGRID = os.path.join(HERE, "bgrid_vectors.npz")
 OUT_CROSS = os.path.join(HERE, "cross_vectors.npz")
+OUT_OFFSETS = os.path.join(HERE, "offsets_vectors.npz")
 
 
 def ti_cases():
@@ -280,12 +288,43 @@ def write_cross():
     print(f"wrote {OUT_CROSS} ({os.path.getsize(OUT_CROSS)} bytes)")
 
 
+OFFSET_CASES = [((5, 5), False), ((7, 4), False), ((9,), False), ((5, 6, 3), False), ((11, 9, 7), False),
+                ((41, 41), False), ((501, 501), False), ((5, 5), True), ((7, 4), True), ((9,), True),
+                ((5, 6, 3), True), ((41, 41), True)]
+
+
+def write_offsets():
+    R = oracle.ref()
+    if R is None:
+        raise SystemExit("oracle/_ref is not built: run make -C oracle with the reference tree present")
+    rng = np.random.default_rng(97)
+    out = {}
+    for i, (kd, nan) in enumerate(OFFSET_CASES):
+        cells = int(np.prod(kd))
+        kern = None
+        if nan:
+            kern = np.ones(cells * 2, np.float32)
+            kern[rng.random(kern.size) < 0.3] = np.nan
+        cnt = int(np.prod([2 * ((d + 1) // 2) - 1 for d in kd]))
+        res = np.zeros(cnt * len(kd), np.int32)
+        n = R.ref_offsets(len(kd), oracle.u32(kd), 2, kern.ctypes.data if nan else None, res)
+        out[f"off_{i}_dims"] = np.array(kd, np.uint32)
+        if nan:
+            out[f"off_{i}_kernel"] = kern
+        out[f"off_{i}_out"] = res[: n * len(kd)].astype(np.int16)
+    np.savez_compressed(OUT_OFFSETS, **out)
+    print(f"wrote {OUT_OFFSETS} ({os.path.getsize(OUT_OFFSETS)} bytes)")
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "bgrid":
         write_bgrid()
     elif len(sys.argv) > 1 and sys.argv[1] == "cross":
         write_cross()
+    elif len(sys.argv) > 1 and sys.argv[1] == "offsets":
+        write_offsets()
     else:
         main()
         write_bgrid()
         write_cross()
+        write_offsets()

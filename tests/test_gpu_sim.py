@@ -535,3 +535,30 @@ def test_sim_wave_size_invariant(kind):
     assert runs[0][2] > runs[2][2] and runs[1][2] >= runs[2][2]
     for s, i, _ in runs[1:]:
         assert s == runs[0][0] and i == runs[0][1]
+
+
+@pytest.mark.parametrize("nvar", [1, 2])
+def test_sim_nan_ti_marginal_draw(nvar):
+    """n = 1 (only the cell itself): every node draws from the marginal. With
+    NaN cells in the TI the draw walks the complete cells
+    (include/simulation.hpp:586-654); the GPU's index of every cell equals
+    the oracle's draw for that node's seed."""
+    rng = np.random.default_rng(40 + nvar)
+    dims = (30, 20)
+    cells = dims[0] * dims[1]
+    ti = rng.standard_normal(cells * nvar).astype(np.float32)
+    ti[rng.random(ti.size) < 0.45] = np.nan
+    ti[0] = np.nan  # cell 0 incomplete
+    sdims = (17, 13)
+    di = np.full(int(np.prod(sdims)) * nvar, np.nan, np.float32)
+    _log(f"marginal draw, TI with NaN, {nvar} variable(s)")
+    sim, idx, st = g2s_amd.qs(ti, dims, di, sdims, [0] * nvar, 1, 1.5, 11)
+    complete = (~np.isnan(ti.reshape(cells, nvar)).any(axis=1)).astype(np.uint8)
+    p, s, b = g2s_amd.path(np.zeros(int(np.prod(sdims)), np.uint8), 11)
+    want = np.array([oracle.marginal_cell(complete, nvar, float(s[i])) for i in range(len(p))], np.int64)
+    got = idx[p].astype(np.int64)
+    walked = int(sum(not complete[int(np.floor(np.float32(s[i]) * np.float32(cells)))] for i in range(len(p))))
+    _log(f"  {len(p)} nodes, {walked} walked the complete cells")
+    assert walked > 0
+    assert np.array_equal(got, want)
+    assert np.array_equal(sim.reshape(-1, nvar), ti.reshape(cells, nvar)[idx], equal_nan=True)

@@ -190,3 +190,45 @@ def test_product_kernel_generation_golden():
         assert np.array_equal(out, GOLD[f"kern_{ki}_out"]), ki
         ki += 1
     assert ki >= 2
+
+
+OFFS = np.load(os.path.join(os.path.dirname(__file__), "golden", "offsets_vectors.npz"))
+
+
+def _offset_cases():
+    i = 0
+    while f"off_{i}_dims" in OFFS.files:
+        kern = OFFS[f"off_{i}_kernel"] if f"off_{i}_kernel" in OFFS.files else None
+        yield i, OFFS[f"off_{i}_dims"], kern, OFFS[f"off_{i}_out"].astype(np.int32)
+        i += 1
+
+
+def test_offset_order_golden():
+    """The oracle's offset list and the product's (qsmi_offsets) in qs.cpp's
+    exact order, ties included, against the order the compiled DataImage sort
+    key gives (src/qs.cpp:1055-1137), 1-3 D, with and without NaN kernel
+    cells, and the 501 x 501 kernel of the 1000^2 benchmark."""
+    import g2s_amd
+
+    n = 0
+    for i, kd, kern, want in _offset_cases():
+        want = want.reshape(-1, len(kd))
+        got = oracle.offsets(kd, kern, 2)
+        assert np.array_equal(got, want), (i, kd)
+        prod = np.asarray(g2s_amd.offsets([int(d) for d in kd], kern, 2)).reshape(-1, len(kd))
+        assert np.array_equal(prod, want), (i, kd)
+        n += 1
+    assert n >= 10
+
+
+@needs_ref
+def test_offset_order_live():
+    rng = np.random.default_rng(202)
+    for kd in ((13, 11), (6, 5, 4), (31,), (3, 3)):
+        kern = np.ones(int(np.prod(kd)) * 2, np.float32)
+        kern[rng.random(kern.size) < 0.25] = np.nan
+        for k in (None, kern):
+            cnt = int(np.prod([2 * ((d + 1) // 2) - 1 for d in kd]))
+            res = np.zeros(cnt * len(kd), np.int32)
+            m = R.ref_offsets(len(kd), oracle.u32(kd), 2, None if k is None else k.ctypes.data, res)
+            assert np.array_equal(oracle.offsets(kd, k, 2), res[: m * len(kd)].reshape(m, len(kd)))

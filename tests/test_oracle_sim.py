@@ -338,3 +338,48 @@ def test_several_tis_of_their_own_extents():
     chosen, _, _, nb = m.node(di0, 0)
     q = int(np.floor(np.float32(s0[b0]) * np.float32(a.size + bb.size)))
     assert nb == 1 and chosen == ((q * 2) if q < a.size else ((q - a.size) * 2 + 1))
+
+
+@pytest.mark.parametrize("nvar,frac", [(1, 0.7), (2, 0.5), (3, 0.3), (1, 0.05)])
+def test_nan_safe_marginal_draw(nvar, frac):
+    """The marginal draw with NaN cells (include/simulation.hpp:586-654),
+    checked by properties a plausible slip fails (an off-by-one rank, a
+    missing nVar factor, a walk that stops on an incomplete cell, a draw that
+    is not monotone): the cell floor(seed * cells) when it is complete; else
+    a complete cell, namely the r-th complete cell (1-based, from the sorted
+    list of complete cells) with r = floor(seed * floor(C / nVar)) * nVar, or
+    cell 0 when r = 0; walked cells non-decreasing in the seed; and across a
+    dense seed grid the walked cells are exactly those ranks' cells."""
+    rng = np.random.default_rng(17 + nvar)
+    S = 257
+    complete = (rng.random(S) < frac).astype(np.uint8)
+    complete[0] = 0  # cell 0 incomplete: the rank-0 case must still return it
+    complete[rng.integers(1, S)] = 1
+    C = int(complete.sum())
+    ranked = np.flatnonzero(complete)  # complete cells in order
+    seeds = np.sort(np.concatenate([np.linspace(0.0, 1.0, 40001, endpoint=False), rng.random(2000)]))
+    walked, prev = set(), -1
+    for s in seeds:
+        c = oracle.marginal_cell(complete, nvar, float(s))
+        p0 = int(np.floor(np.float32(s) * np.float32(S)))
+        if complete[p0]:
+            assert c == p0
+            continue
+        r = int(np.floor(np.float32(s) * np.float32(C // nvar))) * nvar
+        if r == 0:
+            assert c == 0
+        else:
+            assert complete[c] == 1 and c == ranked[r - 1], (s, c, r)
+        assert c >= prev or c == 0
+        prev = max(prev, c)
+        walked.add(c)
+    reach = {0} | {int(ranked[k * nvar - 1]) for k in range(1, C // nvar)}
+    # every walked cell is one of those, and the dense grid finds the ranks
+    # whose seed interval holds an incomplete first cell
+    assert walked <= reach
+    hit = set()
+    for k in range(1, C // nvar):
+        lo, hi = k / (C // nvar), (k + 1) / (C // nvar)
+        if any(not complete[int(np.floor(np.float32(s) * np.float32(S)))] for s in seeds[(seeds >= lo) & (seeds < hi)]):
+            hit.add(int(ranked[k * nvar - 1]))
+    assert hit <= walked
