@@ -79,7 +79,56 @@ def parse():
     return ap.parse_args()
 
 
-def learned_form(cfg, args, exact_ms):
+def field_stats(a, b, dims):
+    """How a realisation b reproduces a (same grid, dims in G2S order): the
+    two-sample KS distance of the values, the variogram gamma(h) = mean((z(x
+    + h) - z(x))^2) / 2 of log values at lags 1..32 along both axes (largest
+    relative difference), and the connectivity of the cells above a's 90th
+    percentile (4-connected clusters of scipy.ndimage.label): the share of
+    those cells in the largest cluster and tau(h), the probability that two
+    such cells h apart along either axis share a cluster."""
+    from scipy import ndimage
+
+    za = a.reshape(dims[1], dims[0]).astype(np.float64)
+    zb = b.reshape(dims[1], dims[0]).astype(np.float64)
+    sa, sb = np.sort(za.ravel()), np.sort(zb.ravel())
+    grid = np.concatenate([sa, sb])
+    ks = float(np.max(np.abs(np.searchsorted(sa, grid, side="right") / sa.size
+                             - np.searchsorted(sb, grid, side="right") / sb.size)))
+    lags = (1, 2, 4, 8, 16, 32)
+
+    def vario(z):
+        lz = np.log(np.maximum(z, 1e-30))
+        return np.array([[0.5 * np.mean((lz[:, h:] - lz[:, :-h]) ** 2) for h in lags],
+                         [0.5 * np.mean((lz[h:, :] - lz[:-h, :]) ** 2) for h in lags]])
+
+    va, vb = vario(za), vario(zb)
+    thr = np.quantile(za, 0.9)
+
+    def conn(z):
+        lab, nlab = ndimage.label(z > thr)
+        sizes = np.bincount(lab.ravel())[1:]
+        largest = float(sizes.max() / sizes.sum()) if nlab else 0.0
+        tau = []
+        for h in lags:
+            same = both = 0
+            for x0, x1 in ((lab[:, h:], lab[:, :-h]), (lab[h:, :], lab[:-h, :])):
+                m = (x0 > 0) & (x1 > 0)
+                both += int(m.sum())
+                same += int((m & (x0 == x1)).sum())
+            tau.append(same / max(both, 1))
+        return largest, np.array(tau)
+
+    la, ta = conn(za)
+    lb, tb = conn(zb)
+    return dict(ks=ks, variogram_lags=list(lags),
+                variogram_max_rel_diff=float(np.max(np.abs(vb - va) / va)),
+                connectivity_threshold="exact realisation's 90th percentile",
+                largest_cluster_share=[la, lb], tau=[ta.round(4).tolist(), tb.round(4).tolist()],
+                tau_max_abs_diff=float(np.max(np.abs(tb - ta))))
+
+
+def learned_form(cfg, args, exact_ms, exact_sim=None):
     """Learned-proposal QS on the same workload: pairs recorded by plain QS on
     other seeded GRF TIs (range 20, exp), the encoders trained on the GPU,
     keys built for this TI, then one timed realisation (every 64th node and
@@ -115,7 +164,8 @@ def learned_form(cfg, args, exact_ms):
     net.build_keys(ti, dims)
     kw = dict(lp=net, lp_shortlist=4096, lp_fallback_every=64, lp_fallback_mse=0.5, lp_min_cover=args.lp_min_cover)
     di = np.full(int(np.prod(sdims)), np.nan, np.float32)
-    _, _, st = g2s_amd.qs(ti, dims, di, sdims, cfg["types"], cfg["n"], cfg["k"], seed=cfg["seed"], **kw)
+    lsim, _, st = g2s_amd.qs(ti, dims, di, sdims, cfg["types"], cfg["n"], cfg["k"], seed=cfg["seed"], **kw)
+    stats = field_stats(exact_sim, lsim, sdims) if exact_sim is not None else None
     a = 300
     au = dict(lp_audit_nodes=0, lp_audit_hits=0, lp_audit_best_hits=0)
     if args.lp_audit:
@@ -132,6 +182,7 @@ def learned_form(cfg, args, exact_ms):
                recall_audit_sample=f"{a}x{a} realisation on the same TI, {au['lp_audit_nodes']} proposed nodes",
                training=dict(pairs=int(sum(len(p[2]) for _, _, p in sets)), steps=args.lp_steps * args.lp_epochs *
                              len(sets), seconds=t_rec + t_train, final_loss=float(losses[-1][-1])),
+               vs_exact_realisation=stats,
                rules=f"exact scan for every 64th node, nodes with < {args.lp_min_cover:.0%} of neighbours on the key "
                      "stencil, best proposed mismatch > 0.5 sigma^2 per neighbour, shortlist overflow")
     net.close()
@@ -402,7 +453,7 @@ def main():
         alt = dict(wave=args.alt_wave, value=sa["nodes"] / (sa["path_loop_ms"] / 1e3), ms_per_step=sa["path_loop_ms"],
                    waves=sa["waves"], spectrum_ms=sa["spectrum_ms"], rows_ms=sa["rows_ms"], merge_ms=sa["merge_ms"])
         print(f"[bench] waves <= {args.alt_wave}: {alt}", file=sys.stderr, flush=True)
-    learned = learned_form(cfg, args, ms) if args.learned else None
+    learned = learned_form(cfg, args, ms, sim) if args.learned else None
     calib = calibration_leg(cfg, args.calib_iters, args.cpu_sample) if args.calib_iters > 0 else None
     if calib:
         print(f"[bench] calibration: {calib}", file=sys.stderr, flush=True)
