@@ -348,7 +348,7 @@ def other_workload(args):
     times, st = [], None
     for s in range(args.steps):
         _, _, st = g2s_amd.qs(ti, dims, di, sdims, cfg["types"], cfg["n"], cfg["k"], seed=cfg["seed"] + s,
-                              wave_size=args.wave, tie_quantum=q, stage_timing=args.stage_every, **kw)
+                              wave_size=args.wave, tie_quantum=q, stage_timing=args.stage_every, progress=30, **kw)
         times.append(st["path_loop_ms"])
         print(f"[bench] step {s}: {st}", file=sys.stderr, flush=True)
     ms = float(np.mean(times))

@@ -371,13 +371,14 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
             wave_size=0, tie_quantum=-1.0, stage_timing=False, record=0, lp=None, lp_shortlist=0,
             lp_fallback_every=0, lp_fallback_mse=0.0, lp_audit=False, lp_min_cover=0.0, no_verbatim=False,
             n_image=None, k_image=None, kernels=None, kernels_dims=None, kernel_index_image=None,
-            circular_sim=False, circular_ti=False, ti_index_image=None, ti_dims_all=None):
+            circular_sim=False, circular_ti=False, ti_index_image=None, ti_dims_all=None, progress=0):
     """In-memory QuickSampling (qsmi_qs_run_in_memory).
 
     Arrays are flat in G2S DataImage order (dimension 0 fastest, variable
     fastest of all): ti has prod(ti_dims)*nVar values, di prod(sim_dims)*nVar
     (NaN = simulate). Returns (simulation, index, stats dict).
 
+    progress > 0: a progress line on stderr about every `progress` seconds.
     stage_timing: True (or 1) times every wave's stages with HIP events,
     k > 1 every k-th wave (stage totals scaled to all waves), False off.
     record > 0: stats["pairs"] = (offsets [m,80,2], values [m,80], best [m])
@@ -420,6 +421,7 @@ def qs(ti, ti_dims, di, sim_dims, var_types, n, k, seed, kernel=None, kernel_dim
     rq.waveSize = int(wave_size)
     rq.tieQuantum = float(tie_quantum)
     rq.stageTiming = int(stage_timing)
+    rq.progressSeconds = int(progress)
     rq.noVerbatim = int(bool(no_verbatim))
     rq.circularSim = int(bool(circular_sim))
     rq.circularTI = int(bool(circular_ti))

@@ -203,6 +203,7 @@ class QsRequest(C.Structure):
         ("tiIndexImage", C.c_void_p),
         ("circularTI", C.c_int),
         ("tiSizes", C.c_void_p),
+        ("progressSeconds", C.c_int),
     ]
 
 

@@ -811,6 +811,7 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
     R.subOf[B] = subVarOf[B];
   }
   R.stageTiming = std::max(0, rq->stageTiming);
+  R.progressSec = std::max(0, rq->progressSeconds);
   R.noVerbatim = rq->noVerbatim != 0;
   R.circSim = rq->circularSim != 0;
   std::vector<int> orderI(order.begin(), order.end());

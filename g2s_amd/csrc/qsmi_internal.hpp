@@ -260,6 +260,7 @@ struct ResidentLoop {
   int conv[16], subOf[16];
   std::vector<int> waveStart;  // wave w = order[waveStart[w] .. waveStart[w+1])
   std::vector<int> waveLevel;  // dependency level of wave w (empty: one stream)
+  int progressSec = 0;          // > 0: a progress line on stderr about every that many seconds
   int stageTiming = 0;         // k >= 1: events around the stages of every k-th wave
   bool noVerbatim = false;     // -nV: no candidate at the continuation of the nearest neighbour's copy
   std::vector<qsmi_device*> extraDevs;  // TIs 1.. of a several-TI run (same extent as this device's)

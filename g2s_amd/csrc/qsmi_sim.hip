@@ -11,6 +11,8 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -493,6 +495,45 @@ struct LaneEvents {
   }
 };
 
+// Progress of the path loop (include/simulation.hpp:685-692 reports the
+// share of simulated cells): every kStep waves an event on the device's
+// stream; before one is reused the host waits for it, so the host runs at
+// most kRing * kStep waves ahead of the GPU and a line on stderr, printed
+// about every `seconds`, tells how far the GPU is.
+struct Progress {
+  static constexpr size_t kStep = 64, kRing = 8;
+  int seconds = 0;
+  hipEvent_t ev[kRing] = {};
+  bool used[kRing] = {};
+  std::chrono::steady_clock::time_point t0, last;
+  explicit Progress(int s) : seconds(s) {
+    if (seconds <= 0) return;
+    for (auto& e : ev)
+      if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) e = nullptr;
+    t0 = last = std::chrono::steady_clock::now();
+  }
+  ~Progress() {
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  // before wave w is enqueued (waves w0 of the waves' first node)
+  void tick(size_t w, size_t nWaves, long long nodesBefore, long long nodesAll, hipStream_t st) {
+    if (seconds <= 0 || w == 0 || w % kStep) return;
+    const size_t slot = (w / kStep) % kRing;
+    if (!ev[slot]) return;
+    if (used[slot]) (void)hipEventSynchronize(ev[slot]);  // the GPU has passed wave w - kRing * kStep
+    (void)hipEventRecord(ev[slot], st);
+    used[slot] = true;
+    const auto now = std::chrono::steady_clock::now();
+    if (std::chrono::duration<double>(now - last).count() < seconds) return;
+    last = now;
+    const size_t done = w > kRing * kStep ? w - kRing * kStep : 0;
+    std::fprintf(stderr, "[qsmi] path loop: %zu of %zu waves done, %lld of %lld nodes enqueued, %.0f s\n", done,
+                 nWaves, nodesBefore, nodesAll, std::chrono::duration<double>(now - t0).count());
+    std::fflush(stderr);
+  }
+};
+
 }  // namespace
 
 int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsOut, long long* scannedOut,
@@ -660,9 +701,12 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
   }
   int prevLevel = -1, lane = 0;
   bool used1 = false, need1 = false;
+  Progress progress(R.progressSec);
+  const long long nodesAll = nWaves ? (long long)R.waveStart[nWaves] : 0;
   for (size_t w = 0; w < nWaves; ++w) {
     const int w0 = R.waveStart[w], nN = R.waveStart[w + 1] - w0;
     if (nN <= 0) continue;
+    progress.tick(w, nWaves, w0, nodesAll, dev->stream);
     if (twoLanes) {
       if (R.waveLevel[w] != prevLevel) {  // a new level starts on lane 0
         if (prevLevel >= 0) {

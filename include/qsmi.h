@@ -375,6 +375,10 @@ typedef struct {
                               (the TI grid where its extents are FFT sizes) */
   const unsigned* tiSizes; /* several TIs of their own extents: nTI x 3 (entry 0 = tiSize);
                               NULL: tiSize for all */
+  int progressSeconds;     /* > 0: a progress line on stderr about every that many
+                              seconds of the path loop (simulated nodes so far; the
+                              reference reports its progress the same way,
+                              include/simulation.hpp:685-692); 0: quiet */
 } qsmi_qs_request;
 
 typedef struct {
