@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--stage-every", type=int, default=32,
                     help="HIP events around the stages of every k-th wave of the timed steps (1: every wave; "
                          "events cost ~1.5%% of the loop when recorded for every wave)")
+    ap.add_argument("--graph-steps", type=int, default=1,
+                    help="also time the step with each wave replayed as a captured HIP graph (0: skip)")
     ap.add_argument("--alt-wave", type=int, default=512,
                     help="also time one realisation with waves of at most this many nodes (0: skip)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
@@ -453,6 +455,19 @@ def main():
         alt = dict(wave=args.alt_wave, value=sa["nodes"] / (sa["path_loop_ms"] / 1e3), ms_per_step=sa["path_loop_ms"],
                    waves=sa["waves"], spectrum_ms=sa["spectrum_ms"], rows_ms=sa["rows_ms"], merge_ms=sa["merge_ms"])
         print(f"[bench] waves <= {args.alt_wave}: {alt}", file=sys.stderr, flush=True)
+    graph = None
+    if args.graph_steps:
+        # the same step with each wave's kernels replayed as one captured HIP
+        # graph per stream (tuning key loop.graph; identical realisation)
+        g2s_amd.set_default_tuning("loop.graph", 1)
+        try:
+            sg, _, sgs = g2s_amd.qs(ti, dims, di, sdims, cfg["types"], n, k, seed=cfg["seed"] + args.steps - 1,
+                                    wave_size=args.wave)
+        finally:
+            g2s_amd.set_default_tuning("loop.graph", 0)
+        graph = dict(value=sgs["nodes"] / (sgs["path_loop_ms"] / 1e3), ms_per_step=sgs["path_loop_ms"],
+                     waves=sgs["waves"], identical_realisation=bool(np.array_equal(sg, sim)))
+        print(f"[bench] captured wave steps: {graph}", file=sys.stderr, flush=True)
     learned = learned_form(cfg, args, ms, sim) if args.learned else None
     calib = calibration_leg(cfg, args.calib_iters, args.cpu_sample) if args.calib_iters > 0 else None
     if calib:
@@ -483,6 +498,7 @@ def main():
         "learned_proposal": learned,
         "calibration": calib,
         "alt_wave": alt,
+        "graph_steps": graph,
         "stats": st,
     }
     print(json.dumps(out))

@@ -9,7 +9,9 @@ tuning keys (``qsmi_set_default_tuning``) and keeps the fastest:
 * on the own grid: the largest FFT radix of the plans (``fft.radix_max``),
   the transforms per FFT block (``fft.lognv_max``) and the path nodes per
   pencil block (``pencil.npb``), one key after the other (coordinate
-  descent from the defaults).
+  descent from the defaults);
+* on the 1024 x 1024 grid: launched kernels against one captured HIP graph
+  per wave step (``loop.graph``).
 
 Choices are cached in a JSON file keyed by the geometry; a later call with
 the same cache reads the choice instead of measuring again. The choice is
@@ -27,7 +29,7 @@ import numpy as np
 
 from . import qs, set_default_tuning
 
-DEFAULTS = {"scan.grid1024_min": 320, "fft.radix_max": 13, "fft.lognv_max": 6, "pencil.npb": 1}
+DEFAULTS = {"scan.grid1024_min": 320, "fft.radix_max": 13, "fft.lognv_max": 6, "pencil.npb": 1, "loop.graph": 0}
 OWN_GRID_CANDIDATES = (("fft.radix_max", (13, 8, 5)), ("fft.lognv_max", (6, 4, 3)), ("pencil.npb", (1, 2, 4)))
 
 
@@ -110,6 +112,9 @@ def autotune(ti, ti_dims, var_types, n, k, cache=None, sim_dims=None, repeats=3,
         town = trial("scan.grid1024_min=1025", dict(best, **{"scan.grid1024_min": 1025}))
         own_grid = town <= t1024
         best["scan.grid1024_min"] = 1025 if own_grid else 2
+    if not own_grid:
+        times = {v: trial(f"loop.graph={v}", dict(best, **{"loop.graph": v})) for v in (0, 1)}
+        best["loop.graph"] = min(times, key=times.get)
     if own_grid:
         for tkey, values in OWN_GRID_CANDIDATES:
             times = {v: trial(f"{tkey}={v}", dict(best, **{tkey: v})) for v in values}

@@ -589,6 +589,7 @@ const TuneKey kTuneKeys[] = {
     {"scan.grid1024_min", &ScanTune::grid1024Min, 2, 1025},
     {"fft.radix_max", &ScanTune::radixMax, 2, 13},
     {"fft.lognv_max", &ScanTune::lognvMax, 0, 6},
+    {"loop.graph", &ScanTune::loopGraph, 0, 1},
 };
 const TuneKey* find_tune_key(const char* key) {
   if (!key) return nullptr;

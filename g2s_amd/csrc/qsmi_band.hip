@@ -565,6 +565,11 @@ void band_view(BandBufs* b, void* base, int K) {
   b->K = K;
 }
 
+int band_reserve(qsmi_device* dev, BandBufs* bb, int nNodes, int K) {
+  const int blocks = use_w1k(dev) ? rows1024_groups() : row_blocks(dev);
+  return band_grow(bb, nNodes, exact_selection_on(dev) ? blocks : 1, K);
+}
+
 int band_read_stats(const BandBufs* b, long long out[4]) {
   unsigned long long v[4] = {0, 0, 0, 0};
   if (b->stats && hipMemcpy(v, b->stats, sizeof(v), hipMemcpyDeviceToHost) != hipSuccess)

@@ -24,6 +24,8 @@ struct ScanTune {
                            // grid (read when the TI is created)
   int radixMax = 13;       // largest radix of the own-grid FFT plans (read when the TI is created)
   int lognvMax = 6;        // at most 2^lognvMax transforms per own-grid FFT block (read at creation)
+  int loopGraph = 0;       // 1: the path loop replays each wave's kernels as one captured HIP graph
+                           // (1024-grid scans on two lanes)
 };
 
 struct qsmi_ti {
@@ -218,6 +220,9 @@ void launch_merge_draw(const ScanGeom& g, const NodeInfo* info, const Cand* cand
 // take it (no tie quantum, no NaN TI, a rows grid it can track).
 bool exact_selection_on(const qsmi_device* dev);
 int band_grow(BandBufs* b, int nNodes, int blocksPerNode, int K);
+// band_grow for band_select calls of up to nNodes nodes of this device (one
+// coefficient layer), so that such calls allocate nothing
+int band_reserve(qsmi_device* dev, BandBufs* bb, int nNodes, int K);
 void band_free(BandBufs* b);
 // counters accumulated since the buffers were made: exact-selection nodes,
 // listed (rescored) positions, second-pass rows blocks, full-fallback nodes

@@ -64,7 +64,8 @@ __device__ __forceinline__ uint32_t seed_bits_dev(float seed) {
 // measurement form node q's count layer is written as node nN + q (same
 // neighbours, the count-layer values, an offset copy of its own). With taps
 // (1024-grid scans) the wavefront then builds the node's tap lists
-// (build_taps1024) from the neighbours it wrote.
+// (build_taps1024) from the neighbours it wrote. wave (a replayed graph):
+// {w0, nN} read on the device, blocks from nN on return.
 __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, const int* __restrict__ order,
                                                const unsigned* __restrict__ pathCells,
                                                const float* __restrict__ pathSeeds, const int* __restrict__ nbIdx,
@@ -77,9 +78,15 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
                                                const uint32_t* __restrict__ complete, uint32_t nComplete,
                                                const float* __restrict__ pathK, const int* __restrict__ nodeKer,
                                                const int* __restrict__ nodeTi, TapInfo1024* __restrict__ taps,
-                                               uint16_t* __restrict__ tapD0, float* __restrict__ tapVal) {
+                                               uint16_t* __restrict__ tapD0, float* __restrict__ tapVal,
+                                               const int* __restrict__ wave) {
   const int q = blockIdx.x;
   const int lane = threadIdx.x;
+  if (wave) {
+    w0 = wave[0];
+    nN = wave[1];
+    if (q >= nN) return;
+  }
   const int i = order[w0 + q];
   const unsigned cell = pathCells[i];
   const float seed = pathSeeds[i];
@@ -534,6 +541,39 @@ struct Progress {
   }
 };
 
+// Captured wave steps (tuning key loop.graph): the kernels of one plain
+// 1024-grid wave are captured once per lane at the largest wave size and
+// the graph is replayed for each of the lane's waves. Its first node,
+// k_wave_next, copies the lane's next {w0, nN} from the lane's list to
+// cur[0..1] and advances the cursor cur[2]; the gather reads {w0, nN} there
+// and every later kernel reads nN (cur + 1) as its device node count, so a
+// replay needs no parameter update.
+__global__ void k_wave_next(const int2* __restrict__ desc, int* __restrict__ cur) {
+  if (threadIdx.x == 0) {
+    const int j = cur[2];
+    const int2 d = desc[j];
+    cur[0] = d.x;
+    cur[1] = d.y;
+    cur[2] = j + 1;
+  }
+}
+
+struct WaveGraphs {
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  int2* dDesc = nullptr;  // [2][nWaves]: each lane's waves {w0, nN} in launch order
+  int* dCur = nullptr;    // [2][4]: {w0, nN, cursor, unused} of each lane
+  ~WaveGraphs() {
+    if (exec[0] || exec[1]) (void)hipDeviceSynchronize();  // replays still in flight on an error return
+    for (int q = 0; q < 2; ++q) {
+      if (exec[q]) (void)hipGraphExecDestroy(exec[q]);
+      if (graph[q]) (void)hipGraphDestroy(graph[q]);
+    }
+    if (dDesc) (void)hipFree(dDesc);
+    if (dCur) (void)hipFree(dCur);
+  }
+};
+
 }  // namespace
 
 int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsOut, long long* scannedOut,
@@ -699,6 +739,77 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
       return rc;
     }
   }
+  // captured wave steps: the plain 1024-grid loop on two lanes (no sampled
+  // stage events then)
+  const bool graphOn = t->tune.loopGraph && twoLanes && use_w1k(dev) && !dev->cross;
+  WaveGraphs G;
+  if (graphOn) {
+    std::vector<int2> desc[2];
+    int prev = -1, ln = 0;  // the lanes the loop below gives the waves
+    for (size_t w = 0; w < nWaves; ++w) {
+      const int w0 = R.waveStart[w], nN = R.waveStart[w + 1] - w0;
+      if (nN <= 0) continue;
+      if (R.waveLevel[w] != prev) {
+        prev = R.waveLevel[w];
+        ln = 0;
+      } else {
+        ln ^= 1;
+      }
+      desc[ln].push_back(make_int2(w0, nN));
+    }
+    const int cur0[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (hipMalloc(&G.dDesc, 2 * nWaves * sizeof(int2)) != hipSuccess || hipMalloc(&G.dCur, sizeof(cur0)) != hipSuccess)
+      rc = fail(QSMI_ERR_NOMEM, "wave graphs: out of device memory");
+    for (int q = 0; q < 2 && !rc; ++q)
+      if (!desc[q].empty() && hipMemcpy(G.dDesc + (size_t)q * nWaves, desc[q].data(), desc[q].size() * sizeof(int2),
+                                        hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(QSMI_ERR_HIP, "wave graphs: wave lists");
+    if (!rc && hipMemcpy(G.dCur, cur0, sizeof(cur0), hipMemcpyHostToDevice) != hipSuccess)
+      rc = fail(QSMI_ERR_HIP, "wave graphs: cursors");
+    // every buffer at the largest wave: the captured calls allocate nothing
+    for (int q = 0; q < 2 && !rc; ++q) {
+      SideSwap sw(dev, q == 1);
+      rc = band_reserve(dev, &dev->wb->band, maxWave, K);
+    }
+    if (rc) {
+      dev->timing = savedTiming;
+      return rc;
+    }
+  }
+  // captures the wave step of the current lane (the device's stream and
+  // buffers, swapped in by SideSwap)
+  auto captureLane = [&](int ln) -> int {
+    hipStream_t cs = dev->stream;
+    WaveBufs* cb = dev->wb;
+    int* cur = G.dCur + 4 * ln;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return fail(QSMI_ERR_HIP, std::string("wave graph: capture: ") + hipGetErrorString(e));
+    hipLaunchKernelGGL(k_wave_next, dim3(1), dim3(64), 0, cs, G.dDesc + (size_t)ln * nWaves, cur);
+    hipLaunchKernelGGL(k_gather, dim3(maxWave), dim3(64), gatherLds, cs, P, 0, maxWave, R.dOrder, R.dPathCells,
+                       R.dPathSeeds, R.dNbIdx, R.dNbCnt, R.dOffs, R.dKernel, R.dCatvals, R.dDi, R.dIndex, R.dTi,
+                       cb->dInfo, cb->dOff, cb->dVal, dev->dCentreOk, dev->dComplete, dev->nComplete, R.dPathK,
+                       R.dNodeKer, R.dNodeTi, dev->dTap, dev->dTapD0, dev->dTapVal, cur);
+    int lrc = launch_spectrum(dev, g, maxWave, cb->dInfo, cb->dOff, cb->dVal, cb->dOrd, cb->dC, cur + 1, true);
+    if (!lrc) {
+      launch_rows(dev, g, cb->dInfo, cb->dC, maxWave, rb, K, cb->dCand, cb->dGcnt, nullptr, cur + 1, 0);
+      lrc = band_select(dev, g, cb->dInfo, cb->dCand, cb->dGcnt, maxWave, K, R.k, cb->dC, cb->dOff, cb->dVal, R.n,
+                        cb->dOutVal, cb->dOutPos, nullptr, nullptr, R.dDi, R.dTi, R.nVar, R.dIndex, &cb->band,
+                        cur + 1);
+    }
+    hipGraph_t gr = nullptr;
+    e = hipStreamEndCapture(cs, &gr);
+    if (lrc || e != hipSuccess || !gr) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return lrc ? lrc : fail(QSMI_ERR_HIP, std::string("wave graph: end capture: ") + hipGetErrorString(e));
+    }
+    G.graph[ln] = gr;
+    e = hipGraphInstantiate(&G.exec[ln], gr, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      G.exec[ln] = nullptr;
+      return fail(QSMI_ERR_HIP, std::string("wave graph: instantiate: ") + hipGetErrorString(e));
+    }
+    return QSMI_OK;
+  };
   int prevLevel = -1, lane = 0;
   bool used1 = false, need1 = false;
   Progress progress(R.progressSec);
@@ -732,6 +843,20 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
     SideSwap sideSwap(dev, twoLanes && lane == 1);
     hipStream_t st = dev->stream;  // this wave's lane
     WaveBufs* b = dev->wb;
+    if (graphOn) {
+      if (!G.exec[lane] && (rc = captureLane(lane))) {
+        dev->timing = savedTiming;
+        return rc;
+      }
+      const hipError_t e = hipGraphLaunch(G.exec[lane], st);
+      if (e != hipSuccess) {
+        dev->timing = savedTiming;
+        return fail(QSMI_ERR_HIP, std::string("wave graph: launch: ") + hipGetErrorString(e));
+      }
+      scanned += nN;
+      waves++;
+      continue;
+    }
     hipEvent_t* ev = (timing && w % every == 0) ? &dev->evPool[(w / every) * 5] : nullptr;
     if (ev) (void)hipEventRecord(ev[0], st);
     // plain 1024-grid waves: the gather also builds the tap lists
@@ -740,7 +865,7 @@ int run_resident_waves(qsmi_device* dev, const ResidentLoop& R, double* stageMsO
                        R.dNbIdx, R.dNbCnt, R.dOffs, R.dKernel, R.dCatvals, R.dDi, R.dIndex, R.dTi, b->dInfo, b->dOff,
                        b->dVal, dev->dCentreOk, dev->dComplete, dev->nComplete, R.dPathK, R.dNodeKer, R.dNodeTi,
                        fuseTaps ? dev->dTap : nullptr, fuseTaps ? dev->dTapD0 : nullptr,
-                       fuseTaps ? dev->dTapVal : nullptr);
+                       fuseTaps ? dev->dTapVal : nullptr, nullptr);
     if (ev) (void)hipEventRecord(ev[1], st);
     const bool useLp = R.lp && (R.waveLp.empty() || R.waveLp[w]);
     if (R.lp && !useLp) R.lpSkipped += nN;

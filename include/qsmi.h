@@ -93,8 +93,17 @@ int qsmi_ti_destroy(qsmi_ti* ti);
  *                 (a 500 x 500 TI with six spectra: 2.5x the own-grid rate);
  *                 read when the TI is created (set it with
  *                 qsmi_set_default_tuning before qsmi_ti_create)
+ *   "fft.radix_max"  largest radix of the own-grid FFT plans: 2..13, default
+ *                 13 (a length with no other factor keeps its larger radix);
+ *                 read when the TI is created
+ *   "fft.lognv_max"  at most 2^value transforms per own-grid FFT block:
+ *                 0..6, default 6; read when the TI is created
+ *   "loop.graph"  1: the path loop of qsmi_qs_run_in_memory replays each
+ *                 wave's kernels as one captured HIP graph per stream (plain
+ *                 1024-grid scans with the two-stream loop; the sampled
+ *                 stage timing is not recorded then): 0..1, default 0
  * Results do not depend on them up to the FP32 map error the exact selection
- * absorbs (pencil.npb: every choice sums in the same order).
+ * absorbs (pencil.npb, loop.graph: every choice gives the same bytes).
  * QSMI_ERR_ARG for an unknown key or a value out of range. */
 int qsmi_ti_set_tuning(qsmi_ti* ti, const char* key, int value);
 int qsmi_ti_get_tuning(const qsmi_ti* ti, const char* key, int* value);

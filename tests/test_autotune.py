@@ -22,7 +22,7 @@ def test_cache_roundtrip(tmp_path):
 
 
 def test_defaults_cover_every_candidate_key():
-    keys = {k for k, _ in autotune.OWN_GRID_CANDIDATES} | {"scan.grid1024_min"}
+    keys = {k for k, _ in autotune.OWN_GRID_CANDIDATES} | {"scan.grid1024_min", "loop.graph"}
     assert keys == set(autotune.DEFAULTS)
     for k, values in autotune.OWN_GRID_CANDIDATES:
         assert autotune.DEFAULTS[k] in values

@@ -537,6 +537,37 @@ def test_sim_wave_size_invariant(kind):
         assert s == runs[0][0] and i == runs[0][1]
 
 
+@pytest.mark.parametrize("kind", ["continuous_1000", "multivariate_500"])
+def test_sim_graph_steps_identical(kind):
+    """Replaying each wave's kernels as a captured HIP graph (tuning key
+    loop.graph: one graph per lane, node counts and wave starts read on the
+    device) gives the same bytes as launching them: 16-node waves (many waves
+    of different sizes per level, on both lanes) and 256-node waves."""
+    if kind == "continuous_1000":
+        w = synth.config2_continuous_2d()
+        ti, dims, types, n, k, gkw = w["ti"], w["ti_dims"], [0], 80, 1.5, {}
+        sdims = (200, 160)
+        di = np.full(int(np.prod(sdims)), np.nan, np.float32)
+    else:
+        c = synth.config4_multivariate_2d()
+        ti, dims, types, n, k = c["ti"], c["ti_dims"], c["types"], c["n"], c["k"]
+        sdims, di = c["sim_dims"], c["di"]
+        kd0 = [d // 2 + 1 for d in dims]  # weights 0.5 / 0.5
+        gkw = dict(kernel=np.tile(np.asarray([0.5, 0.5], np.float32), int(np.prod(kd0))), kernel_dims=kd0)
+    try:
+        for ws in (16, 256):
+            runs = []
+            for graph in (0, 1):
+                g2s_amd.set_default_tuning("loop.graph", graph)
+                _log(f"graph steps {kind}: waves of <= {ws} nodes, loop.graph={graph}")
+                s, i, st = g2s_amd.qs(ti, dims, di, sdims, types, n, k, 9, wave_size=ws, **gkw)
+                runs.append((s.tobytes(), i.tobytes(), st["waves"]))
+            assert runs[0][2] == runs[1][2] and runs[0][2] > 1
+            assert runs[1][0] == runs[0][0] and runs[1][1] == runs[0][1]
+    finally:
+        g2s_amd.set_default_tuning("loop.graph", 0)
+
+
 @pytest.mark.parametrize("nvar", [1, 2])
 def test_sim_nan_ti_marginal_draw(nvar):
     """n = 1 (only the cell itself): every node draws from the marginal. With
