@@ -280,6 +280,29 @@ class AcceleratorDevice:
     def set_value_in_error_array(self, index, value):
         check(lib().qsmi_set_value_in_error_array(self.h, int(index), float(value)))
 
+    def set_cross_layer(self, alpha_x):
+        """The count layer of a TI with NaN (an AcceleratorDevice built
+        withCrossMeasurement): alpha_x (nSub, nSub), the second coefficient
+        matrix; the TI holds the cross sub-images (ti_subimages_x)."""
+        a = _f32(np.asarray(alpha_x, np.float32).T.reshape(-1))  # [A + B*nSub]
+        check(lib().qsmi_device_set_cross_layer(self.h, a))
+
+    def mask_layer_with_variable(self, layer, variable):
+        """AcceleratorDevice::maskLayerWithVariable: layer 0 (map) or 1
+        (count layer) times the mask image of ``variable``."""
+        check(lib().qsmi_mask_layer_with_variable(self.h, int(layer), int(variable)))
+
+    def compensate_missing_data(self):
+        """AcceleratorDevice::compensateMissingData: map = -|E / C^4|, -inf
+        where the count is zero."""
+        check(lib().qsmi_compensate_missing_data(self.h))
+
+    def cross_map(self):
+        """The count layer in position space (getCrossErrorAtPosition)."""
+        out = np.zeros(int(np.prod(self.ti.dims)), np.float32)
+        check(lib().qsmi_get_cross_map(self.h, out))
+        return out
+
     def set_value_in_error_array_with_radius(self, index, value, radius):
         """AcceleratorDevice::setValueInErrorArrayWithRadius: the Euclidean
         disk of ``radius`` around the position of ``index``."""

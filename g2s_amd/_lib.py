@@ -44,6 +44,10 @@ SIGNATURES = {
     "qsmi_get_array_size": (C.c_int, [vp, C.POINTER(C.c_uint)]),
     "qsmi_get_error_map": (C.c_int, [vp, f32p]),
     "qsmi_set_value_in_error_array": (C.c_int, [vp, C.c_uint, C.c_float]),
+    "qsmi_device_set_cross_layer": (C.c_int, [vp, f32p]),
+    "qsmi_mask_layer_with_variable": (C.c_int, [vp, C.c_uint, C.c_uint]),
+    "qsmi_compensate_missing_data": (C.c_int, [vp]),
+    "qsmi_get_cross_map": (C.c_int, [vp, f32p]),
     "qsmi_scan_wave": (C.c_int, [vp, C.c_int, i32p, i32p, f32p, f32p, f32p, C.c_float, f32p, u32p, vp]),
     "qsmi_distribution_wave": (C.c_int, [vp, C.c_int, i32p, i32p, f32p, f32p, f32p, C.c_float, u32p, f32p, f32p,
                                          u32p]),

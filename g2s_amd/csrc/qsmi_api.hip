@@ -33,6 +33,8 @@ void launch_rows_select(const ScanGeom& g, const FftPlan& plan0, int lognv, cons
 void launch_select_map(const float* map, long long S, int K, float quantum, uint32_t seedbits, Cand* cand, int chunks,
                        hipStream_t st);
 void launch_fill(float* p, long long n, float v, hipStream_t st);
+void launch_mask_layer(float* layer, const float* mask, long long n, hipStream_t st);
+void launch_compensate(float* map, const float* cnt, long long n, float cmin, hipStream_t st);
 void launch_fill_disk(float* map, const int* s, uint32_t o, float r2, float v, hipStream_t st);
 long long ct1024_stride();
 int rows1024_groups();
@@ -740,31 +742,131 @@ int qsmi_candidate_for_pattern(qsmi_device* dev, int nb, const int* offsets, con
   if (nb == 0) return QSMI_OK;
   if (!offsets || !values || !delta0) return fail(QSMI_ERR_ARG, "null argument");
   qsmi_ti* t = dev->ti;
-  const int V = (int)dev->active.size();
-  NodeInfo ni;
-  std::vector<int> off((size_t)nb * t->D);
-  std::vector<float> val((size_t)nb * std::max(V, 1));
-  prep_node(dev, nb, offsets, values, delta0[0], 0.f, 0, &ni, off.data(), val.data(), dev->mn, dev->mx);
-  if ((rc = ensure(&dev->dInfo, &dev->capInfo, 1))) return rc;
+  const int V = (int)dev->active.size(), D = t->D, nSub = t->nSub;
+  // with the cross layer the count layer is a second node (same neighbours,
+  // alphaX-combined values) whose map follows the first one's
+  const int layers = dev->xLayer ? 2 : 1;
+  NodeInfo ni[2];
+  std::vector<int> off((size_t)layers * nb * D);
+  std::vector<float> val((size_t)layers * nb * std::max(V, 1));
+  prep_node(dev, nb, offsets, values, delta0[0], 0.f, 0, &ni[0], off.data(), val.data(), dev->mn, dev->mx);
+  if (dev->xLayer) {
+    // the reference's devices add no delta0 with the cross layer
+    // (src/CPUThreadAcceleratorDevice.cpp:348), and the count's division
+    // leaves no exact-selection bound
+    ni[0].delta0 = 0.f;
+    ni[0].eps = 0.f;
+    ni[1] = ni[0];
+    ni[1].nbStart = nb;
+    float wmin = INFINITY;
+    for (int j = 0; j < nb; ++j) {
+      for (int d = 0; d < D; ++d) off[(size_t)(nb + j) * D + d] = offsets[j * D + d];
+      for (int a = 0; a < V; ++a) {
+        const int A = dev->active[a];
+        float acc = 0.f;
+        for (int B = 0; B < nSub; ++B) acc += dev->alphaX[A + B * nSub] * values[j * nSub + B];
+        val[(size_t)(nb + j) * V + a] = acc;
+        if (acc > 0.f) wmin = std::min(wmin, acc);
+      }
+    }
+    dev->xCmin = wmin < INFINITY ? (float)(0.5 * wmin * (dev->trueMismatch ? 1.0 : (double)t->N)) : 0.f;
+  }
+  if ((rc = ensure(&dev->dInfo, &dev->capInfo, (size_t)layers))) return rc;
   if ((rc = ensure(&dev->dOff, &dev->capOff, off.size()))) return rc;
   if ((rc = ensure(&dev->dVal, &dev->capVal, val.size()))) return rc;
-  if ((rc = ensure(&dev->dOrd, &dev->capOrd, (size_t)nb))) return rc;
-  if ((rc = ensure(&dev->dC, &dev->capC, (size_t)c_stride(dev)))) return rc;
+  if ((rc = ensure(&dev->dOrd, &dev->capOrd, (size_t)layers * nb))) return rc;
+  if ((rc = ensure(&dev->dC, &dev->capC, (size_t)layers * c_stride(dev)))) return rc;
   hipStream_t st = dev->stream;
-  HIPCHK(hipMemcpyAsync(dev->dInfo, &ni, sizeof(ni), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(dev->dInfo, ni, layers * sizeof(NodeInfo), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(dev->dOff, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(dev->dVal, val.data(), val.size() * sizeof(float), hipMemcpyHostToDevice, st));
-  launch_fill(dev->dMap, t->S, -INFINITY, st);
+  launch_fill(dev->dMap, layers * t->S, -INFINITY, st);
   ScanGeom g = geom_of(dev);
-  if ((rc = launch_spectrum(dev, g, 1, dev->dInfo, dev->dOff, dev->dVal, dev->dOrd, dev->dC))) return rc;
-  launch_rows(dev, g, dev->dInfo, dev->dC, 1, row_blocks(dev), 1, nullptr, nullptr, dev->dMap);
+  if ((rc = launch_spectrum(dev, g, layers, dev->dInfo, dev->dOff, dev->dVal, dev->dOrd, dev->dC))) return rc;
+  launch_rows(dev, g, dev->dInfo, dev->dC, layers, row_blocks(dev), 1, nullptr, nullptr, dev->dMap);
   HIPCHK(hipGetLastError());
   // the host arrays above are pageable and go out of scope: finish the copies
   HIPCHK(hipStreamSynchronize(st));
   dev->mapValid = true;
   dev->mapEdited = false;
-  dev->lastNode = ni;
+  dev->lastNode = ni[0];
   if (computed) *computed = 1;
+  return QSMI_OK;
+}
+
+int qsmi_device_set_cross_layer(qsmi_device* dev, const float* alphaX) {
+  int rc = check_dev(dev);
+  if (rc) return rc;
+  if (!alphaX) return fail(QSMI_ERR_ARG, "null argument");
+  if (dev->cross) return fail(QSMI_ERR_ARG, "device already runs the in-memory cross form");
+  qsmi_ti* t = dev->ti;
+  const int nSub = t->nSub;
+  bool any = false;
+  for (int q = 0; q < nSub * nSub; ++q) {
+    if (!std::isfinite(alphaX[q])) return fail(QSMI_ERR_ARG, "cross layer: non-finite coefficient");
+    any |= alphaX[q] != 0.f;
+  }
+  if (!any) return fail(QSMI_ERR_ARG, "cross layer: all coefficients zero");
+  float* m2 = nullptr;
+  HIPCHK(dalloc(&m2, (size_t)2 * t->S));
+  hipFree(dev->dMap);
+  dev->dMap = m2;
+  dev->alphaX.assign(alphaX, alphaX + nSub * nSub);
+  for (int A = 0; A < nSub; ++A) {  // the count layer's images are scanned too
+    bool need = false;
+    for (int B = 0; B < nSub; ++B) need |= alphaX[A + B * nSub] != 0.f;
+    if (need && std::find(dev->active.begin(), dev->active.end(), A) == dev->active.end()) dev->active.push_back(A);
+  }
+  std::sort(dev->active.begin(), dev->active.end());
+  dev->xLayer = true;
+  dev->mapValid = false;
+  return QSMI_OK;
+}
+
+int qsmi_mask_layer_with_variable(qsmi_device* dev, unsigned layer, unsigned variable) {
+  int rc = check_dev(dev);
+  if (rc) return rc;
+  if (!dev->xLayer) return fail(QSMI_ERR_ARG, "no cross layer: call qsmi_device_set_cross_layer first");
+  if (layer > 1) return fail(QSMI_ERR_ARG, "layer must be 0 (map) or 1 (count layer)");
+  if (!dev->mapValid) return fail(QSMI_ERR_ARG, "no map: call qsmi_candidate_for_pattern first");
+  // the variable-th sub-variable the count layer reads (a row of alphaX with
+  // a non-zero coefficient, src/CPUThreadAcceleratorDevice.cpp:366-376)
+  const qsmi_ti* t = dev->ti;
+  const int nSub = t->nSub;
+  int sub = -1, left = (int)variable;
+  for (int A = 0; A < nSub && sub < 0; ++A) {
+    bool need = false;
+    for (int B = 0; B < nSub; ++B) need |= dev->alphaX[A + B * nSub] != 0.f;
+    if (need && left-- == 0) sub = A;
+  }
+  if (sub < 0) return fail(QSMI_ERR_ARG, "mask variable out of range");
+  launch_mask_layer(dev->dMap + (size_t)layer * t->S, t->dSub + (size_t)sub * t->S, t->S, dev->stream);
+  HIPCHK(hipGetLastError());
+  if (layer == 0) dev->mapEdited = true;
+  return QSMI_OK;
+}
+
+int qsmi_compensate_missing_data(qsmi_device* dev) {
+  int rc = check_dev(dev);
+  if (rc) return rc;
+  if (!dev->xLayer) return fail(QSMI_ERR_ARG, "no cross layer: call qsmi_device_set_cross_layer first");
+  if (!dev->mapValid) return fail(QSMI_ERR_ARG, "no map: call qsmi_candidate_for_pattern first");
+  const long long S = dev->ti->S;
+  launch_compensate(dev->dMap, dev->dMap + S, S, dev->xCmin, dev->stream);
+  HIPCHK(hipGetLastError());
+  dev->mapEdited = true;
+  return QSMI_OK;
+}
+
+int qsmi_get_cross_map(qsmi_device* dev, float* out) {
+  int rc = check_dev(dev);
+  if (rc) return rc;
+  if (!out) return fail(QSMI_ERR_ARG, "null argument");
+  if (!dev->xLayer) return fail(QSMI_ERR_ARG, "no cross layer: call qsmi_device_set_cross_layer first");
+  if (!dev->mapValid) return fail(QSMI_ERR_ARG, "no map: call qsmi_candidate_for_pattern first");
+  const long long S = dev->ti->S;
+  HIPCHK(hipMemcpyAsync(out, dev->dMap + S, S * sizeof(float), hipMemcpyDeviceToHost, dev->stream));
+  HIPCHK(hipStreamSynchronize(dev->stream));
   return QSMI_OK;
 }
 
@@ -1115,6 +1217,7 @@ int qsmi_wave_create(qsmi_device* dev, int nNodes, const int* nbStart, const int
                      const float* delta0, const float* seeds, float k, qsmi_wave** out) {
   int rc = check_dev(dev);
   if (rc) return rc;
+  if (dev->xLayer) return fail(QSMI_ERR_UNSUPPORTED, "wave calls: not with the per-node cross layer");
   if (!out) return fail(QSMI_ERR_ARG, "null argument");
   *out = nullptr;
   auto* w = new qsmi_wave();
@@ -1163,6 +1266,7 @@ int qsmi_distribution_wave(qsmi_device* dev, int nNodes, const int* nbStart, con
                            const float* verbatimRadius, float* outErrors, unsigned* outPositions) {
   int rc = check_dev(dev);
   if (rc) return rc;
+  if (dev->xLayer) return fail(QSMI_ERR_UNSUPPORTED, "wave calls: not with the per-node cross layer");
   if (!verbatimPos || !verbatimRadius) return fail(QSMI_ERR_ARG, "null verbatim arrays");
   if (!dev->wb) dev->wb = new WaveBufs();
   rc = stage_wave(dev, dev->wb, nNodes, nbStart, offsets, values, delta0, seeds, k, verbatimPos, verbatimRadius);
@@ -1192,6 +1296,7 @@ int qsmi_scan_wave(qsmi_device* dev, int nNodes, const int* nbStart, const int* 
                    unsigned* outChosen) {
   int rc = check_dev(dev);
   if (rc) return rc;
+  if (dev->xLayer) return fail(QSMI_ERR_UNSUPPORTED, "wave calls: not with the per-node cross layer");
   if (!dev->wb) dev->wb = new WaveBufs();
   rc = stage_wave(dev, dev->wb, nNodes, nbStart, offsets, values, delta0, seeds, k);
   if (!rc) rc = run_wave(dev, dev->wb);

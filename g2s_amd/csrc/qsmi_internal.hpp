@@ -85,8 +85,13 @@ struct qsmi_device {
   float quantum = 0.f;
   int mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
   // per-node map state
-  float* dMap = nullptr;
+  float* dMap = nullptr;  // the map; with xLayer the count layer follows at dMap + S
   bool mapValid = false;
+  // per-node cross layer (qsmi_device_set_cross_layer): candidate_for_pattern
+  // computes the count layer with alphaX too; xCmin: the last node's zero-count
+  // threshold in map units
+  bool xLayer = false;
+  float xCmin = 0.f;
   // workspace
   float2* dC = nullptr;
   size_t capC = 0;

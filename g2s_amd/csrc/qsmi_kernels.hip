@@ -681,6 +681,37 @@ __global__ void k_fill(float* __restrict__ p, long long n, float v) {
   if (i < n) p[i] = v;
 }
 
+// maskLayerWithVariable on a position-space layer: every finite value times
+// the mask image at its position (-inf outside the valid box stays -inf)
+__global__ void k_mask_layer(float* __restrict__ layer, const float* __restrict__ mask, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float v = layer[i];
+  if (v > -INFINITY) layer[i] = v * mask[i];
+}
+
+// compensateMissingData: map = -|E / (((C C) C) C)| (the reference's
+// evaluation order), -inf where the count is zero: below cmin (half the
+// node's smallest positive count-layer value, in map units) the FFT value is
+// noise around an exact zero; cmin == 0 compares with zero as the reference does
+__global__ void k_compensate(float* __restrict__ map, const float* __restrict__ cnt, long long n, float cmin) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float e = map[i], c = cnt[i];
+  float r = -INFINITY;
+  const bool zero = cmin > 0.f ? !(c >= cmin) : c == 0.f;
+  if (e > -INFINITY && c > -INFINITY && !zero) r = -fabsf(e / (c * c * c * c));
+  map[i] = r;
+}
+
+void launch_mask_layer(float* layer, const float* mask, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_mask_layer, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, layer, mask, n);
+}
+
+void launch_compensate(float* map, const float* cnt, long long n, float cmin, hipStream_t st) {
+  hipLaunchKernelGGL(k_compensate, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, map, cnt, n, cmin);
+}
+
 // Every position of the map within the disk sum_d (p_d - o_d)^2 <= r2 set to
 // v; the grid covers the disk's bounding box [bl, bl + bn) (clipped to the
 // TI), one thread per position.

@@ -48,7 +48,6 @@ class QsmiAcceleratorDevice : public AcceleratorDevice {
   QsmiAcceleratorDevice(int deviceId, SharedMemoryManager* smm, std::vector<g2s::OperationMatrix> coefficientMatrix,
                         unsigned threadRatio, bool withCrossMeasurement, bool circularTI)
       : AcceleratorDevice(smm, coefficientMatrix, threadRatio, withCrossMeasurement, circularTI) {
-    if (withCrossMeasurement) unsupported("the cross-measurement layer of TIs with NaN");
     int count = 0;
     check(qsmi_get_device_count(&count), "device count");
     if (count < 1) check(QSMI_ERR_HIP, "no GPU");
@@ -60,12 +59,15 @@ class QsmiAcceleratorDevice : public AcceleratorDevice {
     { _sharedMemoryManager->addDevice(this); }
     initDim();  // _srcCplx: this GPU's shared entry, [0].space = its qsmi_ti
     qsmi_ti* ti = static_cast<qsmi_ti*>(_srcCplx.at(0).space);
-    g2s::OperationMatrix& m = _coefficientMatrix.at(0);
-    const unsigned n = m.getNumberOfVariable();
-    std::vector<float> alpha((size_t)n * n);
-    for (unsigned a = 0; a < n; ++a)
-      for (unsigned b = 0; b < n; ++b) alpha[a + (size_t)b * n] = m.getVariableAt((int)a, (int)b);
+    std::vector<float> alpha = layerCoefficients(0);
     check(qsmi_device_create(ti, alpha.data(), circularTI ? 1 : 0, &_dev), "device create");
+    // TIs with NaN: the second layer counts the TI values under the
+    // neighbours (DataImage::generateCoefMatrix4Xcorr with forXMeasurement)
+    if (withCrossMeasurement) {
+      if (_coefficientMatrix.size() < 2) check(QSMI_ERR_ARG, "cross measurement without a second coefficient layer");
+      std::vector<float> alphaX = layerCoefficients(1);
+      check(qsmi_device_set_cross_layer(_dev, alphaX.data()), "cross layer");
+    }
     check(qsmi_set_true_mismatch(_dev, _trueMismatch ? 1 : 0), "true mismatch");
   }
 
@@ -119,6 +121,7 @@ class QsmiAcceleratorDevice : public AcceleratorDevice {
     int computed = 0;
     check(qsmi_candidate_for_pattern(_dev, (int)nb, off.data(), val.data(), delta0.data(), &computed), "scan");
     _hostMap.clear();
+    _hostCross.clear();
     return computed != 0;
   }
 
@@ -156,8 +159,9 @@ class QsmiAcceleratorDevice : public AcceleratorDevice {
     return _hostMap[cvtIndexToPosition(index)];
   }
   float getValueAtPosition(unsigned layer, unsigned index) override {
-    if (layer != 0) unsupported("a second coefficient layer");
-    return getErrorAtPosition(index);
+    if (layer == 0) return getErrorAtPosition(index);
+    if (layer == 1 && _crossMeasurement) return getCrossErrorAtPosition(index);
+    unsupported("a coefficient layer beyond the map and the cross layer");
   }
 
   void setTrueMismatch(bool value) override {
@@ -184,14 +188,42 @@ class QsmiAcceleratorDevice : public AcceleratorDevice {
     unsupported("computeRealMissmatchAndRemoveWrongPattern");
   }
 
-  // The cross-measurement layer of TIs with NaN (rejected at construction).
-  float getCrossErrorAtPosition(unsigned) override { unsupported("getCrossErrorAtPosition"); }
-  void maskLayerWithVariable(unsigned, unsigned) override { unsupported("maskLayerWithVariable"); }
-  void compensateMissingData() override { unsupported("compensateMissingData"); }
+  // The cross-measurement layer of TIs with NaN (layer 1; the base class's
+  // maskCroossError / maskCroossErrorWithVariable call maskLayerWithVariable
+  // on it, src/AcceleratorDevice.cpp:136-145).
+  float getCrossErrorAtPosition(unsigned index) override {
+    if (_hostCross.empty()) {
+      size_t cells = 1;
+      for (unsigned s : _srcSize) cells *= s;
+      _hostCross.resize(cells);
+      check(qsmi_get_cross_map(_dev, _hostCross.data()), "cross layer read");
+    }
+    return _hostCross[cvtIndexToPosition(index)];
+  }
+  void maskLayerWithVariable(unsigned layer, unsigned variable) override {
+    check(qsmi_mask_layer_with_variable(_dev, layer, variable), "mask layer");
+    if (layer == 0) _hostMap.clear();
+    else _hostCross.clear();
+  }
+  void compensateMissingData() override {
+    check(qsmi_compensate_missing_data(_dev), "compensate missing data");
+    _hostMap.clear();
+  }
 
  private:
+  // coefficient layer `layer` as qsmi's alpha[A + B*nSub]
+  std::vector<float> layerCoefficients(size_t layer) {
+    g2s::OperationMatrix& m = _coefficientMatrix.at(layer);
+    const unsigned n = m.getNumberOfVariable();
+    std::vector<float> a((size_t)n * n);
+    for (unsigned i = 0; i < n; ++i)
+      for (unsigned j = 0; j < n; ++j) a[i + (size_t)j * n] = m.getVariableAt((int)i, (int)j);
+    return a;
+  }
+
   qsmi_device* _dev = nullptr;
-  std::vector<float> _hostMap;  // position-space copy of the map for getErrorAtPosition
+  std::vector<float> _hostMap;    // position-space copy of the map for getErrorAtPosition
+  std::vector<float> _hostCross;  // the same for the cross layer
 };
 
 extern "C" {

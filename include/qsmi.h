@@ -195,6 +195,37 @@ int qsmi_set_value_in_error_array(qsmi_device* dev, unsigned index, float value)
  * 286-308), without wrap-around. */
 int qsmi_set_value_in_error_array_with_radius(qsmi_device* dev, unsigned index, float value, float radius);
 
+/* The per-node cross-measurement layer of TIs with NaN: an AcceleratorDevice
+ * constructed withCrossMeasurement (include/AcceleratorDevice.hpp:40;
+ * src/AcceleratorDevice.cpp:33-37), whose second coefficient layer counts
+ * the TI values under the neighbours (DataImage::generateCoefMatrix4Xcorr
+ * with forXMeasurement, include/DataImage.hpp:626-710). alphaX: that
+ * OperationMatrix, nSub x nSub, element (A,B) at alphaX[A + B*nSub]; the
+ * TI's sub-images are convertInput4Xcorr's cross layout (qsmi_ti_subimages_x:
+ * per variable y^2, y and the mask m). Afterwards qsmi_candidate_for_pattern
+ * computes both layers: the map E (delta0 not added, as the reference's
+ * devices skip it with the cross layer, src/CPUThreadAcceleratorDevice.cpp:348)
+ * and the count layer C; qsmi_search_k_biggest selects on the FP32 map (no
+ * exact selection); the wave calls are rejected. */
+int qsmi_device_set_cross_layer(qsmi_device* dev, const float* alphaX);
+/* maskLayerWithVariable (include/AcceleratorDevice.hpp:63,
+ * src/CPUThreadAcceleratorDevice.cpp:360-383): layer 0 (the map) or 1 (the
+ * count layer) times, at every valid position, the TI sub-image the count
+ * layer reads for `variable` (the variable-th row of alphaX with a non-zero
+ * coefficient: that variable's mask image). maskCroossError /
+ * maskCroossErrorWithVariable (src/AcceleratorDevice.cpp:136-145) are
+ * layer 1. */
+int qsmi_mask_layer_with_variable(qsmi_device* dev, unsigned layer, unsigned variable);
+/* compensateMissingData (src/CPUThreadAcceleratorDevice.cpp:414-425):
+ * map = -|E / (C C C C)|, -inf where the count is zero. A zero count comes
+ * out of the FP32 transforms as noise around 0, so "zero" is a count below
+ * half the node's smallest positive count-layer value (the reference
+ * compares with 0.0f). */
+int qsmi_compensate_missing_data(qsmi_device* dev);
+/* getCrossErrorAtPosition for every position (include/AcceleratorDevice.hpp:56):
+ * the count layer, prod(srcSize) floats in position space. */
+int qsmi_get_cross_map(qsmi_device* dev, float* out);
+
 /* Path-wave batch: nNodes independent nodes in one launch sequence
  * (replaces nNodes x [candidateForPattern + searchKBiggest + the
  * QuantileSamplingModule::sample draw, include/quantileSamplingModule.hpp:90-126]).

@@ -18,7 +18,7 @@ import pytest
 import g2s_amd
 import oracle
 from g2s_amd import synth
-from qs_helpers import MAP_RTOL, Problem, grf2d, neighbourhood, neighbourhood_low, valid_box
+from qs_helpers import ERR_C, MAP_RTOL, Problem, grf2d, neighbourhood, neighbourhood_low, valid_box
 
 pytestmark = pytest.mark.gpu
 
@@ -380,3 +380,93 @@ def test_reflected_layout_ti():
         ti.close()
         ci += 1
     assert ci >= 4
+
+
+def _cross_setup(dims, nv, rng, nan_frac=0.2):
+    """A TI with NaN in the cross layout: (raw TI (cells, nv), sub-images
+    [y^2, y, m] per variable, alpha, alphaX) from the library's set-up, which
+    the CPU tests pin to the oracle's restatement of convertInput4Xcorr /
+    generateCoefMatrix4Xcorr."""
+    import ctypes as C
+
+    from g2s_amd._lib import lib
+
+    shape = tuple(reversed(dims))
+    ti = np.stack([np.exp(grf2d(shape, rng, smooth=2.0)).reshape(-1) for _ in range(nv)], axis=1)
+    ti = ti.astype(np.float32)
+    ti[rng.random(ti.shape) < nan_frac] = np.nan
+    n = C.c_int(0)
+    pa = np.zeros(256, np.float32)
+    pax = np.zeros(256, np.float32)
+    pct = np.zeros(16, np.int32)
+    pso = np.zeros(16, np.int32)
+    assert lib().qsmi_coef_matrix_x(nv, np.zeros(nv, np.int32), pa, pax, pct, pso, C.byref(n)) == 0
+    ns = n.value
+    alpha = pa[:ns * ns].reshape(ns, ns).T.copy()  # [A, B]
+    alpha_x = pax[:ns * ns].reshape(ns, ns).T.copy()
+    sub = np.zeros(ns * ti.shape[0], np.float32)
+    assert lib().qsmi_ti_subimages_x(len(dims), oracle.u32(dims), nv, np.zeros(nv, np.int32), ti.reshape(-1),
+                                     sub) == 0
+    return ti, sub.reshape(ns, -1), alpha, alpha_x
+
+
+@pytest.mark.parametrize("dims,nv,nb", [((60, 48), 1, 18), ((60, 48), 2, 14), ((400, 340), 1, 24),
+                                        ((400, 340), 2, 16)])
+def test_cross_layer_per_node(dims, nv, nb):
+    """The per-node cross-measurement layer (an AcceleratorDevice built
+    withCrossMeasurement): both layers against their FP64 closed forms within
+    the stated FP32 bound, maskLayerWithVariable + compensateMissingData
+    against -|E / (C m)^4| (-inf where the masked count is zero), and the
+    K best of the compensated map by the device rule. 60x48 scans on the TI's
+    own grid, 400x340 on the 1024 grid."""
+    rng = np.random.default_rng(dims[0] + 7 * nv)
+    ti, sub, alpha, alpha_x = _cross_setup(dims, nv, rng)
+    ns = sub.shape[0]
+    tih = g2s_amd.TrainingImage(sub, dims)
+    dev = g2s_amd.AcceleratorDevice(tih, alpha)
+    dev.set_cross_layer(alpha_x)
+    n_fft = int(np.prod([oracle.lib().orc_fft_size_1d(d) for d in dims]))
+    rms = np.sqrt((sub.astype(np.float64) ** 2).sum(axis=1) / n_fft)
+    offs_all = oracle.offsets([9] * len(dims))
+    finite = np.flatnonzero(~np.isnan(ti).any(axis=1))
+    for trial in range(3):
+        sel = np.sort(rng.choice(np.arange(1, offs_all.shape[0]), size=nb, replace=False))
+        off = offs_all[sel]
+        x = ti[rng.choice(finite, size=nb)].astype(np.float64) * (1.0 + 0.05 * rng.standard_normal((nb, nv)))
+        u = np.zeros((nb, ns), np.float32)
+        for v in range(nv):  # conversions P0, P1, P2 of the cross layout, weight 1
+            u[:, 3 * v] = 1.0
+            u[:, 3 * v + 1] = x[:, v]
+            u[:, 3 * v + 2] = x[:, v] ** 2
+        assert dev.candidate_for_pattern(off, u, 0.0)
+        E = dev.error_map().astype(np.float64)
+        Cm = dev.cross_map().astype(np.float64)
+        E_ref = oracle.map_direct(dims, sub, alpha, off, u, 0.0)
+        C_ref = oracle.map_direct(dims, sub, alpha_x, off, u, 0.0)
+        valid = np.isfinite(E_ref)
+        assert np.array_equal(np.isfinite(E), valid) and np.array_equal(np.isfinite(Cm), valid)
+        uu = u.astype(np.float64)
+        eps_e = ERR_C * float(np.sum(rms * np.abs(uu @ alpha.astype(np.float64).T).sum(axis=0)))
+        eps_c = ERR_C * float(np.sum(rms * np.abs(uu @ alpha_x.astype(np.float64).T).sum(axis=0)))
+        de = np.abs(E[valid] - E_ref[valid]).max()
+        dc = np.abs(Cm[valid] - C_ref[valid]).max()
+        _log(f"cross layer {dims} nv={nv} trial {trial}: |dE| {de:.3e} (bound {eps_e:.3e}), "
+             f"|dC| {dc:.3e} (bound {eps_c:.3e})")
+        assert de <= eps_e and dc <= eps_c
+        var = trial % nv
+        dev.mask_layer_with_variable(1, var)
+        dev.compensate_missing_data()
+        M = dev.error_map().astype(np.float64)
+        cm = C_ref * sub[3 * var + 2].astype(np.float64)  # the count times the centre's mask of `var`
+        ok = valid & (cm != 0.0)  # counts are sums of unit weights: exact zeros in FP64
+        assert np.array_equal(np.isfinite(M), ok)
+        ref = -np.abs(E_ref[ok] / cm[ok] ** 4)
+        tol = eps_e / cm[ok] ** 4 + 4.0 * np.abs(E_ref[ok]) * eps_c / np.abs(cm[ok]) ** 5 + 1e-6 * np.abs(ref)
+        assert np.all(np.abs(M[ok] - ref) <= tol)
+        lo, hi = valid_box(dims, off)
+        K = 3
+        err, idx = dev.search_k_biggest(K, 0.37)
+        pos = np.array([dev.cvt_index_to_position(int(i)) for i in idx])
+        ov, op, _ = oracle.select_device(dims, M.astype(np.float32), lo, hi, K, 0.37)
+        assert np.array_equal(pos, op.astype(np.int64)) and np.array_equal(err, ov)
+    dev.close()
