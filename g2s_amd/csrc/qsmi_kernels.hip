@@ -117,7 +117,12 @@ __global__ void k_colfft(float2* __restrict__ base, FftPlan plan, const float2* 
 // ---------------------------------------------------------------------------
 constexpr int kPencilThreads = 256;
 constexpr int kMaxEPT = kMaxVecElems / kPencilThreads;  // 16
+// STAGE: the dim-0/1 twiddle tables once per block and each node's
+// neighbours (nb <= kStageNb, V <= kStageV) in LDS, so the column build reads
+// only LDS; the sums are the same operations in the same order either way
+constexpr int kStageNb = 128, kStageV = 8;
 
+template <bool STAGE>
 __global__ void __launch_bounds__(kPencilThreads) k_pencil(ScanGeom g, FftPlan planP, int lognv, int nblk0,
                                                            const float2* __restrict__ Yhat,
                                                            const float2* __restrict__ tw0g,
@@ -134,7 +139,19 @@ __global__ void __launch_bounds__(kPencilThreads) k_pencil(ScanGeom g, FftPlan p
   float2* twP = smem;
   float2* bufA = twP + LP;
   float2* bufB = bufA + LP * nvec;
+  float2* tw0s = bufB + LP * nvec;  // STAGE: f[0] entries, then f[1] (D == 3)
+  float2* tw1s = tw0s + g.f[0];
+  int* sOff = reinterpret_cast<int*>(tw1s + (D == 3 ? g.f[1] : 0));  // [kStageNb][3]
+  float* sVal = reinterpret_cast<float*>(sOff + kStageNb * 3);      // [kStageNb][kStageV]
+  uint32_t* sOrd = reinterpret_cast<uint32_t*>(sVal + kStageNb * kStageV);
   for (int i = tid; i < LP; i += nthr) twP[i] = twPg[i];
+  if (STAGE) {
+    for (int i = tid; i < g.f[0]; i += nthr) tw0s[i] = tw0g[i];
+    if (D == 3)
+      for (int i = tid; i < g.f[1]; i += nthr) tw1s[i] = tw1g[i];
+  }
+  const float2* __restrict__ tw0 = STAGE ? tw0s : tw0g;
+  const float2* __restrict__ tw1 = STAGE ? tw1s : tw1g;
 
   const int blk0 = blockIdx.x % nblk0;
   const int k1 = (D == 3) ? (int)(blockIdx.x / nblk0) : 0;
@@ -150,6 +167,16 @@ __global__ void __launch_bounds__(kPencilThreads) k_pencil(ScanGeom g, FftPlan p
   for (int node = nodeBeg; node < nodeEnd; ++node) {
     const NodeInfo ni = info[node];
     if (ni.lo[P] > ni.hi[P]) continue;  // no valid centre
+    // block-uniform; the staged lists are read after the first barrier below
+    // and not overwritten before the node's last barrier
+    const bool staged = STAGE && ni.nb <= kStageNb;
+    if (staged)
+      for (int j = tid; j < ni.nb; j += nthr) {
+        const long long jj = ni.nbStart + j;
+        sOrd[j] = ord[jj];
+        for (int d = 0; d < D; ++d) sOff[j * 3 + d] = nbOff[jj * D + d];
+        for (int a = 0; a < g.V; ++a) sVal[j * kStageV + a] = nbVal[jj * g.V + a];
+      }
     float2 M[kMaxEPT];
 #pragma unroll
     for (int i = 0; i < kMaxEPT; ++i) M[i] = make_float2(0.f, 0.f);
@@ -160,7 +187,7 @@ __global__ void __launch_bounds__(kPencilThreads) k_pencil(ScanGeom g, FftPlan p
       // the sparse neighbour kernel column(s): the neighbours sorted by
       // their row along the pencil dimension (k_nb_sort); the head of each
       // row's run sums the run in neighbour-list order (fixed order)
-      const uint32_t* op = ord + ni.nbStart;
+      const uint32_t* op = staged ? sOrd : ord + ni.nbStart;
       for (int t = tid; t < ni.nb * nvec; t += nthr) {
         const int i = t >> lognv, v = t & (nvec - 1);
         const int len = (int)(op[i] >> 16);
@@ -170,22 +197,33 @@ __global__ void __launch_bounds__(kPencilThreads) k_pencil(ScanGeom g, FftPlan p
         float2 acc = make_float2(0.f, 0.f);
         int dP = 0;
         for (int r = 0; r < len; ++r) {
-          const int jj = ni.nbStart + (int)(op[i + r] & 0xFFFFu);
-          const float val = nbVal[(long long)jj * g.V + a];
-          const int d0 = nbOff[jj * D + 0];
-          int t0 = (int)(((long long)k0 * d0) % f0);
+          const int j = (int)(op[i + r] & 0xFFFFu);
+          float val;
+          int d0, d1 = 0;
+          if (staged) {
+            val = sVal[j * kStageV + a];
+            d0 = sOff[j * 3 + 0];
+            if (D == 3) d1 = sOff[j * 3 + 1];
+            dP = sOff[j * 3 + P];
+          } else {
+            const int jj = ni.nbStart + j;
+            val = nbVal[(long long)jj * g.V + a];
+            d0 = nbOff[jj * D + 0];
+            if (D == 3) d1 = nbOff[jj * D + 1];
+            dP = nbOff[jj * D + P];
+          }
+          // |k0 d0| < f0^2 / 2 and |k1 d1| < f1^2: 32-bit products
+          int t0 = (k0 * d0) % f0;
           if (t0 < 0) t0 += f0;
-          float2 ph = tw0g[t0];
+          float2 ph = tw0[t0];
           ph.y = -ph.y;  // exp(+2 pi i k0 d0 / f0)
           if (D == 3) {
-            const int d1 = nbOff[jj * D + 1];
-            int t1 = (int)(((long long)k1 * d1) % f1);
+            int t1 = (k1 * d1) % f1;
             if (t1 < 0) t1 += f1;
-            float2 p1 = tw1g[t1];
+            float2 p1 = tw1[t1];
             p1.y = -p1.y;
             ph = cmul(ph, p1);
           }
-          dP = nbOff[jj * D + P];
           acc.x += val * ph.x;
           acc.y += val * ph.y;
         }
@@ -752,6 +790,7 @@ QSMI_INST_K(16)
 namespace qsmi {
 
 static size_t fft_lds_bytes(int L, int nvec) { return (size_t)(L + 2 * L * nvec) * sizeof(float2); }
+static size_t g_ldsLimit = 0;  // dynamic LDS per block the FFT kernels may use (set_lds_limits)
 
 // Raises the dynamic LDS limit of the FFT kernels to what the device allows
 // (160 KiB per workgroup on gfx950) minus a margin for static LDS.
@@ -761,7 +800,9 @@ hipError_t set_lds_limits(int device, size_t* limitOut) {
   if (e != hipSuccess) return e;
   const int lim = maxShared - 1024;
   *limitOut = (size_t)lim;
-  const void* fns[] = {(const void*)k_pencil,         (const void*)k_r2c_rows,        (const void*)k_colfft<-1>,
+  g_ldsLimit = (size_t)lim;
+  const void* fns[] = {(const void*)k_pencil<false>, (const void*)k_pencil<true>, (const void*)k_r2c_rows,
+                       (const void*)k_colfft<-1>,
                        (const void*)k_colfft<+1>,     (const void*)k_rows_select<2>,  (const void*)k_rows_select<4>,
                        (const void*)k_rows_select<8>, (const void*)k_rows_select<16>, (const void*)k_rows_collect};
   for (const void* f : fns) {
@@ -801,8 +842,15 @@ void launch_pencil(const ScanGeom& g, const FftPlan& planP, int lognv, const flo
   const int nblk0 = (g.h0 + nvec - 1) / nvec;
   hipLaunchKernelGGL(k_nb_sort, dim3((unsigned)nNodes), dim3(64), 0, st, info, nbOff, g.D, g.f[g.D - 1], ord);
   dim3 grid((unsigned)(nblk0 * (g.D == 3 ? g.f[1] : 1)), (unsigned)((nNodes + npb - 1) / npb));
-  hipLaunchKernelGGL(k_pencil, grid, dim3(kPencilThreads), fft_lds_bytes(planP.L, nvec), st, g, planP, lognv, nblk0,
-                     Yhat, tw0, tw1, twP, nNodes, npb, info, nbOff, nbVal, ord, C);
+  const size_t base = fft_lds_bytes(planP.L, nvec);
+  const size_t staged = base + (size_t)(g.f[0] + (g.D == 3 ? g.f[1] : 0)) * sizeof(float2) +
+                        (size_t)kStageNb * (3 * sizeof(int) + kStageV * sizeof(float) + sizeof(uint32_t));
+  if (g.V <= kStageV && staged <= g_ldsLimit)
+    hipLaunchKernelGGL(k_pencil<true>, grid, dim3(kPencilThreads), staged, st, g, planP, lognv, nblk0, Yhat, tw0, tw1,
+                       twP, nNodes, npb, info, nbOff, nbVal, ord, C);
+  else
+    hipLaunchKernelGGL(k_pencil<false>, grid, dim3(kPencilThreads), base, st, g, planP, lognv, nblk0, Yhat, tw0, tw1,
+                       twP, nNodes, npb, info, nbOff, nbVal, ord, C);
 }
 
 void launch_direct1d(const ScanGeom& g, const float2* Yhat, const float2* tw0, int nNodes, const NodeInfo* info,
