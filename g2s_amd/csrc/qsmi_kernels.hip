@@ -122,6 +122,17 @@ constexpr int kMaxEPT = kMaxVecElems / kPencilThreads;  // 16
 // only LDS; the sums are the same operations in the same order either way
 constexpr int kStageNb = 128, kStageV = 8;
 
+// x mod f in [0, f): a float estimate of the quotient, corrected once
+// (|x| < 2^23, so x and the product are exact to within one unit of q)
+__device__ __forceinline__ int mod_small(int x, int f, float invf) {
+  if (x <= -(1 << 23) || x >= (1 << 23)) return ((x % f) + f) % f;
+  const int q = (int)floorf((float)x * invf);
+  int r = x - q * f;
+  if (r < 0) r += f;
+  if (r >= f) r -= f;
+  return r;
+}
+
 template <bool STAGE>
 __global__ void __launch_bounds__(kPencilThreads) k_pencil(ScanGeom g, FftPlan planP, int lognv, int nblk0,
                                                            const float2* __restrict__ Yhat,
@@ -144,6 +155,7 @@ __global__ void __launch_bounds__(kPencilThreads) k_pencil(ScanGeom g, FftPlan p
   int* sOff = reinterpret_cast<int*>(tw1s + (D == 3 ? g.f[1] : 0));  // [kStageNb][3]
   float* sVal = reinterpret_cast<float*>(sOff + kStageNb * 3);      // [kStageNb][kStageV]
   uint32_t* sOrd = reinterpret_cast<uint32_t*>(sVal + kStageNb * kStageV);
+  float2* sP1 = reinterpret_cast<float2*>(sOrd + kStageNb);  // D == 3: exp(+2 pi i k1 d1 / f1) per neighbour
   for (int i = tid; i < LP; i += nthr) twP[i] = twPg[i];
   if (STAGE) {
     for (int i = tid; i < g.f[0]; i += nthr) tw0s[i] = tw0g[i];
@@ -161,6 +173,7 @@ __global__ void __launch_bounds__(kPencilThreads) k_pencil(ScanGeom g, FftPlan p
   const int elems = LP * nvec;
   const int fP = g.f[P];
   const int f0 = g.f[0], f1 = g.f[1];
+  const float invf0 = 1.f / (float)f0, invf1 = 1.f / (float)f1;
 
   const int nodeBeg = blockIdx.y * npb;
   const int nodeEnd = min(nNodes, nodeBeg + npb);
@@ -176,6 +189,13 @@ __global__ void __launch_bounds__(kPencilThreads) k_pencil(ScanGeom g, FftPlan p
         sOrd[j] = ord[jj];
         for (int d = 0; d < D; ++d) sOff[j * 3 + d] = nbOff[jj * D + d];
         for (int a = 0; a < g.V; ++a) sVal[j * kStageV + a] = nbVal[jj * g.V + a];
+        if (D == 3) {  // the block's k1 phase of neighbour j, shared by every column and spectrum
+          int t1 = (int)(((long long)k1 * nbOff[jj * D + 1]) % f1);
+          if (t1 < 0) t1 += f1;
+          float2 p1 = tw1g[t1];
+          p1.y = -p1.y;
+          sP1[j] = p1;
+        }
       }
     float2 M[kMaxEPT];
 #pragma unroll
@@ -212,16 +232,16 @@ __global__ void __launch_bounds__(kPencilThreads) k_pencil(ScanGeom g, FftPlan p
             if (D == 3) d1 = nbOff[jj * D + 1];
             dP = nbOff[jj * D + P];
           }
-          // |k0 d0| < f0^2 / 2 and |k1 d1| < f1^2: 32-bit products
-          int t0 = (k0 * d0) % f0;
-          if (t0 < 0) t0 += f0;
-          float2 ph = tw0[t0];
+          float2 ph = tw0[mod_small(k0 * d0, f0, invf0)];
           ph.y = -ph.y;  // exp(+2 pi i k0 d0 / f0)
           if (D == 3) {
-            int t1 = (k1 * d1) % f1;
-            if (t1 < 0) t1 += f1;
-            float2 p1 = tw1[t1];
-            p1.y = -p1.y;
+            float2 p1;
+            if (staged) {
+              p1 = sP1[j];
+            } else {
+              p1 = tw1[mod_small(k1 * d1, f1, invf1)];
+              p1.y = -p1.y;
+            }
             ph = cmul(ph, p1);
           }
           acc.x += val * ph.x;
@@ -844,7 +864,8 @@ void launch_pencil(const ScanGeom& g, const FftPlan& planP, int lognv, const flo
   dim3 grid((unsigned)(nblk0 * (g.D == 3 ? g.f[1] : 1)), (unsigned)((nNodes + npb - 1) / npb));
   const size_t base = fft_lds_bytes(planP.L, nvec);
   const size_t staged = base + (size_t)(g.f[0] + (g.D == 3 ? g.f[1] : 0)) * sizeof(float2) +
-                        (size_t)kStageNb * (3 * sizeof(int) + kStageV * sizeof(float) + sizeof(uint32_t));
+                        (size_t)kStageNb * (3 * sizeof(int) + kStageV * sizeof(float) + sizeof(uint32_t) +
+                                            sizeof(float2));
   if (g.V <= kStageV && staged <= g_ldsLimit)
     hipLaunchKernelGGL(k_pencil<true>, grid, dim3(kPencilThreads), staged, st, g, planP, lognv, nblk0, Yhat, tw0, tw1,
                        twP, nNodes, npb, info, nbOff, nbVal, ord, C);
