@@ -565,11 +565,16 @@ int qsmi_qs_run_in_memory(const qsmi_qs_request* rq, qsmi_qs_result* rs) {
   }
   if (nKer > 1) nOff = (int)(offs.size() / D);
   // path and seeds (src/qs.cpp:1159-1258)
+  // a cell is informed (hard data, path head) when it holds every variable;
+  // a partially informed cell is simulated at its place in the path with
+  // its own values as the centre neighbour, and keeps them
+  // (include/simulation.hpp:324-335, 581-584, 658-666)
   std::vector<unsigned char> informed(SD, 0);
   for (size_t c = 0; c < SD; ++c) {
     int nn = 0;
     for (int v = 0; v < nVar; ++v) nn += std::isnan(rq->di[c * nVar + v]) ? 1 : 0;
-    if (nn != 0 && nn != nVar) return hfail(QSMI_ERR_UNSUPPORTED, "partially informed simulation cells");
+    if (nn != 0 && nn != nVar && cross)
+      return hfail(QSMI_ERR_UNSUPPORTED, "partially informed simulation cells with a TI holding NaN");
     informed[c] = (nn == 0);
   }
   std::vector<unsigned> path(SD);

@@ -709,8 +709,10 @@ __global__ void k_merge_ti(const NodeInfo* __restrict__ info, int nN, int nTI, i
   }
   if (!any) return;
   const uint32_t t = pick.pos % (uint32_t)nTI, p = pick.pos / (uint32_t)nTI;
-  for (int v = 0; v < nVar; ++v)
-    simVals[(long long)ni.cell * nVar + v] = tiVals[((long long)cum.c[t] + p) * nVar + v];
+  for (int v = 0; v < nVar; ++v) {  // the variables the cell does not hold yet
+    float* dst = simVals + (long long)ni.cell * nVar + v;
+    if (isnan(*dst)) *dst = tiVals[((long long)cum.c[t] + p) * nVar + v];
+  }
   indexOut[ni.cell] = pick.pos;
 }
 

@@ -395,9 +395,10 @@ __device__ void block_topk64(TopK64<KT>& t, int K, Cand64* out, Cand64* red) {
 // quantileSamplingModule.hpp:101-113): floor(seed * k * (ceil(1/k)/1)), k
 // from the node's k image when given (include/simulation.hpp:355-357),
 // clamped to the node's list; then the simulation write (include/
-// simulation.hpp:658-666): the TI value at the chosen position and
-// importDataIndex = index * nTI + TI (one TI). pos: the node's K best,
-// best first (0xFFFFFFFF: none).
+// simulation.hpp:658-666): the TI value at the chosen position for every
+// variable the cell does not hold yet (a partially informed cell keeps its
+// values) and importDataIndex = index * nTI + TI (one TI). pos: the node's
+// K best, best first (0xFFFFFFFF: none).
 __device__ __forceinline__ void draw_write(const NodeInfo& ni, const uint32_t* pos, int K, float kf,
                                            uint32_t* chosen, int node, float* simVals, const float* tiVals,
                                            int nVar, uint32_t* indexOut) {
@@ -409,7 +410,10 @@ __device__ __forceinline__ void draw_write(const NodeInfo& ni, const uint32_t* p
   const uint32_t p = pos[r];
   if (chosen) chosen[node] = p;
   if (simVals && p != 0xFFFFFFFFu) {
-    for (int v = 0; v < nVar; ++v) simVals[(long long)ni.cell * nVar + v] = tiVals[(long long)p * nVar + v];
+    for (int v = 0; v < nVar; ++v) {
+      float* dst = simVals + (long long)ni.cell * nVar + v;
+      if (isnan(*dst)) *dst = tiVals[(long long)p * nVar + v];
+    }
     indexOut[ni.cell] = p;
   }
 }

@@ -112,8 +112,13 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
   ni.kLocal = pathK ? pathK[i] : 0.f;
   ni.tiReq = nodeTi ? nodeTi[i] : -1;
   ni.eps = 0.f;
-  if (cnt <= 1) {
-    // only the cell itself: draw from the marginal (simulation.hpp:586-606)
+  // only the cell itself, holding no value: draw from the marginal
+  // (simulation.hpp:586-606); a partially informed cell with only itself as
+  // neighbour is scanned with its own values (simulation.hpp:581-584)
+  bool centreData = false;
+  if (cnt == 1)
+    for (int v = 0; v < P.nVar; ++v) centreData |= !isnan(di[(long long)cell * P.nVar + v]);
+  if (cnt <= 1 && !centreData) {
     if (lane == 0) {
       // over all TIs in order (include/simulation.hpp:586-606): the TI whose
       // cells hold the position
@@ -124,7 +129,8 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
         const unsigned r = (unsigned)(int)floorf(seed * (float)(nComplete / (unsigned)P.nVar)) * (unsigned)P.nVar;
         position = r == 0 ? 0u : complete[r - 1];
       }
-      for (int v = 0; v < P.nVar; ++v) di[(long long)cell * P.nVar + v] = tiVals[(long long)position * P.nVar + v];
+      for (int v = 0; v < P.nVar; ++v)  // the variables the cell does not hold
+        if (isnan(di[(long long)cell * P.nVar + v])) di[(long long)cell * P.nVar + v] = tiVals[(long long)position * P.nVar + v];
       int tt = 0;
       while (tt + 1 < P.nTI && position >= P.tiCum[tt + 1]) ++tt;
       index[cell] = (position - P.tiCum[tt]) * (unsigned)P.nTI + (unsigned)tt;

@@ -70,6 +70,7 @@ class OrcSim(C.Structure):
         ("circularTI", C.c_int),
         ("tiImage", C.c_void_p),
         ("tiDimsAll", C.c_void_p),
+        ("di0", C.c_void_p),
     ]
 
 
@@ -286,7 +287,9 @@ class Sim:
 
     def __init__(self, ti, tidims, types, ncat, catvals, simdims, kdims, kernel_exp, offs, n, k, path, seeds,
                  begin, mode=0, quantum=0.0, cross=False, no_verbatim=False, n_ti=1, n_image=None,
-                 k_image=None, circular_sim=False, circular_ti=False, ti_image=None, ti_dims_all=None):
+                 k_image=None, circular_sim=False, circular_ti=False, ti_image=None, ti_dims_all=None, di0=None):
+        """di0: the simulation grid as given, when it holds partially informed
+        cells (a node's own values when it is reached)."""
         self._keep = []
 
         def keep(a):
@@ -334,6 +337,8 @@ class Sim:
             s.nImage = keep(f32(n_image).reshape(-1))
         if k_image is not None:
             s.kImage = keep(f32(k_image).reshape(-1))
+        if di0 is not None:
+            s.di0 = keep(f32(di0).reshape(-1))
         self.spec = s
         self._index = None
         self.nvar = len(types)

@@ -10,8 +10,11 @@
 // (simulation.hpp:463-473), which makes its result identical to the serial
 // order restated here.
 //
-// Supported here (and by the device): fully-NaN cells to simulate, fully
-// informed hard-data cells, one value of n for all variables, one TI, no
+// Supported here (and by the device): fully-NaN and partially informed cells
+// to simulate (a partially informed cell is its own first neighbour with the
+// values it holds and keeps them, include/simulation.hpp:324-335, 581-584,
+// 658-666), fully informed hard-data cells, one value of n for all
+// variables, one TI, no
 // -nV / -fs / kernel images / transforms. A TI with NaN (continuous
 // variables) runs in the cross-measurement form (`cross`): a cell with any
 // NaN variable loses all of them (src/qs.cpp:1436-1466), the map is
@@ -98,6 +101,8 @@ struct OrcSim {
   int circularTI;       // 1: -cti, every TI position valid, neighbourhoods wrapped
   const float* tiImage; // several TIs: per simulation cell the only TI sampled (nullptr: all)
   const unsigned* tiDimsAll;  // several TIs: nTI x 3 extents (nullptr: tiDims for all)
+  const float* di0;     // the simulation grid as given (nullptr: no partially informed cells): a
+                        // node's own values when it is reached (include/simulation.hpp:463-473)
 };
 
 struct OrcSimCtx {
@@ -292,13 +297,13 @@ static int node_inputs(OrcSimCtx* c, const float* di, unsigned pathIdx, std::vec
       t += (size_t)wrap_if(cc[i] + d, int(c->sd[i]), p.circularSim) * mul;
       mul *= c->sd[i];
     }
-    // one-hot expansion — simulation.hpp:512-537. The node's own cell is
-    // still NaN when the serial loop reaches it, whatever `di` holds.
+    // one-hot expansion — simulation.hpp:512-537. The node's own cell holds
+    // what it was given when the serial loop reaches it, whatever `di` holds.
     const bool isCentre = (t == cell);
     int e = 0, catIdx = 0;
     const float* cv = p.catvals;
     for (int v = 0; v < p.nVar; ++v) {
-      float x = isCentre ? NAN : di[t * p.nVar + v];
+      float x = isCentre ? (p.di0 ? p.di0[(size_t)cell * p.nVar + v] : NAN) : di[t * p.nVar + v];
       if (rawOut) (*rawOut)[(size_t)j * p.nVar + v] = x;
       if (p.types[v] == 0) {
         vals[(size_t)j * c->nExp + e++] = x;
@@ -518,7 +523,14 @@ static unsigned node_impl(OrcSimCtx* c, const float* di, unsigned pathIdx, int K
   float delta0 = 0.f;
   int nb = node_inputs(c, di, pathIdx, off, u, &delta0, nullptr);
   *nbOut = nb;
-  if (nb <= 1) {
+  // only the cell itself: the marginal unless the cell holds values
+  // (include/simulation.hpp:581-584: then its own values are scanned)
+  bool centreData = false;
+  if (nb == 1 && p.di0) {
+    const unsigned cell = p.path[p.beginPath + pathIdx];
+    for (int v = 0; v < p.nVar; ++v) centreData |= !std::isnan(p.di0[(size_t)cell * p.nVar + v]);
+  }
+  if (nb <= 1 && !centreData) {
     // marginal draw: position = floor(seed * cells) in float, over all TIs in
     // order (include/simulation.hpp:586-606); several TIs: the encoded index
     if (c->nTI > 1) {
@@ -697,7 +709,8 @@ void orc_sim_run(void* h, float* di, unsigned* idxOut) {
     idxOut[cell] = tpos;  // importDataIndex = index*nTI + TI
     const float* tiv = p.cross ? c->tiX.data() : p.ti;
     const size_t src = (c->nTI > 1 ? c->cumT[tpos % (unsigned)c->nTI] : 0) + tpos / (unsigned)c->nTI;
-    for (int q = 0; q < p.nVar; ++q) di[(size_t)cell * p.nVar + q] = tiv[src * p.nVar + q];
+    for (int q = 0; q < p.nVar; ++q)  // the variables the cell does not hold (simulation.hpp:658-666)
+      if (std::isnan(di[(size_t)cell * p.nVar + q])) di[(size_t)cell * p.nVar + q] = tiv[src * p.nVar + q];
   }
   c->index = keep;
 }

@@ -75,7 +75,7 @@ def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, nodes=None,
     assert np.array_equal(p, gp) and np.array_equal(s, gs) and b == gb
     orc = oracle.Sim(ti, ti_dims, types, ncat, catvals, sim_dims, kd, kern, offs, n, k, p, s, b, mode=0,
                      quantum=quantum, cross=cross, no_verbatim=no_verbatim, n_image=n_image, k_image=k_image,
-                     circular_sim=circular_sim, circular_ti=circular_ti)
+                     circular_sim=circular_sim, circular_ti=circular_ti, di0=di0)
     orc.set_index(idx)  # -nV reads the positions the neighbours were copied from
     K = int(np.ceil(k))
     simf = sim.reshape(-1, nvar)
@@ -83,9 +83,15 @@ def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, nodes=None,
     hard = informed.astype(bool)
     assert np.array_equal(simf[hard], di0f[hard])
     assert not np.isnan(simf).any()
+    # partially informed cells keep their values (include/simulation.hpp:658-666)
+    given = ~np.isnan(di0f)
+    assert np.array_equal(simf[given], di0f[given])
     tif = ti.reshape(-1, nvar).copy()
     if cross:  # cells with any NaN variable lose all of them (src/qs.cpp:1447-1466)
         tif[np.isnan(tif).any(axis=1)] = np.nan
+    # every simulated cell holds the TI values at its index where it held none
+    sc = p[b:]
+    assert np.array_equal(simf[sc], np.where(np.isnan(di0f[sc]), tif[idx[sc]], di0f[sc]))
     exact = quantum == 0.0 and not cross
     mismatches = 0
     worst_gap = 0.0
@@ -101,7 +107,7 @@ def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, nodes=None,
         if chosen == idx[cell]:
             continue
         mismatches += 1
-        assert nb > 1, f"marginal draw differs at node {i}"
+        assert nb > 1 or not np.isnan(di0f[cell]).all(), f"marginal draw differs at node {i}"
         m, sc, _ = orc.node_map(simf, i)
         a, g = int(chosen), int(idx[cell])
         assert np.isfinite(m[g]), f"node {i}: GPU chose an invalid position {g}"
@@ -114,8 +120,9 @@ def lockstep(ti, ti_dims, di0, sim_dims, types, n, k, seed, quantum, nodes=None,
             bound = quantum + 2.0 * ERR_C * orc.node_norm(simf, i)
         assert gap <= bound, f"node {i}: gpu {g} ({m[g]}) oracle {a} ({m[a]}) gap {gap} > {bound}"
         worst_gap = max(worst_gap, gap / bound)
-        # the value written is the TI value at the GPU's choice
-        assert np.array_equal(simf[cell], tif[idx[cell]])
+        # the value written is the TI value at the GPU's choice (the cell's
+        # own values kept)
+        assert np.array_equal(simf[cell], np.where(np.isnan(di0f[cell]), tif[idx[cell]], di0f[cell]))
     _log(f"  lockstep mismatches {mismatches} of {len(check)} checked nodes (of {len(p) - b}; largest gap / "
          f"bound {worst_gap:.3f})")
     return sim, idx, st
@@ -566,6 +573,64 @@ def test_sim_graph_steps_identical(kind):
             assert runs[1][0] == runs[0][0] and runs[1][1] == runs[0][1]
     finally:
         g2s_amd.set_default_tuning("loop.graph", 0)
+
+
+def _partial_grid(sdims, nvar, rng, frac_partial=0.3, frac_full=0.05, src=None):
+    """A simulation grid with hard data and partially informed cells (each
+    holding a random non-empty proper subset of the variables), values copied
+    from `src` cells."""
+    cells = int(np.prod(sdims))
+    di = np.full((cells, nvar), np.nan, np.float32)
+    u = rng.random(cells)
+    full = u < frac_full
+    part = (u >= frac_full) & (u < frac_full + frac_partial)
+    pick = rng.integers(0, src.shape[0], size=cells)
+    di[full] = src[pick[full]]
+    for c in np.flatnonzero(part):
+        keep = rng.random(nvar) < 0.5
+        if keep.all() or not keep.any():
+            keep = np.zeros(nvar, bool)
+            keep[rng.integers(0, nvar)] = True
+        di[c, keep] = src[pick[c], keep]
+    return di
+
+
+@pytest.mark.parametrize("kind", ["multivariate_generic", "continuous_1024", "centre_only"])
+def test_sim_partially_informed_cells(kind):
+    """Partially informed simulation cells (include/simulation.hpp:324-335,
+    581-584, 658-666): simulated at their place in the path with their own
+    values as the centre neighbour, keeping them. Lockstep against the
+    oracle; centre_only uses n = 1, so a partially informed cell is scanned
+    with its own values alone and an empty cell draws from the marginal."""
+    rng = np.random.default_rng(61)
+    if kind == "multivariate_generic":
+        a = synth.grf((40, 44), 4.0, seed=63)
+        b_ = synth.grf((40, 44), 4.0, seed=64)
+        c = 0.7 * a + 0.71 * b_
+        cat = np.digitize(c, np.quantile(c, [0.25, 0.5, 0.75])).astype(np.float32)
+        ti, dims = synth.to_g2s(a, cat)
+        types, n, k, sdims = [0, 1], 12, 1.5, (30, 32)
+    elif kind == "continuous_1024":
+        a = synth.grf((360, 400), 12.0, seed=65)
+        b_ = synth.grf((360, 400), 12.0, seed=66)
+        ti, dims = synth.to_g2s(np.exp(a), 0.6 * a + 0.8 * b_)
+        types, n, k, sdims = [0, 0], 20, 1.5, (60, 50)
+    else:
+        a = synth.grf((30, 36), 4.0, seed=67)
+        b_ = synth.grf((30, 36), 4.0, seed=68)
+        ti, dims = synth.to_g2s(a, np.exp(b_))
+        types, n, k, sdims = [0, 0], 1, 1.5, (20, 18)
+    nvar = len(types)
+    di = _partial_grid(sdims, nvar, rng, src=ti.reshape(-1, nvar)).reshape(-1)
+    _log(f"partially informed cells {kind}: TI {dims}, sim {sdims}, n={n}")
+    free = int(np.isnan(di.reshape(-1, nvar)).any(axis=1).sum())  # the path nodes
+    nodes = None if kind != "continuous_1024" else _sample_nodes(free, 400, 5)
+    lockstep(ti, dims, di, sdims, types, n, k, 13, 0.0, nodes=nodes, wave_size=64, threads=8)
+
+
+def _sample_nodes(total, count, seed):
+    rng = np.random.default_rng(seed)
+    return np.sort(rng.choice(total, size=min(count, total), replace=False))
 
 
 @pytest.mark.parametrize("nvar", [1, 2])

@@ -383,3 +383,105 @@ def test_nan_safe_marginal_draw(nvar, frac):
         if any(not complete[int(np.floor(np.float32(s) * np.float32(S)))] for s in seeds[(seeds >= lo) & (seeds < hi)]):
             hit.add(int(ranked[k * nvar - 1]))
     assert hit <= walked
+
+
+def _partial_setup(n, seed=71):
+    """Two continuous variables; a simulation grid with hard data, partially
+    informed cells and empty cells."""
+    a = synth.grf((24, 28), 3.0, seed=seed)
+    b_ = synth.grf((24, 28), 3.0, seed=seed + 1)
+    ti, dims = synth.to_g2s(np.exp(a), a + 0.5 * b_)
+    sdims = (14, 12)
+    cells = int(np.prod(sdims))
+    rng = np.random.default_rng(seed)
+    di = np.full((cells, 2), np.nan, np.float32)
+    u = rng.random(cells)
+    src = ti[rng.integers(0, ti.shape[0], cells)]
+    di[u < 0.05] = src[u < 0.05]
+    di[(u >= 0.05) & (u < 0.25), 0] = src[(u >= 0.05) & (u < 0.25), 0]
+    di[(u >= 0.25) & (u < 0.45), 1] = src[(u >= 0.25) & (u < 0.45), 1]
+    informed = (~np.isnan(di).any(axis=1)).astype(np.uint8)
+    p, s, b = oracle.path(informed, seed)
+    kd = [d // 2 + 1 for d in dims]
+    offs = oracle.offsets(kd)
+    kern = np.ones(int(np.prod(kd)) * 2, np.float32)
+    sim = oracle.Sim(ti, dims, [0, 0], [], [], sdims, kd, kern, offs, n, 1.5, p, s, b, mode=0, di0=di)
+    return sim, ti, dims, sdims, di, p, s, b, offs
+
+
+def _brute_map(ti, dims, d, x):
+    """-sum_j sum_v (x_jv - y_v(p + d_j))^2 over the values x_jv given, FP64,
+    -inf where a neighbour leaves the TI."""
+    s0, s1 = dims
+    y = ti.reshape(s1, s0, 2).astype(np.float64)
+    out = np.full(s0 * s1, -np.inf)
+    lo0, hi0 = -min(0, d[:, 0].min()), s0 - 1 - max(0, d[:, 0].max())
+    lo1, hi1 = -min(0, d[:, 1].min()), s1 - 1 - max(0, d[:, 1].max())
+    for q1 in range(lo1, hi1 + 1):
+        for q0 in range(lo0, hi0 + 1):
+            e = 0.0
+            for (a0, a1), xv in zip(d, x):
+                for v in range(2):
+                    if not np.isnan(xv[v]):
+                        e -= (float(xv[v]) - y[q1 + a1, q0 + a0, v]) ** 2
+            out[q1 * s0 + q0] = e
+    return out
+
+
+def test_partially_informed_cells():
+    """A partially informed cell is simulated at its place in the path, is its
+    own first neighbour with the values it holds and keeps them
+    (include/simulation.hpp:324-335, 581-584, 658-666): with n = 1 such a
+    node scans its own values alone (the device rule on the brute-force map)
+    while an empty cell draws from the marginal; with n = 6 a node's map is
+    the brute-force mismatch over its neighbours and its own given values."""
+    sim, ti, dims, sdims, di, p, s, b, offs = _partial_setup(1)
+    out, idx = sim.run(di)
+    out = out.reshape(-1, 2)
+    given = ~np.isnan(di)
+    assert np.array_equal(out[given], di[given]) and not np.isnan(out).any()
+    S = ti.shape[0]
+    lo, hi = [0, 0], [dims[0] - 1, dims[1] - 1]
+    y2max = np.nanmax(ti.astype(np.float64) ** 2, axis=0)
+
+    def tol(x):  # the float32 rounding of the expanded form's cancelling terms
+        return 1e-6 * float(sum(np.nansum(xj.astype(np.float64) ** 2) + y2max[~np.isnan(xj)].sum() for xj in x))
+
+    scanned = 0
+    for i in range(len(p) - b):
+        cell = p[b + i]
+        if np.isnan(di[cell]).all():
+            assert idx[cell] == int(np.floor(np.float32(s[b + i]) * np.float32(S)))
+            continue
+        m = _brute_map(ti, dims, np.zeros((1, 2), np.int64), di[cell][None, :])
+        _, op, _ = oracle.select_device_f64(dims, m, lo, hi, 3, float(s[b + i]))
+        r = min(int(np.floor(np.float32(s[b + i]) * np.float32(1.5))), 1)  # rank floor(seed * k)
+        # the rank-r choice, up to near ties within the inputs' float32 rounding
+        t = tol(di[cell][None, :])
+        assert idx[cell] == op[r] or (abs(m[idx[cell]] - m[op[r]]) <= 2 * t and
+                                      min(abs(m[op[r]] - m[op[r - 1]]) if r else np.inf,
+                                          abs(m[op[r + 1]] - m[op[r]])) <= 2 * t), i
+        assert np.array_equal(out[cell], np.where(given[cell], di[cell], ti[idx[cell]]))
+        scanned += 1
+    assert scanned > 10
+    sim6, ti, dims, sdims, di, p, s, b, offs = _partial_setup(6)
+    out6, idx6 = sim6.run(di)
+    out6 = out6.reshape(-1, 2)
+    assert np.array_equal(out6[given], di[given]) and not np.isnan(out6).any()
+    checked = 0
+    for i in range(0, len(p) - b, 5):
+        cell = p[b + i]
+        if np.isnan(di[cell]).all():
+            continue
+        g = sim6.gather(i)
+        c0, c1 = cell % sdims[0], cell // sdims[0]
+        d = offs[g]
+        x = np.array([di[cell] if (dd[0] == 0 and dd[1] == 0) else out6[(c1 + dd[1]) * sdims[0] + c0 + dd[0]]
+                      for dd in d])
+        m, _, nb = sim6.node_map(out6, i, with_scale=False)
+        ref = _brute_map(ti, dims, d, x)
+        fin = np.isfinite(ref)
+        assert nb == len(g) and np.array_equal(np.isfinite(m), fin)
+        assert np.all(np.abs(m[fin] - ref[fin]) <= tol(x)), i
+        checked += 1
+    assert checked > 5
