@@ -345,8 +345,9 @@ typedef struct {
   const int* varTypes;     /* nVar: 0 continuous, 1 categorical */
   const float* ti;         /* prod(tiSize) x nVar (NaN: no value, continuous variables only) */
   unsigned simSize[3];
-  const float* di;         /* prod(simSize) x nVar; all-NaN cells are simulated,
-                              fully informed cells are hard data */
+  const float* di;         /* prod(simSize) x nVar; fully informed cells are hard
+                              data, the others are simulated (a partially
+                              informed cell keeps the values it holds) */
   unsigned kernelSize[3];  /* all 0: default uniform kernel (src/qs.cpp:1010-1042) */
   const float* kernel;     /* prod(kernelSize) x nVar, or NULL */
   int n;                   /* -n, centre included */
