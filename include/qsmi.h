@@ -113,7 +113,8 @@ int qsmi_set_default_tuning(const char* key, int value);
 
 /* Replaces the c_NvidiaGPUAcceleratorDevice factory
  * (src/NvidiaGPUAcceleratorDevice.cu:229-232) for one coefficient layer
- * (no cross-measurement layer): alpha is the OperationMatrix of
+ * (the cross-measurement layer is added by qsmi_device_set_cross_layer):
+ * alpha is the OperationMatrix of
  * include/OperationMatrix.hpp:9-73, nSub x nSub, element (A,B) at
  * alpha[A + B*nSub]. circularTI as in the reference (requires
  * fftSize == srcSize). The device starts with trueMismatch = 1 (what qs sets
