@@ -11,18 +11,19 @@
 //   kernel gen/expand .... tests/test_oracle_ref.py (vs compiled reference)
 //   fKst select .......... tests/test_oracle_ref.py (vs compiled reference,
 //                          tie-heavy maps, leading -inf runs)
-//   FFT .................. tests/test_oracle_fft.py (vs numpy.fft)
-//   maps ................. tests/test_oracle_map.py (direct FP64 vs FFT FP64
+//   offset list order .... tests/test_oracle_ref.py (golden and live: the
+//                          comparator's weights from the compiled DataImage
+//                          distance/indexWithDelta, the generation loop and
+//                          std::sort restated)
+//   FFT .................. tests/test_oracle_math.py (vs numpy.fft)
+//   maps ................. tests/test_oracle_math.py (direct FP64 vs FFT FP64
 //                          vs numpy brute force; self-retrieval; valid count;
 //                          linearity in alpha; translation)
-//   index maps / draw .... tests/test_oracle_map.py (round trip, closed form)
+//   index maps / draw .... tests/test_oracle_math.py (round trip, closed form)
 //   path / seeds ......... tests/test_oracle_sim.py (libstdc++ routines;
 //                          permutation and range checks)
 //   neighbour gather ..... tests/test_oracle_sim.py (brute-force nearest
 //                          informed search)
-//   offset list order .... parity unpinned beyond "sorted by distance and a
-//                          permutation": the reference sorts in qs.cpp, which
-//                          does not build here (needs FFTW/zmq)
 #include <algorithm>
 #include <climits>
 #include <cmath>
