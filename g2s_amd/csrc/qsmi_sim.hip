@@ -260,13 +260,20 @@ __global__ void __launch_bounds__(64) k_gather(GatherParams P, int w0, int nN, c
     }
   }
   __syncthreads();
-  if (lane == 0) {
-    float delta0 = 0.f;
-    for (int v = 0; v < P.nVar; ++v) {
-      if (P.types[v] != 0) continue;
-      for (int j = 0; j < nT; ++j)
-        if (has[v * P.n + j]) delta0 += term[v * P.n + j];
+  // the sequential sum, in that order, on every lane: each round of 64 terms
+  // is read from LDS by all lanes at once and added lane by lane from
+  // registers (a term without a value adds +0, which leaves the sum as it is)
+  float delta0 = 0.f;
+  for (int v = 0; v < P.nVar; ++v) {
+    if (P.types[v] != 0) continue;
+    for (int j0 = 0; j0 < nT; j0 += 64) {
+      const int j = j0 + lane;
+      const int tb = __float_as_int((j < nT && has[v * P.n + j]) ? term[v * P.n + j] : 0.f);
+      const int cntR = min(64, nT - j0);
+      for (int q = 0; q < cntR; ++q) delta0 += __int_as_float(__builtin_amdgcn_readlane(tb, q));
     }
+  }
+  if (lane == 0) {
     for (int d = 0; d < 3; ++d) {
       if (d >= D) {
         ni.lo[d] = ni.hi[d] = 0;

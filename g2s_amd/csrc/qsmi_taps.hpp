@@ -84,14 +84,25 @@ __device__ __forceinline__ void build_taps1024(int node, int nb, int nbStart, co
   __syncthreads();
   uint16_t* od = tapD0 + (long long)node * kTapNbMax;
   float* ov = tapVal + (long long)node * kTapNbMax * V;
-  for (int t = lane; t < nb; t += 64) {
-    const short tt = tapJ[t];
-    int r = 0;
-    for (int q = 0; q < t; ++q) r += (tapJ[q] == tt) ? 1 : 0;
-    const int slot = startL[tt] + r;
-    const long long jj = nbStart + t;
-    od[slot] = (uint16_t)(nbOff[jj * 2 + 0] & (kTapF - 1));
-    for (int a = 0; a < V; ++a) ov[(long long)slot * V + a] = nbVal[jj * V + a];
+  // neighbour t's slot: its tap's start plus the number of earlier
+  // neighbours on the same tap, by one ballot per tap over 64 neighbours at
+  // a time (startL[u] advances past each round's neighbours of tap u)
+  for (int t0 = 0; t0 < nb; t0 += 64) {  // wave-uniform
+    const int t = t0 + lane;
+    const int tt = t < nb ? (int)tapJ[t] : -1;
+    int slot = 0;
+    for (int u = 0; u < T; ++u) {  // wave-uniform
+      const unsigned long long m = __ballot(tt == u);
+      if (m == 0ull) continue;
+      const int base = startL[u];
+      if (tt == u) slot = base + __popcll(m & ((1ull << lane) - 1ull));
+      if (lane == 0) startL[u] = base + __popcll(m);  // after every lane's read (in order)
+    }
+    if (t < nb) {
+      const long long jj = nbStart + t;
+      od[slot] = (uint16_t)(nbOff[jj * 2 + 0] & (kTapF - 1));
+      for (int a = 0; a < V; ++a) ov[(long long)slot * V + a] = nbVal[jj * V + a];
+    }
   }
 }
 
